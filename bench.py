@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: filter -> hash join -> group-by on one MI355X.
+
+Workload (the reference's radix join + group-by configuration):
+  fact 200,000,000 rows: fk i64 uniform [0,16M), qty i64 uniform [1,100],
+                         price f64 lognormal(3,1)
+  dim   16,000,000 rows: pk = permutation of [0,16M), cat i32 uniform [0,1000)
+  GROUP(cat; SUM price, SUM qty, COUNT price) over
+  JOIN(FILTER(fact, qty > 50), fk = pk, dim)    -- one td.h DAG, seed 42
+
+A step is one td_execute of that DAG with both tables resident in HBM
+(td_dev_put) and the result left resident.  `value` is fact rows/s over
+the K timed steps.  `roofline` prices the dominant kernel's algorithmic
+bytes against its average HIP-event duration over the same timed steps.
+`cpu_baseline` times the reference's own multithreaded CPU path (built from
+source into oracle/_ref) on a smaller sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E, GB/s
+N_FACT, N_DIM, N_CAT, SEED = 200_000_000, 16_000_000, 1000, 42
+CPU_SAMPLE = (50_000_000, 4_000_000)       # reference CPU path sample (rows)
+
+
+def kernel_bytes(name, nf, nd, ncat):
+    """Algorithmic (compulsory) HBM bytes of one launch of a kernel of the
+    fused pipeline: every referenced column read once, outputs written once."""
+    if name.startswith("jg_probe"):
+        # fact fk, qty, price (24 B/row) + the dim-row -> group-slot map read
+        # once (4 B per dim row)
+        return nf * 24 + nd * 4
+    if name == "jg_slotmap":
+        return nd * (8 + 4) + nd * 4       # pk, cat in; slot map out
+    if name == "jg_build":
+        return nd * 8 + nd * 4             # pk in; head array out
+    return None
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(2, min(16, n))
+
+
+def cpu_baseline_reference(nf, nd, ncat, reps=5):
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    thr = cpu_threads()
+    env = dict(os.environ, TD_REF_THREADS=str(thr))
+    out = subprocess.run([exe, "bench", "jg", str(nf), str(nd), str(ncat), str(reps)],
+                         check=True, capture_output=True, text=True, env=env, timeout=600)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": nf / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
+            "kind": "reference",
+            "sample": f"fact {nf:,} x dim {nd:,} rows, {ncat} cats, same distributions and DAG "
+                      f"(join graph + group graph), median of {reps} after 1 warm-up, "
+                      f"td_pool_init({r['threads'] - 1}) workers + caller"}
+
+
+def cpu_baseline_port(nf, nd, ncat):
+    """Single-threaded oracle restatement, used when oracle/_ref is absent."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
+    from teide_amd import workloads as W
+    f, d = W.join_fact(nf, nd, SEED), W.join_dim(nd, ncat, SEED)
+    t0 = time.perf_counter()
+    idx = orc.sel_to_idx(orc.sel_from_bool(orc.cmp_const(f["qty"], "GT", 50)), nf)
+    fk, qty, price = (orc.gather(f[c], idx) for c in ("fk", "qty", "price"))
+    li, ri = orc.join([fk], [d["pk"]], 0)
+    cat = orc.gather(d["cat"], ri)
+    orc.group([cat], ["SUM", "SUM", "COUNT"], [orc.gather(price, li), orc.gather(qty, li), None])
+    dt = time.perf_counter() - t0
+    return {"value": nf / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"fact {nf:,} x dim {nd:,} rows, one run of the scalar oracle"}
+
+
+def load_traffic(name):
+    """HBM bytes per launch of the kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when recorded."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    ent = rec.get(name)
+    return ent.get("bytes_per_launch") if ent else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fact", type=int, default=N_FACT)
+    ap.add_argument("--dim", type=int, default=N_DIM)
+    ap.add_argument("--cats", type=int, default=N_CAT)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-prof", action="store_true",
+                    help="no per-kernel HIP events in the timed steps")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    from teide_amd import workloads as W
+    from teide_amd.td import engine
+    e = engine()
+    if e.L.td_dev_available() != 1:
+        raise SystemExit("bench.py: no usable HIP device")
+
+    t0 = time.perf_counter()
+    fact = W.join_fact(a.fact, a.dim, SEED)
+    dim = W.join_dim(a.dim, a.cats, SEED)
+    hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+    del fact
+    hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"])])
+    del dim
+    t1 = time.perf_counter()
+    tf, td = e.put(hf), e.put(hd)
+    e.L.td_dev_sync()
+    t2 = time.perf_counter()
+    e.release(hf)
+    e.release(hd)
+    put_gbs = (a.fact * 24 + a.dim * 12) / (t2 - t1) / 1e9
+    print(f"# data: gen {t1 - t0:.1f}s, h2d {t2 - t1:.2f}s ({put_gbs:.1f} GB/s)",
+          file=sys.stderr, flush=True)
+
+    def step():
+        g = e.graph(tf)
+        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
+        g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+        root = g.group([g.scan("cat")], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
+                                         ("COUNT", g.scan("price"))])
+        out = g.execute(root)
+        g.free()
+        return out
+
+    for _ in range(a.warmup):
+        e.release(step())
+    plan = e.last_plan()
+    e.L.td_dev_sync()
+    e.prof_reset()
+    e.prof(not a.no_prof)
+    e.L.td_dev_sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        e.release(step())
+    e.L.td_dev_sync()
+    t1 = time.perf_counter()
+    prof = e.prof_read()
+    e.prof(False)
+    ms = (t1 - t0) * 1e3 / a.steps
+
+    roofline = None
+    kernels = {k: {"launches": n, "avg_ms": round(tot / max(n, 1), 5)} for k, (n, tot) in prof.items()}
+    if prof:
+        dom = max(prof, key=lambda k: prof[k][1])
+        n, tot = prof[dom]
+        avg_ms = tot / n
+        nbytes = kernel_bytes(dom, a.fact, a.dim, a.cats)
+        if nbytes:
+            ach = nbytes / (avg_ms / 1e3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": load_traffic(dom), "kernel": dom,
+                        "algorithmic_bytes": nbytes, "avg_ms": round(avg_ms, 5)}
+    query_bytes = a.fact * 24 + a.dim * 12 + a.cats * 28
+    cpu = None
+    if not a.no_cpu_baseline:
+        nf, nd = CPU_SAMPLE
+        nf, nd = min(nf, a.fact), min(nd, a.dim)
+        cpu = cpu_baseline_reference(nf, nd, a.cats) or cpu_baseline_port(
+            min(nf, 5_000_000), min(nd, 400_000), a.cats)
+    line = {
+        "metric": "rows/s + % HBM peak for join+group-by",
+        "value": a.fact / (ms / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i64/f64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": f"filter qty>50 -> join fact {a.fact:,} x dim {a.dim:,} on fk=pk "
+                               f"-> group by cat ({a.cats} cats): SUM price, SUM qty, COUNT",
+                   "plan": plan},
+        "query_gbs": round(query_bytes / (ms / 1e3) / 1e9, 1),
+        "query_frac_hbm": round(query_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    e.release(tf)
+    e.release(td)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+/*
+ * td_dev.h -- MI355X extensions of the td.h API.  None of these has a
+ * counterpart in the reference engine; td.h programs do not need them.
+ *
+ * Residency.  A vector is either host-resident (the reference's layout,
+ * mmod 0) or device-resident: a host td_t proxy with mmod 3 whose elements
+ * live in the HBM arena, a buddy allocator over device memory.  td_execute
+ * accepts both.  When every column a query touches is device-resident its
+ * results stay device-resident (proxies); otherwise host columns are copied
+ * in for the call and results come back as ordinary host vectors, so an
+ * unchanged td.h program behaves exactly as with the reference.
+ *
+ * The elements of a device-resident vector are not host-addressable:
+ * td_data() of a proxy holds the device address, not the values.
+ */
+#ifndef TD_DEV_H
+#define TD_DEV_H
+
+#include <teide/td.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1 when the GPU and the library's kernels are usable (initialises them). */
+int      td_dev_available(void);
+
+/* Device-resident copy of a vector, TD_SEL or table (tables: a new table of
+ * device-resident copies of every column).  Resident input: a device-side
+ * copy.  Returns TD_ERR_PTR(TD_ERR_OOM) when the arena budget is exceeded. */
+td_t*    td_dev_put(td_t* v);
+/* Host copy of a vector, TD_SEL or table (host input: retained as is). */
+td_t*    td_dev_get(td_t* v);
+/* 1 when v (or every column of table v) is device-resident. */
+int      td_dev_is_resident(td_t* v);
+/* Wait for all queued device work. */
+td_err_t td_dev_sync(void);
+
+/* Arena accounting and budget (admission control uses these). */
+typedef struct {
+    uint64_t in_use;         /* bytes handed out */
+    uint64_t peak;
+    uint64_t reserved;       /* device memory held by the arena */
+    uint64_t budget;         /* cap on reserved; 0 = 90% of device memory */
+    uint64_t device_total;
+} td_dev_mem_t;
+void     td_dev_mem_stats(td_dev_mem_t* out);
+void     td_dev_set_budget(uint64_t bytes);
+
+/* Kernel timing: while enabled every kernel launch is timed with HIP events
+ * on the library stream; td_dev_prof_read fills up to cap rows of
+ * (kernel name, launches, total milliseconds). */
+typedef struct { char name[48]; int64_t launches; double total_ms; } td_dev_prof_t;
+void     td_dev_prof_enable(int on);
+void     td_dev_prof_reset(void);
+int      td_dev_prof_read(td_dev_prof_t* rows, int cap);
+
+/* Physical-plan hints for tests and measurement: force the join path
+ * (0 auto, 1 direct map, 2 radix) / group path (0 auto, 1 direct array,
+ * 2 hash) / radix bits (0 auto) for subsequent td_execute calls. */
+void     td_dev_force_paths(int join_path, int group_path, int radix_bits);
+/* The physical plan of the last td_execute as a short text. */
+const char* td_dev_last_plan(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+/*
+ * tdd.h -- the device layer: a thin C ABI between the C17 host engine and the
+ * hand-written HIP kernels for gfx950 (MI355X), implemented in the .hip
+ * files of teide_amd/csrc/dev.
+ *
+ * Plain pointers and sizes only.  All "device" pointers are HBM addresses
+ * (normally blocks of the HBM arena, see td_dev.h); every call runs on the
+ * library's single HIP stream and returns a td_err_t code (TD_OK on
+ * success).  A call that produces host-visible scalars synchronises the
+ * stream before returning.
+ *
+ * Each kernel entry names the reference CPU routine whose semantics it
+ * implements (file:line in the reference tree).
+ */
+#ifndef TDD_H
+#define TDD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- runtime -- */
+
+typedef struct {
+    char     name[64];
+    char     arch[32];          /* e.g. "gfx950" */
+    int32_t  n_cu;
+    int32_t  n_xcd_hint;        /* 8 on MI355X */
+    uint64_t hbm_bytes;
+    uint64_t lds_bytes_per_cu;
+    int32_t  clock_mhz;
+} tdd_info_t;
+
+int  tdd_init(void);                         /* device 0, one stream; idempotent */
+int  tdd_available(void);                    /* 1 when a GPU is usable */
+int  tdd_info(tdd_info_t* out);
+int  tdd_malloc(void** dptr, size_t bytes);  /* raw hipMalloc (arena backing) */
+int  tdd_free(void* dptr);
+int  tdd_h2d(void* dst, const void* src, size_t bytes);
+int  tdd_d2h(void* dst, const void* src, size_t bytes);
+int  tdd_d2d(void* dst, const void* src, size_t bytes);
+int  tdd_fill(void* dst, uint64_t pattern, size_t bytes);  /* kernel fill, 8-byte pattern */
+int  tdd_sync(void);
+int  tdd_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Kernel timing with HIP events on the library stream.  While enabled,
+ * every launch is bracketed by two events; tdd_prof_read() synchronises and
+ * returns per-kernel totals (name, launches, total ms). */
+typedef struct { char name[48]; int64_t launches; double total_ms; } tdd_prof_row_t;
+int  tdd_prof_enable(int on);
+int  tdd_prof_reset(void);
+int  tdd_prof_read(tdd_prof_row_t* rows, int cap, int* n_rows);
+
+/* ------------------------------------------------------------- columns -- */
+
+typedef struct {
+    const void* data;   /* device address of element 0 (NULL = no column) */
+    int8_t   type;      /* td.h type tag */
+    uint8_t  attrs;     /* TD_SYM index width in bits 0-1 */
+    uint8_t  esz;       /* element bytes */
+    uint8_t  pad;
+} tdd_col_t;
+
+/* A device TD_SEL: pointers into one device copy of the td.h TD_SEL payload
+ * (meta | seg_flags | seg_popcnt | bits, td.h:546-589). */
+typedef struct {
+    void*     base;
+    uint8_t*  flags;
+    uint16_t* popcnt;
+    uint64_t* bits;
+    int64_t   nrows;
+    uint32_t  n_segs;
+    uint32_t  pad;
+} tdd_sel_t;
+
+/* ----------------------------------------------------------- predicates -- */
+/* A filter predicate compiled to postfix form over up to 16 columns:
+ * comparisons of a column with a constant or another column, combined by
+ * AND / OR / NOT.  Comparisons run in int64 when both sides are integer
+ * typed and in f64 when either is F64, the reference's typing rule
+ * (exec.c:760-918, 982-1051). */
+#define TDD_PRED_MAX_INS  32
+#define TDD_PRED_MAX_COLS 16
+enum {
+    TDD_P_CMP_CONST = 1,   /* push col[a] CMP constant */
+    TDD_P_CMP_COL   = 2,   /* push col[a] CMP col[b] */
+    TDD_P_AND       = 3,
+    TDD_P_OR        = 4,
+    TDD_P_NOT       = 5,
+    TDD_P_CONST     = 6,   /* push ci != 0 */
+    TDD_P_BOOLCOL   = 7    /* push col[a] != 0 (a BOOL / U8 column) */
+};
+typedef struct {
+    uint8_t kind, cmp, a, b;      /* cmp: OP_EQ..OP_GE */
+    uint8_t as_f64, pad[3];
+    int64_t ci;
+    double  cf;
+} tdd_pred_ins_t;
+typedef struct {
+    int32_t        n_ins, n_cols;
+    tdd_col_t      cols[TDD_PRED_MAX_COLS];
+    tdd_pred_ins_t ins[TDD_PRED_MAX_INS];
+} tdd_pred_t;
+
+/* Evaluate `pred` over rows [0, out->nrows) into a device TD_SEL, ANDed with
+ * `and_with` when given: wave64 ballots form the bit words directly, popcounts
+ * give segment flags and counts.  Replaces expr_eval_full + td_sel_from_pred
+ * (+ td_sel_and for chained filters): exec.c:1254-1273, sel.c:128-161,
+ * sel.c:167-189, exec.c:11706-11731. */
+int tdd_sel_eval(const tdd_pred_t* pred, const tdd_sel_t* and_with, tdd_sel_t* out);
+/* BOOL bytes -> TD_SEL (td_sel_from_pred, sel.c:128-161). */
+int tdd_sel_from_bool(const uint8_t* bools, tdd_sel_t* out);
+/* a AND b (td_sel_and, sel.c:167-189). */
+int tdd_sel_and(const tdd_sel_t* a, const tdd_sel_t* b, tdd_sel_t* out);
+/* Rebuild flags / popcounts / total from the bits (td_sel_recompute, sel.c:77-122). */
+int tdd_sel_recompute(tdd_sel_t* s);
+/* total_pass read back to the host. */
+int tdd_sel_total(const tdd_sel_t* s, int64_t* total);
+
+/* ------------------------------------------------- compaction / gather -- */
+
+/* Row ids of the selected rows, ascending, into idx (total_pass entries):
+ * the match_idx build of sel_compact (exec.c:2362-2388). */
+int tdd_sel_index(const tdd_sel_t* s, int64_t* idx);
+/* Stream compaction of up to 16 columns through a selection, fused with the
+ * index build: dst[c][j] = src[c][row of j-th selected bit].  Replaces
+ * sel_compact + multi_gather_fn (exec.c:2328-2480, 1852-1899). */
+int tdd_sel_compact(const tdd_sel_t* s, int ncols, const tdd_col_t* src, void* const* dst);
+/* dst[i] = src[idx[i]]; idx < 0 writes zeros when nullable (gather_fn,
+ * exec.c:1910-1939). */
+int tdd_gather(const tdd_col_t* src, const int64_t* idx, int64_t n, int nullable, void* dst);
+
+/* -------------------------------------------------------------- group-by -- */
+
+#define TDD_MAX_KEYS 8
+#define TDD_MAX_AGGS 8
+
+typedef struct {
+    int64_t           nrows;
+    const tdd_sel_t*  mask;               /* NULL: every row */
+    int32_t           n_keys, n_aggs;
+    tdd_col_t         keys[TDD_MAX_KEYS];
+    uint16_t          ops[TDD_MAX_AGGS];  /* OP_SUM ... OP_VAR_POP */
+    tdd_col_t         aggs[TDD_MAX_AGGS]; /* data may be NULL for OP_COUNT */
+    int32_t           force_path;         /* 0 auto, 1 direct array, 2 hash */
+    int32_t           pad;
+} tdd_group_spec_t;
+
+typedef struct {
+    int64_t  n_groups;
+    void*    keys[TDD_MAX_KEYS];   /* arena blocks; element type = key column type */
+    void*    vals[TDD_MAX_AGGS];   /* arena blocks; element type = out_types[a] */
+    int8_t   out_types[TDD_MAX_AGGS];
+    int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash */
+    int32_t  pad;
+} tdd_group_result_t;
+
+/* Masked min/max of an integer key column (minmax_scan_fn, exec.c:4544-4604);
+ * *mn > *mx when no row is selected. */
+int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
+               int64_t* mn, int64_t* mx);
+/* Group-by with COUNT / SUM / AVG / MIN / MAX / FIRST / LAST / VAR family.
+ * Integer keys whose masked ranges multiply to a modest slot count use a
+ * direct array (LDS-private per workgroup when it fits, else global atomics),
+ * the device form of the reference's DA path (exec.c:5948-6404); other keys
+ * use a global open-addressing table with LDS pre-aggregation, the device
+ * form of its hash paths (exec.c:3868-4526, 6406-6856).  Output types and
+ * AVG / VAR arithmetic follow emit_agg_columns (exec.c:4632-4773).  Groups
+ * come out in ascending key order on the direct-array paths. */
+int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out);
+
+/* ------------------------------------------------------------------ join -- */
+
+typedef struct {
+    int64_t           nl, nr;
+    int32_t           n_keys, join_type;  /* 0 inner, 1 left, 2 full */
+    tdd_col_t         lkeys[TDD_MAX_KEYS];
+    tdd_col_t         rkeys[TDD_MAX_KEYS];
+    const tdd_sel_t*  lsel;               /* left rows that take part; NULL: all */
+    int32_t           force_path;         /* 0 auto, 1 direct map, 2 radix */
+    int32_t           radix_bits;         /* 0: chosen from the build size */
+} tdd_join_spec_t;
+
+typedef struct {
+    int64_t  n_pairs;
+    int64_t* l_idx;       /* arena blocks, n_pairs each; -1 = no partner */
+    int64_t* r_idx;
+    int32_t  path;        /* 1 direct map, 2 radix partitioned */
+    int32_t  radix_bits;
+} tdd_join_result_t;
+
+/* Equi-join producing row-index pairs.  Dense unique integer build keys use
+ * a direct-indexed chained map; everything else a two-pass radix partition
+ * (LDS histograms, LDS-staged coalesced scatter) followed by per-partition
+ * build + probe with the hash table in LDS.  Replaces exec_join's radix and
+ * chained paths (exec.c:7431-8392); pairs of a left row appear in one
+ * contiguous run, unmatched left rows (left / full) get r = -1 and unmatched
+ * right rows (full) come last with l = -1. */
+int tdd_join(const tdd_join_spec_t* spec, tdd_join_result_t* out);
+
+/* Fused filter -> join -> group-by: the probe side is filtered by `lpred`
+ * (and/or `join.lsel`), probed against the build side and aggregated into
+ * the group table without materialising the filtered table or the join
+ * output.  Group keys and aggregate inputs name a side (0 left, 1 right) and
+ * a column.  Has no single reference counterpart: it computes what the
+ * reference computes with JOIN(FILTER(..)) followed by GROUP on the join
+ * result in a second td_execute (exec.c:11840-11857 then 11759-11838). */
+typedef struct {
+    tdd_join_spec_t   join;               /* inner join */
+    const tdd_pred_t* lpred;              /* NULL: no filter */
+    int32_t           n_keys, n_aggs;
+    tdd_col_t         keys[TDD_MAX_KEYS];
+    uint8_t           key_side[TDD_MAX_KEYS];
+    uint16_t          ops[TDD_MAX_AGGS];
+    tdd_col_t         aggs[TDD_MAX_AGGS];
+    uint8_t           agg_side[TDD_MAX_AGGS];
+} tdd_join_group_spec_t;
+
+int tdd_join_group(const tdd_join_group_spec_t* spec, tdd_group_result_t* out);
+
+/* ------------------------------------------------- host services (C side) -- */
+/* Implemented by the host engine (arena.c); the device layer allocates its
+ * scratch and result blocks from the HBM arena through these. */
+void* td__arena_alloc(size_t bytes);
+void  td__arena_free(void* dptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

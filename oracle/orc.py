@@ -1,0 +1,193 @@
+"""ctypes front end of the oracle library (oracle/_build/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as a checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+TD_BOOL, TD_U8, TD_I16, TD_I32, TD_I64, TD_F64, TD_SYM = 1, 2, 4, 5, 6, 7, 20
+OP = dict(EQ=25, NE=26, LT=27, LE=28, GT=29, GE=30, AND=31, OR=32, NOT=12,
+          SUM=50, PROD=51, MIN=52, MAX=53, COUNT=54, AVG=55, FIRST=56, LAST=57,
+          STDDEV=59, STDDEV_POP=73, VAR=74, VAR_POP=75)
+
+_TYPE_OF = {np.dtype(np.uint8): (TD_U8, 0), np.dtype(np.bool_): (TD_BOOL, 0),
+            np.dtype(np.int16): (TD_I16, 0), np.dtype(np.int32): (TD_I32, 0),
+            np.dtype(np.int64): (TD_I64, 0), np.dtype(np.float64): (TD_F64, 0),
+            np.dtype(np.uint16): (TD_SYM, 1), np.dtype(np.uint32): (TD_SYM, 2)}
+
+
+class _Col(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("type", C.c_int8), ("attrs", C.c_uint8)]
+
+
+class _GroupOut(C.Structure):
+    _fields_ = [("keys", C.c_void_p), ("vals", C.c_void_p), ("sum_abs", C.c_void_p),
+                ("comp", C.c_void_p), ("out_types", C.c_int8 * 8), ("order", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.orc_sel_payload_bytes.restype = C.c_int64
+        L.orc_sel_payload_bytes.argtypes = [C.c_int64]
+        L.orc_sel_from_bool.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_sel_and.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_sel_to_idx.restype = C.c_int64
+        L.orc_sel_to_idx.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_cmp_const.argtypes = [_Col, C.c_int64, C.c_int, C.c_int, C.c_int64,
+                                    C.c_double, C.c_void_p]
+        L.orc_cmp_cols.argtypes = [_Col, _Col, C.c_int64, C.c_int, C.c_void_p]
+        L.orc_bool_op.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_gather.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.c_int]
+        L.orc_join.restype = C.c_int64
+        L.orc_join.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_int64,
+                               C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_group.restype = C.c_int64
+        L.orc_group.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                C.c_int64, C.c_void_p, C.c_int64, C.POINTER(_GroupOut)]
+        _lib = L
+    return _lib
+
+
+def _col(a, type_tag=None, attrs=None):
+    if a is None:
+        return _Col(None, 0, 0)
+    t, at = _TYPE_OF[a.dtype]
+    if type_tag is not None:
+        t = type_tag
+    if attrs is not None:
+        at = attrs
+    return _Col(a.ctypes.data, t, at)
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def sel_from_bool(pred):
+    pred = np.ascontiguousarray(pred, dtype=np.uint8)
+    n = len(pred)
+    out = np.zeros(lib().orc_sel_payload_bytes(n), np.uint8)
+    lib().orc_sel_from_bool(_ptr(pred), n, _ptr(out))
+    return out
+
+
+def sel_and(a, b, n):
+    out = np.zeros(lib().orc_sel_payload_bytes(n), np.uint8)
+    lib().orc_sel_and(_ptr(a), _ptr(b), n, _ptr(out))
+    return out
+
+
+def sel_parts(payload, n):
+    """Split a TD_SEL payload into (total, flags, popcnt, bits)."""
+    nseg = (n + 1023) // 1024
+    al8 = lambda x: (x + 7) & ~7
+    total = int(payload[:8].view(np.int64)[0])
+    flags = payload[16:16 + nseg]
+    o = 16 + al8(nseg)
+    pop = payload[o:o + 2 * nseg].view(np.uint16)
+    o += al8(2 * nseg)
+    bits = payload[o:o + 8 * ((n + 63) // 64)].view(np.uint64)
+    return total, flags, pop, bits
+
+
+def sel_to_idx(payload, n):
+    idx = np.zeros(max(n, 1), np.int64)
+    m = lib().orc_sel_to_idx(_ptr(payload), n, _ptr(idx))
+    return idx[:m]
+
+
+def cmp_const(col, op, const, type_tag=None, attrs=None):
+    col = np.ascontiguousarray(col)
+    out = np.zeros(len(col), np.uint8)
+    is_f = isinstance(const, float)
+    lib().orc_cmp_const(_col(col, type_tag, attrs), len(col), OP[op], int(is_f),
+                        0 if is_f else int(const), float(const), _ptr(out))
+    return out
+
+
+def cmp_cols(a, b, op):
+    out = np.zeros(len(a), np.uint8)
+    lib().orc_cmp_cols(_col(a), _col(b), len(a), OP[op], _ptr(out))
+    return out
+
+
+def bool_op(op, a, b=None):
+    out = np.zeros(len(a), np.uint8)
+    lib().orc_bool_op(OP[op], _ptr(a), _ptr(b if b is not None else a), len(a), _ptr(out))
+    return out
+
+
+def gather(src, idx, nullable=False):
+    src = np.ascontiguousarray(src)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    dst = np.zeros(len(idx), src.dtype)
+    lib().orc_gather(_ptr(src), src.dtype.itemsize, _ptr(idx), len(idx), _ptr(dst),
+                     int(nullable))
+    return dst
+
+
+def join(lkeys, rkeys, join_type=0):
+    """Pairs (l_idx, r_idx) of the equi-join of key column lists."""
+    nk = len(lkeys)
+    lc = (_Col * nk)(*[_col(np.ascontiguousarray(k)) for k in lkeys])
+    rc = (_Col * nk)(*[_col(np.ascontiguousarray(k)) for k in rkeys])
+    nl, nr = len(lkeys[0]), len(rkeys[0])
+    m = lib().orc_join(C.addressof(lc), C.addressof(rc), nk, nl, nr, join_type,
+                       None, None, 0)
+    if m < 0:
+        raise ValueError("unsupported key types")
+    li = np.zeros(max(m, 1), np.int64)
+    ri = np.zeros(max(m, 1), np.int64)
+    lib().orc_join(C.addressof(lc), C.addressof(rc), nk, nl, nr, join_type,
+                   _ptr(li), _ptr(ri), m)
+    return li[:m], ri[:m]
+
+
+def group(keys, ops, aggs, mask_bits=None, key_types=None):
+    """Group-by; returns dict(keys=[arrays], vals=[arrays], sum_abs, comp,
+    out_types, order).  keys/aggs are numpy arrays (aggs may hold None for
+    COUNT inputs that have no column)."""
+    nk, na = len(keys), len(ops)
+    n = len(keys[0])
+    kt = key_types or [None] * nk
+    kc = (_Col * nk)(*[_col(np.ascontiguousarray(k), *(kt[i] or (None, None)))
+                       for i, k in enumerate(keys)])
+    ac = (_Col * max(na, 1))(*[_col(np.ascontiguousarray(a) if a is not None else None)
+                               for a in aggs])
+    opsa = (C.c_uint16 * max(na, 1))(*[OP[o] for o in ops])
+    cap = n + 1
+    kbuf = np.zeros(nk * cap, np.int64)
+    vbuf = np.zeros(max(na, 1) * cap, np.uint64)
+    abuf = np.zeros(max(na, 1) * cap, np.float64)
+    cbuf = np.zeros(max(na, 1) * cap, np.float64)
+    out = _GroupOut(_ptr(kbuf), _ptr(vbuf), _ptr(abuf), _ptr(cbuf))
+    mb = np.ascontiguousarray(mask_bits, dtype=np.uint64) if mask_bits is not None else None
+    g = lib().orc_group(C.addressof(kc), nk, C.addressof(opsa), C.addressof(ac), na, n,
+                        _ptr(mb), cap, C.byref(out))
+    if g < 0:
+        raise ValueError("unsupported aggregate")
+    res = dict(order=out.order, out_types=[out.out_types[a] for a in range(na)])
+    res["keys"] = [kbuf[k * cap:k * cap + g].copy() for k in range(nk)]
+    vals, sabs, comp = [], [], []
+    for a in range(na):
+        raw = vbuf[a * cap:a * cap + g].copy()
+        vals.append(raw.view(np.float64) if res["out_types"][a] == TD_F64
+                    else raw.view(np.int64))
+        sabs.append(abuf[a * cap:a * cap + g].copy())
+        comp.append(cbuf[a * cap:a * cap + g].copy())
+    res["vals"], res["sum_abs"], res["comp"] = vals, sabs, comp
+    return res

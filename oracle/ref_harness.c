@@ -1,0 +1,404 @@
+/*
+ * ref_harness.c -- test infrastructure only.
+ *
+ * Drives the reference library (oracle/_ref/libteide_ref.so, compiled from
+ * the reference's own sources) through its public td.h API on the benchmark
+ * workloads, to (a) write golden result tables for the parity tests and
+ * (b) time the reference's multithreaded CPU path for the benchmark's
+ * cpu_baseline.  Data comes from the shared recipes in datagen.c, so the
+ * reference sees the same bytes as the GPU path.
+ *
+ * The DAG shapes follow the reference's own idioms: a FILTER over a
+ * CONST_TABLE executed to a compacted table (exec.c:11706-11737 +
+ * td_execute's final compaction, exec.c:12312-12319), JOIN with a filtered
+ * left side (exec.c:11840-11857), and GROUP on a table bound to a new graph
+ * (exec.c:11759-11838).
+ *
+ * Usage (all sizes in rows):
+ *   ref_harness fg     N SEED OUT             filter flag==0, group region
+ *   ref_harness fgsel  N SEED OUT             same, GROUP with g->selection set
+ *   ref_harness jg     NFACT NDIM NCAT SEED OUT [JOINOUT]
+ *   ref_harness hc     N NVALUES NSYM SEED OUTKEY OUTSYM
+ *   ref_harness join   NL NR KEYRANGE JTYPE SEED OUT
+ *   ref_harness bench  WORKLOAD ARGS... REPS  -> one JSON line (median ms)
+ * Output files use the TDT1 column format read by tests/tdtio.py.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <teide/td.h>
+#include "datagen.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static unsigned g_ref_threads;           /* pool workers + the caller */
+
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
+}
+
+static void die(const char* what) {
+    fprintf(stderr, "ref_harness: %s\n", what);
+    exit(2);
+}
+
+static td_t* check(td_t* v, const char* what) {
+    if (!v || TD_IS_ERR(v)) {
+        fprintf(stderr, "ref_harness: %s failed (err %d)\n", what,
+                v ? (int)TD_ERR_CODE(v) : -1);
+        exit(3);
+    }
+    return v;
+}
+
+static td_t* vec_of(int8_t type, int64_t n) {
+    td_t* v = check(td_vec_new(type, n), "td_vec_new");
+    v->len = n;
+    return v;
+}
+
+static td_t* add(td_t* tbl, const char* name, td_t* col) {
+    tbl = check(td_table_add_col(tbl, td_sym_intern(name, strlen(name)), col), "add_col");
+    td_release(col);
+    return tbl;
+}
+
+/* ------------------------------------------------------------- TDT1 out -- */
+
+static void write_table(const char* path, td_t* tbl) {
+    FILE* f = fopen(path, "wb");
+    if (!f) die("cannot open output");
+    uint32_t nc = (uint32_t)td_table_ncols(tbl);
+    fwrite("TDT1", 1, 4, f);
+    fwrite(&nc, 4, 1, f);
+    for (uint32_t c = 0; c < nc; c++) {
+        td_t* col = td_table_get_col_idx(tbl, c);
+        td_t* nm = td_sym_str(td_table_col_name(tbl, c));
+        uint32_t nl = nm ? (uint32_t)td_str_len(nm) : 0;
+        fwrite(&nl, 4, 1, f);
+        if (nl) fwrite(td_str_ptr(nm), 1, nl, f);
+        int8_t t = col->type;
+        uint8_t a = col->attrs;
+        uint16_t pad = 0;
+        int64_t len = col->len;
+        fwrite(&t, 1, 1, f);
+        fwrite(&a, 1, 1, f);
+        fwrite(&pad, 2, 1, f);
+        fwrite(&len, 8, 1, f);
+        size_t esz = td_sym_elem_size(t, a);
+        if (len) fwrite(td_data(col), esz, (size_t)len, f);
+    }
+    fclose(f);
+}
+
+/* ----------------------------------------------------------- workloads -- */
+
+static int g_threads = 8;
+
+static td_t* make_fg(int64_t n, uint64_t seed) {
+    td_t* flag = vec_of(TD_I64, n);
+    td_t* region = vec_of(TD_I64, n);
+    td_t* amount = vec_of(TD_F64, n);
+    tdg_recipe_fg(n, seed, g_threads, (int64_t*)td_data(flag),
+                  (int64_t*)td_data(region), (double*)td_data(amount));
+    td_t* t = check(td_table_new(3), "table");
+    t = add(t, "flag", flag);
+    t = add(t, "region", region);
+    t = add(t, "amount", amount);
+    return t;
+}
+
+/* GROUP(region; SUM amount, COUNT amount) over tbl, optionally with sel. */
+static td_t* run_group_fg(td_t* tbl, td_t* sel) {
+    td_graph_t* g = td_graph_new(tbl);
+    if (sel) { td_retain(sel); g->selection = sel; }
+    td_op_t* keys[1] = { td_scan(g, "region") };
+    td_op_t* ins[2] = { td_scan(g, "amount"), td_scan(g, "amount") };
+    uint16_t ops[2] = { OP_SUM, OP_COUNT };
+    td_op_t* root = td_group(g, keys, 1, ops, ins, 2);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "group");
+    td_graph_free(g);
+    return out;
+}
+
+static td_t* run_filter_table(td_t* tbl, const char* col, int op, int64_t c) {
+    td_graph_t* g = td_graph_new(tbl);
+    td_op_t* t = td_const_table(g, tbl);
+    td_op_t* s = td_scan(g, col);
+    td_op_t* k = td_const_i64(g, c);
+    td_op_t* p = op == OP_EQ ? td_eq(g, s, k) : td_gt(g, s, k);
+    td_op_t* root = td_filter(g, t, p);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "filter");
+    td_graph_free(g);
+    return out;
+}
+
+static td_t* wl_fg(int64_t n, uint64_t seed) {
+    td_t* t = make_fg(n, seed);
+    td_t* filtered = run_filter_table(t, "flag", OP_EQ, 0);
+    td_t* out = run_group_fg(filtered, NULL);
+    td_release(filtered);
+    td_release(t);
+    return out;
+}
+
+static td_t* wl_fgsel(int64_t n, uint64_t seed) {
+    td_t* t = make_fg(n, seed);
+    td_t* flag = td_table_get_col(t, td_sym_intern("flag", 4));
+    td_t* pred = vec_of(TD_BOOL, n);
+    for (int64_t i = 0; i < n; i++)
+        ((uint8_t*)td_data(pred))[i] = ((int64_t*)td_data(flag))[i] == 0;
+    td_t* sel = check(td_sel_from_pred(pred), "sel");
+    td_release(pred);
+    td_t* out = run_group_fg(t, sel);
+    td_release(sel);
+    td_release(t);
+    return out;
+}
+
+static void make_jg(int64_t nf, int64_t nd, int64_t ncat, uint64_t seed,
+                    td_t** fact_out, td_t** dim_out) {
+    td_t* pk = vec_of(TD_I64, nd);
+    td_t* cat = vec_of(TD_I32, nd);
+    tdg_recipe_dim(nd, ncat, seed, 200, g_threads, (int64_t*)td_data(pk), (int32_t*)td_data(cat));
+    td_t* fk = vec_of(TD_I64, nf);
+    td_t* qty = vec_of(TD_I64, nf);
+    td_t* price = vec_of(TD_F64, nf);
+    tdg_recipe_fact(nf, nd, seed, 200, g_threads, (int64_t*)td_data(fk),
+                    (int64_t*)td_data(qty), (double*)td_data(price));
+    td_t* dim = check(td_table_new(2), "dim");
+    dim = add(dim, "pk", pk);
+    dim = add(dim, "cat", cat);
+    td_t* fact = check(td_table_new(3), "fact");
+    fact = add(fact, "fk", fk);
+    fact = add(fact, "qty", qty);
+    fact = add(fact, "price", price);
+    *fact_out = fact;
+    *dim_out = dim;
+}
+
+/* JOIN(FILTER(CONST_TABLE fact, GT(SCAN qty, 50)), [SCAN fk],
+ *      CONST_TABLE dim, [SCAN pk], inner) */
+static td_t* run_jg_join(td_t* fact, td_t* dim) {
+    td_graph_t* g = td_graph_new(fact);
+    td_op_t* ft = td_const_table(g, fact);
+    td_op_t* dt = td_const_table(g, dim);
+    td_op_t* pred = td_gt(g, td_scan(g, "qty"), td_const_i64(g, 50));
+    td_op_t* flt = td_filter(g, ft, pred);
+    td_op_t* lk[1] = { td_scan(g, "fk") };
+    td_op_t* rk[1] = { td_scan(g, "pk") };
+    td_op_t* root = td_join(g, flt, lk, dt, rk, 1, 0);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "join");
+    td_graph_free(g);
+    return out;
+}
+
+static td_t* run_jg_group(td_t* joined) {
+    td_graph_t* g = td_graph_new(joined);
+    td_op_t* keys[1] = { td_scan(g, "cat") };
+    td_op_t* ins[3] = { td_scan(g, "price"), td_scan(g, "qty"), td_scan(g, "price") };
+    uint16_t ops[3] = { OP_SUM, OP_SUM, OP_COUNT };
+    td_op_t* root = td_group(g, keys, 1, ops, ins, 3);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "group");
+    td_graph_free(g);
+    return out;
+}
+
+static void make_hc(int64_t n, int64_t nv, uint32_t nsym, uint64_t seed, td_t** out) {
+    td_t* key = vec_of(TD_I64, n);
+    td_t* sym = check(td_sym_vec_new(TD_SYM_W32, n), "sym");
+    sym->len = n;
+    td_t* vi = vec_of(TD_I64, n);
+    td_t* vf = vec_of(TD_F64, n);
+    tdg_recipe_hc(n, nv, 1.1, nsym, seed, g_threads, (int64_t*)td_data(key),
+                  (uint32_t*)td_data(sym), (int64_t*)td_data(vi), (double*)td_data(vf));
+    td_t* t = check(td_table_new(4), "hc");
+    t = add(t, "key", key);
+    t = add(t, "sym", sym);
+    t = add(t, "v_i64", vi);
+    t = add(t, "v_f64", vf);
+    *out = t;
+}
+
+/* GROUP(k; SUM/MIN/MAX v_i64, SUM/MIN/MAX v_f64, COUNT v_i64) */
+static td_t* run_hc_group(td_t* t, const char* k) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* keys[1] = { td_scan(g, k) };
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* ins[7] = { vi, vi, vi, vf, vf, vf, vi };
+    uint16_t ops[7] = { OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_MIN, OP_MAX, OP_COUNT };
+    td_op_t* root = td_group(g, keys, 1, ops, ins, 7);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "hc group");
+    td_graph_free(g);
+    return out;
+}
+
+/* General equi-join on one i64 key: left (k, lv), right (k, rv). */
+static td_t* run_join(int64_t nl, int64_t nr, int64_t krange, int jtype, uint64_t seed) {
+    td_t* lk = vec_of(TD_I64, nl);
+    td_t* lv = vec_of(TD_I64, nl);
+    td_t* rk = vec_of(TD_I64, nr);
+    td_t* rv = vec_of(TD_F64, nr);
+    tdg_i64_uniform((int64_t*)td_data(lk), nl, 0, krange - 1, seed, 601, g_threads);
+    tdg_i64_uniform((int64_t*)td_data(lv), nl, -1000000, 1000000, seed, 602, g_threads);
+    tdg_i64_uniform((int64_t*)td_data(rk), nr, 0, krange - 1, seed, 603, g_threads);
+    tdg_f64_uniform((double*)td_data(rv), nr, -1.0, 1.0, seed, 604, g_threads);
+    td_t* l = check(td_table_new(2), "l");
+    l = add(l, "k", lk);
+    l = add(l, "lv", lv);
+    td_t* r = check(td_table_new(2), "r");
+    r = add(r, "k", rk);
+    r = add(r, "rv", rv);
+    td_graph_t* g = td_graph_new(l);
+    td_op_t* lt = td_const_table(g, l);
+    td_op_t* rt = td_const_table(g, r);
+    td_op_t* key[1] = { td_scan(g, "k") };
+    td_op_t* root = td_join(g, lt, key, rt, key, 1, (uint8_t)jtype);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "join");
+    td_graph_free(g);
+    td_release(l);
+    td_release(r);
+    return out;
+}
+
+/* ---------------------------------------------------------------- bench -- */
+
+static int cmp_d(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : x > y;
+}
+
+static void bench(int argc, char** argv) {
+    if (argc < 4) die("bench: WORKLOAD ARGS... REPS");
+    const char* wl = argv[2];
+    int reps = atoi(argv[argc - 1]);
+    if (reps < 1) reps = 1;
+    if (reps > 64) reps = 64;
+    double t[64];
+    int64_t rows = 0;
+    if (!strcmp(wl, "jg")) {
+        int64_t nf = atoll(argv[3]), nd = atoll(argv[4]), ncat = atoll(argv[5]);
+        td_t *fact, *dim;
+        make_jg(nf, nd, ncat, 42, &fact, &dim);
+        rows = nf;
+        for (int r = -1; r < reps; r++) {             /* r = -1 is the warm-up */
+            double t0 = now_ms();
+            td_t* j = run_jg_join(fact, dim);
+            td_t* o = run_jg_group(j);
+            double t1 = now_ms();
+            td_release(j);
+            td_release(o);
+            if (r >= 0) t[r] = t1 - t0;
+        }
+        td_release(fact);
+        td_release(dim);
+    } else if (!strcmp(wl, "fg")) {
+        int64_t n = atoll(argv[3]);
+        td_t* tb = make_fg(n, 42);
+        rows = n;
+        for (int r = -1; r < reps; r++) {
+            double t0 = now_ms();
+            td_t* f = run_filter_table(tb, "flag", OP_EQ, 0);
+            td_t* o = run_group_fg(f, NULL);
+            double t1 = now_ms();
+            td_release(f);
+            td_release(o);
+            if (r >= 0) t[r] = t1 - t0;
+        }
+        td_release(tb);
+    } else if (!strcmp(wl, "hc")) {
+        int64_t n = atoll(argv[3]), nv = atoll(argv[4]);
+        uint32_t nsym = (uint32_t)atoll(argv[5]);
+        td_t* tb;
+        make_hc(n, nv, nsym, 42, &tb);
+        rows = n;
+        for (int r = -1; r < reps; r++) {
+            double t0 = now_ms();
+            td_t* a = run_hc_group(tb, "key");
+            td_t* b = run_hc_group(tb, "sym");
+            double t1 = now_ms();
+            td_release(a);
+            td_release(b);
+            if (r >= 0) t[r] = t1 - t0;
+        }
+        td_release(tb);
+    } else {
+        die("bench: unknown workload");
+    }
+    qsort(t, (size_t)reps, sizeof(double), cmp_d);
+    double med = t[reps / 2];
+    printf("{\"workload\": \"%s\", \"rows\": %lld, \"reps\": %d, \"median_ms\": %.4f, "
+           "\"min_ms\": %.4f, \"threads\": %u}\n",
+           wl, (long long)rows, reps, med, t[0], g_ref_threads);
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) die("usage: ref_harness CMD ...");
+    td_heap_init();
+    td_sym_init();
+    /* TD_REF_THREADS=T (T >= 2): T-1 pool workers + the caller; otherwise
+     * the reference's default of nproc-1 workers (pool.c:107-110). */
+    const char* tenv = getenv("TD_REF_THREADS");
+    int treq = tenv ? atoi(tenv) : 0;
+    if (treq >= 2) {
+        td_pool_init((uint32_t)(treq - 1));
+        g_ref_threads = (unsigned)treq;
+    } else {
+        td_pool_init(0);
+        g_ref_threads = td_thread_count();
+    }
+    const char* cmd = argv[1];
+    if (!strcmp(cmd, "fg") && argc == 5) {
+        td_t* o = wl_fg(atoll(argv[2]), strtoull(argv[3], NULL, 10));
+        write_table(argv[4], o);
+        td_release(o);
+    } else if (!strcmp(cmd, "fgsel") && argc == 5) {
+        td_t* o = wl_fgsel(atoll(argv[2]), strtoull(argv[3], NULL, 10));
+        write_table(argv[4], o);
+        td_release(o);
+    } else if (!strcmp(cmd, "jg") && (argc == 7 || argc == 8)) {
+        td_t *fact, *dim;
+        make_jg(atoll(argv[2]), atoll(argv[3]), atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &fact, &dim);
+        td_t* j = run_jg_join(fact, dim);
+        if (argc == 8) write_table(argv[7], j);
+        td_t* o = run_jg_group(j);
+        write_table(argv[6], o);
+        td_release(o);
+        td_release(j);
+        td_release(fact);
+        td_release(dim);
+    } else if (!strcmp(cmd, "hc") && argc == 8) {
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        td_t* a = run_hc_group(t, "key");
+        write_table(argv[6], a);
+        td_t* b = run_hc_group(t, "sym");
+        write_table(argv[7], b);
+        td_release(a);
+        td_release(b);
+        td_release(t);
+    } else if (!strcmp(cmd, "join") && argc == 8) {
+        td_t* o = run_join(atoll(argv[2]), atoll(argv[3]), atoll(argv[4]), atoi(argv[5]),
+                           strtoull(argv[6], NULL, 10));
+        write_table(argv[7], o);
+        td_release(o);
+    } else if (!strcmp(cmd, "bench")) {
+        bench(argc, argv);
+    } else {
+        die("bad command or argument count");
+    }
+    td_pool_destroy();
+    return 0;
+}

@@ -1,0 +1,591 @@
+/*
+ * k_group.hip -- group-by kernels.
+ *
+ * Direct-array path (integer keys with a small slot space, the device form
+ * of the reference's DA path, exec.c:5948-6404): slot = sum_k (key_k -
+ * min_k) * stride_k.  When the slot table fits in LDS every workgroup
+ * aggregates into its own LDS copy with LDS atomics and writes it out as a
+ * partial; a second kernel folds the partials in workgroup order.  Larger
+ * slot spaces aggregate with global atomics into one table.
+ *
+ * Hash path (everything else, the device form of exec.c:3868-4526): a
+ * global open-addressing table of 64-bit group keys sized from a
+ * HyperLogLog estimate, with cells updated by global atomics.  A small
+ * per-workgroup LDS table absorbs heavy hitters first (a Zipf key can own
+ * a tenth of all rows) and is merged into the global table at the end.
+ */
+#include "group_plan.h"
+
+#include <math.h>
+#include <stdio.h>
+
+#define GT 512                      /* threads per group-by workgroup */
+#define LDS_TABLE_BYTES (64 * 1024) /* LDS slot table budget per workgroup */
+
+static bool is_int_type(int8_t t) {
+    return t == TD_I64 || t == TD_SYM || t == TD_I32 || t == TD_TIMESTAMP || t == TD_DATE ||
+           t == TD_TIME || t == TD_BOOL || t == TD_U8 || t == TD_I16;
+}
+
+int plan_cells(GroupPlan& P) {
+    int n = 1;
+    P.kind[0] = C_COUNT;
+    P.col[0] = 0;
+    for (int a = 0; a < P.n_aggs; a++) {
+        const tdd_col_t& c = P.aggs[a];
+        bool f = c.data && c.type == TD_F64;
+        uint16_t op = P.ops[a];
+        if (op != OP_COUNT && !c.data) return TD_ERR_NYI;
+        if (op != OP_COUNT && c.type != TD_F64 && !is_int_type(c.type)) return TD_ERR_TYPE;
+        P.cell_of[a] = (uint8_t)n;
+        switch (op) {
+        case OP_COUNT: P.out_types[a] = TD_I64; P.cell_of[a] = 0; break;
+        case OP_SUM:
+            P.out_types[a] = f ? TD_F64 : TD_I64;
+            P.kind[n] = f ? C_ADD_F64 : C_ADD_I64; P.col[n++] = (uint8_t)a; break;
+        case OP_AVG:
+            P.out_types[a] = TD_F64;
+            P.kind[n] = f ? C_ADD_F64 : C_ADD_I64; P.col[n++] = (uint8_t)a; break;
+        case OP_MIN:
+            P.out_types[a] = c.type;
+            P.kind[n] = f ? C_MIN_F64 : C_MIN_I64; P.col[n++] = (uint8_t)a; break;
+        case OP_MAX:
+            P.out_types[a] = c.type;
+            P.kind[n] = f ? C_MAX_F64 : C_MAX_I64; P.col[n++] = (uint8_t)a; break;
+        case OP_FIRST:
+            P.out_types[a] = c.type;
+            P.kind[n] = C_MIN_ROW; P.col[n++] = (uint8_t)a; break;
+        case OP_LAST:
+            P.out_types[a] = c.type;
+            P.kind[n] = C_MAX_ROW; P.col[n++] = (uint8_t)a; break;
+        case OP_VAR: case OP_VAR_POP: case OP_STDDEV: case OP_STDDEV_POP:
+            P.out_types[a] = TD_F64;
+            P.kind[n] = f ? C_ADD_F64 : C_ADD_I64; P.col[n++] = (uint8_t)a;
+            P.kind[n] = C_SQ_F64; P.col[n++] = (uint8_t)a; break;
+        default:
+            return TD_ERR_NYI;       /* PROD, COUNT_DISTINCT */
+        }
+    }
+    P.ncells = n;
+    return TD_OK;
+}
+
+int alloc_group_outputs(const GroupPlan& P, int64_t n, tdd_group_result_t* out) {
+    for (int k = 0; k < P.n_keys; k++) {
+        out->keys[k] = td__arena_alloc((size_t)(n > 0 ? n : 1) * P.keys[k].esz);
+        if (!out->keys[k]) return TD_ERR_OOM;
+    }
+    for (int a = 0; a < P.n_aggs; a++) {
+        int esz = P.out_types[a] == TD_F64 || P.out_types[a] == TD_I64 ? 8 : P.aggs[a].esz;
+        out->vals[a] = td__arena_alloc((size_t)(n > 0 ? n : 1) * esz);
+        if (!out->vals[a]) return TD_ERR_OOM;
+        out->out_types[a] = P.out_types[a];
+    }
+    out->n_groups = n;
+    return TD_OK;
+}
+
+/* ---------------------------------------------------------- row walking -- */
+/* Every kernel walks rows in 64-row words (one per wave step) so the
+ * selection word can gate a whole wave at once. */
+
+struct RowMask {
+    const uint64_t* bits;   /* NULL: all rows */
+    int64_t nrows;
+};
+
+__device__ __forceinline__ uint64_t word_mask(const RowMask& M, int64_t word) {
+    int64_t base = word * 64;
+    uint64_t m = M.bits ? M.bits[word] : ~0ull;
+    int64_t left = M.nrows - base;
+    if (left < 64) m &= (1ull << left) - 1;
+    return m;
+}
+
+/* ---------------------------------------------------------------- minmax -- */
+
+__global__ __launch_bounds__(256) void k_minmax(tdd_col_t col, RowMask M, long long* out) {
+    int64_t n_words = (M.nrows + 63) >> 6;
+    int lane = lane_id();
+    long long mn = 0x7fffffffffffffffLL, mx = (long long)0x8000000000000000ull;
+    int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * blockDim.x / WAVE;
+    for (int64_t w = wave_id; w < n_words; w += n_waves) {
+        uint64_t m = word_mask(M, w);
+        if ((m >> lane) & 1) {
+            long long v = ld_i64(col, w * 64 + lane);
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        long long a = __shfl_xor(mn, d, WAVE), b = __shfl_xor(mx, d, WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+        atomicMin(&out[0], mn);
+        atomicMax(&out[1], mx);
+    }
+}
+
+__global__ void k_init2(long long* p) {
+    if (threadIdx.x == 0) { p[0] = 0x7fffffffffffffffLL; p[1] = (long long)0x8000000000000000ull; }
+}
+
+static RowMask row_mask(const tdd_sel_t* s, int64_t nrows) {
+    RowMask M;
+    M.bits = s ? s->bits : nullptr;
+    M.nrows = nrows;
+    return M;
+}
+
+extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
+                          int64_t* mn, int64_t* mx) {
+    SCRATCH(buf, 16);
+    LAUNCH("minmax", k_init2, 1, 64, 0, buf.as<long long>());
+    if (nrows > 0) {
+        int grid = stream_grid(ceil_div(nrows, 256 * 16), 8);
+        LAUNCH("minmax", k_minmax, grid, 256, 0, *col, row_mask(mask, nrows), buf.as<long long>());
+    }
+    long long h[2];
+    HC(hipMemcpyAsync(h, buf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    *mn = h[0];
+    *mx = h[1];
+    return TD_OK;
+}
+
+/* --------------------------------------------------------- direct array -- */
+
+__device__ __forceinline__ int64_t da_slot(const GroupPlan& P, int64_t row) {
+    return da_slot2(P, row, row);
+}
+
+__global__ void k_cells_init(uint64_t* tab, int64_t n_slots, GroupPlan P) {
+    int64_t n = n_slots * P.ncells;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        tab[i] = cell_identity(P.kind[i % P.ncells]);
+}
+
+/* Each workgroup: private LDS table, then a plain store of the table into
+ * its partial slice (merged in workgroup order by k_da_fold). */
+__global__ __launch_bounds__(GT) void k_da_lds(GroupPlan P, RowMask M, int64_t n_slots,
+                                               uint64_t* partials) {
+    extern __shared__ uint64_t tab[];
+    int64_t ncell_total = n_slots * P.ncells;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x)
+        tab[i] = cell_identity(P.kind[i % P.ncells]);
+    __syncthreads();
+    int64_t n_words = (M.nrows + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (GT / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (GT / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        uint64_t m = word_mask(M, w);
+        if ((m >> lane) & 1) {
+            int64_t row = w * 64 + lane;
+            cells_accum(&tab[da_slot(P, row) * P.ncells], P, row);
+        }
+    }
+    __syncthreads();
+    uint64_t* dst = partials + (int64_t)blockIdx.x * ncell_total;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
+}
+
+/* One thread per cell folds that cell of all workgroups' partials in
+ * workgroup order, so the result does not depend on which workgroup
+ * finished first. */
+__global__ void k_da_fold(GroupPlan P, const uint64_t* partials, int nparts, int64_t n_slots,
+                          uint64_t* tab) {
+    int64_t stride = n_slots * P.ncells;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= stride) return;
+    int kind = P.kind[idx % P.ncells];
+    uint64_t acc = partials[idx];
+    for (int b = 1; b < nparts; b++) {
+        uint64_t s = partials[(int64_t)b * stride + idx];
+        switch (kind) {
+        case C_ADD_F64: case C_SQ_F64:
+            acc = (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) +
+                                                 __longlong_as_double((long long)s));
+            break;
+        case C_COUNT: case C_ADD_I64: acc += s; break;
+        case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW:
+            if ((int64_t)s < (int64_t)acc) acc = s;
+            break;
+        default:
+            if ((int64_t)s > (int64_t)acc) acc = s;
+            break;
+        }
+    }
+    tab[idx] = acc;
+}
+
+__global__ __launch_bounds__(GT) void k_da_global(GroupPlan P, RowMask M, uint64_t* tab) {
+    int64_t n_words = (M.nrows + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (GT / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (GT / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        uint64_t m = word_mask(M, w);
+        if ((m >> lane) & 1) {
+            int64_t row = w * 64 + lane;
+            cells_accum(&tab[da_slot(P, row) * P.ncells], P, row);
+        }
+    }
+}
+
+/* Non-empty slot flags for the output compaction. */
+__global__ void k_nonempty(const uint64_t* tab, int64_t n_slots, int ncells, uint32_t* flag) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_slots) flag[s] = tab[s * ncells] != 0;
+}
+
+/* Direct-array emit: keys decoded from the slot, in ascending slot order. */
+__global__ void k_da_emit(GroupPlan P, const uint64_t* tab, int64_t n_slots, const int64_t* pos,
+                          tdd_group_result_t out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const uint64_t* c = tab + s * P.ncells;
+    if (c[0] == 0) return;
+    int64_t g = pos[s];
+    for (int k = 0; k < P.n_keys; k++) {
+        int64_t v = P.kmin[k] + (s / P.kstride[k]) % P.krange[k];
+        st_int(out.keys[k], g, P.keys[k].esz, v);
+    }
+    for (int a = 0; a < P.n_aggs; a++) cells_emit(c, P, a, out.vals[a], g);
+}
+
+/* Table -> outputs, shared by both direct-array variants. */
+int da_emit(const GroupPlan& P, const uint64_t* tab, int64_t n_slots,
+                   tdd_group_result_t* out) {
+    SCRATCH(flag, (size_t)n_slots * 4);
+    SCRATCH(pos, (size_t)(n_slots + 1) * 8);
+    int blocks = (int)ceil_div(n_slots, 256);
+    LAUNCH("group_emit", k_nonempty, blocks, 256, 0, tab, n_slots, P.ncells, flag.as<uint32_t>());
+    int rc = tdd::scan_u32_i64(flag.as<uint32_t>(), n_slots, pos.as<int64_t>());
+    if (rc) return rc;
+    int64_t ng = 0;
+    HC(hipMemcpyAsync(&ng, pos.as<int64_t>() + n_slots, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    rc = alloc_group_outputs(P, ng, out);
+    if (rc) return rc;
+    if (ng > 0)
+        LAUNCH("group_emit", k_da_emit, blocks, 256, 0, P, tab, n_slots,
+               (const int64_t*)pos.as<int64_t>(), *out);
+    return TD_OK;
+}
+
+int64_t g_da_max_slots = (int64_t)1 << 24;
+
+/* Fold per-workgroup LDS partials in workgroup order, then emit. */
+int da_finish(const GroupPlan& P, const uint64_t* partials, int nparts, int64_t n_slots,
+              tdd_group_result_t* out) {
+    SCRATCH(tab, (size_t)n_slots * P.ncells * 8);
+    int fb = (int)ceil_div(n_slots * P.ncells, 256);
+    LAUNCH("group_da_fold", k_da_fold, fb, 256, 0, P, partials, nparts, n_slots,
+           tab.as<uint64_t>());
+    return da_emit(P, tab.as<uint64_t>(), n_slots, out);
+}
+
+/* Identity-initialised global slot table. */
+int da_table_init(const GroupPlan& P, uint64_t* tab, int64_t n_slots) {
+    int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
+    LAUNCH("group_init", k_cells_init, ib, 256, 0, tab, n_slots, P);
+    return TD_OK;
+}
+
+/* Direct-array group-by over rows selected by M with the slot plan in P. */
+static int run_da(GroupPlan& P, const RowMask& M, int64_t n_slots, tdd_group_result_t* out) {
+    size_t table_bytes = (size_t)n_slots * P.ncells * 8;
+    SCRATCH(tab, table_bytes);
+    if (table_bytes <= LDS_TABLE_BYTES) {
+        int per_cu = (int)(160 * 1024 / (table_bytes + 1024));
+        if (per_cu > 4) per_cu = 4;
+        if (per_cu < 1) per_cu = 1;
+        int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), per_cu);
+        SCRATCH(parts, (size_t)grid * table_bytes);
+        LAUNCH("group_da_lds", k_da_lds, grid, GT, table_bytes, P, M, n_slots,
+               parts.as<uint64_t>());
+        int fb = (int)ceil_div(n_slots * P.ncells, 256);
+        LAUNCH("group_da_fold", k_da_fold, fb, 256, 0, P, (const uint64_t*)parts.as<uint64_t>(),
+               grid, n_slots, tab.as<uint64_t>());
+        out->path = 1;
+    } else {
+        int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
+        LAUNCH("group_init", k_cells_init, ib, 256, 0, tab.as<uint64_t>(), n_slots, P);
+        int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), 4);
+        LAUNCH("group_da_global", k_da_global, grid, GT, 0, P, M, tab.as<uint64_t>());
+        out->path = 2;
+    }
+    return da_emit(P, tab.as<uint64_t>(), n_slots, out);
+}
+
+/* ------------------------------------------------------------------ hash -- */
+
+#define HLL_P 12
+#define HLL_M (1 << HLL_P)
+#define EMPTY_KEY 0xffffffffffffffffull
+#define CACHE_SLOTS 256
+#define CACHE_PROBES 4
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+/* 64-bit group key of a row: the single key's bits, or the mixed-radix
+ * composite of several integer keys (stored in kmin / kstride). */
+__device__ __forceinline__ uint64_t ht_key(const GroupPlan& P, int64_t row) {
+    if (P.n_keys == 1 && P.kstride[0] == 0) return (uint64_t)ld_i64(P.keys[0], row);
+    return (uint64_t)da_slot(P, row);
+}
+
+__global__ __launch_bounds__(256) void k_hll(GroupPlan P, RowMask M, unsigned int* regs) {
+    __shared__ unsigned int r[HLL_M];
+    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) r[i] = 0;
+    __syncthreads();
+    int64_t n_words = (M.nrows + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (256 / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        uint64_t m = word_mask(M, w);
+        if ((m >> lane) & 1) {
+            uint64_t h = mix64(ht_key(P, w * 64 + lane));
+            unsigned idx = (unsigned)(h >> (64 - HLL_P));
+            uint64_t rest = (h << HLL_P) | (1ull << (HLL_P - 1));
+            unsigned rank = (unsigned)__clzll((long long)rest) + 1;
+            atomicMax(&r[idx], rank);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x)
+        if (r[i]) atomicMax(&regs[i], r[i]);
+}
+
+/* Slot of key in the global table, inserting it if absent; -1 when full. */
+__device__ __forceinline__ int64_t ht_find(unsigned long long* keys, uint64_t mask, uint64_t key) {
+    if (key == EMPTY_KEY) return (int64_t)mask + 1;        /* the reserved extra slot */
+    uint64_t i = mix64(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        unsigned long long k = keys[i];
+        if (k == key) return (int64_t)i;
+        if (k == EMPTY_KEY) {
+            unsigned long long prev = atomicCAS(&keys[i], EMPTY_KEY, (unsigned long long)key);
+            if (prev == EMPTY_KEY || prev == key) return (int64_t)i;
+        }
+        i = (i + 1) & mask;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(GT) void k_ht_agg(GroupPlan P, RowMask M, unsigned long long* keys,
+                                               uint64_t mask, uint64_t* cells, int* overflow) {
+    extern __shared__ uint64_t sm[];
+    unsigned long long* ckey = (unsigned long long*)sm;
+    uint64_t* ccell = sm + CACHE_SLOTS;
+    const int nc = P.ncells;
+    for (int i = threadIdx.x; i < CACHE_SLOTS; i += blockDim.x) ckey[i] = EMPTY_KEY;
+    for (int i = threadIdx.x; i < CACHE_SLOTS * nc; i += blockDim.x)
+        ccell[i] = cell_identity(P.kind[i % nc]);
+    __syncthreads();
+    int64_t n_words = (M.nrows + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (GT / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (GT / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        uint64_t m = word_mask(M, w);
+        if (!((m >> lane) & 1)) continue;
+        int64_t row = w * 64 + lane;
+        uint64_t key = ht_key(P, row);
+        int c = -1;
+        if (key != EMPTY_KEY) {
+            uint32_t ci = (uint32_t)mix64(key) & (CACHE_SLOTS - 1);
+            for (int p = 0; p < CACHE_PROBES; p++) {
+                unsigned long long k = ckey[ci];
+                if (k == key) { c = (int)ci; break; }
+                if (k == EMPTY_KEY) {
+                    unsigned long long prev = atomicCAS(&ckey[ci], EMPTY_KEY, (unsigned long long)key);
+                    if (prev == EMPTY_KEY || prev == key) { c = (int)ci; break; }
+                }
+                ci = (ci + 1) & (CACHE_SLOTS - 1);
+            }
+        }
+        if (c >= 0) {
+            cells_accum(&ccell[c * nc], P, row);
+        } else {
+            int64_t s = ht_find(keys, mask, key);
+            if (s < 0) { *overflow = 1; continue; }
+            cells_accum(&cells[s * nc], P, row);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < CACHE_SLOTS; i += blockDim.x) {
+        if (ckey[i] == EMPTY_KEY || ccell[i * nc] == 0) continue;
+        int64_t s = ht_find(keys, mask, ckey[i]);
+        if (s < 0) { *overflow = 1; continue; }
+        cells_merge(&cells[s * nc], &ccell[i * nc], P);
+    }
+}
+
+__global__ void k_ht_emit(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
+                          int64_t n_slots, const int64_t* pos, tdd_group_result_t out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const uint64_t* c = cells + s * P.ncells;
+    if (c[0] == 0) return;
+    int64_t g = pos[s];
+    uint64_t key = s == n_slots - 1 ? EMPTY_KEY : keys[s];
+    if (P.n_keys == 1 && P.kstride[0] == 0) {
+        st_int(out.keys[0], g, P.keys[0].esz, (int64_t)key);
+    } else {
+        for (int k = 0; k < P.n_keys; k++) {
+            int64_t v = P.kmin[k] + ((int64_t)key / P.kstride[k]) % P.krange[k];
+            st_int(out.keys[k], g, P.keys[k].esz, v);
+        }
+    }
+    for (int a = 0; a < P.n_aggs; a++) cells_emit(c, P, a, out.vals[a], g);
+}
+
+static double hll_estimate(const unsigned int* regs) {
+    double sum = 0.0;
+    int zeros = 0;
+    for (int i = 0; i < HLL_M; i++) {
+        sum += ldexp(1.0, -(int)regs[i]);
+        zeros += regs[i] == 0;
+    }
+    double m = HLL_M;
+    double e = 0.7213 / (1.0 + 1.079 / m) * m * m / sum;
+    if (e <= 2.5 * m && zeros) e = m * log(m / zeros);     /* linear counting */
+    return e;
+}
+
+static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, tdd_group_result_t* out) {
+    SCRATCH(regs, HLL_M * 4);
+    int frc = tdd_fill(regs.p, 0, HLL_M * 4);
+    if (frc) return frc;
+    int hgrid = stream_grid(ceil_div((M.nrows + 63) / 64, 4 * 8), 4);
+    LAUNCH("group_hll", k_hll, hgrid, 256, 0, P, M, regs.as<unsigned int>());
+    unsigned int h[HLL_M];
+    HC(hipMemcpyAsync(h, regs.p, sizeof(h), hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    double est = hll_estimate(h) * 1.1 + 64;
+    if (est > (double)n_rows_sel) est = (double)n_rows_sel + 1;
+    uint64_t cap = 1024;
+    while ((double)cap < 2.0 * est) cap <<= 1;
+    for (int attempt = 0; attempt < 6; attempt++, cap <<= 1) {
+        int64_t n_slots = (int64_t)cap + 1;                  /* + the EMPTY_KEY slot */
+        SCRATCH(keys, (size_t)cap * 8);
+        SCRATCH(cells, (size_t)n_slots * P.ncells * 8);
+        SCRATCH(ovf, 8);
+        int rc = tdd_fill(keys.p, EMPTY_KEY, (size_t)cap * 8);
+        if (rc) return rc;
+        int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
+        LAUNCH("group_init", k_cells_init, ib, 256, 0, cells.as<uint64_t>(), n_slots, P);
+        rc = tdd_fill(ovf.p, 0, 8);
+        if (rc) return rc;
+        size_t lds = (size_t)CACHE_SLOTS * 8 * (1 + P.ncells);
+        int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), 2);
+        LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
+               (uint64_t)(cap - 1), cells.as<uint64_t>(), ovf.as<int>());
+        int o = 0;
+        HC(hipMemcpyAsync(&o, ovf.p, 4, hipMemcpyDeviceToHost, tdd::g_stream));
+        HC(hipStreamSynchronize(tdd::g_stream));
+        if (o) continue;                                     /* table too small: retry */
+        SCRATCH(flag, (size_t)n_slots * 4);
+        SCRATCH(pos, (size_t)(n_slots + 1) * 8);
+        int blocks = (int)ceil_div(n_slots, 256);
+        LAUNCH("group_emit", k_nonempty, blocks, 256, 0, (const uint64_t*)cells.as<uint64_t>(),
+               n_slots, P.ncells, flag.as<uint32_t>());
+        rc = tdd::scan_u32_i64(flag.as<uint32_t>(), n_slots, pos.as<int64_t>());
+        if (rc) return rc;
+        int64_t ng = 0;
+        HC(hipMemcpyAsync(&ng, pos.as<int64_t>() + n_slots, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+        HC(hipStreamSynchronize(tdd::g_stream));
+        rc = alloc_group_outputs(P, ng, out);
+        if (rc) return rc;
+        if (ng > 0)
+            LAUNCH("group_emit", k_ht_emit, blocks, 256, 0, P,
+                   (const unsigned long long*)keys.as<unsigned long long>(),
+                   (const uint64_t*)cells.as<uint64_t>(), n_slots,
+                   (const int64_t*)pos.as<int64_t>(), *out);
+        out->path = 3;
+        return TD_OK;
+    }
+    return TD_ERR_OOM;
+}
+
+/* --------------------------------------------------------------- driver -- */
+
+static int64_t masked_rows(const tdd_sel_t* s, int64_t nrows) {
+    if (!s) return nrows;
+    int64_t t = 0;
+    if (tdd_sel_total(s, &t) != TD_OK) return nrows;
+    return t;
+}
+
+extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) {
+    memset(out, 0, sizeof(*out));
+    if (spec->n_keys < 1 || spec->n_keys > TDD_MAX_KEYS || spec->n_aggs > TDD_MAX_AGGS ||
+        spec->n_aggs < 0)
+        return TD_ERR_NYI;
+    GroupPlan P;
+    memset(&P, 0, sizeof(P));
+    P.n_keys = spec->n_keys;
+    P.n_aggs = spec->n_aggs;
+    for (int k = 0; k < P.n_keys; k++) P.keys[k] = spec->keys[k];
+    for (int a = 0; a < P.n_aggs; a++) { P.aggs[a] = spec->aggs[a]; P.ops[a] = spec->ops[a]; }
+    int rc = plan_cells(P);
+    if (rc) return rc;
+    RowMask M = row_mask(spec->mask, spec->nrows);
+    int64_t nsel = masked_rows(spec->mask, spec->nrows);
+
+    /* Integer keys: masked ranges decide between direct array and hash. */
+    bool all_int = true;
+    for (int k = 0; k < P.n_keys; k++) all_int = all_int && is_int_type(P.keys[k].type);
+    bool da = false, composite = false;
+    if (all_int && spec->force_path != 2) {
+        double slots = 1.0;
+        composite = true;
+        for (int k = 0; k < P.n_keys; k++) {
+            int64_t mn, mx;
+            rc = tdd_minmax(&P.keys[k], spec->nrows, spec->mask, &mn, &mx);
+            if (rc) return rc;
+            if (mn > mx) { mn = 0; mx = 0; }               /* no selected rows */
+            P.kmin[k] = mn;
+            P.krange[k] = (int64_t)((uint64_t)mx - (uint64_t)mn + 1);
+            if (P.krange[k] <= 0) { composite = false; slots = 1e30; break; }
+            slots *= (double)P.krange[k];
+        }
+        if (slots > 9.0e18) composite = false;
+        if (composite) {
+            int64_t st = 1;
+            for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st; st *= P.krange[k]; }
+        }
+        da = composite && (slots <= (double)g_da_max_slots || spec->force_path == 1) &&
+             slots * P.ncells * 8 <= 8.0e9;
+        if (da && spec->force_path != 1 && slots > 4.0 * (double)(nsel + 1024)) da = false;
+        if (da) {
+            rc = run_da(P, M, (int64_t)slots, out);
+            if (rc) return rc;
+            HC(hipStreamSynchronize(tdd::g_stream));
+            return TD_OK;
+        }
+    }
+    if (P.n_keys > 1 && !composite) return TD_ERR_NYI;
+    if (P.n_keys == 1 && !composite) P.kstride[0] = 0;      /* raw 64-bit key */
+    if (P.n_keys == 1 && composite && P.keys[0].type != TD_F64) {
+        P.kstride[0] = 0;                                    /* integer key: use its value */
+    }
+    rc = run_ht(P, M, nsel, out);
+    if (rc) return rc;
+    HC(hipStreamSynchronize(tdd::g_stream));
+    return TD_OK;
+}

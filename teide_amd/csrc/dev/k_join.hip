@@ -1,0 +1,846 @@
+/*
+ * k_join.hip -- equi-join kernels and the fused filter -> join -> group-by.
+ *
+ * Direct map (single integer key on both sides, build keys dense in
+ * [min, max]): head[key - min] holds the last build row inserted with that
+ * key and next[row] chains the others, so a probe is one indexed load plus
+ * a chain walk.  This is the device counterpart of the reference's chained
+ * table for small builds (exec.c:7974-8128) without hashing.
+ *
+ * Radix partitioned (everything else, exec.c:7431-7950): both sides are
+ * partitioned by the top bits of a 64-bit key hash in two passes (LDS
+ * histogram per 2048-row tile; prefix sums; LDS counting sort of the tile and
+ * coalesced run writes), then one workgroup per partition builds an LDS
+ * open-addressing table from the build partition (in LDS-sized chunks when a
+ * partition is large) and streams the probe partition through it.
+ *
+ * Output pairs are produced by a count pass and a fill pass, so the output
+ * needs no atomics on a global cursor and the pairs of one probe row are
+ * contiguous.
+ */
+#include "group_plan.h"
+#include "pred_eval.h"
+
+#include <stdio.h>
+
+#define JT 256                  /* threads per join workgroup */
+#define JROWS 8                 /* rows per thread per tile */
+#define JTILE (JT * JROWS)      /* 2048 rows per tile */
+#define PCAP 4096               /* LDS hash-table slots per partition chunk */
+#define PCHUNK 2048             /* build rows per chunk (load <= 0.5) */
+#define EMPTY_ROW 0xffffffffu
+
+__device__ __forceinline__ uint64_t jmix(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+/* Keys of one side.  exact: a single integer key compared by value (the key
+ * word is the value); otherwise the key word is a hash of all key columns
+ * and candidate pairs are verified against the columns. */
+struct JKeys {
+    tdd_col_t cols[TDD_MAX_KEYS];
+    int32_t   n, exact;
+};
+
+__device__ __forceinline__ uint64_t key_word(const JKeys& K, int64_t row) {
+    if (K.exact) return (uint64_t)ld_i64(K.cols[0], row);
+    uint64_t h = 0x2545f4914f6cdd1dull;
+    for (int k = 0; k < K.n; k++) {
+        uint64_t w;
+        if (K.cols[k].type == TD_F64) {
+            double d = ((const double*)K.cols[k].data)[row];
+            if (d == 0.0) d = 0.0;               /* -0.0 == 0.0 must hash alike */
+            w = (uint64_t)__double_as_longlong(d);
+        } else {
+            w = (uint64_t)ld_i64(K.cols[k], row);
+        }
+        h = jmix(h ^ w) + (uint64_t)k;
+    }
+    return h;
+}
+
+/* join_keys_eq (exec.c:7754-7768): F64 keys with ==, others as int64. */
+__device__ __forceinline__ bool keys_equal(const JKeys& L, int64_t l, const JKeys& R, int64_t r) {
+    for (int k = 0; k < L.n; k++) {
+        if (L.cols[k].type == TD_F64) {
+            if (((const double*)L.cols[k].data)[l] != ((const double*)R.cols[k].data)[r]) return false;
+        } else if (ld_i64(L.cols[k], l) != ld_i64(R.cols[k], r)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool row_selected(const uint64_t* bits, int64_t row) {
+    return !bits || ((bits[row >> 6] >> (row & 63)) & 1ull);
+}
+
+/* Block-wide exclusive scan of one int64 per thread (JT threads). */
+__device__ int64_t jblock_scan(int64_t v, int64_t* total) {
+    __shared__ int64_t ws[JT / WAVE];
+    int lane = lane_id(), w = threadIdx.x / WAVE;
+    int64_t x = v;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int64_t y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    if (lane == WAVE - 1) ws[w] = x;
+    __syncthreads();
+    int64_t base = 0, tot = 0;
+    for (int i = 0; i < JT / WAVE; i++) {
+        if (i < w) base += ws[i];
+        tot += ws[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+/* ============================================================ direct map == */
+
+__global__ void k_dm_build(tdd_col_t rk, int64_t nr, int64_t kmin, int32_t* head, int32_t* next,
+                           int* dup) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nr;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = ld_i64(rk, r) - kmin;
+        int32_t old = atomicExch(&head[i], (int32_t)r);
+        next[r] = old;
+        if (old >= 0) *dup = 1;
+    }
+}
+
+/* Matches of left row l (probe side) through the direct map. */
+struct DMap {
+    const int32_t* head;
+    const int32_t* next;
+    int64_t kmin, range;
+};
+
+__device__ __forceinline__ int32_t dm_first(const DMap& D, const tdd_col_t& lk, int64_t l) {
+    uint64_t i = (uint64_t)(ld_i64(lk, l) - D.kmin);
+    return i < (uint64_t)D.range ? D.head[i] : -1;
+}
+
+__device__ __forceinline__ int64_t dm_count(const DMap& D, const tdd_col_t& lk, int64_t l) {
+    int64_t c = 0;
+    for (int32_t r = dm_first(D, lk, l); r >= 0; r = D.next[r]) c++;
+    return c;
+}
+
+/* Count pass; a full join also marks the build rows that found a partner. */
+__global__ __launch_bounds__(JT) void k_dm_count(DMap D, tdd_col_t lk, const uint64_t* lsel,
+                                                 int64_t nl, int outer, int64_t* tile_cnt,
+                                                 uint8_t* matched_r) {
+    int64_t base = (int64_t)blockIdx.x * JTILE;
+    int64_t c = 0;
+    for (int j = 0; j < JROWS; j++) {
+        int64_t l = base + j * JT + threadIdx.x;
+        if (l < nl && row_selected(lsel, l)) {
+            int64_t m = 0;
+            for (int32_t r = dm_first(D, lk, l); r >= 0; r = D.next[r]) {
+                m++;
+                if (matched_r) matched_r[r] = 1;
+            }
+            c += (m == 0 && outer) ? 1 : m;
+        }
+    }
+    int64_t tot;
+    jblock_scan(c, &tot);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(JT) void k_dm_fill(DMap D, tdd_col_t lk, const uint64_t* lsel,
+                                                int64_t nl, int outer, const int64_t* tile_off,
+                                                int64_t* l_idx, int64_t* r_idx) {
+    int64_t base = (int64_t)blockIdx.x * JTILE;
+    int64_t pos = tile_off[blockIdx.x];
+    for (int j = 0; j < JROWS; j++) {
+        int64_t l = base + j * JT + threadIdx.x;
+        int64_t m = 0;
+        bool on = l < nl && row_selected(lsel, l);
+        if (on) m = dm_count(D, lk, l);
+        int64_t c = (on && m == 0 && outer) ? 1 : m;
+        int64_t tot;
+        int64_t off = pos + jblock_scan(c, &tot);
+        if (on) {
+            if (m == 0 && outer) {
+                l_idx[off] = l;
+                r_idx[off] = -1;
+            } else {
+                for (int32_t r = dm_first(D, lk, l); r >= 0; r = D.next[r]) {
+                    l_idx[off] = l;
+                    r_idx[off] = r;
+                    off++;
+                }
+            }
+        }
+        pos += tot;
+    }
+}
+
+/* ================================================================ radix == */
+
+struct PartIn {
+    JKeys           K;
+    const uint64_t* sel;      /* probe side only */
+    int64_t         n;
+    int32_t         bits;     /* radix bits */
+    int32_t         n_tiles;
+};
+
+__device__ __forceinline__ uint32_t part_of(uint64_t kw, int bits) {
+    return (uint32_t)(jmix(kw) >> (64 - bits));
+}
+
+__global__ __launch_bounds__(JT) void k_part_hist(PartIn I, uint32_t* H) {
+    extern __shared__ uint32_t hist[];
+    const int P = 1 << I.bits;
+    for (int i = threadIdx.x; i < P; i += JT) hist[i] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * JTILE;
+    for (int j = 0; j < JROWS; j++) {
+        int64_t r = base + j * JT + threadIdx.x;
+        if (r < I.n && row_selected(I.sel, r)) atomicAdd(&hist[part_of(key_word(I.K, r), I.bits)], 1u);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < P; p += JT) H[(int64_t)p * I.n_tiles + blockIdx.x] = hist[p];
+}
+
+/* LDS counting sort of one tile by partition, then each run is written to
+ * its slot [Off[p][tile], ...) with consecutive lanes on consecutive
+ * addresses. */
+__global__ __launch_bounds__(JT) void k_part_scatter(PartIn I, const int64_t* Off,
+                                                     uint64_t* out_key, uint32_t* out_row) {
+    extern __shared__ uint8_t smem[];
+    const int P = 1 << I.bits;
+    uint32_t* hist = (uint32_t*)smem;                       /* P */
+    uint32_t* loff = hist + P;                              /* P + 1 */
+    uint64_t* skey = (uint64_t*)(loff + P + 2);             /* JTILE, 8-aligned (P even) */
+    uint32_t* srow = (uint32_t*)(skey + JTILE);             /* JTILE */
+    uint16_t* spart = (uint16_t*)(srow + JTILE);            /* JTILE */
+    for (int i = threadIdx.x; i < P; i += JT) hist[i] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * JTILE;
+    uint64_t kw[JROWS];
+    uint32_t pr[JROWS], rank[JROWS];
+    for (int j = 0; j < JROWS; j++) {
+        int64_t r = base + j * JT + threadIdx.x;
+        pr[j] = 0xffffffffu;
+        if (r < I.n && row_selected(I.sel, r)) {
+            kw[j] = key_word(I.K, r);
+            pr[j] = part_of(kw[j], I.bits);
+            rank[j] = atomicAdd(&hist[pr[j]], 1u);
+        }
+    }
+    __syncthreads();
+    /* exclusive scan of the tile histogram (P <= 2048 bins, 8 per thread) */
+    {
+        const int per = (P + JT - 1) / JT;
+        int64_t s = 0;
+        for (int q = 0; q < per; q++) {
+            int p = threadIdx.x * per + q;
+            if (p < P) s += hist[p];
+        }
+        int64_t tot;
+        int64_t pre = jblock_scan(s, &tot);
+        for (int q = 0; q < per; q++) {
+            int p = threadIdx.x * per + q;
+            if (p < P) { loff[p] = (uint32_t)pre; pre += hist[p]; }
+        }
+        if (threadIdx.x == 0) loff[P] = (uint32_t)tot;
+    }
+    __syncthreads();
+    for (int j = 0; j < JROWS; j++) {
+        if (pr[j] == 0xffffffffu) continue;
+        uint32_t at = loff[pr[j]] + rank[j];
+        skey[at] = kw[j];
+        srow[at] = (uint32_t)(base + j * JT + threadIdx.x);
+        spart[at] = (uint16_t)pr[j];
+    }
+    __syncthreads();
+    uint32_t cnt = loff[P];
+    for (uint32_t i = threadIdx.x; i < cnt; i += JT) {
+        uint32_t p = spart[i];
+        int64_t dst = Off[(int64_t)p * I.n_tiles + blockIdx.x] + (i - loff[p]);
+        out_key[dst] = skey[i];
+        out_row[dst] = srow[i];
+    }
+}
+
+struct Parts {
+    const uint64_t* key;
+    const uint32_t* row;
+    const int64_t*  off;      /* tile-major offsets: partition p starts at off[p * n_tiles] */
+    int32_t         n_tiles;
+};
+
+__device__ __forceinline__ int64_t part_start(const Parts& S, int p) {
+    return S.off[(int64_t)p * S.n_tiles];
+}
+
+struct BPArgs {
+    Parts    L, R;            /* probe (left) and build (right) partitions */
+    JKeys    LK, RK;          /* for verification when not exact */
+    int32_t  outer, full, exact, fill;
+    int64_t* pcount;          /* count pass: matches per partition */
+    const int64_t* poff;      /* fill pass: output offset per partition */
+    int64_t* l_idx;
+    int64_t* r_idx;
+    uint8_t* lmatched;        /* per probe entry (outer joins) */
+    uint8_t* rmatched;        /* per build row (full joins) */
+};
+
+__global__ __launch_bounds__(JT) void k_bp(BPArgs A) {
+    __shared__ uint64_t tkey[PCAP];
+    __shared__ uint32_t trow[PCAP];
+    __shared__ unsigned long long cursor;
+    const int p = blockIdx.x;
+    const int64_t bs = part_start(A.R, p), be = part_start(A.R, p + 1);
+    const int64_t ps = part_start(A.L, p), pe = part_start(A.L, p + 1);
+    int64_t cnt = 0;
+    if (threadIdx.x == 0) cursor = 0;
+    __syncthreads();
+    int64_t obase = A.fill ? A.poff[p] : 0;
+    for (int64_t c0 = bs; c0 < be; c0 += PCHUNK) {          /* uniform across the block */
+        int64_t c1 = c0 + PCHUNK < be ? c0 + PCHUNK : be;
+        for (int i = threadIdx.x; i < PCAP; i += JT) trow[i] = EMPTY_ROW;
+        __syncthreads();
+        for (int64_t i = c0 + threadIdx.x; i < c1; i += JT) {
+            uint64_t k = A.R.key[i];
+            uint32_t s = (uint32_t)jmix(k) & (PCAP - 1);
+            while (atomicCAS(&trow[s], EMPTY_ROW, A.R.row[i]) != EMPTY_ROW) s = (s + 1) & (PCAP - 1);
+            tkey[s] = k;
+        }
+        __syncthreads();
+        for (int64_t i = ps + threadIdx.x; i < pe; i += JT) {
+            uint64_t k = A.L.key[i];
+            uint32_t l = A.L.row[i];
+            uint32_t s = (uint32_t)jmix(k) & (PCAP - 1);
+            bool any = false;
+            for (;;) {
+                uint32_t r = trow[s];
+                if (r == EMPTY_ROW) break;
+                if (tkey[s] == k && (A.exact || keys_equal(A.LK, l, A.RK, r))) {
+                    any = true;
+                    if (A.fill) {
+                        int64_t at = obase + (int64_t)atomicAdd(&cursor, 1ull);
+                        A.l_idx[at] = l;
+                        A.r_idx[at] = r;
+                    } else {
+                        cnt++;
+                    }
+                    if (A.full) A.rmatched[r] = 1;
+                }
+                s = (s + 1) & (PCAP - 1);
+            }
+            if (any && A.outer) A.lmatched[i] = 1;
+        }
+        __syncthreads();
+    }
+    if (A.outer) {
+        for (int64_t i = ps + threadIdx.x; i < pe; i += JT) {
+            if (A.lmatched[i]) continue;
+            if (A.fill) {
+                int64_t at = obase + (int64_t)atomicAdd(&cursor, 1ull);
+                A.l_idx[at] = A.L.row[i];
+                A.r_idx[at] = -1;
+            } else {
+                cnt++;
+            }
+        }
+    }
+    if (!A.fill) {
+        int64_t tot;
+        jblock_scan(cnt, &tot);
+        if (threadIdx.x == 0) A.pcount[p] = tot;
+    }
+}
+
+/* Unmatched build rows of a full join: flag, scan, append with l = -1. */
+__global__ void k_unmatched_flag(const uint8_t* matched, int64_t n, uint32_t* flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = matched[i] == 0;
+}
+
+__global__ void k_unmatched_emit(const uint8_t* matched, int64_t n, const int64_t* pos,
+                                 int64_t base, int64_t* l_idx, int64_t* r_idx) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !matched[i]) {
+        l_idx[base + pos[i]] = -1;
+        r_idx[base + pos[i]] = i;
+    }
+}
+
+/* ----------------------------------------------------------- host side -- */
+
+static bool is_int(int8_t t) {
+    return t != TD_F64 && t != TD_GUID && t != TD_LIST && t != TD_TABLE && t != TD_CHAR;
+}
+
+static int64_t read_i64(const void* dptr) {
+    int64_t v = 0;
+    if (hipMemcpyAsync(&v, dptr, 8, hipMemcpyDeviceToHost, tdd::g_stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(tdd::g_stream) != hipSuccess) return -1;
+    return v;
+}
+
+static int alloc_pairs(int64_t n, tdd_join_result_t* out) {
+    out->n_pairs = n;
+    out->l_idx = (int64_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
+    out->r_idx = (int64_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
+    if (!out->l_idx || !out->r_idx) return TD_ERR_OOM;
+    return TD_OK;
+}
+
+/* Append the unmatched build rows of a full join after `base` pairs. */
+static int append_unmatched(const uint8_t* matched, int64_t nr, int64_t* n_unmatched,
+                            Scratch* flag_out, Scratch* pos_out) {
+    int blocks = (int)ceil_div(nr > 0 ? nr : 1, 256);
+    LAUNCH("join_unmatched", k_unmatched_flag, blocks, 256, 0, matched, nr, flag_out->as<uint32_t>());
+    int rc = tdd::scan_u32_i64(flag_out->as<uint32_t>(), nr, pos_out->as<int64_t>());
+    if (rc) return rc;
+    *n_unmatched = read_i64(pos_out->as<int64_t>() + nr);
+    return *n_unmatched < 0 ? TD_ERR_IO : TD_OK;
+}
+
+/* Dense unique-ish integer build keys: chained direct map.  Returns
+ * TD_ERR_NYI when the build keys are not dense enough. */
+static int join_direct(const tdd_join_spec_t* S, tdd_join_result_t* out, bool forced) {
+    int64_t mn, mx;
+    int rc = tdd_minmax(&S->rkeys[0], S->nr, nullptr, &mn, &mx);
+    if (rc) return rc;
+    int64_t range = mn > mx ? 1 : (int64_t)((uint64_t)mx - (uint64_t)mn + 1);
+    if (mn > mx) mn = 0;
+    if (range <= 0 || range > ((int64_t)1 << 31)) return TD_ERR_NYI;
+    if (!forced && range > 4 * S->nr + 4096) return TD_ERR_NYI;
+    if (S->nr >= ((int64_t)1 << 31)) return TD_ERR_NYI;
+    SCRATCH(head, (size_t)range * 4);
+    SCRATCH(next, (size_t)(S->nr > 0 ? S->nr : 1) * 4);
+    SCRATCH(dup, 8);
+    rc = tdd_fill(head.p, ~0ull, (size_t)range * 4);
+    if (rc) return rc;
+    rc = tdd_fill(dup.p, 0, 8);
+    if (rc) return rc;
+    if (S->nr > 0)
+        LAUNCH("join_dm_build", k_dm_build, stream_grid(ceil_div(S->nr, 256 * 4)), 256, 0,
+               S->rkeys[0], S->nr, mn, head.as<int32_t>(), next.as<int32_t>(), dup.as<int>());
+    DMap D{head.as<int32_t>(), next.as<int32_t>(), mn, range};
+    int outer = S->join_type >= 1, full = S->join_type == 2;
+    int64_t n_tiles = ceil_div(S->nl > 0 ? S->nl : 1, JTILE);
+    size_t nr1 = (size_t)(S->nr > 0 ? S->nr : 1);
+    SCRATCH(tcnt, (size_t)n_tiles * 8);
+    SCRATCH(toff, (size_t)(n_tiles + 1) * 8);
+    SCRATCH(matched, nr1);
+    SCRATCH(flag, nr1 * 4);
+    SCRATCH(pos, (nr1 + 1) * 8);
+    if (full) {
+        rc = tdd_fill(matched.p, 0, nr1);
+        if (rc) return rc;
+    }
+    const uint64_t* lsel = S->lsel ? S->lsel->bits : nullptr;
+    LAUNCH("join_dm_count", k_dm_count, (int)n_tiles, JT, 0, D, S->lkeys[0], lsel, S->nl, outer,
+           tcnt.as<int64_t>(), full ? matched.as<uint8_t>() : (uint8_t*)nullptr);
+    rc = tdd::scan_i64(tcnt.as<int64_t>(), n_tiles, toff.as<int64_t>());
+    if (rc) return rc;
+    int64_t npairs = read_i64(toff.as<int64_t>() + n_tiles);
+    if (npairs < 0) return TD_ERR_IO;
+    int64_t n_unm = 0;
+    if (full) {
+        rc = append_unmatched(matched.as<uint8_t>(), S->nr, &n_unm, &flag, &pos);
+        if (rc) return rc;
+    }
+    rc = alloc_pairs(npairs + n_unm, out);
+    if (rc) return rc;
+    if (npairs > 0)
+        LAUNCH("join_dm_fill", k_dm_fill, (int)n_tiles, JT, 0, D, S->lkeys[0], lsel, S->nl,
+               outer, (const int64_t*)toff.as<int64_t>(), out->l_idx, out->r_idx);
+    if (n_unm > 0)
+        LAUNCH("join_unmatched", k_unmatched_emit, (int)ceil_div(S->nr, 256), 256, 0,
+               (const uint8_t*)matched.as<uint8_t>(), S->nr, (const int64_t*)pos.as<int64_t>(),
+               npairs, out->l_idx, out->r_idx);
+    out->path = 1;
+    out->radix_bits = 0;
+    return TD_OK;
+}
+
+/* Partition one side into (key word, row) runs.  Returns TD_OK. */
+static int partition_side(const PartIn& I, uint64_t** key_out, uint32_t** row_out,
+                          int64_t** off_out, int64_t* total) {
+    const int P = 1 << I.bits;
+    size_t nh = (size_t)P * I.n_tiles;
+    SCRATCH(H, nh * 4);
+    int64_t* off = (int64_t*)td__arena_alloc((nh + 1) * 8);
+    if (!off) return TD_ERR_OOM;
+    LAUNCH("join_part_hist", k_part_hist, I.n_tiles, JT, (size_t)P * 4, I, H.as<uint32_t>());
+    int rc = tdd::scan_u32_i64(H.as<uint32_t>(), (int64_t)nh, off);
+    if (rc) { td__arena_free(off); return rc; }
+    int64_t n = read_i64(off + nh);
+    uint64_t* key = (uint64_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
+    uint32_t* row = (uint32_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 4);
+    if (!key || !row || n < 0) {
+        td__arena_free(off); td__arena_free(key); td__arena_free(row);
+        return n < 0 ? TD_ERR_IO : TD_ERR_OOM;
+    }
+    size_t lds = (size_t)(2 * P + 2) * 4 + (size_t)JTILE * (8 + 4 + 2);
+    LAUNCH("join_part_scatter", k_part_scatter, I.n_tiles, JT, lds, I,
+           (const int64_t*)off, key, row);
+    *key_out = key;
+    *row_out = row;
+    *off_out = off;
+    *total = n;
+    return TD_OK;
+}
+
+static int join_radix(const tdd_join_spec_t* S, tdd_join_result_t* out) {
+    if (S->nl >= ((int64_t)1 << 32) || S->nr >= ((int64_t)1 << 32)) return TD_ERR_NYI;
+    int bits = S->radix_bits;
+    if (bits <= 0) {
+        bits = 3;
+        while (bits < 11 && ((int64_t)1 << bits) * 1536 < S->nr) bits++;
+    }
+    if (bits > 11) bits = 11;
+    const int P = 1 << bits;
+    JKeys LK, RK;
+    memset(&LK, 0, sizeof(LK));
+    memset(&RK, 0, sizeof(RK));
+    LK.n = RK.n = S->n_keys;
+    bool exact = S->n_keys == 1 && is_int(S->lkeys[0].type) && is_int(S->rkeys[0].type);
+    LK.exact = RK.exact = exact;
+    for (int k = 0; k < S->n_keys; k++) { LK.cols[k] = S->lkeys[k]; RK.cols[k] = S->rkeys[k]; }
+    PartIn IL{LK, S->lsel ? S->lsel->bits : nullptr, S->nl, bits,
+              (int32_t)ceil_div(S->nl > 0 ? S->nl : 1, JTILE)};
+    PartIn IR{RK, nullptr, S->nr, bits, (int32_t)ceil_div(S->nr > 0 ? S->nr : 1, JTILE)};
+    uint64_t *lkey = nullptr, *rkey = nullptr;
+    uint32_t *lrow = nullptr, *rrow = nullptr;
+    int64_t *loff = nullptr, *roff = nullptr, nlp = 0, nrp = 0;
+    int rc = partition_side(IL, &lkey, &lrow, &loff, &nlp);
+    if (!rc) rc = partition_side(IR, &rkey, &rrow, &roff, &nrp);
+    struct Frees { void* p[6]; ~Frees() { for (void* q : p) if (q) td__arena_free(q); } }
+        fr{{lkey, lrow, loff, rkey, rrow, roff}};
+    if (rc) return rc;
+    BPArgs A;
+    memset(&A, 0, sizeof(A));
+    A.L = Parts{lkey, lrow, loff, IL.n_tiles};
+    A.R = Parts{rkey, rrow, roff, IR.n_tiles};
+    A.LK = LK;
+    A.RK = RK;
+    A.exact = exact;
+    A.outer = S->join_type >= 1;
+    A.full = S->join_type == 2;
+    SCRATCH(pcount, (size_t)P * 8);
+    SCRATCH(poff, (size_t)(P + 1) * 8);
+    SCRATCH(lm, (size_t)(nlp > 0 ? nlp : 1));
+    SCRATCH(rm, (size_t)(S->nr > 0 ? S->nr : 1));
+    if (A.outer) { rc = tdd_fill(lm.p, 0, (size_t)(nlp > 0 ? nlp : 1)); if (rc) return rc; }
+    if (A.full) { rc = tdd_fill(rm.p, 0, (size_t)(S->nr > 0 ? S->nr : 1)); if (rc) return rc; }
+    A.lmatched = lm.as<uint8_t>();
+    A.rmatched = rm.as<uint8_t>();
+    A.pcount = pcount.as<int64_t>();
+    A.fill = 0;
+    LAUNCH("join_build_probe", k_bp, P, JT, 0, A);
+    rc = tdd::scan_i64(pcount.as<int64_t>(), P, poff.as<int64_t>());
+    if (rc) return rc;
+    int64_t npairs = read_i64(poff.as<int64_t>() + P);
+    if (npairs < 0) return TD_ERR_IO;
+    int64_t n_unm = 0;
+    SCRATCH(flag, (size_t)(S->nr > 0 ? S->nr : 1) * 4);
+    SCRATCH(pos, (size_t)(S->nr + 1) * 8);
+    if (A.full) {
+        rc = append_unmatched(rm.as<uint8_t>(), S->nr, &n_unm, &flag, &pos);
+        if (rc) return rc;
+    }
+    rc = alloc_pairs(npairs + n_unm, out);
+    if (rc) return rc;
+    if (A.outer) { rc = tdd_fill(lm.p, 0, (size_t)(nlp > 0 ? nlp : 1)); if (rc) return rc; }
+    A.fill = 1;
+    A.full = 0;                      /* matched build rows are already known */
+    A.poff = poff.as<int64_t>();
+    A.l_idx = out->l_idx;
+    A.r_idx = out->r_idx;
+    if (npairs > 0) LAUNCH("join_build_probe", k_bp, P, JT, 0, A);
+    if (n_unm > 0)
+        LAUNCH("join_unmatched", k_unmatched_emit, (int)ceil_div(S->nr, 256), 256, 0,
+               (const uint8_t*)rm.as<uint8_t>(), S->nr, (const int64_t*)pos.as<int64_t>(), npairs,
+               out->l_idx, out->r_idx);
+    out->path = 2;
+    out->radix_bits = bits;
+    return TD_OK;
+}
+
+extern "C" int tdd_join(const tdd_join_spec_t* S, tdd_join_result_t* out) {
+    memset(out, 0, sizeof(*out));
+    if (S->n_keys < 1 || S->n_keys > TDD_MAX_KEYS || S->join_type < 0 || S->join_type > 2)
+        return TD_ERR_NYI;
+    for (int k = 0; k < S->n_keys; k++) {
+        if (!S->lkeys[k].data || !S->rkeys[k].data) return TD_ERR_SCHEMA;
+        if ((S->lkeys[k].type == TD_F64) != (S->rkeys[k].type == TD_F64)) return TD_ERR_TYPE;
+    }
+    int rc = TD_ERR_NYI;
+    bool single_int = S->n_keys == 1 && is_int(S->lkeys[0].type) && is_int(S->rkeys[0].type);
+    if (single_int && S->force_path != 2) rc = join_direct(S, out, S->force_path == 1);
+    if (rc == TD_ERR_NYI) rc = join_radix(S, out);
+    if (rc) return rc;
+    HC(hipStreamSynchronize(tdd::g_stream));
+    return TD_OK;
+}
+
+/* ===================================================== fused join-group == */
+/* Probe rows are filtered, looked up in the direct map and folded into a
+ * direct-array group table.  slotmap mode: a unique build key whose group
+ * keys all come from the build side maps straight to the group slot, so a
+ * probe row costs one indexed load; chain mode walks the build rows. */
+
+struct JGArgs {
+    tdd_pred_t      pred;
+    int32_t         has_pred;
+    const uint64_t* lsel;
+    int64_t         nl;
+    tdd_col_t       lk;
+    DMap            D;
+    const int32_t*  slotmap;      /* slotmap mode, else NULL */
+    int64_t         n_slots;
+};
+
+template <bool SLOTMAP>
+__global__ __launch_bounds__(512) void k_jg_lds(JGArgs J, GroupPlan P, uint64_t* partials) {
+    extern __shared__ uint64_t tab[];
+    int64_t ncell_total = J.n_slots * P.ncells;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x)
+        tab[i] = cell_identity(P.kind[i % P.ncells]);
+    __syncthreads();
+    int64_t n_words = (J.nl + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        int64_t l = w * 64 + lane;
+        if (l >= J.nl || !row_selected(J.lsel, l)) continue;
+        if (J.has_pred && !pred_eval(J.pred, l)) continue;
+        uint64_t i = (uint64_t)(ld_i64(J.lk, l) - J.D.kmin);
+        if (i >= (uint64_t)J.D.range) continue;
+        if (SLOTMAP) {
+            int32_t s = J.slotmap[i];
+            if (s >= 0) cells_accum2(&tab[(int64_t)s * P.ncells], P, l, 0);
+        } else {
+            for (int32_t r = J.D.head[i]; r >= 0; r = J.D.next[r])
+                cells_accum2(&tab[da_slot2(P, l, r) * P.ncells], P, l, r);
+        }
+    }
+    __syncthreads();
+    uint64_t* dst = partials + (int64_t)blockIdx.x * ncell_total;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
+}
+
+template <bool SLOTMAP>
+__global__ __launch_bounds__(512) void k_jg_global(JGArgs J, GroupPlan P, uint64_t* tab) {
+    int64_t n_words = (J.nl + 63) >> 6;
+    int lane = lane_id();
+    int64_t wave = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t w = wave; w < n_words; w += n_waves) {
+        int64_t l = w * 64 + lane;
+        if (l >= J.nl || !row_selected(J.lsel, l)) continue;
+        if (J.has_pred && !pred_eval(J.pred, l)) continue;
+        uint64_t i = (uint64_t)(ld_i64(J.lk, l) - J.D.kmin);
+        if (i >= (uint64_t)J.D.range) continue;
+        if (SLOTMAP) {
+            int32_t s = J.slotmap[i];
+            if (s >= 0) cells_accum2(&tab[(int64_t)s * P.ncells], P, l, 0);
+        } else {
+            for (int32_t r = J.D.head[i]; r >= 0; r = J.D.next[r])
+                cells_accum2(&tab[da_slot2(P, l, r) * P.ncells], P, l, r);
+        }
+    }
+}
+
+/* slotmap[key - kmin] = group slot of the build row (build-side keys only). */
+__global__ void k_jg_slotmap(tdd_col_t rk, int64_t nr, int64_t kmin, GroupPlan P, int32_t* map) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nr;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = ld_i64(rk, r) - kmin;
+        map[i] = (int32_t)da_slot2(P, 0, r);
+    }
+}
+
+static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out);
+
+extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
+    memset(out, 0, sizeof(*out));
+    const tdd_join_spec_t* J = &S->join;
+    bool fusable = J->join_type == 0 && J->n_keys == 1 && is_int(J->lkeys[0].type) &&
+                   is_int(J->rkeys[0].type) && J->force_path != 2 && S->n_keys >= 1 &&
+                   S->n_keys <= TDD_MAX_KEYS && S->n_aggs <= TDD_MAX_AGGS &&
+                   J->nr < ((int64_t)1 << 31);
+    for (int a = 0; a < S->n_aggs && fusable; a++)
+        fusable = S->ops[a] != OP_FIRST && S->ops[a] != OP_LAST;
+    for (int k = 0; k < S->n_keys && fusable; k++)
+        fusable = is_int(S->keys[k].type);
+    if (!fusable) return jg_compose(S, out);
+
+    GroupPlan P;
+    memset(&P, 0, sizeof(P));
+    P.n_keys = S->n_keys;
+    P.n_aggs = S->n_aggs;
+    for (int k = 0; k < P.n_keys; k++) { P.keys[k] = S->keys[k]; P.key_side[k] = S->key_side[k]; }
+    for (int a = 0; a < P.n_aggs; a++) {
+        P.aggs[a] = S->aggs[a];
+        P.ops[a] = S->ops[a];
+        P.agg_side[a] = S->agg_side[a];
+    }
+    int rc = plan_cells(P);
+    if (rc) return rc;
+
+    /* Build-side key range: dense enough for a direct map? */
+    int64_t mn, mx;
+    rc = tdd_minmax(&J->rkeys[0], J->nr, nullptr, &mn, &mx);
+    if (rc) return rc;
+    if (mn > mx) { mn = 0; mx = 0; }
+    int64_t range = (int64_t)((uint64_t)mx - (uint64_t)mn + 1);
+    if (range <= 0 || range > ((int64_t)1 << 31) ||
+        (range > 4 * J->nr + 4096 && J->force_path != 1))
+        return jg_compose(S, out);
+
+    /* Group slot space from the source columns' ranges (a superset of the
+     * joined rows' ranges; empty slots are dropped at emit). */
+    double slots = 1.0;
+    for (int k = 0; k < P.n_keys; k++) {
+        int64_t kmn, kmx;
+        int64_t n = P.key_side[k] ? J->nr : J->nl;
+        rc = tdd_minmax(&P.keys[k], n, P.key_side[k] ? nullptr : J->lsel, &kmn, &kmx);
+        if (rc) return rc;
+        if (kmn > kmx) { kmn = 0; kmx = 0; }
+        P.kmin[k] = kmn;
+        P.krange[k] = (int64_t)((uint64_t)kmx - (uint64_t)kmn + 1);
+        if (P.krange[k] <= 0) return jg_compose(S, out);
+        slots *= (double)P.krange[k];
+    }
+    if (slots > (double)g_da_max_slots || slots * P.ncells * 8 > 8.0e9) return jg_compose(S, out);
+    int64_t st = 1;
+    for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st; st *= P.krange[k]; }
+    int64_t n_slots = (int64_t)slots;
+
+    SCRATCH(head, (size_t)range * 4);
+    SCRATCH(next, (size_t)(J->nr > 0 ? J->nr : 1) * 4);
+    SCRATCH(dup, 8);
+    rc = tdd_fill(head.p, ~0ull, (size_t)range * 4);
+    if (!rc) rc = tdd_fill(dup.p, 0, 8);
+    if (rc) return rc;
+    if (J->nr > 0)
+        LAUNCH("jg_build", k_dm_build, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0,
+               J->rkeys[0], J->nr, mn, head.as<int32_t>(), next.as<int32_t>(), dup.as<int>());
+    int d = 0;
+    HC(hipMemcpyAsync(&d, dup.p, 4, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+
+    bool right_keys_only = true, left_aggs_only = true;
+    for (int k = 0; k < P.n_keys; k++) right_keys_only = right_keys_only && P.key_side[k] == 1;
+    for (int a = 0; a < P.n_aggs; a++) left_aggs_only = left_aggs_only && P.agg_side[a] == 0;
+    bool slotmap = !d && right_keys_only && left_aggs_only && n_slots < ((int64_t)1 << 31);
+
+    JGArgs A;
+    memset(&A, 0, sizeof(A));
+    if (S->lpred) { A.pred = *S->lpred; A.has_pred = 1; }
+    A.lsel = J->lsel ? J->lsel->bits : nullptr;
+    A.nl = J->nl;
+    A.lk = J->lkeys[0];
+    A.D = DMap{head.as<int32_t>(), next.as<int32_t>(), mn, range};
+    A.n_slots = n_slots;
+    Scratch smap(slotmap ? (size_t)range * 4 : 8);
+    if (!smap.ok()) return TD_ERR_OOM;
+    if (slotmap) {
+        rc = tdd_fill(smap.p, ~0ull, (size_t)range * 4);
+        if (rc) return rc;
+        if (J->nr > 0)
+            LAUNCH("jg_slotmap", k_jg_slotmap, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0,
+                   J->rkeys[0], J->nr, mn, P, smap.as<int32_t>());
+        A.slotmap = smap.as<int32_t>();
+    }
+
+    size_t table_bytes = (size_t)n_slots * P.ncells * 8;
+    int64_t n_words = (J->nl + 63) / 64;
+    if (table_bytes <= 64 * 1024) {
+        int per_cu = (int)(160 * 1024 / (table_bytes + 1024));
+        if (per_cu > 4) per_cu = 4;
+        if (per_cu < 1) per_cu = 1;
+        int grid = stream_grid(ceil_div(n_words > 0 ? n_words : 1, 8), per_cu);
+        SCRATCH(parts, (size_t)grid * table_bytes);
+        if (slotmap)
+            LAUNCH("jg_probe_lds", k_jg_lds<true>, grid, 512, table_bytes, A, P, parts.as<uint64_t>());
+        else
+            LAUNCH("jg_probe_lds", k_jg_lds<false>, grid, 512, table_bytes, A, P, parts.as<uint64_t>());
+        rc = da_finish(P, parts.as<uint64_t>(), grid, n_slots, out);
+        out->path = 1;
+    } else {
+        SCRATCH(tab, table_bytes);
+        rc = da_table_init(P, tab.as<uint64_t>(), n_slots);
+        if (rc) return rc;
+        int grid = stream_grid(ceil_div(n_words > 0 ? n_words : 1, 8), 4);
+        if (slotmap)
+            LAUNCH("jg_probe_global", k_jg_global<true>, grid, 512, 0, A, P, tab.as<uint64_t>());
+        else
+            LAUNCH("jg_probe_global", k_jg_global<false>, grid, 512, 0, A, P, tab.as<uint64_t>());
+        rc = da_emit(P, tab.as<uint64_t>(), n_slots, out);
+        out->path = 2;
+    }
+    if (rc) return rc;
+    HC(hipStreamSynchronize(tdd::g_stream));
+    return TD_OK;
+}
+
+/* General form on the device: filter to a selection, join to index pairs,
+ * gather the key and aggregate columns of the joined rows, group them. */
+static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
+    tdd_join_spec_t J = S->join;
+    int64_t n_segs = (J.nl + 1023) / 1024;
+    size_t sel_bytes = 16 + (size_t)((n_segs + 7) & ~7) + (size_t)((n_segs * 2 + 7) & ~7) +
+                       (size_t)((J.nl + 63) / 64) * 8;
+    Scratch selbuf(S->lpred ? sel_bytes : 8);
+    if (!selbuf.ok()) return TD_ERR_OOM;
+    tdd_sel_t sel;
+    if (S->lpred) {
+        uint8_t* b = selbuf.as<uint8_t>();
+        sel.base = b;
+        sel.flags = b + 16;
+        sel.popcnt = (uint16_t*)(b + 16 + ((n_segs + 7) & ~7));
+        sel.bits = (uint64_t*)(b + 16 + ((n_segs + 7) & ~7) + ((n_segs * 2 + 7) & ~7));
+        sel.nrows = J.nl;
+        sel.n_segs = (uint32_t)n_segs;
+        int rc = tdd_sel_eval(S->lpred, J.lsel, &sel);
+        if (rc) return rc;
+        J.lsel = &sel;
+    }
+    tdd_join_result_t jr;
+    int rc = tdd_join(&J, &jr);
+    if (rc) return rc;
+    struct PairFree { tdd_join_result_t* r; ~PairFree() { td__arena_free(r->l_idx); td__arena_free(r->r_idx); } } pf{&jr};
+    tdd_group_spec_t G;
+    memset(&G, 0, sizeof(G));
+    G.nrows = jr.n_pairs;
+    G.n_keys = S->n_keys;
+    G.n_aggs = S->n_aggs;
+    void* bufs[TDD_MAX_KEYS + TDD_MAX_AGGS] = {};
+    int nb = 0;
+    struct BufFree { void** b; int* n; ~BufFree() { for (int i = 0; i < *n; i++) td__arena_free(b[i]); } } bf{bufs, &nb};
+    for (int i = 0; i < S->n_keys + S->n_aggs; i++) {
+        bool is_key = i < S->n_keys;
+        const tdd_col_t& src = is_key ? S->keys[i] : S->aggs[i - S->n_keys];
+        int side = is_key ? S->key_side[i] : S->agg_side[i - S->n_keys];
+        tdd_col_t c = src;
+        if (src.data) {
+            void* d = td__arena_alloc((size_t)(jr.n_pairs > 0 ? jr.n_pairs : 1) * src.esz);
+            if (!d) return TD_ERR_OOM;
+            bufs[nb++] = d;
+            rc = tdd_gather(&src, side ? jr.r_idx : jr.l_idx, jr.n_pairs, 0, d);
+            if (rc) return rc;
+            c.data = d;
+        }
+        if (is_key) G.keys[i] = c;
+        else { G.aggs[i - S->n_keys] = c; G.ops[i - S->n_keys] = S->ops[i - S->n_keys]; }
+    }
+    return tdd_group(&G, out);
+}

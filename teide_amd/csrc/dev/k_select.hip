@@ -1,0 +1,282 @@
+/*
+ * k_select.hip -- predicate evaluation into TD_SEL bitmaps, selection
+ * algebra, stream compaction and gathers.
+ *
+ * A TD_SEL holds one bit per row, LSB-first in 64-bit words, which is exactly
+ * what a wave64 ballot returns: lane i of a wave evaluates row 64*w + i and
+ * the ballot is word w.  A workgroup of 256 lanes (4 waves) covers one
+ * 1024-row segment per step, four words per wave, and derives the segment's
+ * popcount and NONE / ALL / MIX flag from the ballots it already holds.
+ */
+#include "pred_eval.h"
+
+#define SEG_ROWS 1024
+#define SEL_T 256
+
+/* One 1024-row segment per step: 4 waves x 4 ballots. */
+__global__ __launch_bounds__(SEL_T) void k_sel_eval(tdd_pred_t P, int64_t nrows,
+                                                    uint32_t n_segs, const uint64_t* and_bits,
+                                                    uint8_t* flags, uint16_t* pop,
+                                                    uint64_t* bits,
+                                                    unsigned long long* total) {
+    __shared__ uint32_t wcnt[SEL_T / WAVE];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t n_words = (nrows + 63) >> 6;
+    unsigned long long my_total = 0;
+    for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int64_t word = seg * 16 + w * 4 + k;
+            int64_t row = word * 64 + lane;
+            bool p = row < nrows && pred_eval(P, row);
+            uint64_t m = __ballot(p);
+            if (word < n_words) {
+                if (and_bits) m &= and_bits[word];
+                if (lane == 0) bits[word] = m;
+            }
+            cnt += __popcll(m);
+        }
+        if (lane == 0) wcnt[w] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t c = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            int64_t seg_rows = nrows - seg * SEG_ROWS;
+            if (seg_rows > SEG_ROWS) seg_rows = SEG_ROWS;
+            pop[seg] = (uint16_t)c;
+            flags[seg] = c == 0 ? TD_SEL_NONE : (c == (uint32_t)seg_rows ? TD_SEL_ALL : TD_SEL_MIX);
+            my_total += c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && my_total) atomicAdd(total, my_total);
+}
+
+/* Word-parallel AND + recompute: 16 lanes per segment, one word each; the
+ * popcount masks bits past nrows as td_sel_recompute does (sel.c:95-102). */
+__global__ __launch_bounds__(SEL_T) void k_sel_words(const uint64_t* a, const uint64_t* b,
+                                                     int64_t nrows, uint32_t n_segs,
+                                                     uint8_t* flags, uint16_t* pop,
+                                                     uint64_t* out,
+                                                     unsigned long long* total) {
+    const int64_t n_words = (nrows + 63) >> 6;
+    int64_t seg = (int64_t)blockIdx.x * (SEL_T / 16) + threadIdx.x / 16;
+    int sub = threadIdx.x & 15;
+    int64_t word = seg * 16 + sub;
+    uint32_t pc = 0;
+    if (seg < n_segs && word < n_words) {
+        uint64_t m = a[word];
+        if (b) m &= b[word];
+        out[word] = m;
+        if (word == n_words - 1 && (nrows & 63)) m &= (1ull << (nrows & 63)) - 1;
+        pc = __popcll(m);
+    }
+    for (int d = 8; d >= 1; d >>= 1) pc += __shfl_xor(pc, d, 16);
+    if (sub == 0 && seg < n_segs) {
+        int64_t seg_rows = nrows - seg * SEG_ROWS;
+        if (seg_rows > SEG_ROWS) seg_rows = SEG_ROWS;
+        pop[seg] = (uint16_t)pc;
+        flags[seg] = pc == 0 ? TD_SEL_NONE : (pc == (uint32_t)seg_rows ? TD_SEL_ALL : TD_SEL_MIX);
+        if (pc) atomicAdd(total, (unsigned long long)pc);
+    }
+}
+
+/* meta: total_pass (i64) then n_segs (u32) and padding; the total is
+ * accumulated by atomics so it starts from zero. */
+__global__ void k_sel_meta(uint64_t* meta, uint32_t n_segs) {
+    if (threadIdx.x == 0) {
+        meta[0] = 0;
+        meta[1] = (uint64_t)n_segs;
+    }
+}
+
+static int write_total_start(const tdd_sel_t* s) {
+    LAUNCH("sel_meta", k_sel_meta, 1, 64, 0, (uint64_t*)s->base, s->n_segs);
+    return TD_OK;
+}
+
+extern "C" int tdd_sel_eval(const tdd_pred_t* pred, const tdd_sel_t* and_with, tdd_sel_t* out) {
+    if (pred->n_ins < 1 || pred->n_ins > TDD_PRED_MAX_INS) return TD_ERR_RANGE;
+    if (and_with && and_with->nrows != out->nrows) return TD_ERR_LENGTH;
+    int rc = write_total_start(out);
+    if (rc) return rc;
+    if (out->n_segs == 0) return TD_OK;
+    int grid = stream_grid(out->n_segs, 8);
+    LAUNCH("sel_eval", k_sel_eval, grid, SEL_T, 0, *pred, out->nrows, out->n_segs,
+           (const uint64_t*)(and_with ? and_with->bits : nullptr), out->flags, out->popcnt,
+           out->bits, (unsigned long long*)out->base);
+    return TD_OK;
+}
+
+extern "C" int tdd_sel_from_bool(const uint8_t* bools, tdd_sel_t* out) {
+    tdd_pred_t p;
+    memset(&p, 0, sizeof(p));
+    p.n_ins = 1;
+    p.n_cols = 1;
+    p.cols[0].data = bools;
+    p.cols[0].type = TD_U8;
+    p.cols[0].esz = 1;
+    p.ins[0].kind = TDD_P_BOOLCOL;
+    return tdd_sel_eval(&p, nullptr, out);
+}
+
+static int sel_words(const uint64_t* a, const uint64_t* b, tdd_sel_t* out) {
+    int rc = write_total_start(out);
+    if (rc) return rc;
+    if (out->n_segs == 0) return TD_OK;
+    int grid = (int)ceil_div(out->n_segs, SEL_T / 16);
+    LAUNCH("sel_words", k_sel_words, grid, SEL_T, 0, a, b, out->nrows, out->n_segs,
+           out->flags, out->popcnt, out->bits, (unsigned long long*)out->base);
+    return TD_OK;
+}
+
+extern "C" int tdd_sel_and(const tdd_sel_t* a, const tdd_sel_t* b, tdd_sel_t* out) {
+    if (a->nrows != b->nrows || a->nrows != out->nrows) return TD_ERR_LENGTH;
+    return sel_words(a->bits, b->bits, out);
+}
+
+extern "C" int tdd_sel_recompute(tdd_sel_t* s) {
+    return sel_words(s->bits, nullptr, s);
+}
+
+extern "C" int tdd_sel_total(const tdd_sel_t* s, int64_t* total) {
+    HC(hipMemcpyAsync(total, s->base, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    return TD_OK;
+}
+
+/* ------------------------------------------------------------ compaction -- */
+/* seg_off[s] = selected rows before segment s (scan of the popcounts).  For
+ * each segment the four waves first publish their words' popcounts, then
+ * every selected lane writes to seg_off + (bits in earlier words) + (bits of
+ * its own word in lower lanes). */
+
+#define MAXC 16
+struct GatherCols {
+    const void* src[MAXC];
+    void*       dst[MAXC];
+    uint8_t     esz[MAXC];
+    int32_t     n;
+};
+
+__device__ __forceinline__ void copy_elem(const GatherCols& G, int c, int64_t from, int64_t to) {
+    switch (G.esz[c]) {
+    case 1: ((uint8_t*)G.dst[c])[to] = ((const uint8_t*)G.src[c])[from]; break;
+    case 2: ((uint16_t*)G.dst[c])[to] = ((const uint16_t*)G.src[c])[from]; break;
+    case 4: ((uint32_t*)G.dst[c])[to] = ((const uint32_t*)G.src[c])[from]; break;
+    case 8: ((uint64_t*)G.dst[c])[to] = ((const uint64_t*)G.src[c])[from]; break;
+    default: {
+        const uint8_t* s = (const uint8_t*)G.src[c] + from * G.esz[c];
+        uint8_t* d = (uint8_t*)G.dst[c] + to * G.esz[c];
+        for (int b = 0; b < G.esz[c]; b++) d[b] = s[b];
+    }
+    }
+}
+
+/* mode 0: gather columns; mode 1: write row ids into idx. */
+__global__ __launch_bounds__(SEL_T) void k_compact(const uint64_t* bits, const uint8_t* flags,
+                                                   const int64_t* seg_off, int64_t nrows,
+                                                   uint32_t n_segs, GatherCols G, int64_t* idx,
+                                                   int mode) {
+    __shared__ uint32_t wpc[SEL_T / WAVE];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t n_words = (nrows + 63) >> 6;
+    for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
+        if (flags[seg] == TD_SEL_NONE) continue;       /* uniform across the block */
+        uint64_t m[4];
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int64_t word = seg * 16 + w * 4 + k;
+            m[k] = word < n_words ? bits[word] : 0;
+            if (word == n_words - 1 && (nrows & 63)) m[k] &= (1ull << (nrows & 63)) - 1;
+            c += __popcll(m[k]);
+        }
+        if (lane == 0) wpc[w] = c;
+        __syncthreads();
+        int64_t pos = seg_off[seg];
+        for (int i = 0; i < w; i++) pos += wpc[i];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int64_t row = (seg * 16 + w * 4 + k) * 64 + lane;
+            if ((m[k] >> lane) & 1) {
+                int64_t to = pos + lanes_below(m[k]);
+                if (mode == 1) {
+                    idx[to] = row;
+                } else {
+                    for (int cc = 0; cc < G.n; cc++) copy_elem(G, cc, row, to);
+                }
+            }
+            pos += __popcll(m[k]);
+        }
+        __syncthreads();
+    }
+}
+
+static int compact_impl(const tdd_sel_t* s, const GatherCols& G, int64_t* idx, int mode) {
+    if (s->n_segs == 0) return TD_OK;
+    SCRATCH(off, (size_t)(s->n_segs + 1) * 8);
+    int rc = tdd::scan_u16_i64(s->popcnt, s->n_segs, off.as<int64_t>());
+    if (rc) return rc;
+    int grid = stream_grid(s->n_segs, 8);
+    LAUNCH("compact", k_compact, grid, SEL_T, 0, s->bits, s->flags,
+           (const int64_t*)off.as<int64_t>(), s->nrows, s->n_segs, G, idx, mode);
+    return TD_OK;
+}
+
+extern "C" int tdd_sel_index(const tdd_sel_t* s, int64_t* idx) {
+    GatherCols G;
+    memset(&G, 0, sizeof(G));
+    return compact_impl(s, G, idx, 1);
+}
+
+extern "C" int tdd_sel_compact(const tdd_sel_t* s, int ncols, const tdd_col_t* src,
+                               void* const* dst) {
+    if (ncols < 0 || ncols > MAXC) return TD_ERR_RANGE;
+    GatherCols G;
+    memset(&G, 0, sizeof(G));
+    for (int c = 0; c < ncols; c++) {
+        G.src[c] = src[c].data;
+        G.dst[c] = dst[c];
+        G.esz[c] = src[c].esz;
+    }
+    G.n = ncols;
+    return compact_impl(s, G, nullptr, 0);
+}
+
+/* ---------------------------------------------------------------- gather -- */
+
+template <int E>
+__global__ __launch_bounds__(256) void k_gather(const uint8_t* src, const int64_t* idx,
+                                                int64_t n, int nullable, int esz,
+                                                uint8_t* dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        int64_t r = idx[i];
+        if (E == 8) {
+            ((uint64_t*)dst)[i] = (r < 0 && nullable) ? 0 : ((const uint64_t*)src)[r];
+        } else if (E == 4) {
+            ((uint32_t*)dst)[i] = (r < 0 && nullable) ? 0 : ((const uint32_t*)src)[r];
+        } else {
+            for (int b = 0; b < esz; b++)
+                dst[i * esz + b] = (r < 0 && nullable) ? 0 : src[r * esz + b];
+        }
+    }
+}
+
+extern "C" int tdd_gather(const tdd_col_t* src, const int64_t* idx, int64_t n, int nullable,
+                          void* dst) {
+    if (n <= 0) return TD_OK;
+    int grid = stream_grid(ceil_div(n, 256 * 4));
+    if (src->esz == 8)
+        LAUNCH("gather", k_gather<8>, grid, 256, 0, (const uint8_t*)src->data, idx, n, nullable,
+               8, (uint8_t*)dst);
+    else if (src->esz == 4)
+        LAUNCH("gather", k_gather<4>, grid, 256, 0, (const uint8_t*)src->data, idx, n, nullable,
+               4, (uint8_t*)dst);
+    else
+        LAUNCH("gather", k_gather<0>, grid, 256, 0, (const uint8_t*)src->data, idx, n, nullable,
+               (int)src->esz, (uint8_t*)dst);
+    return TD_OK;
+}

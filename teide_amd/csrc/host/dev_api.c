@@ -1,0 +1,185 @@
+/*
+ * dev_api.c -- the td_dev.h extensions: residency (device-resident copies
+ * of vectors, selections and tables), arena accounting, kernel timing and
+ * physical-plan hints.
+ */
+#include "internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+
+static int g_force_join, g_force_group, g_force_bits;
+static char g_plan[256] = "-";
+
+int td__force_join_path(void) { return g_force_join; }
+int td__force_group_path(void) { return g_force_group; }
+int td__force_radix_bits(void) { return g_force_bits; }
+
+void td__set_plan(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_plan, sizeof(g_plan), fmt, ap);
+    va_end(ap);
+}
+
+const char* td_dev_last_plan(void) { return g_plan; }
+
+void td_dev_force_paths(int join_path, int group_path, int radix_bits) {
+    g_force_join = join_path;
+    g_force_group = group_path;
+    g_force_bits = radix_bits;
+}
+
+int td_dev_available(void) { return td__dev_ready(); }
+
+td_err_t td_dev_sync(void) {
+    if (!td__dev_ready()) return TD_ERR_IO;
+    return (td_err_t)tdd_sync();
+}
+
+static size_t payload_bytes(td_t* v) {
+    if (v->type == TD_SEL) return td__sel_payload_bytes(v->len);
+    return (size_t)v->len * td__esz(v);
+}
+
+/* Device-resident copy of one vector / selection. */
+static td_t* put_vec(td_t* v) {
+    if (v->type <= 0 || v->type == TD_TABLE || v->type == TD_LIST || TD_IS_PARTED(v->type) ||
+        v->type == TD_MAPCOMMON)
+        return TD_ERR_PTR(TD_ERR_TYPE);
+    size_t bytes = payload_bytes(v);
+    td_t* d = td__hbm_vec_new(v->type, v->attrs & TD_SYM_W_MASK, v->len, bytes);
+    if (TD_IS_ERR(d)) return d;
+    int rc;
+    if (v->mmod == TD_MMOD_HBM) {
+        rc = tdd_d2d(td_hbm_ref(d)->dptr, td_hbm_ref(v)->dptr, bytes);
+    } else {
+        const void* src = td_data(v);
+        if (v->attrs & TD_ATTR_SLICE) {
+            td_t* p = v->slice_parent;
+            if (p->mmod == TD_MMOD_HBM) {
+                rc = tdd_d2d(td_hbm_ref(d)->dptr,
+                             (char*)td_hbm_ref(p)->dptr + v->slice_offset * td__esz(p), bytes);
+                goto done;
+            }
+            src = (const char*)td_data(p) + v->slice_offset * td__esz(p);
+        }
+        rc = tdd_h2d(td_hbm_ref(d)->dptr, src, bytes);
+    }
+done:
+    if (!rc) rc = tdd_sync();
+    if (rc) { td_release(d); return TD_ERR_PTR(rc); }
+    return d;
+}
+
+td_t* td_dev_put(td_t* v) {
+    if (!v || TD_IS_ERR(v)) return v;
+    if (!td__dev_ready()) return TD_ERR_PTR(TD_ERR_IO);
+    if (v->type != TD_TABLE) return put_vec(v);
+    int64_t nc = td_table_ncols(v);
+    td_t* t = td_table_new(nc);
+    if (TD_IS_ERR(t)) return t;
+    for (int64_t c = 0; c < nc; c++) {
+        td_t* d = put_vec(td_table_get_col_idx(v, c));
+        if (TD_IS_ERR(d)) { td_release(t); return d; }
+        t = td_table_add_col(t, td_table_col_name(v, c), d);
+        td_release(d);
+        if (TD_IS_ERR(t)) return t;
+    }
+    return t;
+}
+
+/* Host copy of one resident vector / selection. */
+static td_t* get_vec(td_t* v) {
+    size_t bytes = payload_bytes(v);
+    td_t* h;
+    if (v->type == TD_SEL) {
+        h = td_sel_new(v->len);
+    } else if (v->type == TD_SYM) {
+        h = td_sym_vec_new(v->attrs & TD_SYM_W_MASK, v->len);
+    } else {
+        h = td_vec_new(v->type, v->len);
+    }
+    if (TD_IS_ERR(h)) return h;
+    if (v->type != TD_SEL) h->len = v->len;
+    int rc = tdd_d2h(td_data(h), td_hbm_ref(v)->dptr, bytes);
+    if (rc) { td_release(h); return TD_ERR_PTR(rc); }
+    return h;
+}
+
+td_t* td_dev_get(td_t* v) {
+    if (!v || TD_IS_ERR(v)) return v;
+    if (v->type == TD_TABLE) {
+        int64_t nc = td_table_ncols(v);
+        td_t* t = td_table_new(nc);
+        if (TD_IS_ERR(t)) return t;
+        for (int64_t c = 0; c < nc; c++) {
+            td_t* col = td_table_get_col_idx(v, c);
+            td_t* h = col->mmod == TD_MMOD_HBM ? get_vec(col) : (td_retain(col), col);
+            if (TD_IS_ERR(h)) { td_release(t); return h; }
+            t = td_table_add_col(t, td_table_col_name(v, c), h);
+            td_release(h);
+            if (TD_IS_ERR(t)) return t;
+        }
+        return t;
+    }
+    if (v->mmod != TD_MMOD_HBM) { td_retain(v); return v; }
+    return get_vec(v);
+}
+
+td_t* td__to_host(td_t* v) {
+    if (!v || TD_IS_ERR(v)) return v;
+    int need = v->mmod == TD_MMOD_HBM;
+    if (v->type == TD_TABLE)
+        for (int64_t c = 0; c < td_table_ncols(v) && !need; c++)
+            need = td_table_get_col_idx(v, c)->mmod == TD_MMOD_HBM;
+    if (!need) return v;
+    td_t* h = td_dev_get(v);
+    td_release(v);
+    return h;
+}
+
+int td_dev_is_resident(td_t* v) {
+    if (!v || TD_IS_ERR(v)) return 0;
+    if (v->type == TD_TABLE) {
+        for (int64_t c = 0; c < td_table_ncols(v); c++)
+            if (td_table_get_col_idx(v, c)->mmod != TD_MMOD_HBM) return 0;
+        return 1;
+    }
+    return v->mmod == TD_MMOD_HBM;
+}
+
+void td_dev_mem_stats(td_dev_mem_t* out) {
+    memset(out, 0, sizeof(*out));
+    if (!td__dev_ready()) return;
+    out->in_use = td__arena_in_use();
+    out->peak = td__arena_peak();
+    out->reserved = td__arena_reserved();
+    out->budget = td__arena_budget();
+    uint64_t f = 0, t = 0;
+    if (tdd_mem_info(&f, &t) == TD_OK) out->device_total = t;
+}
+
+void td_dev_set_budget(uint64_t bytes) {
+    if (!td__dev_ready()) return;
+    uint64_t f = 0, t = 0;
+    if (!bytes && tdd_mem_info(&f, &t) == TD_OK) bytes = (uint64_t)((double)t * 0.9);
+    td__arena_set_budget((size_t)bytes);
+}
+
+void td_dev_prof_enable(int on) { if (td__dev_ready()) tdd_prof_enable(on); }
+void td_dev_prof_reset(void) { if (td__dev_ready()) tdd_prof_reset(); }
+
+int td_dev_prof_read(td_dev_prof_t* rows, int cap) {
+    if (!td__dev_ready()) return 0;
+    int n = 0;
+    tdd_prof_row_t tmp[128];
+    if (cap > 128) cap = 128;
+    if (tdd_prof_read(tmp, cap, &n) != TD_OK) return 0;
+    for (int i = 0; i < n; i++) {
+        memcpy(rows[i].name, tmp[i].name, sizeof(rows[i].name));
+        rows[i].launches = tmp[i].launches;
+        rows[i].total_ms = tmp[i].total_ms;
+    }
+    return n;
+}

@@ -1,0 +1,949 @@
+/*
+ * exec.c -- td_execute: maps a td.h query DAG onto the device layer.
+ *
+ * The node semantics are the reference's exec_node (src/ops/exec.c:11561-
+ * 12321); every operator on the filter -> join -> group-by path runs on the
+ * GPU through tdd.h and anything else reports TD_ERR_NYI, so a query either
+ * runs natively or fails loudly.  Supported shapes:
+ *
+ *   FILTER(table, P)        P (comparisons of columns with constants or
+ *                           columns under AND/OR/NOT) is evaluated over the
+ *                           bound table into a device TD_SEL, ANDed into
+ *                           g->selection (exec.c:11706-11731); the table is
+ *                           compacted by td_execute's final step
+ *                           (exec.c:12312-12319).
+ *   FILTER(vector, P)       eager compaction of the vector (exec.c:1978-2002).
+ *   GROUP(keys; aggs)       over g->table, masked by g->selection, which the
+ *                           group consumes.  (The reference compacts the group
+ *                           output against the leftover selection as well,
+ *                           reading past the output's end; here a selection
+ *                           always means "rows of the bound table".)
+ *                           Keys and aggregate inputs of the form FILTER(SCAN
+ *                           c, P) with one shared P (the README quick-start
+ *                           shape) mean "filter by P, then group".
+ *   JOIN(L, R)              L masked by g->selection (the reference compacts
+ *                           it first, exec.c:11846-11852); output columns are
+ *                           all left columns then the right non-key columns
+ *                           (exec.c:8607-8728).
+ *   GROUP over a JOIN       when the group's columns are missing from
+ *                           g->table but present in the output of the graph's
+ *                           JOIN(FILTER*(CONST table), CONST table), the group
+ *                           runs over the join output, fused on the device
+ *                           (the reference connects EXPAND -> GROUP the same
+ *                           way, exec.c:11766-11800).
+ *   SUM/COUNT/MIN/MAX/AVG/FIRST/LAST/VAR.. of a vector; ALIAS, MATERIALIZE.
+ *
+ * Residency: when every column a query reads is device-resident its result
+ * stays device-resident; otherwise inputs are copied in for the call and
+ * the result comes back as ordinary host vectors.
+ */
+#include "internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#define MAX_TMP 256
+
+typedef struct {
+    td_graph_t* g;
+    int   all_resident;
+    void* tmp[MAX_TMP];
+    int   ntmp;
+} xctx_t;
+
+static td_t* xnode(xctx_t* x, td_op_t* op);
+
+static int keep_tmp(xctx_t* x, void* d) {
+    if (x->ntmp >= MAX_TMP) { td__arena_free(d); return TD_ERR_OOM; }
+    x->tmp[x->ntmp++] = d;
+    return TD_OK;
+}
+
+static void drop_tmps(xctx_t* x) {
+    for (int i = 0; i < x->ntmp; i++) td__arena_free(x->tmp[i]);
+    x->ntmp = 0;
+}
+
+/* Device view of a column: resident columns in place, slices as an offset
+ * into their parent, host columns copied into a temporary arena block. */
+static int dev_col(xctx_t* x, td_t* v, tdd_col_t* out) {
+    memset(out, 0, sizeof(*out));
+    if (!v || TD_IS_ERR(v)) return TD_ERR_SCHEMA;
+    if (v->type <= 0 || v->type == TD_TABLE || v->type == TD_LIST || v->type == TD_SEL ||
+        TD_IS_PARTED(v->type) || v->type == TD_MAPCOMMON)
+        return TD_ERR_NYI;
+    out->type = v->type;
+    out->attrs = v->attrs & TD_SYM_W_MASK;
+    out->esz = td__esz(v);
+    if (v->mmod == TD_MMOD_HBM) {
+        out->data = td_hbm_ref(v)->dptr;
+        return TD_OK;
+    }
+    td_t* src = v;
+    int64_t off = 0;
+    if (v->attrs & TD_ATTR_SLICE) {
+        src = v->slice_parent;
+        off = v->slice_offset;
+        if (src->mmod == TD_MMOD_HBM) {
+            out->data = (char*)td_hbm_ref(src)->dptr + off * out->esz;
+            return TD_OK;
+        }
+    }
+    x->all_resident = 0;
+    size_t bytes = (size_t)v->len * out->esz;
+    void* d = td__arena_alloc(bytes ? bytes : 8);
+    if (!d) return TD_ERR_OOM;
+    int rc = keep_tmp(x, d);
+    if (rc) return rc;
+    if (bytes && tdd_h2d(d, (const char*)td_data(src) + off * out->esz, bytes) != TD_OK)
+        return TD_ERR_IO;
+    out->data = d;
+    return TD_OK;
+}
+
+/* Device view of a selection (host selections copied in). */
+static int dev_sel(xctx_t* x, td_t* s, tdd_sel_t* out) {
+    void* tmp = NULL;
+    void* d = td__sel_dev_payload(s, &tmp);
+    if (!d) return TD_ERR_OOM;
+    if (tmp) {
+        x->all_resident = 0;
+        int rc = keep_tmp(x, tmp);
+        if (rc) return rc;
+    }
+    *out = td__sel_view(d, s->len);
+    return TD_OK;
+}
+
+static td_t* scan_table(td_graph_t* g, td_op_ext_t* e) {
+    uint16_t stored = 0;
+    memcpy(&stored, e->base.pad, sizeof(stored));
+    if (stored && g->tables && stored - 1 < g->n_tables) return g->tables[stored - 1];
+    return g->table;
+}
+
+/* Column a SCAN node names, in `tbl` (or its own table when tbl is NULL). */
+static td_t* scan_col(td_graph_t* g, td_op_t* op, td_t* tbl) {
+    td_op_ext_t* e = td__find_ext(g, op->id);
+    if (!e || e->base.opcode != OP_SCAN) return NULL;
+    if (!tbl) tbl = scan_table(g, e);
+    return tbl ? td_table_get_col(tbl, e->sym) : NULL;
+}
+
+/* ---------------------------------------------------------- predicates -- */
+
+static int mirror_cmp(int op) {
+    switch (op) {
+    case OP_LT: return OP_GT;
+    case OP_LE: return OP_GE;
+    case OP_GT: return OP_LT;
+    case OP_GE: return OP_LE;
+    default: return op;
+    }
+}
+
+/* Numeric value of an atom (exec.c atom_to_numeric): 1 when f64. */
+static int atom_num(td_t* a, int64_t* i, double* f, int* is_f) {
+    switch (a->type) {
+    case TD_ATOM_F64: *f = a->f64; *i = (int64_t)a->f64; *is_f = 1; return 1;
+    case TD_ATOM_I64: case TD_ATOM_SYM: case TD_ATOM_TIMESTAMP:
+    case TD_ATOM_DATE: case TD_ATOM_TIME:
+        *i = a->i64; *f = (double)a->i64; *is_f = 0; return 1;
+    case TD_ATOM_I32: *i = a->i32; *f = a->i32; *is_f = 0; return 1;
+    case TD_ATOM_I16: *i = a->i16; *f = a->i16; *is_f = 0; return 1;
+    case TD_ATOM_U8: case TD_ATOM_BOOL: *i = a->u8; *f = a->u8; *is_f = 0; return 1;
+    default: return 0;
+    }
+}
+
+typedef struct {
+    xctx_t*     x;
+    td_t*       tbl;
+    tdd_pred_t* P;
+    td_t*       cols[TDD_PRED_MAX_COLS];
+} pcomp_t;
+
+static int pred_col(pcomp_t* c, td_t* col) {
+    for (int i = 0; i < c->P->n_cols; i++) if (c->cols[i] == col) return i;
+    if (c->P->n_cols >= TDD_PRED_MAX_COLS) return -TD_ERR_NYI;
+    int rc = dev_col(c->x, col, &c->P->cols[c->P->n_cols]);
+    if (rc) return -rc;
+    c->cols[c->P->n_cols] = col;
+    return c->P->n_cols++;
+}
+
+static tdd_pred_ins_t* emit(pcomp_t* c) {
+    if (c->P->n_ins >= TDD_PRED_MAX_INS) return NULL;
+    tdd_pred_ins_t* in = &c->P->ins[c->P->n_ins++];
+    memset(in, 0, sizeof(*in));
+    return in;
+}
+
+static int pred_compile(pcomp_t* c, td_op_t* op) {
+    td_graph_t* g = c->x->g;
+    tdd_pred_ins_t* in;
+    switch (op->opcode) {
+    case OP_AND: case OP_OR: {
+        int rc = pred_compile(c, op->inputs[0]);
+        if (!rc) rc = pred_compile(c, op->inputs[1]);
+        if (rc) return rc;
+        if (!(in = emit(c))) return TD_ERR_NYI;
+        in->kind = op->opcode == OP_AND ? TDD_P_AND : TDD_P_OR;
+        return TD_OK;
+    }
+    case OP_NOT: {
+        int rc = pred_compile(c, op->inputs[0]);
+        if (rc) return rc;
+        if (!(in = emit(c))) return TD_ERR_NYI;
+        in->kind = TDD_P_NOT;
+        return TD_OK;
+    }
+    case OP_SCAN: {                                  /* a BOOL column as predicate */
+        td_t* col = scan_col(g, op, c->tbl);
+        if (!col) return TD_ERR_SCHEMA;
+        if (col->type != TD_BOOL && col->type != TD_U8) return TD_ERR_TYPE;
+        int ci = pred_col(c, col);
+        if (ci < 0) return -ci;
+        if (!(in = emit(c))) return TD_ERR_NYI;
+        in->kind = TDD_P_BOOLCOL;
+        in->a = (uint8_t)ci;
+        return TD_OK;
+    }
+    case OP_CONST: {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        int64_t i; double f; int isf;
+        if (!e || !e->literal || !atom_num(e->literal, &i, &f, &isf)) return TD_ERR_NYI;
+        if (!(in = emit(c))) return TD_ERR_NYI;
+        in->kind = TDD_P_CONST;
+        in->ci = isf ? (f != 0.0) : (i != 0);
+        return TD_OK;
+    }
+    case OP_EQ: case OP_NE: case OP_LT: case OP_LE: case OP_GT: case OP_GE: {
+        td_op_t* a = op->inputs[0];
+        td_op_t* b = op->inputs[1];
+        int cmp = op->opcode;
+        if (a->opcode == OP_CONST && b->opcode == OP_SCAN) {
+            td_op_t* t = a; a = b; b = t;
+            cmp = mirror_cmp(cmp);
+        }
+        if (a->opcode != OP_SCAN) return TD_ERR_NYI;   /* general expressions: not yet */
+        td_t* ca = scan_col(g, a, c->tbl);
+        if (!ca) return TD_ERR_SCHEMA;
+        int ia = pred_col(c, ca);
+        if (ia < 0) return -ia;
+        if (b->opcode == OP_SCAN) {
+            td_t* cb = scan_col(g, b, c->tbl);
+            if (!cb) return TD_ERR_SCHEMA;
+            int ib = pred_col(c, cb);
+            if (ib < 0) return -ib;
+            if (!(in = emit(c))) return TD_ERR_NYI;
+            in->kind = TDD_P_CMP_COL;
+            in->cmp = (uint8_t)cmp;
+            in->a = (uint8_t)ia;
+            in->b = (uint8_t)ib;
+            in->as_f64 = ca->type == TD_F64 || cb->type == TD_F64;
+            return TD_OK;
+        }
+        if (b->opcode != OP_CONST) return TD_ERR_NYI;
+        td_op_ext_t* e = td__find_ext(g, b->id);
+        if (!e || !e->literal) return TD_ERR_NYI;
+        if (!(in = emit(c))) return TD_ERR_NYI;
+        int64_t i; double f; int isf;
+        if (!atom_num(e->literal, &i, &f, &isf)) {
+            if (e->literal->type != TD_ATOM_STR) return TD_ERR_NYI;
+            /* A string constant compares as its symbol id (exec.c:826-839);
+             * an unknown string matches nothing. */
+            int64_t sid = td_sym_find(td_str_ptr(e->literal), td_str_len(e->literal));
+            if (sid < 0) {
+                in->kind = TDD_P_CONST;
+                in->ci = cmp == OP_NE;
+                return TD_OK;
+            }
+            i = sid; f = (double)sid; isf = 0;
+        }
+        in->kind = TDD_P_CMP_CONST;
+        in->cmp = (uint8_t)cmp;
+        in->a = (uint8_t)ia;
+        in->as_f64 = ca->type == TD_F64 || isf;
+        in->ci = i;
+        in->cf = isf ? f : (double)i;
+        return TD_OK;
+    }
+    default:
+        return TD_ERR_NYI;
+    }
+}
+
+static int compile_pred(xctx_t* x, td_t* tbl, td_op_t* op, tdd_pred_t* P) {
+    memset(P, 0, sizeof(*P));
+    pcomp_t c;
+    memset(&c, 0, sizeof(c));
+    c.x = x;
+    c.tbl = tbl;
+    c.P = P;
+    return pred_compile(&c, op);
+}
+
+/* Evaluate P over `nrows` into a new device selection, ANDed with `and_with`. */
+static td_t* eval_sel(xctx_t* x, const tdd_pred_t* P, int64_t nrows, td_t* and_with) {
+    td_t* s = td__sel_new_dev(nrows);
+    if (TD_IS_ERR(s)) return s;
+    tdd_sel_t v = td__sel_view(td_hbm_ref(s)->dptr, nrows), w;
+    int rc = TD_OK;
+    if (and_with) rc = dev_sel(x, and_with, &w);
+    if (!rc) rc = tdd_sel_eval(P, and_with ? &w : NULL, &v);
+    if (rc) { td_release(s); return TD_ERR_PTR(rc); }
+    return s;
+}
+
+/* ----------------------------------------------------------- compaction -- */
+
+/* Table rows selected by s, as a new table of device-resident columns. */
+static td_t* compact_table(xctx_t* x, td_t* tbl, td_t* s) {
+    int64_t nrows = td_table_nrows(tbl), ncols = td_table_ncols(tbl);
+    if (s->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
+    tdd_sel_t v;
+    int rc = dev_sel(x, s, &v);
+    if (rc) return TD_ERR_PTR(rc);
+    int64_t pass = 0;
+    rc = tdd_sel_total(&v, &pass);
+    if (rc) return TD_ERR_PTR(rc);
+    td_t* out = td_table_new(ncols);
+    if (TD_IS_ERR(out)) return out;
+    for (int64_t c0 = 0; c0 < ncols; c0 += 16) {
+        int n = (int)(ncols - c0 < 16 ? ncols - c0 : 16);
+        tdd_col_t src[16];
+        void* dst[16];
+        td_t* nc[16];
+        for (int i = 0; i < n; i++) {
+            td_t* col = td_table_get_col_idx(tbl, c0 + i);
+            rc = dev_col(x, col, &src[i]);
+            if (rc) goto fail;
+            nc[i] = td__hbm_vec_new(col->type, col->attrs & TD_SYM_W_MASK, pass,
+                                    (size_t)pass * src[i].esz);
+            if (TD_IS_ERR(nc[i])) { rc = TD_ERR_OOM; goto fail; }
+            dst[i] = td_hbm_ref(nc[i])->dptr;
+            out = td_table_add_col(out, td_table_col_name(tbl, c0 + i), nc[i]);
+            td_release(nc[i]);
+            if (TD_IS_ERR(out)) return out;
+        }
+        rc = tdd_sel_compact(&v, n, src, dst);
+        if (rc) goto fail;
+    }
+    return out;
+fail:
+    td_release(out);
+    return TD_ERR_PTR(rc);
+}
+
+static td_t* compact_vec(xctx_t* x, td_t* vec, td_t* s) {
+    if (s->len != vec->len) return TD_ERR_PTR(TD_ERR_LENGTH);
+    tdd_sel_t v;
+    tdd_col_t src;
+    int rc = dev_sel(x, s, &v);
+    if (!rc) rc = dev_col(x, vec, &src);
+    int64_t pass = 0;
+    if (!rc) rc = tdd_sel_total(&v, &pass);
+    if (rc) return TD_ERR_PTR(rc);
+    td_t* out = td__hbm_vec_new(vec->type, vec->attrs & TD_SYM_W_MASK, pass, (size_t)pass * src.esz);
+    if (TD_IS_ERR(out)) return out;
+    void* dst = td_hbm_ref(out)->dptr;
+    rc = tdd_sel_compact(&v, 1, &src, &dst);
+    if (rc) { td_release(out); return TD_ERR_PTR(rc); }
+    return out;
+}
+
+/* --------------------------------------------------------------- group -- */
+
+static const char* agg_suffix(uint16_t op) {
+    switch (op) {
+    case OP_SUM: return "_sum";
+    case OP_COUNT: return "_count";
+    case OP_AVG: return "_mean";
+    case OP_MIN: return "_min";
+    case OP_MAX: return "_max";
+    case OP_FIRST: return "_first";
+    case OP_LAST: return "_last";
+    case OP_STDDEV: return "_stddev";
+    case OP_STDDEV_POP: return "_stddev_pop";
+    case OP_VAR: return "_var";
+    case OP_VAR_POP: return "_var_pop";
+    default: return "";
+    }
+}
+
+/* Output column name of aggregate a (emit_agg_columns naming,
+ * exec.c:4724-4771): "<input column><suffix>", or "_e<a><suffix>" for a
+ * non-SCAN input. */
+static int64_t agg_name(td_graph_t* g, td_op_t* in, int a, uint16_t op) {
+    char buf[320];
+    td_op_t* src = in;
+    if (src->opcode == OP_FILTER) src = src->inputs[0];     /* README shape */
+    td_op_ext_t* e = td__find_ext(g, src->id);
+    if (e && e->base.opcode == OP_SCAN) {
+        size_t n;
+        const char* s = td__sym_cstr(e->sym, &n);
+        const char* sfx = agg_suffix(op);
+        if (s && n + strlen(sfx) < sizeof(buf)) {
+            memcpy(buf, s, n);
+            memcpy(buf + n, sfx, strlen(sfx));
+            return td_sym_intern(buf, n + strlen(sfx));
+        }
+        return e->sym;
+    }
+    int len = snprintf(buf, sizeof(buf), "_e%d%s", a, agg_suffix(op));
+    return td_sym_intern(buf, (size_t)len);
+}
+
+static int64_t key_name(td_graph_t* g, td_op_t* k, int idx) {
+    td_op_t* src = k->opcode == OP_FILTER ? k->inputs[0] : k;
+    td_op_ext_t* e = td__find_ext(g, src->id);
+    return e && e->base.opcode == OP_SCAN ? e->sym : idx;
+}
+
+/* Wrap a device group result as a table of resident columns. */
+static td_t* group_table(td_graph_t* g, td_op_ext_t* ext, const tdd_group_result_t* r,
+                         td_t* const* key_src, td_t* const* agg_src) {
+    int nk = ext->n_keys, na = ext->n_aggs;
+    td_t* out = td_table_new(nk + na);
+    if (TD_IS_ERR(out)) goto oom;
+    for (int k = 0; k < nk; k++) {
+        td_t* s = key_src[k];
+        td_t* col = td__hbm_wrap(s->type, s->attrs & TD_SYM_W_MASK, r->n_groups,
+                                 r->keys[k], (size_t)r->n_groups * td__esz(s));
+        if (TD_IS_ERR(col)) { td_release(out); goto oom_from_key; }
+        out = td_table_add_col(out, key_name(g, ext->keys[k], k), col);
+        td_release(col);
+        if (TD_IS_ERR(out)) return out;
+        ((tdd_group_result_t*)r)->keys[k] = NULL;
+    }
+    for (int a = 0; a < na; a++) {
+        int8_t t = r->out_types[a];
+        uint8_t attrs = 0, esz = 8;
+        if (t != TD_F64 && t != TD_I64 && agg_src[a]) {
+            attrs = agg_src[a]->attrs & TD_SYM_W_MASK;
+            esz = td__esz(agg_src[a]);
+        }
+        td_t* col = td__hbm_wrap(t, attrs, r->n_groups, r->vals[a], (size_t)r->n_groups * esz);
+        if (TD_IS_ERR(col)) { td_release(out); goto oom_from_agg; }
+        out = td_table_add_col(out, agg_name(g, ext->agg_ins[a], a, ext->agg_ops[a]), col);
+        td_release(col);
+        if (TD_IS_ERR(out)) return out;
+        ((tdd_group_result_t*)r)->vals[a] = NULL;
+    }
+    return out;
+oom:
+oom_from_key:
+oom_from_agg:
+    for (int k = 0; k < nk; k++) td__arena_free(r->keys[k]);
+    for (int a = 0; a < na; a++) td__arena_free(r->vals[a]);
+    return TD_ERR_PTR(TD_ERR_OOM);
+}
+
+/* All FILTER(SCAN, P) nodes of a group sharing one P: that P, else NULL. */
+static td_op_t* readme_pred(td_op_ext_t* ext) {
+    td_op_t* p = NULL;
+    int n = ext->n_keys + ext->n_aggs;
+    for (int i = 0; i < n; i++) {
+        td_op_t* in = i < ext->n_keys ? ext->keys[i] : ext->agg_ins[i - ext->n_keys];
+        if (in->opcode != OP_FILTER) return NULL;
+        if (in->inputs[0]->opcode != OP_SCAN) return NULL;
+        if (p && in->inputs[1] != p) return NULL;
+        p = in->inputs[1];
+    }
+    return p;
+}
+
+static td_op_t* strip_filter(td_op_t* op) {
+    return op->opcode == OP_FILTER ? op->inputs[0] : op;
+}
+
+static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* join);
+
+static td_t* exec_group(xctx_t* x, td_op_t* op) {
+    td_graph_t* g = x->g;
+    td_op_ext_t* ext = td__find_ext(g, op->id);
+    if (!ext) return TD_ERR_PTR(TD_ERR_NYI);
+    if (ext->n_keys < 1 || ext->n_keys > TDD_MAX_KEYS || ext->n_aggs > TDD_MAX_AGGS)
+        return TD_ERR_PTR(TD_ERR_NYI);
+    td_t* tbl = g->table;
+    if (!tbl) return TD_ERR_PTR(TD_ERR_SCHEMA);
+
+    /* GROUP over the graph's JOIN output? */
+    int missing = 0;
+    for (int i = 0; i < ext->n_keys + ext->n_aggs; i++) {
+        td_op_t* in = strip_filter(i < ext->n_keys ? ext->keys[i] : ext->agg_ins[i - ext->n_keys]);
+        if (in->opcode == OP_SCAN && !scan_col(g, in, tbl)) missing = 1;
+    }
+    if (missing) {
+        for (uint32_t i = 0; i < g->node_count; i++)
+            if (g->nodes[i].opcode == OP_JOIN) return exec_join_group(x, op, ext, &g->nodes[i]);
+        return TD_ERR_PTR(TD_ERR_SCHEMA);
+    }
+
+    int64_t nrows = td_table_nrows(tbl);
+    tdd_group_spec_t S;
+    memset(&S, 0, sizeof(S));
+    S.nrows = nrows;
+    S.n_keys = ext->n_keys;
+    S.n_aggs = ext->n_aggs;
+    S.force_path = td__force_group_path();
+    td_t* key_src[TDD_MAX_KEYS];
+    td_t* agg_src[TDD_MAX_AGGS];
+    int rc;
+    for (int k = 0; k < ext->n_keys; k++) {
+        td_op_t* kop = strip_filter(ext->keys[k]);
+        if (kop->opcode != OP_SCAN) return TD_ERR_PTR(TD_ERR_NYI);
+        key_src[k] = scan_col(g, kop, tbl);
+        if (!key_src[k]) return TD_ERR_PTR(TD_ERR_SCHEMA);
+        if (key_src[k]->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
+        if ((rc = dev_col(x, key_src[k], &S.keys[k]))) return TD_ERR_PTR(rc);
+    }
+    for (int a = 0; a < ext->n_aggs; a++) {
+        td_op_t* aop = strip_filter(ext->agg_ins[a]);
+        S.ops[a] = ext->agg_ops[a];
+        agg_src[a] = NULL;
+        if (aop->opcode == OP_SCAN) {
+            agg_src[a] = scan_col(g, aop, tbl);
+            if (!agg_src[a]) return TD_ERR_PTR(TD_ERR_SCHEMA);
+            if (agg_src[a]->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
+            if ((rc = dev_col(x, agg_src[a], &S.aggs[a]))) return TD_ERR_PTR(rc);
+        } else if (!(aop->opcode == OP_CONST && S.ops[a] == OP_COUNT)) {
+            return TD_ERR_PTR(TD_ERR_NYI);               /* expression inputs: not yet */
+        }
+    }
+
+    /* Mask: the README shape's shared predicate, else the lazy selection. */
+    td_t* sel = NULL;
+    td_op_t* rp = readme_pred(ext);
+    if (rp) {
+        tdd_pred_t P;
+        if ((rc = compile_pred(x, tbl, rp, &P))) return TD_ERR_PTR(rc);
+        sel = eval_sel(x, &P, nrows, g->selection);
+        if (TD_IS_ERR(sel)) return sel;
+    } else if (g->selection) {
+        if (g->selection->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
+        sel = g->selection;
+        td_retain(sel);
+    }
+    if (g->selection) { td_release(g->selection); g->selection = NULL; }
+    tdd_sel_t mv;
+    if (sel && (rc = dev_sel(x, sel, &mv))) { td_release(sel); return TD_ERR_PTR(rc); }
+    S.mask = sel ? &mv : NULL;
+    tdd_group_result_t R;
+    rc = tdd_group(&S, &R);
+    td_release(sel);
+    if (rc) return TD_ERR_PTR(rc);
+    td__set_plan("group path=%d groups=%lld", R.path, (long long)R.n_groups);
+    return group_table(g, ext, &R, key_src, agg_src);
+}
+
+/* ---------------------------------------------------------------- join -- */
+
+static int join_is_right_key(td_graph_t* g, td_op_ext_t* je, int64_t name) {
+    for (int k = 0; k < je->join.n_join_keys; k++) {
+        td_op_ext_t* rk = td__find_ext(g, je->join.right_keys[k]->id);
+        if (rk && rk->base.opcode == OP_SCAN && rk->sym == name) return 1;
+    }
+    return 0;
+}
+
+static int join_keys(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* right, tdd_join_spec_t* J) {
+    td_graph_t* g = x->g;
+    J->n_keys = je->join.n_join_keys;
+    if (J->n_keys < 1 || J->n_keys > TDD_MAX_KEYS) return TD_ERR_NYI;
+    for (int k = 0; k < J->n_keys; k++) {
+        td_t* lc = scan_col(g, je->join.left_keys[k], left);
+        td_t* rc_ = scan_col(g, je->join.right_keys[k], right);
+        if (!rc_) {
+            td_op_ext_t* re = td__find_ext(g, je->join.right_keys[k]->id);
+            if (re && re->base.opcode == OP_CONST && re->literal && td_is_vec(re->literal))
+                rc_ = re->literal;
+        }
+        if (!lc || !rc_) return TD_ERR_SCHEMA;
+        int rc = dev_col(x, lc, &J->lkeys[k]);
+        if (!rc) rc = dev_col(x, rc_, &J->rkeys[k]);
+        if (rc) return rc;
+    }
+    return TD_OK;
+}
+
+static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int nullable) {
+    tdd_col_t src;
+    int rc = dev_col(x, col, &src);
+    if (rc) return TD_ERR_PTR(rc);
+    td_t* out = td__hbm_vec_new(col->type, col->attrs & TD_SYM_W_MASK, n, (size_t)n * src.esz);
+    if (TD_IS_ERR(out)) return out;
+    rc = tdd_gather(&src, idx, n, nullable, td_hbm_ref(out)->dptr);
+    if (rc) { td_release(out); return TD_ERR_PTR(rc); }
+    return out;
+}
+
+static td_t* exec_join(xctx_t* x, td_op_t* op) {
+    td_graph_t* g = x->g;
+    td_op_ext_t* je = td__find_ext(g, op->id);
+    if (!je) return TD_ERR_PTR(TD_ERR_NYI);
+    td_t* left = NULL;
+    td_t* right = NULL;
+    left = xnode(x, op->inputs[0]);
+    if (TD_IS_ERR(left)) return left;
+    right = xnode(x, op->inputs[1]);
+    if (TD_IS_ERR(right)) { td_release(left); return right; }
+    td_t* result = NULL;
+    if (left->type != TD_TABLE || right->type != TD_TABLE) {
+        result = TD_ERR_PTR(TD_ERR_TYPE);
+        goto done;
+    }
+    int64_t nl = td_table_nrows(left), nr = td_table_nrows(right);
+    if (nl > INT32_MAX || nr > INT32_MAX) { result = TD_ERR_PTR(TD_ERR_NYI); goto done; }
+    tdd_join_spec_t J;
+    memset(&J, 0, sizeof(J));
+    J.nl = nl;
+    J.nr = nr;
+    J.join_type = je->join.join_type;
+    J.force_path = td__force_join_path();
+    J.radix_bits = td__force_radix_bits();
+    int rc = join_keys(x, je, left, right, &J);
+    if (rc) { result = TD_ERR_PTR(rc); goto done; }
+    td_t* sel = g->selection;
+    tdd_sel_t sv;
+    if (sel) {
+        if (sel->len != nl) { result = TD_ERR_PTR(TD_ERR_LENGTH); goto done; }
+        if ((rc = dev_sel(x, sel, &sv))) { result = TD_ERR_PTR(rc); goto done; }
+        J.lsel = &sv;
+    }
+    tdd_join_result_t R;
+    rc = tdd_join(&J, &R);
+    if (sel) { td_release(g->selection); g->selection = NULL; }
+    if (rc) { result = TD_ERR_PTR(rc); goto done; }
+    td__set_plan("join path=%d radix_bits=%d pairs=%lld", R.path, R.radix_bits,
+                 (long long)R.n_pairs);
+    int64_t lnc = td_table_ncols(left), rnc = td_table_ncols(right);
+    result = td_table_new(lnc + rnc);
+    int ln = 0, rn = 0;
+    for (int64_t c = 0; c < lnc && ln < 16 && !TD_IS_ERR(result); c++) {
+        td_t* col = td_table_get_col_idx(left, c);
+        if (!col) continue;
+        td_t* o = gather_col(x, col, R.l_idx, R.n_pairs, J.join_type == 2);
+        if (TD_IS_ERR(o)) { td_release(result); result = o; break; }
+        result = td_table_add_col(result, td_table_col_name(left, c), o);
+        td_release(o);
+        ln++;
+    }
+    for (int64_t c = 0; c < rnc && rn < 16 && !TD_IS_ERR(result); c++) {
+        td_t* col = td_table_get_col_idx(right, c);
+        int64_t name = td_table_col_name(right, c);
+        if (!col || join_is_right_key(g, je, name)) continue;
+        td_t* o = gather_col(x, col, R.r_idx, R.n_pairs, J.join_type >= 1);
+        if (TD_IS_ERR(o)) { td_release(result); result = o; break; }
+        result = td_table_add_col(result, name, o);
+        td_release(o);
+        rn++;
+    }
+    td__arena_free(R.l_idx);
+    td__arena_free(R.r_idx);
+done:
+    td_release(left);
+    td_release(right);
+    return result;
+}
+
+/* The CONST table under FILTER*( ) and the FILTER predicates on the way. */
+static td_t* const_table_under(td_graph_t* g, td_op_t* op, td_op_t** preds, int* npred) {
+    *npred = 0;
+    while (op->opcode == OP_FILTER) {
+        if (*npred >= 8) return NULL;
+        preds[(*npred)++] = op->inputs[1];
+        op = op->inputs[0];
+    }
+    if (op->opcode != OP_CONST) return NULL;
+    td_op_ext_t* e = td__find_ext(g, op->id);
+    return e && e->literal && e->literal->type == TD_TABLE ? e->literal : NULL;
+}
+
+static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* jop) {
+    td_graph_t* g = x->g;
+    td_op_ext_t* je = td__find_ext(g, jop->id);
+    if (!je || je->join.join_type != 0) return TD_ERR_PTR(TD_ERR_NYI);
+    td_op_t* preds[8];
+    int npred = 0, nrp = 0;
+    td_op_t* rpreds[8];
+    td_t* left = const_table_under(g, jop->inputs[0], preds, &npred);
+    td_t* right = const_table_under(g, jop->inputs[1], rpreds, &nrp);
+    if (!left || !right || nrp) return TD_ERR_PTR(TD_ERR_NYI);
+    (void)op;
+    int64_t nl = td_table_nrows(left), nr = td_table_nrows(right);
+    if (nl > INT32_MAX || nr > INT32_MAX) return TD_ERR_PTR(TD_ERR_NYI);
+    tdd_join_group_spec_t S;
+    memset(&S, 0, sizeof(S));
+    S.join.nl = nl;
+    S.join.nr = nr;
+    S.join.join_type = 0;
+    S.join.force_path = td__force_join_path();
+    int rc = join_keys(x, je, left, right, &S.join);
+    if (rc) return TD_ERR_PTR(rc);
+    /* Filters on the left input: predicates over the bound table (whose rows
+     * must be the left table's), all ANDed, plus any pending selection. */
+    tdd_pred_t P;
+    td_t* sel = g->selection;
+    if (sel) td_retain(sel);
+    if (npred) {
+        if (td_table_nrows(g->table) != nl) { td_release(sel); return TD_ERR_PTR(TD_ERR_LENGTH); }
+        for (int i = 0; i < npred; i++) {
+            if ((rc = compile_pred(x, g->table, preds[i], &P))) { td_release(sel); return TD_ERR_PTR(rc); }
+            if (npred == 1 && !sel) break;                  /* fused into the probe */
+            td_t* s2 = eval_sel(x, &P, nl, sel);
+            td_release(sel);
+            if (TD_IS_ERR(s2)) return s2;
+            sel = s2;
+        }
+    }
+    if (g->selection) { td_release(g->selection); g->selection = NULL; }
+    tdd_sel_t sv;
+    if (sel) {
+        if (sel->len != nl) { td_release(sel); return TD_ERR_PTR(TD_ERR_LENGTH); }
+        if ((rc = dev_sel(x, sel, &sv))) { td_release(sel); return TD_ERR_PTR(rc); }
+        S.join.lsel = &sv;
+    }
+    S.lpred = (npred == 1 && !sel) ? &P : NULL;
+    /* Group columns: left table first, then right non-key columns. */
+    S.n_keys = ext->n_keys;
+    S.n_aggs = ext->n_aggs;
+    td_t* key_src[TDD_MAX_KEYS];
+    td_t* agg_src[TDD_MAX_AGGS];
+    for (int i = 0; i < ext->n_keys + ext->n_aggs && !rc; i++) {
+        int is_key = i < ext->n_keys;
+        td_op_t* in = is_key ? ext->keys[i] : ext->agg_ins[i - ext->n_keys];
+        td_t* col = NULL;
+        int side = 0;
+        if (in->opcode == OP_SCAN) {
+            td_op_ext_t* se = td__find_ext(g, in->id);
+            col = td_table_get_col(left, se->sym);
+            if (!col && !join_is_right_key(g, je, se->sym)) { col = td_table_get_col(right, se->sym); side = 1; }
+            if (!col) rc = TD_ERR_SCHEMA;
+        } else if (is_key || !(in->opcode == OP_CONST && ext->agg_ops[i - ext->n_keys] == OP_COUNT)) {
+            rc = TD_ERR_NYI;
+        }
+        if (rc) break;
+        if (is_key) {
+            key_src[i] = col;
+            S.key_side[i] = (uint8_t)side;
+            rc = dev_col(x, col, &S.keys[i]);
+        } else {
+            int a = i - ext->n_keys;
+            agg_src[a] = col;
+            S.ops[a] = ext->agg_ops[a];
+            S.agg_side[a] = (uint8_t)side;
+            if (col) rc = dev_col(x, col, &S.aggs[a]);
+        }
+    }
+    if (rc) { td_release(sel); return TD_ERR_PTR(rc); }
+    tdd_group_result_t R;
+    rc = tdd_join_group(&S, &R);
+    td_release(sel);
+    if (rc) return TD_ERR_PTR(rc);
+    td__set_plan("join_group path=%d groups=%lld fused_pred=%d", R.path, (long long)R.n_groups,
+                 S.lpred != NULL);
+    return group_table(g, ext, &R, key_src, agg_src);
+}
+
+/* ----------------------------------------------------------- reductions -- */
+
+static td_t* exec_reduce(xctx_t* x, td_op_t* op) {
+    td_t* in = xnode(x, op->inputs[0]);
+    if (TD_IS_ERR(in)) return in;
+    td_t* r;
+    if (op->opcode == OP_COUNT) {
+        /* the reference counts a table's columns (exec.c:1734-1739) */
+        r = td_i64(in->type == TD_TABLE ? td_table_ncols(in) : in->len);
+        td_release(in);
+        return r;
+    }
+    if (in->type <= 0 || in->type == TD_TABLE) { td_release(in); return TD_ERR_PTR(TD_ERR_TYPE); }
+    /* One group keyed by a constant: a zero-filled U8 key column. */
+    int64_t n = in->len;
+    tdd_group_spec_t S;
+    memset(&S, 0, sizeof(S));
+    S.nrows = n;
+    S.n_keys = 1;
+    S.n_aggs = 1;
+    S.ops[0] = op->opcode;
+    int rc = dev_col(x, in, &S.aggs[0]);
+    void* zk = NULL;
+    if (!rc) {
+        zk = td__arena_alloc((size_t)(n > 0 ? n : 1));
+        rc = zk ? tdd_fill(zk, 0, (size_t)(n > 0 ? n : 1)) : TD_ERR_OOM;
+        if (zk && keep_tmp(x, zk)) rc = TD_ERR_OOM;
+    }
+    S.keys[0].data = zk;
+    S.keys[0].type = TD_U8;
+    S.keys[0].esz = 1;
+    S.force_path = 1;
+    tdd_group_result_t R;
+    if (!rc) rc = tdd_group(&S, &R);
+    int8_t it = in->type;
+    td_release(in);
+    if (rc) return TD_ERR_PTR(rc);
+    uint64_t v = 0;
+    int8_t ot = R.out_types[0];
+    if (R.n_groups > 0) {
+        uint8_t esz = ot == TD_F64 || ot == TD_I64 ? 8 : td_type_sizes[ot > 0 && ot < TD_TYPE_COUNT ? ot : TD_I64];
+        rc = tdd_d2h(&v, R.vals[0], esz);
+    }
+    td__arena_free(R.keys[0]);
+    td__arena_free(R.vals[0]);
+    if (rc) return TD_ERR_PTR(rc);
+    if (ot == TD_F64) {
+        double d;
+        memcpy(&d, &v, 8);
+        if (R.n_groups == 0) d = op->opcode == OP_SUM ? 0.0 : __builtin_nan("");
+        return td_f64(d);
+    }
+    int64_t i = (int64_t)v;
+    if (ot == TD_I32 || ot == TD_DATE || ot == TD_TIME) i = (int32_t)(uint32_t)v;
+    else if (ot == TD_I16) i = (int16_t)(uint16_t)v;
+    else if (ot == TD_U8 || ot == TD_BOOL) i = (uint8_t)v;
+    (void)it;
+    return td_i64(i);
+}
+
+/* First / last n rows of a vector, as a device copy. */
+static td_t* slice_vec(xctx_t* x, td_t* v, int64_t n, int tail) {
+    tdd_col_t c;
+    int rc = dev_col(x, v, &c);
+    if (rc) return TD_ERR_PTR(rc);
+    if (n > v->len) n = v->len;
+    if (n < 0) n = 0;
+    int64_t from = tail ? v->len - n : 0;
+    td_t* out = td__hbm_vec_new(v->type, v->attrs & TD_SYM_W_MASK, n, (size_t)n * c.esz);
+    if (TD_IS_ERR(out)) return out;
+    rc = tdd_d2d(td_hbm_ref(out)->dptr, (const char*)c.data + from * c.esz, (size_t)n * c.esz);
+    if (rc) { td_release(out); return TD_ERR_PTR(rc); }
+    return out;
+}
+
+/* HEAD / TAIL of a table or vector (exec.c:11919-12140 without the fused
+ * sort / group / filter variants). */
+static td_t* exec_head_tail(xctx_t* x, td_op_t* op) {
+    td_op_ext_t* e = td__find_ext(x->g, op->id);
+    int64_t n = e ? e->sym : 10;
+    td_t* in = xnode(x, op->inputs[0]);
+    if (TD_IS_ERR(in)) return in;
+    int tail = op->opcode == OP_TAIL;
+    td_t* out;
+    if (in->type == TD_TABLE) {
+        if (x->g->selection && td_table_nrows(in) == x->g->selection->len) {
+            td_t* c = compact_table(x, in, x->g->selection);
+            td_release(x->g->selection);
+            x->g->selection = NULL;
+            td_release(in);
+            if (TD_IS_ERR(c)) return c;
+            in = c;
+        }
+        int64_t nc = td_table_ncols(in);
+        out = td_table_new(nc);
+        for (int64_t c = 0; c < nc && !TD_IS_ERR(out); c++) {
+            td_t* s = slice_vec(x, td_table_get_col_idx(in, c), n, tail);
+            if (TD_IS_ERR(s)) { td_release(out); out = s; break; }
+            out = td_table_add_col(out, td_table_col_name(in, c), s);
+            td_release(s);
+        }
+    } else if (in->type > 0) {
+        out = slice_vec(x, in, n, tail);
+    } else {
+        out = TD_ERR_PTR(TD_ERR_TYPE);
+    }
+    td_release(in);
+    return out;
+}
+
+/* ---------------------------------------------------------- dispatcher -- */
+
+static td_t* xnode(xctx_t* x, td_op_t* op) {
+    td_graph_t* g = x->g;
+    if (!op) return TD_ERR_PTR(TD_ERR_NYI);
+    if (td__cancel_requested()) return TD_ERR_PTR(TD_ERR_CANCEL);
+    switch (op->opcode) {
+    case OP_SCAN: {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        if (!e) return TD_ERR_PTR(TD_ERR_NYI);
+        td_t* tbl = scan_table(g, e);
+        if (!tbl) return TD_ERR_PTR(TD_ERR_SCHEMA);
+        td_t* col = td_table_get_col(tbl, e->sym);
+        if (!col) return TD_ERR_PTR(TD_ERR_SCHEMA);
+        if (TD_IS_PARTED(col->type) || col->type == TD_MAPCOMMON) return TD_ERR_PTR(TD_ERR_NYI);
+        if (col->mmod != TD_MMOD_HBM) x->all_resident = 0;
+        td_retain(col);
+        return col;
+    }
+    case OP_CONST: {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        if (!e || !e->literal) return TD_ERR_PTR(TD_ERR_NYI);
+        td_retain(e->literal);
+        return e->literal;
+    }
+    case OP_FILTER: {
+        if (op->inputs[0]->opcode == OP_GROUP) return TD_ERR_PTR(TD_ERR_NYI);   /* HAVING */
+        td_t* input = xnode(x, op->inputs[0]);
+        if (TD_IS_ERR(input)) return input;
+        tdd_pred_t P;
+        int rc;
+        if (input->type == TD_TABLE) {
+            td_t* tbl = g->table;
+            if (!tbl) { td_release(input); return TD_ERR_PTR(TD_ERR_SCHEMA); }
+            if ((rc = compile_pred(x, tbl, op->inputs[1], &P))) { td_release(input); return TD_ERR_PTR(rc); }
+            td_t* s = eval_sel(x, &P, td_table_nrows(tbl), g->selection);
+            if (TD_IS_ERR(s)) { td_release(input); return s; }
+            if (g->selection) td_release(g->selection);
+            g->selection = s;
+            return input;
+        }
+        /* vector: eager compaction */
+        if ((rc = compile_pred(x, g->table, op->inputs[1], &P))) { td_release(input); return TD_ERR_PTR(rc); }
+        td_t* s = eval_sel(x, &P, input->len, NULL);
+        if (TD_IS_ERR(s)) { td_release(input); return s; }
+        td_t* out = compact_vec(x, input, s);
+        td_release(s);
+        td_release(input);
+        return out;
+    }
+    case OP_GROUP:
+        return exec_group(x, op);
+    case OP_JOIN:
+        return exec_join(x, op);
+    case OP_SUM: case OP_MIN: case OP_MAX: case OP_COUNT: case OP_AVG:
+    case OP_FIRST: case OP_LAST: case OP_STDDEV: case OP_STDDEV_POP:
+    case OP_VAR: case OP_VAR_POP:
+        return exec_reduce(x, op);
+    case OP_ALIAS: case OP_MATERIALIZE:
+        return xnode(x, op->inputs[0]);
+    case OP_HEAD: case OP_TAIL:
+        return exec_head_tail(x, op);
+    default:
+        return TD_ERR_PTR(TD_ERR_NYI);
+    }
+}
+
+td_t* td_execute(td_graph_t* g, td_op_t* root) {
+    if (!g || !root) return TD_ERR_PTR(TD_ERR_NYI);
+    if (!td__dev_ready()) return TD_ERR_PTR(TD_ERR_IO);     /* no GPU: fail loudly */
+    td__cancel_reset();
+    xctx_t x;
+    memset(&x, 0, sizeof(x));
+    x.g = g;
+    x.all_resident = 1;
+    td__set_plan("-");
+    td_t* r = xnode(&x, root);
+    if (!TD_IS_ERR(r) && g->selection && r->type == TD_TABLE) {
+        td_t* c = compact_table(&x, r, g->selection);      /* final compaction */
+        td_release(r);
+        td_release(g->selection);
+        g->selection = NULL;
+        r = c;
+    }
+    if (!TD_IS_ERR(r) && !x.all_resident) r = td__to_host(r);
+    if (!TD_IS_ERR(r) && tdd_sync() != TD_OK) { td_release(r); r = TD_ERR_PTR(TD_ERR_IO); }
+    drop_tmps(&x);
+    return r;
+}

@@ -1,0 +1,37 @@
+/*
+ * stubs.c -- td.h entry points outside the filter -> join -> group-by path
+ * (on-disk storage, CSV, graph relations).  They exist so that programs
+ * written against the reference link unchanged; each reports TD_ERR_NYI.
+ */
+#include "internal.h"
+
+td_err_t td_col_save(td_t* vec, const char* path) { (void)vec; (void)path; return TD_ERR_NYI; }
+td_t* td_col_load(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
+td_t* td_col_mmap(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
+td_err_t td_splay_save(td_t* tbl, const char* dir, const char* sym_path) {
+    (void)tbl; (void)dir; (void)sym_path; return TD_ERR_NYI;
+}
+td_t* td_splay_load(const char* dir) { (void)dir; return TD_ERR_PTR(TD_ERR_NYI); }
+td_t* td_read_splayed(const char* dir, const char* sym_path) {
+    (void)dir; (void)sym_path; return TD_ERR_PTR(TD_ERR_NYI);
+}
+td_t* td_part_load(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
+td_t* td_read_parted(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
+td_err_t td_meta_save_d(td_t* schema, const char* path) { (void)schema; (void)path; return TD_ERR_NYI; }
+td_t* td_meta_load_d(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
+td_t* td_read_csv(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
+td_t* td_read_csv_opts(const char* path, char d, bool h, const int8_t* t, int32_t n) {
+    (void)path; (void)d; (void)h; (void)t; (void)n; return TD_ERR_PTR(TD_ERR_NYI);
+}
+td_err_t td_write_csv(td_t* table, const char* path) { (void)table; (void)path; return TD_ERR_NYI; }
+
+td_rel_t* td_rel_build(td_t* t, const char* fk, int64_t n, bool s) {
+    (void)t; (void)fk; (void)n; (void)s; return NULL;
+}
+td_rel_t* td_rel_from_edges(td_t* t, const char* a, const char* b, int64_t n1, int64_t n2, bool s) {
+    (void)t; (void)a; (void)b; (void)n1; (void)n2; (void)s; return NULL;
+}
+td_err_t td_rel_save(td_rel_t* rel, const char* dir) { (void)rel; (void)dir; return TD_ERR_NYI; }
+td_rel_t* td_rel_load(const char* dir) { (void)dir; return NULL; }
+td_rel_t* td_rel_mmap(const char* dir) { (void)dir; return NULL; }
+void td_rel_free(td_rel_t* rel) { (void)rel; }

@@ -1,0 +1,242 @@
+"""GPU parity: equi-joins (direct-map and radix-partitioned paths) and the
+fused filter -> join -> group pipeline, against the oracle and the
+reference's golden tables.  Every call goes through the td.h C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import orc
+import workload_dags as D
+from parity import compare_groups, row_multiset_equal
+from tdtio import canonical_digest, read_tdt
+from teide_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+# (join path, radix bits): 0 = planner's choice, 1 = direct map, 2 = radix
+PATHS = [(0, 0), (1, 0), (2, 0), (2, 3), (2, 9)]
+
+
+@pytest.fixture
+def forced(eng):
+    def setp(path, bits=0, group=0):
+        eng.force_paths(path, group, bits)
+    yield setp
+    eng.force_paths(0, 0, 0)
+
+
+def run_join(eng, lcols, rcols, lkeys, rkeys, jt, resident=False):
+    lt, rt = eng.table(lcols), eng.table(rcols)
+    if resident:
+        lt2, rt2 = eng.put(lt), eng.put(rt)
+        eng.release(lt)
+        eng.release(rt)
+        lt, rt = lt2, rt2
+    g = eng.graph(lt)
+    root = g.join(g.const_table(lt), [g.scan(k) for k in lkeys], g.const_table(rt),
+                  [g.scan(k) for k in rkeys], jt)
+    out = g.execute(g.optimize(root))
+    assert bool(eng.L.td_dev_is_resident(out)) == resident
+    res = eng.read_table(out)
+    g.free()
+    eng.release(out)
+    eng.release(lt)
+    eng.release(rt)
+    return res
+
+
+def join_case(argv):
+    nl, nr, krange, jt, seed = (int(a) for a in argv[1:6])
+    lk = W.i64_uniform(nl, 0, krange - 1, seed, 601)
+    lv = W.i64_uniform(nl, -1000000, 1000000, seed, 602)
+    rk = W.i64_uniform(nr, 0, krange - 1, seed, 603)
+    rv = W.f64_uniform(nr, -1.0, 1.0, seed, 604)
+    return lk, lv, rk, rv, jt
+
+
+@pytest.mark.parametrize("case", ["join_inner_small", "join_left_small", "join_full_small",
+                                  "join_inner_radix", "join_full_radix"])
+@pytest.mark.parametrize("path", PATHS)
+def test_join_golden(eng, forced, case, path):
+    lk, lv, rk, rv, jt = join_case(MANIFEST[case]["argv"])
+    forced(*path)
+    got = run_join(eng, [("k", lk), ("lv", lv)], [("k", rk), ("rv", rv)], ["k"], ["k"], jt)
+    assert [c[0] for c in got] == ["k", "lv", "rv"]
+    arrays = [c[3] for c in got]
+    ent = MANIFEST[case]
+    if ent["outputs"]:
+        ref = read_tdt(os.path.join(GOLD, ent["outputs"][0]))
+        row_multiset_equal(arrays, [c[3] for c in ref], case)
+    else:
+        assert len(arrays[0]) == ent["rows_out"][0]
+        assert canonical_digest(arrays) == ent["digest"][0]
+
+
+def test_join_kats(eng, forced):
+    # test_exec.c /join (691-744): 4 rows
+    for path in PATHS:
+        forced(*path)
+        got = run_join(eng, [("id", np.array([1, 2, 3]))], [("id", np.array([1, 2, 2, 3])),
+                       ("s", np.array([10, 20, 21, 30]))], ["id"], ["id"], 0)
+        assert len(got[0][3]) == 4
+        assert sorted(got[1][3].tolist()) == [10, 20, 21, 30]
+    # /join_large (747-831): 200,000 rows, score sum 99,999,000,000
+    i = np.arange(100000)
+    for path in PATHS:
+        forced(*path)
+        got = run_join(eng, [("id", i % 50000)], [("id", i % 50000), ("score", i * 10)],
+                       ["id"], ["id"], 0)
+        assert len(got[0][3]) == 200000 and int(got[1][3].sum()) == 99999000000
+    # /join_empty (917-1025): 0 / 0 / 3 / 3 / 3
+    a, e = np.array([1, 2, 3]), np.zeros(0, np.int64)
+    for path in PATHS:
+        forced(*path)
+        n = [len(run_join(eng, [("id", x)], [("id", y), ("v", y * 2)], ["id"], ["id"], t)[0][3])
+             for x, y, t in [(a, e, 0), (e, a, 0), (a, e, 1), (a, e, 2), (e, a, 2)]]
+        assert n == [0, 0, 3, 3, 3]
+    # /join_left_large (1028-1089) and /join_full_large (1092-1151)
+    for path in PATHS:
+        forced(*path)
+        got = run_join(eng, [("id", i)], [("id", i * 2), ("v", i)], ["id"], ["id"], 1)
+        assert len(got[0][3]) == 100000
+        j = np.arange(80000)
+        got = run_join(eng, [("id", j)], [("id", j + 40000), ("v", j)], ["id"], ["id"], 2)
+        assert len(got[0][3]) == 120000
+    # /join_skewed (1154-1228): every score 294
+    forced(0)
+    got = run_join(eng, [("id", np.full(100000, 42))], [("id", i), ("score", i * 7)],
+                   ["id"], ["id"], 0)
+    assert len(got[0][3]) == 100000 and set(got[1][3].tolist()) == {294}
+    # /join_multikey (1299-1369): two keys, scores {1000, 2000}
+    got = run_join(eng, [("a", np.array([1, 1, 2, 2, 3])), ("b", np.array([10., 20., 10., 20., 10.]))],
+                   [("a", np.array([1, 2, 3])), ("b", np.array([10., 20., 30.])),
+                    ("score", np.array([1000, 2000, 3000]))], ["a", "b"], ["a", "b"], 0)
+    assert sorted(got[2][3].tolist()) == [1000, 2000]
+
+
+def oracle_join_table(lcols, rcols, lkeys, rkeys, jt):
+    li, ri = orc.join([dict(lcols)[k] for k in lkeys], [dict(rcols)[k] for k in rkeys], jt)
+    out = [orc.gather(a, li, jt == 2) for _, a in lcols]
+    out += [orc.gather(a, ri, jt >= 1) for n, a in rcols if n not in rkeys]
+    return out
+
+
+KEYGEN = {
+    "i64_dense": lambda r, nl, nr: (r.integers(0, 3000, nl), r.integers(0, 3000, nr)),
+    "i64_sparse": lambda r, nl, nr: (r.integers(-2 ** 40, 2 ** 40, nl) // 997 * 997,
+                                     r.integers(-2 ** 40, 2 ** 40, nr) // 997 * 997),
+    "i32": lambda r, nl, nr: (r.integers(-5000, 5000, nl).astype(np.int32),
+                              r.integers(-5000, 5000, nr).astype(np.int32)),
+    "f64": lambda r, nl, nr: (np.round(r.normal(size=nl), 2), np.round(r.normal(size=nr), 2)),
+}
+
+
+@pytest.mark.parametrize("kind", list(KEYGEN))
+@pytest.mark.parametrize("jt", [0, 1, 2])
+@pytest.mark.parametrize("path", [(0, 0), (2, 0), (2, 5)])
+def test_join_vs_oracle(eng, forced, kind, jt, path):
+    rng = np.random.default_rng(1000 + 10 * jt + list(KEYGEN).index(kind))
+    nl, nr = 120001, 40003
+    lk, rk = KEYGEN[kind](rng, nl, nr)
+    lcols = [("k", lk), ("lv", rng.integers(-10 ** 6, 10 ** 6, nl)),
+             ("lf", rng.normal(size=nl).astype(np.float64))]
+    rcols = [("k", rk), ("rv", rng.integers(0, 100, nr).astype(np.int32)),
+             ("rf", rng.random(nr))]
+    forced(*path)
+    got = run_join(eng, lcols, rcols, ["k"], ["k"], jt)
+    ref = oracle_join_table(lcols, rcols, ["k"], ["k"], jt)
+    assert [c[0] for c in got] == ["k", "lv", "lf", "rv", "rf"]
+    row_multiset_equal([c[3] for c in got], ref, f"{kind}/{jt}/{path}")
+
+
+def test_join_two_keys_and_resident(eng, forced):
+    rng = np.random.default_rng(77)
+    nl, nr = 50000, 20000
+    lcols = [("a", rng.integers(0, 50, nl)), ("b", rng.integers(0, 40, nl).astype(np.int32)),
+             ("v", rng.normal(size=nl))]
+    rcols = [("a", rng.integers(0, 50, nr)), ("b", rng.integers(0, 40, nr).astype(np.int32)),
+             ("w", rng.integers(0, 9, nr))]
+    ref = oracle_join_table(lcols, rcols, ["a", "b"], ["a", "b"], 0)
+    for path, resident in [((0, 0), False), ((2, 4), False), ((0, 0), True)]:
+        forced(*path)
+        got = run_join(eng, lcols, rcols, ["a", "b"], ["a", "b"], 0, resident)
+        row_multiset_equal([c[3] for c in got], ref, f"two keys {path} {resident}")
+
+
+# ------------------------------------------------------- join -> group --
+
+def check_jg(out, case):
+    ref = read_tdt(os.path.join(GOLD, case + ".tdt"))
+    assert [c[0] for c in out] == [c[0] for c in ref]
+    assert [c[1] for c in out] == [c[1] for c in ref]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]],
+                   [ref[0][3].astype(np.int64)], [c[3] for c in ref[1:]],
+                   [True, False, False], case)
+
+
+@pytest.mark.parametrize("case", ["jg_2m_250k", "jg_100k_8k_c1m"])
+@pytest.mark.parametrize("form", ["two_call", "fused"])
+@pytest.mark.parametrize("path", [(0, 0), (2, 0)])
+@pytest.mark.parametrize("resident", [False, True])
+def test_jg_golden(eng, forced, case, form, path, resident):
+    nf, nd, nc, seed = (int(a) for a in MANIFEST[case]["argv"][1:5])
+    tf, td, _, _ = D.jg_tables(eng, nf, nd, nc, seed)
+    if resident:
+        tf2, td2 = eng.put(tf), eng.put(td)
+        eng.release(tf)
+        eng.release(td)
+        tf, td = tf2, td2
+    forced(*path)
+    run = D.jg_fused if form == "fused" else D.jg_two_call
+    out = run(eng, tf, td)
+    assert bool(eng.L.td_dev_is_resident(out)) == resident
+    check_jg(eng.read_table(out), case)
+    eng.release(out)
+    eng.release(tf)
+    eng.release(td)
+
+
+@pytest.mark.parametrize("variant", ["dup_dim", "agg_on_dim", "key_on_fact", "left_join"])
+def test_join_group_general(eng, variant):
+    """Fused JOIN -> GROUP shapes off the one-index fast path: duplicate
+    build keys, aggregates over build columns, keys on the probe side and a
+    left join (unmatched rows carry zero-filled build columns)."""
+    rng = np.random.default_rng(5)
+    nf, nd = 300000, 20000
+    fk = rng.integers(0, nd + 500, nf)
+    qty = rng.integers(1, 101, nf)
+    price = rng.lognormal(3, 1, nf)
+    pk = rng.permutation(nd)
+    if variant == "dup_dim":
+        pk = pk % (nd // 2)
+    cat = rng.integers(0, 77, nd).astype(np.int32)
+    wgt = rng.normal(size=nd)
+    tf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
+    td = eng.table([("pk", pk), ("cat", cat), ("wgt", wgt)])
+    jt = 1 if variant == "left_join" else 0
+    if variant == "key_on_fact":
+        keys, aggs = ["qty"], [("SUM", "price"), ("MAX", "cat"), ("COUNT", "price")]
+    elif variant == "agg_on_dim":
+        keys, aggs = ["cat"], [("SUM", "wgt"), ("MIN", "qty"), ("AVG", "price")]
+    else:
+        keys, aggs = ["cat"], [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")]
+    g = eng.graph(tf)
+    flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
+    g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], jt)
+    root = g.group([g.scan(k) for k in keys], [(o, g.scan(c)) for o, c in aggs])
+    out = eng.read_table(g.execute(root))
+    g.free()
+    # oracle composition
+    m = qty > 50
+    f = {"fk": fk[m], "qty": qty[m], "price": price[m]}
+    li, ri = orc.join([f["fk"]], [pk], jt)
+    cols = {c: orc.gather(a, li) for c, a in f.items()}
+    cols.update({"cat": orc.gather(cat, ri, True), "wgt": orc.gather(wgt, ri, True)})
+    ref = orc.group([cols[k] for k in keys], [o for o, _ in aggs], [cols[c] for _, c in aggs])
+    f64 = [ref["out_types"][i] == 7 and o in ("SUM", "AVG") for i, (o, _) in enumerate(aggs)]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], ref["keys"],
+                   ref["vals"], f64, variant, exact=ref["comp"], sum_abs=ref["sum_abs"])

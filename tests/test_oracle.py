@@ -1,0 +1,277 @@
+"""The oracle (oracle/orc.c) pinned against the reference.
+
+Each check below would fail for a plausible mistake in the restatement:
+golden result tables written by the reference build (tests/golden), the
+known answers of the reference's own C tests (test/test_graph.c,
+test_exec.c, test_pool.c, test_sel.c), and brute-force numpy.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from parity import assert_f64_close, compare_groups, row_multiset_equal
+from tdtio import canonical_digest, read_tdt
+from teide_amd import workloads as W
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def gold(name):
+    return read_tdt(os.path.join(GOLD, name))
+
+
+# ------------------------------------------------------------- selection --
+
+def test_sel_kats():
+    # test_sel.c /from_pred = 3, /and = 2 (test_sel.c:41-92)
+    s = orc.sel_from_bool(np.array([1, 0, 1, 0, 1], np.uint8))
+    assert orc.sel_parts(s, 5)[0] == 3
+    a = orc.sel_from_bool(np.array([1, 1, 0, 0, 1], np.uint8))
+    b = orc.sel_from_bool(np.array([1, 0, 1, 0, 1], np.uint8))
+    assert orc.sel_parts(orc.sel_and(a, b, 5), 5)[0] == 2
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1023, 1024, 1025, 100003])
+def test_sel_layout_brute_force(n):
+    rng = np.random.default_rng(n)
+    pred = (rng.random(n) < 0.37).astype(np.uint8)
+    if n > 3000:
+        pred[1024:2048] = 1          # an ALL segment
+        pred[2048:3072] = 0          # a NONE segment
+    s = orc.sel_from_bool(pred)
+    total, flags, pop, bits = orc.sel_parts(s, n)
+    assert total == int(pred.sum())
+    unpacked = np.unpackbits(bits.view(np.uint8), bitorder="little")[:n]
+    np.testing.assert_array_equal(unpacked, pred)
+    for sgi in range((n + 1023) // 1024):
+        seg = pred[sgi * 1024:(sgi + 1) * 1024]
+        assert pop[sgi] == seg.sum()
+        assert flags[sgi] == (0 if seg.sum() == 0 else 1 if seg.all() else 2)
+    np.testing.assert_array_equal(orc.sel_to_idx(s, n), np.flatnonzero(pred))
+
+
+def test_cmp_typing():
+    # mixed integer / f64 comparisons run in f64, pure integer ones in int64
+    big = np.array([2 ** 53 + 1, 5], np.int64)
+    assert orc.cmp_const(big, "EQ", 2 ** 53).tolist() == [0, 0]
+    assert orc.cmp_const(big, "EQ", float(2 ** 53)).tolist() == [1, 0]
+    x = np.array([10, 20, 30, 40, 50], np.int64)
+    assert int(orc.cmp_const(x, "GT", 25).sum()) == 3          # test_sel.c:101-112
+    v = np.array([1.5, np.nan, -0.0], np.float64)
+    assert orc.cmp_const(v, "EQ", 0.0).tolist() == [0, 0, 1]
+
+
+# ---------------------------------------------------------------- group --
+
+def fg_oracle(d):
+    pred = orc.cmp_const(d["flag"], "EQ", 0)
+    idx = orc.sel_to_idx(orc.sel_from_bool(pred), len(pred))
+    region = orc.gather(d["region"], idx)
+    amount = orc.gather(d["amount"], idx)
+    return orc.group([region], ["SUM", "COUNT"], [amount, amount])
+
+
+@pytest.mark.parametrize("fixture,n,seed", [("fg_1m.tdt", 1000000, 42),
+                                            ("fg_100k_s7.tdt", 100000, 7)])
+def test_fg_matches_reference(fixture, n, seed):
+    ref = gold(fixture)
+    assert [c[0] for c in ref] == ["region", "amount_sum", "amount_count"]
+    r = fg_oracle(W.filter_group(n, seed))
+    assert r["order"] == 1
+    # the reference's direct-array path emits groups in ascending key order
+    np.testing.assert_array_equal(r["keys"][0], ref[0][3])
+    np.testing.assert_array_equal(r["vals"][1], ref[2][3])
+    assert_f64_close(r["vals"][0], ref[1][3], r["comp"][0], r["sum_abs"][0], "amount_sum")
+
+
+def jg_oracle(fact, dim, thr=50):
+    pred = orc.cmp_const(fact["qty"], "GT", thr)
+    idx = orc.sel_to_idx(orc.sel_from_bool(pred), len(pred))
+    fk, qty, price = (orc.gather(fact[c], idx) for c in ("fk", "qty", "price"))
+    li, ri = orc.join([fk], [dim["pk"]], 0)
+    cat = orc.gather(dim["cat"], ri)
+    p, q = orc.gather(price, li), orc.gather(qty, li)
+    return orc.group([cat], ["SUM", "SUM", "COUNT"], [p, q, p])
+
+
+@pytest.mark.parametrize("case", ["jg_2m_250k", "jg_100k_8k_c1m"])
+def test_jg_matches_reference(case):
+    argv = MANIFEST[case]["argv"]
+    nf, nd, nc, seed = (int(a) for a in argv[1:5])
+    ref = gold(case + ".tdt")
+    assert [c[0] for c in ref] == ["cat", "price_sum", "qty_sum", "price_count"]
+    r = jg_oracle(W.join_fact(nf, nd, seed), W.join_dim(nd, nc, seed))
+    compare_groups(r["keys"], r["vals"], [ref[0][3]], [c[3] for c in ref[1:]],
+                   [True, False, False], case, exact=[r["comp"][0], None, None],
+                   sum_abs=[r["sum_abs"][0], None, None])
+
+
+def test_jg_dims_are_permutations():
+    d = W.join_dim(250000, 1000, 42)
+    np.testing.assert_array_equal(np.sort(d["pk"]), np.arange(250000))
+    f = W.join_fact(100000, 250000, 42)
+    assert f["fk"].min() >= 0 and f["fk"].max() < 250000
+    assert f["qty"].min() >= 1 and f["qty"].max() <= 100
+
+
+HC_OPS = ["SUM", "MIN", "MAX", "SUM", "MIN", "MAX", "COUNT"]
+
+
+@pytest.mark.parametrize("by", ["key", "sym"])
+def test_hc_matches_reference(by):
+    d = W.high_card(60000, 4096, 2048, 1.1, 42)
+    ref = gold(f"hc_60k_{by}.tdt")
+    assert [c[0] for c in ref] == [by, "v_i64_sum", "v_i64_min", "v_i64_max", "v_f64_sum",
+                                   "v_f64_min", "v_f64_max", "v_i64_count"]
+    kt = [(20, 2)] if by == "sym" else None
+    aggs = [d["v_i64"]] * 3 + [d["v_f64"]] * 3 + [d["v_i64"]]
+    r = orc.group([d[by]], HC_OPS, aggs, key_types=kt)
+    f64 = [False, False, False, True, False, False, False]
+    compare_groups(r["keys"], r["vals"], [ref[0][3]], [c[3] for c in ref[1:]], f64, by,
+                   exact=r["comp"], sum_abs=r["sum_abs"])
+
+
+def test_group_kats():
+    # test_graph.c /group_sum: ids {1,2,3} -> {200, 150, 200} (test_graph.c:154-205)
+    id1 = np.array([1, 1, 2, 2, 3, 3, 1, 2, 3, 1], np.int64)
+    v1 = np.array([10, 20, 30, 40, 50, 60, 70, 80, 90, 100], np.int64)
+    r = orc.group([id1], ["SUM"], [v1])
+    assert dict(zip(r["keys"][0].tolist(), r["vals"][0].tolist())) == {1: 200, 2: 150, 3: 200}
+    # test_pool.c /parallel_group_sum (test_pool.c:131-216)
+    ids = np.arange(100000, dtype=np.int64) % 4
+    r = orc.group([ids], ["SUM"], [ids + 1])
+    assert r["vals"][0].tolist() == [25000, 50000, 75000, 100000]
+    # test_pool.c /parallel_min_max: -50000.0 / 49999.0
+    x = np.arange(100000, dtype=np.float64) - 50000
+    r = orc.group([np.zeros(100000, np.int64)], ["MIN", "MAX"], [x, x])
+    assert r["vals"][0][0] == -50000.0 and r["vals"][1][0] == 49999.0
+
+
+def test_group_brute_force():
+    rng = np.random.default_rng(3)
+    n = 20000
+    k1 = rng.integers(-5, 5, n)
+    k2 = rng.integers(0, 3, n).astype(np.int32)
+    v = rng.integers(-10 ** 12, 10 ** 12, n)
+    f = rng.normal(size=n)
+    mask_b = rng.random(n) < 0.6
+    bits = np.packbits(mask_b, bitorder="little")
+    bits = np.pad(bits, (0, (-len(bits)) % 8)).view(np.uint64)
+    r = orc.group([k1, k2], ["SUM", "MIN", "MAX", "AVG", "FIRST", "LAST", "VAR", "COUNT"],
+                  [v, v, f, f, v, f, f, None], mask_bits=bits)
+    assert r["order"] == 1
+    m = np.flatnonzero(mask_b)
+    keys = sorted(set(zip(k1[m].tolist(), k2[m].tolist())))
+    assert list(zip(r["keys"][0].tolist(), r["keys"][1].tolist())) == keys
+    for gi, (a, b) in enumerate(keys):
+        rows = m[(k1[m] == a) & (k2[m] == b)]
+        assert r["vals"][0][gi] == int(np.sum(v[rows]))
+        assert r["vals"][1][gi] == v[rows].min()
+        assert r["vals"][2][gi] == f[rows].max()
+        assert abs(r["vals"][3][gi] - f[rows].mean()) <= 1e-12 * max(1, abs(f[rows]).sum())
+        assert r["vals"][4][gi] == v[rows[0]] and r["vals"][5][gi] == f[rows[-1]]
+        assert abs(r["vals"][6][gi] - np.var(f[rows], ddof=1)) < 1e-9
+        assert r["vals"][7][gi] == len(rows)
+
+
+# ----------------------------------------------------------------- joins --
+
+def join_case(argv):
+    nl, nr, krange, jt, seed = (int(a) for a in argv[1:6])
+    lk = W.i64_uniform(nl, 0, krange - 1, seed, 601)
+    lv = W.i64_uniform(nl, -1000000, 1000000, seed, 602)
+    rk = W.i64_uniform(nr, 0, krange - 1, seed, 603)
+    rv = W.f64_uniform(nr, -1.0, 1.0, seed, 604)
+    return lk, lv, rk, rv, jt
+
+
+def join_oracle_table(lk, lv, rk, rv, jt):
+    li, ri = orc.join([lk], [rk], jt)
+    return [orc.gather(lk, li, jt == 2), orc.gather(lv, li, jt == 2),
+            orc.gather(rv, ri, jt >= 1)]
+
+
+@pytest.mark.parametrize("case", ["join_inner_small", "join_left_small", "join_full_small",
+                                  "join_inner_radix", "join_full_radix"])
+def test_join_matches_reference(case):
+    ent = MANIFEST[case]
+    got = join_oracle_table(*join_case(ent["argv"]))
+    if ent["outputs"]:
+        ref = gold(ent["outputs"][0])
+        assert [c[0] for c in ref] == ["k", "lv", "rv"]
+        row_multiset_equal(got, [c[3] for c in ref], case)
+    else:
+        assert len(got[0]) == ent["rows_out"][0]
+        assert canonical_digest(got) == ent["digest"][0]
+
+
+def test_join_kats():
+    # test_exec.c /join: 4 rows (691-744)
+    li, ri = orc.join([np.array([1, 2, 3])], [np.array([1, 2, 2, 3])], 0)
+    assert len(li) == 4
+    # /join_large: 200,000 rows, score sum 99,999,000,000 (747-831)
+    i = np.arange(100000)
+    li, ri = orc.join([i % 50000], [i % 50000], 0)
+    assert len(li) == 200000 and int((ri * 10).sum()) == 99999000000
+    # /join_fallback: val == id * 7 (834-914)
+    li, ri = orc.join([np.arange(100)], [i % 100], 0)
+    assert len(li) == 100000
+    # /join_empty (917-1025): 0 / 0 / 3 / 3 / 3
+    a, e = np.array([1, 2, 3]), np.zeros(0, np.int64)
+    assert [len(orc.join([x], [y], t)[0]) for x, y, t in
+            [(a, e, 0), (e, a, 0), (a, e, 1), (a, e, 2), (e, a, 2)]] == [0, 0, 3, 3, 3]
+    # /join_left_large 100K, /join_full_large 120K (1028-1151)
+    assert len(orc.join([i], [i * 2], 1)[0]) == 100000
+    j = np.arange(80000)
+    assert len(orc.join([j], [j + 40000], 2)[0]) == 120000
+    # /join_skewed: every score 294 (1154-1228)
+    li, ri = orc.join([np.full(100000, 42)], [i], 0)
+    assert len(li) == 100000 and set((i * 7)[ri].tolist()) == {294}
+    # /join_boundary: 65,537 unique ids (1231-1296)
+    b = np.arange(65537)
+    li, ri = orc.join([b], [b], 0)
+    assert len(li) == 65537 and len(set(b[li].tolist())) == 65537
+    # /join_multikey: scores {1000, 2000} (1299-1369)
+    li, ri = orc.join([np.array([1, 1, 2, 2, 3]), np.array([10., 20., 10., 20., 10.])],
+                      [np.array([1, 2, 3]), np.array([10., 20., 30.])], 0)
+    assert sorted(np.array([1000, 2000, 3000])[ri].tolist()) == [1000, 2000]
+
+
+def test_join_brute_force():
+    rng = np.random.default_rng(5)
+    lk = rng.integers(0, 50, 700)
+    rk = rng.integers(0, 60, 500)
+    for jt in (0, 1, 2):
+        li, ri = orc.join([lk], [rk], jt)
+        want = [(l, r) for l in range(700) for r in range(500) if lk[l] == rk[r]]
+        if jt >= 1:
+            want += [(l, -1) for l in range(700) if not (rk == lk[l]).any()]
+        if jt == 2:
+            want += [(-1, r) for r in range(500) if not (lk == rk[r]).any()]
+        assert sorted(zip(li.tolist(), ri.tolist())) == sorted(want)
+
+
+# ------------------------------------------------------------ generators --
+
+def test_generators_deterministic_across_threads():
+    a = W.join_fact(3 * (1 << 20) + 17, 1000, seed=9)
+    W.NTHREADS, keep = 1, W.NTHREADS
+    try:
+        b = W.join_fact(3 * (1 << 20) + 17, 1000, seed=9)
+    finally:
+        W.NTHREADS = keep
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_zipf_head_frequency():
+    n, nv, s = 2000000, 1 << 20, 1.1
+    d = W.high_card(n, nv, 16, s, 1)
+    top = W.lib().tdg_zipf_key(1)
+    h = np.sum(1.0 / np.arange(1, nv + 1) ** s)
+    frac = np.mean(d["key"] == top)
+    assert abs(frac - 1.0 / h) < 0.01
