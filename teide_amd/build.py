@@ -24,10 +24,13 @@ OBJ = os.path.join(REPO, "_build", "obj")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("TEIDE_OFFLOAD_ARCH", "gfx950")
 
-C_FLAGS = ["-std=c17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+C_FLAGS = ["-std=c17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter",
            "-I" + os.path.join(REPO, "include"), "-I" + os.path.join(CSRC, "host")]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
              # packed-FP32 instructions off for all device code
+             # no a*b+c -> fma contraction: aggregate finalisation (variance)
+             # must round like the reference's separate multiply and add
+             "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument",
              "-I" + os.path.join(REPO, "include")]

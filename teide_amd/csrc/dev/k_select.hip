@@ -82,16 +82,19 @@ __global__ __launch_bounds__(SEL_T) void k_sel_words(const uint64_t* a, const ui
 }
 
 /* meta: total_pass (i64) then n_segs (u32) and padding; the total is
- * accumulated by atomics so it starts from zero. */
-__global__ void k_sel_meta(uint64_t* meta, uint32_t n_segs) {
-    if (threadIdx.x == 0) {
-        meta[0] = 0;
-        meta[1] = (uint64_t)n_segs;
-    }
+ * accumulated by atomics so it starts from zero.  The flag and popcount
+ * arrays' padding is zeroed too, as td_sel_new zeroes the whole payload
+ * (sel.c:52-69); the arrays themselves are overwritten by the caller. */
+__global__ void k_sel_meta(uint64_t* meta, uint32_t n_segs, int64_t head_words) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= head_words) return;
+    meta[i] = i == 1 ? (uint64_t)n_segs : 0;
 }
 
 static int write_total_start(const tdd_sel_t* s) {
-    LAUNCH("sel_meta", k_sel_meta, 1, 64, 0, (uint64_t*)s->base, s->n_segs);
+    int64_t head_words = ((char*)s->bits - (char*)s->base) / 8;
+    LAUNCH("sel_meta", k_sel_meta, (int)ceil_div(head_words, 256), 256, 0, (uint64_t*)s->base,
+           s->n_segs, head_words);
     return TD_OK;
 }
 

@@ -460,6 +460,7 @@ static td_op_t* strip_filter(td_op_t* op) {
 }
 
 static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* join);
+static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel);
 
 static td_t* exec_group(xctx_t* x, td_op_t* op) {
     td_graph_t* g = x->g;
@@ -481,7 +482,13 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
             if (g->nodes[i].opcode == OP_JOIN) return exec_join_group(x, op, ext, &g->nodes[i]);
         return TD_ERR_PTR(TD_ERR_SCHEMA);
     }
+    return group_over(x, ext, tbl, 1);
+}
 
+/* GROUP of `tbl`'s columns; with use_sel, the graph's lazy selection (or the
+ * README shape's shared predicate) masks the rows and is consumed. */
+static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel) {
+    td_graph_t* g = x->g;
     int64_t nrows = td_table_nrows(tbl);
     tdd_group_spec_t S;
     memset(&S, 0, sizeof(S));
@@ -517,7 +524,9 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
     /* Mask: the README shape's shared predicate, else the lazy selection. */
     td_t* sel = NULL;
     td_op_t* rp = readme_pred(ext);
-    if (rp) {
+    if (!use_sel) {
+        if (rp) return TD_ERR_PTR(TD_ERR_NYI);
+    } else if (rp) {
         tdd_pred_t P;
         if ((rc = compile_pred(x, tbl, rp, &P))) return TD_ERR_PTR(rc);
         sel = eval_sel(x, &P, nrows, g->selection);
@@ -527,7 +536,7 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
         sel = g->selection;
         td_retain(sel);
     }
-    if (g->selection) { td_release(g->selection); g->selection = NULL; }
+    if (use_sel && g->selection) { td_release(g->selection); g->selection = NULL; }
     tdd_sel_t mv;
     if (sel && (rc = dev_sel(x, sel, &mv))) { td_release(sel); return TD_ERR_PTR(rc); }
     S.mask = sel ? &mv : NULL;
@@ -662,19 +671,30 @@ static td_t* const_table_under(td_graph_t* g, td_op_t* op, td_op_t** preds, int*
     return e && e->literal && e->literal->type == TD_TABLE ? e->literal : NULL;
 }
 
+/* Shapes the fused kernel does not take: materialise the join, then group
+ * its output table. */
+static td_t* join_then_group(xctx_t* x, td_op_ext_t* ext, td_op_t* jop) {
+    td_t* jt = xnode(x, jop);
+    if (TD_IS_ERR(jt)) return jt;
+    td_t* r = group_over(x, ext, jt, 0);
+    td_release(jt);
+    return r;
+}
+
 static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* jop) {
     td_graph_t* g = x->g;
     td_op_ext_t* je = td__find_ext(g, jop->id);
-    if (!je || je->join.join_type != 0) return TD_ERR_PTR(TD_ERR_NYI);
+    if (!je) return TD_ERR_PTR(TD_ERR_NYI);
     td_op_t* preds[8];
     int npred = 0, nrp = 0;
     td_op_t* rpreds[8];
     td_t* left = const_table_under(g, jop->inputs[0], preds, &npred);
     td_t* right = const_table_under(g, jop->inputs[1], rpreds, &nrp);
-    if (!left || !right || nrp) return TD_ERR_PTR(TD_ERR_NYI);
     (void)op;
+    if (je->join.join_type != 0 || !left || !right || nrp ||
+        td_table_nrows(left) > INT32_MAX || td_table_nrows(right) > INT32_MAX)
+        return join_then_group(x, ext, jop);
     int64_t nl = td_table_nrows(left), nr = td_table_nrows(right);
-    if (nl > INT32_MAX || nr > INT32_MAX) return TD_ERR_PTR(TD_ERR_NYI);
     tdd_join_group_spec_t S;
     memset(&S, 0, sizeof(S));
     S.join.nl = nl;

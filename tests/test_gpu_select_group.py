@@ -10,7 +10,7 @@ import pytest
 
 import orc
 import workload_dags as D
-from parity import assert_f64_close, compare_groups
+from parity import assert_f64_close, assert_var_close, compare_groups, key_order
 from tdtio import read_tdt
 from teide_amd import workloads as W
 from teide_amd.td import TD_SEL
@@ -146,6 +146,27 @@ def key64(a):
     return a.view(np.int64) if a.dtype == np.float64 else a.astype(np.int64)
 
 
+VAR_OPS = {"VAR": (False, False), "VAR_POP": (False, True), "STDDEV": (True, False),
+           "STDDEV_POP": (True, True)}       # op -> (compared as square, population)
+
+
+def var_scales(keys, x, mask, ref_keys, pop):
+    """Per group (in ref_keys order) sum(x^2) / (n or n-1): the magnitude the
+    variance formula cancels at."""
+    K = np.stack([key64(k) for k in keys], 1)
+    x = np.asarray(x, np.float64)
+    if mask is not None:
+        K, x = K[mask], x[mask]
+    uk, inv = np.unique(K, axis=0, return_inverse=True)
+    inv = inv.ravel()
+    sq = np.bincount(inv, weights=x * x, minlength=len(uk))
+    cnt = np.bincount(inv, minlength=len(uk)).astype(np.float64)
+    denom = cnt if pop else np.maximum(cnt - 1, 1)
+    pos = {r: i for i, r in enumerate(map(tuple, uk.tolist()))}
+    idx = np.array([pos[r] for r in map(tuple, np.stack(ref_keys, 1).tolist())], np.int64)
+    return sq[idx] / denom[idx]
+
+
 AGG_SETS = [
     (["SUM", "COUNT", "AVG"], ["vi", "vi", "vf"]),
     (["MIN", "MAX", "MIN", "MAX"], ["vi", "vi", "vf", "vf"]),
@@ -177,7 +198,7 @@ def test_group_vs_oracle(eng, kset, aggi, masked):
     t = eng.table(cols)
     g = eng.graph(t)
     ops, ins = AGG_SETS[aggi]
-    mask = None
+    mask = mb = None
     if masked:
         mb = rng.random(n) < 0.3
         sel = eng.check(eng.L.td_sel_from_pred(eng.vec(mb.astype(np.uint8), 1)))
@@ -193,12 +214,19 @@ def test_group_vs_oracle(eng, kset, aggi, masked):
     kt = [(20, 2)] if kset == "sym_w32" else None
     ref = orc.group(keys, ops, [src[c] for c in ins], mask_bits=mask, key_types=kt)
     nk = len(keys)
-    f64 = [ref["out_types"][a] == 7 and ops[a] in ("SUM", "AVG", "VAR", "STDDEV_POP")
-           for a in range(len(ops))]
+    f64 = ["skip" if ops[a] in VAR_OPS else
+           ref["out_types"][a] == 7 and ops[a] in ("SUM", "AVG") for a in range(len(ops))]
     assert [c[1] for c in out[nk:]] == ref["out_types"]
-    compare_groups([key64(c[3]) for c in out[:nk]], [c[3] for c in out[nk:]], ref["keys"],
-                   ref["vals"],
-                   f64, f"{kset}/{aggi}/{masked}", exact=ref["comp"], sum_abs=ref["sum_abs"])
+    gk = [key64(c[3]) for c in out[:nk]]
+    compare_groups(gk, [c[3] for c in out[nk:]], ref["keys"], ref["vals"], f64,
+                   f"{kset}/{aggi}/{masked}", exact=ref["comp"], sum_abs=ref["sum_abs"])
+    go, ro = key_order(gk), key_order(ref["keys"])
+    for a, op in enumerate(ops):
+        if op in VAR_OPS:
+            is_sqrt, pop = VAR_OPS[op]
+            scale = var_scales(keys, src[ins[a]], mb if masked else None, ref["keys"], pop)
+            assert_var_close(out[nk + a][3][go], ref["vals"][a][ro], scale[ro], is_sqrt,
+                             f"{kset}/{aggi}/{masked}: {op}")
     if ref["order"] == 1 and kset != "f64":
         # direct-array eligible in the reference: same ascending key order
         np.testing.assert_array_equal(key64(out[0][3]), ref["keys"][0])
