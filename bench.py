@@ -99,6 +99,29 @@ def load_traffic(name):
     return ent.get("bytes_per_launch") if ent else None
 
 
+def calibrate_copy(e, nbytes):
+    """One device-to-device copy of nbytes through the library's copy kernel:
+    reads nbytes and writes nbytes of HBM, the reference point for scaling
+    FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    L = e.L
+    L.tdd_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.tdd_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.tdd_fill.argtypes = [C.c_void_p, C.c_uint64, C.c_size_t]
+    L.tdd_free.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if L.tdd_malloc(C.byref(src), nbytes) or L.tdd_malloc(C.byref(dst), nbytes):
+        raise SystemExit("calibrate-copy: device allocation failed")
+    L.tdd_fill(src, 0x0123456789abcdef, nbytes)
+    L.tdd_fill(dst, 0, nbytes)
+    L.td_dev_sync()
+    L.tdd_d2d(dst, src, nbytes)
+    L.td_dev_sync()
+    L.tdd_free(src)
+    L.tdd_free(dst)
+    print(f"# calibrate-copy: {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +133,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true",
                     help="no per-kernel HIP events in the timed steps")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device copy of MIB MiB (the known-size kernel that "
+                         "scales rocprof byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -121,6 +147,9 @@ def main():
     e = engine()
     if e.L.td_dev_available() != 1:
         raise SystemExit("bench.py: no usable HIP device")
+
+    if a.calibrate_copy:
+        calibrate_copy(e, a.calibrate_copy << 20)
 
     t0 = time.perf_counter()
     fact = W.join_fact(a.fact, a.dim, SEED)

@@ -8,6 +8,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string.h>
+#include <stdlib.h>
 
 #include <teide/td.h>
 #include <teide/tdd.h>
@@ -134,6 +135,31 @@ static inline int stream_grid(int64_t units, int per_cu = 8) {
     if (units < 1) return 1;
     return (int)(units < cap ? units : cap);
 }
+
+/* Workgroups of `kern` that are resident on the whole chip at once, capped
+ * by the work available: the grid of a grid-stride kernel. */
+template <class K>
+static inline int resident_grid(K kern, int block, size_t shmem, int64_t units) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, shmem) != hipSuccess ||
+        per < 1)
+        per = 1;
+    int64_t cap = (int64_t)tdd::g_num_cu * per;
+    if (units < 1) return 1;
+    return (int)(units < cap ? units : cap);
+}
+
+/* Integer launch parameter from the environment (read once per call site),
+ * else the default: used for sweeps of tile shapes. */
+#define TUNE_INT(name, def)                                           \
+    ([]() {                                                           \
+        static int v__ = -1;                                          \
+        if (v__ < 0) {                                                \
+            const char* e__ = getenv(name);                           \
+            v__ = e__ ? atoi(e__) : (def);                            \
+        }                                                             \
+        return v__;                                                   \
+    }())
 
 /* ----------------------------------------------------- host-side scans -- */
 

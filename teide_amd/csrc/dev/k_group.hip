@@ -15,6 +15,7 @@
  * a tenth of all rows) and is merged into the global table at the end.
  */
 #include "group_plan.h"
+#include "tile.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -104,18 +105,30 @@ __device__ __forceinline__ uint64_t word_mask(const RowMask& M, int64_t word) {
 
 /* ---------------------------------------------------------------- minmax -- */
 
-__global__ __launch_bounds__(256) void k_minmax(tdd_col_t col, RowMask M, long long* out) {
-    int64_t n_words = (M.nrows + 63) >> 6;
-    int lane = lane_id();
+#define MM_R 8
+#define MM_T 256
+
+/* Per-workgroup min / max (no atomics: same-address atomics from thousands
+ * of waves serialise at the memory side), folded by k_minmax_final. */
+__global__ __launch_bounds__(MM_T) void k_minmax(tdd_col_t col, int vec, RowMask M,
+                                                 long long* part) {
+    __shared__ long long smn[MM_T / WAVE], smx[MM_T / WAVE];
+    const int lane = lane_id();
     long long mn = 0x7fffffffffffffffLL, mx = (long long)0x8000000000000000ull;
-    int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    int64_t n_waves = (int64_t)gridDim.x * blockDim.x / WAVE;
-    for (int64_t w = wave_id; w < n_words; w += n_waves) {
-        uint64_t m = word_mask(M, w);
-        if ((m >> lane) & 1) {
-            long long v = ld_i64(col, w * 64 + lane);
-            mn = v < mn ? v : mn;
-            mx = v > mx ? v : mx;
+    const int64_t n_tiles = (M.nrows + 64 * MM_R - 1) / (64 * MM_R);
+    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / WAVE;
+    for (int64_t t = wave_id; t < n_tiles; t += n_waves) {
+        const int64_t base = t * 64 * MM_R + (int64_t)lane * MM_R;
+        int64_t v[MM_R];
+        load_rows<MM_R>(col, base, M.nrows, vec != 0, v);
+        uint32_t m = tile_live<MM_R>(M.bits, base, M.nrows);
+#pragma unroll
+        for (int j = 0; j < MM_R; j++) {
+            if ((m >> j) & 1) {
+                mn = v[j] < mn ? v[j] : mn;
+                mx = v[j] > mx ? v[j] : mx;
+            }
         }
     }
     for (int d = WAVE / 2; d >= 1; d >>= 1) {
@@ -123,14 +136,39 @@ __global__ __launch_bounds__(256) void k_minmax(tdd_col_t col, RowMask M, long l
         mn = a < mn ? a : mn;
         mx = b > mx ? b : mx;
     }
-    if (lane == 0) {
-        atomicMin(&out[0], mn);
-        atomicMax(&out[1], mx);
+    if (lane == 0) { smn[threadIdx.x / WAVE] = mn; smx[threadIdx.x / WAVE] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < MM_T / WAVE; w++) {
+            mn = smn[w] < mn ? smn[w] : mn;
+            mx = smx[w] > mx ? smx[w] : mx;
+        }
+        part[2 * blockIdx.x] = mn;
+        part[2 * blockIdx.x + 1] = mx;
     }
 }
 
-__global__ void k_init2(long long* p) {
-    if (threadIdx.x == 0) { p[0] = 0x7fffffffffffffffLL; p[1] = (long long)0x8000000000000000ull; }
+__global__ __launch_bounds__(MM_T) void k_minmax_final(const long long* part, int n,
+                                                       long long* out) {
+    __shared__ long long smn[MM_T], smx[MM_T];
+    long long mn = 0x7fffffffffffffffLL, mx = (long long)0x8000000000000000ull;
+    for (int i = threadIdx.x; i < n; i += MM_T) {
+        mn = part[2 * i] < mn ? part[2 * i] : mn;
+        mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+    }
+    smn[threadIdx.x] = mn;
+    smx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int d = MM_T / 2; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            smn[threadIdx.x] = smn[threadIdx.x + d] < smn[threadIdx.x] ? smn[threadIdx.x + d]
+                                                                         : smn[threadIdx.x];
+            smx[threadIdx.x] = smx[threadIdx.x + d] > smx[threadIdx.x] ? smx[threadIdx.x + d]
+                                                                         : smx[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = smn[0]; out[1] = smx[0]; }
 }
 
 static RowMask row_mask(const tdd_sel_t* s, int64_t nrows) {
@@ -142,14 +180,15 @@ static RowMask row_mask(const tdd_sel_t* s, int64_t nrows) {
 
 extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
                           int64_t* mn, int64_t* mx) {
-    SCRATCH(buf, 16);
-    LAUNCH("minmax", k_init2, 1, 64, 0, buf.as<long long>());
-    if (nrows > 0) {
-        int grid = stream_grid(ceil_div(nrows, 256 * 16), 8);
-        LAUNCH("minmax", k_minmax, grid, 256, 0, *col, row_mask(mask, nrows), buf.as<long long>());
-    }
+    int grid = stream_grid(ceil_div(nrows > 0 ? nrows : 1, MM_T * MM_R * 2), 4);
+    SCRATCH(buf, 16 + (size_t)grid * 16);
+    long long* out = buf.as<long long>();
+    long long* part = out + 2;
+    LAUNCH("minmax", k_minmax, grid, MM_T, 0, *col, (int)tile_vec_ok(*col),
+           row_mask(mask, nrows), part);
+    LAUNCH("minmax", k_minmax_final, 1, MM_T, 0, (const long long*)part, grid, out);
     long long h[2];
-    HC(hipMemcpyAsync(h, buf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
     *mn = h[0];
     *mx = h[1];
@@ -194,33 +233,54 @@ __global__ __launch_bounds__(GT) void k_da_lds(GroupPlan P, RowMask M, int64_t n
     for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
 }
 
-/* One thread per cell folds that cell of all workgroups' partials in
- * workgroup order, so the result does not depend on which workgroup
- * finished first. */
-__global__ void k_da_fold(GroupPlan P, const uint64_t* partials, int nparts, int64_t n_slots,
-                          uint64_t* tab) {
-    int64_t stride = n_slots * P.ncells;
+__device__ __forceinline__ uint64_t fold_cell(int kind, uint64_t acc, uint64_t s) {
+    switch (kind) {
+    case C_ADD_F64: case C_SQ_F64:
+        return (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) +
+                                              __longlong_as_double((long long)s));
+    case C_COUNT: case C_ADD_I64: return acc + s;
+    case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW:
+        return (int64_t)s < (int64_t)acc ? s : acc;
+    default:
+        return (int64_t)s > (int64_t)acc ? s : acc;
+    }
+}
+
+/* Partials are folded in a fixed order, so the result does not depend on
+ * which workgroup finished first: chunk c of FOLD_CHUNK consecutive
+ * workgroups is folded in workgroup order (one thread per cell, the cells of
+ * a partial read contiguously), then the chunks in chunk order. */
+#define FOLD_CHUNK 32
+
+__global__ void k_da_fold(GroupPlan P, const uint64_t* partials, int nparts, int64_t stride,
+                          int chunk, uint64_t* out) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= stride) return;
+    int p0 = (int)blockIdx.y * chunk;
+    int p1 = p0 + chunk < nparts ? p0 + chunk : nparts;
     int kind = P.kind[idx % P.ncells];
-    uint64_t acc = partials[idx];
-    for (int b = 1; b < nparts; b++) {
-        uint64_t s = partials[(int64_t)b * stride + idx];
-        switch (kind) {
-        case C_ADD_F64: case C_SQ_F64:
-            acc = (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) +
-                                                 __longlong_as_double((long long)s));
-            break;
-        case C_COUNT: case C_ADD_I64: acc += s; break;
-        case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW:
-            if ((int64_t)s < (int64_t)acc) acc = s;
-            break;
-        default:
-            if ((int64_t)s > (int64_t)acc) acc = s;
-            break;
-        }
+    uint64_t acc = partials[(int64_t)p0 * stride + idx];
+    for (int b = p0 + 1; b < p1; b++) acc = fold_cell(kind, acc, partials[(int64_t)b * stride + idx]);
+    out[(int64_t)blockIdx.y * stride + idx] = acc;
+}
+
+/* tab = fold of nparts partial tables of n_slots * ncells cells. */
+static int fold_partials(const GroupPlan& P, const uint64_t* partials, int nparts,
+                         int64_t n_slots, uint64_t* tab) {
+    int64_t stride = n_slots * P.ncells;
+    int bx = (int)ceil_div(stride, 256);
+    int nch = (int)ceil_div(nparts, FOLD_CHUNK);
+    if (nch <= 1) {
+        LAUNCH("group_da_fold", k_da_fold, dim3(bx, 1), 256, 0, P, partials, nparts, stride,
+               nparts, tab);
+        return TD_OK;
     }
-    tab[idx] = acc;
+    SCRATCH(tmp, (size_t)nch * stride * 8);
+    LAUNCH("group_da_fold", k_da_fold, dim3(bx, nch), 256, 0, P, partials, nparts, stride,
+           FOLD_CHUNK, tmp.as<uint64_t>());
+    LAUNCH("group_da_fold", k_da_fold, dim3(bx, 1), 256, 0, P,
+           (const uint64_t*)tmp.as<uint64_t>(), nch, stride, nch, tab);
+    return TD_OK;
 }
 
 __global__ __launch_bounds__(GT) void k_da_global(GroupPlan P, RowMask M, uint64_t* tab) {
@@ -284,9 +344,8 @@ int64_t g_da_max_slots = (int64_t)1 << 24;
 int da_finish(const GroupPlan& P, const uint64_t* partials, int nparts, int64_t n_slots,
               tdd_group_result_t* out) {
     SCRATCH(tab, (size_t)n_slots * P.ncells * 8);
-    int fb = (int)ceil_div(n_slots * P.ncells, 256);
-    LAUNCH("group_da_fold", k_da_fold, fb, 256, 0, P, partials, nparts, n_slots,
-           tab.as<uint64_t>());
+    int rc = fold_partials(P, partials, nparts, n_slots, tab.as<uint64_t>());
+    if (rc) return rc;
     return da_emit(P, tab.as<uint64_t>(), n_slots, out);
 }
 
@@ -309,9 +368,8 @@ static int run_da(GroupPlan& P, const RowMask& M, int64_t n_slots, tdd_group_res
         SCRATCH(parts, (size_t)grid * table_bytes);
         LAUNCH("group_da_lds", k_da_lds, grid, GT, table_bytes, P, M, n_slots,
                parts.as<uint64_t>());
-        int fb = (int)ceil_div(n_slots * P.ncells, 256);
-        LAUNCH("group_da_fold", k_da_fold, fb, 256, 0, P, (const uint64_t*)parts.as<uint64_t>(),
-               grid, n_slots, tab.as<uint64_t>());
+        int rc = fold_partials(P, parts.as<uint64_t>(), grid, n_slots, tab.as<uint64_t>());
+        if (rc) return rc;
         out->path = 1;
     } else {
         int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));

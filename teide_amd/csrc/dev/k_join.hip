@@ -20,6 +20,7 @@
  */
 #include "group_plan.h"
 #include "pred_eval.h"
+#include "tile.h"
 
 #include <stdio.h>
 
@@ -658,13 +659,237 @@ __global__ __launch_bounds__(512) void k_jg_global(JGArgs J, GroupPlan P, uint64
     }
 }
 
-/* slotmap[key - kmin] = group slot of the build row (build-side keys only). */
-__global__ void k_jg_slotmap(tdd_col_t rk, int64_t nr, int64_t kmin, GroupPlan P, int32_t* map) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nr;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = ld_i64(rk, r) - kmin;
-        map[i] = (int32_t)da_slot2(P, 0, r);
+/* ------------------------------------------------- slot map, tiled probe -- */
+
+/* Unique-key slot map: one scatter pass stores every build row's group slot
+ * at its key; the keys are unique exactly when nr entries end up non-empty
+ * (rows sharing a key share an entry), which one streaming count over the
+ * map decides.  Map entries are u16 when the slots fit (EMPTY = 0xffff),
+ * else i32 (EMPTY = -1): the probe's random gathers touch half the lines. */
+template <typename MT>
+__global__ __launch_bounds__(256) void k_sm_scatter(TileCols T, int64_t nr, int64_t kmin,
+                                                    GroupPlan P, MT* map) {
+    const int R = 8;
+    const int lane = lane_id();
+    const int64_t n_tiles = (nr + 64 * R - 1) / (64 * R);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t n_waves = (int64_t)gridDim.x * blockDim.x / WAVE;
+    for (int64_t t = wave; t < n_tiles; t += n_waves) {
+        const int64_t base = t * 64 * R + (int64_t)lane * R;
+        int64_t cv[TL_NC][R];
+        load_tile<R>(T, base, nr, cv);
+        uint32_t m = tile_live<R>(nullptr, base, nr);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            if (!((m >> j) & 1)) continue;
+            int64_t slot = 0;
+            for (int k = 0; k < P.n_keys; k++)          /* key k sits in tile slot k + 1 */
+                slot += (cget<R>(cv, k + 1, j) - P.kmin[k]) * P.kstride[k];
+            map[cget<R>(cv, 0, j) - kmin] = (MT)slot;
+        }
     }
+}
+
+template <typename MT>
+__global__ __launch_bounds__(256) void k_sm_count(const MT* map, int64_t range, MT empty,
+                                                  int64_t* part) {
+    __shared__ int64_t sc[256 / WAVE];
+    int64_t c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < range;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += map[i] != empty;
+    for (int d = WAVE / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, WAVE);
+    if (lane_id() == 0) sc[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / WAVE; w++) c += sc[w];
+        part[blockIdx.x] = c;
+    }
+}
+
+__global__ void k_sum_final(const int64_t* part, int n, int64_t* out) {
+    __shared__ int64_t s[256];
+    int64_t c = 0;
+    for (int i = threadIdx.x; i < n; i += 256) c += part[i];
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+/* Build the slot map into `map` (range entries of MT); returns 1 in *unique
+ * when the build keys are distinct. */
+template <typename MT>
+static int sm_build(const tdd_col_t& rk, int64_t nr, int64_t kmin, int64_t range,
+                    const GroupPlan& P, MT* map, int* unique) {
+    TileCols T;
+    memset(&T, 0, sizeof(T));
+    tile_slot(T, rk);
+    for (int k = 0; k < P.n_keys; k++) {     /* distinct slots: key k at k + 1 */
+        if (T.n >= TL_NC) return TD_ERR_NYI;
+        T.col[T.n] = P.keys[k];
+        if (tile_vec_ok(P.keys[k])) T.vec |= 1u << T.n;
+        T.n++;
+    }
+    int rc = tdd_fill(map, sizeof(MT) == 2 ? 0xffffffffffffffffull : ~0ull, (size_t)range * sizeof(MT));
+    if (rc) return rc;
+    if (nr > 0)
+        LAUNCH("jg_slotmap", k_sm_scatter<MT>, stream_grid(ceil_div(nr, 256 * 8), 8), 256, 0, T,
+               nr, kmin, P, map);
+    int grid = stream_grid(ceil_div(range, 256 * 16), 4);
+    SCRATCH(part, (size_t)(grid + 1) * 8);
+    LAUNCH("jg_slotmap", k_sm_count<MT>, grid, 256, 0, (const MT*)map, range, (MT)~(MT)0,
+           part.as<int64_t>() + 1);
+    LAUNCH("jg_slotmap", k_sum_final, 1, 256, 0, (const int64_t*)part.as<int64_t>() + 1, grid,
+           part.as<int64_t>());
+    int64_t cnt = 0;
+    HC(hipMemcpyAsync(&cnt, part.p, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    *unique = cnt == nr;
+    return TD_OK;
+}
+
+/* Tiled fused probe (slot-map mode): probe-side columns -- the join key,
+ * the predicate's columns and the aggregate inputs -- sit in the tile cache;
+ * per tile a lane issues all its column loads, evaluates the predicate on
+ * R rows at once, issues R independent slot-map gathers and folds the rows
+ * into the workgroup's LDS table. */
+struct JGTile {
+    TileCols        T;
+    tdd_pred_t      pred;          /* operands are tile slots */
+    int32_t         has_pred;
+    int32_t         key_c;         /* tile slot of the join key */
+    const uint64_t* lsel;
+    int64_t         nl;
+    int64_t         kmin, range;
+    const void*     slotmap;       /* MT entries: u16 (EMPTY 0xffff) or i32 (EMPTY -1) */
+    int64_t         n_slots;
+    uint8_t         agg_c[TDD_MAX_AGGS];   /* tile slot of each aggregate input */
+    int32_t         map_nt;        /* slot-map gathers as non-temporal loads */
+};
+
+template <int R, typename MT>
+__global__ __launch_bounds__(512) void k_jg_tile(JGTile A, GroupPlan P, uint64_t* partials) {
+    const MT* slotmap = (const MT*)A.slotmap;
+    extern __shared__ uint64_t tab[];
+    const int64_t ncell_total = A.n_slots * P.ncells;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x)
+        tab[i] = cell_identity(P.kind[i % P.ncells]);
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t n_tiles = (A.nl + 64 * R - 1) / (64 * R);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t t = wave; t < n_tiles; t += n_waves) {
+        const int64_t base = t * 64 * R + (int64_t)lane * R;
+        int64_t cv[TL_NC][R];
+        load_tile<R>(A.T, base, A.nl, cv);
+        uint32_t m = tile_live<R>(A.lsel, base, A.nl);
+        if (A.has_pred) m &= pred_tile<R>(A.pred, A.T, cv);
+        int32_t s[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            s[j] = -1;
+            uint64_t i = (uint64_t)(cget<R>(cv, A.key_c, j) - A.kmin);
+            if (((m >> j) & 1) && i < (uint64_t)A.range) {
+                MT v = A.map_nt ? __builtin_nontemporal_load(slotmap + i) : slotmap[i];
+                s[j] = v == (MT)~(MT)0 ? -1 : (int32_t)v;
+            }
+        }
+        /* cell-major: the kind switch is uniform, the row loop unrolls */
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            if (s[j] >= 0) atomicAdd((unsigned long long*)&tab[(int64_t)s[j] * P.ncells], 1ull);
+        for (int i = 1; i < P.ncells; i++) {
+            const int cs = A.agg_c[P.col[i]];
+            const bool f = A.T.col[cs].type == TD_F64;
+            switch (P.kind[i]) {
+            case C_ADD_F64:
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    if (s[j] < 0) continue;
+                    int64_t raw = cget<R>(cv, cs, j);
+                    atomicAdd((double*)&tab[(int64_t)s[j] * P.ncells + i],
+                              f ? __longlong_as_double(raw) : (double)raw);
+                }
+                break;
+            case C_SQ_F64:
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    if (s[j] < 0) continue;
+                    int64_t raw = cget<R>(cv, cs, j);
+                    double v = f ? __longlong_as_double(raw) : (double)raw;
+                    atomicAdd((double*)&tab[(int64_t)s[j] * P.ncells + i], v * v);
+                }
+                break;
+            case C_ADD_I64:
+#pragma unroll
+                for (int j = 0; j < R; j++)
+                    if (s[j] >= 0)
+                        atomicAdd((unsigned long long*)&tab[(int64_t)s[j] * P.ncells + i],
+                                  (unsigned long long)cget<R>(cv, cs, j));
+                break;
+            case C_MIN_I64: case C_MAX_I64: case C_MIN_F64: case C_MAX_F64: {
+                const bool is_min = P.kind[i] == C_MIN_I64 || P.kind[i] == C_MIN_F64;
+                const bool ord = P.kind[i] == C_MIN_F64 || P.kind[i] == C_MAX_F64;
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    if (s[j] < 0) continue;
+                    int64_t raw = cget<R>(cv, cs, j);
+                    long long v = ord ? (long long)f64_ord(f ? __longlong_as_double(raw) : (double)raw)
+                                      : (long long)raw;
+                    long long* c = (long long*)&tab[(int64_t)s[j] * P.ncells + i];
+                    if (is_min) atomicMin(c, v);
+                    else atomicMax(c, v);
+                }
+                break;
+            }
+            default: break;              /* row cells (FIRST / LAST) are not fused */
+            }
+        }
+    }
+    __syncthreads();
+    uint64_t* dst = partials + (int64_t)blockIdx.x * ncell_total;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
+}
+
+/* Tile description of a slot-map probe, when every probe-side column fits
+ * the tile cache and the predicate's stack fits R-bit entries. */
+template <int R>
+static bool jg_tile_plan(const tdd_join_group_spec_t* S, const GroupPlan& P, JGTile& A) {
+    memset(&A, 0, sizeof(A));
+    int kc = tile_slot(A.T, S->join.lkeys[0]);
+    if (kc < 0) return false;
+    A.key_c = kc;
+    if (S->lpred) {
+        if (pred_depth(*S->lpred) * R > 64) return false;
+        A.pred = *S->lpred;
+        for (int i = 0; i < A.pred.n_ins; i++) {
+            tdd_pred_ins_t& in = A.pred.ins[i];
+            if (in.kind == TDD_P_CMP_CONST || in.kind == TDD_P_CMP_COL || in.kind == TDD_P_BOOLCOL) {
+                int a = tile_slot(A.T, S->lpred->cols[in.a]);
+                if (a < 0) return false;
+                in.a = (uint8_t)a;
+            }
+            if (in.kind == TDD_P_CMP_COL) {
+                int b = tile_slot(A.T, S->lpred->cols[in.b]);
+                if (b < 0) return false;
+                in.b = (uint8_t)b;
+            }
+        }
+        A.has_pred = 1;
+    }
+    for (int a = 0; a < P.n_aggs; a++) {
+        if (!P.aggs[a].data) continue;             /* COUNT without a column */
+        if (P.agg_side[a]) return false;
+        int c = tile_slot(A.T, P.aggs[a]);
+        if (c < 0) return false;
+        A.agg_c[a] = (uint8_t)c;
+    }
+    return true;
 }
 
 static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out);
@@ -724,23 +949,44 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
     for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st; st *= P.krange[k]; }
     int64_t n_slots = (int64_t)slots;
 
-    SCRATCH(head, (size_t)range * 4);
-    SCRATCH(next, (size_t)(J->nr > 0 ? J->nr : 1) * 4);
-    SCRATCH(dup, 8);
-    rc = tdd_fill(head.p, ~0ull, (size_t)range * 4);
-    if (!rc) rc = tdd_fill(dup.p, 0, 8);
-    if (rc) return rc;
-    if (J->nr > 0)
-        LAUNCH("jg_build", k_dm_build, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0,
-               J->rkeys[0], J->nr, mn, head.as<int32_t>(), next.as<int32_t>(), dup.as<int>());
-    int d = 0;
-    HC(hipMemcpyAsync(&d, dup.p, 4, hipMemcpyDeviceToHost, tdd::g_stream));
-    HC(hipStreamSynchronize(tdd::g_stream));
-
     bool right_keys_only = true, left_aggs_only = true;
     for (int k = 0; k < P.n_keys; k++) right_keys_only = right_keys_only && P.key_side[k] == 1;
     for (int a = 0; a < P.n_aggs; a++) left_aggs_only = left_aggs_only && P.agg_side[a] == 0;
-    bool slotmap = !d && right_keys_only && left_aggs_only && n_slots < ((int64_t)1 << 31);
+    bool slotmap = right_keys_only && left_aggs_only && n_slots < ((int64_t)1 << 31);
+
+    size_t table_bytes = (size_t)n_slots * P.ncells * 8;
+    const bool lds_table = table_bytes <= 64 * 1024;
+    JGTile TA;
+    const int TR = TUNE_INT("TEIDE_TILE_ROWS", 8) == 4 ? 4 : 8;      /* rows per lane */
+    const int TB = TUNE_INT("TEIDE_TILE_BLOCK", 512) == 256 ? 256 : 512;
+    bool tiled = slotmap && lds_table &&
+                 (TR == 8 ? jg_tile_plan<8>(S, P, TA) : jg_tile_plan<4>(S, P, TA));
+    const bool small = tiled && n_slots < 0xffff;      /* u16 slot map */
+
+    /* Slot map for unique build keys; duplicates fall back to chains. */
+    Scratch smap(slotmap ? (size_t)range * (small ? 2 : 4) : 8);
+    if (!smap.ok()) return TD_ERR_OOM;
+    if (slotmap) {
+        int uniq = 0;
+        rc = small ? sm_build<uint16_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<uint16_t>(), &uniq)
+                   : sm_build<int32_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<int32_t>(), &uniq);
+        if (rc == TD_ERR_NYI) rc = TD_OK;
+        if (rc) return rc;
+        slotmap = uniq != 0;
+        tiled = tiled && slotmap;
+    }
+    Scratch head(slotmap ? 8 : (size_t)range * 4);
+    Scratch next(slotmap ? 8 : (size_t)(J->nr > 0 ? J->nr : 1) * 4);
+    if (!head.ok() || !next.ok()) return TD_ERR_OOM;
+    if (!slotmap) {
+        SCRATCH(dup, 8);
+        rc = tdd_fill(head.p, ~0ull, (size_t)range * 4);
+        if (!rc) rc = tdd_fill(dup.p, 0, 8);
+        if (rc) return rc;
+        if (J->nr > 0)
+            LAUNCH("jg_build", k_dm_build, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0,
+                   J->rkeys[0], J->nr, mn, head.as<int32_t>(), next.as<int32_t>(), dup.as<int>());
+    }
 
     JGArgs A;
     memset(&A, 0, sizeof(A));
@@ -750,29 +996,45 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
     A.lk = J->lkeys[0];
     A.D = DMap{head.as<int32_t>(), next.as<int32_t>(), mn, range};
     A.n_slots = n_slots;
-    Scratch smap(slotmap ? (size_t)range * 4 : 8);
-    if (!smap.ok()) return TD_ERR_OOM;
-    if (slotmap) {
-        rc = tdd_fill(smap.p, ~0ull, (size_t)range * 4);
-        if (rc) return rc;
-        if (J->nr > 0)
-            LAUNCH("jg_slotmap", k_jg_slotmap, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0,
-                   J->rkeys[0], J->nr, mn, P, smap.as<int32_t>());
-        A.slotmap = smap.as<int32_t>();
-    }
+    if (slotmap) A.slotmap = smap.as<int32_t>();
 
-    size_t table_bytes = (size_t)n_slots * P.ncells * 8;
     int64_t n_words = (J->nl + 63) / 64;
-    if (table_bytes <= 64 * 1024) {
+    if (lds_table) {
         int per_cu = (int)(160 * 1024 / (table_bytes + 1024));
         if (per_cu > 4) per_cu = 4;
         if (per_cu < 1) per_cu = 1;
         int grid = stream_grid(ceil_div(n_words > 0 ? n_words : 1, 8), per_cu);
+        if (tiled) {
+            int64_t tiles = ceil_div(J->nl > 0 ? J->nl : 1, 64 * TR);
+            int64_t units = ceil_div(tiles, TB / WAVE);
+            grid = TR == 8 ? (small ? resident_grid(k_jg_tile<8, uint16_t>, TB, table_bytes, units)
+                                    : resident_grid(k_jg_tile<8, int32_t>, TB, table_bytes, units))
+                           : (small ? resident_grid(k_jg_tile<4, uint16_t>, TB, table_bytes, units)
+                                    : resident_grid(k_jg_tile<4, int32_t>, TB, table_bytes, units));
+        }
         SCRATCH(parts, (size_t)grid * table_bytes);
-        if (slotmap)
+        if (tiled) {
+            TA.lsel = A.lsel;
+            TA.nl = J->nl;
+            TA.kmin = mn;
+            TA.range = range;
+            TA.slotmap = smap.p;
+            TA.n_slots = n_slots;
+            TA.map_nt = TUNE_INT("TEIDE_MAP_NT", 0);
+            uint64_t* pp = parts.as<uint64_t>();
+            if (TR == 8 && small)
+                LAUNCH("jg_probe_tile", (k_jg_tile<8, uint16_t>), grid, TB, table_bytes, TA, P, pp);
+            else if (TR == 8)
+                LAUNCH("jg_probe_tile", (k_jg_tile<8, int32_t>), grid, TB, table_bytes, TA, P, pp);
+            else if (small)
+                LAUNCH("jg_probe_tile", (k_jg_tile<4, uint16_t>), grid, TB, table_bytes, TA, P, pp);
+            else
+                LAUNCH("jg_probe_tile", (k_jg_tile<4, int32_t>), grid, TB, table_bytes, TA, P, pp);
+        } else if (slotmap) {
             LAUNCH("jg_probe_lds", k_jg_lds<true>, grid, 512, table_bytes, A, P, parts.as<uint64_t>());
-        else
+        } else {
             LAUNCH("jg_probe_lds", k_jg_lds<false>, grid, 512, table_bytes, A, P, parts.as<uint64_t>());
+        }
         rc = da_finish(P, parts.as<uint64_t>(), grid, n_slots, out);
         out->path = 1;
     } else {

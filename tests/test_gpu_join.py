@@ -240,3 +240,69 @@ def test_join_group_general(eng, variant):
     f64 = [ref["out_types"][i] == 7 and o in ("SUM", "AVG") for i, (o, _) in enumerate(aggs)]
     compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], ref["keys"],
                    ref["vals"], f64, variant, exact=ref["comp"], sum_abs=ref["sum_abs"])
+
+
+TILED_SHAPES = {
+    # name: (predicate builder, numpy mask, aggregates)
+    "two_filters": (lambda g: [g.gt(g.scan("qty"), g.const_i64(30)),
+                               g.lt(g.scan("price"), g.const_f64(50.0))],
+                    lambda f: (f["qty"] > 30) & (f["price"] < 50.0),
+                    [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")]),
+    "or_not": (lambda g: [g.or_(g.gt(g.scan("qty"), g.const_i64(80)),
+                                g.not_(g.eq(g.scan("disc"), g.const_i64(3))))],
+               lambda f: (f["qty"] > 80) | ~(f["disc"] == 3),
+               [("SUM", "price"), ("MIN", "qty"), ("MAX", "price"), ("AVG", "price")]),
+    "cmp_col": (lambda g: [g.gt(g.scan("qty"), g.scan("disc"))],
+                lambda f: f["qty"] > f["disc"],
+                [("SUM", "disc"), ("MAX", "disc"), ("COUNT", "qty")]),
+    "i32_key": (lambda g: [g.ge(g.scan("qty"), g.const_i64(50))],
+                lambda f: f["qty"] >= 50,
+                [("SUM", "price"), ("MIN", "price"), ("SUM", "disc")]),
+}
+
+
+@pytest.mark.parametrize("shape", list(TILED_SHAPES))
+@pytest.mark.parametrize("resident", [False, True])
+def test_jg_tiled_shapes(eng, shape, resident):
+    """Fused JOIN -> GROUP through the tiled slot-map probe: stacked
+    filters (a selection), OR / NOT and column-column predicates, 4-byte key
+    and aggregate columns, probe keys missing from the build side, and a row
+    count that leaves a partial last tile."""
+    rng = np.random.default_rng(list(TILED_SHAPES).index(shape))
+    nf, nd = 500003, 40000
+    f = {"fk": rng.integers(0, nd + 300, nf), "qty": rng.integers(1, 101, nf),
+         "price": rng.lognormal(3, 1, nf), "disc": rng.integers(0, 10, nf).astype(np.int32)}
+    if shape == "i32_key":
+        f["fk"] = f["fk"].astype(np.int32)
+    pk = rng.permutation(nd)
+    cat = rng.integers(0, 300, nd).astype(np.int32)
+    tf = eng.table(list(f.items()))
+    td = eng.table([("pk", pk), ("cat", cat)])
+    if resident:
+        tf2, td2 = eng.put(tf), eng.put(td)
+        eng.release(tf)
+        eng.release(td)
+        tf, td = tf2, td2
+    preds, mask_fn, aggs = TILED_SHAPES[shape]
+    g = eng.graph(tf)
+    node = g.const_table(tf)
+    for p in preds(g):
+        node = g.filter(node, p)
+    g.join(node, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+    root = g.group([g.scan("cat")], [(o, g.scan(c)) for o, c in aggs])
+    out = eng.read_table(g.execute(root))
+    plan = eng.last_plan()
+    g.free()
+    eng.release(tf)
+    eng.release(td)
+    assert plan.startswith("join_group"), plan
+    m = mask_fn(f)
+    fm = {c: a[m] for c, a in f.items()}
+    li, ri = orc.join([fm["fk"]], [pk], 0)
+    cols = {c: orc.gather(a, li) for c, a in fm.items()}
+    cols["cat"] = orc.gather(cat, ri)
+    ref = orc.group([cols["cat"]], [o for o, _ in aggs], [cols[c] for _, c in aggs])
+    f64 = [ref["out_types"][i] == 7 and o in ("SUM", "AVG") for i, (o, _) in enumerate(aggs)]
+    assert [c[1] for c in out[1:]] == ref["out_types"]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], ref["keys"],
+                   ref["vals"], f64, shape, exact=ref["comp"], sum_abs=ref["sum_abs"])
