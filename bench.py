@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--dim", type=int, default=N_DIM)
     ap.add_argument("--cats", type=int, default=N_CAT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--join-path", type=int, default=0,
+                    help="0 planner's choice, 1 direct slot map, 3 radix-partitioned")
     ap.add_argument("--no-prof", action="store_true",
                     help="no per-kernel HIP events in the timed steps")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -150,6 +152,7 @@ def main():
 
     if a.calibrate_copy:
         calibrate_copy(e, a.calibrate_copy << 20)
+    e.force_paths(a.join_path, 0, 0)
 
     t0 = time.perf_counter()
     fact = W.join_fact(a.fact, a.dim, SEED)

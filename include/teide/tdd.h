@@ -61,7 +61,9 @@ typedef struct {
     int8_t   type;      /* td.h type tag */
     uint8_t  attrs;     /* TD_SYM index width in bits 0-1 */
     uint8_t  esz;       /* element bytes */
-    uint8_t  pad;
+    uint8_t  ranged;    /* 1: every element (widened to int64) lies in [lo, hi] */
+    uint8_t  pad[4];
+    int64_t  lo, hi;    /* known bounds of an integer column (column statistics) */
 } tdd_col_t;
 
 /* A device TD_SEL: pointers into one device copy of the td.h TD_SEL payload
@@ -180,7 +182,7 @@ typedef struct {
     tdd_col_t         lkeys[TDD_MAX_KEYS];
     tdd_col_t         rkeys[TDD_MAX_KEYS];
     const tdd_sel_t*  lsel;               /* left rows that take part; NULL: all */
-    int32_t           force_path;         /* 0 auto, 1 direct map, 2 radix */
+    int32_t           force_path;         /* 0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group */
     int32_t           radix_bits;         /* 0: chosen from the build size */
 } tdd_join_spec_t;
 

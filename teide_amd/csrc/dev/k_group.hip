@@ -180,6 +180,11 @@ static RowMask row_mask(const tdd_sel_t* s, int64_t nrows) {
 
 extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
                           int64_t* mn, int64_t* mx) {
+    if (col->ranged && !mask && nrows > 0) {          /* known column statistics */
+        *mn = col->lo;
+        *mx = col->hi;
+        return TD_OK;
+    }
     int grid = stream_grid(ceil_div(nrows > 0 ? nrows : 1, MM_T * MM_R * 2), 4);
     SCRATCH(buf, 16 + (size_t)grid * 16);
     long long* out = buf.as<long long>();

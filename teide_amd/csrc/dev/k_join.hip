@@ -21,6 +21,7 @@
 #include "group_plan.h"
 #include "pred_eval.h"
 #include "tile.h"
+#include "star.h"
 
 #include <stdio.h>
 
@@ -856,6 +857,72 @@ __global__ __launch_bounds__(512) void k_jg_tile(JGTile A, GroupPlan P, uint64_t
     for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
 }
 
+/* Star-shaped slot-map probe: key in slot 0, a range-check predicate on
+ * slots 1..NPRED, aggregate inputs reached by a uniform switch per cell. */
+#define STAR_R 4          /* rows per lane per tile, star probe */
+
+struct JGStar {
+    TileCols        T;
+    StarPred        SP;
+    const uint64_t* lsel;
+    int64_t         nl;
+    int64_t         kmin, range;
+    const void*     slotmap;       /* MT entries: u16 (EMPTY 0xffff) or i32 (EMPTY -1) */
+    int64_t         n_slots;
+    uint8_t         cell_slot[GP_MAX_CELLS];
+};
+
+template <int R, typename MT, int NPRED>
+__global__ __launch_bounds__(512) void k_jg_star(JGStar A, GroupPlan P, uint64_t* partials) {
+    extern __shared__ uint64_t tab[];
+    const MT* slotmap = (const MT*)A.slotmap;
+    const int64_t ncell_total = A.n_slots * P.ncells;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x)
+        tab[i] = cell_identity(P.kind[i % P.ncells]);
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t n_tiles = (A.nl + 64 * R - 1) / (64 * R);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    for (int64_t t = wave; t < n_tiles; t += n_waves) {
+        const int64_t base = t * 64 * R + (int64_t)lane * R;
+        int64_t cv[TL_NC][R];
+        load_tile<R>(A.T, base, A.nl, cv);
+        const uint32_t m = tile_live<R>(A.lsel, base, A.nl) & star_pred<R, NPRED>(A.SP, A.T, cv);
+        int32_t s[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            s[j] = -1;
+            uint64_t i = (uint64_t)(cv[0][j] - A.kmin);
+            if (((m >> j) & 1) && i < (uint64_t)A.range) {
+                MT v = slotmap[i];
+                s[j] = v == (MT)~(MT)0 ? -1 : (int32_t)v;
+            }
+        }
+        acc_tile<R>(tab, P, A.cell_slot, A.T, cv, s);
+    }
+    __syncthreads();
+    uint64_t* dst = partials + (int64_t)blockIdx.x * ncell_total;
+    for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x) dst[i] = tab[i];
+}
+
+/* Host: star plan of a slot-map probe (probe-side aggregates only). */
+static bool jg_star_plan(const tdd_join_group_spec_t* S, const GroupPlan& P, JGStar& A) {
+    memset(&A, 0, sizeof(A));
+    if (!star_plan(S->join.lkeys[0], S->lpred, A.T, A.SP)) return false;
+    int agg_slot[TDD_MAX_AGGS];
+    for (int a = 0; a < P.n_aggs; a++) {
+        agg_slot[a] = 0;
+        if (!P.aggs[a].data) continue;
+        if (P.agg_side[a]) return false;
+        int c = tile_slot(A.T, P.aggs[a]);
+        if (c < 0) return false;
+        agg_slot[a] = c;
+    }
+    for (int i = 1; i < P.ncells; i++) A.cell_slot[i] = (uint8_t)agg_slot[P.col[i]];
+    return true;
+}
+
 /* Tile description of a slot-map probe, when every probe-side column fits
  * the tile cache and the predicate's stack fits R-bit entries. */
 template <int R>
@@ -893,6 +960,8 @@ static bool jg_tile_plan(const tdd_join_group_spec_t* S, const GroupPlan& P, JGT
 }
 
 static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out);
+int jg_partitioned(const tdd_join_group_spec_t* S, const GroupPlan& GP, int64_t kmin,
+                   int64_t range, int64_t n_slots, bool forced, tdd_group_result_t* out);
 
 extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
     memset(out, 0, sizeof(*out));
@@ -956,12 +1025,22 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
 
     size_t table_bytes = (size_t)n_slots * P.ncells * 8;
     const bool lds_table = table_bytes <= 64 * 1024;
+
+    /* Large build key ranges: radix-partitioned probe with per-CU LDS
+     * slices of the slot map (force_path 3 takes it at any size). */
+    if (slotmap && (J->force_path == 0 || J->force_path == 3)) {
+        rc = jg_partitioned(S, P, mn, range, n_slots, J->force_path == 3, out);
+        if (rc != TD_ERR_NYI) return rc;
+        memset(out, 0, sizeof(*out));
+    }
     JGTile TA;
+    JGStar SA;
     const int TR = TUNE_INT("TEIDE_TILE_ROWS", 8) == 4 ? 4 : 8;      /* rows per lane */
     const int TB = TUNE_INT("TEIDE_TILE_BLOCK", 512) == 256 ? 256 : 512;
-    bool tiled = slotmap && lds_table &&
+    bool star = slotmap && lds_table && jg_star_plan(S, P, SA);
+    bool tiled = slotmap && lds_table && !star &&
                  (TR == 8 ? jg_tile_plan<8>(S, P, TA) : jg_tile_plan<4>(S, P, TA));
-    const bool small = tiled && n_slots < 0xffff;      /* u16 slot map */
+    const bool small = (tiled || star) && n_slots < 0xffff;      /* u16 slot map */
 
     /* Slot map for unique build keys; duplicates fall back to chains. */
     Scratch smap(slotmap ? (size_t)range * (small ? 2 : 4) : 8);
@@ -974,6 +1053,7 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
         if (rc) return rc;
         slotmap = uniq != 0;
         tiled = tiled && slotmap;
+        star = star && slotmap;
     }
     Scratch head(slotmap ? 8 : (size_t)range * 4);
     Scratch next(slotmap ? 8 : (size_t)(J->nr > 0 ? J->nr : 1) * 4);
@@ -1004,7 +1084,12 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
         if (per_cu > 4) per_cu = 4;
         if (per_cu < 1) per_cu = 1;
         int grid = stream_grid(ceil_div(n_words > 0 ? n_words : 1, 8), per_cu);
-        if (tiled) {
+        const int np = SA.SP.npred;
+        if (star) {
+            int64_t tiles = ceil_div(J->nl > 0 ? J->nl : 1, 64 * STAR_R);
+            int64_t units = ceil_div(tiles, 512 / WAVE);
+            grid = resident_grid(k_jg_star<STAR_R, int32_t, 2>, 512, table_bytes, units);
+        } else if (tiled) {
             int64_t tiles = ceil_div(J->nl > 0 ? J->nl : 1, 64 * TR);
             int64_t units = ceil_div(tiles, TB / WAVE);
             grid = TR == 8 ? (small ? resident_grid(k_jg_tile<8, uint16_t>, TB, table_bytes, units)
@@ -1013,7 +1098,27 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
                                     : resident_grid(k_jg_tile<4, int32_t>, TB, table_bytes, units));
         }
         SCRATCH(parts, (size_t)grid * table_bytes);
-        if (tiled) {
+        if (star) {
+            SA.lsel = A.lsel;
+            SA.nl = J->nl;
+            SA.kmin = mn;
+            SA.range = range;
+            SA.slotmap = smap.p;
+            SA.n_slots = n_slots;
+            uint64_t* pp = parts.as<uint64_t>();
+#define STAR_LAUNCH(MT, NP) \
+            LAUNCH("jg_probe_star", (k_jg_star<STAR_R, MT, NP>), grid, 512, table_bytes, SA, P, pp)
+            if (small) {
+                if (np == 0) STAR_LAUNCH(uint16_t, 0);
+                else if (np == 1) STAR_LAUNCH(uint16_t, 1);
+                else STAR_LAUNCH(uint16_t, 2);
+            } else {
+                if (np == 0) STAR_LAUNCH(int32_t, 0);
+                else if (np == 1) STAR_LAUNCH(int32_t, 1);
+                else STAR_LAUNCH(int32_t, 2);
+            }
+#undef STAR_LAUNCH
+        } else if (tiled) {
             TA.lsel = A.lsel;
             TA.nl = J->nl;
             TA.kmin = mn;

@@ -183,3 +183,106 @@ static inline int tile_slot(TileCols& T, const tdd_col_t& c) {
     if (tile_vec_ok(c)) T.vec |= 1u << T.n;
     return T.n++;
 }
+
+/* ------------------------------------------------------------ star tiles -- */
+/* The common shape of a fused probe: the join key in tile slot 0 and a
+ * predicate that is a conjunction of at most two comparisons of a probe
+ * column with a constant, evaluated as range checks on slots 1..npred.
+ * Everything the row loop touches then sits at a compile-time slot;
+ * aggregate inputs are reached through a uniform switch per cell (see
+ * acc_cell), so no per-row selection between cached columns remains. */
+
+struct StarPred {
+    int32_t npred;
+    int32_t f64[2];            /* compare as doubles */
+    int32_t neg[2];            /* negate (NE) */
+    int32_t lo_open[2], hi_open[2];
+    int64_t lo[2], hi[2];      /* integer bounds, inclusive */
+    double  flo[2], fhi[2];    /* f64 bounds */
+};
+
+template <int R>
+__device__ __forceinline__ uint32_t range_mask(const StarPred& S, int i, bool col_f64,
+                                               const int64_t (&v)[R]) {
+    uint32_t m = 0;
+    if (S.f64[i]) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            double x = col_f64 ? __longlong_as_double(v[j]) : (double)v[j];
+            bool in = (S.lo_open[i] ? x > S.flo[i] : x >= S.flo[i]) &&
+                      (S.hi_open[i] ? x < S.fhi[i] : x <= S.fhi[i]);
+            m |= (uint32_t)(in != (S.neg[i] != 0)) << j;
+        }
+    } else {
+        const uint64_t span = (uint64_t)(S.hi[i] - S.lo[i]);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            bool in = (uint64_t)(v[j] - S.lo[i]) <= span;
+            m |= (uint32_t)(in != (S.neg[i] != 0)) << j;
+        }
+    }
+    return m;
+}
+
+template <int R, int NPRED>
+__device__ __forceinline__ uint32_t star_pred(const StarPred& S, const TileCols& T,
+                                              const int64_t (&cv)[TL_NC][R]) {
+    uint32_t m = (1u << R) - 1;
+    if (NPRED >= 1) m &= range_mask<R>(S, 0, T.col[1].type == TD_F64, cv[1]);
+    if (NPRED >= 2) m &= range_mask<R>(S, 1, T.col[2].type == TD_F64, cv[2]);
+    return m;
+}
+
+/* Host: tile with the key at slot 0 and a star predicate at slots 1..npred;
+ * false when the predicate is not a conjunction of <= 2 column-constant
+ * comparisons. */
+static inline bool star_plan(const tdd_col_t& key, const tdd_pred_t* pred, TileCols& T,
+                             StarPred& S) {
+    memset(&T, 0, sizeof(T));
+    memset(&S, 0, sizeof(S));
+    T.col[0] = key;
+    if (tile_vec_ok(key)) T.vec |= 1u;
+    T.n = 1;
+    if (!pred) return true;
+    const tdd_pred_t& P = *pred;
+    int n = P.n_ins;
+    bool shape1 = n == 1 && P.ins[0].kind == TDD_P_CMP_CONST;
+    bool shape2 = n == 3 && P.ins[0].kind == TDD_P_CMP_CONST &&
+                  P.ins[1].kind == TDD_P_CMP_CONST && P.ins[2].kind == TDD_P_AND;
+    if (!shape1 && !shape2) return false;
+    S.npred = shape1 ? 1 : 2;
+    for (int i = 0; i < S.npred; i++) {
+        const tdd_pred_ins_t& in = P.ins[i];
+        const tdd_col_t& c = P.cols[in.a];
+        T.col[T.n] = c;
+        if (tile_vec_ok(c)) T.vec |= 1u << T.n;
+        T.n++;
+        if (in.as_f64) {
+            const double inf = __builtin_inf(), x = in.cf;
+            S.f64[i] = 1;
+            S.flo[i] = -inf;
+            S.fhi[i] = inf;
+            switch (in.cmp) {
+            case OP_EQ: S.flo[i] = S.fhi[i] = x; break;
+            case OP_NE: S.flo[i] = S.fhi[i] = x; S.neg[i] = 1; break;
+            case OP_LT: S.fhi[i] = x; S.hi_open[i] = 1; break;
+            case OP_LE: S.fhi[i] = x; break;
+            case OP_GT: S.flo[i] = x; S.lo_open[i] = 1; break;
+            default:    S.flo[i] = x; break;                          /* GE */
+            }
+        } else {
+            const int64_t mn = INT64_MIN, mx = INT64_MAX, x = in.ci;
+            S.lo[i] = mn;
+            S.hi[i] = mx;
+            switch (in.cmp) {
+            case OP_EQ: S.lo[i] = S.hi[i] = x; break;
+            case OP_NE: S.lo[i] = S.hi[i] = x; S.neg[i] = 1; break;
+            case OP_LT: if (x == mn) S.neg[i] = 1; else S.hi[i] = x - 1; break;
+            case OP_LE: S.hi[i] = x; break;
+            case OP_GT: if (x == mx) S.neg[i] = 1; else S.lo[i] = x + 1; break;
+            default:    S.lo[i] = x; break;                           /* GE */
+            }
+        }
+    }
+    return true;
+}

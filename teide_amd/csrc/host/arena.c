@@ -232,6 +232,7 @@ td_t* td__hbm_vec_new(int8_t type, uint8_t attrs, int64_t len, size_t bytes) {
     v->type = type;
     v->attrs = attrs;
     v->len = len;
+    memset(td_hbm_ref(v), 0, sizeof(td_hbm_ref_t));
     td_hbm_ref(v)->dptr = d;
     td_hbm_ref(v)->bytes = bytes;
     return v;
@@ -245,6 +246,7 @@ td_t* td__hbm_wrap(int8_t type, uint8_t attrs, int64_t len, void* dptr, size_t b
     v->type = type;
     v->attrs = attrs;
     v->len = len;
+    memset(td_hbm_ref(v), 0, sizeof(td_hbm_ref_t));
     td_hbm_ref(v)->dptr = dptr;
     td_hbm_ref(v)->bytes = bytes;
     return v;

@@ -45,11 +45,19 @@
 
 #define MAX_TMP 256
 
+#define MAX_UPLOADS 64
+
 typedef struct {
     td_graph_t* g;
     int   all_resident;
     void* tmp[MAX_TMP];
     int   ntmp;
+    /* host columns uploaded during this execution, so a column used by a
+     * predicate and an aggregate is one device copy (each held, so its
+     * address cannot be reused while the entry lives) */
+    td_t* up_v[MAX_UPLOADS];
+    void* up_d[MAX_UPLOADS];
+    int   nup;
 } xctx_t;
 
 static td_t* xnode(xctx_t* x, td_op_t* op);
@@ -63,6 +71,8 @@ static int keep_tmp(xctx_t* x, void* d) {
 static void drop_tmps(xctx_t* x) {
     for (int i = 0; i < x->ntmp; i++) td__arena_free(x->tmp[i]);
     x->ntmp = 0;
+    for (int i = 0; i < x->nup; i++) td_release(x->up_v[i]);
+    x->nup = 0;
 }
 
 /* Device view of a column: resident columns in place, slices as an offset
@@ -77,7 +87,11 @@ static int dev_col(xctx_t* x, td_t* v, tdd_col_t* out) {
     out->attrs = v->attrs & TD_SYM_W_MASK;
     out->esz = td__esz(v);
     if (v->mmod == TD_MMOD_HBM) {
-        out->data = td_hbm_ref(v)->dptr;
+        td_hbm_ref_t* r = td_hbm_ref(v);
+        out->data = r->dptr;
+        out->ranged = (uint8_t)(r->ranged != 0);
+        out->lo = r->lo;
+        out->hi = r->hi;
         return TD_OK;
     }
     td_t* src = v;
@@ -86,11 +100,17 @@ static int dev_col(xctx_t* x, td_t* v, tdd_col_t* out) {
         src = v->slice_parent;
         off = v->slice_offset;
         if (src->mmod == TD_MMOD_HBM) {
-            out->data = (char*)td_hbm_ref(src)->dptr + off * out->esz;
+            td_hbm_ref_t* r = td_hbm_ref(src);
+            out->data = (char*)r->dptr + off * out->esz;
+            out->ranged = (uint8_t)(r->ranged != 0);   /* parent bounds hold for a slice */
+            out->lo = r->lo;
+            out->hi = r->hi;
             return TD_OK;
         }
     }
     x->all_resident = 0;
+    for (int i = 0; i < x->nup; i++)
+        if (x->up_v[i] == v) { out->data = x->up_d[i]; return TD_OK; }
     size_t bytes = (size_t)v->len * out->esz;
     void* d = td__arena_alloc(bytes ? bytes : 8);
     if (!d) return TD_ERR_OOM;
@@ -99,6 +119,11 @@ static int dev_col(xctx_t* x, td_t* v, tdd_col_t* out) {
     if (bytes && tdd_h2d(d, (const char*)td_data(src) + off * out->esz, bytes) != TD_OK)
         return TD_ERR_IO;
     out->data = d;
+    if (x->nup < MAX_UPLOADS) {
+        td_retain(v);
+        x->up_v[x->nup] = v;
+        x->up_d[x->nup++] = d;
+    }
     return TD_OK;
 }
 

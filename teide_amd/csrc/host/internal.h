@@ -23,6 +23,9 @@
 typedef struct {
     void*    dptr;     /* device address of the elements (or TD_SEL payload) */
     uint64_t bytes;    /* allocated device bytes */
+    int64_t  lo, hi;   /* integer columns loaded by td_dev_put: min / max */
+    int32_t  ranged;   /* lo / hi valid (resident columns are immutable) */
+    int32_t  pad;
 } td_hbm_ref_t;
 
 static inline td_hbm_ref_t* td_hbm_ref(td_t* v) { return (td_hbm_ref_t*)td_data(v); }

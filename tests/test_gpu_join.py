@@ -180,7 +180,7 @@ def check_jg(out, case):
 
 @pytest.mark.parametrize("case", ["jg_2m_250k", "jg_100k_8k_c1m"])
 @pytest.mark.parametrize("form", ["two_call", "fused"])
-@pytest.mark.parametrize("path", [(0, 0), (2, 0)])
+@pytest.mark.parametrize("path", [(0, 0), (2, 0), (3, 0)])
 @pytest.mark.parametrize("resident", [False, True])
 def test_jg_golden(eng, forced, case, form, path, resident):
     nf, nd, nc, seed = (int(a) for a in MANIFEST[case]["argv"][1:5])
@@ -200,14 +200,20 @@ def test_jg_golden(eng, forced, case, form, path, resident):
     eng.release(td)
 
 
-@pytest.mark.parametrize("variant", ["dup_dim", "agg_on_dim", "key_on_fact", "left_join"])
-def test_join_group_general(eng, variant):
+@pytest.mark.parametrize("variant", ["dup_dim", "agg_on_dim", "key_on_fact", "left_join",
+                                     "zipf_fk"])
+@pytest.mark.parametrize("jpath", [0, 3])
+def test_join_group_general(eng, forced, variant, jpath):
     """Fused JOIN -> GROUP shapes off the one-index fast path: duplicate
     build keys, aggregates over build columns, keys on the probe side and a
     left join (unmatched rows carry zero-filled build columns)."""
     rng = np.random.default_rng(5)
-    nf, nd = 300000, 20000
+    nf, nd = 300000, (200000 if variant == "zipf_fk" else 20000)
     fk = rng.integers(0, nd + 500, nf)
+    if variant == "zipf_fk":
+        # heavy key skew: the partitioned probe's regions overflow and the
+        # direct-map probe takes over
+        fk = np.minimum(rng.zipf(1.3, nf) - 1, nd + 499).astype(np.int64)
     qty = rng.integers(1, 101, nf)
     price = rng.lognormal(3, 1, nf)
     pk = rng.permutation(nd)
@@ -224,6 +230,7 @@ def test_join_group_general(eng, variant):
         keys, aggs = ["cat"], [("SUM", "wgt"), ("MIN", "qty"), ("AVG", "price")]
     else:
         keys, aggs = ["cat"], [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")]
+    forced(jpath)
     g = eng.graph(tf)
     flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
     g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], jt)
@@ -263,7 +270,8 @@ TILED_SHAPES = {
 
 @pytest.mark.parametrize("shape", list(TILED_SHAPES))
 @pytest.mark.parametrize("resident", [False, True])
-def test_jg_tiled_shapes(eng, shape, resident):
+@pytest.mark.parametrize("jpath", [0, 3])
+def test_jg_tiled_shapes(eng, forced, shape, resident, jpath):
     """Fused JOIN -> GROUP through the tiled slot-map probe: stacked
     filters (a selection), OR / NOT and column-column predicates, 4-byte key
     and aggregate columns, probe keys missing from the build side, and a row
@@ -284,6 +292,7 @@ def test_jg_tiled_shapes(eng, shape, resident):
         eng.release(td)
         tf, td = tf2, td2
     preds, mask_fn, aggs = TILED_SHAPES[shape]
+    forced(jpath)
     g = eng.graph(tf)
     node = g.const_table(tf)
     for p in preds(g):
@@ -296,6 +305,10 @@ def test_jg_tiled_shapes(eng, shape, resident):
     eng.release(tf)
     eng.release(td)
     assert plan.startswith("join_group"), plan
+    if jpath == 3 and shape in ("two_filters", "i32_key"):
+        assert "path=3" in plan, plan          # the partitioned probe ran
+    # OR / NOT and column-column predicates are not range checks: the
+    # partitioned probe declines them and the direct-map probe runs
     m = mask_fn(f)
     fm = {c: a[m] for c, a in f.items()}
     li, ri = orc.join([fm["fk"]], [pk], 0)
