@@ -122,6 +122,209 @@ def calibrate_copy(e, nbytes):
     print(f"# calibrate-copy: {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+class JoinGroup:
+    """filter qty > 50 -> join fact x dim on fk = pk -> group by cat."""
+    name = "jg"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        self.e, self.a = e, a
+        t0 = time.perf_counter()
+        fact = W.join_fact(a.fact, a.dim, SEED)
+        dim = W.join_dim(a.dim, a.cats, SEED)
+        hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+        del fact
+        hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"])])
+        del dim
+        t1 = time.perf_counter()
+        self.tf, self.td = e.put(hf), e.put(hd)
+        e.L.td_dev_sync()
+        t2 = time.perf_counter()
+        e.release(hf)
+        e.release(hd)
+        gbs = (a.fact * 24 + a.dim * 12) / (t2 - t1) / 1e9
+        print(f"# data: gen {t1 - t0:.1f}s, h2d {t2 - t1:.2f}s ({gbs:.1f} GB/s)",
+              file=sys.stderr, flush=True)
+        self.rows = a.fact
+        self.query_bytes = a.fact * 24 + a.dim * 12 + a.cats * 28
+        self.config = {"workload": f"filter qty>50 -> join fact {a.fact:,} x dim {a.dim:,} on "
+                                   f"fk=pk -> group by cat ({a.cats} cats): SUM price, SUM qty, "
+                                   f"COUNT"}
+
+    def step(self):
+        e, tf, td = self.e, self.tf, self.td
+        g = e.graph(tf)
+        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
+        g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+        root = g.group([g.scan("cat")], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
+                                         ("COUNT", g.scan("price"))])
+        out = g.execute(root)
+        g.free()
+        e.release(out)
+
+    def kernel_bytes(self, name):
+        return kernel_bytes(name, self.a.fact, self.a.dim, self.a.cats)
+
+    def cpu_baseline(self):
+        nf, nd = min(CPU_SAMPLE[0], self.a.fact), min(CPU_SAMPLE[1], self.a.dim)
+        return cpu_baseline_reference(nf, nd, self.a.cats) or cpu_baseline_port(
+            min(nf, 5_000_000), min(nd, 400_000), self.a.cats)
+
+    def release(self):
+        self.e.release(self.tf)
+        self.e.release(self.td)
+
+
+HC_AGGS = [("SUM", "v_i64"), ("MIN", "v_i64"), ("MAX", "v_i64"), ("SUM", "v_f64"),
+           ("MIN", "v_f64"), ("MAX", "v_f64"), ("COUNT", "v_i64")]
+
+
+class HighCard:
+    """group by a Zipf(1.1) key over 64 M values and by a u32 symbol column
+    over 1 M symbols: SUM/MIN/MAX of an i64 and an f64 column, and COUNT."""
+    name = "hc"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        self.e, self.a = e, a
+        t0 = time.perf_counter()
+        d = W.high_card(a.hc_rows, a.hc_values, a.hc_syms, 1.1, SEED)
+        h = e.table([("key", d["key"]), ("sym", d["sym"], 20, 2), ("v_i64", d["v_i64"]),
+                     ("v_f64", d["v_f64"])])
+        del d
+        t1 = time.perf_counter()
+        self.t = e.put(h)
+        e.L.td_dev_sync()
+        e.release(h)
+        print(f"# data: gen {t1 - t0:.1f}s", file=sys.stderr, flush=True)
+        self.rows = 2 * a.hc_rows                       # two group-bys per step
+        out = self._group("key")
+        self.g_key = self.e.L.td_table_nrows(out)
+        self.e.release(out)
+        self.query_bytes = (a.hc_rows * 24 + self.g_key * 64) + (a.hc_rows * 20 + a.hc_syms * 60)
+        self.config = {"workload": f"group {a.hc_rows:,} rows by key (zipf 1.1 over "
+                                   f"{a.hc_values:,} values, {self.g_key:,} groups) and by sym "
+                                   f"({a.hc_syms:,} symbols): SUM/MIN/MAX v_i64, v_f64, COUNT"}
+
+    def _group(self, by):
+        e = self.e
+        g = e.graph(self.t)
+        root = g.group([g.scan(by)], [(op, g.scan(c)) for op, c in HC_AGGS])
+        out = g.execute(g.optimize(root))
+        g.free()
+        return out
+
+    def step(self):
+        self.e.release(self._group("key"))
+        self.e.release(self._group("sym"))
+
+    def kernel_bytes(self, name):
+        return None
+
+    def cpu_baseline(self):
+        exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+        if not os.path.exists(exe):
+            return None
+        n = min(self.a.hc_rows, 30_000_000)
+        nv = max(1, self.a.hc_values * n // self.a.hc_rows)
+        ns = max(1, self.a.hc_syms * n // self.a.hc_rows)
+        env = dict(os.environ, TD_REF_THREADS=str(cpu_threads()))
+        out = subprocess.run([exe, "bench", "hc", str(n), str(nv), str(ns), "3"], check=True,
+                             capture_output=True, text=True, env=env, timeout=900)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": 2 * n / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
+                "kind": "reference",
+                "sample": f"{n:,} rows, {nv:,} zipf values, {ns:,} symbols, both group-bys, "
+                          f"median of 3 after 1 warm-up"}
+
+    def release(self):
+        self.e.release(self.t)
+
+
+class BatchQueries:
+    """Many independent filter -> join -> group-by queries: fact rows
+    log-uniform in [1K, 2M], dim = fact / 8 with unique keys, group
+    cardinality log-uniform in [4, 65536], payloads as in the join workload.
+    The step runs all of them as one batch (prepared once, replayed as a
+    captured graph); the plain form (one td_execute per query) is timed
+    beside it."""
+    name = "batch"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        import numpy as np
+        self.e, self.a = e, a
+        rng = np.random.default_rng(SEED)
+        n = a.batch_queries
+        nf = np.exp(rng.uniform(np.log(1_000), np.log(2_000_000), n)).astype(np.int64)
+        ncat = np.exp(rng.uniform(np.log(4), np.log(65_536), n)).astype(np.int64)
+        thr = rng.integers(1, 101, n)
+        t0 = time.perf_counter()
+        self.tables, self.dags = [], []
+        qbytes = 0
+        for q in range(n):
+            nd = max(1, int(nf[q]) // 8)
+            fact = W.join_fact(int(nf[q]), nd, SEED + q)
+            dim = W.join_dim(nd, int(ncat[q]), SEED + q)
+            hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+            hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"])])
+            tf, td = e.put(hf), e.put(hd)
+            e.release(hf)
+            e.release(hd)
+            self.tables += [tf, td]
+            g = e.graph(tf)
+            flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(int(thr[q]))))
+            g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+            root = g.group([g.scan("cat")], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
+                                             ("COUNT", g.scan("price"))])
+            self.dags.append((g, root))
+            qbytes += int(nf[q]) * 24 + nd * 12 + min(nd, int(ncat[q])) * 28
+        print(f"# data: {n} queries, {int(nf.sum()):,} fact rows, {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+        self.n = n
+        self.rows = int(nf.sum())
+        self.query_bytes = qbytes
+        t0 = time.perf_counter()
+        self.h = e.batch_prepare([g for g, _ in self.dags], [r for _, r in self.dags])
+        self.prepare_s = time.perf_counter() - t0
+        nb, tr, bl = e.batch_info(self.h)
+        self.config = {"workload": f"{n} filter+join+group-by queries, fact rows log-uniform "
+                                   f"[1K,2M] (sum {self.rows:,}), dim = fact/8 unique keys, "
+                                   f"groups log-uniform [4,65536]; one batch, graph replay",
+                       "batched": nb, "tile_rows": tr, "block": bl,
+                       "prepare_s": round(self.prepare_s, 3)}
+
+    def step(self):
+        rc, res = self.e.batch_run(self.h, self.n)
+        if rc:
+            raise SystemExit(f"batch run failed: {rc}")
+        for r in res:
+            self.e.release(r)
+
+    def plain(self, reps):
+        e = self.e
+        e.L.td_dev_sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for g, root in self.dags:
+                e.release(g.execute(root))
+        e.L.td_dev_sync()
+        return (time.perf_counter() - t0) / reps
+
+    def kernel_bytes(self, name):
+        return self.query_bytes if name == "batch_probe" else None
+
+    def cpu_baseline(self):
+        return None
+
+    def release(self):
+        self.e.L.td_dev_batch_free(self.h)
+        for g, _ in self.dags:
+            g.free()
+        for t in self.tables:
+            self.e.release(t)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +334,13 @@ def main():
     ap.add_argument("--dim", type=int, default=N_DIM)
     ap.add_argument("--cats", type=int, default=N_CAT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["jg", "hc", "batch"], default="jg",
+                    help="jg: join + group-by (default); hc: high-cardinality group-by; "
+                         "batch: many small join + group-by queries per batch")
+    ap.add_argument("--batch-queries", type=int, default=4096)
+    ap.add_argument("--hc-rows", type=int, default=300_000_000)
+    ap.add_argument("--hc-values", type=int, default=64_000_000)
+    ap.add_argument("--hc-syms", type=int, default=1_000_000)
     ap.add_argument("--join-path", type=int, default=0,
                     help="0 planner's choice, 1 direct slot map, 3 radix-partitioned")
     ap.add_argument("--no-prof", action="store_true",
@@ -144,7 +354,6 @@ def main():
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
 
-    from teide_amd import workloads as W
     from teide_amd.td import engine
     e = engine()
     if e.L.td_dev_available() != 1:
@@ -154,35 +363,10 @@ def main():
         calibrate_copy(e, a.calibrate_copy << 20)
     e.force_paths(a.join_path, 0, 0)
 
-    t0 = time.perf_counter()
-    fact = W.join_fact(a.fact, a.dim, SEED)
-    dim = W.join_dim(a.dim, a.cats, SEED)
-    hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
-    del fact
-    hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"])])
-    del dim
-    t1 = time.perf_counter()
-    tf, td = e.put(hf), e.put(hd)
-    e.L.td_dev_sync()
-    t2 = time.perf_counter()
-    e.release(hf)
-    e.release(hd)
-    put_gbs = (a.fact * 24 + a.dim * 12) / (t2 - t1) / 1e9
-    print(f"# data: gen {t1 - t0:.1f}s, h2d {t2 - t1:.2f}s ({put_gbs:.1f} GB/s)",
-          file=sys.stderr, flush=True)
-
-    def step():
-        g = e.graph(tf)
-        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
-        g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
-        root = g.group([g.scan("cat")], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
-                                         ("COUNT", g.scan("price"))])
-        out = g.execute(root)
-        g.free()
-        return out
+    wl = {"hc": HighCard, "batch": BatchQueries}.get(a.workload, JoinGroup)(e, a)
 
     for _ in range(a.warmup):
-        e.release(step())
+        wl.step()
     plan = e.last_plan()
     e.L.td_dev_sync()
     e.prof_reset()
@@ -190,7 +374,7 @@ def main():
     e.L.td_dev_sync()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        e.release(step())
+        wl.step()
     e.L.td_dev_sync()
     t1 = time.perf_counter()
     prof = e.prof_read()
@@ -203,23 +387,24 @@ def main():
         dom = max(prof, key=lambda k: prof[k][1])
         n, tot = prof[dom]
         avg_ms = tot / n
-        nbytes = kernel_bytes(dom, a.fact, a.dim, a.cats)
+        nbytes = wl.kernel_bytes(dom)
         if nbytes:
             ach = nbytes / (avg_ms / 1e3) / 1e9
             roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": load_traffic(dom), "kernel": dom,
                         "algorithmic_bytes": nbytes, "avg_ms": round(avg_ms, 5)}
-    query_bytes = a.fact * 24 + a.dim * 12 + a.cats * 28
-    cpu = None
-    if not a.no_cpu_baseline:
-        nf, nd = CPU_SAMPLE
-        nf, nd = min(nf, a.fact), min(nd, a.dim)
-        cpu = cpu_baseline_reference(nf, nd, a.cats) or cpu_baseline_port(
-            min(nf, 5_000_000), min(nd, 400_000), a.cats)
+    cpu = None if a.no_cpu_baseline else wl.cpu_baseline()
+    query_gbs = wl.query_bytes / (ms / 1e3) / 1e9
+    extra = {}
+    if a.workload == "batch":
+        plain_s = wl.plain(1)
+        extra = {"batched_qps": round(wl.n / (ms / 1e3), 1),
+                 "plain_qps": round(wl.n / plain_s, 1),
+                 "batched_vs_plain": round((plain_s * 1e3) / ms, 2)}
     line = {
         "metric": "rows/s + % HBM peak for join+group-by",
-        "value": a.fact / (ms / 1e3),
+        "value": wl.rows / (ms / 1e3),
         "unit": "rows/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -230,17 +415,15 @@ def main():
         "vs_baseline": None,
         "dtype": "i64/f64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": f"filter qty>50 -> join fact {a.fact:,} x dim {a.dim:,} on fk=pk "
-                               f"-> group by cat ({a.cats} cats): SUM price, SUM qty, COUNT",
-                   "plan": plan},
-        "query_gbs": round(query_bytes / (ms / 1e3) / 1e9, 1),
-        "query_frac_hbm": round(query_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "config": dict(wl.config, plan=plan),
+        "query_gbs": round(query_gbs, 1),
+        "query_frac_hbm": round(query_gbs / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
-    e.release(tf)
-    e.release(td)
+    line.update(extra)
+    wl.release()
     print(json.dumps(line), flush=True)
 
 

@@ -55,9 +55,29 @@ void     td_dev_prof_enable(int on);
 void     td_dev_prof_reset(void);
 int      td_dev_prof_read(td_dev_prof_t* rows, int cap);
 
+/* Batched execution: many DAGs of the fused shape GROUP over the graph's
+ * inner JOIN(FILTER*(fact), dim), with resident inputs, run as one batch of
+ * launches (same results as td_execute on each; results are resident).
+ * DAGs the batch cannot take run through td_execute.  results[i] receives
+ * an owned table or an error pointer.  Returns TD_OK or the first error.
+ * No reference counterpart (the reference runs queries one at a time). */
+td_err_t td_dev_execute_batch(td_graph_t* const* graphs, td_op_t* const* roots, int64_t n,
+                              td_t** results);
+/* The same, prepared once and run many times: the first run captures the
+ * launch sequence as a hipGraph, later runs replay it.  The graphs and
+ * their tables must outlive the handle. */
+typedef struct td_dev_batch td_dev_batch_t;
+td_dev_batch_t* td_dev_batch_prepare(td_graph_t* const* graphs, td_op_t* const* roots,
+                                     int64_t n);
+td_err_t td_dev_batch_run(td_dev_batch_t* b, td_t** results);
+/* batched queries, autotuned probe tile rows and workgroup size */
+int64_t  td_dev_batch_info(const td_dev_batch_t* b, int64_t* tile_rows, int* block);
+void     td_dev_batch_free(td_dev_batch_t* b);
+
 /* Physical-plan hints for tests and measurement: force the join path
- * (0 auto, 1 direct map, 2 radix) / group path (0 auto, 1 direct array,
- * 2 hash) / radix bits (0 auto) for subsequent td_execute calls. */
+ * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group) / group
+ * path (0 auto, 1 direct array, 2 hash) / radix bits (0 auto) for
+ * subsequent td_execute calls. */
 void     td_dev_force_paths(int join_path, int group_path, int radix_bits);
 /* The physical plan of the last td_execute as a short text. */
 const char* td_dev_last_plan(void);

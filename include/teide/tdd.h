@@ -223,6 +223,25 @@ typedef struct {
 
 int tdd_join_group(const tdd_join_group_spec_t* spec, tdd_group_result_t* out);
 
+/* Batched fused filter -> join -> group-by: many queries of one output
+ * shape per launch sequence (see tdd_join_group for a single query).  No
+ * reference counterpart: the reference runs each query's two td_execute
+ * calls on its own.  prepare() plans the batch, checks the build keys of
+ * every query for uniqueness and (tune != 0) times candidate probe launch
+ * shapes once per batch-size class; run() executes it (use_graph: capture
+ * the launch sequence as a hipGraph on the first run and replay it after),
+ * writing every batched query's groups into one output block per column,
+ * query q's rows at [offsets[q], offsets[q+1]).  status(q) != 0 marks a
+ * query the batch could not take (run it with tdd_join_group). */
+typedef struct tdd_jg_batch tdd_jg_batch_t;
+int  tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, int n, int tune,
+                          tdd_jg_batch_t** out);
+int  tdd_jg_batch_status(const tdd_jg_batch_t* b, int q);
+int  tdd_jg_batch_shape(const tdd_jg_batch_t* b, int64_t* tile_rows, int* block);
+int  tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
+                      int64_t* offsets);
+void tdd_jg_batch_free(tdd_jg_batch_t* b);
+
 /* ------------------------------------------------- host services (C side) -- */
 /* Implemented by the host engine (arena.c); the device layer allocates its
  * scratch and result blocks from the HBM arena through these. */

@@ -1,0 +1,566 @@
+/*
+ * k_batch.hip -- many fused filter -> join -> group-by queries per launch.
+ *
+ * A batch holds queries of one shape (same group keys, aggregate ops and
+ * column types); each has its own probe and build tables, predicate
+ * constants and key ranges.  Every phase is one launch over a flat list of
+ * work tiles (query, row range), so a batch of thousands of small queries
+ * costs a handful of launches instead of thousands:
+ *
+ *   map      one fill of all slot maps, one scatter of every build row's
+ *            group slot, one count of occupied entries per query (unique
+ *            build keys <=> count == build rows);
+ *   probe    a persistent grid walks the probe tiles: star predicate, slot
+ *            map gather, and a fold into the query's group table -- in LDS
+ *            per tile when the table is small (flushed with one atomic per
+ *            touched cell), else straight into the global table;
+ *   emit     one workgroup per query counts non-empty slots, then writes its
+ *            groups in ascending key order into shared output columns at the
+ *            query's offset.
+ *
+ * After the first run the launch sequence (which needs no host decisions
+ * any more) is captured as a hipGraph and replayed.  Tile size and block
+ * size come from a small autotuner that times the probe launch on the
+ * batch itself the first time a batch of a given size class is prepared.
+ */
+#include "group_plan.h"
+#include "tile.h"
+#include "star.h"
+
+#include <map>
+#include <vector>
+
+#define BQ_R 4                      /* rows per lane per tile step */
+#define BQ_LDS_MAX (48 * 1024)      /* per-tile LDS group table limit */
+
+/* Per-query device descriptor. */
+struct BQuery {
+    GroupPlan P;                    /* build-side keys, probe-side aggregates */
+    TileCols  T;                    /* key slot 0, predicate slots, aggregate sources */
+    StarPred  SP;
+    uint8_t   cell_slot[GP_MAX_CELLS];
+    int32_t   map_u16, lds;
+    int32_t   active, pad;          /* 1: batched (else run on its own) */
+    int64_t   nl, nr;
+    int64_t   kmin, range;          /* build key bounds */
+    tdd_col_t rk;
+    int64_t   map_off;              /* bytes into the map arena */
+    int64_t   tab_off;              /* cells into the table arena */
+    int64_t   n_slots;
+};
+
+struct BTile {
+    int32_t q, pad;
+    int64_t r0, r1;                 /* row range within the query */
+};
+
+/* ---------------------------------------------------------------- maps -- */
+
+__global__ __launch_bounds__(256) void k_bq_scatter(const BQuery* Q, const BTile* tiles,
+                                                    int64_t n_tiles, uint8_t* maps) {
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        for (int64_t r = W.r0 + threadIdx.x; r < W.r1; r += blockDim.x) {
+            const int64_t i = ld_i64(q.rk, r) - q.kmin;
+            int64_t slot = 0;
+            for (int k = 0; k < q.P.n_keys; k++)
+                slot += (ld_i64(q.P.keys[k], r) - q.P.kmin[k]) * q.P.kstride[k];
+            if (q.map_u16) ((uint16_t*)(maps + q.map_off))[i] = (uint16_t)slot;
+            else ((int32_t*)(maps + q.map_off))[i] = (int32_t)slot;
+        }
+    }
+}
+
+/* Occupied entries per query, over tiles of map entries. */
+__global__ __launch_bounds__(256) void k_bq_count(const BQuery* Q, const BTile* tiles,
+                                                  int64_t n_tiles, const uint8_t* maps,
+                                                  unsigned long long* occupied) {
+    __shared__ unsigned long long ws[256 / WAVE];
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        unsigned long long c = 0;
+        for (int64_t i = W.r0 + threadIdx.x; i < W.r1; i += blockDim.x)
+            c += q.map_u16 ? ((const uint16_t*)(maps + q.map_off))[i] != 0xffff
+                           : ((const int32_t*)(maps + q.map_off))[i] != -1;
+        for (int d = WAVE / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, WAVE);
+        if (lane_id() == 0) ws[threadIdx.x / WAVE] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long s = 0;
+            for (int w = 0; w < 256 / WAVE; w++) s += ws[w];
+            if (s) atomicAdd(&occupied[W.q], s);
+        }
+        __syncthreads();
+    }
+}
+
+/* Identity-initialise every query's group table. */
+__global__ __launch_bounds__(256) void k_bq_init(const BQuery* Q, const BTile* tiles,
+                                                 int64_t n_tiles, uint64_t* tabs) {
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        for (int64_t i = W.r0 + threadIdx.x; i < W.r1; i += blockDim.x)
+            tabs[q.tab_off + i] = cell_identity(q.P.kind[i % q.P.ncells]);
+    }
+}
+
+/* --------------------------------------------------------------- probe -- */
+
+__device__ __forceinline__ uint32_t bq_pred(const BQuery& q, const int64_t (&cv)[TL_NC][BQ_R]) {
+    uint32_t m = (1u << BQ_R) - 1;
+    if (q.SP.npred >= 1) m &= range_mask<BQ_R>(q.SP, 0, q.T.col[1].type == TD_F64, cv[1]);
+    if (q.SP.npred >= 2) m &= range_mask<BQ_R>(q.SP, 1, q.T.col[2].type == TD_F64, cv[2]);
+    return m;
+}
+
+/* Fold a cell table `src` (LDS) into the query's global table. */
+__device__ __forceinline__ void bq_flush(const GroupPlan& P, const uint64_t* src, uint64_t* dst,
+                                         int64_t n_slots) {
+    for (int64_t s = threadIdx.x; s < n_slots; s += blockDim.x) {
+        const uint64_t* c = src + s * P.ncells;
+        if (c[0] == 0) continue;
+        cells_merge(dst + s * P.ncells, c, P);
+    }
+}
+
+__global__ __launch_bounds__(512) void k_bq_probe(const BQuery* Q, const BTile* tiles,
+                                                  int64_t n_tiles, const uint8_t* maps,
+                                                  uint64_t* tabs) {
+    extern __shared__ uint64_t lt[];
+    const int lane = lane_id();
+    const int nwaves = blockDim.x / WAVE, wv = threadIdx.x / WAVE;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        const GroupPlan& P = q.P;
+        uint64_t* gtab = tabs + q.tab_off;
+        const int64_t ncell_total = q.n_slots * P.ncells;
+        uint64_t* tab = q.lds ? lt : gtab;
+        if (q.lds) {
+            for (int64_t i = threadIdx.x; i < ncell_total; i += blockDim.x)
+                lt[i] = cell_identity(P.kind[i % P.ncells]);
+            __syncthreads();
+        }
+        const uint16_t* m16 = (const uint16_t*)(maps + q.map_off);
+        const int32_t* m32 = (const int32_t*)(maps + q.map_off);
+        for (int64_t base = W.r0 + ((int64_t)wv * WAVE + lane) * BQ_R; base < W.r1;
+             base += (int64_t)nwaves * WAVE * BQ_R) {
+            int64_t cv[TL_NC][BQ_R];
+            load_tile<BQ_R>(q.T, base, W.r1, cv);
+            const uint32_t m = tile_live<BQ_R>(nullptr, base, W.r1) & bq_pred(q, cv);
+            int32_t s[BQ_R];
+#pragma unroll
+            for (int j = 0; j < BQ_R; j++) {
+                s[j] = -1;
+                const uint64_t i = (uint64_t)(cv[0][j] - q.kmin);
+                if (((m >> j) & 1) && i < (uint64_t)q.range) {
+                    if (q.map_u16) {
+                        uint16_t v = m16[i];
+                        s[j] = v == 0xffff ? -1 : (int32_t)v;
+                    } else {
+                        s[j] = m32[i];
+                    }
+                }
+            }
+            acc_tile<BQ_R>(tab, P, q.cell_slot, q.T, cv, s);
+        }
+        if (q.lds) {
+            __syncthreads();
+            bq_flush(P, lt, gtab, q.n_slots);
+            __syncthreads();
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- emit -- */
+
+/* One workgroup per query: non-empty slots (count cell > 0). */
+__global__ __launch_bounds__(256) void k_bq_ngroups(const BQuery* Q, const uint64_t* tabs,
+                                                    int64_t* ngroups) {
+    __shared__ int64_t ws[256 / WAVE];
+    const BQuery& q = Q[blockIdx.x];
+    const uint64_t* tab = tabs + q.tab_off;
+    int64_t c = 0;
+    if (q.active)
+        for (int64_t s = threadIdx.x; s < q.n_slots; s += 256) c += tab[s * q.P.ncells] != 0;
+    for (int d = WAVE / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, WAVE);
+    if (lane_id() == 0) ws[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = 0;
+        for (int w = 0; w < 256 / WAVE; w++) s += ws[w];
+        ngroups[blockIdx.x] = s;
+    }
+}
+
+/* One workgroup per query: groups in ascending slot (= key) order at the
+ * query's output offset. */
+__global__ __launch_bounds__(256) void k_bq_emit(const BQuery* Q, const uint64_t* tabs,
+                                                 const int64_t* offs, tdd_group_result_t out) {
+    __shared__ int64_t ws[256 / WAVE];
+    __shared__ int64_t carry;
+    const BQuery& q = Q[blockIdx.x];
+    if (!q.active) return;
+    const GroupPlan& P = q.P;
+    const uint64_t* tab = tabs + q.tab_off;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t s0 = 0; s0 < q.n_slots; s0 += 256) {
+        const int64_t s = s0 + threadIdx.x;
+        const bool live = s < q.n_slots && tab[s * P.ncells] != 0;
+        const uint64_t bal = __ballot(live);
+        const int64_t below = lanes_below(bal);
+        if (lane_id() == 0) ws[threadIdx.x / WAVE] = __popcll(bal);
+        __syncthreads();
+        int64_t off = carry + below;
+        for (int w = 0; w < (int)threadIdx.x / WAVE; w++) off += ws[w];
+        if (live) {
+            const int64_t g = offs[blockIdx.x] + off;
+            const uint64_t* c = tab + s * P.ncells;
+            for (int k = 0; k < P.n_keys; k++) {
+                const int64_t v = P.kmin[k] + (s / P.kstride[k]) % P.krange[k];
+                st_int(out.keys[k], g, P.keys[k].esz, v);
+            }
+            for (int a = 0; a < P.n_aggs; a++) cells_emit(c, P, a, out.vals[a], g);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t tot = 0;
+            for (int w = 0; w < 256 / WAVE; w++) tot += ws[w];
+            carry += tot;
+        }
+        __syncthreads();
+    }
+}
+
+/* -------------------------------------------------------------- driver -- */
+
+struct tdd_jg_batch {
+    int n;
+    std::vector<BQuery> hq;
+    std::vector<int> status;                /* per query: 0 batched, else TD_ERR_NYI */
+    void* dq = nullptr;                     /* BQuery[n] */
+    void* maps = nullptr;
+    void* tabs = nullptr;
+    void* occ = nullptr;                    /* u64[n] */
+    void* ng = nullptr;                     /* i64[n] */
+    void* offs = nullptr;                   /* i64[n + 1] */
+    void* tl_build = nullptr; int64_t n_build = 0;
+    void* tl_map = nullptr;   int64_t n_map = 0;
+    void* tl_tab = nullptr;   int64_t n_tab = 0;
+    void* tl_probe = nullptr; int64_t n_probe = 0;
+    size_t map_bytes = 0, tab_cells = 0;
+    int grid_probe = 0, block_probe = 512;
+    size_t lds_probe = 0;
+    int64_t tile_rows = 16384;
+    GroupPlan shape;                        /* output types of the shape */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void bq_release(tdd_jg_batch* b) {
+    void* ps[] = {b->dq, b->maps, b->tabs, b->occ, b->ng, b->offs, b->tl_build, b->tl_map,
+                  b->tl_tab, b->tl_probe};
+    for (void* p : ps) if (p) tdd::release(p);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+}
+
+static int upload(void** dst, const void* src, size_t bytes) {
+    *dst = tdd::scratch(bytes ? bytes : 8);
+    if (!*dst) return TD_ERR_OOM;
+    if (bytes) HC(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, tdd::g_stream));
+    return TD_OK;
+}
+
+/* Tiles of `chunk` rows over each query's range [0, n(q)). */
+template <class F>
+static std::vector<BTile> make_tiles(int n, int64_t chunk, F count) {
+    std::vector<BTile> v;
+    for (int q = 0; q < n; q++) {
+        const int64_t m = count(q);
+        for (int64_t r = 0; r < m; r += chunk) v.push_back(BTile{q, 0, r, r + chunk < m ? r + chunk : m});
+    }
+    return v;
+}
+
+/* Launch sequence of one run (capturable: kernels only). */
+static int bq_launch(tdd_jg_batch* b) {
+    int rc = tdd_fill(b->maps, ~0ull, b->map_bytes);
+    if (rc) return rc;
+    const BQuery* Q = (const BQuery*)b->dq;
+    if (b->n_build)
+        LAUNCH("batch_map", k_bq_scatter, stream_grid(b->n_build, 8), 256, 0, Q,
+               (const BTile*)b->tl_build, b->n_build, (uint8_t*)b->maps);
+    if (b->n_tab)
+        LAUNCH("batch_init", k_bq_init, stream_grid(b->n_tab, 8), 256, 0, Q, (const BTile*)b->tl_tab,
+               b->n_tab, (uint64_t*)b->tabs);
+    if (b->n_probe)
+        LAUNCH("batch_probe", k_bq_probe, b->grid_probe, b->block_probe, b->lds_probe, Q,
+               (const BTile*)b->tl_probe, b->n_probe, (const uint8_t*)b->maps, (uint64_t*)b->tabs);
+    return TD_OK;
+}
+
+/* Autotuned probe launch shape, per size class (log2 of the probe rows). */
+static std::map<int, std::pair<int64_t, int>> g_bq_tuned;
+
+static int bq_tiles_for(tdd_jg_batch* b, int64_t chunk) {
+    if (b->tl_probe) { tdd::release(b->tl_probe); b->tl_probe = nullptr; }
+    std::vector<BTile> tp = make_tiles(b->n, chunk, [&](int q) {
+        return b->status[q] ? (int64_t)0 : b->hq[q].nl;
+    });
+    b->n_probe = (int64_t)tp.size();
+    b->tile_rows = chunk;
+    return upload(&b->tl_probe, tp.data(), tp.size() * sizeof(BTile));
+}
+
+static int bq_autotune(tdd_jg_batch* b, int64_t total_rows) {
+    int cls = 0;
+    while ((int64_t)1 << (cls + 1) <= total_rows) cls++;
+    auto it = g_bq_tuned.find(cls);
+    if (it != g_bq_tuned.end()) {
+        b->block_probe = it->second.second;
+        return bq_tiles_for(b, it->second.first);
+    }
+    const int64_t chunks[] = {4096, 16384, 65536};
+    const int blocks[] = {256, 512};
+    float best = 1e30f;
+    int64_t best_c = 16384;
+    int best_b = 512;
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    for (int64_t c : chunks) {
+        int rc = bq_tiles_for(b, c);
+        if (rc) return rc;
+        for (int bs : blocks) {
+            b->block_probe = bs;
+            b->grid_probe = resident_grid(k_bq_probe, bs, b->lds_probe, b->n_probe);
+            rc = bq_launch(b);                     /* warm + reset tables */
+            if (rc) return rc;
+            HC(hipEventRecord(e0, tdd::g_stream));
+            rc = bq_launch(b);
+            if (rc) return rc;
+            HC(hipEventRecord(e1, tdd::g_stream));
+            HC(hipEventSynchronize(e1));
+            float ms = 0;
+            HC(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_c = c; best_b = bs; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_bq_tuned[cls] = {best_c, best_b};
+    b->block_probe = best_b;
+    return bq_tiles_for(b, best_c);
+}
+
+extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, int n,
+                                    int tune, tdd_jg_batch_t** out) {
+    *out = nullptr;
+    if (n <= 0) return TD_ERR_RANGE;
+    tdd_jg_batch* b = new tdd_jg_batch();
+    b->n = n;
+    b->hq.resize(n);
+    b->status.assign(n, 0);
+    size_t map_bytes = 0, tab_cells = 0;
+    size_t lds_max = 0;
+    int64_t total_rows = 0;
+    int shape_q = -1;
+    for (int i = 0; i < n; i++) {
+        const tdd_join_group_spec_t* S = specs[i];
+        const tdd_join_spec_t& J = S->join;
+        BQuery& q = b->hq[i];
+        memset(&q, 0, sizeof(q));
+        int st = TD_OK;
+        bool ok = J.join_type == 0 && J.n_keys == 1 && !J.lsel && J.lkeys[0].type != TD_F64 &&
+                  J.rkeys[0].type != TD_F64 && J.rkeys[0].ranged && S->n_keys >= 1 &&
+                  S->n_aggs <= TDD_MAX_AGGS;
+        GroupPlan& P = q.P;
+        if (ok) {
+            P.n_keys = S->n_keys;
+            P.n_aggs = S->n_aggs;
+            for (int k = 0; k < P.n_keys; k++) {
+                P.keys[k] = S->keys[k];
+                P.key_side[k] = S->key_side[k];
+                ok = ok && S->key_side[k] == 1 && S->keys[k].ranged && S->keys[k].type != TD_F64;
+            }
+            for (int a = 0; a < P.n_aggs; a++) {
+                P.aggs[a] = S->aggs[a];
+                P.ops[a] = S->ops[a];
+                P.agg_side[a] = S->agg_side[a];
+                ok = ok && S->agg_side[a] == 0 && S->ops[a] != OP_FIRST && S->ops[a] != OP_LAST;
+            }
+        }
+        if (ok) ok = plan_cells(P) == TD_OK;
+        if (ok) {
+            double slots = 1;
+            for (int k = 0; k < P.n_keys; k++) {
+                P.kmin[k] = P.keys[k].lo;
+                P.krange[k] = P.keys[k].hi - P.keys[k].lo + 1;
+                slots *= (double)P.krange[k];
+            }
+            int64_t st1 = 1;
+            for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st1; st1 *= P.krange[k]; }
+            q.n_slots = (int64_t)slots;
+            q.kmin = J.rkeys[0].lo;
+            q.range = J.rkeys[0].hi - J.rkeys[0].lo + 1;
+            ok = slots <= (double)(1 << 24) && q.range > 0 && q.range <= 4 * J.nr + 4096 &&
+                 q.range < ((int64_t)1 << 31);
+        }
+        {
+            if (ok) ok = star_plan(J.lkeys[0], S->lpred, q.T, q.SP);
+            int agg_slot[TDD_MAX_AGGS] = {};
+            for (int a = 0; ok && a < P.n_aggs; a++) {
+                if (!P.aggs[a].data) continue;
+                int c = tile_slot(q.T, P.aggs[a]);
+                if (c < 0) ok = false;
+                else agg_slot[a] = c;
+            }
+            for (int c = 1; ok && c < P.ncells; c++) q.cell_slot[c] = (uint8_t)agg_slot[P.col[c]];
+        }
+        if (ok && shape_q >= 0) {                   /* one output shape per batch */
+            const GroupPlan& P0 = b->hq[shape_q].P;
+            ok = P0.n_keys == P.n_keys && P0.n_aggs == P.n_aggs;
+            for (int k = 0; ok && k < P.n_keys; k++) ok = P0.keys[k].esz == P.keys[k].esz;
+            for (int a = 0; ok && a < P.n_aggs; a++)
+                ok = P0.ops[a] == P.ops[a] && P0.out_types[a] == P.out_types[a] &&
+                     P0.aggs[a].esz == P.aggs[a].esz;
+        }
+        if (!ok) st = TD_ERR_NYI;
+        b->status[i] = st;
+        if (st) continue;
+        if (shape_q < 0) shape_q = i;
+        q.active = 1;
+        q.nl = J.nl;
+        q.nr = J.nr;
+        q.rk = J.rkeys[0];
+        q.map_u16 = q.n_slots < 0xffff;
+        q.map_off = (int64_t)map_bytes;
+        map_bytes += (((size_t)q.range * (q.map_u16 ? 2 : 4)) + 255) & ~(size_t)255;
+        q.tab_off = (int64_t)tab_cells;
+        tab_cells += (size_t)q.n_slots * P.ncells;
+        const size_t tb = (size_t)q.n_slots * P.ncells * 8;
+        q.lds = tb <= BQ_LDS_MAX;
+        if (q.lds && tb > lds_max) lds_max = tb;
+        total_rows += q.nl;
+    }
+    if (shape_q < 0) { delete b; return TD_ERR_NYI; }   /* nothing to batch */
+    b->shape = b->hq[shape_q].P;
+    b->map_bytes = map_bytes ? map_bytes : 256;
+    b->tab_cells = tab_cells ? tab_cells : 1;
+    b->lds_probe = lds_max;
+    int rc = TD_OK;
+    b->maps = tdd::scratch(b->map_bytes);
+    b->tabs = tdd::scratch(b->tab_cells * 8);
+    b->occ = tdd::scratch((size_t)n * 8);
+    b->ng = tdd::scratch((size_t)n * 8);
+    b->offs = tdd::scratch((size_t)(n + 1) * 8);
+    if (!b->maps || !b->tabs || !b->occ || !b->ng || !b->offs) rc = TD_ERR_OOM;
+    std::vector<BTile> tb_ = make_tiles(n, 16384, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].nr; });
+    std::vector<BTile> tm_ = make_tiles(n, 65536, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].range; });
+    std::vector<BTile> tt_ = make_tiles(n, 65536, [&](int q) {
+        return b->status[q] ? (int64_t)0 : b->hq[q].n_slots * b->hq[q].P.ncells;
+    });
+    b->n_build = (int64_t)tb_.size();
+    b->n_map = (int64_t)tm_.size();
+    b->n_tab = (int64_t)tt_.size();
+    if (!rc) rc = upload(&b->dq, b->hq.data(), sizeof(BQuery) * n);
+    if (!rc) rc = upload(&b->tl_build, tb_.data(), tb_.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_map, tm_.data(), tm_.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_tab, tt_.data(), tt_.size() * sizeof(BTile));
+    if (!rc) rc = bq_tiles_for(b, 16384);
+    if (!rc) b->grid_probe = resident_grid(k_bq_probe, b->block_probe, b->lds_probe, b->n_probe);
+    /* unique build keys: one map pass and a count, checked on the host */
+    if (!rc) rc = tdd_fill(b->maps, ~0ull, b->map_bytes);
+    if (!rc) rc = tdd_fill(b->occ, 0, (size_t)n * 8);
+    if (!rc && b->n_build)
+        LAUNCH("batch_map", k_bq_scatter, stream_grid(b->n_build, 8), 256, 0, (const BQuery*)b->dq,
+               (const BTile*)b->tl_build, b->n_build, (uint8_t*)b->maps);
+    if (!rc && b->n_map)
+        LAUNCH("batch_map", k_bq_count, stream_grid(b->n_map, 8), 256, 0, (const BQuery*)b->dq,
+               (const BTile*)b->tl_map, b->n_map, (const uint8_t*)b->maps,
+               (unsigned long long*)b->occ);
+    if (!rc) {
+        std::vector<uint64_t> occ(n);
+        HC(hipMemcpyAsync(occ.data(), b->occ, (size_t)n * 8, hipMemcpyDeviceToHost, tdd::g_stream));
+        HC(hipStreamSynchronize(tdd::g_stream));
+        bool any_dup = false;
+        for (int i = 0; i < n; i++)
+            if (!b->status[i] && (int64_t)occ[i] != b->hq[i].nr) { b->status[i] = TD_ERR_NYI; any_dup = true; }
+        if (any_dup) {                               /* drop them from the work lists */
+            std::vector<BTile> a2 = make_tiles(n, 16384, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].nr; });
+            std::vector<BTile> c2 = make_tiles(n, 65536, [&](int q) {
+                return b->status[q] ? (int64_t)0 : b->hq[q].n_slots * b->hq[q].P.ncells;
+            });
+            tdd::release(b->tl_build);
+            tdd::release(b->tl_tab);
+            b->tl_build = b->tl_tab = nullptr;
+            b->n_build = (int64_t)a2.size();
+            b->n_tab = (int64_t)c2.size();
+            rc = upload(&b->tl_build, a2.data(), a2.size() * sizeof(BTile));
+            if (!rc) rc = upload(&b->tl_tab, c2.data(), c2.size() * sizeof(BTile));
+            if (!rc) rc = bq_tiles_for(b, 16384);
+        }
+    }
+    if (!rc && tune) rc = bq_autotune(b, total_rows);
+    if (!rc) b->grid_probe = resident_grid(k_bq_probe, b->block_probe, b->lds_probe, b->n_probe);
+    if (rc) { bq_release(b); delete b; return rc; }
+    *out = b;
+    return TD_OK;
+}
+
+extern "C" int tdd_jg_batch_status(const tdd_jg_batch_t* b, int i) { return b->status[i]; }
+
+extern "C" int tdd_jg_batch_shape(const tdd_jg_batch_t* b, int64_t* tile_rows, int* block) {
+    *tile_rows = b->tile_rows;
+    *block = b->block_probe;
+    return TD_OK;
+}
+
+/* One run: replays the captured graph after the first run.  Outputs: one
+ * arena block per output column holding every batched query's groups;
+ * offsets[q] / offsets[q + 1] bound query q's rows (n + 1 entries). */
+extern "C" int tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
+                                int64_t* offsets) {
+    memset(out, 0, sizeof(*out));
+    int rc;
+    if (use_graph && b->exec) {
+        HC(hipGraphLaunch(b->exec, tdd::g_stream));
+    } else if (use_graph) {
+        HC(hipStreamBeginCapture(tdd::g_stream, hipStreamCaptureModeThreadLocal));
+        rc = bq_launch(b);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(tdd::g_stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        HC(ce);
+        b->graph = g;
+        HC(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+        HC(hipGraphLaunch(b->exec, tdd::g_stream));
+    } else {
+        rc = bq_launch(b);
+        if (rc) return rc;
+    }
+    const int n = b->n;
+    LAUNCH("batch_emit", k_bq_ngroups, n, 256, 0, (const BQuery*)b->dq, (const uint64_t*)b->tabs,
+           (int64_t*)b->ng);
+    rc = tdd::scan_i64((const int64_t*)b->ng, n, (int64_t*)b->offs);
+    if (rc) return rc;
+    HC(hipMemcpyAsync(offsets, b->offs, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    rc = alloc_group_outputs(b->shape, offsets[n], out);
+    if (rc) return rc;
+    LAUNCH("batch_emit", k_bq_emit, n, 256, 0, (const BQuery*)b->dq, (const uint64_t*)b->tabs,
+           (const int64_t*)b->offs, *out);
+    HC(hipStreamSynchronize(tdd::g_stream));
+    return TD_OK;
+}
+
+extern "C" void tdd_jg_batch_free(tdd_jg_batch_t* b) {
+    if (!b) return;
+    bq_release(b);
+    delete b;
+}

@@ -253,6 +253,25 @@ td_t* td__hbm_wrap(int8_t type, uint8_t attrs, int64_t len, void* dptr, size_t b
 }
 
 void td__hbm_free(td_t* v) {
-    td__arena_free(td_hbm_ref(v)->dptr);
-    td_hbm_ref(v)->dptr = NULL;
+    td_hbm_ref_t* r = td_hbm_ref(v);
+    if (r->owner) td_release(r->owner);
+    else td__arena_free(r->dptr);
+    r->dptr = NULL;
+    r->owner = NULL;
+}
+
+td_t* td__hbm_view(td_t* owner, int8_t type, uint8_t attrs, int64_t len, size_t off,
+                   size_t bytes) {
+    td_t* v = td_alloc(sizeof(td_hbm_ref_t));
+    if (!v) return TD_ERR_PTR(TD_ERR_OOM);
+    v->mmod = TD_MMOD_HBM;
+    v->type = type;
+    v->attrs = attrs;
+    v->len = len;
+    memset(td_hbm_ref(v), 0, sizeof(td_hbm_ref_t));
+    td_hbm_ref(v)->dptr = (char*)td_hbm_ref(owner)->dptr + off;
+    td_hbm_ref(v)->bytes = bytes;
+    td_retain(owner);
+    td_hbm_ref(v)->owner = owner;
+    return v;
 }

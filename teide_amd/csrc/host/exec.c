@@ -992,3 +992,242 @@ td_t* td_execute(td_graph_t* g, td_op_t* root) {
     drop_tmps(&x);
     return r;
 }
+
+/* ------------------------------------------------------ batched queries -- */
+
+typedef struct {
+    tdd_join_group_spec_t S;
+    tdd_pred_t   P;
+    td_t*        key_src[TDD_MAX_KEYS];
+    td_t*        agg_src[TDD_MAX_AGGS];
+    td_op_ext_t* ext;
+    int          ok;
+} jg_item_t;
+
+/* The spec of a DAG the batch can take: root GROUP over the graph's inner
+ * JOIN of FILTER*(CONST fact) (one predicate at most) with CONST dim, no
+ * pending selection, every referenced column resident (nothing uploaded). */
+static int jg_plan(td_graph_t* g, td_op_t* root, jg_item_t* it) {
+    memset(it, 0, sizeof(*it));
+    if (!g || !root || root->opcode != OP_GROUP || g->selection || !g->table) return TD_ERR_NYI;
+    td_op_ext_t* ext = td__find_ext(g, root->id);
+    if (!ext || ext->n_keys < 1 || ext->n_keys > TDD_MAX_KEYS || ext->n_aggs > TDD_MAX_AGGS)
+        return TD_ERR_NYI;
+    td_op_t* jop = NULL;
+    for (uint32_t i = 0; i < g->node_count && !jop; i++)
+        if (g->nodes[i].opcode == OP_JOIN) jop = &g->nodes[i];
+    if (!jop) return TD_ERR_NYI;
+    td_op_ext_t* je = td__find_ext(g, jop->id);
+    td_op_t* preds[8];
+    td_op_t* rpreds[8];
+    int npred = 0, nrp = 0;
+    td_t* left = const_table_under(g, jop->inputs[0], preds, &npred);
+    td_t* right = const_table_under(g, jop->inputs[1], rpreds, &nrp);
+    if (!je || je->join.join_type != 0 || !left || !right || nrp || npred > 1)
+        return TD_ERR_NYI;
+    if (npred && td_table_nrows(g->table) != td_table_nrows(left)) return TD_ERR_NYI;
+    xctx_t x;
+    memset(&x, 0, sizeof(x));
+    x.g = g;
+    x.all_resident = 1;
+    tdd_join_group_spec_t* S = &it->S;
+    S->join.nl = td_table_nrows(left);
+    S->join.nr = td_table_nrows(right);
+    int rc = join_keys(&x, je, left, right, &S->join);
+    if (!rc && npred) rc = compile_pred(&x, g->table, preds[0], &it->P);
+    if (!rc && npred) S->lpred = &it->P;
+    S->n_keys = ext->n_keys;
+    S->n_aggs = ext->n_aggs;
+    for (int i = 0; i < ext->n_keys + ext->n_aggs && !rc; i++) {
+        int is_key = i < ext->n_keys;
+        td_op_t* in = is_key ? ext->keys[i] : ext->agg_ins[i - ext->n_keys];
+        td_t* col = NULL;
+        int side = 0;
+        if (in->opcode == OP_SCAN) {
+            td_op_ext_t* se = td__find_ext(g, in->id);
+            col = td_table_get_col(left, se->sym);
+            if (!col && !join_is_right_key(g, je, se->sym)) { col = td_table_get_col(right, se->sym); side = 1; }
+            if (!col) rc = TD_ERR_SCHEMA;
+        } else if (is_key || !(in->opcode == OP_CONST && ext->agg_ops[i - ext->n_keys] == OP_COUNT)) {
+            rc = TD_ERR_NYI;
+        }
+        if (rc) break;
+        if (is_key) {
+            it->key_src[i] = col;
+            S->key_side[i] = (uint8_t)side;
+            rc = dev_col(&x, col, &S->keys[i]);
+        } else {
+            int a = i - ext->n_keys;
+            it->agg_src[a] = col;
+            S->ops[a] = ext->agg_ops[a];
+            S->agg_side[a] = (uint8_t)side;
+            if (col) rc = dev_col(&x, col, &S->aggs[a]);
+        }
+    }
+    int uploaded = x.ntmp > 0 || x.nup > 0 || !x.all_resident;
+    drop_tmps(&x);
+    if (!rc && uploaded) rc = TD_ERR_NYI;
+    it->ext = ext;
+    it->ok = rc == 0;
+    return rc;
+}
+
+/* Result table of batched query i: views into the batch's shared output
+ * blocks (owners) at rows [off, off + n). */
+static td_t* jg_view_table(td_graph_t* g, const jg_item_t* it, td_t* const* owners, int64_t off,
+                           int64_t n) {
+    td_op_ext_t* ext = it->ext;
+    int nk = ext->n_keys, na = ext->n_aggs;
+    td_t* out = td_table_new(nk + na);
+    for (int c = 0; c < nk + na && !TD_IS_ERR(out); c++) {
+        td_t* own = owners[c];
+        td_t* src = c < nk ? it->key_src[c] : it->agg_src[c - nk];
+        int8_t t = own->type;
+        uint8_t attrs = own->attrs;
+        size_t esz = td__esz(own);
+        td_t* v = td__hbm_view(own, t, attrs, n, (size_t)off * esz, (size_t)n * esz);
+        (void)src;
+        if (TD_IS_ERR(v)) { td_release(out); return v; }
+        int64_t name = c < nk ? key_name(g, ext->keys[c], c)
+                              : agg_name(g, ext->agg_ins[c - nk], c - nk, ext->agg_ops[c - nk]);
+        out = td_table_add_col(out, name, v);
+        td_release(v);
+    }
+    return out;
+}
+
+struct td_dev_batch {
+    int64_t          n;
+    td_graph_t**     graphs;
+    td_op_t**        roots;
+    jg_item_t*       items;
+    int*             bidx;        /* query -> batch index, -1: runs alone */
+    int              nb;
+    tdd_jg_batch_t*  dev;
+};
+
+td_dev_batch_t* td_dev_batch_prepare(td_graph_t* const* graphs, td_op_t* const* roots,
+                                     int64_t n) {
+    if (n <= 0 || n > INT32_MAX || !td__dev_ready()) return NULL;
+    td_dev_batch_t* b = calloc(1, sizeof(*b));
+    if (!b) return NULL;
+    b->n = n;
+    b->graphs = malloc(sizeof(td_graph_t*) * (size_t)n);
+    b->roots = malloc(sizeof(td_op_t*) * (size_t)n);
+    b->items = calloc((size_t)n, sizeof(jg_item_t));
+    b->bidx = malloc(sizeof(int) * (size_t)n);
+    const tdd_join_group_spec_t** specs = malloc(sizeof(void*) * (size_t)n);
+    if (!b->graphs || !b->roots || !b->items || !b->bidx || !specs) {
+        free(specs);
+        td_dev_batch_free(b);
+        return NULL;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        b->graphs[i] = graphs[i];
+        b->roots[i] = roots[i];
+        b->bidx[i] = -1;
+        if (jg_plan(graphs[i], roots[i], &b->items[i]) == TD_OK) {
+            b->bidx[i] = b->nb;
+            specs[b->nb++] = &b->items[i].S;
+        }
+    }
+    /* lpred points into the item: fix up after the items array is final */
+    for (int64_t i = 0; i < n; i++)
+        if (b->items[i].ok && b->items[i].S.lpred) b->items[i].S.lpred = &b->items[i].P;
+    if (b->nb > 0 && tdd_jg_batch_prepare(specs, b->nb, 1, &b->dev) != TD_OK) b->dev = NULL;
+    free(specs);
+    if (b->dev) {
+        for (int64_t i = 0; i < n; i++)
+            if (b->bidx[i] >= 0 && tdd_jg_batch_status(b->dev, b->bidx[i]) != 0) b->bidx[i] = -1;
+    } else {
+        for (int64_t i = 0; i < n; i++) b->bidx[i] = -1;
+    }
+    return b;
+}
+
+int64_t td_dev_batch_info(const td_dev_batch_t* b, int64_t* tile_rows, int* block) {
+    *tile_rows = 0;
+    *block = 0;
+    if (!b) return 0;
+    int64_t c = 0;
+    for (int64_t i = 0; i < b->n; i++) c += b->bidx[i] >= 0;
+    if (b->dev) tdd_jg_batch_shape(b->dev, tile_rows, block);
+    return c;
+}
+
+static td_err_t batch_run(td_dev_batch_t* b, int use_graph, td_t** results) {
+    td_err_t err = TD_OK;
+    for (int64_t i = 0; i < b->n; i++) results[i] = NULL;
+    if (b->dev) {
+        tdd_group_result_t R;
+        int64_t* offs = malloc(sizeof(int64_t) * (size_t)(b->nb + 1));
+        if (!offs) return TD_ERR_OOM;
+        int rc = tdd_jg_batch_run(b->dev, use_graph, &R, offs);
+        if (rc) { free(offs); return (td_err_t)rc; }
+        /* owners of the shared output blocks, typed like the first batched
+         * query's columns */
+        const jg_item_t* it0 = NULL;
+        for (int64_t i = 0; i < b->n && !it0; i++) if (b->bidx[i] >= 0) it0 = &b->items[i];
+        td_t* owners[TDD_MAX_KEYS + TDD_MAX_AGGS] = {0};
+        int nk = it0 ? it0->ext->n_keys : 0, na = it0 ? it0->ext->n_aggs : 0;
+        int64_t tot = offs[b->nb];
+        for (int c = 0; c < nk + na; c++) {
+            td_t* s = c < nk ? it0->key_src[c] : it0->agg_src[c - nk];
+            int8_t t;
+            uint8_t attrs = 0;
+            size_t esz = 8;
+            if (c < nk) {
+                t = s->type;
+                attrs = s->attrs & TD_SYM_W_MASK;
+                esz = td__esz(s);
+            } else {
+                t = R.out_types[c - nk];
+                if (t != TD_F64 && t != TD_I64 && s) { attrs = s->attrs & TD_SYM_W_MASK; esz = td__esz(s); }
+            }
+            void* blk = c < nk ? R.keys[c] : R.vals[c - nk];
+            owners[c] = td__hbm_wrap(t, attrs, tot, blk, (size_t)(tot > 0 ? tot : 1) * esz);
+            if (TD_IS_ERR(owners[c])) { err = TD_ERR_OOM; owners[c] = NULL; }
+        }
+        for (int64_t i = 0; i < b->n && !err; i++) {
+            int q = b->bidx[i];
+            if (q < 0) continue;
+            results[i] = jg_view_table(b->graphs[i], &b->items[i], owners, offs[q],
+                                       offs[q + 1] - offs[q]);
+            if (TD_IS_ERR(results[i])) err = TD_ERR_OOM;
+        }
+        for (int c = 0; c < nk + na; c++) if (owners[c]) td_release(owners[c]);
+        free(offs);
+    }
+    for (int64_t i = 0; i < b->n; i++) {
+        if (results[i]) continue;
+        results[i] = td_execute(b->graphs[i], b->roots[i]);
+        if (TD_IS_ERR(results[i]) && !err) err = (td_err_t)(intptr_t)results[i];
+    }
+    td__set_plan("batch queries=%lld batched=%d", (long long)b->n, b->dev ? b->nb : 0);
+    return err;
+}
+
+td_err_t td_dev_batch_run(td_dev_batch_t* b, td_t** results) {
+    if (!b || !results) return TD_ERR_NYI;
+    return batch_run(b, 1, results);
+}
+
+void td_dev_batch_free(td_dev_batch_t* b) {
+    if (!b) return;
+    if (b->dev) tdd_jg_batch_free(b->dev);
+    free(b->graphs);
+    free(b->roots);
+    free(b->items);
+    free(b->bidx);
+    free(b);
+}
+
+td_err_t td_dev_execute_batch(td_graph_t* const* graphs, td_op_t* const* roots, int64_t n,
+                              td_t** results) {
+    if (!td__dev_ready()) return TD_ERR_IO;
+    td_dev_batch_t* b = td_dev_batch_prepare(graphs, roots, n);
+    if (!b) return TD_ERR_OOM;
+    td_err_t e = batch_run(b, 0, results);
+    td_dev_batch_free(b);
+    return e;
+}

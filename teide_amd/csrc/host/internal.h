@@ -26,6 +26,7 @@ typedef struct {
     int64_t  lo, hi;   /* integer columns loaded by td_dev_put: min / max */
     int32_t  ranged;   /* lo / hi valid (resident columns are immutable) */
     int32_t  pad;
+    td_t*    owner;    /* non-NULL: a view into owner's block (held) */
 } td_hbm_ref_t;
 
 static inline td_hbm_ref_t* td_hbm_ref(td_t* v) { return (td_hbm_ref_t*)td_data(v); }
@@ -52,6 +53,10 @@ void   td__dev_forget(td_t* v);                /* td_free hook: drop cached uplo
 td_t*  td__hbm_vec_new(int8_t type, uint8_t attrs, int64_t len, size_t bytes);
 td_t*  td__hbm_wrap(int8_t type, uint8_t attrs, int64_t len, void* dptr, size_t bytes);
 void   td__hbm_free(td_t* v);
+/* A resident vector viewing len elements of `owner`'s block from byte
+ * offset `off`; holds a reference on owner. */
+td_t*  td__hbm_view(td_t* owner, int8_t type, uint8_t attrs, int64_t len, size_t off,
+                    size_t bytes);
 
 /* ---- dev_api.c ---- */
 td_t*  td__to_host(td_t* v);          /* consumes v: host copy of a proxy / table */

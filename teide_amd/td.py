@@ -118,6 +118,11 @@ class Engine:
                 "td_dev_force_paths": (None, [C.c_int, C.c_int, C.c_int]),
                 "td_dev_last_plan": (C.c_char_p, []), "td_dev_mem_stats": (None, [P]),
                 "td_dev_set_budget": (None, [C.c_uint64]),
+                "td_dev_execute_batch": (C.c_int, [P, P, I64, P]),
+                "td_dev_batch_prepare": (P, [P, P, I64]),
+                "td_dev_batch_run": (C.c_int, [P, P]),
+                "td_dev_batch_info": (I64, [P, P, P]),
+                "td_dev_batch_free": (None, [P]),
             })
         for name, (res, args) in sig.items():
             f = getattr(self.L, name)
@@ -233,6 +238,34 @@ class Engine:
 
     def last_plan(self):
         return self.L.td_dev_last_plan().decode()
+
+    def execute_batch(self, graphs, roots):
+        """td_dev_execute_batch over Graph objects and their root ops."""
+        n = len(graphs)
+        ga = (C.c_void_p * n)(*[g.g for g in graphs])
+        ra = (C.c_void_p * n)(*roots)
+        res = (C.c_void_p * n)()
+        rc = self.L.td_dev_execute_batch(C.addressof(ga), C.addressof(ra), n, C.addressof(res))
+        return rc, [res[i] for i in range(n)]
+
+    def batch_prepare(self, graphs, roots):
+        n = len(graphs)
+        ga = (C.c_void_p * n)(*[g.g for g in graphs])
+        ra = (C.c_void_p * n)(*roots)
+        h = self.L.td_dev_batch_prepare(C.addressof(ga), C.addressof(ra), n)
+        if not h:
+            raise TdError(1, "td_dev_batch_prepare")
+        return h
+
+    def batch_run(self, h, n):
+        res = (C.c_void_p * n)()
+        rc = self.L.td_dev_batch_run(h, C.addressof(res))
+        return rc, [res[i] for i in range(n)]
+
+    def batch_info(self, h):
+        tr, bl = C.c_int64(), C.c_int()
+        nb = self.L.td_dev_batch_info(h, C.byref(tr), C.byref(bl))
+        return nb, tr.value, bl.value
 
     def mem_stats(self):
         m = DevMem()
