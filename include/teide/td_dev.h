@@ -74,6 +74,51 @@ td_err_t td_dev_batch_run(td_dev_batch_t* b, td_t** results);
 int64_t  td_dev_batch_info(const td_dev_batch_t* b, int64_t* tile_rows, int* block);
 void     td_dev_batch_free(td_dev_batch_t* b);
 
+/* Serving: a request loop over a resident fact / dim pair.  Requests are
+ * td.h DAGs; those of the form
+ *   GROUP(dim key; SUM / AVG / COUNT of fact columns)
+ *     over the graph's inner JOIN(FILTER(fact, pcol CMP constant), fk = pk)
+ * with the configured columns share one scan of the fact table (continuous
+ * batching: a request submitted while others run joins the scan at the next
+ * step and finishes after one pass over every chunk); any other DAG runs
+ * through td_execute when admitted.  Admission is in submission order while
+ * request slots (max_active) and the device-memory budget (resident tables +
+ * server tables + every active request's accumulator) allow; a request that
+ * could never fit finishes with TD_ERR_OOM.  Results equal td_execute's on
+ * the same DAG (integers exact, f64 sums in another order).  Graphs must
+ * outlive their results.  No reference counterpart. */
+typedef struct td_dev_server td_dev_server_t;
+typedef struct {
+    td_t*    fact;             /* resident tables (td_dev_put) */
+    td_t*    dim;
+    int64_t  fk, pk;           /* join columns (symbol ids); pk unique integers */
+    int64_t  pcol;             /* fact column filtered by requests: small integer range */
+    int64_t  vals[4];          /* fact value columns requests aggregate */
+    int64_t  keys[4];          /* dim group-key columns */
+    int32_t  n_vals, n_keys;
+    uint64_t budget;           /* device bytes; 0: no cap beyond the arena's */
+    int64_t  chunk_rows;       /* fact rows per step; 0: an eighth of the table */
+    int32_t  max_active;       /* request slots; 0: 4096 */
+    int32_t  pad;
+} td_dev_serve_cfg_t;
+typedef struct {
+    int64_t  queued, active, ready;    /* waiting, running, finished not yet polled */
+    int64_t  steps, chunks;            /* steps run; steps per request */
+    int64_t  shared, alone;            /* requests admitted to the scan / run alone */
+    uint64_t held, base, budget;       /* bytes held now, of which fixed; budget */
+} td_dev_serve_info_t;
+td_dev_server_t* td_dev_serve_open(const td_dev_serve_cfg_t* cfg);
+/* Queue a request; returns its id (>= 0) or -error. */
+int64_t  td_dev_serve_submit(td_dev_server_t* s, td_graph_t* g, td_op_t* root);
+/* Admit what fits, run one scan step, finish requests that have seen every
+ * chunk; returns the number of requests finished (or -error). */
+int      td_dev_serve_step(td_dev_server_t* s);
+/* Pop a finished request: 1 with *id / *result (owned table or error
+ * pointer), 0 when none is ready. */
+int      td_dev_serve_poll(td_dev_server_t* s, int64_t* id, td_t** result);
+void     td_dev_serve_info(const td_dev_server_t* s, td_dev_serve_info_t* out);
+void     td_dev_serve_close(td_dev_server_t* s);
+
 /* Physical-plan hints for tests and measurement: force the join path
  * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group) / group
  * path (0 auto, 1 direct array, 2 hash) / radix bits (0 auto) for

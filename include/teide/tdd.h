@@ -242,6 +242,64 @@ int  tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
                       int64_t* offsets);
 void tdd_jg_batch_free(tdd_jg_batch_t* b);
 
+/* --------------------------------------------------- shared-scan serving -- */
+/* Requests of the family
+ *   GROUP(dim.key; SUM / AVG / COUNT of fact columns)
+ *     over JOIN(FILTER(fact, fact.pcol CMP t), fact.fk = dim.pk)
+ * with a per-request constant t, comparison and group key share one scan of
+ * the resident fact table.  The scan is cut into chunks; a step folds one
+ * chunk into a cube per group key, cell (pcol value, group): row count and
+ * one sum per value column.  Prefix / suffix sums over the pcol axis then
+ * answer every active request's predicate for that chunk, and each request
+ * adds them into its own direct-array group table.  A request that starts
+ * at any step sees every chunk once after n_chunks steps.  COUNT and
+ * integer sums are exact; f64 sums add the same values in another order.
+ * The dim join keys must be unique integers (a direct map per group key,
+ * built at open and kept resident).  No reference counterpart: the
+ * reference runs each request as its own two td_execute calls. */
+#define TDD_SRV_MAX_KEYS 4
+#define TDD_SRV_MAX_VALS 4
+
+typedef struct {
+    int64_t   nf, nd;                     /* fact rows, dim rows */
+    tdd_col_t fk, pk;                     /* join key columns (integer) */
+    tdd_col_t pcol;                       /* filtered fact column: integer, ranged */
+    int32_t   n_vals, n_keys;
+    tdd_col_t vals[TDD_SRV_MAX_VALS];     /* fact value columns (f64 or integer) */
+    tdd_col_t keys[TDD_SRV_MAX_KEYS];     /* dim group-key columns (integer, ranged) */
+    int64_t   chunk_rows;                 /* 0: an eighth of the fact table */
+} tdd_srv_spec_t;
+
+typedef struct {
+    int32_t  key;                         /* index into spec.keys */
+    int32_t  cmp;                         /* OP_EQ, OP_LT, OP_LE, OP_GT or OP_GE */
+    int64_t  t;                           /* pcol CMP t */
+    int32_t  n_aggs, pad;
+    uint16_t ops[TDD_MAX_AGGS];           /* OP_SUM, OP_AVG or OP_COUNT */
+    int8_t   src[TDD_MAX_AGGS];           /* value column index, -2: pcol, -1: none (COUNT) */
+} tdd_srv_req_t;
+
+typedef struct tdd_srv tdd_srv_t;
+typedef struct tdd_srv_acc tdd_srv_acc_t;
+
+/* Builds the per-key direct maps and the chunk cubes.  TD_ERR_NYI when the
+ * build keys are not unique integers or pcol is not a small integer range. */
+int     tdd_srv_open(const tdd_srv_spec_t* spec, tdd_srv_t** out);
+int64_t tdd_srv_chunks(const tdd_srv_t* s);
+int64_t tdd_srv_bytes(const tdd_srv_t* s);          /* device bytes held by the server */
+/* Device bytes a request holds while active (accumulator + output bound);
+ * TD_ERR_NYI when the server cannot answer it. */
+int     tdd_srv_req_bytes(const tdd_srv_t* s, const tdd_srv_req_t* r, int64_t* bytes);
+/* Allocates and clears a request's accumulator. */
+int     tdd_srv_begin(tdd_srv_t* s, const tdd_srv_req_t* r, tdd_srv_acc_t** out);
+/* One step: scan chunk (step number mod n_chunks) for the keys the active
+ * requests use, then fold it into each of them. */
+int     tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n);
+/* Groups of finished requests in ascending key order: out[i] for acc[i]. */
+int     tdd_srv_finish(tdd_srv_t* s, tdd_srv_acc_t* const* acc, int n, tdd_group_result_t* out);
+void    tdd_srv_acc_free(tdd_srv_acc_t* a);
+void    tdd_srv_close(tdd_srv_t* s);
+
 /* ------------------------------------------------- host services (C side) -- */
 /* Implemented by the host engine (arena.c); the device layer allocates its
  * scratch and result blocks from the HBM arena through these. */

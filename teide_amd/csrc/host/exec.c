@@ -1231,3 +1231,320 @@ td_err_t td_dev_execute_batch(td_graph_t* const* graphs, td_op_t* const* roots, 
     td_dev_batch_free(b);
     return e;
 }
+
+/* ------------------------------------------------------ served requests -- */
+/* A request loop over a resident fact / dim pair: DAGs of the served family
+ * (see tdd_srv_* in tdd.h) share the device scan; others run through
+ * td_execute when admitted.  Requests are admitted in arrival order while
+ * request slots and the device-memory budget allow, join the running scan
+ * at the next step and finish after one pass over every chunk. */
+
+typedef struct {
+    int64_t        id;
+    td_graph_t*    g;
+    td_op_t*       root;
+    jg_item_t      it;
+    tdd_srv_req_t  rq;
+    int            shared;       /* 1: answered by the shared scan */
+    int64_t        bytes;
+    int64_t        steps_left;
+    tdd_srv_acc_t* acc;
+} srv_item_t;
+
+typedef struct { int64_t id; td_t* result; } srv_done_t;
+
+struct td_dev_server {
+    tdd_srv_t*      dev;
+    tdd_srv_spec_t  spec;
+    td_t*           fact;
+    td_t*           dim;
+    uint64_t        budget, base, held;
+    int             max_active;
+    int64_t         chunks, next_id, steps, n_shared, n_alone;
+    srv_item_t**    queue;       /* ring of pending requests */
+    int64_t         q_head, q_len, q_cap;
+    srv_item_t**    active;
+    int             n_active;
+    srv_done_t*     done;        /* ring of finished results */
+    int64_t         d_head, d_len, d_cap;
+};
+
+static uint64_t table_dev_bytes(td_t* t) {
+    uint64_t b = 0;
+    for (int64_t c = 0; c < td_table_ncols(t); c++) {
+        td_t* col = td_table_get_col_idx(t, c);
+        if (col && !TD_IS_ERR(col)) b += (uint64_t)col->len * td__esz(col);
+    }
+    return b;
+}
+
+static int srv_col(td_t* tbl, int64_t sym, tdd_col_t* out) {
+    td_t* col = td_table_get_col(tbl, sym);
+    if (!col || col->mmod != TD_MMOD_HBM) return TD_ERR_SCHEMA;
+    xctx_t x;
+    memset(&x, 0, sizeof(x));
+    x.all_resident = 1;
+    int rc = dev_col(&x, col, out);
+    drop_tmps(&x);
+    return rc;
+}
+
+td_dev_server_t* td_dev_serve_open(const td_dev_serve_cfg_t* cfg) {
+    if (!cfg || !td__dev_ready() || !cfg->fact || !cfg->dim || !td_dev_is_resident(cfg->fact) ||
+        !td_dev_is_resident(cfg->dim) || cfg->n_vals < 0 || cfg->n_vals > TDD_SRV_MAX_VALS ||
+        cfg->n_keys < 1 || cfg->n_keys > TDD_SRV_MAX_KEYS)
+        return NULL;
+    td_dev_server_t* s = calloc(1, sizeof(*s));
+    if (!s) return NULL;
+    tdd_srv_spec_t* S = &s->spec;
+    S->nf = td_table_nrows(cfg->fact);
+    S->nd = td_table_nrows(cfg->dim);
+    S->n_vals = cfg->n_vals;
+    S->n_keys = cfg->n_keys;
+    S->chunk_rows = cfg->chunk_rows;
+    int rc = srv_col(cfg->fact, cfg->fk, &S->fk);
+    if (!rc) rc = srv_col(cfg->dim, cfg->pk, &S->pk);
+    if (!rc) rc = srv_col(cfg->fact, cfg->pcol, &S->pcol);
+    for (int j = 0; j < cfg->n_vals && !rc; j++) rc = srv_col(cfg->fact, cfg->vals[j], &S->vals[j]);
+    for (int k = 0; k < cfg->n_keys && !rc; k++) rc = srv_col(cfg->dim, cfg->keys[k], &S->keys[k]);
+    if (!rc) rc = tdd_srv_open(S, &s->dev);
+    s->q_cap = 64;
+    s->d_cap = 64;
+    s->max_active = cfg->max_active > 0 ? cfg->max_active : 4096;
+    s->queue = malloc(sizeof(srv_item_t*) * (size_t)s->q_cap);
+    s->active = malloc(sizeof(srv_item_t*) * (size_t)s->max_active);
+    s->done = malloc(sizeof(srv_done_t) * (size_t)s->d_cap);
+    if (rc || !s->queue || !s->active || !s->done) {
+        if (s->dev) tdd_srv_close(s->dev);
+        free(s->queue);
+        free(s->active);
+        free(s->done);
+        free(s);
+        return NULL;
+    }
+    td_retain(cfg->fact);
+    td_retain(cfg->dim);
+    s->fact = cfg->fact;
+    s->dim = cfg->dim;
+    s->chunks = tdd_srv_chunks(s->dev);
+    s->base = table_dev_bytes(cfg->fact) + table_dev_bytes(cfg->dim) +
+              (uint64_t)tdd_srv_bytes(s->dev);
+    s->held = s->base;
+    s->budget = cfg->budget;
+    return s;
+}
+
+/* The served-family request of a planned DAG, or -1. */
+static int srv_match(const td_dev_server_t* s, srv_item_t* r) {
+    const tdd_srv_spec_t* S = &s->spec;
+    const tdd_join_group_spec_t* J = &r->it.S;
+    tdd_srv_req_t* q = &r->rq;
+    memset(q, 0, sizeof(*q));
+    if (J->join.n_keys != 1 || J->join.nl != S->nf || J->join.nr != S->nd ||
+        J->join.lkeys[0].data != S->fk.data || J->join.rkeys[0].data != S->pk.data)
+        return -1;
+    if (J->lpred) {
+        const tdd_pred_t* P = J->lpred;
+        if (P->n_ins != 1 || P->ins[0].kind != TDD_P_CMP_CONST || P->ins[0].as_f64 ||
+            P->cols[P->ins[0].a].data != S->pcol.data)
+            return -1;
+        q->cmp = P->ins[0].cmp;
+        q->t = P->ins[0].ci;
+    } else {
+        q->cmp = OP_GE;
+        q->t = INT64_MIN;
+    }
+    if (J->n_keys != 1 || J->key_side[0] != 1) return -1;
+    q->key = -1;
+    for (int k = 0; k < S->n_keys; k++)
+        if (J->keys[0].data == S->keys[k].data) q->key = k;
+    if (q->key < 0) return -1;
+    q->n_aggs = J->n_aggs;
+    for (int a = 0; a < J->n_aggs; a++) {
+        q->ops[a] = J->ops[a];
+        if (J->ops[a] == OP_COUNT) { q->src[a] = -1; continue; }
+        if (J->agg_side[a] != 0) return -1;
+        q->src[a] = -3;
+        if (J->aggs[a].data == S->pcol.data) q->src[a] = -2;
+        for (int j = 0; j < S->n_vals; j++)
+            if (J->aggs[a].data == S->vals[j].data) q->src[a] = (int8_t)j;
+        if (q->src[a] == -3) return -1;
+    }
+    return tdd_srv_req_bytes(s->dev, q, &r->bytes) == TD_OK ? 0 : -1;
+}
+
+static int srv_push_done(td_dev_server_t* s, int64_t id, td_t* res) {
+    if (s->d_len == s->d_cap) {
+        srv_done_t* n = malloc(sizeof(srv_done_t) * (size_t)s->d_cap * 2);
+        if (!n) return TD_ERR_OOM;
+        for (int64_t i = 0; i < s->d_len; i++) n[i] = s->done[(s->d_head + i) % s->d_cap];
+        free(s->done);
+        s->done = n;
+        s->d_head = 0;
+        s->d_cap *= 2;
+    }
+    s->done[(s->d_head + s->d_len++) % s->d_cap] = (srv_done_t){id, res};
+    return TD_OK;
+}
+
+int64_t td_dev_serve_submit(td_dev_server_t* s, td_graph_t* g, td_op_t* root) {
+    if (!s || !g || !root) return -TD_ERR_NYI;
+    if (s->q_len == s->q_cap) {
+        srv_item_t** n = malloc(sizeof(srv_item_t*) * (size_t)s->q_cap * 2);
+        if (!n) return -TD_ERR_OOM;
+        for (int64_t i = 0; i < s->q_len; i++) n[i] = s->queue[(s->q_head + i) % s->q_cap];
+        free(s->queue);
+        s->queue = n;
+        s->q_head = 0;
+        s->q_cap *= 2;
+    }
+    srv_item_t* r = calloc(1, sizeof(*r));
+    if (!r) return -TD_ERR_OOM;
+    r->id = s->next_id++;
+    r->g = g;
+    r->root = root;
+    if (jg_plan(g, root, &r->it) == TD_OK) {
+        if (r->it.S.lpred) r->it.S.lpred = &r->it.P;
+        r->shared = srv_match(s, r) == 0;
+    }
+    s->queue[(s->q_head + s->q_len++) % s->q_cap] = r;
+    return r->id;
+}
+
+/* Finished shared requests -> result tables. */
+static int srv_finish(td_dev_server_t* s, srv_item_t** fin, int n) {
+    if (n <= 0) return TD_OK;
+    tdd_group_result_t* R = calloc((size_t)n, sizeof(tdd_group_result_t));
+    tdd_srv_acc_t** acc = malloc(sizeof(tdd_srv_acc_t*) * (size_t)n);
+    if (!R || !acc) { free(R); free(acc); return TD_ERR_OOM; }
+    for (int i = 0; i < n; i++) acc[i] = fin[i]->acc;
+    int rc = tdd_srv_finish(s->dev, acc, n, R);
+    for (int i = 0; i < n; i++) {
+        srv_item_t* r = fin[i];
+        td_t* res;
+        if (rc) {
+            for (int k = 0; k < TDD_MAX_KEYS; k++) td__arena_free(R[i].keys[k]);
+            for (int a = 0; a < TDD_MAX_AGGS; a++) td__arena_free(R[i].vals[a]);
+            res = TD_ERR_PTR(rc);
+        } else {
+            res = group_table(r->g, r->it.ext, &R[i], r->it.key_src, r->it.agg_src);
+        }
+        tdd_srv_acc_free(r->acc);
+        s->held -= (uint64_t)r->bytes;
+        int e = srv_push_done(s, r->id, res);
+        if (e && !rc) rc = e;
+        free(r);
+    }
+    free(R);
+    free(acc);
+    return rc;
+}
+
+int td_dev_serve_step(td_dev_server_t* s) {
+    if (!s) return -TD_ERR_NYI;
+    int finished = 0, rc = TD_OK;
+    /* admission, in arrival order */
+    while (s->q_len > 0) {
+        srv_item_t* r = s->queue[s->q_head];
+        if (!r->shared) {
+            td_t* res = td_execute(r->g, r->root);
+            s->q_head = (s->q_head + 1) % s->q_cap;
+            s->q_len--;
+            s->n_alone++;
+            finished++;
+            rc = srv_push_done(s, r->id, res);
+            free(r);
+            if (rc) return -rc;
+            continue;
+        }
+        if (s->n_active >= s->max_active) break;
+        int fits = !s->budget || s->held + (uint64_t)r->bytes <= s->budget;
+        int e = fits ? tdd_srv_begin(s->dev, &r->rq, &r->acc) : TD_ERR_OOM;
+        if (e) {
+            if (s->n_active > 0) break;        /* wait for active requests to free memory */
+            /* nothing left to wait for: the request cannot fit at all */
+            s->q_head = (s->q_head + 1) % s->q_cap;
+            s->q_len--;
+            finished++;
+            rc = srv_push_done(s, r->id, TD_ERR_PTR(e));
+            free(r);
+            if (rc) return -rc;
+            continue;
+        }
+        s->held += (uint64_t)r->bytes;
+        r->steps_left = s->chunks;
+        s->active[s->n_active++] = r;
+        s->q_head = (s->q_head + 1) % s->q_cap;
+        s->q_len--;
+        s->n_shared++;
+    }
+    if (s->n_active == 0) return finished;
+    tdd_srv_acc_t* accs[256];
+    tdd_srv_acc_t** A = s->n_active <= 256 ? accs : malloc(sizeof(void*) * (size_t)s->n_active);
+    if (!A) return -TD_ERR_OOM;
+    for (int i = 0; i < s->n_active; i++) A[i] = s->active[i]->acc;
+    rc = tdd_srv_step(s->dev, A, s->n_active);
+    if (A != accs) free(A);
+    if (rc) return -rc;
+    s->steps++;
+    /* requests that have now seen every chunk */
+    srv_item_t* fin_buf[256];
+    srv_item_t** fin = s->n_active <= 256 ? fin_buf : malloc(sizeof(void*) * (size_t)s->n_active);
+    if (!fin) return -TD_ERR_OOM;
+    int nf = 0, keep = 0;
+    for (int i = 0; i < s->n_active; i++) {
+        srv_item_t* r = s->active[i];
+        if (--r->steps_left == 0) fin[nf++] = r;
+        else s->active[keep++] = r;
+    }
+    s->n_active = keep;
+    rc = srv_finish(s, fin, nf);
+    if (fin != fin_buf) free(fin);
+    if (rc) return -rc;
+    return finished + nf;
+}
+
+int td_dev_serve_poll(td_dev_server_t* s, int64_t* id, td_t** result) {
+    if (!s || s->d_len == 0) return 0;
+    srv_done_t d = s->done[s->d_head];
+    s->d_head = (s->d_head + 1) % s->d_cap;
+    s->d_len--;
+    *id = d.id;
+    *result = d.result;
+    return 1;
+}
+
+void td_dev_serve_info(const td_dev_server_t* s, td_dev_serve_info_t* out) {
+    memset(out, 0, sizeof(*out));
+    if (!s) return;
+    out->queued = s->q_len;
+    out->active = s->n_active;
+    out->ready = s->d_len;
+    out->steps = s->steps;
+    out->chunks = s->chunks;
+    out->shared = s->n_shared;
+    out->alone = s->n_alone;
+    out->held = s->held;
+    out->base = s->base;
+    out->budget = s->budget;
+}
+
+void td_dev_serve_close(td_dev_server_t* s) {
+    if (!s) return;
+    for (int i = 0; i < s->n_active; i++) {
+        tdd_srv_acc_free(s->active[i]->acc);
+        free(s->active[i]);
+    }
+    for (int64_t i = 0; i < s->q_len; i++) free(s->queue[(s->q_head + i) % s->q_cap]);
+    for (int64_t i = 0; i < s->d_len; i++) {
+        td_t* r = s->done[(s->d_head + i) % s->d_cap].result;
+        if (r && !TD_IS_ERR(r)) td_release(r);
+    }
+    tdd_srv_close(s->dev);
+    td_release(s->fact);
+    td_release(s->dim);
+    free(s->queue);
+    free(s->active);
+    free(s->done);
+    free(s);
+}

@@ -72,6 +72,19 @@ class DevMem(C.Structure):
                 ("budget", C.c_uint64), ("device_total", C.c_uint64)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("fact", C.c_void_p), ("dim", C.c_void_p), ("fk", C.c_int64), ("pk", C.c_int64),
+                ("pcol", C.c_int64), ("vals", C.c_int64 * 4), ("keys", C.c_int64 * 4),
+                ("n_vals", C.c_int32), ("n_keys", C.c_int32), ("budget", C.c_uint64),
+                ("chunk_rows", C.c_int64), ("max_active", C.c_int32), ("pad", C.c_int32)]
+
+
+class ServeInfo(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("queued", "active", "ready", "steps", "chunks",
+                                         "shared", "alone")] + \
+               [(n, C.c_uint64) for n in ("held", "base", "budget")]
+
+
 class Engine:
     """One td.h library.  Pointers are Python ints (0 / < 32: error)."""
 
@@ -123,6 +136,9 @@ class Engine:
                 "td_dev_batch_run": (C.c_int, [P, P]),
                 "td_dev_batch_info": (I64, [P, P, P]),
                 "td_dev_batch_free": (None, [P]),
+                "td_dev_serve_open": (P, [P]), "td_dev_serve_submit": (I64, [P, P, P]),
+                "td_dev_serve_step": (C.c_int, [P]), "td_dev_serve_poll": (C.c_int, [P, P, P]),
+                "td_dev_serve_info": (None, [P, P]), "td_dev_serve_close": (None, [P]),
             })
         for name, (res, args) in sig.items():
             f = getattr(self.L, name)
@@ -266,6 +282,52 @@ class Engine:
         tr, bl = C.c_int64(), C.c_int()
         nb = self.L.td_dev_batch_info(h, C.byref(tr), C.byref(bl))
         return nb, tr.value, bl.value
+
+    # -------------------------------------------------------------- serving
+    def serve_open(self, fact, dim, fk, pk, pcol, vals, keys, budget=0, chunk_rows=0,
+                   max_active=0):
+        """td_dev_serve_open over resident tables; column arguments are names."""
+        cfg = ServeCfg()
+        cfg.fact, cfg.dim = fact, dim
+        cfg.fk, cfg.pk, cfg.pcol = self.sym(fk), self.sym(pk), self.sym(pcol)
+        for i, v in enumerate(vals):
+            cfg.vals[i] = self.sym(v)
+        for i, k in enumerate(keys):
+            cfg.keys[i] = self.sym(k)
+        cfg.n_vals, cfg.n_keys = len(vals), len(keys)
+        cfg.budget, cfg.chunk_rows, cfg.max_active = int(budget), int(chunk_rows), int(max_active)
+        h = self.L.td_dev_serve_open(C.byref(cfg))
+        if not h:
+            raise TdError(1, "td_dev_serve_open")
+        return h
+
+    def serve_submit(self, h, graph, root):
+        rid = self.L.td_dev_serve_submit(h, graph.g, root)
+        if rid < 0:
+            raise TdError(-rid, "td_dev_serve_submit")
+        return rid
+
+    def serve_step(self, h):
+        n = self.L.td_dev_serve_step(h)
+        if n < 0:
+            raise TdError(-n, "td_dev_serve_step")
+        return n
+
+    def serve_poll(self, h):
+        """All finished requests as [(id, result pointer)]."""
+        out = []
+        rid, res = C.c_int64(), C.c_void_p()
+        while self.L.td_dev_serve_poll(h, C.byref(rid), C.byref(res)):
+            out.append((rid.value, res.value or 0))
+        return out
+
+    def serve_info(self, h):
+        i = ServeInfo()
+        self.L.td_dev_serve_info(h, C.byref(i))
+        return {k: getattr(i, k) for k, _ in ServeInfo._fields_}
+
+    def serve_close(self, h):
+        self.L.td_dev_serve_close(h)
 
     def mem_stats(self):
         m = DevMem()

@@ -1,0 +1,200 @@
+"""GPU parity of the request server (td_dev_serve_*): every request's result
+in a mix of concurrent requests equals the same DAG run alone through
+td_execute and the same request run alone through the server -- keys,
+group sets, COUNT and integer sums bit-exact, f64 sums within the f64
+tolerance -- with requests joining the shared scan at different steps,
+under request-slot and device-memory admission limits, and with requests
+outside the served family answered by td_execute."""
+import numpy as np
+import pytest
+
+from test_gpu_batch import check_same
+
+pytestmark = pytest.mark.gpu
+
+CMPS = ["gt", "ge", "lt", "le", "eq"]
+
+
+@pytest.fixture(scope="module")
+def tables(eng):
+    rng = np.random.default_rng(31)
+    nf, nd = 300_000, 40_000
+    fk = rng.integers(0, nd + nd // 10, nf)            # a tenth of the probes miss
+    qty = rng.integers(1, 101, nf)
+    price = rng.lognormal(3, 1, nf)
+    pk = rng.permutation(nd) + 5
+    cat = rng.integers(0, 50, nd).astype(np.int32)
+    k64 = rng.integers(0, 3000, nd)
+    tf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
+    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64)])
+    rf, rd = eng.put(tf), eng.put(td)
+    eng.release(tf)
+    eng.release(td)
+    yield rf, rd
+    eng.release(rf)
+    eng.release(rd)
+
+
+def request_dag(eng, rf, rd, key, cmp, t, aggs, filtered=True):
+    g = eng.graph(rf)
+    src = g.const_table(rf)
+    if filtered:
+        src = g.filter(src, getattr(g, cmp)(g.scan("qty"), g.const_i64(t)))
+    g.join(src, [g.scan("fk")], g.const_table(rd), [g.scan("pk")], 0)
+    root = g.group([g.scan(key)], [(o, g.scan(c)) for o, c in aggs])
+    return g, root
+
+
+AGG_SETS = [
+    [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")],
+    [("AVG", "price"), ("AVG", "qty"), ("COUNT", "qty")],
+    [("COUNT", "price")],
+    [("SUM", "qty"), ("SUM", "price"), ("AVG", "price")],
+]
+
+
+def random_requests(rng, n):
+    reqs = []
+    for i in range(n):
+        cmp = CMPS[int(rng.integers(0, len(CMPS)))] if i % 3 else "gt"
+        t = int(rng.integers(1, 101))
+        if i % 11 == 5:
+            t = int(rng.choice([-5, 0, 100, 250]))       # empty / everything
+        reqs.append(dict(key="cat" if rng.random() < 0.5 else "k64", cmp=cmp, t=t,
+                         aggs=AGG_SETS[int(rng.integers(0, len(AGG_SETS)))]))
+    return reqs
+
+
+def open_server(eng, rf, rd, **kw):
+    return eng.serve_open(rf, rd, "fk", "pk", "qty", ["price"], ["cat", "k64"],
+                          chunk_rows=kw.pop("chunk_rows", 40960), **kw)
+
+
+def run_to_end(eng, h, results):
+    for _ in range(10000):
+        info = eng.serve_info(h)
+        if info["queued"] == 0 and info["active"] == 0:
+            break
+        eng.serve_step(h)
+        for rid, res in eng.serve_poll(h):
+            results[rid] = res
+    for rid, res in eng.serve_poll(h):
+        results[rid] = res
+
+
+def alone(eng, rf, rd, r):
+    g, root = request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"],
+                          r.get("filtered", True))
+    out = g.execute(root)
+    g.free()
+    return out
+
+
+def test_serve_mix_matches_alone(eng, tables):
+    rf, rd = tables
+    rng = np.random.default_rng(32)
+    reqs = random_requests(rng, 36)
+    reqs.append(dict(key="k64", cmp="gt", t=0, aggs=AGG_SETS[0], filtered=False))
+    # outside the served family: answered by td_execute at admission
+    reqs.append(dict(key="cat", cmp="gt", t=40, aggs=[("MIN", "price"), ("MAX", "qty")]))
+    h = open_server(eng, rf, rd)
+    assert eng.serve_info(h)["chunks"] == 8
+    dags, ids, results = {}, {}, {}
+    # requests arrive in waves while earlier ones are mid-scan
+    waves = [reqs[0:10], reqs[10:11], reqs[11:25], reqs[25:]]
+    for w in waves:
+        for r in w:
+            g, root = request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"],
+                                  r.get("filtered", True))
+            rid = eng.serve_submit(h, g, root)
+            dags[rid] = g
+            ids[rid] = r
+        for _ in range(3):
+            eng.serve_step(h)
+            for rid, res in eng.serve_poll(h):
+                results[rid] = res
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 1 and info["shared"] == len(reqs) - 1
+    assert info["held"] == info["base"]
+    assert sorted(results) == sorted(ids)
+    for rid, r in ids.items():
+        ref = alone(eng, rf, rd, r)
+        check_same(eng, results[rid], ref, f"request {rid} {r}")
+        eng.release(results[rid])
+        eng.release(ref)
+    # the same requests, each alone in the server
+    for rid in list(ids)[:8]:
+        r = ids[rid]
+        g, root = request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"],
+                              r.get("filtered", True))
+        one = {}
+        rid1 = eng.serve_submit(h, g, root)
+        run_to_end(eng, h, one)
+        ref = alone(eng, rf, rd, r)
+        check_same(eng, one[rid1], ref, f"alone in server {r}")
+        eng.release(one[rid1])
+        eng.release(ref)
+        g.free()
+    eng.serve_close(h)
+    for g in dags.values():
+        g.free()
+
+
+def test_serve_slots_and_budget(eng, tables):
+    rf, rd = tables
+    req = dict(key="k64", cmp="ge", t=30, aggs=AGG_SETS[0])
+
+    def submit(h, n):
+        out = []
+        for _ in range(n):
+            g, root = request_dag(eng, rf, rd, req["key"], req["cmp"], req["t"], req["aggs"])
+            out.append((eng.serve_submit(h, g, root), g))
+        return out
+
+    ref = alone(eng, rf, rd, req)
+    # bytes one request holds while active
+    h = open_server(eng, rf, rd)
+    sub = submit(h, 1)
+    eng.serve_step(h)
+    info = eng.serve_info(h)
+    per = info["held"] - info["base"]
+    assert per > 0 and info["active"] == 1
+    res = {}
+    run_to_end(eng, h, res)
+    eng.release(res[sub[0][0]])
+    sub[0][1].free()
+    base = info["base"]
+    eng.serve_close(h)
+
+    for kw, cap in ((dict(max_active=3), 3), (dict(budget=base + 2 * per + per // 2), 2)):
+        h = open_server(eng, rf, rd, **kw)
+        sub = submit(h, 7)
+        res = {}
+        peak = 0
+        while True:
+            info = eng.serve_info(h)
+            if info["queued"] == 0 and info["active"] == 0:
+                break
+            eng.serve_step(h)
+            peak = max(peak, eng.serve_info(h)["active"])
+            if kw.get("budget"):
+                assert eng.serve_info(h)["held"] <= kw["budget"]
+            for rid, r in eng.serve_poll(h):
+                res[rid] = r
+        assert peak == cap
+        for rid, g in sub:
+            check_same(eng, res[rid], ref, f"admission {kw} request {rid}")
+            eng.release(res[rid])
+            g.free()
+        eng.serve_close(h)
+
+    # a request that can never fit the budget finishes with an error
+    h = open_server(eng, rf, rd, budget=base + per // 2)
+    sub = submit(h, 1)
+    eng.serve_step(h)
+    done = eng.serve_poll(h)
+    assert len(done) == 1 and done[0][0] == sub[0][0] and done[0][1] < 32
+    sub[0][1].free()
+    eng.serve_close(h)
+    eng.release(ref)
