@@ -325,6 +325,178 @@ class BatchQueries:
             self.e.release(t)
 
 
+class Served:
+    """Request stream over resident tables: fact 100M rows and dim 8M rows
+    with the join workload's distributions, plus k64 on dim (i64 uniform over
+    65,536 values).  Each request is GROUP(key; SUM price, SUM qty, COUNT)
+    over JOIN(FILTER(fact, qty > t), fk = pk) with t uniform in [1,100] and
+    key cat or k64 at random, submitted to the request server (shared scan,
+    continuous batching, admission against a 200 GB device-memory budget).
+    Reported: the open-loop Poisson stream at the offered rate (achieved
+    qps, p50 / p99 latency from arrival to result), the closed-loop capacity
+    (a fixed number of outstanding requests), and the plain form (one
+    td_execute per request) on the same tables."""
+    name = "served"
+    BUDGET = 200 * 10**9
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        import numpy as np
+        self.e, self.a, self.np = e, a, np
+        nf, nd = a.serve_fact, a.serve_dim
+        t0 = time.perf_counter()
+        fact = W.join_fact(nf, nd, SEED)
+        dim = W.join_dim(nd, N_CAT, SEED)
+        k64 = W.i64_uniform(nd, 0, 65535, SEED, 301)
+        hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+        del fact
+        hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"]), ("k64", k64)])
+        del dim, k64
+        self.tf, self.td = e.put(hf), e.put(hd)
+        e.L.td_dev_sync()
+        e.release(hf)
+        e.release(hd)
+        print(f"# served data: {nf:,} x {nd:,}, {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+        # the clients' request pool: DAGs built ahead, submitted many times
+        rng = np.random.default_rng(SEED + 7)
+        self.pool = []
+        for _ in range(a.serve_pool):
+            t = int(rng.integers(1, 101))
+            key = "cat" if rng.random() < 0.5 else "k64"
+            self.pool.append(self._dag(key, t) + (key, t))
+        self.req_bytes = nf * 24 + nd * 12 + 2000 * 28      # one request scanned alone
+
+    def _dag(self, key, t):
+        e, tf, td = self.e, self.tf, self.td
+        g = e.graph(tf)
+        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(t)))
+        g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+        root = g.group([g.scan(key)], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
+                                        ("COUNT", g.scan("price"))])
+        return g, root
+
+    def _open(self):
+        return self.e.serve_open(self.tf, self.td, "fk", "pk", "qty", ["price"], ["cat", "k64"],
+                                 budget=self.BUDGET, chunk_rows=self.a.serve_chunk,
+                                 max_active=self.a.serve_slots)
+
+    def plain(self, n):
+        """Sequential td_execute of n pool requests: seconds per request."""
+        e = self.e
+        e.release(self.pool[0][0].execute(self.pool[0][1]))
+        e.L.td_dev_sync()
+        t0 = time.perf_counter()
+        for i in range(n):
+            g, root = self.pool[i % len(self.pool)][:2]
+            e.release(g.execute(root))
+        e.L.td_dev_sync()
+        return (time.perf_counter() - t0) / n
+
+    def stream(self, rate, seconds, seed):
+        """Open loop: Poisson arrivals at `rate` for `seconds`."""
+        np, e = self.np, self.e
+        rng = np.random.default_rng(seed)
+        n = max(1, int(rate * seconds))
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        pick = rng.integers(0, len(self.pool), n)
+        lat = np.zeros(n)
+        h = self._open()
+        owner, done, i, steps = {}, 0, 0, 0
+        t0 = time.perf_counter()
+        while done < n:
+            now = time.perf_counter() - t0
+            while i < n and arr[i] <= now:
+                g, root = self.pool[pick[i]][:2]
+                owner[e.serve_submit(h, g, root)] = i
+                i += 1
+            if i == done:                                  # idle until the next arrival
+                time.sleep(max(0.0, arr[i] - now))
+                continue
+            e.serve_step(h)
+            steps += 1
+            t = time.perf_counter() - t0
+            for rid, res in e.serve_poll(h):
+                k = owner.pop(rid)
+                lat[k] = t - arr[k]
+                if res < 32:
+                    raise SystemExit(f"served request failed: {res}")
+                e.release(res)
+                done += 1
+        end = time.perf_counter() - t0
+        info = e.serve_info(h)
+        e.serve_close(h)
+        return {"offered_qps": rate, "requests": n, "qps": round(n / (end - arr[0]), 1),
+                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                "max_ms": round(float(lat.max()) * 1e3, 2), "steps": steps,
+                "shared": info["shared"], "alone": info["alone"]}
+
+    def closed(self, clients, seconds):
+        """Closed loop: `clients` outstanding requests, each replaced on
+        completion, for about `seconds`."""
+        np, e = self.np, self.e
+        h = self._open()
+        owner, lat, nxt = {}, [], 0
+        t0 = time.perf_counter()
+        for _ in range(clients):
+            g, root = self.pool[nxt % len(self.pool)][:2]
+            owner[e.serve_submit(h, g, root)] = 0.0
+            nxt += 1
+        done = 0
+        while True:
+            e.serve_step(h)
+            t = time.perf_counter() - t0
+            for rid, res in e.serve_poll(h):
+                lat.append(t - owner.pop(rid))
+                if res < 32:
+                    raise SystemExit(f"served request failed: {res}")
+                e.release(res)
+                done += 1
+                if t < seconds:
+                    g, root = self.pool[nxt % len(self.pool)][:2]
+                    owner[e.serve_submit(h, g, root)] = t
+                    nxt += 1
+            if not owner:
+                break
+        end = time.perf_counter() - t0
+        info = e.serve_info(h)
+        e.serve_close(h)
+        lat = np.array(lat)
+        return {"clients": clients, "requests": done, "qps": round(done / end, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                "chunks": info["chunks"], "held_gb_fixed": round(info["base"] / 1e9, 2)}
+
+    def run(self):
+        e = self.e
+        e.prof_reset()
+        e.prof(True)
+        st = self.stream(self.a.serve_rate, self.a.serve_seconds, SEED + 11)
+        prof = e.prof_read()
+        e.prof(False)
+        cap = self.closed(self.a.serve_clients, self.a.serve_cap_seconds)
+        plain_s = self.plain(self.a.serve_plain)
+        plain_qps = 1.0 / plain_s
+        kern = {k: {"launches": n, "avg_ms": round(tot / max(n, 1), 4)}
+                for k, (n, tot) in prof.items()}
+        return {"workload": f"Poisson {self.a.serve_rate}/s for {self.a.serve_seconds}s over "
+                            f"resident fact {self.a.serve_fact:,} x dim {self.a.serve_dim:,}; "
+                            f"qty > t (t uniform 1..100), group by cat or k64 (65,536 values); "
+                            f"SUM price, SUM qty, COUNT; budget 200 GB",
+                "stream": st, "capacity": cap, "plain_qps": round(plain_qps, 1),
+                "served_vs_plain": round(cap["qps"] / plain_qps, 2),
+                "stream_vs_plain": round(st["qps"] / plain_qps, 2),
+                "effective_gbs": round(cap["qps"] * self.req_bytes / 1e9, 1),
+                "kernels": kern}
+
+    def release(self):
+        for g, _, _, _ in self.pool:
+            g.free()
+        self.e.release(self.tf)
+        self.e.release(self.td)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,10 +506,22 @@ def main():
     ap.add_argument("--dim", type=int, default=N_DIM)
     ap.add_argument("--cats", type=int, default=N_CAT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch"], default="jg",
+    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served"], default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
-                         "batch: many small join + group-by queries per batch")
+                         "batch: many small join + group-by queries per batch; "
+                         "served: request stream through the request server")
     ap.add_argument("--batch-queries", type=int, default=4096)
+    ap.add_argument("--serve-fact", type=int, default=100_000_000)
+    ap.add_argument("--serve-dim", type=int, default=8_000_000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-clients", type=int, default=2048,
+                    help="outstanding requests of the closed-loop capacity run")
+    ap.add_argument("--serve-cap-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-chunk", type=int, default=0, help="fact rows per scan step")
+    ap.add_argument("--serve-slots", type=int, default=4096)
+    ap.add_argument("--serve-pool", type=int, default=512, help="distinct request DAGs")
+    ap.add_argument("--serve-plain", type=int, default=40, help="plain requests timed")
     ap.add_argument("--hc-rows", type=int, default=300_000_000)
     ap.add_argument("--hc-values", type=int, default=64_000_000)
     ap.add_argument("--hc-syms", type=int, default=1_000_000)
@@ -362,6 +546,23 @@ def main():
     if a.calibrate_copy:
         calibrate_copy(e, a.calibrate_copy << 20)
     e.force_paths(a.join_path, 0, 0)
+
+    if a.workload == "served":
+        wl = Served(e, a)
+        r = wl.run()
+        wl.release()
+        st = r["stream"]
+        line = {"metric": "served qps + p50/p99 latency (request stream)",
+                "value": st["qps"], "unit": "requests/s", "n_gpus": 1,
+                "steps": st["steps"], "warmup": 0,
+                "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "i64/f64",
+                "data": "synthetic (seeded generator, tables resident in HBM)",
+                "config": {"workload": r.pop("workload")}, "roofline": None,
+                "cpu_baseline": None}
+        line.update(r)
+        print(json.dumps(line), flush=True)
+        return
 
     wl = {"hc": HighCard, "batch": BatchQueries}.get(a.workload, JoinGroup)(e, a)
 
