@@ -15,6 +15,14 @@ bytes against its average HIP-event duration over the same timed steps.
 `cpu_baseline` times the reference's own multithreaded CPU path (built from
 source into oracle/_ref) on a smaller sample of the same workload.
 
+The same line then carries the two serving forms on their own workloads
+(skip them with --no-forms): "batched" -- 4096 small filter+join+group-by
+queries run as one batch, beside one td_execute per query -- and "served"
+-- a Poisson request stream (2000/s) through the request server over
+resident fact 100M x dim 8M tables, with p50/p99 latency, the closed-loop
+capacity and the plain per-request rate.  `--workload batch|served|hc` runs
+one of those workloads (or the high-cardinality group-by) alone.
+
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
 import argparse
@@ -497,6 +505,44 @@ class Served:
         self.e.release(self.td)
 
 
+def run_form(fn):
+    try:
+        return fn()
+    except Exception as ex:                       # reported in the line, not fatal
+        return {"error": f"{type(ex).__name__}: {ex}"}
+
+
+def batched_summary(e, a, steps=5, warmup=2):
+    """The batched form on its own workload (4096 queries), beside the
+    plain form (one td_execute per query)."""
+    t0 = time.perf_counter()
+    wl = BatchQueries(e, a)
+    for _ in range(warmup):
+        wl.step()
+    e.L.td_dev_sync()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        wl.step()
+    e.L.td_dev_sync()
+    ms = (time.perf_counter() - t1) * 1e3 / steps
+    plain_s = wl.plain(1)
+    out = {"workload": wl.config["workload"], "queries": wl.n, "ms_per_batch": round(ms, 3),
+           "batched_qps": round(wl.n / (ms / 1e3), 1), "plain_qps": round(wl.n / plain_s, 1),
+           "batched_vs_plain": round(plain_s * 1e3 / ms, 2),
+           "rows_per_s": wl.rows / (ms / 1e3),
+           "tile_rows": wl.config["tile_rows"], "block": wl.config["block"],
+           "setup_s": round(t1 - t0, 1)}
+    wl.release()
+    return out
+
+
+def served_summary(e, a):
+    wl = Served(e, a)
+    r = wl.run()
+    wl.release()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,6 +552,8 @@ def main():
     ap.add_argument("--dim", type=int, default=N_DIM)
     ap.add_argument("--cats", type=int, default=N_CAT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="join workload only: skip the batched and served forms")
     ap.add_argument("--workload", choices=["jg", "hc", "batch", "served"], default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
@@ -604,7 +652,7 @@ def main():
                  "plain_qps": round(wl.n / plain_s, 1),
                  "batched_vs_plain": round((plain_s * 1e3) / ms, 2)}
     line = {
-        "metric": "rows/s + % HBM peak for join+group-by",
+        "metric": "rows/s + % HBM peak for join+group-by; batched/served qps & p99 vs plain",
         "value": wl.rows / (ms / 1e3),
         "unit": "rows/s",
         "n_gpus": 1,
@@ -625,6 +673,9 @@ def main():
     }
     line.update(extra)
     wl.release()
+    if a.workload == "jg" and not a.no_forms:
+        line["batched"] = run_form(lambda: batched_summary(e, a))
+        line["served"] = run_form(lambda: served_summary(e, a))
     print(json.dumps(line), flush=True)
 
 
