@@ -120,7 +120,8 @@ void     td_dev_serve_info(const td_dev_server_t* s, td_dev_serve_info_t* out);
 void     td_dev_serve_close(td_dev_server_t* s);
 
 /* Physical-plan hints for tests and measurement: force the join path
- * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group) / group
+ * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group,
+ * 4 XCD-partitioned fused join-group) / group
  * path (0 auto, 1 direct array, 2 hash) / radix bits (0 auto) for
  * subsequent td_execute calls. */
 void     td_dev_force_paths(int join_path, int group_path, int radix_bits);

@@ -182,7 +182,7 @@ typedef struct {
     tdd_col_t         lkeys[TDD_MAX_KEYS];
     tdd_col_t         rkeys[TDD_MAX_KEYS];
     const tdd_sel_t*  lsel;               /* left rows that take part; NULL: all */
-    int32_t           force_path;         /* 0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group */
+    int32_t           force_path;         /* 0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group, 4 XCD-partitioned fused join-group */
     int32_t           radix_bits;         /* 0: chosen from the build size */
 } tdd_join_spec_t;
 

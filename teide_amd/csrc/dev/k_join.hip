@@ -962,6 +962,9 @@ static bool jg_tile_plan(const tdd_join_group_spec_t* S, const GroupPlan& P, JGT
 static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out);
 int jg_partitioned(const tdd_join_group_spec_t* S, const GroupPlan& GP, int64_t kmin,
                    int64_t range, int64_t n_slots, bool forced, tdd_group_result_t* out);
+int jg_xpart(const tdd_join_group_spec_t* S, const GroupPlan& GP, const TileCols& T0,
+             const StarPred& SP, const uint8_t* cell_slot, const uint16_t* map, int64_t kmin,
+             int64_t range, int64_t n_slots, tdd_group_result_t* out);
 
 extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
     memset(out, 0, sizeof(*out));
@@ -1077,6 +1080,16 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
     A.D = DMap{head.as<int32_t>(), next.as<int32_t>(), mn, range};
     A.n_slots = n_slots;
     if (slotmap) A.slotmap = smap.as<int32_t>();
+
+    /* XCD-partitioned probe: keeps each XCD's gathers inside its L2 at the
+     * cost of writing and re-reading the passing rows; it measures level
+     * with the single-pass star probe on 200 M x 16 M, so only a forced
+     * path takes it. */
+    if (star && small && lds_table && J->force_path == 4) {
+        rc = jg_xpart(S, P, SA.T, SA.SP, SA.cell_slot, smap.as<uint16_t>(), mn, range, n_slots, out);
+        if (rc != TD_ERR_NYI) return rc;
+        memset(out, 0, sizeof(*out));
+    }
 
     int64_t n_words = (J->nl + 63) / 64;
     if (lds_table) {

@@ -180,7 +180,7 @@ def check_jg(out, case):
 
 @pytest.mark.parametrize("case", ["jg_2m_250k", "jg_100k_8k_c1m"])
 @pytest.mark.parametrize("form", ["two_call", "fused"])
-@pytest.mark.parametrize("path", [(0, 0), (2, 0), (3, 0)])
+@pytest.mark.parametrize("path", [(0, 0), (2, 0), (3, 0), (4, 0)])
 @pytest.mark.parametrize("resident", [False, True])
 def test_jg_golden(eng, forced, case, form, path, resident):
     nf, nd, nc, seed = (int(a) for a in MANIFEST[case]["argv"][1:5])
@@ -202,7 +202,7 @@ def test_jg_golden(eng, forced, case, form, path, resident):
 
 @pytest.mark.parametrize("variant", ["dup_dim", "agg_on_dim", "key_on_fact", "left_join",
                                      "zipf_fk"])
-@pytest.mark.parametrize("jpath", [0, 3])
+@pytest.mark.parametrize("jpath", [0, 3, 4])
 def test_join_group_general(eng, forced, variant, jpath):
     """Fused JOIN -> GROUP shapes off the one-index fast path: duplicate
     build keys, aggregates over build columns, keys on the probe side and a
@@ -211,7 +211,7 @@ def test_join_group_general(eng, forced, variant, jpath):
     nf, nd = 300000, (200000 if variant == "zipf_fk" else 20000)
     fk = rng.integers(0, nd + 500, nf)
     if variant == "zipf_fk":
-        # heavy key skew: the partitioned probe's regions overflow and the
+        # heavy key skew: the partitioned probes' regions overflow and the
         # direct-map probe takes over
         fk = np.minimum(rng.zipf(1.3, nf) - 1, nd + 499).astype(np.int64)
     qty = rng.integers(1, 101, nf)
@@ -270,7 +270,7 @@ TILED_SHAPES = {
 
 @pytest.mark.parametrize("shape", list(TILED_SHAPES))
 @pytest.mark.parametrize("resident", [False, True])
-@pytest.mark.parametrize("jpath", [0, 3])
+@pytest.mark.parametrize("jpath", [0, 3, 4])
 def test_jg_tiled_shapes(eng, forced, shape, resident, jpath):
     """Fused JOIN -> GROUP through the tiled slot-map probe: stacked
     filters (a selection), OR / NOT and column-column predicates, 4-byte key
@@ -305,8 +305,8 @@ def test_jg_tiled_shapes(eng, forced, shape, resident, jpath):
     eng.release(tf)
     eng.release(td)
     assert plan.startswith("join_group"), plan
-    if jpath == 3 and shape in ("two_filters", "i32_key"):
-        assert "path=3" in plan, plan          # the partitioned probe ran
+    if jpath in (3, 4) and shape in ("two_filters", "i32_key"):
+        assert f"path={jpath}" in plan, plan   # the partitioned probe ran
     # OR / NOT and column-column predicates are not range checks: the
     # partitioned probe declines them and the direct-map probe runs
     m = mask_fn(f)
