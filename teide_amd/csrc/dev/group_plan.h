@@ -56,9 +56,36 @@ __host__ __device__ __forceinline__ uint64_t cell_identity(int kind) {
     }
 }
 
+/* MIN / MAX into a cell, skipping the atomic when a plain read already
+ * shows the value cannot improve it.  A cell only ever moves toward its
+ * final value, so a stale read is never past the current one and a skip is
+ * never wrong; most rows of a group stop issuing atomics after the first
+ * few.  (The read may see another workgroup's update late, which only
+ * costs an atomic that changes nothing.) */
+__device__ __forceinline__ void cell_min(uint64_t* c, int64_t v) {
+    if (v < *(volatile int64_t*)c) atomicMin((long long*)c, (long long)v);
+}
+__device__ __forceinline__ void cell_max(uint64_t* c, int64_t v) {
+    if (v > *(volatile int64_t*)c) atomicMax((long long*)c, (long long)v);
+}
+template <bool SKIP>
+__device__ __forceinline__ void min_cell(uint64_t* c, int64_t v) {
+    if (SKIP) cell_min(c, v);
+    else atomicMin((long long*)c, (long long)v);
+}
+template <bool SKIP>
+__device__ __forceinline__ void max_cell(uint64_t* c, int64_t v) {
+    if (SKIP) cell_max(c, v);
+    else atomicMax((long long*)c, (long long)v);
+}
+
 /* Fold one row into the cells at `c` (LDS or global; the address space is
  * resolved after inlining).  A fused join passes the probe and build rows;
- * each aggregate reads the row of its side. */
+ * each aggregate reads the row of its side.  SKIP: MIN / MAX through
+ * cell_min / cell_max -- a win where the cell read is cheap (LDS, a table
+ * that stays in cache); off where the read would be one more dependent
+ * trip to memory per row (a large hash table). */
+template <bool SKIP = true>
 __device__ __forceinline__ void cells_accum2(uint64_t* c, const GroupPlan& P, int64_t lrow,
                                              int64_t rrow) {
     atomicAdd((unsigned long long*)&c[0], 1ull);
@@ -68,19 +95,58 @@ __device__ __forceinline__ void cells_accum2(uint64_t* c, const GroupPlan& P, in
         switch (P.kind[i]) {
         case C_ADD_F64: atomicAdd((double*)&c[i], ld_f64(a, row)); break;
         case C_ADD_I64: atomicAdd((unsigned long long*)&c[i], (unsigned long long)ld_i64(a, row)); break;
-        case C_MIN_I64: atomicMin((long long*)&c[i], (long long)ld_i64(a, row)); break;
-        case C_MAX_I64: atomicMax((long long*)&c[i], (long long)ld_i64(a, row)); break;
-        case C_MIN_F64: atomicMin((long long*)&c[i], (long long)f64_ord(ld_f64(a, row))); break;
-        case C_MAX_F64: atomicMax((long long*)&c[i], (long long)f64_ord(ld_f64(a, row))); break;
-        case C_MIN_ROW: atomicMin((long long*)&c[i], (long long)row); break;
-        case C_MAX_ROW: atomicMax((long long*)&c[i], (long long)row); break;
+        case C_MIN_I64: min_cell<SKIP>(&c[i], ld_i64(a, row)); break;
+        case C_MAX_I64: max_cell<SKIP>(&c[i], ld_i64(a, row)); break;
+        case C_MIN_F64: min_cell<SKIP>(&c[i], f64_ord(ld_f64(a, row))); break;
+        case C_MAX_F64: max_cell<SKIP>(&c[i], f64_ord(ld_f64(a, row))); break;
+        case C_MIN_ROW: min_cell<SKIP>(&c[i], row); break;
+        case C_MAX_ROW: max_cell<SKIP>(&c[i], row); break;
         default: { double v = ld_f64(a, row); atomicAdd((double*)&c[i], v * v); } break;
         }
     }
 }
 
+template <bool SKIP = true>
 __device__ __forceinline__ void cells_accum(uint64_t* c, const GroupPlan& P, int64_t row) {
-    cells_accum2(c, P, row, row);
+    cells_accum2<SKIP>(c, P, row, row);
+}
+
+/* ----------------------------------------------------------- hash keys -- */
+
+#define EMPTY_KEY 0xffffffffffffffffull
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+/* 64-bit group key of a row: the single key's bits, or the mixed-radix
+ * composite of several integer keys (stored in kmin / kstride). */
+__device__ __forceinline__ uint64_t ht_key(const GroupPlan& P, int64_t row) {
+    if (P.n_keys == 1 && P.kstride[0] == 0) return (uint64_t)ld_i64(P.keys[0], row);
+    return (uint64_t)da_slot2(P, row, row);
+}
+
+/* Slot of key in a global open-addressing table (linear probing, keys
+ * claimed with CAS), inserting it if absent; the all-ones key has the
+ * extra slot after the table; -1 when the table is full. */
+__device__ __forceinline__ int64_t ht_find(unsigned long long* keys, uint64_t mask, uint64_t key) {
+    if (key == EMPTY_KEY) return (int64_t)mask + 1;
+    uint64_t i = mix64(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        unsigned long long k = keys[i];
+        if (k == key) return (int64_t)i;
+        if (k == EMPTY_KEY) {
+            unsigned long long prev = atomicCAS(&keys[i], EMPTY_KEY, (unsigned long long)key);
+            if (prev == EMPTY_KEY || prev == key) return (int64_t)i;
+        }
+        i = (i + 1) & mask;
+    }
+    return -1;
 }
 
 /* Merge a partial cell block `src` into `dst` with atomics. */
@@ -92,9 +158,9 @@ __device__ __forceinline__ void cells_merge(uint64_t* dst, const uint64_t* src, 
             atomicAdd((double*)&dst[i], __longlong_as_double((long long)src[i])); break;
         case C_ADD_I64: atomicAdd((unsigned long long*)&dst[i], (unsigned long long)src[i]); break;
         case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW:
-            atomicMin((long long*)&dst[i], (long long)src[i]); break;
+            cell_min(&dst[i], (int64_t)src[i]); break;
         default:
-            atomicMax((long long*)&dst[i], (long long)src[i]); break;
+            cell_max(&dst[i], (int64_t)src[i]); break;
         }
     }
 }

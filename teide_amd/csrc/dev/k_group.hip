@@ -390,25 +390,8 @@ static int run_da(GroupPlan& P, const RowMask& M, int64_t n_slots, tdd_group_res
 
 #define HLL_P 12
 #define HLL_M (1 << HLL_P)
-#define EMPTY_KEY 0xffffffffffffffffull
 #define CACHE_SLOTS 256
 #define CACHE_PROBES 4
-
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-    z ^= z >> 33;
-    z *= 0xff51afd7ed558ccdull;
-    z ^= z >> 33;
-    z *= 0xc4ceb9fe1a85ec53ull;
-    z ^= z >> 33;
-    return z;
-}
-
-/* 64-bit group key of a row: the single key's bits, or the mixed-radix
- * composite of several integer keys (stored in kmin / kstride). */
-__device__ __forceinline__ uint64_t ht_key(const GroupPlan& P, int64_t row) {
-    if (P.n_keys == 1 && P.kstride[0] == 0) return (uint64_t)ld_i64(P.keys[0], row);
-    return (uint64_t)da_slot(P, row);
-}
 
 __global__ __launch_bounds__(256) void k_hll(GroupPlan P, RowMask M, unsigned int* regs) {
     __shared__ unsigned int r[HLL_M];
@@ -433,21 +416,6 @@ __global__ __launch_bounds__(256) void k_hll(GroupPlan P, RowMask M, unsigned in
         if (r[i]) atomicMax(&regs[i], r[i]);
 }
 
-/* Slot of key in the global table, inserting it if absent; -1 when full. */
-__device__ __forceinline__ int64_t ht_find(unsigned long long* keys, uint64_t mask, uint64_t key) {
-    if (key == EMPTY_KEY) return (int64_t)mask + 1;        /* the reserved extra slot */
-    uint64_t i = mix64(key) & mask;
-    for (uint64_t probe = 0; probe <= mask; probe++) {
-        unsigned long long k = keys[i];
-        if (k == key) return (int64_t)i;
-        if (k == EMPTY_KEY) {
-            unsigned long long prev = atomicCAS(&keys[i], EMPTY_KEY, (unsigned long long)key);
-            if (prev == EMPTY_KEY || prev == key) return (int64_t)i;
-        }
-        i = (i + 1) & mask;
-    }
-    return -1;
-}
 
 __global__ __launch_bounds__(GT) void k_ht_agg(GroupPlan P, RowMask M, unsigned long long* keys,
                                                uint64_t mask, uint64_t* cells, int* overflow) {
@@ -486,7 +454,7 @@ __global__ __launch_bounds__(GT) void k_ht_agg(GroupPlan P, RowMask M, unsigned 
         } else {
             int64_t s = ht_find(keys, mask, key);
             if (s < 0) { *overflow = 1; continue; }
-            cells_accum(&cells[s * nc], P, row);
+            cells_accum<false>(&cells[s * nc], P, row);
         }
     }
     __syncthreads();
@@ -498,14 +466,8 @@ __global__ __launch_bounds__(GT) void k_ht_agg(GroupPlan P, RowMask M, unsigned 
     }
 }
 
-__global__ void k_ht_emit(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
-                          int64_t n_slots, const int64_t* pos, tdd_group_result_t out) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_slots) return;
-    const uint64_t* c = cells + s * P.ncells;
-    if (c[0] == 0) return;
-    int64_t g = pos[s];
-    uint64_t key = s == n_slots - 1 ? EMPTY_KEY : keys[s];
+__device__ __forceinline__ void ht_emit_key(const GroupPlan& P, uint64_t key, int64_t g,
+                                            const tdd_group_result_t& out) {
     if (P.n_keys == 1 && P.kstride[0] == 0) {
         st_int(out.keys[0], g, P.keys[0].esz, (int64_t)key);
     } else {
@@ -514,6 +476,27 @@ __global__ void k_ht_emit(GroupPlan P, const unsigned long long* keys, const uin
             st_int(out.keys[k], g, P.keys[k].esz, v);
         }
     }
+}
+
+/* Groups of the list: entry i is output row i. */
+__global__ void k_ht_emit_list(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
+                               int64_t n, tdd_group_result_t out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ht_emit_key(P, keys[i], i, out);
+    for (int a = 0; a < P.n_aggs; a++) cells_emit(cells + i * P.ncells, P, a, out.vals[a], i);
+}
+
+__global__ void k_ht_emit(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
+                          int64_t n_slots, const int64_t* pos, int64_t off,
+                          tdd_group_result_t out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const uint64_t* c = cells + s * P.ncells;
+    if (c[0] == 0) return;
+    int64_t g = off + pos[s];
+    uint64_t key = s == n_slots - 1 ? EMPTY_KEY : keys[s];
+    ht_emit_key(P, key, g, out);
     for (int a = 0; a < P.n_aggs; a++) cells_emit(c, P, a, out.vals[a], g);
 }
 
@@ -530,7 +513,20 @@ static double hll_estimate(const unsigned int* regs) {
     return e;
 }
 
-static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, tdd_group_result_t* out) {
+struct tdd_pht;
+int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, double est,
+                  int64_t lds_bytes, bool forced, tdd_pht** out);
+int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gcells,
+             unsigned long long* lkeys, uint64_t* lcells, unsigned long long* lcount,
+             int64_t lcap_out, int* overflow);
+void pht_free(tdd_pht* h);
+
+/* Hash group-by.  Many rows with many distinct keys are first split by the
+ * key's hash until each part fits an LDS table (pht_*); otherwise rows go
+ * straight to the global table behind a small LDS cache (k_ht_agg).
+ * force 4: the split path with small LDS tables. */
+static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
+                  tdd_group_result_t* out) {
     SCRATCH(regs, HLL_M * 4);
     int frc = tdd_fill(regs.p, 0, HLL_M * 4);
     if (frc) return frc;
@@ -543,25 +539,43 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, tdd_group_
     if (est > (double)n_rows_sel) est = (double)n_rows_sel + 1;
     uint64_t cap = 1024;
     while ((double)cap < 2.0 * est) cap <<= 1;
+    tdd_pht* ph = nullptr;
+    struct PhFree { tdd_pht** h; ~PhFree() { pht_free(*h); } } phf{&ph};
+    if (force == 4 || (force == 0 && n_rows_sel >= ((int64_t)1 << 22) && est > 16384)) {
+        int prc = pht_partition(P, M.bits, M.nrows, est, force == 4 ? 4096 : 96 * 1024,
+                                force == 4, &ph);
+        if (prc && prc != TD_ERR_NYI) return prc;
+    }
     for (int attempt = 0; attempt < 6; attempt++, cap <<= 1) {
         int64_t n_slots = (int64_t)cap + 1;                  /* + the EMPTY_KEY slot */
         SCRATCH(keys, (size_t)cap * 8);
         SCRATCH(cells, (size_t)n_slots * P.ncells * 8);
-        SCRATCH(ovf, 8);
+        SCRATCH(ovf, 16);
         int rc = tdd_fill(keys.p, EMPTY_KEY, (size_t)cap * 8);
         if (rc) return rc;
         int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
         LAUNCH("group_init", k_cells_init, ib, 256, 0, cells.as<uint64_t>(), n_slots, P);
-        rc = tdd_fill(ovf.p, 0, 8);
+        rc = tdd_fill(ovf.p, 0, 16);                         /* flag, list count */
         if (rc) return rc;
-        size_t lds = (size_t)CACHE_SLOTS * 8 * (1 + P.ncells);
-        int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), 2);
-        LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
-               (uint64_t)(cap - 1), cells.as<uint64_t>(), ovf.as<int>());
-        int o = 0;
-        HC(hipMemcpyAsync(&o, ovf.p, 4, hipMemcpyDeviceToHost, tdd::g_stream));
+        Scratch lkeys(ph ? (size_t)cap * 8 : 8), lcells(ph ? (size_t)cap * P.ncells * 8 : 8);
+        if (!lkeys.ok() || !lcells.ok()) return TD_ERR_OOM;
+        unsigned long long* lcount = (unsigned long long*)(ovf.as<uint8_t>() + 8);
+        if (ph) {
+            rc = pht_fold(ph, keys.as<unsigned long long>(), (uint64_t)(cap - 1),
+                          cells.as<uint64_t>(), lkeys.as<unsigned long long>(),
+                          lcells.as<uint64_t>(), lcount, (int64_t)cap, ovf.as<int>());
+            if (rc) return rc;
+        } else {
+            size_t lds = (size_t)CACHE_SLOTS * 8 * (1 + P.ncells);
+            int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), 2);
+            LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
+                   (uint64_t)(cap - 1), cells.as<uint64_t>(), ovf.as<int>());
+        }
+        int64_t ol[2] = {0, 0};
+        HC(hipMemcpyAsync(ol, ovf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
-        if (o) continue;                                     /* table too small: retry */
+        if ((int32_t)ol[0]) continue;                        /* table too small: retry */
+        const int64_t n_list = ph ? ol[1] : 0;
         SCRATCH(flag, (size_t)n_slots * 4);
         SCRATCH(pos, (size_t)(n_slots + 1) * 8);
         int blocks = (int)ceil_div(n_slots, 256);
@@ -569,21 +583,29 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, tdd_group_
                n_slots, P.ncells, flag.as<uint32_t>());
         rc = tdd::scan_u32_i64(flag.as<uint32_t>(), n_slots, pos.as<int64_t>());
         if (rc) return rc;
-        int64_t ng = 0;
-        HC(hipMemcpyAsync(&ng, pos.as<int64_t>() + n_slots, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+        int64_t nh = 0;
+        HC(hipMemcpyAsync(&nh, pos.as<int64_t>() + n_slots, 8, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
+        const int64_t ng = n_list + nh;
         rc = alloc_group_outputs(P, ng, out);
         if (rc) return rc;
-        if (ng > 0)
+        if (n_list > 0)
+            LAUNCH("group_emit", k_ht_emit_list, (int)ceil_div(n_list, 256), 256, 0, P,
+                   (const unsigned long long*)lkeys.as<unsigned long long>(),
+                   (const uint64_t*)lcells.as<uint64_t>(), n_list, *out);
+        if (nh > 0)
             LAUNCH("group_emit", k_ht_emit, blocks, 256, 0, P,
                    (const unsigned long long*)keys.as<unsigned long long>(),
                    (const uint64_t*)cells.as<uint64_t>(), n_slots,
-                   (const int64_t*)pos.as<int64_t>(), *out);
-        out->path = 3;
+                   (const int64_t*)pos.as<int64_t>(), n_list, *out);
+        out->path = ph ? 5 : 3;
         return TD_OK;
     }
     return TD_ERR_OOM;
 }
+
+int run_pda(const GroupPlan& P, const uint64_t* mask, int64_t nrows, int64_t n_slots,
+            int64_t leaf_bytes, tdd_group_result_t* out);
 
 /* --------------------------------------------------------------- driver -- */
 
@@ -632,9 +654,26 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
             int64_t st = 1;
             for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st; st *= P.krange[k]; }
         }
+        /* many rows into a table beyond LDS: radix-partitioned direct array
+         * (force 3 takes it at any size, with small leaf tables) */
+        const bool pda_ok = composite && slots <= (double)((int64_t)1 << 26) &&
+                            slots * P.ncells * 8 <= 8.0e9 && slots <= 4.0 * (double)(nsel + 1024);
+        if (pda_ok && spec->force_path != 4 && (spec->force_path == 3 ||
+                       (spec->force_path == 0 && nsel >= ((int64_t)1 << 22) &&
+                        slots * P.ncells * 8 > LDS_TABLE_BYTES))) {
+            rc = run_pda(P, M.bits, M.nrows, (int64_t)slots,
+                         spec->force_path == 3 ? 64 : 96 * 1024, out);
+            if (rc != TD_ERR_NYI) {
+                if (rc) return rc;
+                HC(hipStreamSynchronize(tdd::g_stream));
+                return TD_OK;
+            }
+            memset(out, 0, sizeof(*out));
+        }
         da = composite && (slots <= (double)g_da_max_slots || spec->force_path == 1) &&
              slots * P.ncells * 8 <= 8.0e9;
         if (da && spec->force_path != 1 && slots > 4.0 * (double)(nsel + 1024)) da = false;
+        if (spec->force_path == 4) da = false;
         if (da) {
             rc = run_da(P, M, (int64_t)slots, out);
             if (rc) return rc;
@@ -647,7 +686,7 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
     if (P.n_keys == 1 && composite && P.keys[0].type != TD_F64) {
         P.kstride[0] = 0;                                    /* integer key: use its value */
     }
-    rc = run_ht(P, M, nsel, out);
+    rc = run_ht(P, M, nsel, spec->force_path, out);
     if (rc) return rc;
     HC(hipStreamSynchronize(tdd::g_stream));
     return TD_OK;
