@@ -32,7 +32,7 @@
 #define PG_W     (PG_T / WAVE)
 #define PG_R     4                        /* 64-item sub-tiles per wave per round */
 #define PG_ROUND (PG_T * PG_R)
-#define PG_MAXB  6                        /* bits per pass: up to 64 sub-partitions */
+#define PG_MAXB  8                        /* bits per pass: up to 256 sub-partitions */
 #define PG_MAXP  (1 << PG_MAXB)
 #define PG_MAXV  4                        /* value words per record */
 #define PG_JOB   ((int64_t)1 << 17)       /* items per job */
@@ -111,7 +111,8 @@ __global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJo
 /* The scatter of one pass. */
 template <bool ROWS, bool HK, int NV>
 __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out, const PGJob* jobs,
-                                                  int njobs, int shift, int PL, const int64_t* O) {
+                                                  int njobs, int shift, int PL, const int64_t* O,
+                                                  int64_t nout) {
     __shared__ int64_t cur[PG_MAXP];
     __shared__ uint32_t tot[PG_MAXP];
     __shared__ uint32_t wcnt[PG_W][PG_MAXP];
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out,
             }
             /* ranks within the wave, per sub-partition: the wave's own row of
              * LDS counters (its LDS operations complete in program order) */
-            if (lane < PL) wcnt[wv][lane] = 0;
+            for (int s2 = lane; s2 < PL; s2 += WAVE) wcnt[wv][s2] = 0;   /* PL may exceed 64 */
 #pragma unroll
             for (int k = 0; k < PG_R; k++)
                 rk[k] = sub[k] >= 0 ? (int32_t)atomicAdd(&wcnt[wv][sub[k]], 1u) : 0;
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out,
             for (int k = 0; k < PG_R; k++) {
                 if (sub[k] < 0) continue;
                 const int64_t at = cur[sub[k]] + wcnt[wv][sub[k]] + rk[k];
+                if (at < 0 || at >= nout) continue;          /* never taken when counts agree */
                 if (HK) out.key[at] = slot[k];
                 else out.slot[at] = (uint32_t)slot[k];
 #pragma unroll
@@ -200,7 +202,8 @@ __device__ __forceinline__ void pg_fold(uint64_t* c, const GroupPlan& P, const i
 template <int NV>
 __global__ __launch_bounds__(PG_T) void k_pg_agg(GroupPlan P, PGBuf in, const PGJob* jobs,
                                                  int njobs, int lbits, const int8_t* cw_dev,
-                                                 const uint8_t* vf64_dev, uint64_t* tab) {
+                                                 const uint8_t* vf64_dev, uint64_t* tab,
+                                                 int64_t n_slots) {
     extern __shared__ uint64_t lt[];
     __shared__ int8_t cw[GP_MAX_CELLS];
     __shared__ uint8_t vf[PG_MAXV];
@@ -227,14 +230,16 @@ __global__ __launch_bounds__(PG_T) void k_pg_agg(GroupPlan P, PGBuf in, const PG
 #pragma unroll
             for (int u = 0; u < PG_U; u++) {
                 if (i0 + (int64_t)u * blockDim.x >= J.r1) break;
-                pg_fold<NV, true>(&lt[(int64_t)(sl[u] - base) * nc], P, cw, vf, v[u]);
+                const uint32_t local = sl[u] - base;
+                if (local >= (uint32_t)nl) continue;     /* never taken for a correct split */
+                pg_fold<NV, true>(&lt[(int64_t)local * nc], P, cw, vf, v[u]);
             }
         }
         __syncthreads();
         for (int64_t s = threadIdx.x; s < nl; s += blockDim.x) {
             const uint64_t* c = &lt[s * nc];
-            if (c[0] == 0) continue;
-            cells_merge(&tab[(int64_t)(base + (uint32_t)s) * nc], c, P);
+            if (c[0] == 0 || (int64_t)base + s >= n_slots) continue;
+            cells_merge(&tab[((int64_t)base + s) * nc], c, P);
         }
         __syncthreads();
     }
@@ -360,10 +365,11 @@ static int pg_hist_and_offsets(bool rows, const PGRows& S, const PGBuf& in, cons
 
 template <bool ROWS, bool HK>
 static int pg_pass_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
-                          const PGJob* d_jobs, int njobs, int shift, int PL, const int64_t* O) {
+                          const PGJob* d_jobs, int njobs, int shift, int PL, const int64_t* O,
+                          int64_t nout) {
     const int grid = stream_grid(njobs, 2);
 #define PGP(NV_) LAUNCH("group_pg_pass", (k_pg_pass<ROWS, HK, NV_>), grid, PG_T, 0, S, in, out, \
-                        d_jobs, njobs, shift, PL, O)
+                        d_jobs, njobs, shift, PL, O, nout)
     switch (nv) {
     case 0: PGP(0); break;
     case 1: PGP(1); break;
@@ -487,9 +493,9 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
         if (rc) return rc;
         if (njobs > 0) {
             rc = rows ? pg_pass_launch<true, HK>(nv, S, in, outb, djobs.as<PGJob>(), njobs, shift,
-                                                 PL, O.as<int64_t>())
+                                                 PL, O.as<int64_t>(), nrows)
                       : pg_pass_launch<false, HK>(nv, S, in, outb, djobs.as<PGJob>(), njobs, shift,
-                                                  PL, O.as<int64_t>());
+                                                  PL, O.as<int64_t>(), nrows);
             if (rc) return rc;
         }
         /* partition bounds: (parent, s) starts at the offset of the parent's
@@ -521,7 +527,11 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
 }
 
 static void pg_split_bits(int upper, int* pbits, int* npass) {
-    *npass = (upper + PG_MAXB - 1) / PG_MAXB;
+    int mb = TUNE_INT("TEIDE_PG_BITS", 8);           /* bits per pass */
+    if (mb < 1) mb = 1;
+    if (mb > PG_MAXB) mb = PG_MAXB;
+    while ((upper + mb - 1) / mb > 8 && mb < PG_MAXB) mb++;     /* at most 8 passes */
+    *npass = (upper + mb - 1) / mb;
     for (int k = 0; k < *npass; k++) pbits[k] = upper / *npass + (k < upper % *npass ? 1 : 0);
 }
 
@@ -575,7 +585,7 @@ int run_pda(const GroupPlan& P, const uint64_t* mask, int64_t nrows, int64_t n_s
         const int8_t* dcw = (const int8_t*)meta.p;
         const uint8_t* dvf = meta.as<uint8_t>() + GP_MAX_CELLS;
 #define PGA(NV_) LAUNCH("group_pg_agg", k_pg_agg<NV_>, grid, PG_T, lds, P, in, djobs.as<PGJob>(), \
-                        njobs, lbits, dcw, dvf, tab.as<uint64_t>())
+                        njobs, lbits, dcw, dvf, tab.as<uint64_t>(), n_slots)
         switch (nv) {
         case 0: PGA(0); break;
         case 1: PGA(1); break;
