@@ -242,6 +242,23 @@ int  tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
                       int64_t* offsets);
 void tdd_jg_batch_free(tdd_jg_batch_t* b);
 
+/* ----------------------------------------------------------------- sort -- */
+/* One sort key: a column, its direction and NaN placement, and for SYM
+ * columns the rank of every symbol id by its string (device array). */
+typedef struct {
+    tdd_col_t       col;
+    int32_t         desc, nulls_first;
+    const uint32_t* rank;
+} tdd_sort_key_t;
+
+/* perm[0..n) = rows in sorted order: keys compared in order, each by the
+ * order-preserving word of the reference's radix encoding
+ * (radix_encode_fn, exec.c:2739-2870), ties kept in row order.  Replaces
+ * exec_sort's radix / merge sorts (exec.c:3370-3862); top-N is a prefix of
+ * this permutation (the reference's heap top-N, exec.c:3176-3340, leaves
+ * the order of equal keys at the boundary unspecified). */
+int tdd_sort_perm(const tdd_sort_key_t* keys, int nk, int64_t n, int64_t* perm);
+
 /* --------------------------------------------------- shared-scan serving -- */
 /* Requests of the family
  *   GROUP(dim.key; SUM / AVG / COUNT of fact columns)

@@ -191,3 +191,53 @@ def group(keys, ops, aggs, mask_bits=None, key_types=None):
         comp.append(cbuf[a * cap:a * cap + g].copy())
     res["vals"], res["sum_abs"], res["comp"] = vals, sabs, comp
     return res
+
+
+def sort_words(col, desc=False, nulls_first=None, rank=None):
+    """Order-preserving 64-bit words of a sort key column, as the
+    reference's radix path encodes them (radix_encode_fn,
+    exec.c:2739-2870): I64 with the sign bit flipped; I32 / I16 the same on
+    their own width; F64 by IEEE bits with negatives inverted and NaN at
+    0 or all-ones by (nulls_first xor desc); SYM by rank[id] (string order);
+    BOOL / U8 as is; descending keys inverted."""
+    nf = desc if nulls_first is None else nulls_first
+    a = np.asarray(col)
+    if rank is not None:
+        e = np.asarray(rank, np.uint64)[a.astype(np.int64)]
+    elif a.dtype == np.float64:
+        bits = a.view(np.uint64)
+        mask = np.where(bits >> np.uint64(63), np.uint64(0xFFFFFFFFFFFFFFFF),
+                        np.uint64(1 << 63))
+        e = bits ^ mask
+        nan = np.isnan(a)
+        nan_e = np.uint64(0) if (bool(nf) ^ bool(desc)) else np.uint64(0xFFFFFFFFFFFFFFFF)
+        e = np.where(nan, ~nan_e if desc else nan_e, ~e if desc else e)
+        return e.astype(np.uint64)
+    elif a.dtype == np.int64:
+        e = a.view(np.uint64) ^ np.uint64(1 << 63)
+    elif a.dtype == np.int32:
+        e = (a.view(np.uint32) ^ np.uint32(1 << 31)).astype(np.uint64)
+    elif a.dtype == np.int16:
+        e = (a.view(np.uint16) ^ np.uint16(1 << 15)).astype(np.uint64)
+    else:
+        e = a.astype(np.uint64)
+    return ~e if desc else e
+
+
+def sort_perm(cols, descs, nulls_first=None, ranks=None):
+    """Row order of a stable multi-key sort: keys compared in order by their
+    sort_words, equal rows kept in input order."""
+    nk = len(cols)
+    nfs = nulls_first or [None] * nk
+    rks = ranks or [None] * nk
+    words = [sort_words(c, d, f, r) for c, d, f, r in zip(cols, descs, nfs, rks)]
+    return np.lexsort(words[::-1], axis=-1) if nk else np.arange(0)
+
+
+def sym_rank(strings):
+    """rank[id] of symbol strings (bytes) ordered by their bytes then
+    length, ties by id (build_enum_rank, exec.c:2876-2970)."""
+    order = sorted(range(len(strings)), key=lambda i: (strings[i], i))
+    rank = np.empty(len(strings), np.uint32)
+    rank[np.asarray(order, np.int64)] = np.arange(len(strings), dtype=np.uint32)
+    return rank

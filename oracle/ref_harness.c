@@ -20,6 +20,8 @@
  *   ref_harness jg     NFACT NDIM NCAT SEED OUT [JOINOUT]
  *   ref_harness hc     N NVALUES NSYM SEED OUTKEY OUTSYM
  *   ref_harness join   NL NR KEYRANGE JTYPE SEED OUT
+ *   ref_harness having N NVALUES NSYM SEED MINCOUNT MAXSUM TOP OUT OUTTOP
+ *   ref_harness sort   N NVALUES NSYM SEED TOP OUT OUTTOP
  *   ref_harness bench  WORKLOAD ARGS... REPS  -> one JSON line (median ms)
  * Output files use the TDT1 column format read by tests/tdtio.py.
  */
@@ -227,6 +229,48 @@ static void make_hc(int64_t n, int64_t nv, uint32_t nsym, uint64_t seed, td_t** 
     *out = t;
 }
 
+/* HAVING on the group output (exec.c:11674-11704): GROUP(sym; SUM v_f64,
+ * COUNT v_i64, MAX v_i64) filtered to v_i64_count >= MINC and
+ * v_f64_sum < MAXS; with TOP > 0, HEAD(SORT(that, v_f64_sum desc), TOP). */
+static td_t* run_having(td_t* t, int64_t minc, double maxs, int64_t top) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* keys[1] = { td_scan(g, "sym") };
+    td_op_t* ins[3] = { td_scan(g, "v_f64"), td_scan(g, "v_i64"), td_scan(g, "v_i64") };
+    uint16_t ops[3] = { OP_SUM, OP_COUNT, OP_MAX };
+    td_op_t* grp = td_group(g, keys, 1, ops, ins, 3);
+    td_op_t* pred = td_and(g, td_ge(g, td_scan(g, "v_i64_count"), td_const_i64(g, minc)),
+                           td_lt(g, td_scan(g, "v_f64_sum"), td_const_f64(g, maxs)));
+    td_op_t* root = td_filter(g, grp, pred);
+    if (top > 0) {
+        td_op_t* sk[1] = { td_scan(g, "v_f64_sum") };
+        uint8_t desc[1] = { 1 }, nf[1] = { 1 };
+        root = td_head(g, td_sort_op(g, root, sk, desc, nf, 1), top);
+    }
+    td_t* out = check(td_execute(g, root), "having");
+    td_graph_free(g);
+    return out;
+}
+
+/* SORT of the table by (key asc, v_i64 desc); with TOP > 0, HEAD(SORT(t,
+ * v_f64 desc), TOP) -- the reference's heap top-N (exec.c:3455-3460). */
+static td_t* run_sort(td_t* t, int64_t top) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* tn = td_const_table(g, t);
+    td_op_t* root;
+    if (top > 0) {
+        td_op_t* sk[1] = { td_scan(g, "v_f64") };
+        uint8_t desc[1] = { 1 }, nf[1] = { 1 };
+        root = td_head(g, td_sort_op(g, tn, sk, desc, nf, 1), top);
+    } else {
+        td_op_t* sk[2] = { td_scan(g, "key"), td_scan(g, "v_i64") };
+        uint8_t desc[2] = { 0, 1 }, nf[2] = { 0, 1 };
+        root = td_sort_op(g, tn, sk, desc, nf, 2);
+    }
+    td_t* out = check(td_execute(g, root), "sort");
+    td_graph_free(g);
+    return out;
+}
+
 /* GROUP(k; SUM/MIN/MAX v_i64, SUM/MIN/MAX v_f64, COUNT v_i64) */
 static td_t* run_hc_group(td_t* t, const char* k) {
     td_graph_t* g = td_graph_new(t);
@@ -386,6 +430,28 @@ int main(int argc, char** argv) {
         write_table(argv[6], a);
         td_t* b = run_hc_group(t, "sym");
         write_table(argv[7], b);
+        td_release(a);
+        td_release(b);
+        td_release(t);
+    } else if (!strcmp(cmd, "having") && argc == 11) {
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        td_t* a = run_having(t, atoll(argv[6]), atof(argv[7]), 0);
+        write_table(argv[9], a);
+        td_t* b = run_having(t, atoll(argv[6]), atof(argv[7]), atoll(argv[8]));
+        write_table(argv[10], b);
+        td_release(a);
+        td_release(b);
+        td_release(t);
+    } else if (!strcmp(cmd, "sort") && argc == 9) {
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        td_t* a = run_sort(t, 0);
+        write_table(argv[7], a);
+        td_t* b = run_sort(t, atoll(argv[6]));
+        write_table(argv[8], b);
         td_release(a);
         td_release(b);
         td_release(t);

@@ -870,9 +870,123 @@ static td_t* slice_vec(xctx_t* x, td_t* v, int64_t n, int tail) {
 
 /* HEAD / TAIL of a table or vector (exec.c:11919-12140 without the fused
  * sort / group / filter variants). */
+/* ---------------------------------------------------------------- sort -- */
+
+/* Symbol ids 0..n_ids-1 ordered by their strings (bytes of the common
+ * prefix, then length; ids without a string first), ties by id -- the
+ * order build_enum_rank gives (exec.c:2876-2970); rank[id] = position.
+ * A stable bottom-up merge sort. */
+static int sym_str_cmp(uint32_t a, uint32_t b) {
+    td_t* sa = td_sym_str(a);
+    td_t* sb = td_sym_str(b);
+    size_t la = sa ? td_str_len(sa) : 0, lb = sb ? td_str_len(sb) : 0;
+    size_t ml = la < lb ? la : lb;
+    int c = ml ? memcmp(td_str_ptr(sa), td_str_ptr(sb), ml) : 0;
+    if (c == 0) c = (la > lb) - (la < lb);
+    return c;
+}
+
+static uint32_t* sym_ranks(uint32_t n_ids) {
+    uint32_t* ids = malloc(sizeof(uint32_t) * n_ids);
+    uint32_t* tmp = malloc(sizeof(uint32_t) * n_ids);
+    uint32_t* rank = malloc(sizeof(uint32_t) * n_ids);
+    if (!ids || !tmp || !rank) { free(ids); free(tmp); free(rank); return NULL; }
+    for (uint32_t i = 0; i < n_ids; i++) ids[i] = i;
+    for (uint32_t w = 1; w < n_ids; w *= 2) {
+        for (uint32_t lo = 0; lo < n_ids; lo += 2 * w) {
+            uint32_t mid = lo + w < n_ids ? lo + w : n_ids;
+            uint32_t hi = lo + 2 * w < n_ids ? lo + 2 * w : n_ids;
+            uint32_t a = lo, b = mid, k = lo;
+            while (a < mid && b < hi) tmp[k++] = sym_str_cmp(ids[a], ids[b]) <= 0 ? ids[a++] : ids[b++];
+            while (a < mid) tmp[k++] = ids[a++];
+            while (b < hi) tmp[k++] = ids[b++];
+        }
+        uint32_t* t = ids; ids = tmp; tmp = t;
+    }
+    for (uint32_t i = 0; i < n_ids; i++) rank[ids[i]] = i;
+    free(ids);
+    free(tmp);
+    return rank;
+}
+
+/* SORT of a table by its SCAN keys (exec.c:3370-3862): a device sort
+ * permutation, then every column gathered through it; limit >= 0 keeps the
+ * first `limit` rows (HEAD over SORT, exec.c:11895-11917), -1 all. */
+static td_t* exec_sort(xctx_t* x, td_op_t* op, int64_t limit) {
+    td_graph_t* g = x->g;
+    td_op_ext_t* ext = td__find_ext(g, op->id);
+    if (!ext || ext->sort.n_cols < 1 || ext->sort.n_cols > 16) return TD_ERR_PTR(TD_ERR_NYI);
+    td_t* in = xnode(x, op->inputs[0]);
+    if (TD_IS_ERR(in)) return in;
+    td_t* tbl = in->type == TD_TABLE ? in : g->table;
+    if (!tbl) { td_release(in); return TD_ERR_PTR(TD_ERR_SCHEMA); }
+    td_retain(tbl);
+    td_release(in);
+    if (g->selection && td_table_nrows(tbl) == g->selection->len) {
+        td_t* c = compact_table(x, tbl, g->selection);
+        td_release(g->selection);
+        g->selection = NULL;
+        td_release(tbl);
+        if (TD_IS_ERR(c)) return c;
+        tbl = c;
+    }
+    const int nk = ext->sort.n_cols;
+    const int64_t n = td_table_nrows(tbl);
+    tdd_sort_key_t K[16];
+    void* ranks[16] = {0};
+    memset(K, 0, sizeof(K));
+    int rc = TD_OK;
+    for (int k = 0; k < nk && !rc; k++) {
+        td_op_t* kop = ext->sort.columns[k];
+        td_op_ext_t* ke = td__find_ext(g, kop->id);
+        td_t* col = (ke && ke->base.opcode == OP_SCAN) ? td_table_get_col(tbl, ke->sym) : NULL;
+        if (!col) { rc = kop->opcode == OP_SCAN ? TD_ERR_SCHEMA : TD_ERR_NYI; break; }
+        rc = dev_col(x, col, &K[k].col);
+        K[k].desc = ext->sort.desc ? ext->sort.desc[k] : 0;
+        K[k].nulls_first = ext->sort.nulls_first ? ext->sort.nulls_first[k] : K[k].desc;
+        if (!rc && col->type == TD_SYM && n > 0) {
+            int64_t mn, mx;
+            rc = tdd_minmax(&K[k].col, n, NULL, &mn, &mx);
+            if (!rc && (mx < 0 || mx >= (int64_t)UINT32_MAX - 1)) rc = TD_ERR_NYI;
+            uint32_t* hr = rc ? NULL : sym_ranks((uint32_t)mx + 1);
+            if (!rc && !hr) rc = TD_ERR_OOM;
+            if (!rc) {
+                ranks[k] = td__arena_alloc(sizeof(uint32_t) * ((size_t)mx + 1));
+                if (!ranks[k]) rc = TD_ERR_OOM;
+                else rc = tdd_h2d(ranks[k], hr, sizeof(uint32_t) * ((size_t)mx + 1));
+                K[k].rank = ranks[k];
+            }
+            free(hr);
+        }
+    }
+    int64_t* perm = rc ? NULL : td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
+    if (!rc && !perm) rc = TD_ERR_OOM;
+    if (!rc) rc = tdd_sort_perm(K, nk, n, perm);
+    td_t* out = TD_ERR_PTR(rc ? rc : TD_ERR_OOM);
+    if (!rc) {
+        const int64_t m = limit >= 0 && limit < n ? limit : n;
+        int64_t nc = td_table_ncols(tbl);
+        out = td_table_new(nc);
+        for (int64_t c = 0; c < nc && !TD_IS_ERR(out); c++) {
+            td_t* v = gather_col(x, td_table_get_col_idx(tbl, c), perm, m, 0);
+            if (TD_IS_ERR(v)) { td_release(out); out = v; break; }
+            out = td_table_add_col(out, td_table_col_name(tbl, c), v);
+            td_release(v);
+        }
+    }
+    if (perm) { tdd_sync(); td__arena_free(perm); }
+    for (int k = 0; k < nk; k++) if (ranks[k]) td__arena_free(ranks[k]);
+    td_release(tbl);
+    td__set_plan("sort keys=%d rows=%lld", nk, (long long)n);
+    return out;
+}
+
 static td_t* exec_head_tail(xctx_t* x, td_op_t* op) {
     td_op_ext_t* e = td__find_ext(x->g, op->id);
     int64_t n = e ? e->sym : 10;
+    if (op->opcode == OP_HEAD && op->inputs[0]->opcode == OP_SORT)
+        return exec_sort(x, op->inputs[0], n > 0 ? n : 0);      /* top-N: a prefix */
+
     td_t* in = xnode(x, op->inputs[0]);
     if (TD_IS_ERR(in)) return in;
     int tail = op->opcode == OP_TAIL;
@@ -903,6 +1017,25 @@ static td_t* exec_head_tail(xctx_t* x, td_op_t* op) {
     return out;
 }
 
+/* FILTER over GROUP (HAVING): the predicate's SCANs name columns of the
+ * group result, and the groups that pass come out compacted, in group
+ * order (exec.c:11674-11704 swaps the graph's table to the group output
+ * while it evaluates the predicate). */
+static td_t* exec_having(xctx_t* x, td_op_t* op) {
+    td_t* gr = xnode(x, op->inputs[0]);
+    if (TD_IS_ERR(gr)) return gr;
+    if (gr->type != TD_TABLE) { td_release(gr); return TD_ERR_PTR(TD_ERR_TYPE); }
+    tdd_pred_t P;
+    int rc = compile_pred(x, gr, op->inputs[1], &P);
+    if (rc) { td_release(gr); return TD_ERR_PTR(rc); }
+    td_t* s = eval_sel(x, &P, td_table_nrows(gr), NULL);
+    if (TD_IS_ERR(s)) { td_release(gr); return s; }
+    td_t* out = compact_table(x, gr, s);
+    td_release(s);
+    td_release(gr);
+    return out;
+}
+
 /* ---------------------------------------------------------- dispatcher -- */
 
 static td_t* xnode(xctx_t* x, td_op_t* op) {
@@ -929,7 +1062,7 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
         return e->literal;
     }
     case OP_FILTER: {
-        if (op->inputs[0]->opcode == OP_GROUP) return TD_ERR_PTR(TD_ERR_NYI);   /* HAVING */
+        if (op->inputs[0]->opcode == OP_GROUP) return exec_having(x, op);
         td_t* input = xnode(x, op->inputs[0]);
         if (TD_IS_ERR(input)) return input;
         tdd_pred_t P;
@@ -965,6 +1098,8 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
         return xnode(x, op->inputs[0]);
     case OP_HEAD: case OP_TAIL:
         return exec_head_tail(x, op);
+    case OP_SORT:
+        return exec_sort(x, op, -1);
     default:
         return TD_ERR_PTR(TD_ERR_NYI);
     }

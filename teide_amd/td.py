@@ -111,6 +111,7 @@ class Engine:
             "td_const_str": (P, [P, C.c_char_p]), "td_const_table": (P, [P, P]),
             "td_const_vec": (P, [P, P]),
             "td_filter": (P, [P, P, P]), "td_group": (P, [P, P, U8, P, P, U8]),
+            "td_sort_op": (P, [P, P, P, P, P, U8]),
             "td_join": (P, [P, P, P, P, P, U8, U8]), "td_optimize": (P, [P, P]),
             "td_execute": (P, [P, P]), "td_head": (P, [P, P, I64]), "td_tail": (P, [P, P, I64]),
             "td_sel_from_pred": (P, [P]), "td_sel_and": (P, [P, P]), "td_sel_new": (P, [I64]),
@@ -399,6 +400,15 @@ class Graph:
 
     def head(self, inp, n):
         return self.L.td_head(self.g, inp, n)
+
+    def sort(self, inp, keys, descs, nulls_first=None):
+        """td_sort_op: sort `inp` (a table node) by key nodes."""
+        n = len(keys)
+        ka = (C.c_void_p * n)(*keys)
+        da = (C.c_uint8 * n)(*[int(d) for d in descs])
+        nf = (C.c_uint8 * n)(*[int(f) for f in (nulls_first or descs)])
+        return self.L.td_sort_op(self.g, inp, C.addressof(ka), C.addressof(da),
+                                 C.addressof(nf), n)
 
     def optimize(self, root):
         return self.L.td_optimize(self.g, root)

@@ -39,6 +39,10 @@ CASES = {
                                 **W.join_dim(8000, 1000000, 5)}),
     "hc_60k": (["hc", "60000", "4096", "2048", "42"],
                lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
+    "having_60k": (["having", "60000", "4096", "2048", "42", "29", "0.0", "10"],
+                   lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
+    "sort_8k": (["sort", "8000", "4096", "2048", "42", "25"],
+                lambda: W.high_card(8000, 4096, 2048, 1.1, 42)),
     "join_inner_small": (["join", "3000", "2000", "2500", "0", "42"], None),
     "join_left_small": (["join", "3000", "2000", "2500", "1", "42"], None),
     "join_full_small": (["join", "3000", "2000", "2500", "2", "42"], None),
@@ -59,6 +63,8 @@ def main():
         outs = []
         if argv[0] == "hc":
             outs = [f"{name}_key.tdt", f"{name}_sym.tdt"]
+        elif argv[0] in ("having", "sort"):
+            outs = [f"{name}.tdt", f"{name}_top.tdt"]
         else:
             outs = [f"{name}.tdt"]
         subprocess.run([HARNESS] + argv + [os.path.join(HERE, o) for o in outs], check=True)

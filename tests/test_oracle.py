@@ -135,6 +135,73 @@ def test_hc_matches_reference(by):
                    exact=r["comp"], sum_abs=r["sum_abs"])
 
 
+def oracle_having(d, minc, maxs):
+    """GROUP(sym; SUM v_f64, COUNT v_i64, MAX v_i64) then the HAVING filter."""
+    r = orc.group([d["sym"]], ["SUM", "COUNT", "MAX"], [d["v_f64"], d["v_i64"], d["v_i64"]],
+                  key_types=[(20, 2)])
+    keep = (r["vals"][1] >= minc) & (r["vals"][0] < maxs)
+    return ([r["keys"][0][keep]], [v[keep] for v in r["vals"]],
+            [c[keep] for c in r["comp"]], [a[keep] for a in r["sum_abs"]])
+
+
+def test_having_matches_reference():
+    # FILTER over GROUP (exec.c:11674-11704), golden from the reference
+    argv = MANIFEST["having_60k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("having_60k.tdt")
+    assert [c[0] for c in ref] == ["sym", "v_f64_sum", "v_i64_count", "v_i64_max"]
+    keys, vals, comp, sabs = oracle_having(d, int(argv[5]), float(argv[6]))
+    compare_groups(keys, vals, [ref[0][3]], [c[3] for c in ref[1:]], [True, False, False],
+                   "having", exact=comp, sum_abs=sabs)
+    np.testing.assert_array_equal(keys[0], ref[0][3])          # group (ascending) order
+    # top groups by sum: the reference's heap top-N over the HAVING result
+    top = gold("having_60k_top.tdt")
+    order = orc.sort_perm([vals[0]], [True])[: int(argv[7])]
+    np.testing.assert_array_equal(keys[0][order], top[0][3])
+
+
+def test_sort_matches_reference():
+    # SORT by (key asc, v_i64 desc) and HEAD(SORT(v_f64 desc), 25), goldens
+    # from the reference (radix / merge sort and heap top-N, exec.c:3370-3862)
+    argv = MANIFEST["sort_8k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("sort_8k.tdt")
+    assert [c[0] for c in ref] == ["key", "sym", "v_i64", "v_f64"]
+    p = orc.sort_perm([d["key"], d["v_i64"]], [False, True])
+    for c in ref:
+        np.testing.assert_array_equal(np.asarray(d[c[0]])[p].view(np.uint8),
+                                      np.asarray(c[3]).view(np.uint8), err_msg=c[0])
+    top = gold("sort_8k_top.tdt")
+    p = orc.sort_perm([d["v_f64"]], [True])[: int(argv[5])]
+    for c in top:
+        np.testing.assert_array_equal(np.asarray(d[c[0]])[p], c[3], err_msg=c[0])
+
+
+def test_sort_words_order():
+    # the encoding orders like the values: brute force on a mixed f64 column
+    # (negatives, zeros of both signs, infinities; NaN placed by nulls_first)
+    x = np.array([3.5, -0.0, 0.0, -np.inf, np.inf, -2.25, np.nan, 1e-300, -1e300, 7.0])
+    for desc in (False, True):
+        for nf in (False, True):
+            w = orc.sort_words(x, desc, nf)
+            order = np.argsort(w, kind="stable")
+            vals = x[order]
+            finite = vals[~np.isnan(vals)]
+            exp = np.sort(x[~np.isnan(x)])
+            exp = exp[::-1] if desc else exp
+            np.testing.assert_array_equal(finite.view(np.uint64)[finite != 0],
+                                          exp.view(np.uint64)[exp != 0])
+            nan_first = np.isnan(vals[0])
+            assert nan_first == nf, (desc, nf)
+    i = np.array([5, -3, 0, np.iinfo(np.int64).min, np.iinfo(np.int64).max, -1], np.int64)
+    np.testing.assert_array_equal(i[np.argsort(orc.sort_words(i), kind="stable")], np.sort(i))
+    np.testing.assert_array_equal(i[np.argsort(orc.sort_words(i, True), kind="stable")],
+                                  np.sort(i)[::-1])
+    # symbol ranks: bytes, then length ("ab" < "abc" < "b"), ties by id
+    r = orc.sym_rank([b"b", b"abc", b"ab", b"", b"abc"])
+    assert r.tolist() == [4, 2, 1, 0, 3]
+
+
 def test_group_kats():
     # test_graph.c /group_sum: ids {1,2,3} -> {200, 150, 200} (test_graph.c:154-205)
     id1 = np.array([1, 1, 2, 2, 3, 3, 1, 2, 3, 1], np.int64)
