@@ -505,6 +505,52 @@ class Served:
         self.e.release(self.td)
 
 
+def load_summary(e, a):
+    """Splayed-table load into HBM: the join workload's fact columns saved
+    in the reference's on-disk format, then loaded (file in the page cache)
+    straight into the arena (td_dev_splay_load), beside the host load +
+    td_dev_put path."""
+    import shutil
+    import tempfile
+    import ctypes as C
+    from teide_amd import workloads as W
+    n = a.load_rows
+    fact = W.join_fact(n, max(1, n // 12), SEED)
+    t = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+    del fact
+    tmp = tempfile.mkdtemp(prefix="teide_load_", dir="/tmp")
+    try:
+        d, sym = os.path.join(tmp, "fact").encode(), os.path.join(tmp, "fact.sym").encode()
+        if e.L.td_splay_save(t, d, sym) != 0:
+            raise RuntimeError("td_splay_save failed")
+        e.release(t)
+        nbytes = n * 24
+        dev, host = [], []
+        for _ in range(a.load_reps):
+            t0 = time.perf_counter()
+            r = e.L.td_dev_splay_load(d, None)
+            e.L.td_dev_sync()
+            dev.append(time.perf_counter() - t0)
+            if not r or r < 32:
+                raise RuntimeError(f"td_dev_splay_load failed: {r}")
+            e.release(r)
+            t0 = time.perf_counter()
+            h = e.L.td_splay_load(d)
+            r = e.L.td_dev_put(h)
+            e.L.td_dev_sync()
+            host.append(time.perf_counter() - t0)
+            e.release(r)
+            e.release(h)
+        dv, hv = sorted(dev)[len(dev) // 2], sorted(host)[len(host) // 2]
+        return {"workload": f"splayed fact {n:,} rows x 3 columns (fk, qty, price), "
+                            f"{nbytes / 1e9:.2f} GB, file in the page cache",
+                "dev_load_gbs": round(nbytes / dv / 1e9, 2), "dev_load_s": round(dv, 4),
+                "host_load_then_put_gbs": round(nbytes / hv / 1e9, 2),
+                "host_load_then_put_s": round(hv, 4), "reps": a.load_reps}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def run_form(fn):
     try:
         return fn()
@@ -554,10 +600,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served"], default="jg",
+    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load"], default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
-                         "served: request stream through the request server")
+                         "served: request stream through the request server; "
+                         "load: splayed-table load into HBM")
+    ap.add_argument("--load-rows", type=int, default=50_000_000)
+    ap.add_argument("--load-reps", type=int, default=3)
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-fact", type=int, default=100_000_000)
     ap.add_argument("--serve-dim", type=int, default=8_000_000)
@@ -594,6 +643,18 @@ def main():
     if a.calibrate_copy:
         calibrate_copy(e, a.calibrate_copy << 20)
     e.force_paths(a.join_path, 0, 0)
+
+    if a.workload == "load":
+        r = load_summary(e, a)
+        line = {"metric": "splayed table file -> HBM load rate", "value": r["dev_load_gbs"],
+                "unit": "GB/s", "n_gpus": 1, "steps": r["reps"], "warmup": 0,
+                "ms_per_step": round(r["dev_load_s"] * 1e3, 3), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "i64/f64",
+                "data": "synthetic (seeded generator, saved in the reference's column format)",
+                "config": {"workload": r.pop("workload")}, "roofline": None, "cpu_baseline": None}
+        line.update(r)
+        print(json.dumps(line), flush=True)
+        return
 
     if a.workload == "served":
         wl = Served(e, a)

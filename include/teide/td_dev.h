@@ -33,6 +33,12 @@ td_t*    td_dev_put(td_t* v);
 td_t*    td_dev_get(td_t* v);
 /* 1 when v (or every column of table v) is device-resident. */
 int      td_dev_is_resident(td_t* v);
+/* Column files and splayed tables (the reference's on-disk format,
+ * store/col.c, store/splay.c) loaded straight into the HBM arena, through
+ * pinned staging buffers, with the statistics td_dev_put computes.  The
+ * splayed form loads the symbol table first when sym_path is given. */
+td_t*    td_dev_col_load(const char* path);
+td_t*    td_dev_splay_load(const char* dir, const char* sym_path);
 /* Wait for all queued device work. */
 td_err_t td_dev_sync(void);
 

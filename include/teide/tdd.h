@@ -43,6 +43,11 @@ int  tdd_h2d(void* dst, const void* src, size_t bytes);
 int  tdd_d2h(void* dst, const void* src, size_t bytes);
 int  tdd_d2d(void* dst, const void* src, size_t bytes);
 int  tdd_fill(void* dst, uint64_t pattern, size_t bytes);  /* kernel fill, 8-byte pattern */
+/* bytes of open file fd from offset off into device memory dst: chunks read
+ * into two pinned staging buffers in turn (several reader threads per
+ * chunk), each copied asynchronously while the next is read.  The
+ * reference reads column files into host memory (store/col.c:449-560). */
+int  tdd_file_h2d(int fd, int64_t off, void* dst, size_t bytes);
 int  tdd_sync(void);
 int  tdd_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
 

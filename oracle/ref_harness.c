@@ -22,6 +22,7 @@
  *   ref_harness join   NL NR KEYRANGE JTYPE SEED OUT
  *   ref_harness having N NVALUES NSYM SEED MINCOUNT MAXSUM TOP OUT OUTTOP
  *   ref_harness sort   N NVALUES NSYM SEED TOP OUT OUTTOP
+ *   ref_harness save   N NVALUES NSYM SEED DIR SYMFILE   splayed table + symbols
  *   ref_harness bench  WORKLOAD ARGS... REPS  -> one JSON line (median ms)
  * Output files use the TDT1 column format read by tests/tdtio.py.
  */
@@ -271,6 +272,16 @@ static td_t* run_sort(td_t* t, int64_t top) {
     return out;
 }
 
+/* The high-cardinality recipe's table saved by the reference as a splayed
+ * directory (store/splay.c:46-101) with its symbol file. */
+static void run_save(td_t* t, const char* dir, const char* sym_path) {
+    td_err_t e = td_splay_save(t, dir, sym_path);
+    if (e != TD_OK) {
+        fprintf(stderr, "ref_harness: td_splay_save failed (err %d)\n", (int)e);
+        exit(3);
+    }
+}
+
 /* GROUP(k; SUM/MIN/MAX v_i64, SUM/MIN/MAX v_f64, COUNT v_i64) */
 static td_t* run_hc_group(td_t* t, const char* k) {
     td_graph_t* g = td_graph_new(t);
@@ -454,6 +465,12 @@ int main(int argc, char** argv) {
         write_table(argv[8], b);
         td_release(a);
         td_release(b);
+        td_release(t);
+    } else if (!strcmp(cmd, "save") && argc == 8) {
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        run_save(t, argv[6], argv[7]);
         td_release(t);
     } else if (!strcmp(cmd, "join") && argc == 8) {
         td_t* o = run_join(atoll(argv[2]), atoll(argv[3]), atoll(argv[4]), atoi(argv[5]),

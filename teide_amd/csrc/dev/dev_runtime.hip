@@ -6,6 +6,8 @@
 #include "dev_common.h"
 
 #include <mutex>
+#include <thread>
+#include <unistd.h>
 #include <stdio.h>
 #include <string.h>
 #include <vector>
@@ -154,6 +156,69 @@ extern "C" int tdd_free(void* dptr) {
 extern "C" int tdd_h2d(void* dst, const void* src, size_t bytes) {
     if (!bytes) return TD_OK;
     HC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    HC(hipStreamSynchronize(g_stream));
+    return TD_OK;
+}
+
+/* ------------------------------------------------------ file staging -- */
+
+#define STAGE_BYTES   ((size_t)64 << 20)   /* per staging buffer */
+#define STAGE_READERS 4                    /* reader threads per chunk */
+
+static std::mutex g_stage_lock;
+static void* g_stage[2];
+static hipEvent_t g_stage_ev[2];
+
+static bool pread_all(int fd, char* p, size_t n, int64_t off) {
+    while (n) {
+        ssize_t r = pread(fd, p, n, (off_t)off);
+        if (r <= 0) return false;
+        p += r;
+        n -= (size_t)r;
+        off += r;
+    }
+    return true;
+}
+
+extern "C" int tdd_file_h2d(int fd, int64_t off, void* dst, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_stage_lock);
+    if (!g_stage[0]) {
+        for (int b = 0; b < 2; b++) {
+            HC(hipHostMalloc(&g_stage[b], STAGE_BYTES, hipHostMallocDefault));
+            HC(hipEventCreateWithFlags(&g_stage_ev[b], hipEventDisableTiming));
+            HC(hipEventRecord(g_stage_ev[b], g_stream));
+        }
+    }
+    size_t pos = 0;
+    int b = 0;
+    while (pos < bytes) {
+        const size_t n = bytes - pos < STAGE_BYTES ? bytes - pos : STAGE_BYTES;
+        HC(hipEventSynchronize(g_stage_ev[b]));          /* its previous copy is done */
+        char* buf = (char*)g_stage[b];
+        const size_t part = ((n + STAGE_READERS - 1) / STAGE_READERS + 4095) & ~(size_t)4095;
+        bool ok[STAGE_READERS];
+        std::thread th[STAGE_READERS];
+        for (int r = 0; r < STAGE_READERS; r++) {
+            const size_t s0 = (size_t)r * part;
+            ok[r] = true;
+            if (s0 >= n) continue;
+            const size_t len = n - s0 < part ? n - s0 : part;
+            th[r] = std::thread([=, &ok] { ok[r] = pread_all(fd, buf + s0, len, off + (int64_t)(pos + s0)); });
+        }
+        bool good = true;
+        for (int r = 0; r < STAGE_READERS; r++) {
+            if (th[r].joinable()) th[r].join();
+            good = good && ok[r];
+        }
+        if (!good) {
+            HC(hipStreamSynchronize(g_stream));
+            return TD_ERR_IO;
+        }
+        HC(hipMemcpyAsync((char*)dst + pos, buf, n, hipMemcpyHostToDevice, g_stream));
+        HC(hipEventRecord(g_stage_ev[b], g_stream));
+        pos += n;
+        b ^= 1;
+    }
     HC(hipStreamSynchronize(g_stream));
     return TD_OK;
 }

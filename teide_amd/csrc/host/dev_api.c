@@ -42,6 +42,30 @@ static size_t payload_bytes(td_t* v) {
     return (size_t)v->len * td__esz(v);
 }
 
+/* Column statistics for an immutable resident vector: min / max of an
+ * integer column, computed once so queries size direct maps and group
+ * tables without a scan. */
+int td__dev_stats(td_t* d) {
+    int8_t t = d->type;
+    int int_col = t == TD_BOOL || t == TD_U8 || t == TD_I16 || t == TD_I32 || t == TD_I64 ||
+                  t == TD_DATE || t == TD_TIME || t == TD_TIMESTAMP || t == TD_SYM;
+    if (!int_col || d->len <= 0) return TD_OK;
+    tdd_col_t c;
+    memset(&c, 0, sizeof(c));
+    c.data = td_hbm_ref(d)->dptr;
+    c.type = t;
+    c.attrs = d->attrs & TD_SYM_W_MASK;
+    c.esz = (uint8_t)td__esz(d);
+    int64_t lo, hi;
+    int rc = tdd_minmax(&c, d->len, NULL, &lo, &hi);
+    if (!rc) {
+        td_hbm_ref(d)->lo = lo;
+        td_hbm_ref(d)->hi = hi;
+        td_hbm_ref(d)->ranged = 1;
+    }
+    return rc;
+}
+
 /* Device-resident copy of one vector / selection. */
 static td_t* put_vec(td_t* v) {
     if (v->type <= 0 || v->type == TD_TABLE || v->type == TD_LIST || TD_IS_PARTED(v->type) ||
@@ -67,27 +91,7 @@ static td_t* put_vec(td_t* v) {
         rc = tdd_h2d(td_hbm_ref(d)->dptr, src, bytes);
     }
 done:
-    int8_t t = v->type;
-    int int_col = t == TD_BOOL || t == TD_U8 || t == TD_I16 || t == TD_I32 || t == TD_I64 ||
-                  t == TD_DATE || t == TD_TIME || t == TD_TIMESTAMP || t == TD_SYM;
-    if (!rc && int_col && v->len > 0) {
-        /* Column statistics for the immutable resident copy: min / max of an
-         * integer column, computed once here so queries size direct maps and
-         * group tables without a scan. */
-        tdd_col_t c;
-        memset(&c, 0, sizeof(c));
-        c.data = td_hbm_ref(d)->dptr;
-        c.type = v->type;
-        c.attrs = v->attrs & TD_SYM_W_MASK;
-        c.esz = (uint8_t)td__esz(v);
-        int64_t lo, hi;
-        rc = tdd_minmax(&c, v->len, NULL, &lo, &hi);
-        if (!rc) {
-            td_hbm_ref(d)->lo = lo;
-            td_hbm_ref(d)->hi = hi;
-            td_hbm_ref(d)->ranged = 1;
-        }
-    }
+    if (!rc) rc = td__dev_stats(d);
     if (!rc) rc = tdd_sync();
     if (rc) { td_release(d); return TD_ERR_PTR(rc); }
     return d;

@@ -51,6 +51,7 @@ void   td__dev_forget(td_t* v);                /* td_free hook: drop cached uplo
 /* Device-resident vector: host proxy with mmod == 3 and the elements in the
  * arena.  bytes >= len * elem size (TD_SEL: full payload). */
 td_t*  td__hbm_vec_new(int8_t type, uint8_t attrs, int64_t len, size_t bytes);
+int    td__dev_stats(td_t* d);                  /* min / max of a resident integer column */
 td_t*  td__hbm_wrap(int8_t type, uint8_t attrs, int64_t len, void* dptr, size_t bytes);
 void   td__hbm_free(td_t* v);
 /* A resident vector viewing len elements of `owner`'s block from byte

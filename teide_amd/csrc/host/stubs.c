@@ -1,20 +1,11 @@
 /*
  * stubs.c -- td.h entry points outside the filter -> join -> group-by path
- * (on-disk storage, CSV, graph relations).  They exist so that programs
- * written against the reference link unchanged; each reports TD_ERR_NYI.
+ * and its loaders (parted tables, CSV, graph relations).  They exist so that
+ * programs written against the reference link unchanged; each reports
+ * TD_ERR_NYI.  Column files and splayed tables live in storage.c.
  */
 #include "internal.h"
 
-td_err_t td_col_save(td_t* vec, const char* path) { (void)vec; (void)path; return TD_ERR_NYI; }
-td_t* td_col_load(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
-td_t* td_col_mmap(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
-td_err_t td_splay_save(td_t* tbl, const char* dir, const char* sym_path) {
-    (void)tbl; (void)dir; (void)sym_path; return TD_ERR_NYI;
-}
-td_t* td_splay_load(const char* dir) { (void)dir; return TD_ERR_PTR(TD_ERR_NYI); }
-td_t* td_read_splayed(const char* dir, const char* sym_path) {
-    (void)dir; (void)sym_path; return TD_ERR_PTR(TD_ERR_NYI);
-}
 td_t* td_part_load(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
 td_t* td_read_parted(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
 td_err_t td_meta_save_d(td_t* schema, const char* path) { (void)schema; (void)path; return TD_ERR_NYI; }

@@ -112,6 +112,11 @@ class Engine:
             "td_const_vec": (P, [P, P]),
             "td_filter": (P, [P, P, P]), "td_group": (P, [P, P, U8, P, P, U8]),
             "td_sort_op": (P, [P, P, P, P, P, U8]),
+            "td_col_save": (C.c_int, [P, C.c_char_p]), "td_col_load": (P, [C.c_char_p]),
+            "td_col_mmap": (P, [C.c_char_p]),
+            "td_splay_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+            "td_splay_load": (P, [C.c_char_p]), "td_read_splayed": (P, [C.c_char_p, C.c_char_p]),
+            "td_sym_save": (C.c_int, [C.c_char_p]), "td_sym_load": (C.c_int, [C.c_char_p]),
             "td_join": (P, [P, P, P, P, P, U8, U8]), "td_optimize": (P, [P, P]),
             "td_execute": (P, [P, P]), "td_head": (P, [P, P, I64]), "td_tail": (P, [P, P, I64]),
             "td_sel_from_pred": (P, [P]), "td_sel_and": (P, [P, P]), "td_sel_new": (P, [I64]),
@@ -137,6 +142,8 @@ class Engine:
                 "td_dev_batch_run": (C.c_int, [P, P]),
                 "td_dev_batch_info": (I64, [P, P, P]),
                 "td_dev_batch_free": (None, [P]),
+                "td_dev_col_load": (P, [C.c_char_p]),
+                "td_dev_splay_load": (P, [C.c_char_p, C.c_char_p]),
                 "td_dev_serve_open": (P, [P]), "td_dev_serve_submit": (I64, [P, P, P]),
                 "td_dev_serve_step": (C.c_int, [P]), "td_dev_serve_poll": (C.c_int, [P, P, P]),
                 "td_dev_serve_info": (None, [P, P]), "td_dev_serve_close": (None, [P]),

@@ -43,6 +43,8 @@ CASES = {
                    lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "sort_8k": (["sort", "8000", "4096", "2048", "42", "25"],
                 lambda: W.high_card(8000, 4096, 2048, 1.1, 42)),
+    "splay_hc3k": (["save", "3000", "4096", "2048", "42"],
+                   lambda: W.high_card(3000, 4096, 2048, 1.1, 42)),
     "join_inner_small": (["join", "3000", "2000", "2500", "0", "42"], None),
     "join_left_small": (["join", "3000", "2000", "2500", "1", "42"], None),
     "join_full_small": (["join", "3000", "2000", "2500", "2", "42"], None),
@@ -65,6 +67,17 @@ def main():
             outs = [f"{name}_key.tdt", f"{name}_sym.tdt"]
         elif argv[0] in ("having", "sort"):
             outs = [f"{name}.tdt", f"{name}_top.tdt"]
+        elif argv[0] == "save":
+            # a splayed directory written by the reference, with its symbol
+            # file: the on-disk format the loaders must read byte for byte
+            import shutil
+            shutil.rmtree(os.path.join(HERE, name), ignore_errors=True)
+            subprocess.run([HARNESS] + argv + [os.path.join(HERE, name),
+                                                os.path.join(HERE, name + ".sym")], check=True)
+            manifest[name] = {"argv": argv, "outputs": [name, name + ".sym"],
+                              "inputs_sha256": {k: digest(v) for k, v in gen().items()}}
+            print(name, "splayed")
+            continue
         else:
             outs = [f"{name}.tdt"]
         subprocess.run([HARNESS] + argv + [os.path.join(HERE, o) for o in outs], check=True)
