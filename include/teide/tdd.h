@@ -247,6 +247,37 @@ int  tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
                       int64_t* offsets);
 void tdd_jg_batch_free(tdd_jg_batch_t* b);
 
+/* ---------------------------------------------------------- expressions -- */
+/* An element-wise expression compiled to registers and instructions, the
+ * reference's expression compiler form (exec.c:696-918): registers are
+ * columns, constants or temporaries, each computed as TD_I64, TD_F64 or
+ * TD_BOOL; an instruction applies one opcode (OP_NEG .. OP_CAST, OP_ADD ..
+ * OP_MAX2) to one or two registers; promotions are explicit OP_CASTs. */
+#define TDD_EXPR_MAX_REGS 16
+#define TDD_EXPR_MAX_INS  48
+enum { TDD_R_COL = 0, TDD_R_CONST = 1, TDD_R_TMP = 2 };
+typedef struct { uint8_t op, dst, src1, src2; } tdd_expr_ins_t;   /* src2 0xFF: unary */
+typedef struct {
+    uint8_t   kind;
+    int8_t    type;
+    uint8_t   pad[6];
+    tdd_col_t col;
+    double    cf;
+    int64_t   ci;
+} tdd_expr_reg_t;
+typedef struct {
+    int32_t        n_regs, n_ins, out_reg, pad;
+    tdd_expr_reg_t regs[TDD_EXPR_MAX_REGS];
+    tdd_expr_ins_t ins[TDD_EXPR_MAX_INS];
+} tdd_expr_t;
+
+/* out[0..n) = the expression over rows 0..n: 8-byte i64 / f64 or 1-byte
+ * bool elements by the output register's type.  Arithmetic as the
+ * reference evaluates it (exec.c:960-1100): i64 wrapping, DIV / MOD by zero
+ * (and INT64_MIN / -1) give 0, f64 DIV / MOD by zero give 0, CAST f64 ->
+ * i64 clamps. */
+int tdd_expr_eval(const tdd_expr_t* e, int64_t n, void* out);
+
 /* ----------------------------------------------------------------- sort -- */
 /* One sort key: a column, its direction and NaN placement, and for SYM
  * columns the rank of every symbol id by its string (device array). */

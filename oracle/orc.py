@@ -241,3 +241,217 @@ def sym_rank(strings):
     rank = np.empty(len(strings), np.uint32)
     rank[np.asarray(order, np.int64)] = np.arange(len(strings), dtype=np.uint32)
     return rank
+
+
+# ---------------------------------------------------------------- expressions
+# A restatement of the reference's fused expression path: typing as
+# expr_compile (exec.c:760-918), arithmetic as the morsel evaluator
+# (exec.c:980-1100).  An expression is a nested tuple:
+#   ("col", array)  ("i64", v)  ("f64", v)  ("cast", e, "i64"|"f64")
+#   (op, e) for NEG ABS NOT SQRT LOG EXP CEIL FLOOR
+#   (op, e1, e2) for ADD SUB MUL DIV MOD MIN2 MAX2 EQ NE LT LE GT GE AND OR
+# and evaluates to (type, array) with type "i64", "f64" or "bool".
+
+_CMP = {"EQ": np.equal, "NE": np.not_equal, "LT": np.less, "LE": np.less_equal,
+        "GT": np.greater, "GE": np.greater_equal}
+_I64_MIN = np.int64(-2 ** 63)
+
+
+def _as(t, v, to):
+    """expr_ensure_type's CAST (exec.c:743-757) and the unary CAST rules."""
+    if t == to:
+        return v
+    if to == "f64":
+        return v.astype(np.float64)
+    if t == "f64":                                   # clamp, exec.c:1072-1077
+        # NaN fails both clamps and reaches the C cast, whose x86 result
+        # (cvttsd2si's integer indefinite) is INT64_MIN
+        out = np.where(v >= 9.223372036854775807e18, np.iinfo(np.int64).max,
+                       np.iinfo(np.int64).min)
+        mid = (v < 9.223372036854775807e18) & (v > -9.223372036854775808e18)
+        out = out.astype(np.int64)
+        out[mid] = np.trunc(v[mid]).astype(np.int64)
+        return out
+    return v.astype(np.int64)
+
+
+def expr_eval(e):
+    """(type, values) of expression `e` over its columns' rows."""
+    kind = e[0]
+    if kind == "col":
+        a = np.asarray(e[1])
+        return ("f64", a.astype(np.float64)) if a.dtype == np.float64 else \
+            ("i64", a.astype(np.int64))
+    if kind == "i64":
+        return "i64", np.int64(e[1])
+    if kind == "f64":
+        return "f64", np.float64(e[1])
+    if kind == "bool":                               # a folded constant: an i64 register
+        return "i64", np.int64(e[1])
+    if kind == "cast":
+        t, v = expr_eval(e[1])
+        return e[2], _as(t, np.asarray(v), e[2])
+    op = kind
+    t1, a = expr_eval(e[1])
+    t2, b = expr_eval(e[2]) if len(e) > 2 else (t1, None)
+    if op in _CMP or op in ("AND", "OR", "NOT"):
+        ot = "bool"
+    elif "f64" in (t1, t2) or op in ("DIV", "SQRT", "LOG", "EXP"):
+        ot = "f64"
+    else:
+        ot = "i64"
+    if ot == "f64":
+        a = _as(t1, np.asarray(a), "f64")
+        if b is not None:
+            b = _as(t2, np.asarray(b), "f64")
+    elif ot == "bool" and b is not None and t1 != t2:
+        pt = "f64" if "f64" in (t1, t2) else "i64"
+        a, b = _as(t1, np.asarray(a), pt), _as(t2, np.asarray(b), pt)
+    with np.errstate(all="ignore"):
+        if ot == "bool":
+            if op == "NOT":
+                return ot, np.logical_not(a)
+            if op == "AND":
+                return ot, np.logical_and(a, b)
+            if op == "OR":
+                return ot, np.logical_or(a, b)
+            return ot, _CMP[op](a, b)
+        if ot == "f64":
+            a, b = np.asarray(a, np.float64), (None if b is None else np.asarray(b, np.float64))
+            if op in ("NEG", "ABS", "SQRT", "LOG", "EXP", "CEIL", "FLOOR"):
+                f = {"NEG": np.negative, "ABS": np.abs, "SQRT": np.sqrt, "LOG": np.log,
+                     "EXP": np.exp, "CEIL": np.ceil, "FLOOR": np.floor}[op]
+                return ot, f(a)
+            a, b = np.broadcast_arrays(a, b)
+            if op == "ADD":
+                return ot, a + b
+            if op == "SUB":
+                return ot, a - b
+            if op == "MUL":
+                return ot, a * b
+            if op == "DIV":
+                return ot, np.where(b != 0, a / np.where(b != 0, b, 1), 0.0)
+            if op == "MOD":
+                return ot, np.where(b != 0, np.fmod(a, np.where(b != 0, b, 1)), 0.0)
+            if op == "MIN2":
+                return ot, np.where(a < b, a, b)
+            if op == "MAX2":
+                return ot, np.where(a > b, a, b)
+        # i64: two's-complement wrapping (exec.c:1003-1013, 1060-1064)
+        a = np.asarray(a, np.int64)
+        if op == "NEG":
+            return ot, (np.uint64(0) - a.astype(np.uint64)).astype(np.int64)
+        if op == "ABS":
+            return ot, np.where(a < 0, (np.uint64(0) - a.astype(np.uint64)).astype(np.int64), a)
+        if op in ("CEIL", "FLOOR"):
+            return ot, a            # no i64 case in the reference's unary switch
+        a, b = np.broadcast_arrays(a, np.asarray(b, np.int64))
+        ua, ub = a.astype(np.uint64), b.astype(np.uint64)
+        if op == "ADD":
+            return ot, (ua + ub).astype(np.int64)
+        if op == "SUB":
+            return ot, (ua - ub).astype(np.int64)
+        if op == "MUL":
+            return ot, (ua * ub).astype(np.int64)
+        if op == "MOD":
+            ok = (b != 0) & ~((a == _I64_MIN) & (b == -1))
+            return ot, np.where(ok, np.fmod(a, np.where(ok, b, 1)), 0).astype(np.int64)
+        if op == "MIN2":
+            return ot, np.where(a < b, a, b)
+        if op == "MAX2":
+            return ot, np.where(a > b, a, b)
+    raise ValueError(f"expression op {op}")
+
+
+def _const(e):
+    return e[0] in ("i64", "f64", "bool")
+
+
+def _num(e):
+    """(int value, float value, is_f64) of a constant, as atom_to_numeric."""
+    if e[0] == "f64":
+        f = float(e[1])
+        return (int(f) if np.isfinite(f) else 0), f, True
+    i = int(e[1])
+    return i, float(i), False
+
+
+def _wrap(v):
+    return int(np.array(v % (1 << 64), np.uint64).astype(np.int64))
+
+
+def expr_fold(e):
+    """The tree after the reference optimizer's constant folding
+    (opt.c:305-447): element-wise nodes over constants become constants,
+    typed by what the graph builders declared (DIV f64, comparisons and
+    logic bool, otherwise the wider operand type) and computed with IEEE
+    division, fmod, fmin / fmax for f64 and wrapping guarded arithmetic for
+    i64; a negation of INT64_MIN is left to the evaluator."""
+    if e[0] in ("col", "i64", "f64", "bool"):
+        return e
+    if e[0] == "cast":
+        return ("cast", expr_fold(e[1]), e[2])
+    op = e[0]
+    kids = [expr_fold(c) for c in e[1:]]
+    if not all(_const(k) for k in kids):
+        return (op,) + tuple(kids)
+    with np.errstate(all="ignore"):
+        if len(kids) == 1:
+            i, f, isf = _num(kids[0])
+            x = f if isf else float(i)
+            if op == "NEG":
+                return ("f64", -f) if isf else ((op,) + tuple(kids) if i == -2 ** 63
+                                                 else ("i64", _wrap(-i)))
+            if op == "ABS":
+                return ("f64", abs(f)) if isf else ((op,) + tuple(kids) if i == -2 ** 63
+                                                     else ("i64", abs(i)))
+            if op == "NOT":
+                return ("bool", int(f == 0.0 if isf else i == 0))
+            if op in ("SQRT", "LOG", "EXP"):
+                fn = {"SQRT": np.sqrt, "LOG": np.log, "EXP": np.exp}[op]
+                return ("f64", float(fn(np.float64(x))))
+            if op in ("CEIL", "FLOOR"):
+                return ("f64", float((np.ceil if op == "CEIL" else np.floor)(f))) if isf \
+                    else ("i64", i)
+            return (op,) + tuple(kids)
+        (li, lf, lfl), (ri, rf, rfl) = _num(kids[0]), _num(kids[1])
+        types = {k[0] for k in kids}
+        if op == "DIV":
+            t = "f64"
+        elif op in _CMP or op in ("AND", "OR"):
+            t = "bool"
+        else:
+            t = "f64" if "f64" in types else "i64" if "i64" in types else "bool"
+        a = np.float64(lf if lfl else li)
+        b = np.float64(rf if rfl else ri)
+        if t == "f64":
+            r = {"ADD": lambda: a + b, "SUB": lambda: a - b, "MUL": lambda: a * b,
+                 "DIV": lambda: a / b, "MOD": lambda: np.fmod(a, b),
+                 "MIN2": lambda: np.fmin(a, b), "MAX2": lambda: np.fmax(a, b)}.get(op)
+            return ("f64", float(r())) if r else (op,) + tuple(kids)
+        if t == "i64":
+            x, y = li, ri
+            ok = y != 0 and not (x == -2 ** 63 and y == -1)
+            if op == "ADD":
+                return ("i64", _wrap(x + y))
+            if op == "SUB":
+                return ("i64", _wrap(x - y))
+            if op == "MUL":
+                return ("i64", _wrap(x * y))
+            if op in ("DIV", "MOD"):
+                if not ok:
+                    return ("i64", 0)
+                q = abs(x) // abs(y) * (1 if (x < 0) == (y < 0) else -1)
+                return ("i64", _wrap(q if op == "DIV" else x - q * y))
+            if op == "MIN2":
+                return ("i64", min(x, y))
+            if op == "MAX2":
+                return ("i64", max(x, y))
+            return (op,) + tuple(kids)
+        if op in _CMP:
+            return ("bool", int(_CMP[op](a, b)))
+        if op == "AND":
+            return ("bool", int(a != 0 and b != 0))
+        if op == "OR":
+            return ("bool", int(a != 0 or b != 0))
+        return (op,) + tuple(kids)

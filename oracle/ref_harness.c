@@ -297,6 +297,89 @@ static td_t* run_hc_group(td_t* t, const char* k) {
     return out;
 }
 
+/* Element-wise expressions (exec.c:760-1260).  MODE "grp": the table
+ * filtered by an arithmetic predicate, then GROUP(sym) of expression inputs;
+ * MODE "vec": a table of expression vectors over every row. */
+static td_t* run_expr_grp(td_t* t) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* p1 = td_gt(g, td_add(g, td_mod(g, vi, td_const_i64(g, 7)), td_scan(g, "sym")),
+                        td_const_i64(g, 1000));
+    td_op_t* p2 = td_lt(g, td_sub(g, td_mul(g, vf, td_const_f64(g, 2.5)), td_const_f64(g, 1.0)),
+                        td_add(g, td_abs(g, vf), td_const_f64(g, 1.0)));
+    td_op_t* root = td_filter(g, td_const_table(g, t), td_and(g, p1, p2));
+    root = td_optimize(g, root);
+    td_t* f = check(td_execute(g, root), "expr filter");
+    td_graph_free(g);
+
+    g = td_graph_new(f);
+    vi = td_scan(g, "v_i64");
+    vf = td_scan(g, "v_f64");
+    td_op_t* key = td_scan(g, "key");
+    td_op_t* keys[1] = { td_scan(g, "sym") };
+    td_op_t* ins[6] = {
+        td_sub(g, td_mul(g, vi, td_const_i64(g, 3)), key),
+        td_mul(g, vf, td_cast(g, td_mod(g, vi, td_const_i64(g, 100)), TD_F64)),
+        td_mod(g, key, td_const_i64(g, 1000003)),
+        td_div(g, vf, td_add(g, td_abs(g, td_mod(g, key, td_const_i64(g, 1000))),
+                             td_const_i64(g, 1))),
+        td_sqrt_op(g, td_abs(g, vf)),
+        vi,
+    };
+    uint16_t ops[6] = { OP_SUM, OP_SUM, OP_MAX, OP_MIN, OP_AVG, OP_COUNT };
+    root = td_group(g, keys, 1, ops, ins, 6);
+    root = td_optimize(g, root);
+    td_t* out = check(td_execute(g, root), "expr group");
+    td_graph_free(g);
+    td_release(f);
+    return out;
+}
+
+static td_op_t* expr_vec_op(td_graph_t* g, int i) {
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* key = td_scan(g, "key");
+    td_op_t* sym = td_scan(g, "sym");
+    switch (i) {
+    case 0:   /* f64 transcendental mix */
+        return td_add(g, td_sqrt_op(g, td_abs(g, vf)),
+                      td_mul(g, td_log_op(g, td_add(g, td_abs(g, vf), td_const_f64(g, 1.0))),
+                             td_exp_op(g, td_div(g, vf, td_const_f64(g, 4.0)))));
+    case 1:   /* wrapping i64 arithmetic */
+        return td_sub(g, td_mod(g, td_add(g, td_mul(g, key, td_const_i64(g, 31)), vi),
+                                td_const_i64(g, 1000003)),
+                      td_max2(g, vi, td_min2(g, key, td_const_i64(g, 0))));
+    case 2:   /* f64 vs f64 comparison */
+        return td_gt(g, vf, td_div(g, vi, td_const_f64(g, 1e9)));
+    case 3:   /* mixed f64 / i64 comparison */
+        return td_le(g, td_mul(g, vf, td_const_f64(g, 1e9)), vi);
+    case 4:   /* f64 -> i64 cast, clamped */
+        return td_cast(g, td_mul(g, vf, td_const_f64(g, 3e18)), TD_I64);
+    case 5:
+        return td_neg(g, td_abs(g, td_min2(g, vi, key)));
+    case 6:
+        return td_add(g, td_floor_op(g, td_mul(g, vf, td_const_f64(g, 10.0))),
+                      td_ceil_op(g, vf));
+    default:  /* i64 MOD by zero gives 0 */
+        return td_mod(g, vi, td_mod(g, sym, td_const_i64(g, 3)));
+    }
+}
+
+static td_t* run_expr_vec(td_t* t) {
+    td_t* out = check(td_table_new(8), "expr vec");
+    for (int i = 0; i < 8; i++) {
+        td_graph_t* g = td_graph_new(t);
+        td_op_t* root = td_optimize(g, expr_vec_op(g, i));
+        td_t* v = check(td_execute(g, root), "expr vec");
+        td_graph_free(g);
+        char name[8];
+        snprintf(name, sizeof(name), "e%d", i);
+        out = add(out, name, v);
+    }
+    return out;
+}
+
 /* General equi-join on one i64 key: left (k, lv), right (k, rv). */
 static td_t* run_join(int64_t nl, int64_t nr, int64_t krange, int jtype, uint64_t seed) {
     td_t* lk = vec_of(TD_I64, nl);
@@ -465,6 +548,14 @@ int main(int argc, char** argv) {
         write_table(argv[8], b);
         td_release(a);
         td_release(b);
+        td_release(t);
+    } else if (!strcmp(cmd, "expr") && argc == 8) {
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        td_t* o = !strcmp(argv[6], "grp") ? run_expr_grp(t) : run_expr_vec(t);
+        write_table(argv[7], o);
+        td_release(o);
         td_release(t);
     } else if (!strcmp(cmd, "save") && argc == 8) {
         td_t* t;

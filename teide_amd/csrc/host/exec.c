@@ -300,6 +300,188 @@ static int pred_compile(pcomp_t* c, td_op_t* op) {
     }
 }
 
+/* ---------------------------------------------------------- expressions -- */
+
+static int expr_elementwise(uint16_t op) {
+    return (op >= OP_NEG && op <= OP_CAST && op != OP_ISNULL) || (op >= OP_ADD && op <= OP_MAX2);
+}
+
+#define EXPR_FULL 1000             /* internal: registers or instructions exhausted */
+
+typedef struct {
+    xctx_t*     x;
+    td_t*       tbl;
+    tdd_expr_t* E;
+    td_op_t*    pre_op[2];             /* inputs already evaluated into vectors */
+    td_t*       pre[2];
+    uint8_t     reg_of[4096];          /* node id -> register, 0xFF: none yet */
+} ecomp_t;
+
+static int ereg_new(ecomp_t* c, uint8_t kind, int8_t type) {
+    tdd_expr_t* E = c->E;
+    if (E->n_regs >= TDD_EXPR_MAX_REGS) return -1;
+    int r = E->n_regs++;
+    memset(&E->regs[r], 0, sizeof(E->regs[r]));
+    E->regs[r].kind = kind;
+    E->regs[r].type = type;
+    return r;
+}
+
+static int eins(ecomp_t* c, uint8_t op, int dst, int s1, int s2) {
+    if (c->E->n_ins >= TDD_EXPR_MAX_INS) return -1;
+    c->E->ins[c->E->n_ins++] = (tdd_expr_ins_t){op, (uint8_t)dst, (uint8_t)s1,
+                                                (uint8_t)(s2 < 0 ? 0xFF : s2)};
+    return 0;
+}
+
+/* Register `r` as type t: itself, or a new register holding CAST(r). */
+static int ecast(ecomp_t* c, int r, int8_t t) {
+    if (c->E->regs[r].type == t) return r;
+    int d = ereg_new(c, TDD_R_TMP, t);
+    if (d < 0 || eins(c, OP_CAST, d, r, -1)) return -1;
+    return d;
+}
+
+/* Register of a node (compiled once per node).  Typing as the reference's
+ * expr_compile (exec.c:760-918): a column is f64 if its type is, else i64;
+ * comparisons and logic give bool; arithmetic gives f64 when an operand is
+ * f64 or the op is DIV / SQRT / LOG / EXP, else i64; f64 results promote
+ * their operands, and mixed comparisons compare in the wider type. */
+static int ecompile(ecomp_t* c, td_op_t* op) {
+    td_graph_t* g = c->x->g;
+    if (op->id < 4096 && c->reg_of[op->id] != 0xFF) return c->reg_of[op->id];
+    int r;
+    int pi = op == c->pre_op[0] ? 0 : op == c->pre_op[1] ? 1 : -1;
+    if (pi >= 0) {
+        td_t* v = c->pre[pi];
+        r = ereg_new(c, TDD_R_COL, v->type);
+        if (r < 0) return -EXPR_FULL;
+        int rc = dev_col(c->x, v, &c->E->regs[r].col);
+        if (rc) return -rc;
+    } else if (op->opcode == OP_SCAN) {
+        td_t* col = scan_col(g, op, c->tbl);
+        if (!col) return -TD_ERR_SCHEMA;
+        if (col->type == TD_MAPCOMMON || TD_IS_PARTED(col->type) || col->type <= 0 ||
+            col->type == TD_TABLE || col->type == TD_LIST)
+            return -TD_ERR_NYI;
+        if (col->len != td_table_nrows(c->tbl)) return -TD_ERR_LENGTH;
+        r = ereg_new(c, TDD_R_COL, col->type == TD_F64 ? TD_F64 : TD_I64);
+        if (r < 0) return -EXPR_FULL;
+        int rc = dev_col(c->x, col, &c->E->regs[r].col);
+        if (rc) return -rc;
+    } else if (op->opcode == OP_CONST) {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        int64_t i; double f; int isf;
+        if (!e || !e->literal) return -TD_ERR_NYI;
+        if (!atom_num(e->literal, &i, &f, &isf)) {
+            if (e->literal->type != TD_ATOM_STR) return -TD_ERR_NYI;
+            i = td_sym_find(td_str_ptr(e->literal), td_str_len(e->literal));
+            if (i < 0) return -TD_ERR_NYI;
+            f = (double)i;
+            isf = 0;
+        }
+        r = ereg_new(c, TDD_R_CONST, isf ? TD_F64 : TD_I64);
+        if (r < 0) return -EXPR_FULL;
+        c->E->regs[r].ci = i;
+        c->E->regs[r].cf = isf ? f : (double)i;
+    } else if (expr_elementwise(op->opcode)) {
+        if (!op->inputs[0]) return -TD_ERR_NYI;
+        int s1 = ecompile(c, op->inputs[0]);
+        if (s1 < 0) return s1;
+        int s2 = -1;
+        if (op->arity >= 2 && op->inputs[1]) {
+            s2 = ecompile(c, op->inputs[1]);
+            if (s2 < 0) return s2;
+        }
+        int8_t t1 = c->E->regs[s1].type, t2 = s2 >= 0 ? c->E->regs[s2].type : t1;
+        uint16_t o = op->opcode;
+        int8_t ot;
+        if (o == OP_CAST) {
+            ot = op->out_type;
+            if (ot != TD_F64 && ot != TD_I64) return -TD_ERR_NYI;
+        } else if ((o >= OP_EQ && o <= OP_GE) || o == OP_AND || o == OP_OR || o == OP_NOT) {
+            ot = TD_BOOL;
+        } else if (t1 == TD_F64 || t2 == TD_F64 || o == OP_DIV || o == OP_SQRT || o == OP_LOG ||
+                   o == OP_EXP) {
+            ot = TD_F64;
+        } else {
+            ot = TD_I64;
+        }
+        if (o != OP_CAST && ot == TD_F64) {
+            s1 = ecast(c, s1, TD_F64);
+            if (s2 >= 0) s2 = ecast(c, s2, TD_F64);
+        } else if (ot == TD_BOOL && s2 >= 0 && t1 != t2) {
+            int8_t pt = t1 == TD_F64 || t2 == TD_F64 ? TD_F64 : TD_I64;
+            s1 = ecast(c, s1, pt);
+            s2 = ecast(c, s2, pt);
+        }
+        if (s1 < 0 || (op->arity >= 2 && op->inputs[1] && s2 < 0)) return -EXPR_FULL;
+        r = ereg_new(c, TDD_R_TMP, ot);
+        if (r < 0 || eins(c, (uint8_t)o, r, s1, s2)) return -EXPR_FULL;
+    } else {
+        return -TD_ERR_NYI;
+    }
+    if (op->id < 4096) c->reg_of[op->id] = (uint8_t)r;
+    return r;
+}
+
+/* An element-wise expression over the rows of `tbl`, as a device vector
+ * (TD_I64, TD_F64 or TD_BOOL). */
+static td_t* eval_expr(xctx_t* x, td_t* tbl, td_op_t* op) {
+    if (!tbl || tbl->type != TD_TABLE) return TD_ERR_PTR(TD_ERR_SCHEMA);
+    tdd_expr_t E;
+    ecomp_t* c = calloc(1, sizeof(ecomp_t));
+    if (!c) return TD_ERR_PTR(TD_ERR_OOM);
+    c->x = x;
+    c->tbl = tbl;
+    c->E = &E;
+    int r = -EXPR_FULL;
+    for (int split = 0; split < 2 && r == -EXPR_FULL; split++) {
+        if (split) {
+            /* Too large for one kernel: the element-wise inputs become
+             * vectors of their own (each split again as needed), and the
+             * top operation reads them as columns. */
+            for (int i = 0; i < 2 && i < op->arity; i++) {
+                td_op_t* in = op->inputs[i];
+                if (!in || !expr_elementwise(in->opcode)) continue;
+                td_t* v = eval_expr(x, tbl, in);
+                if (TD_IS_ERR(v)) { r = -(int)TD_ERR_CODE(v); break; }
+                c->pre_op[i] = in;
+                c->pre[i] = v;
+            }
+            if (r != -EXPR_FULL) break;
+        }
+        memset(&E, 0, sizeof(E));
+        memset(c->reg_of, 0xFF, sizeof(c->reg_of));
+        r = ecompile(c, op);
+    }
+    td_t* pre[2] = {c->pre[0], c->pre[1]};
+    free(c);
+    if (r < 0) {
+        for (int i = 0; i < 2; i++) if (pre[i]) td_release(pre[i]);
+        return TD_ERR_PTR(r == -EXPR_FULL ? TD_ERR_NYI : -r);
+    }
+    if (E.n_ins == 0) {                                        /* a bare SCAN / CONST */
+        for (int i = 0; i < 2; i++) if (pre[i]) td_release(pre[i]);
+        return TD_ERR_PTR(TD_ERR_NYI);
+    }
+    E.out_reg = r;
+    int8_t ot = E.regs[r].type;
+    int64_t n = td_table_nrows(tbl);
+    size_t esz = ot == TD_BOOL ? 1 : 8;
+    td_t* v = td__hbm_vec_new(ot, 0, n, (size_t)(n > 0 ? n : 1) * esz);
+    if (TD_IS_ERR(v)) {
+        for (int i = 0; i < 2; i++) if (pre[i]) td_release(pre[i]);
+        return v;
+    }
+    int rc = tdd_expr_eval(&E, n, td_hbm_ref(v)->dptr);
+    /* the kernel is queued on the library's stream: releasing the inputs
+     * now returns their blocks only to work ordered after it */
+    for (int i = 0; i < 2; i++) if (pre[i]) td_release(pre[i]);
+    if (rc) { td_release(v); return TD_ERR_PTR(rc); }
+    return v;
+}
+
 static int compile_pred(xctx_t* x, td_t* tbl, td_op_t* op, tdd_pred_t* P) {
     memset(P, 0, sizeof(*P));
     pcomp_t c;
@@ -318,6 +500,27 @@ static td_t* eval_sel(xctx_t* x, const tdd_pred_t* P, int64_t nrows, td_t* and_w
     int rc = TD_OK;
     if (and_with) rc = dev_sel(x, and_with, &w);
     if (!rc) rc = tdd_sel_eval(P, and_with ? &w : NULL, &v);
+    if (rc) { td_release(s); return TD_ERR_PTR(rc); }
+    return s;
+}
+
+/* A predicate with arithmetic: its BOOL vector as a device selection,
+ * ANDed with `and_with`. */
+static td_t* pred_expr_sel(xctx_t* x, td_t* tbl, td_op_t* pred, td_t* and_with) {
+    td_t* b = eval_expr(x, tbl, pred);
+    if (TD_IS_ERR(b)) return b;
+    if (b->type != TD_BOOL) { td_release(b); return TD_ERR_PTR(TD_ERR_TYPE); }
+    int64_t n = b->len;
+    td_t* s = td__sel_new_dev(n);
+    if (TD_IS_ERR(s)) { td_release(b); return s; }
+    tdd_sel_t v = td__sel_view(td_hbm_ref(s)->dptr, n);
+    int rc = tdd_sel_from_bool((const uint8_t*)td_hbm_ref(b)->dptr, &v);
+    td_release(b);
+    if (!rc && and_with) {
+        tdd_sel_t w;
+        rc = dev_sel(x, and_with, &w);
+        if (!rc) rc = tdd_sel_and(&v, &w, &v);
+    }
     if (rc) { td_release(s); return TD_ERR_PTR(rc); }
     return s;
 }
@@ -510,9 +713,21 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
     return group_over(x, ext, tbl, 1);
 }
 
+static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, td_t** ev);
+
 /* GROUP of `tbl`'s columns; with use_sel, the graph's lazy selection (or the
- * README shape's shared predicate) masks the rows and is consumed. */
+ * README shape's shared predicate) masks the rows and is consumed.  Key and
+ * aggregate inputs that are expressions are evaluated over the rows first
+ * (into `ev`, released once the group result holds its own columns). */
 static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel) {
+    td_t* ev[TDD_MAX_KEYS + TDD_MAX_AGGS] = {0};
+    td_t* r = group_over_ev(x, ext, tbl, use_sel, ev);
+    for (int i = 0; i < TDD_MAX_KEYS + TDD_MAX_AGGS; i++)
+        if (ev[i]) td_release(ev[i]);
+    return r;
+}
+
+static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, td_t** ev) {
     td_graph_t* g = x->g;
     int64_t nrows = td_table_nrows(tbl);
     tdd_group_spec_t S;
@@ -526,8 +741,15 @@ static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel) {
     int rc;
     for (int k = 0; k < ext->n_keys; k++) {
         td_op_t* kop = strip_filter(ext->keys[k]);
-        if (kop->opcode != OP_SCAN) return TD_ERR_PTR(TD_ERR_NYI);
-        key_src[k] = scan_col(g, kop, tbl);
+        if (expr_elementwise(kop->opcode)) {
+            ev[k] = eval_expr(x, tbl, kop);
+            if (TD_IS_ERR(ev[k])) { td_t* e = ev[k]; ev[k] = NULL; return e; }
+            key_src[k] = ev[k];
+        } else if (kop->opcode == OP_SCAN) {
+            key_src[k] = scan_col(g, kop, tbl);
+        } else {
+            return TD_ERR_PTR(TD_ERR_NYI);
+        }
         if (!key_src[k]) return TD_ERR_PTR(TD_ERR_SCHEMA);
         if (key_src[k]->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
         if ((rc = dev_col(x, key_src[k], &S.keys[k]))) return TD_ERR_PTR(rc);
@@ -541,8 +763,13 @@ static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel) {
             if (!agg_src[a]) return TD_ERR_PTR(TD_ERR_SCHEMA);
             if (agg_src[a]->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
             if ((rc = dev_col(x, agg_src[a], &S.aggs[a]))) return TD_ERR_PTR(rc);
+        } else if (expr_elementwise(aop->opcode)) {
+            td_t* e = eval_expr(x, tbl, aop);
+            if (TD_IS_ERR(e)) return e;
+            ev[TDD_MAX_KEYS + a] = agg_src[a] = e;
+            if ((rc = dev_col(x, e, &S.aggs[a]))) return TD_ERR_PTR(rc);
         } else if (!(aop->opcode == OP_CONST && S.ops[a] == OP_COUNT)) {
-            return TD_ERR_PTR(TD_ERR_NYI);               /* expression inputs: not yet */
+            return TD_ERR_PTR(TD_ERR_NYI);
         }
     }
 
@@ -553,8 +780,10 @@ static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel) {
         if (rp) return TD_ERR_PTR(TD_ERR_NYI);
     } else if (rp) {
         tdd_pred_t P;
-        if ((rc = compile_pred(x, tbl, rp, &P))) return TD_ERR_PTR(rc);
-        sel = eval_sel(x, &P, nrows, g->selection);
+        rc = compile_pred(x, tbl, rp, &P);
+        if (rc == TD_ERR_NYI) sel = pred_expr_sel(x, tbl, rp, g->selection);
+        else if (rc) return TD_ERR_PTR(rc);
+        else sel = eval_sel(x, &P, nrows, g->selection);
         if (TD_IS_ERR(sel)) return sel;
     } else if (g->selection) {
         if (g->selection->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
@@ -1070,7 +1299,16 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
         if (input->type == TD_TABLE) {
             td_t* tbl = g->table;
             if (!tbl) { td_release(input); return TD_ERR_PTR(TD_ERR_SCHEMA); }
-            if ((rc = compile_pred(x, tbl, op->inputs[1], &P))) { td_release(input); return TD_ERR_PTR(rc); }
+            rc = compile_pred(x, tbl, op->inputs[1], &P);
+            if (rc == TD_ERR_NYI) {
+                /* arithmetic in the predicate: evaluate it as an expression */
+                td_t* s = pred_expr_sel(x, tbl, op->inputs[1], g->selection);
+                if (TD_IS_ERR(s)) { td_release(input); return s; }
+                if (g->selection) td_release(g->selection);
+                g->selection = s;
+                return input;
+            }
+            if (rc) { td_release(input); return TD_ERR_PTR(rc); }
             td_t* s = eval_sel(x, &P, td_table_nrows(tbl), g->selection);
             if (TD_IS_ERR(s)) { td_release(input); return s; }
             if (g->selection) td_release(g->selection);
@@ -1078,8 +1316,13 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
             return input;
         }
         /* vector: eager compaction */
-        if ((rc = compile_pred(x, g->table, op->inputs[1], &P))) { td_release(input); return TD_ERR_PTR(rc); }
-        td_t* s = eval_sel(x, &P, input->len, NULL);
+        if (!g->table || td_table_nrows(g->table) != input->len) {
+            td_release(input);
+            return TD_ERR_PTR(TD_ERR_LENGTH);
+        }
+        rc = compile_pred(x, g->table, op->inputs[1], &P);
+        td_t* s = rc == TD_ERR_NYI ? pred_expr_sel(x, g->table, op->inputs[1], NULL)
+                  : rc ? TD_ERR_PTR(rc) : eval_sel(x, &P, input->len, NULL);
         if (TD_IS_ERR(s)) { td_release(input); return s; }
         td_t* out = compact_vec(x, input, s);
         td_release(s);
@@ -1101,6 +1344,12 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
     case OP_SORT:
         return exec_sort(x, op, -1);
     default:
+        /* element-wise subtree over the graph's table, fused into one kernel
+         * (exec.c:11620-11640) */
+        if (expr_elementwise(op->opcode)) {
+            if (!g->table) return TD_ERR_PTR(TD_ERR_SCHEMA);
+            return eval_expr(x, g->table, op);
+        }
         return TD_ERR_PTR(TD_ERR_NYI);
     }
 }

@@ -2,17 +2,21 @@
  * opt.c -- td_optimize: the rewrites that matter on the filter -> join ->
  * group-by path, plus the plan printer.
  *
- * Of the reference's ten passes (src/ops/opt.c:1641-1675) only two change
- * this path's DAGs: folding a FILTER whose predicate is a constant into
- * MATERIALIZE (keep all rows) or HEAD 0 (keep none) (opt.c:461-494), and
- * splitting FILTER(AND(a, b)) into FILTER(FILTER(x, a), b) (opt.c:1304-1342),
- * which the device evaluates as one AND-ed selection anyway.  The others
- * (type inference, SIP, factorisation, pushdown into SELECT/EXPAND,
- * projection marking, partition pruning, fusion flags, DCE) do not change
- * the result of the supported shapes and are not run.
+ * Of the reference's ten passes (src/ops/opt.c:1641-1675) three change
+ * this path's DAGs: constant folding of element-wise nodes over constants
+ * (opt.c:192-507), whose arithmetic differs from the fused evaluator's in
+ * places (IEEE division by zero, fmin / fmax), so results depend on it;
+ * folding a FILTER whose predicate is a constant into MATERIALIZE (keep all
+ * rows) or HEAD 0 (keep none) (opt.c:461-494); and splitting
+ * FILTER(AND(a, b)) into FILTER(FILTER(x, a), b) (opt.c:1304-1342), which
+ * the device evaluates as one AND-ed selection anyway.  The others (type
+ * inference, SIP, factorisation, pushdown into SELECT/EXPAND, projection
+ * marking, partition pruning, fusion flags, DCE) do not change the result
+ * of the supported shapes and are not run.
  */
 #include "internal.h"
 
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -45,6 +49,144 @@ static td_op_ext_t* ensure_ext(td_graph_t* g, td_op_t* node) {
     e->base = *node;
     g->ext_nodes[g->ext_count++] = e;
     return e;
+}
+
+/* ---------------------------------------------------- constant folding -- */
+
+/* Numeric value of a CONST node's atom: integer atoms widen to i64, f64
+ * atoms also give their truncation (opt.c:242-276). */
+static int const_num(td_graph_t* g, td_op_t* op, int64_t* i, double* f, int* isf) {
+    if (!op || op->opcode != OP_CONST) return 0;
+    td_op_ext_t* e = td__find_ext(g, op->id);
+    if (!e || !e->literal || !td_is_atom(e->literal)) return 0;
+    td_t* a = e->literal;
+    *isf = 0;
+    switch (a->type) {
+    case TD_ATOM_F64:
+        *f = a->f64;
+        *i = (int64_t)a->f64;
+        *isf = 1;
+        return 1;
+    case TD_ATOM_I64: case TD_ATOM_SYM: case TD_ATOM_DATE: case TD_ATOM_TIME:
+    case TD_ATOM_TIMESTAMP:
+        *i = a->i64; break;
+    case TD_ATOM_I32: *i = a->i32; break;
+    case TD_ATOM_I16: *i = a->i16; break;
+    case TD_ATOM_U8: case TD_ATOM_BOOL: *i = a->u8; break;
+    default: return 0;
+    }
+    *f = (double)*i;
+    return 1;
+}
+
+/* The value of a unary element-wise op over a constant, as the reference's
+ * optimizer computes it (opt.c:305-358); NULL when it leaves the node. */
+static td_t* fold_unary(uint16_t op, int64_t i, double f, int isf) {
+    double x = isf ? f : (double)i;
+    switch (op) {
+    case OP_NEG:
+        if (isf) return td_f64(-f);
+        return i == INT64_MIN ? NULL : td_i64(-i);
+    case OP_ABS:
+        if (isf) return td_f64(fabs(f));
+        return i == INT64_MIN ? NULL : td_i64(i < 0 ? -i : i);
+    case OP_NOT: return td_bool(isf ? f == 0.0 : i == 0);
+    case OP_SQRT: return td_f64(sqrt(x));
+    case OP_LOG: return td_f64(log(x));
+    case OP_EXP: return td_f64(exp(x));
+    case OP_CEIL: return isf ? td_f64(ceil(f)) : td_i64(i);
+    case OP_FLOOR: return isf ? td_f64(floor(f)) : td_i64(i);
+    default: return NULL;
+    }
+}
+
+/* A binary element-wise op over two constants in the node's declared type
+ * (opt.c:360-447): f64 with IEEE division and fmod, fmin / fmax; i64
+ * wrapping with DIV / MOD by zero giving 0; bool comparisons in f64. */
+static td_t* fold_binary(uint16_t op, int8_t t, int64_t li, double lf, int lfl, int64_t ri,
+                         double rf, int rfl) {
+    double a = lfl ? lf : (double)li, b = rfl ? rf : (double)ri;
+    if (t == TD_F64) {
+        switch (op) {
+        case OP_ADD: return td_f64(a + b);
+        case OP_SUB: return td_f64(a - b);
+        case OP_MUL: return td_f64(a * b);
+        case OP_DIV: return td_f64(a / b);
+        case OP_MOD: return td_f64(fmod(a, b));
+        case OP_MIN2: return td_f64(fmin(a, b));
+        case OP_MAX2: return td_f64(fmax(a, b));
+        default: return NULL;
+        }
+    }
+    if (t == TD_I64) {
+        int64_t x = lfl ? (int64_t)lf : li, y = rfl ? (int64_t)rf : ri;
+        int ok = y != 0 && !(x == INT64_MIN && y == -1);
+        switch (op) {
+        case OP_ADD: return td_i64((int64_t)((uint64_t)x + (uint64_t)y));
+        case OP_SUB: return td_i64((int64_t)((uint64_t)x - (uint64_t)y));
+        case OP_MUL: return td_i64((int64_t)((uint64_t)x * (uint64_t)y));
+        case OP_DIV: return td_i64(ok ? x / y : 0);
+        case OP_MOD: return td_i64(ok ? x % y : 0);
+        case OP_MIN2: return td_i64(x < y ? x : y);
+        case OP_MAX2: return td_i64(x > y ? x : y);
+        default: return NULL;
+        }
+    }
+    if (t == TD_BOOL) {
+        switch (op) {
+        case OP_EQ: return td_bool(a == b);
+        case OP_NE: return td_bool(a != b);
+        case OP_LT: return td_bool(a < b);
+        case OP_LE: return td_bool(a <= b);
+        case OP_GT: return td_bool(a > b);
+        case OP_GE: return td_bool(a >= b);
+        case OP_AND: return td_bool(a != 0.0 && b != 0.0);
+        case OP_OR: return td_bool(a != 0.0 || b != 0.0);
+        default: return NULL;
+        }
+    }
+    return NULL;
+}
+
+/* Element-wise nodes whose inputs are all constants become constants,
+ * children first, so whole constant subtrees collapse (opt.c:496-507).
+ * Nodes that already carry an ext record are left alone. */
+static void fold_consts(td_graph_t* g, td_op_t* op, uint8_t* seen) {
+    if (!op || seen[op->id]) return;
+    seen[op->id] = 1;
+    for (int i = 0; i < op->arity && i < 2; i++) fold_consts(g, op->inputs[i], seen);
+    td_op_ext_t* e = td__find_ext(g, op->id);
+    if (e && op->opcode == OP_GROUP) {
+        for (int k = 0; k < e->n_keys; k++) fold_consts(g, e->keys[k], seen);
+        for (int a = 0; a < e->n_aggs; a++) fold_consts(g, e->agg_ins[a], seen);
+        return;
+    }
+    if (e && op->opcode == OP_SORT) {
+        for (int k = 0; k < e->sort.n_cols; k++) fold_consts(g, e->sort.columns[k], seen);
+        return;
+    }
+    if (e) return;
+    td_t* v = NULL;
+    int64_t li, ri;
+    double lf, rf;
+    int lfl, rfl;
+    if (op->arity == 1 && op->opcode >= OP_NEG && op->opcode <= OP_FLOOR &&
+        const_num(g, op->inputs[0], &li, &lf, &lfl)) {
+        v = fold_unary(op->opcode, li, lf, lfl);
+    } else if (op->arity == 2 && op->opcode >= OP_ADD && op->opcode <= OP_MAX2 &&
+               const_num(g, op->inputs[0], &li, &lf, &lfl) &&
+               const_num(g, op->inputs[1], &ri, &rf, &rfl)) {
+        v = fold_binary(op->opcode, op->out_type, li, lf, lfl, ri, rf, rfl);
+    }
+    if (!v || TD_IS_ERR(v)) return;
+    td_op_ext_t* x = ensure_ext(g, op);
+    if (!x) { td_release(v); return; }
+    op->opcode = OP_CONST;
+    op->arity = 0;
+    op->inputs[0] = op->inputs[1] = NULL;
+    op->out_type = (int8_t)(v->type < 0 ? -v->type : v->type);
+    x->base = *op;
+    x->literal = v;
 }
 
 static void fold_filters(td_graph_t* g, td_op_t* op, uint8_t* seen) {
@@ -105,6 +247,8 @@ td_op_t* td_optimize(td_graph_t* g, td_op_t* root) {
     if (!g || !root) return root;
     uint8_t* seen = calloc(g->node_count + 1, 1);
     if (!seen) return root;
+    fold_consts(g, root, seen);
+    memset(seen, 0, g->node_count + 1);
     fold_filters(g, root, seen);
     free(seen);
     return split_ands(g, root);

@@ -124,10 +124,13 @@ class Engine:
             "td_err_str": (C.c_char_p, [C.c_int]), "td_pool_init": (C.c_int, [C.c_uint32]),
             "td_thread_count": (C.c_uint32, []),
         }
-        for op in ["eq", "ne", "lt", "le", "gt", "ge", "and", "or", "add", "sub", "mul"]:
+        for op in ["eq", "ne", "lt", "le", "gt", "ge", "and", "or", "add", "sub", "mul", "div",
+                   "mod", "min2", "max2"]:
             sig["td_" + op] = (P, [P, P, P])
-        for op in ["not", "sum", "count", "avg", "min_op", "max_op", "first", "last"]:
+        for op in ["not", "sum", "count", "avg", "min_op", "max_op", "first", "last", "neg",
+                   "abs", "sqrt_op", "log_op", "exp_op", "ceil_op", "floor_op"]:
             sig["td_" + op] = (P, [P, P])
+        sig["td_cast"] = (P, [P, P, I8])
         if device:
             sig.update({
                 "td_dev_available": (C.c_int, []), "td_dev_put": (P, [P]),

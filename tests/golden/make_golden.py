@@ -7,7 +7,10 @@ together with SHA-256 digests of every input column, so a later run can tell
 a data change from a behaviour change.  Needs the reference tree to build the
 harness; the fixtures it writes are committed and read by tests/ everywhere.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [CASE ...]
+
+With case names, only those fixtures are rewritten and the other manifest
+entries are kept as they are.
 """
 import hashlib
 import json
@@ -43,6 +46,10 @@ CASES = {
                    lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "sort_8k": (["sort", "8000", "4096", "2048", "42", "25"],
                 lambda: W.high_card(8000, 4096, 2048, 1.1, 42)),
+    "expr_grp_60k": (["expr", "60000", "4096", "2048", "42", "grp"],
+                     lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
+    "expr_vec_4k": (["expr", "4000", "4096", "2048", "42", "vec"],
+                    lambda: W.high_card(4000, 4096, 2048, 1.1, 42)),
     "splay_hc3k": (["save", "3000", "4096", "2048", "42"],
                    lambda: W.high_card(3000, 4096, 2048, 1.1, 42)),
     "join_inner_small": (["join", "3000", "2000", "2500", "0", "42"], None),
@@ -60,8 +67,12 @@ def digest(arr):
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref")
-    manifest = {}
+    only = sys.argv[1:]
+    mpath = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(mpath)) if only else {}
     for name, (argv, gen) in CASES.items():
+        if only and name not in only:
+            continue
         outs = []
         if argv[0] == "hc":
             outs = [f"{name}_key.tdt", f"{name}_sym.tdt"]
@@ -95,7 +106,7 @@ def main():
             entry["outputs"] = []
         manifest[name] = entry
         print(name, entry["rows_out"])
-    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+    with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
 

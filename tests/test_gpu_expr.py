@@ -1,0 +1,356 @@
+"""GPU parity of the fused element-wise expression path (k_expr): the trees
+of tests/golden/expr_* against the reference's outputs, random typed trees
+over every integer width, symbols and f64 against the oracle
+(orc.expr_eval) and, where oracle/_ref is built, the reference library
+itself; arithmetic predicates in FILTER (lazy selection and the shared
+predicate of GROUP inputs), expression keys and aggregate inputs of GROUP,
+SUM over an expression, and expressions too large for one kernel."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import expr_cases as X
+import orc
+from parity import compare_groups
+from tdtio import read_tdt
+from teide_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def hc(name):
+    argv = MANIFEST[name]["argv"]
+    return W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+
+
+def hc_table(eng, d, resident):
+    t = eng.table([("key", d["key"]), ("sym", d["sym"], 20, 2), ("v_i64", d["v_i64"]),
+                   ("v_f64", d["v_f64"])])
+    if resident:
+        r = eng.put(t)
+        eng.release(t)
+        t = r
+    return t
+
+
+def run_vec(eng, t, e):
+    """The tree's value over table t: an array, or for a tree the
+    optimizer folds to a constant, that constant."""
+    g = eng.graph(t)
+    try:
+        v = g.execute(g.optimize(X.build(g, e)))
+    finally:
+        g.free()
+    if eng.hdr(v).type < 0:
+        out = eng.atom_value(v)
+    else:
+        out = eng.read_vec(v)
+    eng.release(v)
+    return out
+
+
+def oracle(e, cols):
+    """(type, value) of the tree as the reference computes it: constant
+    subtrees folded by the optimizer's rules, the rest fused."""
+    f = orc.expr_fold(e)
+    if f[0] in ("i64", "f64", "bool"):
+        return f[0], f[1]
+    t, v = orc.expr_eval(X.bind(f, cols))
+    n = len(next(iter(cols.values())))
+    return t, np.broadcast_to(v, (n,))          # CAST of a constant stays a column
+
+
+def check(got, typ, ref, what):
+    if np.ndim(ref) == 0:                       # folded to a constant
+        assert np.ndim(got) == 0, what
+        if typ == "f64" and np.isnan(ref):
+            assert np.isnan(got), what
+        else:
+            assert got == ref, (what, got, ref)
+    else:
+        X.check_vec(got, typ, ref, what)
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_expr_vec_golden(eng, resident):
+    d = hc("expr_vec_4k")
+    t = hc_table(eng, d, resident)
+    ref = read_tdt(os.path.join(GOLD, "expr_vec_4k.tdt"))
+    for i, e in enumerate(X.VEC):
+        got = run_vec(eng, t, e)
+        t_out = {6: "i64", 7: "f64", 1: "bool"}[ref[i][1]]
+        X.check_vec(got, t_out, ref[i][3], f"e{i}")
+    eng.release(t)
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_expr_grp_golden(eng, resident):
+    d = hc("expr_grp_60k")
+    t = hc_table(eng, d, resident)
+    g = eng.graph(t)
+    f = g.execute(g.optimize(g.filter(g.const_table(t), X.build(g, X.GRP_PRED))))
+    g.free()
+    g = eng.graph(f)
+    root = g.group([g.scan("sym")], [(o, X.build(g, e)) for o, e in X.GRP_AGGS])
+    out = eng.read_table(g.execute(g.optimize(root)))
+    g.free()
+    ref = read_tdt(os.path.join(GOLD, "expr_grp_60k.tdt"))
+    assert [c[0] for c in out] == [c[0] for c in ref]
+    assert [c[1] for c in out] == [c[1] for c in ref]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]],
+                   [ref[0][3].astype(np.int64)], [c[3] for c in ref[1:]],
+                   [False, True, False, False, True, False], "expr grp")
+    eng.release(f)
+    eng.release(t)
+
+
+# ------------------------------------------------------------ random trees
+
+INT_COLS = {"a": np.int64, "b": np.int32, "c": np.int16, "d": np.uint8, "s": np.uint16,
+            "h": np.int64}
+
+
+def rand_cols(rng, n):
+    cols = {
+        "a": rng.integers(-1000, 1000, n),
+        "b": rng.integers(-2 ** 31, 2 ** 31 - 1, n).astype(np.int32),
+        "c": rng.integers(-5, 5, n).astype(np.int16),
+        "d": rng.integers(0, 4, n).astype(np.uint8),
+        "s": rng.integers(0, 9, n).astype(np.uint16),
+        "h": rng.integers(-2 ** 63, 2 ** 63 - 1, n, dtype=np.int64),
+        "f": np.round(rng.normal(size=n) * 3, 2),
+        "z": rng.normal(size=n) * 1e10,
+    }
+    cols["f"][rng.integers(0, n, n // 50)] = 0.0
+    return cols
+
+
+def table_of(eng, cols):
+    spec = []
+    for k, v in cols.items():
+        spec.append((k, v, 20, 1) if k == "s" else (k, v))
+    return eng.table(spec)
+
+
+def num_tree(rng, depth, top=True):
+    """(tree, type) of a numeric tree whose every operation the reference's
+    fused evaluator defines for its operand types.  SQRT / LOG / EXP only at
+    the top: libm implementations differ in the last bit, which a MOD or a
+    cancelling SUB above them would magnify past any fixed tolerance."""
+    if depth == 0 or rng.random() < 0.25:
+        r = rng.random()
+        if r < 0.55:
+            name = str(rng.choice(list(INT_COLS)))
+            return ("col", name), "i64"
+        if r < 0.75:
+            return ("col", str(rng.choice(["f", "z"]))), "f64"
+        if r < 0.9:
+            return ("i64", int(rng.integers(-9, 10))), "i64"
+        return ("f64", float(np.round(rng.normal() * 4, 2))), "f64"
+    r = rng.random()
+    if r < 0.6:
+        op = str(rng.choice(["ADD", "SUB", "MUL", "DIV", "MOD", "MIN2", "MAX2"]))
+        a, ta = num_tree(rng, depth - 1, False)
+        b, tb = num_tree(rng, depth - 1, False)
+        ot = "f64" if "f64" in (ta, tb) or op == "DIV" else "i64"
+        return (op, a, b), ot
+    a, ta = num_tree(rng, depth - 1, False)
+    if r < 0.72:
+        return (str(rng.choice(["NEG", "ABS"])), a), ta
+    if r < 0.82 and top:
+        return (str(rng.choice(["SQRT", "LOG", "EXP"])), ("DIV", a, ("f64", 64.0))), "f64"
+    if r < 0.9 and ta == "f64":
+        return (str(rng.choice(["CEIL", "FLOOR"])), a), "f64"
+    return ("cast", a, "i64" if ta == "f64" else "f64"), "i64" if ta == "f64" else "f64"
+
+
+def bool_tree(rng, depth):
+    r = rng.random()
+    if depth == 0 or r < 0.5:
+        op = str(rng.choice(["EQ", "NE", "LT", "LE", "GT", "GE"]))
+        return (op, num_tree(rng, max(depth - 1, 0))[0], num_tree(rng, max(depth - 1, 0))[0])
+    if r < 0.85:
+        return (str(rng.choice(["AND", "OR"])), bool_tree(rng, depth - 1),
+                bool_tree(rng, depth - 1))
+    return ("NOT", bool_tree(rng, depth - 1))
+
+
+def regs(e):
+    """(type, registers) the compiler uses for `e` (one per node plus the
+    promotion casts), to keep random trees within one kernel."""
+    if e[0] == "col":
+        return ("f64" if e[1] in ("f", "z") else "i64"), 1
+    if e[0] in ("i64", "f64", "bool"):
+        return ("i64" if e[0] == "bool" else e[0]), 1
+    if e[0] == "cast":
+        t, n = regs(e[1])
+        return e[2], n + 1
+    op = e[0]
+    t1, n1 = regs(e[1])
+    t2, n2 = regs(e[2]) if len(e) > 2 else (t1, 0)
+    n = n1 + n2 + 1
+    if op in ("EQ", "NE", "LT", "LE", "GT", "GE", "AND", "OR", "NOT"):
+        if len(e) > 2 and t1 != t2:
+            n += 1
+        return "bool", n
+    if "f64" in (t1, t2) or op in ("DIV", "SQRT", "LOG", "EXP"):
+        n += (t1 != "f64") + (len(e) > 2 and t2 != "f64")
+        return "f64", n
+    return "i64", n
+
+
+BOOL_OPS = ("EQ", "NE", "LT", "LE", "GT", "GE", "AND", "OR", "NOT")
+
+
+def const_only(e):
+    return e[0] != "col" and all(const_only(c) for c in e[1:] if isinstance(c, tuple))
+
+
+def const_bool_inside(e):
+    """A comparison or logic subtree over constants alone, which folding
+    turns into a bool constant that the reference's evaluator then reads
+    as raw bytes of another type; such trees are left out."""
+    if e[0] in BOOL_OPS and const_only(e):
+        return True
+    return any(const_bool_inside(c) for c in e[1:] if isinstance(c, tuple) and c[0] != "col")
+
+
+def trees(seed, count, kind):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        e = num_tree(rng, 3)[0] if kind == "num" else bool_tree(rng, 3)
+        if e[0] in ("col", "i64", "f64"):
+            continue
+        if not const_only(e) and const_bool_inside(e):
+            continue
+        if regs(orc.expr_fold(e))[1] <= 16:
+            out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("kind", ["num", "bool"])
+def test_expr_random_vs_oracle(eng, kind):
+    rng = np.random.default_rng(5)
+    cols = rand_cols(rng, 70001)
+    t = table_of(eng, cols)
+    rt = eng.put(t)
+    for i, e in enumerate(trees(11 if kind == "num" else 12, 40, kind)):
+        typ, ref = oracle(e, cols)
+        got = run_vec(eng, rt if i % 2 else t, e)
+        check(got, typ, ref, f"{kind} tree {i}: {e}")
+    eng.release(rt)
+    eng.release(t)
+
+
+def test_expr_random_vs_reference(eng, ref_engine):
+    """The same random trees through the reference library (oracle/_ref)."""
+    rng = np.random.default_rng(6)
+    cols = rand_cols(rng, 20000)
+    t = table_of(eng, cols)
+    rtab = table_of(ref_engine, cols)
+    for i, e in enumerate(trees(21, 30, "num") + trees(22, 30, "bool")):
+        got = run_vec(eng, t, e)
+        ref = run_vec(ref_engine, rtab, e)
+        check(got, oracle(e, cols)[0], ref, f"tree {i}: {e}")
+    ref_engine.release(rtab)
+    eng.release(t)
+
+
+# ----------------------------------------------------- in FILTER and GROUP
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_arith_filter_then_group(eng, resident):
+    """FILTER(table, arithmetic predicate) leaves a lazy selection that the
+    next GROUP consumes; the README shape (FILTER on every group input with
+    one shared predicate) takes the same path."""
+    rng = np.random.default_rng(9)
+    n = 200003
+    k = rng.integers(0, 300, n)
+    v = rng.integers(-10 ** 6, 10 ** 6, n)
+    f = rng.normal(size=n)
+    tt = eng.table([("k", k), ("v", v), ("f", f)])
+    t = eng.put(tt) if resident else tt
+    pe = ("GT", ("ADD", ("MUL", ("col", "f"), ("f64", 2.0)), ("MOD", ("col", "v"), ("i64", 3))),
+          ("f64", 0.5))
+    _, keep = orc.expr_eval(X.bind(pe, {"k": k, "v": v, "f": f}))
+    ref = orc.group([k[keep]], ["SUM", "COUNT", "MAX"], [v[keep], v[keep], f[keep]])
+    # README shape
+    g = eng.graph(t)
+    p = X.build(g, pe)
+    root = g.group([g.filter(g.scan("k"), p)],
+                   [("SUM", g.filter(g.scan("v"), p)), ("COUNT", g.filter(g.scan("v"), p)),
+                    ("MAX", g.filter(g.scan("f"), p))])
+    out = eng.read_table(g.execute(root))
+    g.free()
+    compare_groups([out[0][3]], [c[3] for c in out[1:]], ref["keys"], ref["vals"],
+                   [False, False, False], "readme arith")
+    # a FILTER of the table first: the group sees the selection
+    g = eng.graph(t)
+    flt = g.filter(g.const_table(t), X.build(g, pe))
+    sel_tbl = g.execute(flt)
+    g.free()
+    g = eng.graph(sel_tbl)
+    root = g.group([g.scan("k")], [("SUM", g.scan("v")), ("COUNT", g.scan("v")),
+                                   ("MAX", g.scan("f"))])
+    out = eng.read_table(g.execute(root))
+    g.free()
+    compare_groups([out[0][3]], [c[3] for c in out[1:]], ref["keys"], ref["vals"],
+                   [False, False, False], "filter arith")
+    eng.release(sel_tbl)
+    if resident:
+        eng.release(t)
+    eng.release(tt)
+
+
+def test_expression_keys_and_sum(eng):
+    rng = np.random.default_rng(10)
+    n = 150001
+    a = rng.integers(-2 ** 62, 2 ** 62, n)
+    b = rng.integers(0, 1000, n).astype(np.int32)
+    f = rng.normal(size=n)
+    t = eng.table([("a", a), ("b", b), ("f", f)])
+    cols = {"a": a, "b": b, "f": f}
+    ke = ("MOD", ("col", "a"), ("i64", 97))
+    ve = ("SUB", ("MUL", ("col", "a"), ("col", "b")), ("i64", 5))
+    fe = ("MUL", ("col", "f"), ("cast", ("col", "b"), "f64"))
+    _, kv = orc.expr_eval(X.bind(ke, cols))
+    _, vv = orc.expr_eval(X.bind(ve, cols))
+    _, fv = orc.expr_eval(X.bind(fe, cols))
+    g = eng.graph(t)
+    root = g.group([X.build(g, ke)], [("SUM", X.build(g, ve)), ("MIN", X.build(g, fe)),
+                                      ("COUNT", g.scan("a"))])
+    out = eng.read_table(g.execute(root))
+    g.free()
+    ref = orc.group([kv], ["SUM", "MIN", "COUNT"], [vv, fv, vv])
+    compare_groups([out[0][3]], [c[3] for c in out[1:]], ref["keys"], ref["vals"],
+                   [False, False, False], "expr keys")
+    # SUM over an expression (no group): wrapping i64 sum, exact
+    g = eng.graph(t)
+    s = g.execute(g.sum(X.build(g, ve)))
+    g.free()
+    assert eng.atom_value(s) == int(vv.astype(np.uint64).sum(dtype=np.uint64).astype(np.int64))
+    eng.release(s)
+    eng.release(t)
+
+
+def test_large_expression_splits(eng):
+    """A tree past one kernel's 16 registers is evaluated in pieces with the
+    same typing and results."""
+    rng = np.random.default_rng(12)
+    n = 50000
+    cols = {"x": rng.integers(-100, 100, n), "y": rng.normal(size=n)}
+    t = eng.table([("x", cols["x"]), ("y", cols["y"])])
+    e = ("col", "x")
+    for i in range(14):
+        e = ("ADD" if i % 3 else "MUL", e, ("MOD", ("col", "x"), ("i64", i + 2)))
+    e = ("ADD", e, ("DIV", ("col", "y"), ("f64", 3.0)))
+    assert regs(e)[1] > 16
+    typ, ref = orc.expr_eval(X.bind(e, cols))
+    X.check_vec(run_vec(eng, t, e), typ, ref, "split")
+    eng.release(t)

@@ -342,3 +342,87 @@ def test_zipf_head_frequency():
     h = np.sum(1.0 / np.arange(1, nv + 1) ** s)
     frac = np.mean(d["key"] == top)
     assert abs(frac - 1.0 / h) < 0.01
+
+
+# ---------------------------------------------------------------- expressions
+
+def test_expr_vec_matches_reference():
+    # fused element-wise expressions (exec.c:760-1100), golden from the
+    # reference: transcendental f64, wrapping i64, mixed comparisons, the
+    # clamped f64 -> i64 cast, MOD by zero
+    import expr_cases as X
+    argv = MANIFEST["expr_vec_4k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("expr_vec_4k.tdt")
+    assert [c[0] for c in ref] == [f"e{i}" for i in range(8)]
+    for i, e in enumerate(X.VEC):
+        t, v = orc.expr_eval(X.bind(e, d))
+        assert ref[i][1] == {"i64": 6, "f64": 7, "bool": 1}[t], i
+        X.check_vec(v, t, ref[i][3], f"e{i}")
+
+
+def expr_grp_oracle(d):
+    import expr_cases as X
+    _, keep = orc.expr_eval(X.bind(X.GRP_PRED, d))
+    f = {k: v[keep] for k, v in d.items()}
+    aggs = [orc.expr_eval(X.bind(e, f))[1] if e[0] != "col" else f[e[1]]
+            for _, e in X.GRP_AGGS]
+    r = orc.group([f["sym"]], [o for o, _ in X.GRP_AGGS], aggs, key_types=[(20, 2)])
+    return r
+
+
+def test_expr_grp_matches_reference():
+    # FILTER by an arithmetic predicate, then GROUP(sym) of expression inputs
+    # (exec.c:5700-5712 evaluates them before aggregating)
+    argv = MANIFEST["expr_grp_60k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("expr_grp_60k.tdt")
+    assert [c[0] for c in ref] == ["sym", "_e0_sum", "_e1_sum", "_e2_max", "_e3_min",
+                                   "_e4_mean", "v_i64_count"]
+    r = expr_grp_oracle(d)
+    compare_groups(r["keys"], r["vals"], [ref[0][3]], [c[3] for c in ref[1:]],
+                   [False, True, False, False, True, False], "expr grp",
+                   exact=r["comp"], sum_abs=r["sum_abs"])
+
+
+def test_expr_oracle_closed_forms():
+    # what C's two's-complement and IEEE rules fix, independent of the
+    # reference run: wrap-around, division / modulo guards, truncating MOD,
+    # the clamped cast, promotion of mixed comparisons
+    i64 = np.array([2 ** 63 - 1, -2 ** 63, -7, 7, 0, 5], np.int64)
+    dv = np.array([1, -1, 3, -3, 0, 0], np.int64)
+    c = ("col", i64)
+    assert orc.expr_eval(("ADD", c, ("i64", 1)))[1][0] == -2 ** 63
+    assert orc.expr_eval(("NEG", c))[1][1] == -2 ** 63
+    assert orc.expr_eval(("ABS", c))[1][1] == -2 ** 63
+    assert orc.expr_eval(("MUL", c, ("i64", 2)))[1][0] == -2
+    np.testing.assert_array_equal(orc.expr_eval(("MOD", c, ("col", dv)))[1],
+                                  [0, 0, -1, 1, 0, 0])
+    t, q = orc.expr_eval(("DIV", c, ("col", dv)))
+    assert t == "f64" and q[4] == 0.0 and q[5] == 0.0 and q[2] == -7 / 3
+    t, m = orc.expr_eval(("MOD", ("f64", -7.5), ("col", dv)))
+    assert t == "f64"
+    np.testing.assert_array_equal(m, [-0.5, -0.5, -1.5, -1.5, 0.0, 0.0])
+    big = ("col", np.array([1e19, -1e19, 9.3e18, -2.5, 2.5, 0.0, np.nan, -np.inf]))
+    np.testing.assert_array_equal(orc.expr_eval(("cast", big, "i64"))[1],
+                                  [2 ** 63 - 1, -2 ** 63, 2 ** 63 - 1, -2, 2, 0, -2 ** 63,
+                                   -2 ** 63])
+    t, b = orc.expr_eval(("LT", ("col", np.array([1.5, 2.0])), ("col", np.array([2, 2]))))
+    assert t == "bool" and list(b) == [True, False]
+    t, s = orc.expr_eval(("MIN2", ("col", np.array([np.nan, 1.0])), ("f64", 0.0)))
+    assert s[0] == 0.0 and s[1] == 0.0                  # a < b ? a : b with NaN a
+
+
+def test_expr_oracle_vs_reference_random(ref_engine):
+    """Random typed expression trees over every integer width, symbols and
+    f64: the oracle against the reference library (oracle/_ref) run here."""
+    import expr_cases as X
+    from test_gpu_expr import check, oracle, rand_cols, run_vec, table_of, trees
+    cols = rand_cols(np.random.default_rng(6), 20000)
+    rtab = table_of(ref_engine, cols)
+    for i, e in enumerate(trees(21, 30, "num") + trees(22, 30, "bool") +
+                          trees(11, 40, "num") + trees(12, 40, "bool") +
+                          trees(31, 200, "num") + trees(32, 200, "bool")):
+        typ, ref = oracle(e, cols)
+        check(ref, typ, run_vec(ref_engine, rtab, e), f"tree {i}: {e}")
+    ref_engine.release(rtab)
