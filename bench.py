@@ -249,6 +249,60 @@ class HighCard:
         self.e.release(self.t)
 
 
+class Charge:
+    """The pricing expression of a line-item scan,
+    price * (1 - disc) * (1 + tax), over three resident f64 columns into a
+    new f64 column: one fused expression kernel, 24 bytes read and 8
+    written per row."""
+    name = "charge"
+    metric = "rows/s + % HBM peak of a fused expression (price * (1 - disc) * (1 + tax))"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        self.e, self.a = e, a
+        n = a.expr_rows
+        cols = [("price", W.f64_uniform(n, 900.0, 105000.0, SEED, 701)),
+                ("disc", W.f64_uniform(n, 0.0, 0.1, SEED, 702)),
+                ("tax", W.f64_uniform(n, 0.0, 0.08, SEED, 703))]
+        h = e.table(cols)
+        del cols
+        self.t = e.put(h)
+        e.L.td_dev_sync()
+        e.release(h)
+        self.rows = n
+        self.query_bytes = n * 32
+        self.config = {"workload": f"expression price * (1 - disc) * (1 + tax) over {n:,} rows "
+                                   "of three f64 columns into an f64 column"}
+
+    def step(self):
+        e = self.e
+        g = e.graph(self.t)
+        root = g.mul(g.mul(g.scan("price"), g.sub(g.const_f64(1.0), g.scan("disc"))),
+                     g.add(g.const_f64(1.0), g.scan("tax")))
+        v = g.execute(g.optimize(root))
+        g.free()
+        e.release(v)
+
+    def kernel_bytes(self, name):
+        return self.rows * 32 if name == "expr" else None
+
+    def cpu_baseline(self):
+        exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+        if not os.path.exists(exe):
+            return None
+        n = min(self.a.expr_rows, 100_000_000)
+        env = dict(os.environ, TD_REF_THREADS=str(cpu_threads()))
+        out = subprocess.run([exe, "bench", "charge", str(n), "5"], check=True,
+                             capture_output=True, text=True, env=env, timeout=900)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": n / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
+                "kind": "reference",
+                "sample": f"{n:,} rows, median of 5 after 1 warm-up"}
+
+    def release(self):
+        self.e.release(self.t)
+
+
 class BatchQueries:
     """Many independent filter -> join -> group-by queries: fact rows
     log-uniform in [1K, 2M], dim = fact / 8 with unique keys, group
@@ -600,12 +654,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load"], default="jg",
+    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge"],
+                    default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
                          "served: request stream through the request server; "
                          "load: splayed-table load into HBM")
     ap.add_argument("--load-rows", type=int, default=50_000_000)
+    ap.add_argument("--expr-rows", type=int, default=250_000_000)
     ap.add_argument("--load-reps", type=int, default=3)
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-fact", type=int, default=100_000_000)
@@ -673,7 +729,8 @@ def main():
         print(json.dumps(line), flush=True)
         return
 
-    wl = {"hc": HighCard, "batch": BatchQueries}.get(a.workload, JoinGroup)(e, a)
+    wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge}.get(a.workload,
+                                                                        JoinGroup)(e, a)
 
     for _ in range(a.warmup):
         wl.step()
@@ -713,7 +770,8 @@ def main():
                  "plain_qps": round(wl.n / plain_s, 1),
                  "batched_vs_plain": round((plain_s * 1e3) / ms, 2)}
     line = {
-        "metric": "rows/s + % HBM peak for join+group-by; batched/served qps & p99 vs plain",
+        "metric": getattr(wl, "metric",
+                          "rows/s + % HBM peak for join+group-by; batched/served qps & p99 vs plain"),
         "value": wl.rows / (ms / 1e3),
         "unit": "rows/s",
         "n_gpus": 1,

@@ -366,6 +366,33 @@ static td_op_t* expr_vec_op(td_graph_t* g, int i) {
     }
 }
 
+/* The pricing expression of a line-item scan, price * (1 - disc) * (1 + tax),
+ * over three f64 columns: price in [900, 105000), disc in [0, 0.1), tax in
+ * [0, 0.08). */
+static td_t* make_charge(int64_t n, uint64_t seed) {
+    td_t* price = vec_of(TD_F64, n);
+    td_t* disc = vec_of(TD_F64, n);
+    td_t* tax = vec_of(TD_F64, n);
+    tdg_f64_uniform((double*)td_data(price), n, 900.0, 105000.0, seed, 701, g_threads);
+    tdg_f64_uniform((double*)td_data(disc), n, 0.0, 0.1, seed, 702, g_threads);
+    tdg_f64_uniform((double*)td_data(tax), n, 0.0, 0.08, seed, 703, g_threads);
+    td_t* t = check(td_table_new(3), "charge");
+    t = add(t, "price", price);
+    t = add(t, "disc", disc);
+    t = add(t, "tax", tax);
+    return t;
+}
+
+static td_t* run_charge(td_t* t) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* e = td_mul(g, td_mul(g, td_scan(g, "price"),
+                                  td_sub(g, td_const_f64(g, 1.0), td_scan(g, "disc"))),
+                        td_add(g, td_const_f64(g, 1.0), td_scan(g, "tax")));
+    td_t* out = check(td_execute(g, td_optimize(g, e)), "charge");
+    td_graph_free(g);
+    return out;
+}
+
 static td_t* run_expr_vec(td_t* t) {
     td_t* out = check(td_table_new(8), "expr vec");
     for (int i = 0; i < 8; i++) {
@@ -467,6 +494,18 @@ static void bench(int argc, char** argv) {
             double t1 = now_ms();
             td_release(a);
             td_release(b);
+            if (r >= 0) t[r] = t1 - t0;
+        }
+        td_release(tb);
+    } else if (!strcmp(wl, "charge")) {
+        int64_t n = atoll(argv[3]);
+        td_t* tb = make_charge(n, 42);
+        rows = n;
+        for (int r = -1; r < reps; r++) {
+            double t0 = now_ms();
+            td_t* v = run_charge(tb);
+            double t1 = now_ms();
+            td_release(v);
             if (r >= 0) t[r] = t1 - t0;
         }
         td_release(tb);
