@@ -278,6 +278,13 @@ typedef struct {
  * i64 clamps. */
 int tdd_expr_eval(const tdd_expr_t* e, int64_t n, void* out);
 
+/* The bias of an affine aggregate input, SUM / AVG(col +- c), applied to
+ * the group results of SUM / AVG(col) as the reference's emit does
+ * (exec.c:4660-4705): f64 SUM += bias_f * count, i64 SUM += bias_i * count
+ * (wrapping), AVG += bias_f.  counts: per-group row counts (SUM only). */
+int tdd_affine_fix(void* vals, int8_t type, int avg, const int64_t* counts, int64_t n,
+                   int64_t bias_i, double bias_f);
+
 /* ----------------------------------------------------------------- sort -- */
 /* One sort key: a column, its direction and NaN placement, and for SYM
  * columns the rank of every symbol id by its string (device array). */

@@ -336,6 +336,30 @@ static td_t* run_expr_grp(td_t* t) {
     return out;
 }
 
+/* GROUP(sym) of affine aggregate inputs, SUM / AVG(col +- c), which the
+ * reference aggregates as the column plus a bias (exec.c:614-694), beside
+ * one that is not affine (a fractional bias on an integer column). */
+static td_t* run_expr_aff(td_t* t) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* keys[1] = { td_scan(g, "sym") };
+    td_op_t* ins[7] = {
+        td_add(g, vf, td_const_f64(g, 2.5)),
+        td_sub(g, vi, td_const_i64(g, 7)),
+        td_add(g, vi, td_const_f64(g, 3.0)),
+        td_add(g, td_const_i64(g, 10), vf),
+        td_sub(g, vf, td_const_f64(g, 1.0)),
+        td_add(g, vi, td_const_f64(g, 0.5)),
+        vi,
+    };
+    uint16_t ops[7] = { OP_SUM, OP_AVG, OP_SUM, OP_SUM, OP_AVG, OP_SUM, OP_COUNT };
+    td_op_t* root = td_group(g, keys, 1, ops, ins, 7);
+    td_t* out = check(td_execute(g, root), "expr aff");
+    td_graph_free(g);
+    return out;
+}
+
 static td_op_t* expr_vec_op(td_graph_t* g, int i) {
     td_op_t* vi = td_scan(g, "v_i64");
     td_op_t* vf = td_scan(g, "v_f64");
@@ -592,7 +616,8 @@ int main(int argc, char** argv) {
         td_t* t;
         make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
                 strtoull(argv[5], NULL, 10), &t);
-        td_t* o = !strcmp(argv[6], "grp") ? run_expr_grp(t) : run_expr_vec(t);
+        td_t* o = !strcmp(argv[6], "grp") ? run_expr_grp(t)
+                : !strcmp(argv[6], "aff") ? run_expr_aff(t) : run_expr_vec(t);
         write_table(argv[7], o);
         td_release(o);
         td_release(t);

@@ -399,3 +399,27 @@ extern "C" int tdd_expr_eval(const tdd_expr_t* e, int64_t n, void* out) {
            shmem, X, n, out);
     return TD_OK;
 }
+
+/* ------------------------------------------------- affine aggregate bias -- */
+
+__global__ void k_affine_fix(void* vals, int is_f64, int avg, const int64_t* counts, int64_t n,
+                             int64_t bias_i, double bias_f) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    if (is_f64) {
+        double* v = (double*)vals;
+        v[g] = avg ? v[g] + bias_f : v[g] + bias_f * (double)counts[g];
+    } else {
+        uint64_t* v = (uint64_t*)vals;
+        v[g] += (uint64_t)bias_i * (uint64_t)counts[g];
+    }
+}
+
+extern "C" int tdd_affine_fix(void* vals, int8_t type, int avg, const int64_t* counts,
+                              int64_t n, int64_t bias_i, double bias_f) {
+    if (n <= 0) return TD_OK;
+    if (!avg && !counts) return TD_ERR_NYI;
+    LAUNCH("affine_fix", k_affine_fix, ceil_div(n, 256), 256, 0, vals, type == TD_F64, avg,
+           counts, n, bias_i, bias_f);
+    return TD_OK;
+}

@@ -39,6 +39,7 @@
  */
 #include "internal.h"
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -713,6 +714,134 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
     return group_over(x, ext, tbl, 1);
 }
 
+/* ---------------------------------------------- affine aggregate inputs -- */
+
+/* The value of a constant subtree as the reference's aggregate rewrites
+ * evaluate it (eval_const_numeric_expr, exec.c:380-458): constants, NEG /
+ * ABS, and binary arithmetic over constants, in f64 when an operand is f64,
+ * the node is typed f64 or the op is DIV (division or modulo by zero give
+ * 0), otherwise in wrapping i64. */
+static int const_numeric(td_graph_t* g, td_op_t* op, double* f, int64_t* i, int* isf) {
+    if (!op) return 0;
+    if (op->opcode == OP_CONST) {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        if (!e || !e->literal || !td_is_atom(e->literal)) return 0;
+        return atom_num(e->literal, i, f, isf);
+    }
+    if ((op->opcode == OP_NEG || op->opcode == OP_ABS) && op->arity == 1) {
+        double af;
+        int64_t ai;
+        int af64;
+        if (!const_numeric(g, op->inputs[0], &af, &ai, &af64)) return 0;
+        if (af64 || op->out_type == TD_F64) {
+            double v = af64 ? af : (double)ai;
+            *f = op->opcode == OP_NEG ? -v : fabs(v);
+            *i = (int64_t)*f;
+            *isf = 1;
+            return 1;
+        }
+        uint64_t u = (uint64_t)ai;
+        *i = (int64_t)(op->opcode == OP_NEG || ai < 0 ? 0 - u : u);
+        if (op->opcode == OP_ABS && ai >= 0) *i = ai;
+        *f = (double)*i;
+        *isf = 0;
+        return 1;
+    }
+    if (op->arity != 2 || op->opcode < OP_ADD || op->opcode > OP_MAX2) return 0;
+    double lf, rf;
+    int64_t li, ri;
+    int lf64, rf64;
+    if (!const_numeric(g, op->inputs[0], &lf, &li, &lf64) ||
+        !const_numeric(g, op->inputs[1], &rf, &ri, &rf64))
+        return 0;
+    if (op->out_type == TD_F64 || lf64 || rf64 || op->opcode == OP_DIV) {
+        double a = lf64 ? lf : (double)li, b = rf64 ? rf : (double)ri, r;
+        switch (op->opcode) {
+        case OP_ADD: r = a + b; break;
+        case OP_SUB: r = a - b; break;
+        case OP_MUL: r = a * b; break;
+        case OP_DIV: r = b != 0.0 ? a / b : 0.0; break;
+        case OP_MOD: r = b != 0.0 ? fmod(a, b) : 0.0; break;
+        case OP_MIN2: r = a < b ? a : b; break;
+        case OP_MAX2: r = a > b ? a : b; break;
+        default: return 0;
+        }
+        *f = r;
+        *i = (int64_t)r;
+        *isf = 1;
+        return 1;
+    }
+    int ok = ri != 0 && !(li == INT64_MIN && ri == -1);
+    int64_t r;
+    switch (op->opcode) {
+    case OP_ADD: r = (int64_t)((uint64_t)li + (uint64_t)ri); break;
+    case OP_SUB: r = (int64_t)((uint64_t)li - (uint64_t)ri); break;
+    case OP_MUL: r = (int64_t)((uint64_t)li * (uint64_t)ri); break;
+    case OP_DIV: r = ok ? li / ri : 0; break;
+    case OP_MOD: r = ok ? li % ri : 0; break;
+    case OP_MIN2: r = li < ri ? li : ri; break;
+    case OP_MAX2: r = li > ri ? li : ri; break;
+    default: return 0;
+    }
+    *i = r;
+    *f = (double)r;
+    *isf = 0;
+    return 1;
+}
+
+typedef struct {
+    int     on;
+    int64_t bias_i;
+    double  bias_f;
+} affine_t;
+
+/* SUM / AVG(col + c), (c + col) or (col - c) over a column of `tbl`: the
+ * column and the bias (try_affine_sumavg_input, exec.c:614-694).  An
+ * integer column takes only an integral bias. */
+static td_t* affine_input(td_graph_t* g, td_t* tbl, td_op_t* op, affine_t* A) {
+    if ((op->opcode != OP_ADD && op->opcode != OP_SUB) || op->arity != 2) return NULL;
+    double lf, rf, cf;
+    int64_t li, ri, ci;
+    int lf64, rf64, cf64;
+    int lc = const_numeric(g, op->inputs[0], &lf, &li, &lf64);
+    int rc = const_numeric(g, op->inputs[1], &rf, &ri, &rf64);
+    td_op_t* base;
+    int sign = 1;
+    if (op->opcode == OP_ADD && lc) {
+        base = op->inputs[1]; cf = lf; ci = li; cf64 = lf64;
+    } else if (op->opcode == OP_ADD && rc) {
+        base = op->inputs[0]; cf = rf; ci = ri; cf64 = rf64;
+    } else if (op->opcode == OP_SUB && rc) {
+        base = op->inputs[0]; cf = rf; ci = ri; cf64 = rf64; sign = -1;
+    } else {
+        return NULL;
+    }
+    if (base->opcode != OP_SCAN) return NULL;
+    td_t* col = scan_col(g, base, tbl);
+    if (!col) return NULL;
+    int8_t t = col->type;
+    if (t == TD_F64) {
+        A->bias_f = (double)sign * (cf64 ? cf : (double)ci);
+        A->bias_i = (int64_t)A->bias_f;
+    } else if (t == TD_I64 || t == TD_TIMESTAMP || t == TD_I32 || t == TD_I16 || t == TD_U8 ||
+               t == TD_BOOL || t == TD_SYM) {
+        int64_t c = ci;
+        if (cf64) {
+            double ip;
+            if (!isfinite(cf) || modf(cf, &ip) != 0.0 || ip > (double)INT64_MAX ||
+                ip < (double)INT64_MIN)
+                return NULL;
+            c = (int64_t)ip;
+        }
+        A->bias_i = sign > 0 ? c : (int64_t)(0 - (uint64_t)c);
+        A->bias_f = (double)A->bias_i;
+    } else {
+        return NULL;
+    }
+    A->on = 1;
+    return col;
+}
+
 static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, td_t** ev);
 
 /* GROUP of `tbl`'s columns; with use_sel, the graph's lazy selection (or the
@@ -738,6 +867,8 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
     S.force_path = td__force_group_path();
     td_t* key_src[TDD_MAX_KEYS];
     td_t* agg_src[TDD_MAX_AGGS];
+    affine_t aff[TDD_MAX_AGGS];
+    int need_count = 0;
     int rc;
     for (int k = 0; k < ext->n_keys; k++) {
         td_op_t* kop = strip_filter(ext->keys[k]);
@@ -758,11 +889,20 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
         td_op_t* aop = strip_filter(ext->agg_ins[a]);
         S.ops[a] = ext->agg_ops[a];
         agg_src[a] = NULL;
+        aff[a].on = 0;
+        td_t* base = NULL;
         if (aop->opcode == OP_SCAN) {
             agg_src[a] = scan_col(g, aop, tbl);
             if (!agg_src[a]) return TD_ERR_PTR(TD_ERR_SCHEMA);
             if (agg_src[a]->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
             if ((rc = dev_col(x, agg_src[a], &S.aggs[a]))) return TD_ERR_PTR(rc);
+        } else if ((S.ops[a] == OP_SUM || S.ops[a] == OP_AVG) &&
+                   (base = affine_input(g, tbl, aop, &aff[a]))) {
+            /* aggregate the column, add the bias to the results */
+            if (base->len != nrows) return TD_ERR_PTR(TD_ERR_LENGTH);
+            agg_src[a] = base;
+            if ((rc = dev_col(x, base, &S.aggs[a]))) return TD_ERR_PTR(rc);
+            if (S.ops[a] == OP_SUM) need_count = 1;
         } else if (expr_elementwise(aop->opcode)) {
             td_t* e = eval_expr(x, tbl, aop);
             if (TD_IS_ERR(e)) return e;
@@ -794,10 +934,29 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
     tdd_sel_t mv;
     if (sel && (rc = dev_sel(x, sel, &mv))) { td_release(sel); return TD_ERR_PTR(rc); }
     S.mask = sel ? &mv : NULL;
+    /* an affine SUM needs the group counts: a hidden COUNT after the others */
+    int hidden = -1;
+    if (need_count) {
+        if (S.n_aggs >= TDD_MAX_AGGS) { td_release(sel); return TD_ERR_PTR(TD_ERR_NYI); }
+        hidden = S.n_aggs++;
+        S.ops[hidden] = OP_COUNT;
+        memset(&S.aggs[hidden], 0, sizeof(S.aggs[hidden]));
+    }
     tdd_group_result_t R;
     rc = tdd_group(&S, &R);
     td_release(sel);
     if (rc) return TD_ERR_PTR(rc);
+    for (int a = 0; a < ext->n_aggs && !rc; a++)
+        if (aff[a].on)
+            rc = tdd_affine_fix(R.vals[a], R.out_types[a], S.ops[a] == OP_AVG,
+                                hidden >= 0 ? (const int64_t*)R.vals[hidden] : NULL, R.n_groups,
+                                aff[a].bias_i, aff[a].bias_f);
+    if (hidden >= 0) td__arena_free(R.vals[hidden]);
+    if (rc) {
+        for (int k = 0; k < ext->n_keys; k++) td__arena_free(R.keys[k]);
+        for (int a = 0; a < ext->n_aggs; a++) td__arena_free(R.vals[a]);
+        return TD_ERR_PTR(rc);
+    }
     td__set_plan("group path=%d groups=%lld", R.path, (long long)R.n_groups);
     return group_table(g, ext, &R, key_src, agg_src);
 }

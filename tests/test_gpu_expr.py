@@ -354,3 +354,38 @@ def test_large_expression_splits(eng):
     typ, ref = orc.expr_eval(X.bind(e, cols))
     X.check_vec(run_vec(eng, t, e), typ, ref, "split")
     eng.release(t)
+
+
+@pytest.mark.parametrize("resident", [False, True])
+@pytest.mark.parametrize("optimize", [False, True])
+def test_affine_aggregates_golden(eng, resident, optimize):
+    """SUM / AVG(col +- c) aggregate the column and add the bias to the
+    results, as the reference does; an integral bias on an integer column
+    keeps the integer SUM, a fractional one is an f64 expression."""
+    from test_oracle import AFF_CASES
+    d = hc("expr_aff_60k")
+    t = hc_table(eng, d, resident)
+    g = eng.graph(t)
+    ins = []
+    for op, col, c in AFF_CASES:
+        s = g.scan(col)
+        if c is None:
+            ins.append((op, s))
+        elif col == "v_f64" and c == 10:
+            ins.append((op, g.add(g.const_i64(10), s)))
+        elif c < 0:
+            ins.append((op, g.sub(s, g.const_f64(-c) if isinstance(c, float) else g.const_i64(-c))))
+        else:
+            ins.append((op, g.add(s, g.const_f64(c) if isinstance(c, float) else g.const_i64(c))))
+    root = g.group([g.scan("sym")], ins)
+    if optimize:
+        root = g.optimize(root)
+    out = eng.read_table(g.execute(root))
+    g.free()
+    ref = read_tdt(os.path.join(GOLD, "expr_aff_60k.tdt"))
+    assert [c[0] for c in out] == [c[0] for c in ref]
+    assert [c[1] for c in out] == [c[1] for c in ref]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]],
+                   [ref[0][3].astype(np.int64)], [c[3] for c in ref[1:]],
+                   [True, True, False, True, True, True, False], "affine")
+    eng.release(t)

@@ -426,3 +426,52 @@ def test_expr_oracle_vs_reference_random(ref_engine):
         typ, ref = oracle(e, cols)
         check(ref, typ, run_vec(ref_engine, rtab, e), f"tree {i}: {e}")
     ref_engine.release(rtab)
+
+
+AFF_CASES = [  # (op, base column, bias sign * constant, integral-only on integers)
+    ("SUM", "v_f64", 2.5), ("AVG", "v_i64", -7), ("SUM", "v_i64", 3.0), ("SUM", "v_f64", 10),
+    ("AVG", "v_f64", -1.0), ("SUM", "v_i64", 0.5), ("COUNT", "v_i64", None)]
+
+
+def affine_oracle(d):
+    """GROUP(sym) of the affine inputs of tests/golden/expr_aff_60k: SUM / AVG
+    of the base column plus the bias at emit (exec.c:4660-4705) where the
+    rewrite applies (f64 base, or an integral bias on an integer base),
+    else the expression itself; returns (keys, vals, exact-comp, sum_abs)."""
+    vals, comp, sabs = [], [], []
+    keys = None
+    for op, col, c in AFF_CASES:
+        x = d[col]
+        affine = c is not None and (x.dtype == np.float64 or float(c).is_integer())
+        src = x if (c is None or affine) else x.astype(np.float64) + c
+        r = orc.group([d["sym"]], [op, "COUNT"], [src, src], key_types=[(20, 2)])
+        keys = r["keys"]
+        v, cnt = r["vals"][0], r["vals"][1]
+        if affine and op == "SUM":
+            v = v + (np.float64(c) * cnt if x.dtype == np.float64
+                     else np.int64(c) * cnt)
+        elif affine and op == "AVG":
+            v = v + np.float64(c)
+        vals.append(v)
+        cv = r["comp"][0]
+        if affine and op == "SUM":
+            cv = cv + np.float64(c) * cnt
+        elif affine and op == "AVG":
+            cv = cv + np.float64(c)
+        comp.append(cv)
+        sabs.append(r["sum_abs"][0] + (abs(np.float64(c)) * cnt if c is not None else 0))
+    return keys, vals, comp, sabs
+
+
+def test_expr_affine_matches_reference():
+    argv = MANIFEST["expr_aff_60k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("expr_aff_60k.tdt")
+    assert [c[0] for c in ref] == ["sym", "_e0_sum", "_e1_mean", "_e2_sum", "_e3_sum",
+                                   "_e4_mean", "_e5_sum", "v_i64_count"]
+    # an integral bias on an integer column keeps the integer SUM
+    assert [c[1] for c in ref[1:]] == [7, 7, 6, 7, 7, 7, 6]
+    keys, vals, comp, sabs = affine_oracle(d)
+    compare_groups(keys, vals, [ref[0][3]], [c[3] for c in ref[1:]],
+                   [True, True, False, True, True, True, False], "affine",
+                   exact=comp, sum_abs=sabs)
