@@ -303,6 +303,97 @@ class Charge:
         self.e.release(self.t)
 
 
+class Parted:
+    """A date-partitioned table: P daily partitions of the high-cardinality
+    recipe's rows (key, sym, v_i64, v_f64) saved in the reference's on-disk
+    layout (page cache), loaded straight into HBM (td_dev_parted_load, each
+    column's partitions back to back in one block), then per step
+    GROUP(date; SUM v_i64, SUM v_f64, MIN v_f64, MAX v_i64, COUNT) and
+    GROUP(sym; SUM v_i64, SUM v_f64, COUNT), the date key repeated over
+    each partition's rows on the device."""
+    name = "parted"
+    metric = "rows/s of group-bys over a date-partitioned table in HBM"
+
+    def __init__(self, e, a):
+        import shutil
+        import tempfile
+        from teide_amd import workloads as W
+        self.e, self.a = e, a
+        n, P = a.parted_rows, a.parted_parts
+        nsym = 100_000
+        for i in range(nsym):                     # the sym ids name real symbols
+            e.sym(f"s{i:06d}")
+        d = W.high_card(n, 4_000_000, nsym, 1.1, SEED)
+        self.root = tempfile.mkdtemp(prefix="teide_parted_", dir="/tmp")
+        t0 = time.perf_counter()
+        per = n // P
+        for p in range(P):
+            lo, hi = p * per, (n if p == P - 1 else (p + 1) * per)
+            t = e.table([("key", d["key"][lo:hi]), ("sym", d["sym"][lo:hi], 20, 2),
+                         ("v_i64", d["v_i64"][lo:hi]), ("v_f64", d["v_f64"][lo:hi])])
+            part = os.path.join(self.root, f"2024.{1 + p // 28:02d}.{1 + p % 28:02d}")
+            os.makedirs(part)
+            if e.L.td_splay_save(t, os.path.join(part, "t").encode(), None) != 0:
+                raise RuntimeError("td_splay_save failed")
+            e.release(t)
+        del d
+        if e.L.td_sym_save(os.path.join(self.root, "sym").encode()) != 0:
+            raise RuntimeError("td_sym_save failed")
+        t1 = time.perf_counter()
+        self.t = e.L.td_dev_parted_load(self.root.encode(), b"t")
+        e.L.td_dev_sync()
+        t2 = time.perf_counter()
+        if not self.t or self.t < 32:
+            raise RuntimeError(f"td_dev_parted_load failed: {self.t}")
+        self.load_gbs = n * 28 / (t2 - t1) / 1e9
+        print(f"# data: save {t1 - t0:.1f}s, HBM load {t2 - t1:.2f}s", file=sys.stderr, flush=True)
+        self.rows = 2 * n
+        self.query_bytes = n * (4 + 8 + 8) + n * (4 + 8 + 8)
+        self.config = {"workload": f"{n:,} rows in {P} daily partitions (key, sym u32 over "
+                                   f"{nsym:,} symbols, v_i64, v_f64), loaded into HBM by "
+                                   "td_dev_parted_load; GROUP(date; 5 aggregates) and "
+                                   "GROUP(sym; 3 aggregates)",
+                       "hbm_load_gbs": round(self.load_gbs, 2)}
+        self._shutil = shutil
+
+    def _group(self, w):
+        e = self.e
+        g = e.graph(self.t)
+        vi, vf = g.scan("v_i64"), g.scan("v_f64")
+        if w == 0:
+            root = g.group([g.scan("date")], [("SUM", vi), ("SUM", vf), ("MIN", vf),
+                                              ("MAX", vi), ("COUNT", vi)])
+        else:
+            root = g.group([g.scan("sym")], [("SUM", vi), ("SUM", vf), ("COUNT", vi)])
+        out = g.execute(g.optimize(root))
+        g.free()
+        return out
+
+    def step(self):
+        self.e.release(self._group(0))
+        self.e.release(self._group(1))
+
+    def kernel_bytes(self, name):
+        return None
+
+    def cpu_baseline(self):
+        exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+        if not os.path.exists(exe):
+            return None
+        env = dict(os.environ, TD_REF_THREADS=str(cpu_threads()))
+        out = subprocess.run([exe, "bench", "partgroup", self.root, "3"], check=True,
+                             capture_output=True, text=True, env=env, timeout=900)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": 2 * r["rows"] / (r["median_ms"] / 1e3), "unit": "rows/s",
+                "cores": r["threads"], "kind": "reference",
+                "sample": f"the same database ({r['rows']:,} rows) read by td_read_parted, "
+                          "both group-bys, median of 3 after 1 warm-up"}
+
+    def release(self):
+        self.e.release(self.t)
+        self._shutil.rmtree(self.root, ignore_errors=True)
+
+
 class BatchQueries:
     """Many independent filter -> join -> group-by queries: fact rows
     log-uniform in [1K, 2M], dim = fact / 8 with unique keys, group
@@ -654,7 +745,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge"],
+    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge",
+                                           "parted"],
                     default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
@@ -662,6 +754,8 @@ def main():
                          "load: splayed-table load into HBM")
     ap.add_argument("--load-rows", type=int, default=50_000_000)
     ap.add_argument("--expr-rows", type=int, default=250_000_000)
+    ap.add_argument("--parted-rows", type=int, default=120_000_000)
+    ap.add_argument("--parted-parts", type=int, default=30)
     ap.add_argument("--load-reps", type=int, default=3)
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-fact", type=int, default=100_000_000)
@@ -729,8 +823,8 @@ def main():
         print(json.dumps(line), flush=True)
         return
 
-    wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge}.get(a.workload,
-                                                                        JoinGroup)(e, a)
+    wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge,
+          "parted": Parted}.get(a.workload, JoinGroup)(e, a)
 
     for _ in range(a.warmup):
         wl.step()

@@ -31,7 +31,8 @@ int      td_dev_available(void);
 td_t*    td_dev_put(td_t* v);
 /* Host copy of a vector, TD_SEL or table (host input: retained as is). */
 td_t*    td_dev_get(td_t* v);
-/* 1 when v (or every column of table v) is device-resident. */
+/* 1 when v (or every column of table v) is device-resident; a parted
+ * column is when all its segments are, a MAPCOMMON column always is. */
 int      td_dev_is_resident(td_t* v);
 /* Column files and splayed tables (the reference's on-disk format,
  * store/col.c, store/splay.c) loaded straight into the HBM arena, through
@@ -39,6 +40,12 @@ int      td_dev_is_resident(td_t* v);
  * splayed form loads the symbol table first when sym_path is given. */
 td_t*    td_dev_col_load(const char* path);
 td_t*    td_dev_splay_load(const char* dir, const char* sym_path);
+/* A partitioned database's table (store/part.c: td_read_parted's layout,
+ * a MAPCOMMON partition-key column then parted columns) with every parted
+ * column's partitions back to back in one arena block, its segments views
+ * into that block; queries read such a column as one flat column in place.
+ * The MAPCOMMON column (partition keys and row counts) stays on the host. */
+td_t*    td_dev_parted_load(const char* db_root, const char* table_name);
 /* Wait for all queued device work. */
 td_err_t td_dev_sync(void);
 

@@ -285,6 +285,13 @@ int tdd_expr_eval(const tdd_expr_t* e, int64_t n, void* out);
 int tdd_affine_fix(void* vals, int8_t type, int avg, const int64_t* counts, int64_t n,
                    int64_t bias_i, double bias_f);
 
+/* The per-row column of a MAPCOMMON (partition key) column: out[r] = kv[p]
+ * for rows r in [offs[p], offs[p+1]) (device arrays; offs has n_parts + 1
+ * entries, offs[0] = 0), elements of esz bytes (materialize_mapcommon,
+ * exec.c:201-241). */
+int tdd_mapcommon_fill(void* out, int esz, const void* kv, const int64_t* offs, int n_parts,
+                       int64_t n);
+
 /* ----------------------------------------------------------------- sort -- */
 /* One sort key: a column, its direction and NaN placement, and for SYM
  * columns the rank of every symbol id by its string (device array). */

@@ -34,6 +34,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <errno.h>
+#include <sys/stat.h>
 
 static unsigned g_ref_threads;           /* pool workers + the caller */
 
@@ -431,6 +433,66 @@ static td_t* run_expr_vec(td_t* t) {
     return out;
 }
 
+/* A date-partitioned database (store/part.c layout) of the high-cardinality
+ * recipe's rows: NP partitions 2024.01.01, 2024.01.02, ... of table `t`,
+ * partition p holding the next (p + 1) / (NP (NP + 1) / 2) share of the
+ * rows, and the symbol file `sym`. */
+static void run_partsave(td_t* t, int np, const char* root) {
+    int64_t n = td_table_nrows(t), at = 0, tri = (int64_t)np * (np + 1) / 2;
+    if (mkdir(root, 0755) != 0 && errno != EEXIST) die("partsave: mkdir");
+    char dir[1024];
+    for (int p = 0; p < np; p++) {
+        int64_t len = p == np - 1 ? n - at : n * (p + 1) / tri;
+        snprintf(dir, sizeof(dir), "%s/2024.01.%02d", root, p + 1);
+        if (mkdir(dir, 0755) != 0 && errno != EEXIST) die("partsave: mkdir");
+        td_t* part = check(td_table_new(4), "part");
+        for (int64_t c = 0; c < td_table_ncols(t); c++) {
+            td_t* col = td_table_get_col_idx(t, c);
+            uint8_t esz = td_sym_elem_size(col->type, col->attrs);
+            td_t* v = col->type == TD_SYM ? check(td_sym_vec_new(col->attrs & TD_SYM_W_MASK, len), "seg")
+                                          : vec_of(col->type, len);
+            v->len = len;
+            memcpy(td_data(v), (char*)td_data(col) + at * esz, (size_t)len * esz);
+            part = check(td_table_add_col(part, td_table_col_name(t, c), v), "seg");
+            td_release(v);
+        }
+        snprintf(dir, sizeof(dir), "%s/2024.01.%02d/t", root, p + 1);
+        if (td_splay_save(part, dir, NULL) != TD_OK) die("partsave: splay");
+        td_release(part);
+        at += len;
+    }
+    snprintf(dir, sizeof(dir), "%s/sym", root);
+    if (td_sym_save(dir) != TD_OK) die("partsave: sym");
+}
+
+/* GROUP over td_read_parted of that database: by the partition key (date),
+ * by a parted column (sym), and by both. */
+static td_t* run_partgroup(td_t* t, int which) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* root;
+    if (which == 0) {
+        td_op_t* keys[1] = { td_scan(g, "date") };
+        td_op_t* ins[5] = { vi, vf, vf, vi, vi };
+        uint16_t ops[5] = { OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_COUNT };
+        root = td_group(g, keys, 1, ops, ins, 5);
+    } else if (which == 1) {
+        td_op_t* keys[1] = { td_scan(g, "sym") };
+        td_op_t* ins[3] = { vi, vf, vi };
+        uint16_t ops[3] = { OP_SUM, OP_SUM, OP_COUNT };
+        root = td_group(g, keys, 1, ops, ins, 3);
+    } else {
+        td_op_t* keys[2] = { td_scan(g, "date"), td_scan(g, "sym") };
+        td_op_t* ins[2] = { vi, vi };
+        uint16_t ops[2] = { OP_COUNT, OP_SUM };
+        root = td_group(g, keys, 2, ops, ins, 2);
+    }
+    td_t* out = check(td_execute(g, td_optimize(g, root)), "partgroup");
+    td_graph_free(g);
+    return out;
+}
+
 /* General equi-join on one i64 key: left (k, lv), right (k, rv). */
 static td_t* run_join(int64_t nl, int64_t nr, int64_t krange, int jtype, uint64_t seed) {
     td_t* lk = vec_of(TD_I64, nl);
@@ -466,6 +528,8 @@ static int cmp_d(const void* a, const void* b) {
     double x = *(const double*)a, y = *(const double*)b;
     return x < y ? -1 : x > y;
 }
+
+static td_t* run_partgroup(td_t* t, int which);
 
 static void bench(int argc, char** argv) {
     if (argc < 4) die("bench: WORKLOAD ARGS... REPS");
@@ -515,6 +579,21 @@ static void bench(int argc, char** argv) {
             double t0 = now_ms();
             td_t* a = run_hc_group(tb, "key");
             td_t* b = run_hc_group(tb, "sym");
+            double t1 = now_ms();
+            td_release(a);
+            td_release(b);
+            if (r >= 0) t[r] = t1 - t0;
+        }
+        td_release(tb);
+    } else if (!strcmp(wl, "partgroup")) {
+        /* a partitioned database on disk (argv[3]): td_read_parted once,
+         * then per repetition GROUP(date) and GROUP(sym) as run_partgroup */
+        td_t* tb = check(td_read_parted(argv[3], "t"), "read_parted");
+        rows = td_table_nrows(tb);
+        for (int r = -1; r < reps; r++) {
+            double t0 = now_ms();
+            td_t* a = run_partgroup(tb, 0);
+            td_t* b = run_partgroup(tb, 1);
             double t1 = now_ms();
             td_release(a);
             td_release(b);
@@ -620,6 +699,27 @@ int main(int argc, char** argv) {
                 : !strcmp(argv[6], "aff") ? run_expr_aff(t) : run_expr_vec(t);
         write_table(argv[7], o);
         td_release(o);
+        td_release(t);
+    } else if (!strcmp(cmd, "partsave") && argc == 8) {
+        /* the sym column's ids name real symbols (the reference's parted
+         * group-by sizes a bitmap by the symbol count) */
+        char nm[32];
+        for (int64_t i = 0; i < atoll(argv[4]); i++) {
+            int k = snprintf(nm, sizeof(nm), "s%05lld", (long long)i);
+            td_sym_intern(nm, (size_t)k);
+        }
+        td_t* t;
+        make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
+                strtoull(argv[5], NULL, 10), &t);
+        run_partsave(t, atoi(argv[6]), argv[7]);
+        td_release(t);
+    } else if (!strcmp(cmd, "partgroup") && argc == 6) {
+        td_t* t = check(td_read_parted(argv[2], "t"), "read_parted");
+        for (int w = 0; w < 3; w++) {
+            td_t* o = run_partgroup(t, w);
+            write_table(argv[3 + w], o);
+            td_release(o);
+        }
         td_release(t);
     } else if (!strcmp(cmd, "save") && argc == 8) {
         td_t* t;

@@ -111,6 +111,87 @@ __device__ __forceinline__ void cells_accum(uint64_t* c, const GroupPlan& P, int
     cells_accum2<SKIP>(c, P, row, row);
 }
 
+/* Wave-wide reductions (every lane gets the result; fixed butterfly order). */
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = (int64_t)((uint64_t)v + (uint64_t)__shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { int64_t u = __shfl_xor(v, o); v = u < v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { int64_t u = __shfl_xor(v, o); v = u > v ? u : v; }
+    return v;
+}
+
+/* A wave whose selected rows (`act`, mask m) all fall in one group: the
+ * rows folded across the wave first, then one update per cell by the
+ * `leader` lane.  Every lane of the wave calls this together.  Runs of one
+ * key -- sorted or partitioned data, a partition key repeated over its
+ * rows -- otherwise send all 64 lanes' atomics to the same cells. */
+__device__ __forceinline__ void cells_accum_wave(uint64_t* c, const GroupPlan& P, int64_t row,
+                                                 bool act, uint64_t m, bool leader) {
+    if (leader) atomicAdd((unsigned long long*)&c[0], (unsigned long long)__popcll(m));
+    for (int i = 1; i < P.ncells; i++) {
+        const tdd_col_t& a = P.aggs[P.col[i]];
+        switch (P.kind[i]) {
+        case C_ADD_F64: {
+            double v = wave_sum_f64(act ? ld_f64(a, row) : 0.0);
+            if (leader) atomicAdd((double*)&c[i], v);
+        } break;
+        case C_SQ_F64: {
+            double x = act ? ld_f64(a, row) : 0.0;
+            double v = wave_sum_f64(x * x);
+            if (leader) atomicAdd((double*)&c[i], v);
+        } break;
+        case C_ADD_I64: {
+            int64_t v = wave_sum_i64(act ? ld_i64(a, row) : 0);
+            if (leader) atomicAdd((unsigned long long*)&c[i], (unsigned long long)v);
+        } break;
+        case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW: {
+            int64_t x = !act ? INT64_MAX : P.kind[i] == C_MIN_I64 ? ld_i64(a, row)
+                      : P.kind[i] == C_MIN_F64 ? f64_ord(ld_f64(a, row)) : row;
+            int64_t v = wave_min_i64(x);
+            if (leader) cell_min(&c[i], v);
+        } break;
+        default: {                                  /* C_MAX_I64 / C_MAX_F64 / C_MAX_ROW */
+            int64_t x = !act ? INT64_MIN : P.kind[i] == C_MAX_I64 ? ld_i64(a, row)
+                      : P.kind[i] == C_MAX_F64 ? f64_ord(ld_f64(a, row)) : row;
+            int64_t v = wave_max_i64(x);
+            if (leader) cell_max(&c[i], v);
+        } break;
+        }
+    }
+}
+
+/* One wave of rows w * 64 + lane (selected by m) into a direct-array table:
+ * one group for the whole wave takes the wave-folded path, else each row
+ * its own atomics.  Every lane of the wave calls this together. */
+template <bool SKIP = true>
+__device__ __forceinline__ void da_wave(uint64_t* tab, const GroupPlan& P, int64_t w,
+                                        uint64_t m, int lane, int64_t (*slot_of)(const GroupPlan&,
+                                                                                 int64_t)) {
+    const bool act = (m >> lane) & 1;
+    const int64_t row = w * 64 + lane;
+    const int64_t slot = act ? slot_of(P, row) : -1;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    const int64_t s0 = __shfl(slot, first);
+    if (__ballot(act && slot != s0) == 0) {
+        cells_accum_wave(&tab[s0 * P.ncells], P, row, act, m, lane == first);
+    } else if (act) {
+        cells_accum<SKIP>(&tab[slot * P.ncells], P, row);
+    }
+}
+
 /* ----------------------------------------------------------- hash keys -- */
 
 #define EMPTY_KEY 0xffffffffffffffffull

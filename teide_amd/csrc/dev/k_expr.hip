@@ -423,3 +423,36 @@ extern "C" int tdd_affine_fix(void* vals, int8_t type, int avg, const int64_t* c
            counts, n, bias_i, bias_f);
     return TD_OK;
 }
+
+/* ----------------------------------------------------- partition column -- */
+
+/* out[r] = kv[p] for the partition p whose rows [offs[p], offs[p+1])
+ * hold r: the per-row key of a MAPCOMMON column (materialize_mapcommon,
+ * exec.c:201-241), esz 1, 2, 4 or 8. */
+__global__ void k_mapcommon(void* out, int esz, const void* kv, const int64_t* offs, int n_parts,
+                            int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        int lo = 0, hi = n_parts - 1;                 /* last p with offs[p] <= r */
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= r) lo = mid;
+            else hi = mid - 1;
+        }
+        switch (esz) {
+        case 1: ((uint8_t*)out)[r] = ((const uint8_t*)kv)[lo]; break;
+        case 2: ((uint16_t*)out)[r] = ((const uint16_t*)kv)[lo]; break;
+        case 4: ((uint32_t*)out)[r] = ((const uint32_t*)kv)[lo]; break;
+        default: ((uint64_t*)out)[r] = ((const uint64_t*)kv)[lo]; break;
+        }
+    }
+}
+
+extern "C" int tdd_mapcommon_fill(void* out, int esz, const void* kv, const int64_t* offs,
+                                  int n_parts, int64_t n) {
+    if (n <= 0) return TD_OK;
+    if (n_parts < 1 || (esz != 1 && esz != 2 && esz != 4 && esz != 8)) return TD_ERR_NYI;
+    LAUNCH("mapcommon", k_mapcommon, stream_grid(ceil_div(n, 256), 8), 256, 0, out, esz, kv,
+           offs, n_parts, n);
+    return TD_OK;
+}

@@ -228,10 +228,7 @@ __global__ __launch_bounds__(GT) void k_da_lds(GroupPlan P, RowMask M, int64_t n
     int64_t n_waves = (int64_t)gridDim.x * (GT / WAVE);
     for (int64_t w = wave; w < n_words; w += n_waves) {
         uint64_t m = word_mask(M, w);
-        if ((m >> lane) & 1) {
-            int64_t row = w * 64 + lane;
-            cells_accum(&tab[da_slot(P, row) * P.ncells], P, row);
-        }
+        if (m) da_wave(tab, P, w, m, lane, da_slot);
     }
     __syncthreads();
     uint64_t* dst = partials + (int64_t)blockIdx.x * ncell_total;
@@ -295,10 +292,7 @@ __global__ __launch_bounds__(GT) void k_da_global(GroupPlan P, RowMask M, uint64
     int64_t n_waves = (int64_t)gridDim.x * (GT / WAVE);
     for (int64_t w = wave; w < n_words; w += n_waves) {
         uint64_t m = word_mask(M, w);
-        if ((m >> lane) & 1) {
-            int64_t row = w * 64 + lane;
-            cells_accum(&tab[da_slot(P, row) * P.ncells], P, row);
-        }
+        if (m) da_wave(tab, P, w, m, lane, da_slot);
     }
 }
 

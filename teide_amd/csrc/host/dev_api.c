@@ -97,15 +97,75 @@ done:
     return d;
 }
 
+/* A parted column with all its segments back to back in one arena block,
+ * each a view into it; the block carries the column statistics. */
+static td_t* put_parted(td_t* v) {
+    td_t** segs = (td_t**)td_data(v);
+    int8_t base = (int8_t)TD_PARTED_BASETYPE(v->type);
+    uint8_t attrs = 0;
+    for (int64_t i = 0; i < v->len; i++)
+        if (segs[i]) { attrs = segs[i]->attrs & TD_SYM_W_MASK; break; }
+    uint8_t esz = td_sym_elem_size(base, attrs);
+    int64_t total = td_parted_nrows(v);
+    td_t* block = td__hbm_vec_new(base, attrs, total, (size_t)total * esz);
+    if (TD_IS_ERR(block)) return block;
+    td_t* out = td_alloc((size_t)v->len * sizeof(td_t*));
+    if (!out) { td_release(block); return TD_ERR_PTR(TD_ERR_OOM); }
+    out->type = v->type;
+    out->attrs = 0;
+    out->len = v->len;
+    memset(td_data(out), 0, (size_t)v->len * sizeof(td_t*));
+    int64_t at = 0;
+    int rc = TD_OK;
+    for (int64_t i = 0; i < v->len && !rc; i++) {
+        td_t* s = segs[i];
+        if (!s) continue;
+        if (s->type != base || td__esz(s) != esz) { rc = TD_ERR_TYPE; break; }
+        size_t off = (size_t)at * esz, bytes = (size_t)s->len * esz;
+        char* dst = (char*)td_hbm_ref(block)->dptr + off;
+        if (bytes) {
+            if (s->mmod == TD_MMOD_HBM) {
+                rc = tdd_d2d(dst, td_hbm_ref(s)->dptr, bytes);
+            } else if (s->attrs & TD_ATTR_SLICE) {
+                td_t* p = s->slice_parent;
+                rc = p->mmod == TD_MMOD_HBM
+                   ? tdd_d2d(dst, (char*)td_hbm_ref(p)->dptr + s->slice_offset * esz, bytes)
+                   : tdd_h2d(dst, (const char*)td_data(p) + s->slice_offset * esz, bytes);
+            } else {
+                rc = tdd_h2d(dst, td_data(s), bytes);
+            }
+        }
+        td_t* view = rc ? NULL : td__hbm_view(block, base, attrs, s->len, off, bytes);
+        if (view && TD_IS_ERR(view)) { rc = TD_ERR_CODE(view); break; }
+        ((td_t**)td_data(out))[i] = view;
+        at += s->len;
+    }
+    if (!rc) rc = td__dev_stats(block);
+    if (!rc) rc = tdd_sync();
+    td_release(block);                       /* the views hold it */
+    if (rc) { td_release(out); return TD_ERR_PTR(rc); }
+    return out;
+}
+
 td_t* td_dev_put(td_t* v) {
     if (!v || TD_IS_ERR(v)) return v;
     if (!td__dev_ready()) return TD_ERR_PTR(TD_ERR_IO);
+    if (TD_IS_PARTED(v->type)) return put_parted(v);
     if (v->type != TD_TABLE) return put_vec(v);
     int64_t nc = td_table_ncols(v);
     td_t* t = td_table_new(nc);
     if (TD_IS_ERR(t)) return t;
     for (int64_t c = 0; c < nc; c++) {
-        td_t* d = put_vec(td_table_get_col_idx(v, c));
+        td_t* col = td_table_get_col_idx(v, c);
+        td_t* d;
+        if (col->type == TD_MAPCOMMON) {
+            td_retain(col);                  /* partition keys: host metadata */
+            d = col;
+        } else if (TD_IS_PARTED(col->type)) {
+            d = put_parted(col);
+        } else {
+            d = put_vec(col);
+        }
         if (TD_IS_ERR(d)) { td_release(t); return d; }
         t = td_table_add_col(t, td_table_col_name(v, c), d);
         td_release(d);
@@ -132,6 +192,36 @@ static td_t* get_vec(td_t* v) {
     return h;
 }
 
+/* Host copy of a parted column: resident segments copied, host ones held. */
+static td_t* get_parted(td_t* v) {
+    td_t* out = td_alloc((size_t)v->len * sizeof(td_t*));
+    if (!out) return TD_ERR_PTR(TD_ERR_OOM);
+    out->type = v->type;
+    out->attrs = 0;
+    out->len = v->len;
+    memset(td_data(out), 0, (size_t)v->len * sizeof(td_t*));
+    for (int64_t i = 0; i < v->len; i++) {
+        td_t* s = ((td_t**)td_data(v))[i];
+        if (!s) continue;
+        td_t* h = s->mmod == TD_MMOD_HBM ? get_vec(s) : (td_retain(s), s);
+        if (TD_IS_ERR(h)) { td_release(out); return h; }
+        ((td_t**)td_data(out))[i] = h;
+    }
+    return out;
+}
+
+static int col_resident(td_t* c) {
+    if (c->type == TD_MAPCOMMON) return 1;
+    if (TD_IS_PARTED(c->type)) {
+        for (int64_t i = 0; i < c->len; i++) {
+            td_t* s = ((td_t**)td_data(c))[i];
+            if (s && s->mmod != TD_MMOD_HBM) return 0;
+        }
+        return 1;
+    }
+    return c->mmod == TD_MMOD_HBM;
+}
+
 td_t* td_dev_get(td_t* v) {
     if (!v || TD_IS_ERR(v)) return v;
     if (v->type == TD_TABLE) {
@@ -140,7 +230,9 @@ td_t* td_dev_get(td_t* v) {
         if (TD_IS_ERR(t)) return t;
         for (int64_t c = 0; c < nc; c++) {
             td_t* col = td_table_get_col_idx(v, c);
-            td_t* h = col->mmod == TD_MMOD_HBM ? get_vec(col) : (td_retain(col), col);
+            td_t* h = col->mmod == TD_MMOD_HBM ? get_vec(col)
+                    : TD_IS_PARTED(col->type) ? get_parted(col)
+                    : (td_retain(col), col);
             if (TD_IS_ERR(h)) { td_release(t); return h; }
             t = td_table_add_col(t, td_table_col_name(v, c), h);
             td_release(h);
@@ -148,6 +240,7 @@ td_t* td_dev_get(td_t* v) {
         }
         return t;
     }
+    if (TD_IS_PARTED(v->type)) return get_parted(v);
     if (v->mmod != TD_MMOD_HBM) { td_retain(v); return v; }
     return get_vec(v);
 }
@@ -168,10 +261,10 @@ int td_dev_is_resident(td_t* v) {
     if (!v || TD_IS_ERR(v)) return 0;
     if (v->type == TD_TABLE) {
         for (int64_t c = 0; c < td_table_ncols(v); c++)
-            if (td_table_get_col_idx(v, c)->mmod != TD_MMOD_HBM) return 0;
+            if (!col_resident(td_table_get_col_idx(v, c))) return 0;
         return 1;
     }
-    return v->mmod == TD_MMOD_HBM;
+    return col_resident(v);
 }
 
 void td_dev_mem_stats(td_dev_mem_t* out) {

@@ -59,6 +59,10 @@ typedef struct {
     td_t* up_v[MAX_UPLOADS];
     void* up_d[MAX_UPLOADS];
     int   nup;
+    /* parted tables of this execution and their flat forms (held) */
+    td_t* fl_src[8];
+    td_t* fl_dst[8];
+    int   nfl;
 } xctx_t;
 
 static td_t* xnode(xctx_t* x, td_op_t* op);
@@ -74,6 +78,146 @@ static void drop_tmps(xctx_t* x) {
     x->ntmp = 0;
     for (int i = 0; i < x->nup; i++) td_release(x->up_v[i]);
     x->nup = 0;
+    for (int i = 0; i < x->nfl; i++) td_release(x->fl_dst[i]);
+    x->nfl = 0;
+}
+
+/* ------------------------------------------------------ parted tables -- */
+
+/* A table of parted columns (one vector per partition) and a MAPCOMMON
+ * partition-key column queries as the concatenation of its partitions
+ * with the key repeated over each partition's rows, as the reference's
+ * concatenating paths read it (exec.c:201-241, 5443-5537, 11581-11608).
+ * Here the concatenation is a flat device column: in place when the
+ * segments are consecutive views of one arena block (td_dev_put,
+ * td_dev_parted_load), else copied into a new block; the key column is
+ * filled on the device. */
+static int has_parted(td_t* t) {
+    for (int64_t c = 0; t && t->type == TD_TABLE && c < td_table_ncols(t); c++) {
+        td_t* col = td_table_get_col_idx(t, c);
+        if (col && (TD_IS_PARTED(col->type) || col->type == TD_MAPCOMMON)) return 1;
+    }
+    return 0;
+}
+
+static td_t* flat_parted(xctx_t* x, td_t* v) {
+    td_t** segs = (td_t**)td_data(v);
+    int8_t base = (int8_t)TD_PARTED_BASETYPE(v->type);
+    uint8_t attrs = 0;
+    for (int64_t i = 0; i < v->len; i++)
+        if (segs[i]) { attrs = segs[i]->attrs & TD_SYM_W_MASK; break; }
+    uint8_t esz = td_sym_elem_size(base, attrs);
+    int64_t total = td_parted_nrows(v);
+    td_t* own = NULL;
+    const char* next = NULL;
+    int inplace = 1;
+    for (int64_t i = 0; i < v->len && inplace; i++) {
+        td_t* s = segs[i];
+        if (!s) continue;
+        if (s->mmod != TD_MMOD_HBM || !td_hbm_ref(s)->owner) { inplace = 0; break; }
+        if (!own) {
+            own = td_hbm_ref(s)->owner;
+            next = (const char*)td_hbm_ref(own)->dptr;
+        }
+        inplace = td_hbm_ref(s)->owner == own && (const char*)td_hbm_ref(s)->dptr == next &&
+                  td__esz(s) == esz;
+        next += (size_t)s->len * esz;
+    }
+    if (inplace && own && own->len == total && own->type == base && td__esz(own) == esz) {
+        td_retain(own);
+        return own;
+    }
+    td_t* d = td__hbm_vec_new(base, attrs, total, (size_t)total * esz);
+    if (TD_IS_ERR(d)) return d;
+    int64_t at = 0;
+    int rc = TD_OK;
+    for (int64_t i = 0; i < v->len && !rc; i++) {
+        td_t* s = segs[i];
+        if (!s || s->len == 0) continue;
+        if (s->type != base || td__esz(s) != esz) { rc = TD_ERR_TYPE; break; }
+        char* dst = (char*)td_hbm_ref(d)->dptr + (size_t)at * esz;
+        size_t bytes = (size_t)s->len * esz;
+        td_t* src = s;
+        size_t off = 0;
+        if (s->mmod != TD_MMOD_HBM && (s->attrs & TD_ATTR_SLICE)) {
+            src = s->slice_parent;
+            off = (size_t)s->slice_offset * esz;
+        }
+        if (src->mmod == TD_MMOD_HBM) {
+            rc = tdd_d2d(dst, (const char*)td_hbm_ref(src)->dptr + off, bytes);
+        } else {
+            x->all_resident = 0;                  /* a host partition: results go back */
+            rc = tdd_h2d(dst, (const char*)td_data(src) + off, bytes);
+        }
+        at += s->len;
+    }
+    if (!rc) rc = td__dev_stats(d);
+    if (rc) { td_release(d); return TD_ERR_PTR(rc); }
+    return d;
+}
+
+static td_t* flat_mapcommon(xctx_t* x, td_t* v) {
+    td_t* kv = ((td_t**)td_data(v))[0];
+    td_t* rc_v = ((td_t**)td_data(v))[1];
+    if (!kv || !rc_v || kv->len != rc_v->len || kv->len < 1 || rc_v->type != TD_I64)
+        return TD_ERR_PTR(TD_ERR_CORRUPT);
+    int64_t n = kv->len, total = 0;
+    uint8_t esz = td__esz(kv);
+    int64_t* offs = malloc((size_t)(n + 1) * sizeof(int64_t));
+    if (!offs) return TD_ERR_PTR(TD_ERR_OOM);
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t p = 0; p < n; p++) {
+        offs[p] = total;
+        total += ((const int64_t*)td_data(rc_v))[p];
+        int64_t k = kv->type == TD_DATE || kv->type == TD_I32 ? ((const int32_t*)td_data(kv))[p]
+                                                              : ((const int64_t*)td_data(kv))[p];
+        if (k < lo) lo = k;
+        if (k > hi) hi = k;
+    }
+    offs[n] = total;
+    td_t* d = td__hbm_vec_new(kv->type, kv->attrs & TD_SYM_W_MASK, total, (size_t)total * esz);
+    void* meta = TD_IS_ERR(d) ? NULL : td__arena_alloc((size_t)n * 8 + (size_t)(n + 1) * 8);
+    int rc = TD_IS_ERR(d) ? TD_ERR_CODE(d) : meta ? keep_tmp(x, meta) : TD_ERR_OOM;
+    if (!rc) rc = tdd_h2d(meta, td_data(kv), (size_t)n * esz);
+    if (!rc) rc = tdd_h2d((char*)meta + (size_t)n * 8, offs, (size_t)(n + 1) * sizeof(int64_t));
+    if (!rc) rc = tdd_mapcommon_fill(td_hbm_ref(d)->dptr, esz, meta,
+                                     (const int64_t*)((char*)meta + (size_t)n * 8), (int)n, total);
+    /* the staged keys and offsets are read before this call returns */
+    if (!rc) rc = tdd_sync();
+    free(offs);
+    if (rc) {
+        if (!TD_IS_ERR(d)) td_release(d);
+        return TD_ERR_PTR(rc);
+    }
+    td_hbm_ref(d)->lo = lo;
+    td_hbm_ref(d)->hi = hi;
+    td_hbm_ref(d)->ranged = 1;
+    return d;
+}
+
+/* The flat form of a table with parted columns (the table itself, held,
+ * when it has none), made once per execution. */
+static td_t* flat_table(xctx_t* x, td_t* t) {
+    if (!has_parted(t)) { td_retain(t); return t; }
+    for (int i = 0; i < x->nfl; i++)
+        if (x->fl_src[i] == t) { td_retain(x->fl_dst[i]); return x->fl_dst[i]; }
+    if (x->nfl >= 8) return TD_ERR_PTR(TD_ERR_NYI);
+    int64_t nc = td_table_ncols(t);
+    td_t* out = td_table_new(nc);
+    for (int64_t c = 0; c < nc && !TD_IS_ERR(out); c++) {
+        td_t* col = td_table_get_col_idx(t, c);
+        td_t* f = col->type == TD_MAPCOMMON ? flat_mapcommon(x, col)
+                : TD_IS_PARTED(col->type) ? flat_parted(x, col)
+                : (td_retain(col), col);
+        if (TD_IS_ERR(f)) { td_release(out); return f; }
+        out = td_table_add_col(out, td_table_col_name(t, c), f);
+        td_release(f);
+    }
+    if (TD_IS_ERR(out)) return out;
+    x->fl_src[x->nfl] = t;
+    x->fl_dst[x->nfl++] = out;
+    td_retain(out);
+    return out;
 }
 
 /* Device view of a column: resident columns in place, slices as an offset
@@ -1446,6 +1590,7 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
     case OP_CONST: {
         td_op_ext_t* e = td__find_ext(g, op->id);
         if (!e || !e->literal) return TD_ERR_PTR(TD_ERR_NYI);
+        if (e->literal->type == TD_TABLE) return flat_table(x, e->literal);
         td_retain(e->literal);
         return e->literal;
     }
@@ -1522,7 +1667,24 @@ td_t* td_execute(td_graph_t* g, td_op_t* root) {
     x.g = g;
     x.all_resident = 1;
     td__set_plan("-");
-    td_t* r = xnode(&x, root);
+    /* parted tables run as their flat forms */
+    td_t* saved = g->table;
+    td_t* saved_tabs[8];
+    int nt = g->tables && g->n_tables <= 8 ? (int)g->n_tables : 0;
+    td_t* r = NULL;
+    if (g->table) {
+        td_t* f = flat_table(&x, g->table);
+        if (TD_IS_ERR(f)) r = f;
+        else { td_release(f); g->table = f; }     /* held by the execution's cache */
+    }
+    for (int i = 0; i < nt; i++) {
+        saved_tabs[i] = g->tables[i];
+        if (r || !g->tables[i]) continue;
+        td_t* f = flat_table(&x, g->tables[i]);
+        if (TD_IS_ERR(f)) r = f;
+        else { td_release(f); g->tables[i] = f; }
+    }
+    if (!r) r = xnode(&x, root);
     if (!TD_IS_ERR(r) && g->selection && r->type == TD_TABLE) {
         td_t* c = compact_table(&x, r, g->selection);      /* final compaction */
         td_release(r);
@@ -1532,6 +1694,8 @@ td_t* td_execute(td_graph_t* g, td_op_t* root) {
     }
     if (!TD_IS_ERR(r) && !x.all_resident) r = td__to_host(r);
     if (!TD_IS_ERR(r) && tdd_sync() != TD_OK) { td_release(r); r = TD_ERR_PTR(TD_ERR_IO); }
+    g->table = saved;
+    for (int i = 0; i < nt; i++) g->tables[i] = saved_tabs[i];
     drop_tmps(&x);
     return r;
 }

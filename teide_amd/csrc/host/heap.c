@@ -169,7 +169,9 @@ static void release_children(td_t* v) {
         if (slots[0] && !TD_IS_ERR(slots[0])) td_release(slots[0]);
         for (int64_t i = 0; i < v->len; i++)
             if (slots[1 + i] && !TD_IS_ERR(slots[1 + i])) td_release(slots[1 + i]);
-    } else if (v->type == TD_LIST) {
+    } else if (v->type == TD_LIST || TD_IS_PARTED(v->type) || v->type == TD_MAPCOMMON) {
+        /* lists, parted columns (one vector per partition) and MAPCOMMON
+         * (key values, row counts) hold references to their items */
         td_t** items = (td_t**)td_data(v);
         for (int64_t i = 0; i < v->len; i++)
             if (items[i] && !TD_IS_ERR(items[i])) td_release(items[i]);
@@ -227,9 +229,9 @@ td_t* td_alloc_copy(td_t* v) {
         td_t** s = (td_t**)td_data(c);
         if (s[0]) { td_t* sc = td_alloc_copy(s[0]); s[0] = sc; }
         for (int64_t i = 0; i < c->len; i++) td_retain(s[1 + i]);
-    } else if (c->type == TD_LIST) {
+    } else if (c->type == TD_LIST || TD_IS_PARTED(c->type) || c->type == TD_MAPCOMMON) {
         td_t** it = (td_t**)td_data(c);
-        for (int64_t i = 0; i < c->len; i++) td_retain(it[i]);
+        for (int64_t i = 0; i < c->len; i++) if (it[i]) td_retain(it[i]);
     } else if (td_is_atom(c) && atom_owns_obj(c)) {
         td_retain(c->obj);
     } else if ((c->attrs & TD_ATTR_NULLMAP_EXT) && c->ext_nullmap) {

@@ -6,8 +6,6 @@
  */
 #include "internal.h"
 
-td_t* td_part_load(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
-td_t* td_read_parted(const char* root, const char* name) { (void)root; (void)name; return TD_ERR_PTR(TD_ERR_NYI); }
 td_err_t td_meta_save_d(td_t* schema, const char* path) { (void)schema; (void)path; return TD_ERR_NYI; }
 td_t* td_meta_load_d(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }
 td_t* td_read_csv(const char* path) { (void)path; return TD_ERR_PTR(TD_ERR_NYI); }

@@ -117,6 +117,8 @@ class Engine:
             "td_splay_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
             "td_splay_load": (P, [C.c_char_p]), "td_read_splayed": (P, [C.c_char_p, C.c_char_p]),
             "td_sym_save": (C.c_int, [C.c_char_p]), "td_sym_load": (C.c_int, [C.c_char_p]),
+            "td_part_load": (P, [C.c_char_p, C.c_char_p]),
+            "td_read_parted": (P, [C.c_char_p, C.c_char_p]),
             "td_join": (P, [P, P, P, P, P, U8, U8]), "td_optimize": (P, [P, P]),
             "td_execute": (P, [P, P]), "td_head": (P, [P, P, I64]), "td_tail": (P, [P, P, I64]),
             "td_sel_from_pred": (P, [P]), "td_sel_and": (P, [P, P]), "td_sel_new": (P, [I64]),
@@ -147,6 +149,7 @@ class Engine:
                 "td_dev_batch_free": (None, [P]),
                 "td_dev_col_load": (P, [C.c_char_p]),
                 "td_dev_splay_load": (P, [C.c_char_p, C.c_char_p]),
+                "td_dev_parted_load": (P, [C.c_char_p, C.c_char_p]),
                 "td_dev_serve_open": (P, [P]), "td_dev_serve_submit": (I64, [P, P, P]),
                 "td_dev_serve_step": (C.c_int, [P]), "td_dev_serve_poll": (C.c_int, [P, P, P]),
                 "td_dev_serve_info": (None, [P, P]), "td_dev_serve_close": (None, [P]),

@@ -52,6 +52,8 @@ CASES = {
                      lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "expr_vec_4k": (["expr", "4000", "4096", "2048", "42", "vec"],
                     lambda: W.high_card(4000, 4096, 2048, 1.1, 42)),
+    "parted_8k": (["partsave", "8000", "4096", "2048", "42", "5"],
+                  lambda: W.high_card(8000, 4096, 2048, 1.1, 42)),
     "splay_hc3k": (["save", "3000", "4096", "2048", "42"],
                    lambda: W.high_card(3000, 4096, 2048, 1.1, 42)),
     "join_inner_small": (["join", "3000", "2000", "2500", "0", "42"], None),
@@ -80,6 +82,22 @@ def main():
             outs = [f"{name}_key.tdt", f"{name}_sym.tdt"]
         elif argv[0] in ("having", "sort"):
             outs = [f"{name}.tdt", f"{name}_top.tdt"]
+        elif argv[0] == "partsave":
+            # a date-partitioned database written by the reference, and its
+            # group-bys through td_read_parted (a second, fresh process)
+            import shutil
+            db = os.path.join(HERE, name + "_db")
+            shutil.rmtree(db, ignore_errors=True)
+            subprocess.run([HARNESS] + argv + [db], check=True)
+            outs = [f"{name}_date.tdt", f"{name}_sym.tdt", f"{name}_date_sym.tdt"]
+            subprocess.run([HARNESS, "partgroup", db] + [os.path.join(HERE, o) for o in outs],
+                           check=True)
+            tables = [read_tdt(os.path.join(HERE, o)) for o in outs]
+            manifest[name] = {"argv": argv, "outputs": [name + "_db"] + outs,
+                              "rows_out": [len(t[0][3]) for t in tables],
+                              "inputs_sha256": {k: digest(v) for k, v in gen().items()}}
+            print(name, manifest[name]["rows_out"])
+            continue
         elif argv[0] == "save":
             # a splayed directory written by the reference, with its symbol
             # file: the on-disk format the loaders must read byte for byte

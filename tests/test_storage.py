@@ -99,3 +99,57 @@ def test_col_load_and_errors(tmp_path):
     np.testing.assert_array_equal(e.read_vec(w), d["v_f64"][:777])
     e.release(v)
     e.release(w)
+
+
+PDB = os.path.join(GOLD, "parted_8k_db")
+
+
+def parted_data():
+    argv = json.load(open(os.path.join(GOLD, "manifest.json")))["parted_8k"]["argv"]
+    n, np_ = int(argv[1]), int(argv[5])
+    d = W.high_card(n, int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    tri = np_ * (np_ + 1) // 2
+    sizes = [n * (p + 1) // tri for p in range(np_ - 1)]
+    sizes.append(n - sum(sizes))
+    return d, sizes
+
+
+def test_read_parted_reference_db(tmp_path):
+    """td_read_parted of the reference's partitioned database: a MAPCOMMON
+    date column (days since 2000-01-01 of each YYYY.MM.DD directory, row
+    counts) then parted columns whose segments are the partitions' rows;
+    td_part_load concatenates the same rows."""
+    out = fresh(f"""
+t = e.L.td_read_parted({PDB.encode()!r}, b"t")
+assert t and t > 32, t
+import json
+meta = []
+for i in range(e.L.td_table_ncols(t)):
+    col = e.L.td_table_get_col_idx(t, i)
+    h = e.hdr(col)
+    meta.append([e.sym_name(e.L.td_table_col_name(t, i)), h.type, h.attrs, h.val])
+    ptrs = [C.c_void_p.from_address(col + 32 + 8 * k).value for k in range(h.val)]
+    if h.type == 64:
+        np.save("kv.npy", e.read_vec(ptrs[0])); np.save("rc.npy", e.read_vec(ptrs[1]))
+    else:
+        np.save(f"c{{i}}.npy", np.concatenate([e.read_vec(p) for p in ptrs]))
+        np.save(f"n{{i}}.npy", np.array([e.hdr(p).val for p in ptrs]))
+print(json.dumps(meta))
+f = e.L.td_part_load({PDB.encode()!r}, b"t")
+assert f and f > 32, f
+for name, typ, attrs, vals in e.read_table(f):
+    np.save("flat_" + name + ".npy", np.asarray(vals))
+assert e.L.td_read_parted(b"/nonexistent", b"t") == 8            # TD_ERR_IO
+assert e.L.td_read_parted({PDB.encode()!r}, b"../t") == 8
+""", tmp_path)
+    meta = json.loads(out.strip().splitlines()[-1])
+    assert [m[0] for m in meta] == ["date", "key", "sym", "v_i64", "v_f64"]
+    assert [m[1] for m in meta] == [64, 32 + 6, 32 + 20, 32 + 6, 32 + 7]
+    assert meta[0][2] == 1                                      # TD_MC_DATE
+    d, sizes = parted_data()
+    np.testing.assert_array_equal(np.load(tmp_path / "kv.npy"), 8766 + np.arange(5))
+    np.testing.assert_array_equal(np.load(tmp_path / "rc.npy"), sizes)
+    for i, c in enumerate(["key", "sym", "v_i64", "v_f64"], start=1):
+        np.testing.assert_array_equal(np.load(tmp_path / f"c{i}.npy"), d[c], err_msg=c)
+        np.testing.assert_array_equal(np.load(tmp_path / f"n{i}.npy"), sizes)
+        np.testing.assert_array_equal(np.load(tmp_path / f"flat_{c}.npy"), d[c], err_msg=c)
