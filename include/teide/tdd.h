@@ -176,7 +176,9 @@ int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
  * use a global open-addressing table with LDS pre-aggregation, the device
  * form of its hash paths (exec.c:3868-4526, 6406-6856).  Output types and
  * AVG / VAR arithmetic follow emit_agg_columns (exec.c:4632-4773).  Groups
- * come out in ascending key order on the direct-array paths. */
+ * come out in ascending key order on the direct-array paths.  n_keys == 0:
+ * one group of every selected row, none when no row is selected (the
+ * scalar aggregate, exec.c:5766-5940). */
 int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out);
 
 /* ------------------------------------------------------------------ join -- */

@@ -362,6 +362,28 @@ static td_t* run_expr_aff(td_t* t) {
     return out;
 }
 
+/* GROUP with no keys (the scalar aggregate, exec.c:5766-5940): plain,
+ * affine and integer-linear inputs (an integral f64 constant keeps the
+ * linear sum in i64), and one that is neither. */
+static td_t* run_expr_scalar(td_t* t) {
+    td_graph_t* g = td_graph_new(t);
+    td_op_t* vi = td_scan(g, "v_i64");
+    td_op_t* vf = td_scan(g, "v_f64");
+    td_op_t* key = td_scan(g, "key");
+    td_op_t* ins[7] = {
+        vi, vi, vf,
+        td_add(g, vf, td_const_f64(g, 1.5)),
+        td_add(g, td_mul(g, vi, td_const_f64(g, 2.0)), key),
+        td_sub(g, vi, td_mul(g, td_const_i64(g, 3), key)),
+        td_mul(g, vi, td_const_f64(g, 0.5)),
+    };
+    uint16_t ops[7] = { OP_SUM, OP_COUNT, OP_AVG, OP_SUM, OP_SUM, OP_AVG, OP_SUM };
+    td_op_t* root = td_group(g, NULL, 0, ops, ins, 7);
+    td_t* out = check(td_execute(g, root), "expr scalar");
+    td_graph_free(g);
+    return out;
+}
+
 static td_op_t* expr_vec_op(td_graph_t* g, int i) {
     td_op_t* vi = td_scan(g, "v_i64");
     td_op_t* vf = td_scan(g, "v_f64");
@@ -696,7 +718,8 @@ int main(int argc, char** argv) {
         make_hc(atoll(argv[2]), atoll(argv[3]), (uint32_t)atoll(argv[4]),
                 strtoull(argv[5], NULL, 10), &t);
         td_t* o = !strcmp(argv[6], "grp") ? run_expr_grp(t)
-                : !strcmp(argv[6], "aff") ? run_expr_aff(t) : run_expr_vec(t);
+                : !strcmp(argv[6], "aff") ? run_expr_aff(t)
+                : !strcmp(argv[6], "scalar") ? run_expr_scalar(t) : run_expr_vec(t);
         write_table(argv[7], o);
         td_release(o);
         td_release(t);

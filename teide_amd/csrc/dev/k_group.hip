@@ -612,7 +612,7 @@ static int64_t masked_rows(const tdd_sel_t* s, int64_t nrows) {
 
 extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) {
     memset(out, 0, sizeof(*out));
-    if (spec->n_keys < 1 || spec->n_keys > TDD_MAX_KEYS || spec->n_aggs > TDD_MAX_AGGS ||
+    if (spec->n_keys < 0 || spec->n_keys > TDD_MAX_KEYS || spec->n_aggs > TDD_MAX_AGGS ||
         spec->n_aggs < 0)
         return TD_ERR_NYI;
     GroupPlan P;
@@ -625,6 +625,14 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
     if (rc) return rc;
     RowMask M = row_mask(spec->mask, spec->nrows);
     int64_t nsel = masked_rows(spec->mask, spec->nrows);
+    if (P.n_keys == 0) {
+        /* no keys: one group of every selected row (none when no row is
+         * selected), the reference's scalar aggregate (exec.c:5766-5940) */
+        rc = run_da(P, M, 1, out);
+        if (rc) return rc;
+        HC(hipStreamSynchronize(tdd::g_stream));
+        return TD_OK;
+    }
 
     /* Integer keys: masked ranges decide between direct array and hash. */
     bool all_int = true;

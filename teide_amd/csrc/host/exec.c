@@ -458,6 +458,7 @@ typedef struct {
     td_t*       tbl;
     tdd_expr_t* E;
     td_op_t*    pre_op[2];             /* inputs already evaluated into vectors */
+    int         int_mode;              /* constant subtrees as i64 (integer-linear sums) */
     td_t*       pre[2];
     uint8_t     reg_of[4096];          /* node id -> register, 0xFF: none yet */
 } ecomp_t;
@@ -492,10 +493,35 @@ static int ecast(ecomp_t* c, int r, int8_t t) {
  * comparisons and logic give bool; arithmetic gives f64 when an operand is
  * f64 or the op is DIV / SQRT / LOG / EXP, else i64; f64 results promote
  * their operands, and mixed comparisons compare in the wider type. */
+static int const_numeric(td_graph_t* g, td_op_t* op, double* f, int64_t* i, int* isf);
+
+/* An integral value of a constant subtree (const_expr_to_i64,
+ * exec.c:461-476). */
+static int const_i64(td_graph_t* g, td_op_t* op, int64_t* out) {
+    double f, ip;
+    int64_t i;
+    int isf;
+    if (!const_numeric(g, op, &f, &i, &isf)) return 0;
+    if (!isf) { *out = i; return 1; }
+    if (!isfinite(f) || modf(f, &ip) != 0.0 || ip > (double)INT64_MAX || ip < (double)INT64_MIN)
+        return 0;
+    *out = (int64_t)ip;
+    return 1;
+}
+
 static int ecompile(ecomp_t* c, td_op_t* op) {
     td_graph_t* g = c->x->g;
     if (op->id < 4096 && c->reg_of[op->id] != 0xFF) return c->reg_of[op->id];
     int r;
+    int64_t ci;
+    if (c->int_mode && op->opcode != OP_SCAN && const_i64(g, op, &ci)) {
+        r = ereg_new(c, TDD_R_CONST, TD_I64);
+        if (r < 0) return -EXPR_FULL;
+        c->E->regs[r].ci = ci;
+        c->E->regs[r].cf = (double)ci;
+        if (op->id < 4096) c->reg_of[op->id] = (uint8_t)r;
+        return r;
+    }
     int pi = op == c->pre_op[0] ? 0 : op == c->pre_op[1] ? 1 : -1;
     if (pi >= 0) {
         td_t* v = c->pre[pi];
@@ -572,7 +598,13 @@ static int ecompile(ecomp_t* c, td_op_t* op) {
 
 /* An element-wise expression over the rows of `tbl`, as a device vector
  * (TD_I64, TD_F64 or TD_BOOL). */
+static td_t* eval_expr_mode(xctx_t* x, td_t* tbl, td_op_t* op, int int_mode);
+
 static td_t* eval_expr(xctx_t* x, td_t* tbl, td_op_t* op) {
+    return eval_expr_mode(x, tbl, op, 0);
+}
+
+static td_t* eval_expr_mode(xctx_t* x, td_t* tbl, td_op_t* op, int int_mode) {
     if (!tbl || tbl->type != TD_TABLE) return TD_ERR_PTR(TD_ERR_SCHEMA);
     tdd_expr_t E;
     ecomp_t* c = calloc(1, sizeof(ecomp_t));
@@ -580,6 +612,7 @@ static td_t* eval_expr(xctx_t* x, td_t* tbl, td_op_t* op) {
     c->x = x;
     c->tbl = tbl;
     c->E = &E;
+    c->int_mode = int_mode;
     int r = -EXPR_FULL;
     for (int split = 0; split < 2 && r == -EXPR_FULL; split++) {
         if (split) {
@@ -589,7 +622,7 @@ static td_t* eval_expr(xctx_t* x, td_t* tbl, td_op_t* op) {
             for (int i = 0; i < 2 && i < op->arity; i++) {
                 td_op_t* in = op->inputs[i];
                 if (!in || !expr_elementwise(in->opcode)) continue;
-                td_t* v = eval_expr(x, tbl, in);
+                td_t* v = eval_expr_mode(x, tbl, in, int_mode);
                 if (TD_IS_ERR(v)) { r = -(int)TD_ERR_CODE(v); break; }
                 c->pre_op[i] = in;
                 c->pre[i] = v;
@@ -839,7 +872,7 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
     td_graph_t* g = x->g;
     td_op_ext_t* ext = td__find_ext(g, op->id);
     if (!ext) return TD_ERR_PTR(TD_ERR_NYI);
-    if (ext->n_keys < 1 || ext->n_keys > TDD_MAX_KEYS || ext->n_aggs > TDD_MAX_AGGS)
+    if (ext->n_keys < 0 || ext->n_keys > TDD_MAX_KEYS || ext->n_aggs > TDD_MAX_AGGS)
         return TD_ERR_PTR(TD_ERR_NYI);
     td_t* tbl = g->table;
     if (!tbl) return TD_ERR_PTR(TD_ERR_SCHEMA);
@@ -851,6 +884,7 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
         if (in->opcode == OP_SCAN && !scan_col(g, in, tbl)) missing = 1;
     }
     if (missing) {
+        if (ext->n_keys == 0) return TD_ERR_PTR(TD_ERR_NYI);
         for (uint32_t i = 0; i < g->node_count; i++)
             if (g->nodes[i].opcode == OP_JOIN) return exec_join_group(x, op, ext, &g->nodes[i]);
         return TD_ERR_PTR(TD_ERR_SCHEMA);
@@ -986,6 +1020,71 @@ static td_t* affine_input(td_graph_t* g, td_t* tbl, td_op_t* op, affine_t* A) {
     return col;
 }
 
+/* Integer-linear form of an aggregate input, sum of coefficient x column
+ * plus a bias over integer-class columns (parse_linear_i64_expr,
+ * exec.c:528-596): terms merged by column, zero terms dropped, at most 8. */
+typedef struct {
+    int     n;
+    int64_t sym[8], coeff[8];
+} lin_t;
+
+static int lin_add(lin_t* L, int64_t sym, int64_t k) {
+    if (k == 0) return 1;
+    for (int i = 0; i < L->n; i++) {
+        if (L->sym[i] != sym) continue;
+        int64_t nx = (int64_t)((uint64_t)L->coeff[i] + (uint64_t)k);
+        if (nx) { L->coeff[i] = nx; return 1; }
+        for (int j = i + 1; j < L->n; j++) { L->sym[j - 1] = L->sym[j]; L->coeff[j - 1] = L->coeff[j]; }
+        L->n--;
+        return 1;
+    }
+    if (L->n >= 8) return 0;
+    L->sym[L->n] = sym;
+    L->coeff[L->n++] = k;
+    return 1;
+}
+
+static int lin_parse(td_graph_t* g, td_op_t* op, int64_t scale, lin_t* L) {
+    int64_t c;
+    if (!op) return 0;
+    if (const_i64(g, op, &c)) return 1;                   /* the bias: no terms */
+    if (op->opcode == OP_SCAN) {
+        td_op_ext_t* e = td__find_ext(g, op->id);
+        return e && e->base.opcode == OP_SCAN && lin_add(L, e->sym, scale);
+    }
+    if (op->opcode == OP_NEG && op->arity == 1)
+        return lin_parse(g, op->inputs[0], (int64_t)(0 - (uint64_t)scale), L);
+    if ((op->opcode == OP_ADD || op->opcode == OP_SUB) && op->arity == 2)
+        return lin_parse(g, op->inputs[0], scale, L) &&
+               lin_parse(g, op->inputs[1], op->opcode == OP_ADD ? scale
+                                                                 : (int64_t)(0 - (uint64_t)scale), L);
+    if (op->opcode == OP_MUL && op->arity == 2) {
+        if (const_i64(g, op->inputs[0], &c))
+            return lin_parse(g, op->inputs[1], (int64_t)((uint64_t)scale * (uint64_t)c), L);
+        if (const_i64(g, op->inputs[1], &c))
+            return lin_parse(g, op->inputs[0], (int64_t)((uint64_t)scale * (uint64_t)c), L);
+    }
+    return 0;
+}
+
+/* SUM / AVG of an integer-linear input with no group keys: the reference
+ * sums it in wrapping i64 (try_linear_sumavg_input_i64, exec.c:598-612),
+ * integral f64 constants included. */
+static int linear_i64(td_graph_t* g, td_t* tbl, td_op_t* op) {
+    lin_t L;
+    L.n = 0;
+    if (op->opcode == OP_SCAN || !lin_parse(g, op, 1, &L)) return 0;
+    for (int i = 0; i < L.n; i++) {
+        td_t* col = td_table_get_col(tbl, L.sym[i]);
+        if (!col) return 0;
+        int8_t t = col->type;
+        if (!(t == TD_I64 || t == TD_TIMESTAMP || t == TD_I32 || t == TD_DATE || t == TD_TIME ||
+              t == TD_I16 || t == TD_U8 || t == TD_BOOL || t == TD_SYM))
+            return 0;
+    }
+    return 1;
+}
+
 static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, td_t** ev);
 
 /* GROUP of `tbl`'s columns; with use_sel, the graph's lazy selection (or the
@@ -1048,7 +1147,10 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
             if ((rc = dev_col(x, base, &S.aggs[a]))) return TD_ERR_PTR(rc);
             if (S.ops[a] == OP_SUM) need_count = 1;
         } else if (expr_elementwise(aop->opcode)) {
-            td_t* e = eval_expr(x, tbl, aop);
+            /* no keys and an integer-linear SUM / AVG input: i64 arithmetic */
+            int lin = ext->n_keys == 0 && (S.ops[a] == OP_SUM || S.ops[a] == OP_AVG) &&
+                      linear_i64(g, tbl, aop);
+            td_t* e = eval_expr_mode(x, tbl, aop, lin);
             if (TD_IS_ERR(e)) return e;
             ev[TDD_MAX_KEYS + a] = agg_src[a] = e;
             if ((rc = dev_col(x, e, &S.aggs[a]))) return TD_ERR_PTR(rc);

@@ -50,6 +50,8 @@ CASES = {
                      lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "expr_aff_60k": (["expr", "60000", "4096", "2048", "42", "aff"],
                      lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
+    "expr_scalar_60k": (["expr", "60000", "4096", "2048", "42", "scalar"],
+                        lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "expr_vec_4k": (["expr", "4000", "4096", "2048", "42", "vec"],
                     lambda: W.high_card(4000, 4096, 2048, 1.1, 42)),
     "parted_8k": (["partsave", "8000", "4096", "2048", "42", "5"],

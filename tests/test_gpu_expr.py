@@ -389,3 +389,34 @@ def test_affine_aggregates_golden(eng, resident, optimize):
                    [ref[0][3].astype(np.int64)], [c[3] for c in ref[1:]],
                    [True, True, False, True, True, True, False], "affine")
     eng.release(t)
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_scalar_aggregates_golden(eng, resident):
+    """GROUP with no keys: one row, plain / affine / integer-linear inputs
+    typed and summed as the reference does (tests/golden/expr_scalar_60k),
+    and no row for an empty table."""
+    d = hc("expr_scalar_60k")
+    t = hc_table(eng, d, resident)
+    g = eng.graph(t)
+    vi, vf, key = g.scan("v_i64"), g.scan("v_f64"), g.scan("key")
+    aggs = [("SUM", vi), ("COUNT", vi), ("AVG", vf), ("SUM", g.add(vf, g.const_f64(1.5))),
+            ("SUM", g.add(g.mul(vi, g.const_f64(2.0)), key)),
+            ("AVG", g.sub(vi, g.mul(g.const_i64(3), key))), ("SUM", g.mul(vi, g.const_f64(0.5)))]
+    out = eng.read_table(g.execute(g.group([], aggs)))
+    g.free()
+    ref = read_tdt(os.path.join(GOLD, "expr_scalar_60k.tdt"))
+    assert [c[0] for c in out] == [c[0] for c in ref]
+    assert [c[1] for c in out] == [c[1] for c in ref]
+    for i, (a, b) in enumerate(zip(out, ref)):
+        if a[1] == 7:
+            np.testing.assert_allclose(a[3], b[3], rtol=1e-12, err_msg=a[0])
+        else:
+            np.testing.assert_array_equal(a[3], b[3], err_msg=a[0])
+    eng.release(t)
+    e0 = eng.table([("a", np.zeros(0, np.int64))])
+    g = eng.graph(e0)
+    out = eng.read_table(g.execute(g.group([], [("SUM", g.scan("a")), ("COUNT", g.scan("a"))])))
+    g.free()
+    eng.release(e0)
+    assert [len(c[3]) for c in out] == [0, 0]

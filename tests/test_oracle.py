@@ -475,3 +475,24 @@ def test_expr_affine_matches_reference():
     compare_groups(keys, vals, [ref[0][3]], [c[3] for c in ref[1:]],
                    [True, True, False, True, True, True, False], "affine",
                    exact=comp, sum_abs=sabs)
+
+
+def test_scalar_aggregates_match_reference():
+    # GROUP with no keys (exec.c:5766-5940): one row; affine and
+    # integer-linear inputs summed the reference's way (the linear sum in
+    # wrapping i64, an integral f64 constant included)
+    argv = MANIFEST["expr_scalar_60k"]["argv"]
+    d = W.high_card(int(argv[1]), int(argv[2]), int(argv[3]), 1.1, int(argv[4]))
+    ref = gold("expr_scalar_60k.tdt")
+    assert [c[1] for c in ref] == [6, 6, 7, 7, 6, 7, 7]
+    vi, vf, key = d["v_i64"], d["v_f64"], d["key"]
+    n = len(vi)
+    wsum = lambda a: int(a.astype(np.uint64).sum(dtype=np.uint64).astype(np.int64))
+    wrap = lambda v: int(np.array(v % (1 << 64), np.uint64).astype(np.int64))
+    assert ref[0][3][0] == wsum(vi)
+    assert ref[1][3][0] == n
+    assert_f64_close(ref[2][3], np.array([vf.sum() / n]))
+    assert_f64_close(ref[3][3], np.array([vf.sum() + 1.5 * n]))
+    assert ref[4][3][0] == wrap(2 * wsum(vi) + wsum(key))
+    assert_f64_close(ref[5][3], np.array([wrap(wsum(vi) - 3 * wsum(key)) / n]))
+    assert_f64_close(ref[6][3], np.array([(vi * 0.5).sum()]))
