@@ -96,10 +96,23 @@ __global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJo
         for (int s = threadIdx.x; s < PL; s += blockDim.x) h[s] = 0;
         __syncthreads();
         uint64_t v[PG_MAXV];
-        for (int64_t i = J.r0 + threadIdx.x; i < J.r1; i += blockDim.x) {
-            uint64_t key;
-            if (pg_item<ROWS, HK, 0>(S, in, i, J.r1, key, v))
-                atomicAdd(&h[(pg_bits<HK>(key) >> shift) & (PL - 1)], 1u);
+        const int lane = lane_id();
+        /* every lane of a wave runs the same trip count (i advances by the
+         * block), so the wave can count a run of one sub-partition at once */
+        for (int64_t i0 = J.r0; i0 < J.r1; i0 += blockDim.x) {
+            const int64_t i = i0 + threadIdx.x;
+            uint64_t key = 0;
+            const bool live = pg_item<ROWS, HK, 0>(S, in, i, J.r1, key, v);
+            const int s = live ? (int)((pg_bits<HK>(key) >> shift) & (PL - 1)) : -1;
+            const uint64_t lm = __ballot(live);
+            if (!lm) continue;
+            const int first = __ffsll((unsigned long long)lm) - 1;
+            const int s0 = __shfl(s, first);
+            if (__ballot(live && s != s0) == 0) {
+                if (lane == first) atomicAdd(&h[s0], (uint32_t)__popcll(lm));
+            } else if (live) {
+                atomicAdd(&h[s], 1u);
+            }
         }
         __syncthreads();
         for (int s = threadIdx.x; s < PL; s += blockDim.x)
@@ -136,8 +149,21 @@ __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out,
              * LDS counters (its LDS operations complete in program order) */
             for (int s2 = lane; s2 < PL; s2 += WAVE) wcnt[wv][s2] = 0;   /* PL may exceed 64 */
 #pragma unroll
-            for (int k = 0; k < PG_R; k++)
-                rk[k] = sub[k] >= 0 ? (int32_t)atomicAdd(&wcnt[wv][sub[k]], 1u) : 0;
+            for (int k = 0; k < PG_R; k++) {
+                /* a wave whose items all go to one sub-partition takes its
+                 * ranks with one counter update */
+                const uint64_t lm = __ballot(sub[k] >= 0);
+                const int first = lm ? __ffsll((unsigned long long)lm) - 1 : 0;
+                const int s0 = __shfl(sub[k], first);
+                if (lm && __ballot(sub[k] >= 0 && sub[k] != s0) == 0) {
+                    uint32_t base = 0;
+                    if (lane == first) base = atomicAdd(&wcnt[wv][s0], (uint32_t)__popcll(lm));
+                    base = __shfl(base, first);
+                    rk[k] = sub[k] >= 0 ? (int32_t)(base + lanes_below(lm)) : 0;
+                } else {
+                    rk[k] = sub[k] >= 0 ? (int32_t)atomicAdd(&wcnt[wv][sub[k]], 1u) : 0;
+                }
+            }
             __syncthreads();
             /* wave offsets within the round, then the round's totals */
             if (threadIdx.x < PL) {
