@@ -232,6 +232,28 @@ static void make_hc(int64_t n, int64_t nv, uint32_t nsym, uint64_t seed, td_t** 
     *out = t;
 }
 
+/* GROUP(cat) of expression inputs over the join output: products of fact
+ * columns, an affine AVG, a fact x dim product and a MAX of an integer
+ * expression (the group runs on the joined table, exec.c:5700-5712). */
+static td_t* run_jg_expr_group(td_t* joined) {
+    td_graph_t* g = td_graph_new(joined);
+    td_op_t* price = td_scan(g, "price");
+    td_op_t* qty = td_scan(g, "qty");
+    td_op_t* keys[1] = { td_scan(g, "cat") };
+    td_op_t* ins[5] = {
+        td_mul(g, price, td_cast(g, qty, TD_F64)),
+        td_mul(g, price, td_const_f64(g, 0.9)),
+        td_add(g, qty, td_const_i64(g, 1)),
+        td_mul(g, price, td_cast(g, td_scan(g, "cat"), TD_F64)),
+        td_sub(g, td_mul(g, qty, td_const_i64(g, 2)), td_const_i64(g, 1)),
+    };
+    uint16_t ops[5] = { OP_SUM, OP_SUM, OP_AVG, OP_SUM, OP_MAX };
+    td_op_t* root = td_group(g, keys, 1, ops, ins, 5);
+    td_t* out = check(td_execute(g, root), "jg expr group");
+    td_graph_free(g);
+    return out;
+}
+
 /* HAVING on the group output (exec.c:11674-11704): GROUP(sym; SUM v_f64,
  * COUNT v_i64, MAX v_i64) filtered to v_i64_count >= MINC and
  * v_f64_sum < MAXS; with TOP > 0, HEAD(SORT(that, v_f64_sum desc), TOP). */
@@ -675,6 +697,17 @@ int main(int argc, char** argv) {
         td_t* j = run_jg_join(fact, dim);
         if (argc == 8) write_table(argv[7], j);
         td_t* o = run_jg_group(j);
+        write_table(argv[6], o);
+        td_release(o);
+        td_release(j);
+        td_release(fact);
+        td_release(dim);
+    } else if (!strcmp(cmd, "jgexpr") && argc == 7) {
+        td_t *fact, *dim;
+        make_jg(atoll(argv[2]), atoll(argv[3]), atoll(argv[4]), strtoull(argv[5], NULL, 10),
+                &fact, &dim);
+        td_t* j = run_jg_join(fact, dim);
+        td_t* o = run_jg_expr_group(j);
         write_table(argv[6], o);
         td_release(o);
         td_release(j);

@@ -868,6 +868,17 @@ static td_op_t* strip_filter(td_op_t* op) {
 static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* join);
 static td_t* group_over(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel);
 
+/* A SCAN at or under `op` (through element-wise nodes) names a column
+ * `tbl` lacks. */
+static int scan_missing(td_graph_t* g, td_op_t* op, td_t* tbl) {
+    if (!op) return 0;
+    if (op->opcode == OP_SCAN) return !scan_col(g, op, tbl);
+    if (!expr_elementwise(op->opcode)) return 0;
+    for (int i = 0; i < op->arity && i < 2; i++)
+        if (scan_missing(g, op->inputs[i], tbl)) return 1;
+    return 0;
+}
+
 static td_t* exec_group(xctx_t* x, td_op_t* op) {
     td_graph_t* g = x->g;
     td_op_ext_t* ext = td__find_ext(g, op->id);
@@ -881,7 +892,7 @@ static td_t* exec_group(xctx_t* x, td_op_t* op) {
     int missing = 0;
     for (int i = 0; i < ext->n_keys + ext->n_aggs; i++) {
         td_op_t* in = strip_filter(i < ext->n_keys ? ext->keys[i] : ext->agg_ins[i - ext->n_keys]);
-        if (in->opcode == OP_SCAN && !scan_col(g, in, tbl)) missing = 1;
+        missing |= scan_missing(g, in, tbl);
     }
     if (missing) {
         if (ext->n_keys == 0) return TD_ERR_PTR(TD_ERR_NYI);
@@ -1340,6 +1351,48 @@ static td_t* join_then_group(xctx_t* x, td_op_ext_t* ext, td_op_t* jop) {
     return r;
 }
 
+/* The side (0 left, 1 right) whose columns every SCAN under `op` reads,
+ * -1 when they span both, name a right key, or are missing. */
+static int expr_side(td_graph_t* g, td_op_ext_t* je, td_op_t* op, td_t* left, td_t* right) {
+    if (!op) return -1;
+    if (op->opcode == OP_SCAN) {
+        td_op_ext_t* se = td__find_ext(g, op->id);
+        if (!se) return -1;
+        if (td_table_get_col(left, se->sym)) return 0;
+        if (!join_is_right_key(g, je, se->sym) && td_table_get_col(right, se->sym)) return 1;
+        return -1;
+    }
+    if (op->opcode == OP_CONST) return 2;               /* either side */
+    int s = 2;
+    for (int i = 0; i < op->arity && i < 2; i++) {
+        int t = expr_side(g, je, op->inputs[i], left, right);
+        if (t < 0) return -1;
+        if (t == 2) continue;
+        if (s != 2 && s != t) return -1;
+        s = t;
+    }
+    return s == 2 ? 0 : s;
+}
+
+/* Inputs of the fused join -> group-by: SCANs and expressions over one
+ * side, and no input the reference aggregates through a rewrite (affine
+ * SUM / AVG, exec.c:614-694), which the join-then-group form applies. */
+static int join_group_inputs_ok(td_graph_t* g, td_op_ext_t* ext, td_op_ext_t* je, td_t* left,
+                                td_t* right) {
+    for (int a = 0; a < ext->n_aggs; a++) {
+        td_op_t* in = ext->agg_ins[a];
+        if (in->opcode == OP_SCAN || in->opcode == OP_CONST) continue;
+        if (!expr_elementwise(in->opcode) || expr_side(g, je, in, left, right) < 0) return 0;
+        uint16_t o = ext->agg_ops[a];
+        affine_t A;
+        td_t* side = expr_side(g, je, in, left, right) ? right : left;
+        if ((o == OP_SUM || o == OP_AVG) && affine_input(g, side, in, &A)) return 0;
+    }
+    for (int k = 0; k < ext->n_keys; k++)
+        if (ext->keys[k]->opcode != OP_SCAN) return 0;
+    return 1;
+}
+
 static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* jop) {
     td_graph_t* g = x->g;
     td_op_ext_t* je = td__find_ext(g, jop->id);
@@ -1351,7 +1404,8 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
     td_t* right = const_table_under(g, jop->inputs[1], rpreds, &nrp);
     (void)op;
     if (je->join.join_type != 0 || !left || !right || nrp ||
-        td_table_nrows(left) > INT32_MAX || td_table_nrows(right) > INT32_MAX)
+        td_table_nrows(left) > INT32_MAX || td_table_nrows(right) > INT32_MAX ||
+        !join_group_inputs_ok(g, ext, je, left, right))
         return join_then_group(x, ext, jop);
     int64_t nl = td_table_nrows(left), nr = td_table_nrows(right);
     tdd_join_group_spec_t S;
@@ -1386,11 +1440,14 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
         S.join.lsel = &sv;
     }
     S.lpred = (npred == 1 && !sel) ? &P : NULL;
-    /* Group columns: left table first, then right non-key columns. */
+    /* Group columns: left table first, then right non-key columns.  An
+     * aggregate input that is an expression over one side's columns is
+     * evaluated over that side first and aggregated as a column of it. */
     S.n_keys = ext->n_keys;
     S.n_aggs = ext->n_aggs;
     td_t* key_src[TDD_MAX_KEYS];
     td_t* agg_src[TDD_MAX_AGGS];
+    td_t* ev[TDD_MAX_AGGS] = {0};
     for (int i = 0; i < ext->n_keys + ext->n_aggs && !rc; i++) {
         int is_key = i < ext->n_keys;
         td_op_t* in = is_key ? ext->keys[i] : ext->agg_ins[i - ext->n_keys];
@@ -1401,6 +1458,11 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
             col = td_table_get_col(left, se->sym);
             if (!col && !join_is_right_key(g, je, se->sym)) { col = td_table_get_col(right, se->sym); side = 1; }
             if (!col) rc = TD_ERR_SCHEMA;
+        } else if (!is_key && expr_elementwise(in->opcode) &&
+                   (side = expr_side(g, je, in, left, right)) >= 0) {
+            td_t* e = eval_expr(x, side ? right : left, in);
+            if (TD_IS_ERR(e)) rc = TD_ERR_CODE(e);
+            else col = ev[i - ext->n_keys] = e;
         } else if (is_key || !(in->opcode == OP_CONST && ext->agg_ops[i - ext->n_keys] == OP_COUNT)) {
             rc = TD_ERR_NYI;
         }
@@ -1417,10 +1479,11 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
             if (col) rc = dev_col(x, col, &S.aggs[a]);
         }
     }
-    if (rc) { td_release(sel); return TD_ERR_PTR(rc); }
     tdd_group_result_t R;
-    rc = tdd_join_group(&S, &R);
+    if (!rc) rc = tdd_join_group(&S, &R);
     td_release(sel);
+    for (int a = 0; a < TDD_MAX_AGGS; a++)
+        if (ev[a]) td_release(ev[a]);          /* the launches are ordered before any reuse */
     if (rc) return TD_ERR_PTR(rc);
     td__set_plan("join_group path=%d groups=%lld fused_pred=%d", R.path, (long long)R.n_groups,
                  S.lpred != NULL);

@@ -40,6 +40,8 @@ CASES = {
     "jg_100k_8k_c1m": (["jg", "100000", "8000", "1000000", "5"],
                        lambda: {**W.join_fact(100000, 8000, 5),
                                 **W.join_dim(8000, 1000000, 5)}),
+    "jg_expr_400k": (["jgexpr", "400000", "50000", "300", "9"],
+                     lambda: {**W.join_fact(400000, 50000, 9), **W.join_dim(50000, 300, 9)}),
     "hc_60k": (["hc", "60000", "4096", "2048", "42"],
                lambda: W.high_card(60000, 4096, 2048, 1.1, 42)),
     "having_60k": (["having", "60000", "4096", "2048", "42", "29", "0.0", "10"],

@@ -420,3 +420,61 @@ def test_scalar_aggregates_golden(eng, resident):
     g.free()
     eng.release(e0)
     assert [len(c[3]) for c in out] == [0, 0]
+
+
+def _jg_expr_aggs(g, full=True):
+    price, qty, cat = g.scan("price"), g.scan("qty"), g.scan("cat")
+    aggs = [("SUM", g.mul(price, g.cast(qty, X.TD_F64))),
+            ("SUM", g.mul(price, g.const_f64(0.9)))]
+    if full:
+        aggs += [("AVG", g.add(qty, g.const_i64(1))),
+                 ("SUM", g.mul(price, g.cast(cat, X.TD_F64)))]
+    aggs.append(("MAX", g.sub(g.mul(qty, g.const_i64(2)), g.const_i64(1))))
+    return aggs
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_join_group_expression_inputs(eng, resident):
+    """GROUP over a JOIN with expression inputs, against the reference's
+    group of its join output (tests/golden/jg_expr_400k): in one DAG --
+    fact-side expressions evaluated over the fact table and fused into the
+    join -> group-by; an affine or a fact x dim input makes the whole group
+    run on the materialised join -- and in the reference's two-call form."""
+    import workload_dags as D
+    argv = MANIFEST["jg_expr_400k"]["argv"]
+    tf, td, _, _ = D.jg_tables(eng, int(argv[1]), int(argv[2]), int(argv[3]), int(argv[4]))
+    if resident:
+        rf, rd = eng.put(tf), eng.put(td)
+        eng.release(tf)
+        eng.release(td)
+        tf, td = rf, rd
+    ref = read_tdt(os.path.join(GOLD, "jg_expr_400k.tdt"))
+    rk = [ref[0][3].astype(np.int64)]
+    f64 = [True, True, True, True, False]
+
+    def one_dag(full):
+        g = eng.graph(tf)
+        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(50)))
+        g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+        out = eng.read_table(g.execute(g.group([g.scan("cat")], _jg_expr_aggs(g, full))))
+        g.free()
+        return out, eng.last_plan()
+
+    out, plan = one_dag(True)
+    assert [c[0] for c in out] == [c[0] for c in ref]
+    assert [c[1] for c in out] == [c[1] for c in ref]
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], rk,
+                   [c[3] for c in ref[1:]], f64, "join-group expr (materialised)")
+    out, plan = one_dag(False)
+    assert plan.startswith("join_group"), plan                 # fused
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], rk,
+                   [ref[1][3], ref[2][3], ref[5][3]], [True, True, False], "join-group expr (fused)")
+    j = D.jg_join(eng, tf, td)
+    g = eng.graph(j)
+    out = eng.read_table(g.execute(g.group([g.scan("cat")], _jg_expr_aggs(g, True))))
+    g.free()
+    eng.release(j)
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], rk,
+                   [c[3] for c in ref[1:]], f64, "join then group expr")
+    eng.release(tf)
+    eng.release(td)
