@@ -1424,7 +1424,18 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
     if (npred) {
         if (td_table_nrows(g->table) != nl) { td_release(sel); return TD_ERR_PTR(TD_ERR_LENGTH); }
         for (int i = 0; i < npred; i++) {
-            if ((rc = compile_pred(x, g->table, preds[i], &P))) { td_release(sel); return TD_ERR_PTR(rc); }
+            rc = compile_pred(x, g->table, preds[i], &P);
+            if (rc == TD_ERR_NYI) {
+                /* arithmetic in the predicate: an expression selection */
+                td_t* s2 = pred_expr_sel(x, g->table, preds[i], sel);
+                td_release(sel);
+                if (TD_IS_ERR(s2)) return s2;
+                sel = s2;
+                rc = TD_OK;
+                if (i == npred - 1) npred = 0;               /* nothing left to fuse */
+                continue;
+            }
+            if (rc) { td_release(sel); return TD_ERR_PTR(rc); }
             if (npred == 1 && !sel) break;                  /* fused into the probe */
             td_t* s2 = eval_sel(x, &P, nl, sel);
             td_release(sel);
@@ -1724,8 +1735,8 @@ static td_t* exec_having(xctx_t* x, td_op_t* op) {
     if (gr->type != TD_TABLE) { td_release(gr); return TD_ERR_PTR(TD_ERR_TYPE); }
     tdd_pred_t P;
     int rc = compile_pred(x, gr, op->inputs[1], &P);
-    if (rc) { td_release(gr); return TD_ERR_PTR(rc); }
-    td_t* s = eval_sel(x, &P, td_table_nrows(gr), NULL);
+    td_t* s = rc == TD_ERR_NYI ? pred_expr_sel(x, gr, op->inputs[1], NULL)   /* arithmetic */
+            : rc ? TD_ERR_PTR(rc) : eval_sel(x, &P, td_table_nrows(gr), NULL);
     if (TD_IS_ERR(s)) { td_release(gr); return s; }
     td_t* out = compact_table(x, gr, s);
     td_release(s);

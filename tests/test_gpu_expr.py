@@ -478,3 +478,30 @@ def test_join_group_expression_inputs(eng, resident):
                    [c[3] for c in ref[1:]], f64, "join then group expr")
     eng.release(tf)
     eng.release(td)
+
+
+def test_arith_predicates_before_join_and_in_having(eng):
+    """An arithmetic FILTER under a fused join -> group-by, and a HAVING
+    with arithmetic over the group's columns, against the oracle."""
+    import workload_dags as D
+    tf, td, fact, dim = D.jg_tables(eng, 300000, 40000, 200, 13)
+    g = eng.graph(tf)
+    pe = ("GT", ("MUL", ("col", "qty"), ("col", "price")), ("f64", 2000.0))
+    flt = g.filter(g.const_table(tf), X.build(g, pe))
+    g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
+    grp = g.group([g.scan("cat")], [("SUM", g.scan("price")), ("COUNT", g.scan("qty"))])
+    having = g.filter(grp, g.gt(g.div(g.scan("price_sum"), g.scan("qty_count")),
+                                g.const_f64(40.0)))
+    out = eng.read_table(g.execute(having))
+    g.free()
+    _, keep = orc.expr_eval(X.bind(pe, fact))
+    cat_of = np.zeros(int(dim["pk"].max()) + 1, np.int64)
+    cat_of[dim["pk"]] = dim["cat"]
+    ok = keep & np.isin(fact["fk"], dim["pk"])
+    r = orc.group([cat_of[fact["fk"][ok]]], ["SUM", "COUNT"], [fact["price"][ok], fact["qty"][ok]])
+    pass_ = r["vals"][0] / r["vals"][1] > 40.0
+    compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]],
+                   [r["keys"][0][pass_]], [v[pass_] for v in r["vals"]], [True, False],
+                   "arith filter -> join -> group -> arith having")
+    eng.release(tf)
+    eng.release(td)
