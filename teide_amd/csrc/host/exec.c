@@ -1644,10 +1644,17 @@ static td_t* exec_sort(xctx_t* x, td_op_t* op, int64_t limit) {
     void* ranks[16] = {0};
     memset(K, 0, sizeof(K));
     int rc = TD_OK;
+    td_t* kev[16] = {0};                         /* expression keys, evaluated over the input */
     for (int k = 0; k < nk && !rc; k++) {
         td_op_t* kop = ext->sort.columns[k];
         td_op_ext_t* ke = td__find_ext(g, kop->id);
         td_t* col = (ke && ke->base.opcode == OP_SCAN) ? td_table_get_col(tbl, ke->sym) : NULL;
+        if (!col && expr_elementwise(kop->opcode)) {
+            /* exec.c:3395-3403: the key evaluated with the sort input as table */
+            td_t* e = eval_expr(x, tbl, kop);
+            if (TD_IS_ERR(e)) { rc = TD_ERR_CODE(e); break; }
+            col = kev[k] = e;
+        }
         if (!col) { rc = kop->opcode == OP_SCAN ? TD_ERR_SCHEMA : TD_ERR_NYI; break; }
         rc = dev_col(x, col, &K[k].col);
         K[k].desc = ext->sort.desc ? ext->sort.desc[k] : 0;
@@ -1683,7 +1690,10 @@ static td_t* exec_sort(xctx_t* x, td_op_t* op, int64_t limit) {
         }
     }
     if (perm) { tdd_sync(); td__arena_free(perm); }
-    for (int k = 0; k < nk; k++) if (ranks[k]) td__arena_free(ranks[k]);
+    for (int k = 0; k < nk; k++) {
+        if (ranks[k]) td__arena_free(ranks[k]);
+        if (kev[k]) td_release(kev[k]);
+    }
     td_release(tbl);
     td__set_plan("sort keys=%d rows=%lld", nk, (long long)n);
     return out;

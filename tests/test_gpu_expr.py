@@ -505,3 +505,37 @@ def test_arith_predicates_before_join_and_in_having(eng):
                    "arith filter -> join -> group -> arith having")
     eng.release(tf)
     eng.release(td)
+
+
+def test_sort_by_expression(eng):
+    """SORT keys that are expressions, evaluated over the sort's input (a
+    table, and a group result), against the oracle's sort of the evaluated
+    key."""
+    rng = np.random.default_rng(21)
+    n = 30011
+    a = rng.integers(-1000, 1000, n)
+    f = np.round(rng.normal(size=n), 3)
+    t = eng.table([("a", a), ("f", f)])
+    g = eng.graph(t)
+    ke = ("ADD", ("MUL", ("col", "f"), ("f64", 3.0)), ("MOD", ("col", "a"), ("i64", 7)))
+    root = g.sort(g.const_table(t), [X.build(g, ke), g.scan("a")], [True, False])
+    out = eng.read_table(g.execute(root))
+    g.free()
+    _, kv = orc.expr_eval(X.bind(ke, {"a": a, "f": f}))
+    p = orc.sort_perm([kv, a], [True, False])
+    np.testing.assert_array_equal(out[0][3], a[p])
+    np.testing.assert_array_equal(out[1][3], f[p])
+    # ORDER BY an expression of a group result's columns, top 5
+    k = a % 50
+    t2 = eng.table([("k", k), ("a", a), ("f", f)])
+    g = eng.graph(t2)
+    grp = g.group([g.scan("k")], [("SUM", g.scan("f")), ("COUNT", g.scan("a"))])
+    key = g.div(g.scan("f_sum"), g.scan("a_count"))
+    out = eng.read_table(g.execute(g.head(g.sort(grp, [key], [True]), 5)))
+    g.free()
+    r = orc.group([k], ["SUM", "COUNT"], [f, a])
+    mean = r["vals"][0] / r["vals"][1]
+    top = orc.sort_perm([mean], [True])[:5]
+    np.testing.assert_array_equal(out[0][3], r["keys"][0][top])
+    eng.release(t)
+    eng.release(t2)
