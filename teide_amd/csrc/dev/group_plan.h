@@ -34,8 +34,12 @@ struct GroupPlan {
     uint16_t  ops[TDD_MAX_AGGS];
     int8_t    out_types[TDD_MAX_AGGS];
     uint8_t   cell_of[TDD_MAX_AGGS];   /* first cell of each aggregate */
-    uint8_t   kind[GP_MAX_CELLS];
-    uint8_t   col[GP_MAX_CELLS];       /* aggregate column feeding the cell */
+    /* per-cell facts read with a run-time cell index: dwords, so the reads
+     * are scalar loads (gfx950 scalar loads have no byte form; a byte array
+     * is read with vector loads, whose waits also wait for the row loads
+     * in flight) */
+    int32_t   kind[GP_MAX_CELLS];
+    int32_t   col[GP_MAX_CELLS];       /* aggregate column feeding the cell */
     uint8_t   key_side[TDD_MAX_KEYS];  /* fused join: 0 probe row, 1 build row */
     uint8_t   agg_side[TDD_MAX_AGGS];
 };
