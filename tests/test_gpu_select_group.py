@@ -182,6 +182,8 @@ KEYSETS = {
     "f64": lambda r, n: [np.round(r.normal(size=n), 1)],                 # hash on raw bits
     "two_keys": lambda r, n: [r.integers(0, 30, n), r.integers(-4, 4, n).astype(np.int16)],
     "sym_w32": lambda r, n: [r.integers(0, 5000, n).astype(np.uint32)],
+    # runs of one key (sorted or partitioned data): whole waves fold into one slot
+    "i64_sorted": lambda r, n: [np.sort(r.integers(-20, 40, n))],
 }
 
 
