@@ -275,6 +275,54 @@ __device__ __forceinline__ void pg_fold(uint64_t* c, const GroupPlan& P, const i
 /* Leaf fold: job j covers records of one leaf partition (slot range
  * [parent << lbits, +2^lbits)), folded into an LDS table, then merged into
  * the global direct-array table. */
+/* Cell k of U records (bits of `live`) at LDS table rows loc[u], the input
+ * from record word Q: one dispatch on the cell's kind for all U records. */
+template <int U, int Q>
+__device__ __forceinline__ void pg_cell_rows(uint64_t* lt, int nc, int k, int kind, bool f,
+                                             const uint32_t (&loc)[U], uint32_t live,
+                                             const uint64_t (&v)[U][PG_MAXV]) {
+#define PG_ROWS(BODY)                                                      \
+    _Pragma("unroll") for (int u = 0; u < U; u++) {                        \
+        if (!((live >> u) & 1)) continue;                                  \
+        uint64_t* c = &lt[(int64_t)loc[u] * nc + k];                       \
+        const uint64_t x = v[u][Q];                                        \
+        const double d = f ? __longlong_as_double((long long)x) : (double)(int64_t)x; \
+        (void)d;                                                           \
+        BODY;                                                              \
+    }
+    switch (kind) {
+    case C_ADD_F64: PG_ROWS(atomicAdd((double*)c, d)) break;
+    case C_ADD_I64: PG_ROWS(atomicAdd((unsigned long long*)c, (unsigned long long)x)) break;
+    case C_MIN_I64: case C_MIN_ROW: PG_ROWS(cell_min(c, (int64_t)x)) break;
+    case C_MAX_I64: case C_MAX_ROW: PG_ROWS(cell_max(c, (int64_t)x)) break;
+    case C_MIN_F64: PG_ROWS(cell_min(c, f64_ord(d))) break;
+    case C_MAX_F64: PG_ROWS(cell_max(c, f64_ord(d))) break;
+    default: PG_ROWS(atomicAdd((double*)c, d * d)) break;
+    }
+#undef PG_ROWS
+}
+
+/* U records into the LDS table: COUNT, then cell by cell. */
+template <int U, int NV>
+__device__ __forceinline__ void pg_fold_rows(uint64_t* lt, const GroupPlan& P, const int8_t* cw,
+                                             const uint8_t* vf, const uint32_t (&loc)[U],
+                                             uint32_t live, const uint64_t (&v)[U][PG_MAXV]) {
+    const int nc = P.ncells;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if ((live >> u) & 1) atomicAdd((unsigned long long*)&lt[(int64_t)loc[u] * nc], 1ull);
+    for (int k = 1; k < nc; k++) {
+        const int q = cw[k];
+        const bool f = vf[q];
+        switch (q) {                              /* compile-time record word */
+        case 0: pg_cell_rows<U, 0>(lt, nc, k, P.kind[k], f, loc, live, v); break;
+        case 1: pg_cell_rows<U, (NV > 1 ? 1 : 0)>(lt, nc, k, P.kind[k], f, loc, live, v); break;
+        case 2: pg_cell_rows<U, (NV > 2 ? 2 : 0)>(lt, nc, k, P.kind[k], f, loc, live, v); break;
+        default: pg_cell_rows<U, (NV > 3 ? 3 : 0)>(lt, nc, k, P.kind[k], f, loc, live, v); break;
+        }
+    }
+}
+
 template <int NV>
 __global__ __launch_bounds__(PG_T) void k_pg_agg(GroupPlan P, PGBuf in, const PGJob* jobs,
                                                  int njobs, int lbits, const int8_t* cw_dev,
@@ -303,13 +351,15 @@ __global__ __launch_bounds__(PG_T) void k_pg_agg(GroupPlan P, PGBuf in, const PG
 #pragma unroll
                 for (int q = 0; q < NV; q++) v[u][q] = i < J.r1 ? in.val[q][i] : 0ull;
             }
+            uint32_t loc[PG_U];
+            uint32_t live = 0;
 #pragma unroll
             for (int u = 0; u < PG_U; u++) {
-                if (i0 + (int64_t)u * blockDim.x >= J.r1) break;
-                const uint32_t local = sl[u] - base;
-                if (local >= (uint32_t)nl) continue;     /* never taken for a correct split */
-                pg_fold<NV, true>(&lt[(int64_t)local * nc], P, cw, vf, v[u]);
+                loc[u] = sl[u] - base;
+                if (i0 + (int64_t)u * blockDim.x < J.r1 && loc[u] < (uint32_t)nl)  /* always, */
+                    live |= 1u << u;                                  /* for a correct split */
             }
+            pg_fold_rows<PG_U, NV>(lt, P, cw, vf, loc, live, v);
         }
         __syncthreads();
         for (int64_t s = threadIdx.x; s < nl; s += blockDim.x) {
