@@ -12,6 +12,8 @@
  * (exec.c:960-1100) operation by operation; HIP code here is built without
  * FP contraction, so a * b + c rounds twice as it does there.
  */
+#include <type_traits>
+
 #include "dev_common.h"
 
 #define EX_T    256
@@ -429,21 +431,68 @@ extern "C" int tdd_affine_fix(void* vals, int8_t type, int avg, const int64_t* c
 /* out[r] = kv[p] for the partition p whose rows [offs[p], offs[p+1])
  * hold r: the per-row key of a MAPCOMMON column (materialize_mapcommon,
  * exec.c:201-241), esz 1, 2, 4 or 8. */
-__global__ void k_mapcommon(void* out, int esz, const void* kv, const int64_t* offs, int n_parts,
-                            int64_t n) {
+#define MC_LDS 1024                 /* partitions whose offsets and keys go to LDS */
+
+/* Key of element r, advancing the partition index p (offs ascending,
+ * offs[0] = 0). */
+__device__ __forceinline__ int mc_part(const int64_t* offs, int n_parts, int p, int64_t r) {
+    while (p + 1 < n_parts && offs[p + 1] <= r) p++;
+    return p;
+}
+
+/* Each thread fills 16-byte chunks: one binary search for the chunk's first
+ * element, then a linear advance; 4- and 8-byte keys leave as one vector
+ * store per chunk.  Offsets and keys are read from LDS when they fit. */
+template <int ESZ, bool LDS>
+__global__ __launch_bounds__(256) void k_mapcommon(void* out, const void* kv, const int64_t* goffs,
+                                                   int n_parts, int64_t n, bool vec) {
+    typedef typename std::conditional<ESZ == 8, uint64_t,
+            typename std::conditional<ESZ == 4, uint32_t,
+            typename std::conditional<ESZ == 2, uint16_t, uint8_t>::type>::type>::type E;
+    constexpr int PER = 16 / ESZ;
+    __shared__ int64_t so[LDS ? MC_LDS : 1];
+    __shared__ E sk[LDS ? MC_LDS : 1];
+    const int64_t* offs = goffs;
+    const E* keys = (const E*)kv;
+    if (LDS) {
+        for (int i = threadIdx.x; i < n_parts; i += blockDim.x) {
+            so[i] = goffs[i];
+            sk[i] = ((const E*)kv)[i];
+        }
+        __syncthreads();
+        offs = so;
+        keys = sk;
+    }
+    E* o = (E*)out;
+    const int64_t n_chunks = (n + PER - 1) / PER;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-        int lo = 0, hi = n_parts - 1;                 /* last p with offs[p] <= r */
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks; c += stride) {
+        const int64_t r0 = c * PER;
+        int lo = 0, hi = n_parts - 1;                 /* last p with offs[p] <= r0 */
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (offs[mid] <= r) lo = mid;
+            if (offs[mid] <= r0) lo = mid;
             else hi = mid - 1;
         }
-        switch (esz) {
-        case 1: ((uint8_t*)out)[r] = ((const uint8_t*)kv)[lo]; break;
-        case 2: ((uint16_t*)out)[r] = ((const uint16_t*)kv)[lo]; break;
-        case 4: ((uint32_t*)out)[r] = ((const uint32_t*)kv)[lo]; break;
-        default: ((uint64_t*)out)[r] = ((const uint64_t*)kv)[lo]; break;
+        if (vec && r0 + PER <= n && (ESZ == 4 || ESZ == 8)) {
+            E v[PER];
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                lo = mc_part(offs, n_parts, lo, r0 + e);
+                v[e] = keys[lo];
+            }
+            if (ESZ == 8) {
+                ((ulonglong2*)o)[c] = make_ulonglong2((unsigned long long)v[0],
+                                                      (unsigned long long)v[PER > 1 ? 1 : 0]);
+            } else {
+                ((uint4*)o)[c] = make_uint4((uint32_t)v[0], (uint32_t)v[PER > 1 ? 1 : 0],
+                                            (uint32_t)v[PER > 2 ? 2 : 0], (uint32_t)v[PER > 3 ? 3 : 0]);
+            }
+        } else {
+            for (int e = 0; e < PER && r0 + e < n; e++) {
+                lo = mc_part(offs, n_parts, lo, r0 + e);
+                o[r0 + e] = keys[lo];
+            }
         }
     }
 }
@@ -452,7 +501,22 @@ extern "C" int tdd_mapcommon_fill(void* out, int esz, const void* kv, const int6
                                   int n_parts, int64_t n) {
     if (n <= 0) return TD_OK;
     if (n_parts < 1 || (esz != 1 && esz != 2 && esz != 4 && esz != 8)) return TD_ERR_NYI;
-    LAUNCH("mapcommon", k_mapcommon, stream_grid(ceil_div(n, 256), 8), 256, 0, out, esz, kv,
-           offs, n_parts, n);
+    const bool vec = ((uintptr_t)out & 15) == 0;
+    const int grid = stream_grid(ceil_div(ceil_div(n, 16 / esz), 256), 8);
+    const bool lds = n_parts <= MC_LDS;
+#define MC(ESZ_)                                                                            \
+    do {                                                                                    \
+        if (lds) LAUNCH("mapcommon", (k_mapcommon<ESZ_, true>), grid, 256, 0, out, kv, offs, \
+                        n_parts, n, vec);                                                   \
+        else LAUNCH("mapcommon", (k_mapcommon<ESZ_, false>), grid, 256, 0, out, kv, offs,   \
+                    n_parts, n, vec);                                                       \
+    } while (0)
+    switch (esz) {
+    case 1: MC(1); break;
+    case 2: MC(2); break;
+    case 4: MC(4); break;
+    default: MC(8); break;
+    }
+#undef MC
     return TD_OK;
 }

@@ -138,3 +138,61 @@ e.release(r); g.free()
     z = np.load(tmp_path / "sort.npz")
     for i, c in enumerate(cols):
         np.testing.assert_array_equal(z[f"arr_{i}"], c[p], err_msg=f"sort col {i}")
+
+
+def test_parted_many_sym_partitions(eng, tmp_path):
+    """1,100 partitions named 1.0000 .. 1.1099 (not dates: a SYM MAPCOMMON
+    column of 8-byte ids, partitions beyond what the fill stages in LDS),
+    of 1-7 rows each: the partition column of the flattened table repeats
+    each partition's id over its rows, in name order, and a group-by over it
+    gives each partition's row count and sum."""
+    body = """
+import os
+rng = np.random.default_rng(11)
+sizes = rng.integers(1, 8, 1100)
+vals = rng.integers(-10 ** 6, 10 ** 6, int(sizes.sum()))
+os.makedirs("db")
+off = 0
+for p, s in enumerate(sizes):
+    os.makedirs(f"db/1.{p:04d}")
+    h = e.table([("v", vals[off:off + s])])
+    assert e.L.td_splay_save(h, f"db/1.{p:04d}/t".encode(), b"db/sym") == 0
+    e.release(h)
+    off += s
+t = e.L.td_dev_parted_load(b"db", b"t")
+assert t and t > 32, t
+g = e.graph(t)
+r = g.execute(g.head(g.const_table(t), int(sizes.sum())))
+np.savez("rows.npz", *[np.asarray(c[3]) for c in e.read_table(r)])
+e.release(r); g.free()
+g = e.graph(t)
+r = g.execute(g.group([g.scan("part")], [("SUM", g.scan("v")), ("COUNT", g.scan("v"))]))
+np.savez("grp.npz", *[np.asarray(c[3]) for c in e.read_table(r)])
+e.release(r); g.free()
+np.savez("src.npz", sizes=sizes, vals=vals)
+"""
+    prog = (f"import sys, json, numpy as np\nsys.path[:0] = [{REPO!r}, {HERE!r}]\n"
+            "from teide_amd.td import Engine, LIB_PATH\n"
+            "e = Engine(LIB_PATH)\nassert e.L.td_dev_available() == 1\n" + body)
+    print("parted many-partition case in a fresh process", flush=True)
+    r = subprocess.run([sys.executable, "-c", prog], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-4000:]
+    src = np.load(tmp_path / "src.npz")
+    sizes, vals = src["sizes"], src["vals"]
+    rows = np.load(tmp_path / "rows.npz")
+    part, v = rows["arr_0"].astype(np.int64), rows["arr_1"]
+    np.testing.assert_array_equal(v, vals)
+    ids = part[np.concatenate([[0], np.cumsum(sizes)[:-1]])]      # first row of each partition
+    assert len(np.unique(ids)) == len(sizes)
+    assert np.all(np.diff(ids) > 0), "ids follow the name order in which they were interned"
+    np.testing.assert_array_equal(part, np.repeat(ids, sizes))
+    grp = np.load(tmp_path / "grp.npz")
+    gk = grp["arr_0"].astype(np.int64)
+    order = np.argsort(gk)
+    pos = np.searchsorted(ids, gk[order])
+    np.testing.assert_array_equal(gk[order], ids[pos])
+    np.testing.assert_array_equal(grp["arr_2"][order], sizes[pos])
+    sums = np.add.reduceat(vals, np.concatenate([[0], np.cumsum(sizes)[:-1]]))
+    np.testing.assert_array_equal(grp["arr_1"][order], sums[pos])
+    assert len(gk) == len(sizes)
