@@ -207,11 +207,23 @@ __device__ __forceinline__ int64_t da_slot(const GroupPlan& P, int64_t row) {
     return da_slot2(P, row, row);
 }
 
-__global__ void k_cells_init(uint64_t* tab, int64_t n_slots, GroupPlan P) {
-    int64_t n = n_slots * P.ncells;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        tab[i] = cell_identity(P.kind[i % P.ncells]);
+/* Identity of every cell: the cell index advances by the grid stride (one
+ * division per thread), the identities come from LDS. */
+__global__ __launch_bounds__(256) void k_cells_init(uint64_t* tab, int64_t n_slots, GroupPlan P) {
+    __shared__ uint64_t id[GP_MAX_CELLS];
+    const int nc = P.ncells;
+    if (threadIdx.x < GP_MAX_CELLS) id[threadIdx.x] = cell_identity(threadIdx.x < nc ? P.kind[threadIdx.x] : 0);
+    __syncthreads();
+    const int64_t n = n_slots * nc;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int step = (int)(stride % nc);
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = (int)(i % nc);
+    for (; i < n; i += stride) {
+        tab[i] = id[c];
+        c += step;
+        if (c >= nc) c -= nc;
+    }
 }
 
 /* Each workgroup: private LDS table, then a plain store of the table into
