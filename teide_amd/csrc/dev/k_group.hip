@@ -521,6 +521,39 @@ __global__ __launch_bounds__(256) void k_hll(GroupPlan P, RowMask M, unsigned in
         if (r[i]) atomicMax(&regs[i], r[i]);
 }
 
+/* The same estimate for one raw key column with R consecutive rows per lane
+ * (all loads of a tile issued first). */
+__device__ __forceinline__ void hll_add(unsigned int* r, uint64_t key) {
+    const uint64_t h = mix64(key);
+    const unsigned idx = (unsigned)(h >> (64 - HLL_P));
+    const uint64_t rest = (h << HLL_P) | (1ull << (HLL_P - 1));
+    atomicMax(&r[idx], (unsigned)__clzll((long long)rest) + 1);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_hll_tile(tdd_col_t key, bool vec, RowMask M,
+                                                  unsigned int* regs) {
+    __shared__ unsigned int r[HLL_M];
+    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) r[i] = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t n_tiles = (M.nrows + 64 * R - 1) / (64 * R);
+    const int64_t wave = (int64_t)blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+    const int64_t n_waves = (int64_t)gridDim.x * (256 / WAVE);
+    for (int64_t t = wave; t < n_tiles; t += n_waves) {
+        const int64_t base = t * 64 * R + (int64_t)lane * R;
+        int64_t v[R];
+        load_rows<R>(key, base, M.nrows, vec, v);
+        const uint32_t m = tile_live<R>(M.bits, base, M.nrows);
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            if ((m >> j) & 1) hll_add(r, (uint64_t)v[j]);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x)
+        if (r[i]) atomicMax(&regs[i], r[i]);
+}
+
 
 __global__ __launch_bounds__(GT) void k_ht_agg(GroupPlan P, RowMask M, unsigned long long* keys,
                                                uint64_t mask, uint64_t* cells, int* overflow) {
@@ -636,7 +669,11 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
     int frc = tdd_fill(regs.p, 0, HLL_M * 4);
     if (frc) return frc;
     int hgrid = stream_grid(ceil_div((M.nrows + 63) / 64, 4 * 8), 4);
-    LAUNCH("group_hll", k_hll, hgrid, 256, 0, P, M, regs.as<unsigned int>());
+    if (P.n_keys == 1 && P.kstride[0] == 0 && P.keys[0].data)      /* the raw key: ht_key */
+        LAUNCH("group_hll", k_hll_tile<4>, hgrid, 256, 0, P.keys[0], tile_vec_ok(P.keys[0]), M,
+               regs.as<unsigned int>());
+    else
+        LAUNCH("group_hll", k_hll, hgrid, 256, 0, P, M, regs.as<unsigned int>());
     unsigned int h[HLL_M];
     HC(hipMemcpyAsync(h, regs.p, sizeof(h), hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
