@@ -63,10 +63,26 @@ struct PGRows {                           /* the first pass reads the group's ro
 
 /* Item i of a job: group key (the slot, or the hash key) and value words,
  * from rows or from records. */
-template <bool ROWS, bool HK, int NV>
+template <bool ROWS, bool HK, int NV, int KE = 0>
 __device__ __forceinline__ bool pg_item(const PGRows& S, const PGBuf& in, int64_t i, int64_t r1,
                                         uint64_t& key, uint64_t (&v)[PG_MAXV]) {
     if (i >= r1) return false;
+    if (ROWS && KE != 0) {
+        /* one key of KE bytes, 8-byte value columns: plain loads, no width
+         * switch, so the loads of a round are in flight together */
+        if (S.mask && !((S.mask[i >> 6] >> (i & 63)) & 1)) return false;
+        const tdd_col_t& kc = S.P.keys[0];
+        const int64_t raw = KE == 8 ? ((const int64_t*)kc.data)[i]
+                          : kc.type == TD_SYM ? (int64_t)((const uint32_t*)kc.data)[i]
+                                              : (int64_t)((const int32_t*)kc.data)[i];
+        key = HK && S.P.kstride[0] == 0 ? (uint64_t)raw
+                                        : (uint64_t)((raw - S.P.kmin[0]) * S.P.kstride[0]);
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            v[k] = S.vsrc[k] >= 0 ? (uint64_t)((const int64_t*)S.P.aggs[S.vsrc[k]].data)[i]
+                                  : (uint64_t)i;
+        return true;
+    }
     if (ROWS) {
         if (S.mask && !((S.mask[i >> 6] >> (i & 63)) & 1)) return false;
         key = HK ? ht_key(S.P, i) : (uint64_t)da_slot2(S.P, i, i);
@@ -91,7 +107,7 @@ __device__ __forceinline__ uint64_t pg_bits(uint64_t key) {
 /* Counts of one job's items per sub-partition, written where the exclusive
  * scan turns them into write offsets: parent block (js * PL), then
  * sub-partition s (s * nj), then job (j - js). */
-template <bool ROWS, bool HK>
+template <bool ROWS, bool HK, int KE = 0>
 __global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJob* jobs, int njobs,
                                                   int shift, int PL, uint32_t* Q) {
     __shared__ uint32_t h[PG_MAXP];
@@ -106,7 +122,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJo
         for (int64_t i0 = J.r0; i0 < J.r1; i0 += blockDim.x) {
             const int64_t i = i0 + threadIdx.x;
             uint64_t key = 0;
-            const bool live = pg_item<ROWS, HK, 0>(S, in, i, J.r1, key, v);
+            const bool live = pg_item<ROWS, HK, 0, KE>(S, in, i, J.r1, key, v);
             const int s = live ? (int)((pg_bits<HK>(key) >> shift) & (PL - 1)) : -1;
             const uint64_t lm = __ballot(live);
             if (!lm) continue;
@@ -126,7 +142,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJo
 }
 
 /* The scatter of one pass. */
-template <bool ROWS, bool HK, int NV, int R>
+template <bool ROWS, bool HK, int NV, int R, int KE = 0>
 __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out, const PGJob* jobs,
                                                   int njobs, int shift, int PL, const int64_t* O,
                                                   int64_t nout) {
@@ -152,7 +168,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out,
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 const int64_t i = b + (int64_t)(wv * R + k) * 64 + lane;
-                const bool live = pg_item<ROWS, HK, NV>(S, in, i, J.r1, slot[k], v[k]);
+                const bool live = pg_item<ROWS, HK, NV, KE>(S, in, i, J.r1, slot[k], v[k]);
                 sub[k] = live ? (int32_t)((pg_bits<HK>(slot[k]) >> shift) & (PL - 1)) : -1;
             }
             /* ranks within the wave, per sub-partition: the wave's own row of
@@ -478,24 +494,59 @@ __global__ __launch_bounds__(PG_T) void k_pg_hagg(GroupPlan P, PGBuf in, const P
 
 /* ------------------------------------------------------------------ host -- */
 
+/* Host: the element bytes ld_i64 reads a column with when that is a plain
+ * signed 8- or 4-byte load (4-byte SYM: unsigned), else 0. */
+static int pg_plain(const tdd_col_t& c) {
+    switch (c.type) {
+    case TD_I64: case TD_TIMESTAMP: case TD_F64: return 8;
+    case TD_I32: case TD_DATE: case TD_TIME: return 4;
+    case TD_SYM:
+        switch (c.attrs & TD_SYM_W_MASK) {
+        case TD_SYM_W32: return 4;
+        case TD_SYM_W8: case TD_SYM_W16: return 0;
+        default: return 8;
+        }
+    default: return 0;
+    }
+}
+
 template <bool HK>
 static int pg_hist_and_offsets(bool rows, const PGRows& S, const PGBuf& in, const PGJob* d_jobs,
                                int njobs, int shift, int PL, uint32_t* Q, int64_t* O) {
     const int grid = stream_grid(njobs, 4);
-    if (rows)
+    const int ke = rows && S.P.n_keys == 1 && S.P.keys[0].data ? pg_plain(S.P.keys[0]) : 0;
+    if (rows && ke == 8)
+        LAUNCH("group_pg_hist", (k_pg_hist<true, HK, 8>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
+    else if (rows && ke == 4)
+        LAUNCH("group_pg_hist", (k_pg_hist<true, HK, 4>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
+    else if (rows)
         LAUNCH("group_pg_hist", (k_pg_hist<true, HK>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
     else
         LAUNCH("group_pg_hist", (k_pg_hist<false, HK>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
     return tdd::scan_u32_i64(Q, (int64_t)njobs * PL, O);
 }
 
+
 template <bool ROWS, bool HK>
 static int pg_pass_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
                           const PGJob* d_jobs, int njobs, int shift, int PL, const int64_t* O,
                           int64_t nout) {
     const int grid = stream_grid(njobs, 2);
-#define PGP(NV_) LAUNCH("group_pg_pass", (k_pg_pass<ROWS, HK, NV_, PG_R>), grid, PG_T, 0, S, in, \
-                        out, d_jobs, njobs, shift, PL, O, nout)
+    /* rows of one 4- or 8-byte key and 8-byte value columns: the plain-load form */
+    int ke = 0;
+    if (ROWS && S.P.n_keys == 1 && S.P.keys[0].data) {
+        ke = pg_plain(S.P.keys[0]);
+        for (int k = 0; k < nv; k++)
+            if (S.vsrc[k] >= 0 && pg_plain(S.P.aggs[S.vsrc[k]]) != 8) ke = 0;
+    }
+#define PGP1(NV_, KE_) LAUNCH("group_pg_pass", (k_pg_pass<ROWS, HK, NV_, PG_R, KE_>), grid, PG_T, 0, \
+                              S, in, out, d_jobs, njobs, shift, PL, O, nout)
+#define PGP(NV_)                                      \
+    do {                                              \
+        if (ROWS && ke == 8) PGP1(NV_, (ROWS ? 8 : 0)); \
+        else if (ROWS && ke == 4) PGP1(NV_, (ROWS ? 4 : 0)); \
+        else PGP1(NV_, 0);                            \
+    } while (0)
     switch (nv) {
     case 0: PGP(0); break;
     case 1: PGP(1); break;
@@ -504,6 +555,7 @@ static int pg_pass_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf&
     default: PGP(4); break;
     }
 #undef PGP
+#undef PGP1
     return TD_OK;
 }
 
