@@ -658,6 +658,7 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
              unsigned long long* lkeys, uint64_t* lcells, unsigned long long* lcount,
              int64_t lcap_out, int* overflow);
 void pht_free(tdd_pht* h);
+int64_t pht_global_keys(const tdd_pht* h);
 
 /* Hash group-by.  Many rows with many distinct keys are first split by the
  * key's hash until each part fits an LDS table (pht_*); otherwise rows go
@@ -688,12 +689,26 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
                                 force == 4, &ph);
         if (prc && prc != TD_ERR_NYI) return prc;
     }
-    for (int attempt = 0; attempt < 6; attempt++, cap <<= 1) {
-        int64_t n_slots = (int64_t)cap + 1;                  /* + the EMPTY_KEY slot */
-        SCRATCH(keys, (size_t)cap * 8);
+    /* partitioned: most groups leave through the list (cap entries); the
+     * global table takes the split parts' keys, so it starts at their
+     * bound and grows to cap if that overflows */
+    uint64_t gcap = cap;
+    if (ph) {
+        const int64_t want = pht_global_keys(ph);
+        uint64_t g = 1024;
+        while ((double)g < 2.0 * (double)want) g <<= 1;
+        if (g < gcap) gcap = g;
+    }
+    for (int attempt = 0; attempt < 7; attempt++) {
+        if (attempt > 0) {                                   /* the last try overflowed */
+            if (gcap < cap) gcap = cap;
+            else { cap <<= 1; gcap = cap; }
+        }
+        int64_t n_slots = (int64_t)gcap + 1;                 /* + the EMPTY_KEY slot */
+        SCRATCH(keys, (size_t)gcap * 8);
         SCRATCH(cells, (size_t)n_slots * P.ncells * 8);
         SCRATCH(ovf, 16);
-        int rc = tdd_fill(keys.p, EMPTY_KEY, (size_t)cap * 8);
+        int rc = tdd_fill(keys.p, EMPTY_KEY, (size_t)gcap * 8);
         if (rc) return rc;
         int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
         LAUNCH("group_init", k_cells_init, ib, 256, 0, cells.as<uint64_t>(), n_slots, P);
@@ -703,7 +718,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         if (!lkeys.ok() || !lcells.ok()) return TD_ERR_OOM;
         unsigned long long* lcount = (unsigned long long*)(ovf.as<uint8_t>() + 8);
         if (ph) {
-            rc = pht_fold(ph, keys.as<unsigned long long>(), (uint64_t)(cap - 1),
+            rc = pht_fold(ph, keys.as<unsigned long long>(), (uint64_t)(gcap - 1),
                           cells.as<uint64_t>(), lkeys.as<unsigned long long>(),
                           lcells.as<uint64_t>(), lcount, (int64_t)cap, ovf.as<int>());
             if (rc) return rc;
@@ -711,7 +726,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
             size_t lds = (size_t)CACHE_SLOTS * 8 * (1 + P.ncells);
             int grid = stream_grid(ceil_div((M.nrows + 63) / 64, GT / WAVE), 2);
             LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
-                   (uint64_t)(cap - 1), cells.as<uint64_t>(), ovf.as<int>());
+                   (uint64_t)(gcap - 1), cells.as<uint64_t>(), ovf.as<int>());
         }
         int64_t ol[2] = {0, 0};
         HC(hipMemcpyAsync(ol, ovf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));

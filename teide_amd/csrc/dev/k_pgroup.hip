@@ -867,6 +867,17 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
     return TD_OK;
 }
 
+/* Keys the global table can expect from a partitioned fold: the parts
+ * split over several jobs merge there (at most one leaf table of keys per
+ * job of such a part), plus an allowance for keys that find no room in
+ * their part's leaf table. */
+int64_t pht_global_keys(const tdd_pht* h) {
+    int64_t n = 4096;
+    for (size_t j = 0; j < h->jobs.size(); j++)
+        if (h->jobs[j].nj > 1) n += h->lcap;
+    return n;
+}
+
 void pht_free(tdd_pht* h) {
     if (!h) return;
     (void)hipStreamSynchronize(tdd::g_stream);
