@@ -303,6 +303,68 @@ class Charge:
         self.e.release(self.t)
 
 
+class Join:
+    """A plain inner join (the radix path): fact rows with a scattered 64-bit
+    key and an f64 payload against a dim table of unique scattered keys and
+    an i64 payload; every fact row matches one dim row.  Output: the joined
+    table (fk, x, w) in HBM."""
+    name = "join"
+    metric = "fact rows/s of a radix hash join into a resident joined table"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        import numpy as np
+        self.e, self.a = e, a
+        n, m = a.join_fact, a.join_dim
+        t0 = time.perf_counter()
+        pk = W.permutation(m, 0, SEED, 901) * 1_000_003 + 7     # unique, range far beyond m
+        idx = W.i64_uniform(n, 0, m - 1, SEED, 902)
+        fk = pk[idx]
+        del idx
+        hf = e.table([("fk", fk), ("x", W.f64_uniform(n, 0.0, 1.0, SEED, 903))])
+        del fk
+        hd = e.table([("pk", pk), ("w", W.i64_uniform(m, -10 ** 9, 10 ** 9, SEED, 904))])
+        del pk
+        self.tf, self.td = e.put(hf), e.put(hd)
+        e.L.td_dev_sync()
+        e.release(hf)
+        e.release(hd)
+        print(f"# data: gen + h2d {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        self.rows = n
+        # fact fk + x read, dim pk + w read, joined rows (fk, x, w) written
+        self.query_bytes = n * 16 + m * 16 + n * 24
+        self.config = {"workload": f"inner join fact {n:,} (fk, x) x dim {m:,} (pk, w) on "
+                                   "fk = pk, scattered 64-bit keys, every fact row matching "
+                                   "one dim row -> joined table (fk, x, w)"}
+
+    def step(self):
+        e = self.e
+        g = e.graph(self.tf)
+        root = g.join(g.const_table(self.tf), [g.scan("fk")], g.const_table(self.td),
+                      [g.scan("pk")], 0)
+        out = g.execute(root)
+        g.free()
+        e.release(out)
+
+    def kernel_bytes(self, name):
+        n, m = self.a.join_fact, self.a.join_dim
+        if name == "join_build_probe":
+            # both sides' partition entries (64-bit key + 32-bit row) read by
+            # each of the two launches (count, fill); the fill launch writes
+            # the (left, right) row pairs: half of that per launch on average
+            return (n + m) * 12 + n * 16 // 2
+        if name == "gather":
+            return n * (8 + 8 + 8)             # row index and element read, element written
+        return None
+
+    def cpu_baseline(self):
+        return None
+
+    def release(self):
+        self.e.release(self.tf)
+        self.e.release(self.td)
+
+
 class Parted:
     """A date-partitioned table: P daily partitions of the high-cardinality
     recipe's rows (key, sym, v_i64, v_f64) saved in the reference's on-disk
@@ -745,7 +807,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge",
+    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge", "join",
                                            "parted"],
                     default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
@@ -754,6 +816,8 @@ def main():
                          "load: splayed-table load into HBM")
     ap.add_argument("--load-rows", type=int, default=50_000_000)
     ap.add_argument("--expr-rows", type=int, default=250_000_000)
+    ap.add_argument("--join-fact", type=int, default=100_000_000)
+    ap.add_argument("--join-dim", type=int, default=10_000_000)
     ap.add_argument("--parted-rows", type=int, default=120_000_000)
     ap.add_argument("--parted-parts", type=int, default=30)
     ap.add_argument("--load-reps", type=int, default=3)
@@ -824,7 +888,7 @@ def main():
         return
 
     wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge,
-          "parted": Parted}.get(a.workload, JoinGroup)(e, a)
+          "parted": Parted, "join": Join}.get(a.workload, JoinGroup)(e, a)
 
     for _ in range(a.warmup):
         wl.step()
