@@ -193,7 +193,60 @@ struct PartIn {
     int64_t         n;
     int32_t         bits;     /* radix bits */
     int32_t         n_tiles;
+    int32_t         plain;    /* exact key read by a plain 8- or 4-byte load, else 0 */
 };
+
+/* Host: the element bytes of an exact single key that ld_i64 reads with a
+ * plain signed load (4-byte SYM: unsigned), else 0. */
+static int part_plain(const JKeys& K) {
+    if (!K.exact || K.n != 1) return 0;
+    const tdd_col_t& c = K.cols[0];
+    switch (c.type) {
+    case TD_I64: case TD_TIMESTAMP: return 8;
+    case TD_I32: case TD_DATE: case TD_TIME: return 4;
+    case TD_SYM:
+        switch (c.attrs & TD_SYM_W_MASK) {
+        case TD_SYM_W32: return 4;
+        case TD_SYM_W8: case TD_SYM_W16: return 0;
+        default: return 8;
+        }
+    default: return 0;
+    }
+}
+
+/* Key words of a thread's JROWS rows of the tile at `base` (bit j of the
+ * result: row j exists and is selected).  A plain key loads every row
+ * first, with no width switch between the loads. */
+__device__ __forceinline__ uint32_t part_keys(const PartIn& I, int64_t base,
+                                              uint64_t (&kw)[JROWS]) {
+    uint32_t live = 0;
+#pragma unroll
+    for (int j = 0; j < JROWS; j++) {
+        const int64_t r = base + j * JT + threadIdx.x;
+        if (r < I.n && row_selected(I.sel, r)) live |= 1u << j;
+    }
+    const void* d = I.K.cols[0].data;
+    if (I.plain == 8) {
+#pragma unroll
+        for (int j = 0; j < JROWS; j++) {
+            const int64_t r = base + j * JT + threadIdx.x;
+            kw[j] = r < I.n ? (uint64_t)((const int64_t*)d)[r] : 0;
+        }
+    } else if (I.plain == 4) {
+        const bool uns = I.K.cols[0].type == TD_SYM;
+#pragma unroll
+        for (int j = 0; j < JROWS; j++) {
+            const int64_t r = base + j * JT + threadIdx.x;
+            kw[j] = r >= I.n ? 0
+                  : uns ? (uint64_t)((const uint32_t*)d)[r]
+                        : (uint64_t)(int64_t)((const int32_t*)d)[r];
+        }
+    } else {
+        for (int j = 0; j < JROWS; j++)
+            kw[j] = ((live >> j) & 1) ? key_word(I.K, base + j * JT + threadIdx.x) : 0;
+    }
+    return live;
+}
 
 __device__ __forceinline__ uint32_t part_of(uint64_t kw, int bits) {
     return (uint32_t)(jmix(kw) >> (64 - bits));
@@ -205,10 +258,11 @@ __global__ __launch_bounds__(JT) void k_part_hist(PartIn I, uint32_t* H) {
     for (int i = threadIdx.x; i < P; i += JT) hist[i] = 0;
     __syncthreads();
     int64_t base = (int64_t)blockIdx.x * JTILE;
-    for (int j = 0; j < JROWS; j++) {
-        int64_t r = base + j * JT + threadIdx.x;
-        if (r < I.n && row_selected(I.sel, r)) atomicAdd(&hist[part_of(key_word(I.K, r), I.bits)], 1u);
-    }
+    uint64_t kw[JROWS];
+    const uint32_t live = part_keys(I, base, kw);
+#pragma unroll
+    for (int j = 0; j < JROWS; j++)
+        if ((live >> j) & 1) atomicAdd(&hist[part_of(kw[j], I.bits)], 1u);
     __syncthreads();
     for (int p = threadIdx.x; p < P; p += JT) H[(int64_t)p * I.n_tiles + blockIdx.x] = hist[p];
 }
@@ -230,11 +284,10 @@ __global__ __launch_bounds__(JT) void k_part_scatter(PartIn I, const int64_t* Of
     int64_t base = (int64_t)blockIdx.x * JTILE;
     uint64_t kw[JROWS];
     uint32_t pr[JROWS], rank[JROWS];
+    const uint32_t live = part_keys(I, base, kw);
     for (int j = 0; j < JROWS; j++) {
-        int64_t r = base + j * JT + threadIdx.x;
         pr[j] = 0xffffffffu;
-        if (r < I.n && row_selected(I.sel, r)) {
-            kw[j] = key_word(I.K, r);
+        if ((live >> j) & 1) {
             pr[j] = part_of(kw[j], I.bits);
             rank[j] = atomicAdd(&hist[pr[j]], 1u);
         }
@@ -517,6 +570,8 @@ static int join_radix(const tdd_join_spec_t* S, tdd_join_result_t* out) {
     PartIn IL{LK, S->lsel ? S->lsel->bits : nullptr, S->nl, bits,
               (int32_t)ceil_div(S->nl > 0 ? S->nl : 1, JTILE)};
     PartIn IR{RK, nullptr, S->nr, bits, (int32_t)ceil_div(S->nr > 0 ? S->nr : 1, JTILE)};
+    IL.plain = part_plain(LK);
+    IR.plain = part_plain(RK);
     uint64_t *lkey = nullptr, *rkey = nullptr;
     uint32_t *lrow = nullptr, *rrow = nullptr;
     int64_t *loff = nullptr, *roff = nullptr, nlp = 0, nrp = 0;
