@@ -469,11 +469,8 @@ class BatchQueries:
         from teide_amd import workloads as W
         import numpy as np
         self.e, self.a = e, a
-        rng = np.random.default_rng(SEED)
         n = a.batch_queries
-        nf = np.exp(rng.uniform(np.log(1_000), np.log(2_000_000), n)).astype(np.int64)
-        ncat = np.exp(rng.uniform(np.log(4), np.log(65_536), n)).astype(np.int64)
-        thr = rng.integers(1, 101, n)
+        nf, ncat, thr = W.batch_spec(n, SEED)
         t0 = time.perf_counter()
         self.tables, self.dags = [], []
         qbytes = 0

@@ -132,6 +132,15 @@ int      td_dev_serve_poll(td_dev_server_t* s, int64_t* id, td_t** result);
 void     td_dev_serve_info(const td_dev_server_t* s, td_dev_serve_info_t* out);
 void     td_dev_serve_close(td_dev_server_t* s);
 
+/* Cancellation points: td_execute polls td_cancel() between DAG nodes and
+ * the device layer polls it between the launches of multi-pass operators
+ * (partition passes, retries, join phases); a cancelled query returns
+ * TD_ERR_CANCEL with its device scratch freed.  Test hook: the k-th poll
+ * from now acts as a td_cancel() (0 disables); td_dev_cancel_polls counts
+ * the polls made so far. */
+void     td_dev_cancel_after(int64_t polls);
+int64_t  td_dev_cancel_polls(void);
+
 /* Physical-plan hints for tests and measurement: force the join path
  * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group,
  * 4 XCD-partitioned fused join-group) / group path (0 auto, 1 direct

@@ -181,6 +181,15 @@ int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
  * scalar aggregate, exec.c:5766-5940). */
 int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out);
 
+/* Product of rows [0, n) of col into out8 (8 bytes): a double for an F64
+ * column, else the wrapping int64 product of the values widened to int64;
+ * 1 for n == 0.  A fixed-shape tree, so the f64 bits depend only on the
+ * input.  Replaces exec_reduction's OP_PROD (exec.c:1589-1617, 1779, 1812). */
+int tdd_reduce_prod(const tdd_col_t* col, int64_t n, void* out8);
+/* out[i] = in[i]'s bits with NaNs made one canonical NaN and -0.0 made
+ * +0.0: the value normalisation of exec_count_distinct (exec.c:1697-1701). */
+int tdd_f64_canon(const double* in, int64_t n, int64_t* out);
+
 /* ------------------------------------------------------------------ join -- */
 
 typedef struct {

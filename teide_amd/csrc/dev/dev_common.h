@@ -47,6 +47,14 @@ void  release(void* p);
         HC(hipGetLastError());                                                \
     } while (0)
 
+/* A cancellation point between launches (td_cancel, host engine): the
+ * caller's scratch is freed by its destructors on the way out. */
+extern "C" int td__cancel_poll(void);
+#define POLL_CANCEL()                                            \
+    do {                                                         \
+        if (td__cancel_poll()) return TD_ERR_CANCEL;             \
+    } while (0)
+
 /* Scoped arena scratch: freed on scope exit. */
 struct Scratch {
     void* p = nullptr;

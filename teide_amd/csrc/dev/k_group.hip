@@ -700,6 +700,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         if (g < gcap) gcap = g;
     }
     for (int attempt = 0; attempt < 7; attempt++) {
+        POLL_CANCEL();
         if (attempt > 0) {                                   /* the last try overflowed */
             if (gcap < cap) gcap = cap;
             else { cap <<= 1; gcap = cap; }

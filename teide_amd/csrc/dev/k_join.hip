@@ -509,6 +509,7 @@ static int join_direct(const tdd_join_spec_t* S, tdd_join_result_t* out, bool fo
         rc = append_unmatched(matched.as<uint8_t>(), S->nr, &n_unm, &flag, &pos);
         if (rc) return rc;
     }
+    POLL_CANCEL();
     rc = alloc_pairs(npairs + n_unm, out);
     if (rc) return rc;
     if (npairs > 0)
@@ -580,6 +581,7 @@ static int join_radix(const tdd_join_spec_t* S, tdd_join_result_t* out) {
     struct Frees { void* p[6]; ~Frees() { for (void* q : p) if (q) td__arena_free(q); } }
         fr{{lkey, lrow, loff, rkey, rrow, roff}};
     if (rc) return rc;
+    POLL_CANCEL();
     BPArgs A;
     memset(&A, 0, sizeof(A));
     A.L = Parts{lkey, lrow, loff, IL.n_tiles};
@@ -1023,6 +1025,7 @@ int jg_xpart(const tdd_join_group_spec_t* S, const GroupPlan& GP, const TileCols
 
 extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
     memset(out, 0, sizeof(*out));
+    POLL_CANCEL();
     const tdd_join_spec_t* J = &S->join;
     bool fusable = J->join_type == 0 && J->n_keys == 1 && is_int(J->lkeys[0].type) &&
                    is_int(J->rkeys[0].type) && J->force_path != 2 && S->n_keys >= 1 &&

@@ -656,6 +656,7 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
     int shift = top;
     int64_t parents = 1;
     for (int pass = 0; pass < npass; pass++) {
+        POLL_CANCEL();
         const int PL = 1 << pbits[pass];
         shift -= pbits[pass];
         const int njobs = (int)jobs.size();

@@ -96,3 +96,17 @@ void td_pool_destroy(void) {}
 void td_cancel(void) { atomic_store(&g_cancel, 1); }
 int  td__cancel_requested(void) { return atomic_load(&g_cancel); }
 void td__cancel_reset(void) { atomic_store(&g_cancel, 0); }
+
+/* Cancellation points: td_execute polls between DAG nodes and the device
+ * layer between the launches of its multi-pass operators.  A poll sees a
+ * td_cancel() from any thread; the test hook td_dev_cancel_after(k) makes
+ * the k-th poll from now act as one. */
+static _Atomic(int64_t) g_cancel_after, g_polls;
+int td__cancel_poll(void) {
+    atomic_fetch_add(&g_polls, 1);
+    int64_t k = atomic_load(&g_cancel_after);
+    if (k > 0 && atomic_fetch_sub(&g_cancel_after, 1) == 1) atomic_store(&g_cancel, 1);
+    return atomic_load(&g_cancel);
+}
+void    td_dev_cancel_after(int64_t polls) { atomic_store(&g_cancel_after, polls > 0 ? polls : 0); }
+int64_t td_dev_cancel_polls(void) { return atomic_load(&g_polls); }

@@ -1503,6 +1503,45 @@ static td_t* exec_join_group(xctx_t* x, td_op_t* op, td_op_ext_t* ext, td_op_t* 
 
 /* ----------------------------------------------------------- reductions -- */
 
+/* Distinct values of a vector (exec_count_distinct, exec.c:1658-1727):
+ * integer-like types by their widened value, F64 by its bits with every NaN
+ * one value and -0.0 equal to 0.0.  Counted as the groups of a one-key
+ * group-by on the device. */
+static td_t* count_distinct(xctx_t* x, td_t* in) {
+    switch (in->type) {
+    case TD_BOOL: case TD_U8: case TD_CHAR: case TD_I16: case TD_I32: case TD_I64:
+    case TD_F64: case TD_DATE: case TD_TIME: case TD_TIMESTAMP: case TD_SYM:
+        break;
+    default:
+        return TD_ERR_PTR(TD_ERR_TYPE);
+    }
+    int64_t n = in->len;
+    if (n == 0) return td_i64(0);
+    tdd_group_spec_t S;
+    memset(&S, 0, sizeof(S));
+    S.nrows = n;
+    S.n_keys = 1;
+    S.n_aggs = 1;
+    S.ops[0] = OP_COUNT;
+    int rc = dev_col(x, in, &S.keys[0]);
+    void* canon = NULL;
+    if (!rc && in->type == TD_F64) {
+        canon = td__arena_alloc((size_t)n * 8);
+        rc = canon ? tdd_f64_canon((const double*)S.keys[0].data, n, canon) : TD_ERR_OOM;
+        S.keys[0].data = canon;
+        S.keys[0].type = TD_I64;
+        S.keys[0].ranged = 0;
+    }
+    tdd_group_result_t R;
+    if (!rc) rc = tdd_group(&S, &R);
+    if (!rc) {
+        td__arena_free(R.keys[0]);
+        td__arena_free(R.vals[0]);
+    }
+    if (canon) td__arena_free(canon);
+    return rc ? TD_ERR_PTR(rc) : td_i64(R.n_groups);
+}
+
 static td_t* exec_reduce(xctx_t* x, td_op_t* op) {
     td_t* in = xnode(x, op->inputs[0]);
     if (TD_IS_ERR(in)) return in;
@@ -1514,6 +1553,29 @@ static td_t* exec_reduce(xctx_t* x, td_op_t* op) {
         return r;
     }
     if (in->type <= 0 || in->type == TD_TABLE) { td_release(in); return TD_ERR_PTR(TD_ERR_TYPE); }
+    if (op->opcode == OP_PROD) {
+        /* exec_reduction's product (exec.c:1779, 1812): f64 for an F64
+         * input, else the wrapping i64 product of the widened values; 1 for
+         * no rows */
+        tdd_col_t c;
+        uint64_t v = 0;
+        int rc = dev_col(x, in, &c);
+        if (!rc) rc = tdd_reduce_prod(&c, in->len, &v);
+        int isf = in->type == TD_F64;
+        td_release(in);
+        if (rc) return TD_ERR_PTR(rc);
+        if (isf) {
+            double d;
+            memcpy(&d, &v, 8);
+            return td_f64(d);
+        }
+        return td_i64((int64_t)v);
+    }
+    if (op->opcode == OP_COUNT_DISTINCT) {
+        r = count_distinct(x, in);
+        td_release(in);
+        return r;
+    }
     /* One group keyed by a constant: a zero-filled U8 key column. */
     int64_t n = in->len;
     tdd_group_spec_t S;
@@ -1550,7 +1612,11 @@ static td_t* exec_reduce(xctx_t* x, td_op_t* op) {
     if (ot == TD_F64) {
         double d;
         memcpy(&d, &v, 8);
-        if (R.n_groups == 0) d = op->opcode == OP_SUM ? 0.0 : __builtin_nan("");
+        /* no rows: the reference's accumulator defaults (exec.c:1808-1832) --
+         * 0 for SUM / MIN / MAX / AVG / FIRST / LAST, NaN for the variances */
+        if (R.n_groups == 0)
+            d = (op->opcode == OP_VAR || op->opcode == OP_VAR_POP || op->opcode == OP_STDDEV ||
+                 op->opcode == OP_STDDEV_POP) ? __builtin_nan("") : 0.0;
         return td_f64(d);
     }
     int64_t i = (int64_t)v;
@@ -1759,7 +1825,7 @@ static td_t* exec_having(xctx_t* x, td_op_t* op) {
 static td_t* xnode(xctx_t* x, td_op_t* op) {
     td_graph_t* g = x->g;
     if (!op) return TD_ERR_PTR(TD_ERR_NYI);
-    if (td__cancel_requested()) return TD_ERR_PTR(TD_ERR_CANCEL);
+    if (td__cancel_poll()) return TD_ERR_PTR(TD_ERR_CANCEL);
     switch (op->opcode) {
     case OP_SCAN: {
         td_op_ext_t* e = td__find_ext(g, op->id);
@@ -1825,7 +1891,7 @@ static td_t* xnode(xctx_t* x, td_op_t* op) {
         return exec_join(x, op);
     case OP_SUM: case OP_MIN: case OP_MAX: case OP_COUNT: case OP_AVG:
     case OP_FIRST: case OP_LAST: case OP_STDDEV: case OP_STDDEV_POP:
-    case OP_VAR: case OP_VAR_POP:
+    case OP_VAR: case OP_VAR_POP: case OP_PROD: case OP_COUNT_DISTINCT:
         return exec_reduce(x, op);
     case OP_ALIAS: case OP_MATERIALIZE:
         return xnode(x, op->inputs[0]);

@@ -112,6 +112,9 @@ static td_op_t* new_node(td_graph_t* g) {
     return n;
 }
 
+/* A fresh node for the optimizer's rewrites (may move g->nodes). */
+td_op_t* td__graph_new_node(td_graph_t* g) { return new_node(g); }
+
 /* An ext record plus `extra` trailing bytes for its arrays; its node slot is
  * reserved now and filled by publish(). */
 static td_op_ext_t* new_ext(td_graph_t* g, size_t extra) {

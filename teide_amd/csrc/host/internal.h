@@ -78,6 +78,7 @@ void*  td__sel_dev_payload(td_t* s, void** tmp);
 
 /* ---- core.c ---- */
 int    td__cancel_requested(void);
+int    td__cancel_poll(void);
 void   td__cancel_reset(void);
 
 /* ---- sym / table helpers ---- */
@@ -86,6 +87,7 @@ const char* td__sym_cstr(int64_t id, size_t* len);
 
 /* ---- exec helpers shared by exec.c and dev_api.c ---- */
 td_op_ext_t* td__find_ext(td_graph_t* g, uint32_t node_id);
+td_op_t*     td__graph_new_node(td_graph_t* g);
 
 /* Element size honouring TD_SYM width. */
 static inline uint8_t td__esz(const td_t* v) {

@@ -7,9 +7,10 @@
  * (opt.c:192-507), whose arithmetic differs from the fused evaluator's in
  * places (IEEE division by zero, fmin / fmax), so results depend on it;
  * folding a FILTER whose predicate is a constant into MATERIALIZE (keep all
- * rows) or HEAD 0 (keep none) (opt.c:461-494); and splitting
- * FILTER(AND(a, b)) into FILTER(FILTER(x, a), b) (opt.c:1304-1342), which
- * the device evaluates as one AND-ed selection anyway.  The others (type
+ * rows) or HEAD 0 (keep none) (opt.c:461-494); and splitting every
+ * FILTER(AND(a, b)) into FILTER(FILTER(x, a), b) with each chain of filters
+ * ordered cheapest-first (opt.c:1258-1459), which the device evaluates as
+ * one AND-ed selection anyway.  The others (type
  * inference, SIP, factorisation, pushdown into SELECT/EXPAND, projection
  * marking, partition pruning, fusion flags, DCE) do not change the result
  * of the supported shapes and are not run.
@@ -217,41 +218,125 @@ static void fold_filters(td_graph_t* g, td_op_t* op, uint8_t* seen) {
     }
 }
 
-/* FILTER(x, AND(a, b)) -> FILTER(FILTER(x, a), b), repeatedly. */
-static td_op_t* split_ands(td_graph_t* g, td_op_t* root) {
-    if (!root || root->opcode != OP_FILTER || root->arity != 2) return root;
-    td_op_t* p = root->inputs[1];
-    if (!p || p->opcode != OP_AND) return root;
-    uint32_t rid = root->id, aid = p->inputs[0]->id, bid = p->inputs[1]->id;
-    td_op_t* outer = NULL;
-    if (g->node_count < g->node_cap) {
-        outer = &g->nodes[g->node_count];
-        memset(outer, 0, sizeof(*outer));
-        outer->id = g->node_count++;
-    } else {
-        return root;                       /* no room to split: leave as is */
+/* Every FILTER(x, AND(a, b)) in the graph becomes FILTER(FILTER(x, a), b),
+ * until no AND predicate is left under a FILTER (the split of
+ * pass_filter_reorder, opt.c:1304-1395).  The rewritten node keeps its id as
+ * the outer filter on b and a new node filters x on a, so whatever consumed
+ * the node still sees the outer filter and needs no redirection. */
+static void split_ands(td_graph_t* g) {
+    for (int round = 0; round < 16; round++) {
+        int changed = 0;
+        uint32_t nc = g->node_count;
+        for (uint32_t i = 0; i < nc; i++) {
+            td_op_t* n = &g->nodes[i];
+            if (n->opcode != OP_FILTER || n->arity != 2 || !n->inputs[0] || !n->inputs[1] ||
+                n->inputs[1]->opcode != OP_AND || n->inputs[1]->arity != 2)
+                continue;
+            uint32_t xid = n->inputs[0]->id, aid = n->inputs[1]->inputs[0]->id,
+                     bid = n->inputs[1]->inputs[1]->id;
+            td_op_t* inner = td__graph_new_node(g);          /* may move g->nodes */
+            if (!inner) return;
+            n = &g->nodes[i];
+            inner->opcode = OP_FILTER;
+            inner->arity = 2;
+            inner->inputs[0] = &g->nodes[xid];
+            inner->inputs[1] = &g->nodes[aid];
+            inner->out_type = n->out_type;
+            inner->est_rows = n->est_rows;
+            n->inputs[0] = inner;
+            n->inputs[1] = &g->nodes[bid];
+            changed = 1;
+        }
+        if (!changed) return;
     }
-    root = &g->nodes[rid];
-    root->inputs[1] = &g->nodes[aid];
-    outer->opcode = OP_FILTER;
-    outer->arity = 2;
-    outer->inputs[0] = root;
-    outer->inputs[1] = &g->nodes[bid];
-    outer->out_type = root->out_type;
-    outer->est_rows = root->est_rows;
-    split_ands(g, root);
-    return split_ands(g, outer);
+}
+
+/* Cost of a filter predicate, lower runs first (filter_cost,
+ * opt.c:1258-1299): +4 when neither input is a constant, +0..3 by the
+ * first input's width (BOOL / U8, I16, I32 / DATE / TIME, wider), +0 for
+ * EQ / NE, +2 for ordered comparisons, +1 otherwise. */
+static int pred_cost(td_op_t* p) {
+    if (!p) return 99;
+    int c = 0, has_const = 0;
+    for (int i = 0; i < p->arity && i < 2; i++)
+        if (p->inputs[i] && p->inputs[i]->opcode == OP_CONST) has_const = 1;
+    if (!has_const) c += 4;
+    int8_t t = p->arity >= 1 && p->inputs[0] ? p->inputs[0]->out_type : p->out_type;
+    c += t == TD_BOOL || t == TD_U8 ? 0 : t == TD_I16 ? 1 :
+         (t == TD_I32 || t == TD_DATE || t == TD_TIME) ? 2 : 3;
+    switch (p->opcode) {
+    case OP_EQ: case OP_NE: break;
+    case OP_LT: case OP_LE: case OP_GT: case OP_GE: c += 2; break;
+    case OP_LIKE: case OP_ILIKE: c += 4; break;
+    default: c += 1; break;
+    }
+    return c;
+}
+
+/* Nodes and ext records that take node `id` as an input. */
+static int consumers(td_graph_t* g, uint32_t id) {
+    int n = 0;
+    for (uint32_t i = 0; i < g->node_count; i++)
+        for (int k = 0; k < g->nodes[i].arity && k < 2; k++)
+            if (g->nodes[i].inputs[k] && g->nodes[i].inputs[k]->id == id) n++;
+    for (uint32_t e = 0; e < g->ext_count; e++) {
+        td_op_ext_t* x = g->ext_nodes[e];
+        if (!x || x->base.opcode != OP_GROUP) continue;
+        for (int k = 0; k < x->n_keys; k++) if (x->keys[k] && x->keys[k]->id == id) n++;
+        for (int a = 0; a < x->n_aggs; a++) if (x->agg_ins[a] && x->agg_ins[a]->id == id) n++;
+    }
+    return n;
+}
+
+/* Within each chain of stacked FILTERs that nothing else shares, order the
+ * predicates by cost: the cheapest on the innermost filter (evaluated
+ * first), ties in their original order (pass_filter_reorder's stable
+ * insertion sort, opt.c:1397-1459).  Predicates move between the chain's
+ * nodes; the nodes stay where they are. */
+static void reorder_chains(td_graph_t* g) {
+    uint8_t* done = calloc(g->node_count + 1, 1);
+    if (!done) return;
+    for (uint32_t i = 0; i < g->node_count; i++) {
+        td_op_t* n = &g->nodes[i];
+        if (n->opcode != OP_FILTER || done[i]) continue;
+        td_op_t* chain[64];
+        int cost[64], len = 0, shared = 0;
+        for (td_op_t* c = n; c && c->opcode == OP_FILTER && c->arity == 2 && len < 64;
+             c = c->inputs[0]) {
+            chain[len++] = c;
+            done[c->id] = 1;
+        }
+        if (len < 2) continue;
+        for (int c = 0; c < len && !shared; c++) shared = consumers(g, chain[c]->id) > 1;
+        if (shared) continue;
+        for (int c = 0; c < len; c++) cost[c] = pred_cost(chain[c]->inputs[1]);
+        for (int c = 1; c < len; c++) {          /* descending cost, outermost first */
+            td_op_t* p = chain[c]->inputs[1];
+            int k = cost[c], j = c - 1;
+            while (j >= 0 && cost[j] < k) {
+                chain[j + 1]->inputs[1] = chain[j]->inputs[1];
+                cost[j + 1] = cost[j];
+                j--;
+            }
+            chain[j + 1]->inputs[1] = p;
+            cost[j + 1] = k;
+        }
+    }
+    free(done);
 }
 
 td_op_t* td_optimize(td_graph_t* g, td_op_t* root) {
     if (!g || !root) return root;
+    uint32_t rid = root->id;
     uint8_t* seen = calloc(g->node_count + 1, 1);
     if (!seen) return root;
     fold_consts(g, root, seen);
     memset(seen, 0, g->node_count + 1);
     fold_filters(g, root, seen);
     free(seen);
-    return split_ands(g, root);
+    split_ands(g);
+    reorder_chains(g);
+    return &g->nodes[rid];
 }
 
 void td_fuse_pass(td_graph_t* g, td_op_t* root) { (void)g; (void)root; }

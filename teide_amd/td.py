@@ -124,13 +124,14 @@ class Engine:
             "td_sel_from_pred": (P, [P]), "td_sel_and": (P, [P, P]), "td_sel_new": (P, [I64]),
             "td_sel_recompute": (None, [P]), "td_vec_from_raw": (P, [I8, P, I64]),
             "td_err_str": (C.c_char_p, [C.c_int]), "td_pool_init": (C.c_int, [C.c_uint32]),
-            "td_thread_count": (C.c_uint32, []),
+            "td_thread_count": (C.c_uint32, []), "td_cancel": (None, []),
         }
         for op in ["eq", "ne", "lt", "le", "gt", "ge", "and", "or", "add", "sub", "mul", "div",
                    "mod", "min2", "max2"]:
             sig["td_" + op] = (P, [P, P, P])
         for op in ["not", "sum", "count", "avg", "min_op", "max_op", "first", "last", "neg",
-                   "abs", "sqrt_op", "log_op", "exp_op", "ceil_op", "floor_op"]:
+                   "abs", "sqrt_op", "log_op", "exp_op", "ceil_op", "floor_op", "prod",
+                   "count_distinct", "stddev", "stddev_pop", "var", "var_pop"]:
             sig["td_" + op] = (P, [P, P])
         sig["td_cast"] = (P, [P, P, I8])
         if device:
@@ -153,6 +154,8 @@ class Engine:
                 "td_dev_serve_open": (P, [P]), "td_dev_serve_submit": (I64, [P, P, P]),
                 "td_dev_serve_step": (C.c_int, [P]), "td_dev_serve_poll": (C.c_int, [P, P, P]),
                 "td_dev_serve_info": (None, [P, P]), "td_dev_serve_close": (None, [P]),
+                "td_dev_cancel_after": (None, [I64]), "td_dev_cancel_polls": (I64, []),
+                "td_heap_gc": (None, []),
             })
         for name, (res, args) in sig.items():
             f = getattr(self.L, name)

@@ -108,3 +108,15 @@ def permutation(n, base, seed, stream):
 
 def unit(seed, a, b):
     return lib().tdg_unit(seed, a, b)
+
+
+def batch_spec(n=4096, seed=42):
+    """The batched workload's per-query sizes: fact rows log-uniform in
+    [1K, 2M], group-key cardinality log-uniform in [4, 65536] and the
+    `qty > thr` threshold uniform in [1, 100]; query q's dim has fact/8 rows
+    and its tables come from join_fact / join_dim with seed + q."""
+    rng = np.random.default_rng(seed)
+    nf = np.exp(rng.uniform(np.log(1_000), np.log(2_000_000), n)).astype(np.int64)
+    ncat = np.exp(rng.uniform(np.log(4), np.log(65_536), n)).astype(np.int64)
+    thr = rng.integers(1, 101, n)
+    return nf, ncat, thr

@@ -1,14 +1,21 @@
 """GPU parity of batched execution (td_dev_execute_batch / td_dev_batch_*):
-every query's result equals its own td_execute result -- integers bit-exact,
-f64 sums within the f64 tolerance -- including queries the batch hands back
-to td_execute (host-resident inputs, duplicate build keys, other shapes),
-and captured-graph replays interleaved with other device work."""
+every query's result equals the oracle's join + group-by of the same
+columns and its own td_execute result -- keys, COUNT and integer sums
+bit-exact, f64 sums within the f64 tolerance -- including queries the batch
+hands back to td_execute (host-resident inputs, duplicate build keys, other
+shapes), captured-graph replays interleaved with other device work, and a
+sample of the full 4,096-query seed-42 batch."""
 import numpy as np
 import pytest
 
+import orc
 from parity import compare_groups
+from teide_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
+
+# host copies of each query's columns, by fact-table handle, for the oracle
+HOST = {}
 
 
 def make_query(eng, rng, nf, nd, ncat, thr, resident=True, dup=False):
@@ -26,7 +33,31 @@ def make_query(eng, rng, nf, nd, ncat, thr, resident=True, dup=False):
         eng.release(tf)
         eng.release(td)
         tf, td = tf2, td2
+    HOST[tf] = dict(fk=fk, qty=qty, price=price, pk=pk, cat=cat)
     return tf, td, thr
+
+
+def oracle_query(cols, thr, aggs=None):
+    """GROUP(cat; aggs) over JOIN(FILTER(fact, qty > thr), fk = pk): the
+    oracle's inner join, gathers and group-by."""
+    aggs = aggs or [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")]
+    m = cols["qty"] > thr
+    li, ri = orc.join([cols["fk"][m]], [cols["pk"]], 0)
+    src = {"price": orc.gather(cols["price"][m], li), "qty": orc.gather(cols["qty"][m], li)}
+    return orc.group([orc.gather(cols["cat"], ri)], [o for o, _ in aggs],
+                     [src[c] for _, c in aggs])
+
+
+def check_oracle(eng, got, cols, thr, what, aggs=None):
+    ref = oracle_query(cols, thr, aggs)
+    a = eng.read_table(got)
+    assert [c[1] for c in a[1:]] == ref["out_types"], what
+    f64 = [t == 7 and o in ("SUM", "AVG") for t, (o, _) in
+           zip(ref["out_types"], aggs or [("SUM", 0), ("SUM", 0), ("COUNT", 0)])]
+    compare_groups([a[0][3].astype(np.int64)], [c[3] for c in a[1:]], ref["keys"], ref["vals"],
+                   f64, what, exact=ref["comp"], sum_abs=ref["sum_abs"])
+    if ref["order"] == 1:                            # direct-array order: ascending keys
+        np.testing.assert_array_equal(a[0][3].astype(np.int64), ref["keys"][0], err_msg=what)
 
 
 def build_dag(eng, tf, td, thr, aggs=None):
@@ -74,6 +105,7 @@ def test_batch_matches_single_queries(eng):
     assert rc == 0
     for i, (a, b) in enumerate(zip(got, ref)):
         check_same(eng, a, b, f"query {i}")
+        check_oracle(eng, a, HOST[qs[i][0]], qs[i][2], f"query {i} vs oracle")
         eng.release(a)
         eng.release(b)
     for g, _ in dags:
@@ -94,6 +126,8 @@ def test_batch_graph_replay(eng):
         assert rc == 0
         for i, (a, b) in enumerate(zip(got, ref)):
             check_same(eng, a, b, f"replay {rep} query {i}")
+            if rep in (0, 3):
+                check_oracle(eng, a, HOST[qs[i][0]], qs[i][2], f"replay {rep} query {i} oracle")
             eng.release(a)
         # unrelated eager device work between replays
         extra = run_singles(eng, qs[:2])
@@ -120,7 +154,54 @@ def test_batch_mixed_shapes_and_preds(eng):
     assert rc == 0
     for i, (a, b) in enumerate(zip(got, ref)):
         check_same(eng, a, b, f"query {i}")
+        check_oracle(eng, a, HOST[qs[i][0]], qs[i][2], f"query {i} vs oracle",
+                     [(o, c) for o, c in (shapes[i] or
+                      [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")])])
         eng.release(a)
         eng.release(b)
     for g, _ in dags:
         g.free()
+
+
+def test_batch_full_size_sample(eng):
+    """The full batched workload -- 4,096 queries of seed 42 (fact rows
+    log-uniform in [1K, 2M], 1.08e9 in all, dim = fact / 8, groups
+    log-uniform in [4, 65536]) -- prepared, autotuned and run as one batch
+    with graph replay; 64 of its queries, spread over the size range, are
+    checked against the oracle on the host copies of their columns."""
+    n = 4096
+    nf, ncat, thr = W.batch_spec(n, 42)
+    tables, dags = [], []
+    for q in range(n):
+        nd = max(1, int(nf[q]) // 8)
+        fact = W.join_fact(int(nf[q]), nd, 42 + q)
+        dim = W.join_dim(nd, int(ncat[q]), 42 + q)
+        hf = eng.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
+        hd = eng.table([("pk", dim["pk"]), ("cat", dim["cat"])])
+        tf, td = eng.put(hf), eng.put(hd)
+        eng.release(hf)
+        eng.release(hd)
+        tables += [tf, td]
+        dags.append(build_dag(eng, tf, td, int(thr[q])))
+    h = eng.batch_prepare([g for g, _ in dags], [r for _, r in dags])
+    nb, _, _ = eng.batch_info(h)
+    assert nb == n                                   # every query batched
+    order = np.argsort(nf)
+    sample = sorted(set(order[np.linspace(0, n - 1, 64).astype(int)].tolist()) |
+                    {int(np.argmax(ncat)), int(np.argmin(ncat))})
+    assert len(sample) >= 64
+    for rep in range(2):                             # capture, then replay
+        rc, got = eng.batch_run(h, n)
+        assert rc == 0
+        for q in (sample if rep else sample[:8]):
+            nd = max(1, int(nf[q]) // 8)
+            cols = dict(W.join_fact(int(nf[q]), nd, 42 + q))
+            cols.update(W.join_dim(nd, int(ncat[q]), 42 + q))
+            check_oracle(eng, got[q], cols, int(thr[q]), f"C4 query {q} run {rep}")
+        for r in got:
+            eng.release(r)
+    eng.L.td_dev_batch_free(h)
+    for g, _ in dags:
+        g.free()
+    for t in tables:
+        eng.release(t)

@@ -153,6 +153,44 @@ def test_join_vs_oracle(eng, forced, kind, jt, path):
     row_multiset_equal([c[3] for c in got], ref, f"{kind}/{jt}/{path}")
 
 
+# SYM key columns of each index width (td.h:121-133): dtype, attrs
+SYM_W = {8: (np.uint8, 0), 16: (np.uint16, 1), 32: (np.uint32, 2), 64: (np.int64, 3)}
+
+
+@pytest.mark.parametrize("widths", [(8, 64), (16, 32), (32, 16), (64, 8), (8, 8), (64, "i64")])
+@pytest.mark.parametrize("path", [(0, 0), (1, 0), (2, 0), (2, 4)])
+def test_join_sym_widths(eng, forced, widths, path):
+    """SYM keys join by their widened index value (read_col_i64, exec.c:90-109,
+    compared in join_keys_eq, exec.c:7754-7768), whatever the two widths."""
+    rng = np.random.default_rng(500 + 8 * widths[0] + (widths[1] if widths[1] != "i64" else 1))
+    nl, nr = 60001, 9001
+    hi = min(256, 1 << min(w if w != "i64" else 64 for w in widths))
+    lk = rng.integers(0, hi, nl)
+    rk = rng.permutation(np.arange(nr) % hi)[:nr] if nr > hi else rng.integers(0, hi, nr)
+    if widths == (64, "i64"):
+        lk, rk = lk * 7919 + 2 ** 40, rk * 7919 + 2 ** 40
+
+    def col(name, k, w):
+        if w == "i64":
+            return (name, k.astype(np.int64))
+        dt, attrs = SYM_W[w]
+        return (name, k.astype(dt), 20, attrs)
+
+    lv = rng.integers(-10 ** 6, 10 ** 6, nl)
+    rv = rng.random(nr)
+    lcols = [col("k", lk, widths[0]), ("lv", lv)]
+    rcols = [col("k", rk, widths[1]), ("rv", rv)]
+    forced(*path)
+    got = run_join(eng, lcols, rcols, ["k"], ["k"], 0)
+    assert [c[0] for c in got] == ["k", "lv", "rv"]
+    if widths[0] != "i64":
+        assert got[0][1] == 20 and got[0][2] == SYM_W[widths[0]][1]    # left key keeps its width
+    li, ri = orc.join([lk.astype(np.int64)], [rk.astype(np.int64)], 0)
+    ref = [lk.astype(np.int64)[li], lv[li], rv[ri]]
+    row_multiset_equal([got[0][3].astype(np.int64), got[1][3], got[2][3]], ref,
+                       f"sym {widths} {path}")
+
+
 def test_join_two_keys_and_resident(eng, forced):
     rng = np.random.default_rng(77)
     nl, nr = 50000, 20000

@@ -182,6 +182,9 @@ KEYSETS = {
     "f64": lambda r, n: [np.round(r.normal(size=n), 1)],                 # hash on raw bits
     "two_keys": lambda r, n: [r.integers(0, 30, n), r.integers(-4, 4, n).astype(np.int16)],
     "sym_w32": lambda r, n: [r.integers(0, 5000, n).astype(np.uint32)],
+    "sym_w8": lambda r, n: [r.integers(0, 256, n).astype(np.uint8)],
+    "sym_w16": lambda r, n: [r.integers(0, 65536, n).astype(np.uint16)],
+    "sym_w64": lambda r, n: [r.integers(0, 3000, n) * 7919 + 2 ** 40],
     # runs of one key (sorted or partitioned data): whole waves fold into one slot
     "i64_sorted": lambda r, n: [np.sort(r.integers(-20, 40, n))],
 }
@@ -191,7 +194,10 @@ KEYSETS = {
 # tables so that even these sizes take two or three scatter passes; group
 # path 4: the hash-partitioned hash group-by with small LDS tables (keys
 # that do not fit a part's table fold into the global table)
-PDA_KEYSETS = ("i64_small", "i64_medium", "i32", "two_keys", "sym_w32", "zipf")
+PDA_KEYSETS = ("i64_small", "i64_medium", "i32", "two_keys", "sym_w32", "sym_w8", "sym_w16",
+               "zipf")
+# SYM key columns: (td.h type tag, index width in attrs bits 0-1)
+SYM_KEYS = {"sym_w8": (20, 0), "sym_w16": (20, 1), "sym_w32": (20, 2), "sym_w64": (20, 3)}
 
 
 @pytest.fixture
@@ -207,8 +213,6 @@ def group_path(eng):
 @pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("gpath", [0, 3, 4])
 def test_group_vs_oracle(eng, group_path, kset, aggi, masked, gpath):
-    if kset == "zipf" and gpath == 0 and masked:
-        pytest.skip("covered unmasked")
     rng = np.random.default_rng(zlib.crc32(f"{kset}/{aggi}/{masked}".encode()))
     n = 200003
     if kset == "zipf":
@@ -218,7 +222,8 @@ def test_group_vs_oracle(eng, group_path, kset, aggi, masked, gpath):
     group_path(gpath)
     vi = rng.integers(-10 ** 9, 10 ** 9, n)
     vf = rng.normal(size=n) * 100
-    cols = [(f"k{i}", k) for i, k in enumerate(keys)] + [("vi", vi), ("vf", vf)]
+    ktag = SYM_KEYS.get(kset)
+    cols = [(f"k{i}", k) + (ktag or ()) for i, k in enumerate(keys)] + [("vi", vi), ("vf", vf)]
     t = eng.table(cols)
     g = eng.graph(t)
     ops, ins = AGG_SETS[aggi]
@@ -240,7 +245,7 @@ def test_group_vs_oracle(eng, group_path, kset, aggi, masked, gpath):
     if gpath == 4:
         assert "path=5" in plan, plan              # hash-partitioned hash
     src = {"vi": vi, "vf": vf}
-    kt = [(20, 2)] if kset == "sym_w32" else None
+    kt = [ktag] if ktag else None
     ref = orc.group(keys, ops, [src[c] for c in ins], mask_bits=mask, key_types=kt)
     nk = len(keys)
     f64 = ["skip" if ops[a] in VAR_OPS else

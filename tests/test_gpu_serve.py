@@ -8,11 +8,17 @@ outside the served family answered by td_execute."""
 import numpy as np
 import pytest
 
+import orc
+from parity import compare_groups
 from test_gpu_batch import check_same
 
 pytestmark = pytest.mark.gpu
 
 CMPS = ["gt", "ge", "lt", "le", "eq"]
+CMP_NP = {"gt": np.greater, "ge": np.greater_equal, "lt": np.less, "le": np.less_equal,
+          "eq": np.equal}
+# host copies of the served tables' columns, for the oracle
+HOST = {}
 
 
 @pytest.fixture(scope="module")
@@ -25,6 +31,7 @@ def tables(eng):
     pk = rng.permutation(nd) + 5
     cat = rng.integers(0, 50, nd).astype(np.int32)
     k64 = rng.integers(0, 3000, nd)
+    HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64)
     tf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
     td = eng.table([("pk", pk), ("cat", cat), ("k64", k64)])
     rf, rd = eng.put(tf), eng.put(td)
@@ -90,6 +97,25 @@ def alone(eng, rf, rd, r):
     return out
 
 
+def check_oracle(eng, got, r, what):
+    """GROUP(key; aggs) over JOIN(FILTER(fact, qty CMP t), fk = pk) by the
+    oracle's join, gathers and group-by on the host copies."""
+    h = HOST
+    m = CMP_NP[r["cmp"]](h["qty"], r["t"]) if r.get("filtered", True) else \
+        np.ones(len(h["qty"]), bool)
+    li, ri = orc.join([h["fk"][m]], [h["pk"]], 0)
+    src = {"price": orc.gather(h["price"][m], li), "qty": orc.gather(h["qty"][m], li)}
+    ops = [o for o, _ in r["aggs"]]
+    ref = orc.group([orc.gather(h[r["key"]], ri)], ops, [src[c] for _, c in r["aggs"]])
+    a = eng.read_table(got)
+    assert [c[1] for c in a[1:]] == ref["out_types"], what
+    f64 = [t == 7 and o in ("SUM", "AVG") for t, o in zip(ref["out_types"], ops)]
+    compare_groups([a[0][3].astype(np.int64)], [c[3] for c in a[1:]], ref["keys"], ref["vals"],
+                   f64, what, exact=ref["comp"], sum_abs=ref["sum_abs"])
+    if ref["order"] == 1:
+        np.testing.assert_array_equal(a[0][3].astype(np.int64), ref["keys"][0], err_msg=what)
+
+
 def test_serve_mix_matches_alone(eng, tables):
     rf, rd = tables
     rng = np.random.default_rng(32)
@@ -121,6 +147,7 @@ def test_serve_mix_matches_alone(eng, tables):
     for rid, r in ids.items():
         ref = alone(eng, rf, rd, r)
         check_same(eng, results[rid], ref, f"request {rid} {r}")
+        check_oracle(eng, results[rid], r, f"request {rid} {r} vs oracle")
         eng.release(results[rid])
         eng.release(ref)
     # the same requests, each alone in the server
@@ -133,6 +160,7 @@ def test_serve_mix_matches_alone(eng, tables):
         run_to_end(eng, h, one)
         ref = alone(eng, rf, rd, r)
         check_same(eng, one[rid1], ref, f"alone in server {r}")
+        check_oracle(eng, one[rid1], r, f"alone in server {r} vs oracle")
         eng.release(one[rid1])
         eng.release(ref)
         g.free()
@@ -185,6 +213,7 @@ def test_serve_slots_and_budget(eng, tables):
         assert peak == cap
         for rid, g in sub:
             check_same(eng, res[rid], ref, f"admission {kw} request {rid}")
+            check_oracle(eng, res[rid], req, f"admission {kw} request {rid} vs oracle")
             eng.release(res[rid])
             g.free()
         eng.serve_close(h)

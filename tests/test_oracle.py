@@ -54,6 +54,76 @@ def test_sel_layout_brute_force(n):
     np.testing.assert_array_equal(orc.sel_to_idx(s, n), np.flatnonzero(pred))
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1023, 1024, 1025, 70001])
+def test_sel_and_brute_force(n):
+    # every word ANDed (a dropped or shifted word fails), the padding bits of
+    # the last word left clear, flags / popcounts / total recomputed from the
+    # ANDed bits rather than copied from either input
+    rng = np.random.default_rng(1000 + n)
+    pa = rng.random(n) < 0.6
+    pb = rng.random(n) < 0.5
+    if n > 3000:
+        pa[1024:2048] = pb[1024:2048] = True      # ALL in both -> ALL
+        pa[2048:3072] = True                      # ALL AND MIX -> MIX
+        pb[3072:4096] = False                     # -> NONE
+    out = orc.sel_and(orc.sel_from_bool(pa.astype(np.uint8)),
+                      orc.sel_from_bool(pb.astype(np.uint8)), n)
+    total, flags, pop, bits = orc.sel_parts(out, n)
+    both = pa & pb
+    assert total == int(both.sum())
+    unpacked = np.unpackbits(bits.view(np.uint8), bitorder="little")
+    np.testing.assert_array_equal(unpacked[:n], both.astype(np.uint8))
+    assert not unpacked[n:].any()
+    for sgi in range((n + 1023) // 1024):
+        seg = both[sgi * 1024:(sgi + 1) * 1024]
+        assert pop[sgi] == seg.sum()
+        assert flags[sgi] == (0 if seg.sum() == 0 else 1 if seg.all() else 2)
+
+
+CMP_NP = {"EQ": np.equal, "NE": np.not_equal, "LT": np.less, "LE": np.less_equal,
+          "GT": np.greater, "GE": np.greater_equal}
+
+
+@pytest.mark.parametrize("op", list(CMP_NP))
+def test_cmp_cols_brute_force(op):
+    # column-column comparisons: integer pairs compare as int64 (values past
+    # 2^53 that an f64 compare would merge stay apart), pairs with an F64
+    # side compare as f64; narrow and SYM columns are widened first
+    rng = np.random.default_rng(7 + len(op))
+    n = 5000
+    a64 = rng.integers(-5, 5, n) + 2 ** 53
+    b64 = rng.integers(-5, 5, n) + 2 ** 53
+    got = orc.cmp_cols(a64, b64, op)
+    np.testing.assert_array_equal(got, CMP_NP[op](a64, b64).astype(np.uint8))
+    assert (got != CMP_NP[op](a64.astype(np.float64), b64.astype(np.float64))).any() \
+        or op in ("LT", "GT", "NE")
+    a32 = rng.integers(-50, 50, n).astype(np.int32)
+    bf = np.round(rng.normal(size=n) * 30, 0) + 0.5 * (rng.random(n) < 0.2)
+    np.testing.assert_array_equal(orc.cmp_cols(a32, bf, op),
+                                  CMP_NP[op](a32.astype(np.float64), bf).astype(np.uint8))
+    s16 = rng.integers(0, 40000, n).astype(np.uint16)       # SYM W16: unsigned
+    i16 = rng.integers(-20000, 20000, n).astype(np.int16)
+    np.testing.assert_array_equal(orc.cmp_cols(s16, i16, op),
+                                  CMP_NP[op](s16.astype(np.int64),
+                                             i16.astype(np.int64)).astype(np.uint8))
+    fa, fb = rng.normal(size=n), rng.normal(size=n)
+    fa[::17] = np.nan
+    fb[::5] = fa[::5]
+    np.testing.assert_array_equal(orc.cmp_cols(fa, fb, op), CMP_NP[op](fa, fb).astype(np.uint8))
+
+
+def test_bool_op_brute_force():
+    rng = np.random.default_rng(99)
+    a = (rng.random(4099) < 0.5).astype(np.uint8)
+    b = (rng.random(4099) < 0.3).astype(np.uint8)
+    np.testing.assert_array_equal(orc.bool_op("AND", a, b), (a & b))
+    np.testing.assert_array_equal(orc.bool_op("OR", a, b), (a | b))
+    np.testing.assert_array_equal(orc.bool_op("NOT", a), 1 - a)
+    # any non-zero byte is true
+    c = a * rng.integers(1, 255, 4099).astype(np.uint8)
+    np.testing.assert_array_equal(orc.bool_op("AND", c, b), (a & b))
+
+
 def test_cmp_typing():
     # mixed integer / f64 comparisons run in f64, pure integer ones in int64
     big = np.array([2 ** 53 + 1, 5], np.int64)
