@@ -72,16 +72,23 @@ int plan_cells(GroupPlan& P) {
     return TD_OK;
 }
 
+/* Output columns of n groups; on failure none is left allocated. */
 int alloc_group_outputs(const GroupPlan& P, int64_t n, tdd_group_result_t* out) {
+    bool ok = true;
     for (int k = 0; k < P.n_keys; k++) {
         out->keys[k] = td__arena_alloc((size_t)(n > 0 ? n : 1) * P.keys[k].esz);
-        if (!out->keys[k]) return TD_ERR_OOM;
+        ok = ok && out->keys[k];
     }
     for (int a = 0; a < P.n_aggs; a++) {
         int esz = P.out_types[a] == TD_F64 || P.out_types[a] == TD_I64 ? 8 : P.aggs[a].esz;
         out->vals[a] = td__arena_alloc((size_t)(n > 0 ? n : 1) * esz);
-        if (!out->vals[a]) return TD_ERR_OOM;
+        ok = ok && out->vals[a];
         out->out_types[a] = P.out_types[a];
+    }
+    if (!ok) {
+        for (int k = 0; k < P.n_keys; k++) { td__arena_free(out->keys[k]); out->keys[k] = nullptr; }
+        for (int a = 0; a < P.n_aggs; a++) { td__arena_free(out->vals[a]); out->vals[a] = nullptr; }
+        return TD_ERR_OOM;
     }
     out->n_groups = n;
     return TD_OK;

@@ -448,7 +448,12 @@ static int alloc_pairs(int64_t n, tdd_join_result_t* out) {
     out->n_pairs = n;
     out->l_idx = (int64_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
     out->r_idx = (int64_t*)td__arena_alloc((size_t)(n > 0 ? n : 1) * 8);
-    if (!out->l_idx || !out->r_idx) return TD_ERR_OOM;
+    if (!out->l_idx || !out->r_idx) {
+        td__arena_free(out->l_idx);
+        td__arena_free(out->r_idx);
+        out->l_idx = out->r_idx = nullptr;
+        return TD_ERR_OOM;
+    }
     return TD_OK;
 }
 

@@ -1936,7 +1936,17 @@ td_t* td_execute(td_graph_t* g, td_op_t* root) {
         if (TD_IS_ERR(f)) r = f;
         else { td_release(f); g->tables[i] = f; }
     }
+    td_t* sel_in = g->selection;
+    if (sel_in) td_retain(sel_in);
     if (!r) r = xnode(&x, root);
+    if (TD_IS_ERR(r) && g->selection != sel_in) {
+        /* a failed (e.g. cancelled) query drops the selection its FILTERs
+         * made and leaves the graph's own as it was */
+        if (g->selection) td_release(g->selection);
+        g->selection = sel_in;
+        sel_in = NULL;
+    }
+    if (sel_in) td_release(sel_in);
     if (!TD_IS_ERR(r) && g->selection && r->type == TD_TABLE) {
         td_t* c = compact_table(&x, r, g->selection);      /* final compaction */
         td_release(r);
