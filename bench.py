@@ -348,6 +348,20 @@ class Join:
 
     def kernel_bytes(self, name):
         n, m = self.a.join_fact, self.a.join_dim
+        # materialising radix join: the histogram runs once per side, the
+        # scatter twice per side (4,096 partitions = two 64-way passes);
+        # bytes are per launch on average
+        if name == "rjoin_scatter":
+            # fact: fk + x read and written per pass (the second pass writes
+            # the output's left columns); dim: pk + w read, w + key word
+            # written, then both read and written again
+            return (n * 64 + m * 64) // 4
+        if name == "rjoin_hist":
+            return (n * 8 + m * 8) // 2        # the key column read
+        if name == "rjoin_build_probe":
+            # dim key words + w read once; fact keys read, w written, and the
+            # inner join's per-row match flag written
+            return m * 16 + n * (8 + 8 + 1)
         if name == "join_build_probe":
             # both sides' partition entries (64-bit key + 32-bit row) read by
             # each of the two launches (count, fill); the fill launch writes

@@ -219,6 +219,35 @@ typedef struct {
  * right rows (full) come last with l = -1. */
 int tdd_join(const tdd_join_spec_t* spec, tdd_join_result_t* out);
 
+/* The materialising radix join: INNER or LEFT equi-join that writes the
+ * output columns directly -- lcols[0..n_lcols) of every output row's left
+ * row, then rcols[0..n_rcols) of its right row (zeros for a LEFT row with no
+ * partner) -- with no pair arrays and no random gathers.  Both sides are
+ * radix partitioned once by key (the left side carrying its output columns,
+ * so its partitioned buffers are the output); each partition's build keys
+ * go into an LDS hash table and its probe rows stream through it.  Rows come
+ * out in partition order, as the reference's radix join emits them
+ * (exec.c:7431-7950, output columns exec.c:8607-8728).  Returns TD_ERR_NYI,
+ * with nothing allocated, when the build side repeats a key, a build
+ * partition outgrows an LDS table, the build keys are dense enough for the
+ * direct map of tdd_join, or the join is FULL: run tdd_join then. */
+typedef struct {
+    tdd_join_spec_t   join;
+    int32_t           n_lcols, n_rcols;
+    tdd_col_t         lcols[16];
+    tdd_col_t         rcols[16];
+} tdd_join_mat_spec_t;
+
+typedef struct {
+    int64_t  n_rows;
+    void*    lcols[16];           /* arena blocks, element bytes of lcols[c] */
+    void*    rcols[16];
+    int32_t  path;                /* 3 materialising radix */
+    int32_t  radix_bits;
+} tdd_join_mat_result_t;
+
+int tdd_join_mat(const tdd_join_mat_spec_t* spec, tdd_join_mat_result_t* out);
+
 /* Fused filter -> join -> group-by: the probe side is filtered by `lpred`
  * (and/or `join.lsel`), probed against the build side and aggregated into
  * the group table without materialising the filtered table or the join

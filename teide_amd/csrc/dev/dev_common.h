@@ -62,6 +62,12 @@ struct Scratch {
     ~Scratch() { if (p) tdd::release(p); }
     template <class T> T* as() const { return static_cast<T*>(p); }
     bool ok() const { return p != nullptr; }
+    /* replace the block by a new one of `bytes` */
+    bool reset(size_t bytes) {
+        if (p) tdd::release(p);
+        p = tdd::scratch(bytes ? bytes : 8);
+        return p != nullptr;
+    }
     Scratch(const Scratch&) = delete;
     Scratch& operator=(const Scratch&) = delete;
 };

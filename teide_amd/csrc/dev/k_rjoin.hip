@@ -1,0 +1,821 @@
+/*
+ * k_rjoin.hip -- the materialising radix join (exec.c:7431-7950, 8607-8728):
+ * an equi-join whose output columns are written directly, with no pair
+ * arrays and no random gathers.
+ *
+ *  1. Partition passes (both sides): a key word per row (the value of a
+ *     single integer key, else a hash of all keys), the partition is the top
+ *     `bits` of its mix.  Each partition's region is cut into one run per
+ *     XCD (workgroup b runs on XCD b mod 8): a histogram pass counts rows per
+ *     (partition, XCD) -- LDS counts per workgroup, added into the global
+ *     counts -- an exclusive scan gives the runs' offsets, and the scatter
+ *     pass appends each row to its run through that run's cursor, carrying
+ *     every carried column along.  All workgroups of an XCD append to the
+ *     same run of a partition, so the open end of every run (a cache line
+ *     per partition and column, ~1 MB for 4,096 partitions) stays in that
+ *     XCD's L2 until it is full and is written back whole.  The probe side
+ *     carries every output left column, so its partitioned buffers *are* the
+ *     join's left output columns; the build side carries the right output
+ *     columns.
+ *  2. Build + probe, one workgroup per partition: the build partition's key
+ *     words go into LDS with a table of 32-bit slots (load <= 0.5, sized to
+ *     leave room for two workgroups per CU), then the probe partition streams
+ *     through it; a hit copies the build row's payload into the right output
+ *     columns at the probe record's own index.
+ *
+ * This form needs every build key to be unique (each probe row then has at
+ * most one partner, so outputs are slotted by probe record) and every build
+ * partition to fit one LDS table; the kernel reports either violation and
+ * the caller runs the pair-producing join instead.  LEFT joins keep every
+ * probe row (zeros for a missing partner); INNER joins compact the slotted
+ * output when some probe rows found no partner.
+ */
+#include "join_keys.h"
+
+#include <vector>
+
+#define RJ_MAXC  16              /* carried columns per side */
+#define RJ_T     512             /* threads per workgroup */
+#define RJ_RPT   8               /* rows per thread per step */
+#define RJ_STEP  (RJ_T * RJ_RPT)
+#define RJ_SLOTS 8192            /* LDS hash slots per partition table */
+#define RJ_CAP   4096            /* build rows per partition table */
+#define RJ_EMPTY 0xffffffffu
+#define RJ_MAXB  14              /* radix bits */
+
+#define RJ_XCD   8               /* XCDs: workgroup b runs on XCD b mod 8 */
+#define RJ_FMAX  256             /* ways of one scatter pass */
+
+/* One scatter pass over one side.  Pass 1 reads the side's own columns
+ * (rows [b * span, ...) per workgroup, through the selection); pass 2 reads
+ * the pass-1 buffers, workgroup b taking the tiles of XCD b mod 8 from the
+ * tile table.  A row's digit is `dbits` bits of its key word's mix below the
+ * top `dshift` bits already used. */
+struct RJPass {
+    JKeys           K;           /* key columns (pass 2: the carried key column) */
+    const uint64_t* sel;         /* pass 1: rows taking part; NULL: all */
+    int64_t         n;           /* pass 1: rows */
+    int64_t         span;        /* pass 1: rows per workgroup (a multiple of RJ_STEP) */
+    const int64_t*  tiles;       /* pass 2: (start, end) per tile, by XCD */
+    const int32_t*  xtiles;      /* pass 2: first tile of each XCD, RJ_XCD + 1 entries */
+    int32_t         pass2;
+    int32_t         plain;       /* key read by a plain 4- or 8-byte load, else 0 */
+    int32_t         dshift, dbits;
+    int32_t         xruns;       /* cursors per digit: RJ_XCD (pass 1) or 1 (pass 2) */
+    int32_t         ncols;
+    const void*     src[RJ_MAXC];
+    void*           dst[RJ_MAXC];
+    uint8_t         esz[RJ_MAXC];
+    const uint64_t* kw_src;      /* pass 2: carried key words (key read from them) */
+    uint64_t*       kw_dst;      /* carried key words out, NULL: not carried */
+    const uint32_t* row_src;     /* pass 2: carried rows */
+    uint32_t*       row_dst;     /* carried source rows out, NULL: not carried */
+    unsigned long long* cur;     /* run cursors: digit * xruns + run */
+};
+
+/* Histogram pass: counts per (final partition, XCD) of each side's rows. */
+struct RJHist {
+    JKeys           K;
+    const uint64_t* sel;
+    int64_t         n, span;
+    int32_t         bits, plain;
+    uint32_t*       C;           /* C[q * RJ_XCD + x] */
+};
+
+__device__ __forceinline__ uint32_t rj_part(uint64_t kw, int bits) {
+    return bits ? (uint32_t)(jmix(kw) >> (64 - bits)) : 0u;
+}
+
+__device__ __forceinline__ uint32_t rj_slot(uint64_t kw) {
+    return (uint32_t)jmix(kw) & (RJ_SLOTS - 1);
+}
+
+/* Key word of row r of key set K read the fast way when `plain`. */
+__device__ __forceinline__ uint64_t rj_kw(const JKeys& K, int plain, int64_t r) {
+    const void* d = K.cols[0].data;
+    if (plain == 8) return (uint64_t)((const int64_t*)d)[r];
+    if (plain == 4)
+        return K.cols[0].type == TD_SYM ? (uint64_t)((const uint32_t*)d)[r]
+                                        : (uint64_t)(int64_t)((const int32_t*)d)[r];
+    return key_word(K, r);
+}
+
+__global__ __launch_bounds__(RJ_T) void k_rj_hist(RJHist H) {
+    extern __shared__ uint32_t hist[];
+    const int Q = 1 << H.bits;
+    for (int i = threadIdx.x; i < Q; i += RJ_T) hist[i] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * H.span;
+    const int64_t r1 = r0 + H.span < H.n ? r0 + H.span : H.n;
+    for (int64_t base = r0; base < r1; base += RJ_STEP) {
+        uint64_t kw[RJ_RPT];
+        uint32_t live = 0;
+#pragma unroll
+        for (int j = 0; j < RJ_RPT; j++) {
+            const int64_t r = base + j * RJ_T + threadIdx.x;
+            if (r < r1 && row_selected(H.sel, r)) live |= 1u << j;
+        }
+        if (H.plain) {
+#pragma unroll
+            for (int j = 0; j < RJ_RPT; j++) {
+                const int64_t r = base + j * RJ_T + threadIdx.x;
+                kw[j] = rj_kw(H.K, H.plain, r < r1 ? r : r0);
+            }
+        } else {
+            for (int j = 0; j < RJ_RPT; j++)
+                kw[j] = ((live >> j) & 1) ? key_word(H.K, base + j * RJ_T + threadIdx.x) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < RJ_RPT; j++)
+            if ((live >> j) & 1) atomicAdd(&hist[rj_part(kw[j], H.bits)], 1u);
+    }
+    __syncthreads();
+    const int x = blockIdx.x % RJ_XCD;
+    for (int q = threadIdx.x; q < Q; q += RJ_T)
+        if (hist[q]) atomicAdd(&H.C[(int64_t)q * RJ_XCD + x], hist[q]);
+}
+
+/* Write a column's tile rows in sorted order: sorted position i holds tile
+ * row perm[i] and goes to gpos(i). */
+template <class T>
+__device__ __forceinline__ void rj_move(const void* src, void* dst, int64_t tb, int cnt,
+                                        const uint16_t* perm, const uint32_t* pos) {
+    T v[RJ_RPT];
+#pragma unroll
+    for (int k = 0; k < RJ_RPT; k++) {
+        const int i = threadIdx.x + k * RJ_T;
+        if (i < cnt) v[k] = ((const T*)src)[tb + perm[i]];
+    }
+#pragma unroll
+    for (int k = 0; k < RJ_RPT; k++) {
+        const int i = threadIdx.x + k * RJ_T;
+        if (i < cnt) ((T*)dst)[pos[k]] = v[k];
+    }
+}
+
+/* One scatter pass.  Per tile of up to RJ_STEP consecutive rows: each row's
+ * digit and LDS rank, one global reservation per digit of the tile in its
+ * run, then the rows written in digit order so that consecutive lanes store
+ * consecutive addresses of a run. */
+__global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
+    __shared__ uint32_t hist[RJ_FMAX], loff[RJ_FMAX], gbase[RJ_FMAX];
+    __shared__ uint16_t perm[RJ_STEP];
+    __shared__ uint8_t spart[RJ_STEP];
+    __shared__ uint64_t skw[RJ_STEP];
+    __shared__ uint32_t s_hi;                    /* the tile's digits above this pass */
+    const int F = 1 << S.dbits;
+    const int x = blockIdx.x % RJ_XCD;
+    const int run = S.xruns > 1 ? x : 0;
+    if (threadIdx.x == 0) s_hi = 0;             /* pass 1: no digits above; published by
+                                                   the first tile's barrier */
+    int64_t tb = 0, te = 0;
+    int64_t next = blockIdx.x;                  /* pass 1: span; pass 2: tile of this XCD */
+    int32_t t_end = 0, t_step = 1;
+    int64_t r_end = 0;
+    if (S.pass2) {
+        next = S.xtiles[x] + blockIdx.x / RJ_XCD;
+        t_end = S.xtiles[x + 1];
+        t_step = (gridDim.x + RJ_XCD - 1 - x) / RJ_XCD;  /* workgroups of this XCD */
+        if (t_step < 1) t_step = 1;
+    } else {
+        tb = (int64_t)blockIdx.x * S.span;
+        r_end = tb + S.span < S.n ? tb + S.span : S.n;
+    }
+    for (;;) {
+        if (S.pass2) {
+            if (next >= t_end) break;
+            tb = S.tiles[2 * next];
+            te = S.tiles[2 * next + 1];
+            next += t_step;
+        } else {
+            if (tb >= r_end) break;
+            te = tb + RJ_STEP < r_end ? tb + RJ_STEP : r_end;
+        }
+        for (int i = threadIdx.x; i < F; i += RJ_T) hist[i] = 0;
+        __syncthreads();
+        uint64_t kw[RJ_RPT];
+        uint32_t dig[RJ_RPT], rank[RJ_RPT];
+        uint32_t live = 0;
+#pragma unroll
+        for (int j = 0; j < RJ_RPT; j++) {
+            const int64_t r = tb + j * RJ_T + threadIdx.x;
+            if (r < te && (S.pass2 || row_selected(S.sel, r))) live |= 1u << j;
+        }
+        if (S.kw_src) {
+#pragma unroll
+            for (int j = 0; j < RJ_RPT; j++) {
+                const int64_t r = tb + j * RJ_T + threadIdx.x;
+                kw[j] = S.kw_src[r < te ? r : tb];
+            }
+        } else if (S.plain) {
+#pragma unroll
+            for (int j = 0; j < RJ_RPT; j++) {
+                const int64_t r = tb + j * RJ_T + threadIdx.x;
+                kw[j] = rj_kw(S.K, S.plain, r < te ? r : tb);
+            }
+        } else {
+            for (int j = 0; j < RJ_RPT; j++)
+                kw[j] = ((live >> j) & 1) ? key_word(S.K, tb + j * RJ_T + threadIdx.x) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < RJ_RPT; j++) {
+            if (!((live >> j) & 1)) continue;
+            const uint64_t h = jmix(kw[j]);
+            dig[j] = (uint32_t)(h >> (64 - S.dshift - S.dbits)) & (uint32_t)(F - 1);
+            rank[j] = atomicAdd(&hist[dig[j]], 1u);
+            /* pass 2: every row of a tile lies in one pass-1 partition */
+            if (S.dshift && j == 0 && threadIdx.x == 0) s_hi = (uint32_t)(h >> (64 - S.dshift));
+        }
+        __syncthreads();
+        if (threadIdx.x < WAVE) {                /* scan of <= 256 digit counts by one wave */
+            uint32_t carry = 0;
+            for (int d0 = 0; d0 < F; d0 += WAVE) {
+                const int d = d0 + threadIdx.x;
+                const uint32_t c = d < F ? hist[d] : 0;
+                uint32_t incl = c;
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    const uint32_t y = __shfl_up(incl, o, WAVE);
+                    if (threadIdx.x >= o) incl += y;
+                }
+                if (d < F) {
+                    loff[d] = carry + incl - c;
+                    const int64_t q = ((int64_t)s_hi << S.dbits) | d;
+                    gbase[d] = c ? (uint32_t)atomicAdd(&S.cur[q * S.xruns + run],
+                                                       (unsigned long long)c) : 0u;
+                }
+                carry += __shfl(incl, WAVE - 1, WAVE);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RJ_RPT; j++) {
+            if (!((live >> j) & 1)) continue;
+            const uint32_t at = loff[dig[j]] + rank[j];
+            perm[at] = (uint16_t)(j * RJ_T + threadIdx.x);
+            spart[at] = (uint8_t)dig[j];
+            if (S.kw_dst) skw[at] = kw[j];
+        }
+        __syncthreads();
+        const int cnt = (int)(loff[F - 1] + hist[F - 1]);
+        uint32_t pos[RJ_RPT];
+#pragma unroll
+        for (int k = 0; k < RJ_RPT; k++) {
+            const int i = threadIdx.x + k * RJ_T;
+            pos[k] = i < cnt ? gbase[spart[i]] + (uint32_t)(i - (int)loff[spart[i]]) : 0u;
+        }
+        for (int c = 0; c < S.ncols; c++) {                 /* uniform across the block */
+            switch (S.esz[c]) {
+            case 8: rj_move<uint64_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
+            case 4: rj_move<uint32_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
+            case 2: rj_move<uint16_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
+            default: rj_move<uint8_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
+            }
+        }
+        if (S.kw_dst) {
+#pragma unroll
+            for (int k = 0; k < RJ_RPT; k++) {
+                const int i = threadIdx.x + k * RJ_T;
+                if (i < cnt) S.kw_dst[pos[k]] = skw[i];
+            }
+        }
+        if (S.row_dst) {
+#pragma unroll
+            for (int k = 0; k < RJ_RPT; k++) {
+                const int i = threadIdx.x + k * RJ_T;
+                if (i < cnt)
+                    S.row_dst[pos[k]] = S.row_src ? S.row_src[tb + perm[i]]
+                                                  : (uint32_t)(tb + perm[i]);
+            }
+        }
+        if (!S.pass2) tb = te;
+        __syncthreads();                         /* LDS reused by the next tile */
+    }
+}
+
+/* Build + probe arguments: partition p of a side spans
+ * [off[p], off[p + 1]) of its partitioned buffers. */
+struct RJBP {
+    const int64_t*  loff;
+    const int64_t*  roff;
+    tdd_col_t       lkey;        /* partitioned probe key column (exact keys, when carried) */
+    const uint64_t* lkw;         /* else partitioned probe key words */
+    const uint64_t* rkw;         /* partitioned build key words */
+    const uint32_t* lrow;        /* hashed keys: source rows, for verification */
+    const uint32_t* rrow;
+    JKeys           LK, RK;      /* original key columns (verification) */
+    int32_t         exact, left_join, nrc, pad;
+    const void*     rsrc[RJ_MAXC];   /* partitioned build payload */
+    void*           rdst[RJ_MAXC];   /* right output columns, slotted by probe record */
+    uint8_t         resz[RJ_MAXC];
+    uint8_t*        lflag;       /* 1 when the probe record found its partner; NULL: not kept */
+    unsigned long long* stat;    /* [0] matched records, [1] repeated build key, [2] oversized partition */
+};
+
+#define RJ_PB 4                  /* probe records per thread in flight */
+
+/* Payload of the build rows hit[] into a right output column at the probe
+ * records i0 + j * RJ_T (zero for a LEFT row without a partner). */
+template <class T>
+__device__ __forceinline__ void rj_fetch(const void* src, void* dst, int64_t i0, int64_t pe,
+                                         const int64_t (&hit)[RJ_PB], bool left_join) {
+    T v[RJ_PB];
+#pragma unroll
+    for (int j = 0; j < RJ_PB; j++) v[j] = hit[j] >= 0 ? ((const T*)src)[hit[j]] : (T)0;
+#pragma unroll
+    for (int j = 0; j < RJ_PB; j++) {
+        const int64_t i = i0 + j * RJ_T;
+        if (i < pe && (hit[j] >= 0 || left_join)) ((T*)dst)[i] = v[j];
+    }
+}
+
+template <int PLAIN>
+__global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
+    extern __shared__ uint8_t sm[];
+    uint32_t* slot = (uint32_t*)sm;                            /* RJ_SLOTS */
+    uint64_t* tkey = (uint64_t*)(sm + RJ_SLOTS * 4);           /* RJ_CAP */
+    __shared__ uint32_t wcnt[RJ_T / WAVE];
+    const int p = blockIdx.x;
+    const int64_t bs = A.roff[p], be = A.roff[p + 1];
+    const int64_t ps = A.loff[p], pe = A.loff[p + 1];
+    if (be - bs > RJ_CAP) {                      /* uniform: the caller joins another way */
+        if (threadIdx.x == 0) atomicAdd(&A.stat[2], 1ull);
+        return;
+    }
+    for (int i = threadIdx.x; i < RJ_SLOTS; i += RJ_T) slot[i] = RJ_EMPTY;
+    __syncthreads();
+    for (int64_t i = bs + threadIdx.x; i < be; i += RJ_T) {
+        const uint64_t k = A.rkw[i];
+        const uint32_t li = (uint32_t)(i - bs);
+        tkey[li] = k;                            /* before the slot is published */
+        uint32_t s = rj_slot(k);
+        for (;;) {
+            const uint32_t prev = atomicCAS(&slot[s], RJ_EMPTY, li);
+            if (prev == RJ_EMPTY) break;
+            if (tkey[prev] == k &&
+                (A.exact || keys_equal(A.RK, A.rrow[bs + prev], A.RK, A.rrow[i])))
+                A.stat[1] = 1;                   /* a repeated build key */
+            s = (s + 1) & (RJ_SLOTS - 1);
+        }
+    }
+    __syncthreads();
+    uint32_t hits = 0;
+    for (int64_t i0 = ps + threadIdx.x; i0 < pe; i0 += RJ_PB * RJ_T) {
+        uint64_t k[RJ_PB];
+        int64_t hit[RJ_PB];
+#pragma unroll
+        for (int j = 0; j < RJ_PB; j++) {           /* every key load first */
+            const int64_t i = i0 + j * RJ_T;
+            const int64_t ii = i < pe ? i : ps;
+            if (PLAIN == 8) k[j] = (uint64_t)((const int64_t*)A.lkey.data)[ii];
+            else if (PLAIN == 4)
+                k[j] = A.lkey.type == TD_SYM ? (uint64_t)((const uint32_t*)A.lkey.data)[ii]
+                                             : (uint64_t)(int64_t)((const int32_t*)A.lkey.data)[ii];
+            else k[j] = A.lkw ? A.lkw[ii] : (uint64_t)ld_i64(A.lkey, ii);
+        }
+#pragma unroll
+        for (int j = 0; j < RJ_PB; j++) {
+            const int64_t i = i0 + j * RJ_T;
+            hit[j] = -1;
+            if (i >= pe) continue;
+            uint32_t s = rj_slot(k[j]);
+            for (;;) {
+                const uint32_t li = slot[s];
+                if (li == RJ_EMPTY) break;
+                if (tkey[li] == k[j] &&
+                    (A.exact || keys_equal(A.LK, A.lrow[i], A.RK, A.rrow[bs + li]))) {
+                    hit[j] = bs + li;
+                    break;
+                }
+                s = (s + 1) & (RJ_SLOTS - 1);
+            }
+            hits += hit[j] >= 0;
+        }
+        for (int c = 0; c < A.nrc; c++) {                   /* uniform across the block */
+            switch (A.resz[c]) {
+            case 8: rj_fetch<uint64_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
+            case 4: rj_fetch<uint32_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
+            case 2: rj_fetch<uint16_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
+            default: rj_fetch<uint8_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
+            }
+        }
+        if (A.lflag) {
+#pragma unroll
+            for (int j = 0; j < RJ_PB; j++)
+                if (i0 + j * RJ_T < pe) A.lflag[i0 + j * RJ_T] = hit[j] >= 0;
+        }
+    }
+    for (int d = WAVE / 2; d >= 1; d >>= 1) hits += __shfl_xor(hits, d);
+    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = hits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < RJ_T / WAVE; w++) t += wcnt[w];
+        if (t) atomicAdd(&A.stat[0], t);
+    }
+}
+
+/* ----------------------------------------------------------- host side -- */
+
+static bool rj_is_int(int8_t t) {
+    return t != TD_F64 && t != TD_GUID && t != TD_LIST && t != TD_TABLE && t != TD_CHAR;
+}
+
+static int rj_plain(const JKeys& K) {
+    if (!K.exact || K.n != 1) return 0;
+    const tdd_col_t& c = K.cols[0];
+    switch (c.type) {
+    case TD_I64: case TD_TIMESTAMP: return 8;
+    case TD_I32: case TD_DATE: case TD_TIME: return 4;
+    case TD_SYM:
+        switch (c.attrs & TD_SYM_W_MASK) {
+        case TD_SYM_W32: return 4;
+        case TD_SYM_W8: case TD_SYM_W16: return 0;
+        default: return 8;
+        }
+    default: return 0;
+    }
+}
+
+/* Owned device blocks freed on scope exit unless released. */
+struct RJBlocks {
+    void* p[4 * RJ_MAXC + 8] = {};
+    int n = 0;
+    void* take(size_t bytes) {
+        void* q = td__arena_alloc(bytes ? bytes : 8);
+        if (q) p[n++] = q;
+        return q;
+    }
+    void keep(void* q) { for (int i = 0; i < n; i++) if (p[i] == q) p[i] = nullptr; }
+    ~RJBlocks() { for (int i = 0; i < n; i++) if (p[i]) td__arena_free(p[i]); }
+};
+
+/* One side through the partition passes, host view. */
+struct RJHostSide {
+    JKeys           K;
+    const uint64_t* sel = nullptr;
+    int64_t         n = 0;
+    int             plain = 0, ncols = 0;
+    const void*     src[RJ_MAXC] = {};
+    uint8_t         esz[RJ_MAXC] = {};
+    int             key_col = -1;        /* carried column holding the exact key */
+    bool            carry_kw = false, carry_row = false;
+    int64_t         span = 0;
+    int32_t         nspan = 0;
+    std::vector<uint32_t> C;             /* counts per (partition, XCD) */
+    int64_t         total = 0;
+    /* partitioned results */
+    void*           dst[RJ_MAXC] = {};
+    uint64_t*       kw = nullptr;
+    uint32_t*       row = nullptr;
+};
+
+/* Workgroup spans of pass 1: enough workgroups to fill the chip twice. */
+static void rj_spans(RJHostSide& S) {
+    const int64_t steps = ceil_div(S.n > 0 ? S.n : 1, RJ_STEP);
+    int64_t want = (int64_t)tdd::g_num_cu * TUNE_INT("TEIDE_RJ_WG_PER_CU", 2);
+    if (want > steps) want = steps;
+    if (want < 1) want = 1;
+    S.span = ceil_div(steps, want) * RJ_STEP;
+    S.nspan = (int32_t)ceil_div(S.n > 0 ? S.n : 1, S.span);
+}
+
+static int rj_hist(RJHostSide& S, int bits, Scratch& C) {
+    const int Q = 1 << bits;
+    int rc = tdd_fill(C.p, 0, (size_t)Q * RJ_XCD * 4);
+    if (rc) return rc;
+    RJHist H;
+    memset(&H, 0, sizeof(H));
+    H.K = S.K;
+    H.sel = S.sel;
+    H.n = S.n;
+    H.span = S.span;
+    H.bits = bits;
+    H.plain = S.plain;
+    H.C = C.as<uint32_t>();
+    LAUNCH("rjoin_hist", k_rj_hist, S.nspan, RJ_T, (size_t)Q * 4, H);
+    return TD_OK;
+}
+
+/* The scatter passes of one side after its counts came back: run offsets,
+ * buffers, then one pass (bits <= 8) or two (b1 top bits, then b2). */
+static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch& offs,
+                        Scratch& cur1, Scratch& cur2, Scratch& tiles, Scratch& xt,
+                        Scratch& mid, Scratch& kw, Scratch& kwm, Scratch& row, Scratch& rowm) {
+    const int Q = 1 << bits;
+    const int b1 = bits <= 8 ? bits : (bits + 1) / 2, b2 = bits - b1;
+    const int P1 = 1 << b1;
+    /* final partition offsets and pass-1 run cursors */
+    std::vector<int64_t> off((size_t)Q + 1), c1((size_t)P1 * RJ_XCD), c2;
+    int64_t acc = 0;
+    for (int q = 0; q < Q; q++) {
+        off[(size_t)q] = acc;
+        for (int x = 0; x < RJ_XCD; x++) acc += S.C[(size_t)q * RJ_XCD + x];
+    }
+    off[(size_t)Q] = acc;
+    S.total = acc;
+    acc = 0;
+    std::vector<int64_t> p1start((size_t)P1 + 1);
+    for (int p1 = 0; p1 < P1; p1++) {
+        p1start[(size_t)p1] = acc;
+        for (int x = 0; x < RJ_XCD; x++) {
+            c1[(size_t)p1 * RJ_XCD + x] = acc;
+            for (int q = p1 << b2; q < (p1 + 1) << b2; q++) acc += S.C[(size_t)q * RJ_XCD + x];
+        }
+    }
+    p1start[(size_t)P1] = acc;
+    HC(hipMemcpyAsync(offs.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, tdd::g_stream));
+    HC(hipMemcpyAsync(cur1.p, c1.data(), c1.size() * 8, hipMemcpyHostToDevice, tdd::g_stream));
+    const int64_t n = S.total > 0 ? S.total : 1;
+    /* pass-2 tiles: the pass-1 partitions of XCD x are those p1 = x mod 8 */
+    std::vector<int64_t> tl;
+    std::vector<int32_t> xtiles(RJ_XCD + 1, 0);
+    if (b2) {
+        for (int x = 0; x < RJ_XCD; x++) {
+            xtiles[(size_t)x] = (int32_t)(tl.size() / 2);
+            for (int p1 = x; p1 < P1; p1 += RJ_XCD)
+                for (int64_t t = p1start[(size_t)p1]; t < p1start[(size_t)p1 + 1]; t += RJ_STEP) {
+                    tl.push_back(t);
+                    tl.push_back(t + RJ_STEP < p1start[(size_t)p1 + 1] ? t + RJ_STEP
+                                                                       : p1start[(size_t)p1 + 1]);
+                }
+        }
+        xtiles[RJ_XCD] = (int32_t)(tl.size() / 2);
+        HC(hipMemcpyAsync(cur2.p, off.data(), (size_t)Q * 8, hipMemcpyHostToDevice,
+                          tdd::g_stream));
+        if (!tl.empty())
+            HC(hipMemcpyAsync(tiles.p, tl.data(), tl.size() * 8, hipMemcpyHostToDevice,
+                              tdd::g_stream));
+        HC(hipMemcpyAsync(xt.p, xtiles.data(), xtiles.size() * 4, hipMemcpyHostToDevice,
+                          tdd::g_stream));
+    }
+    /* buffers: final, and the pass-1 copies when there is a second pass */
+    size_t mid_off[RJ_MAXC] = {};
+    if (b2) {
+        size_t need = 0;
+        for (int c = 0; c < S.ncols; c++) {
+            mid_off[c] = need;
+            need += ((size_t)n * S.esz[c] + 255) & ~(size_t)255;
+        }
+        if (!mid.reset(need)) return TD_ERR_OOM;
+    }
+    if (S.carry_kw) {
+        if (!kw.reset((size_t)n * 8) || (b2 && !kwm.reset((size_t)n * 8))) return TD_ERR_OOM;
+        S.kw = kw.as<uint64_t>();
+    }
+    if (S.carry_row) {
+        if (!row.reset((size_t)n * 4) || (b2 && !rowm.reset((size_t)n * 4))) return TD_ERR_OOM;
+        S.row = row.as<uint32_t>();
+    }
+    for (int c = 0; c < S.ncols; c++) S.dst[c] = final_dst[c];
+
+    RJPass A;
+    memset(&A, 0, sizeof(A));
+    A.K = S.K;
+    A.sel = S.sel;
+    A.n = S.n;
+    A.span = S.span;
+    A.plain = S.plain;
+    A.dshift = 0;
+    A.dbits = b1;
+    A.xruns = RJ_XCD;
+    A.ncols = S.ncols;
+    for (int c = 0; c < S.ncols; c++) {
+        A.src[c] = S.src[c];
+        A.esz[c] = S.esz[c];
+        A.dst[c] = b2 ? (void*)(mid.as<uint8_t>() + mid_off[c]) : S.dst[c];
+    }
+    A.kw_dst = S.carry_kw ? (b2 ? kwm.as<uint64_t>() : S.kw) : nullptr;
+    A.row_dst = S.carry_row ? (b2 ? rowm.as<uint32_t>() : S.row) : nullptr;
+    A.cur = cur1.as<unsigned long long>();
+    LAUNCH("rjoin_scatter", k_rj_scatter, S.nspan, RJ_T, 0, A);
+    if (!b2) return TD_OK;
+    POLL_CANCEL();
+
+    RJPass B = A;
+    B.pass2 = 1;
+    B.sel = nullptr;
+    B.tiles = tiles.as<int64_t>();
+    B.xtiles = xt.as<int32_t>();
+    B.dshift = b1;
+    B.dbits = b2;
+    B.xruns = 1;
+    for (int c = 0; c < S.ncols; c++) {
+        B.src[c] = A.dst[c];
+        B.dst[c] = S.dst[c];
+    }
+    if (S.carry_kw) {
+        B.kw_src = kwm.as<uint64_t>();
+        B.kw_dst = S.kw;
+    } else {
+        B.K.cols[0].data = A.dst[S.key_col];     /* the carried exact key */
+    }
+    B.row_src = S.carry_row ? rowm.as<uint32_t>() : nullptr;
+    B.row_dst = S.carry_row ? S.row : nullptr;
+    B.cur = cur2.as<unsigned long long>();
+    int maxt = 0;
+    for (int x = 0; x < RJ_XCD; x++)
+        if (xtiles[(size_t)x + 1] - xtiles[(size_t)x] > maxt) maxt = xtiles[(size_t)x + 1] - xtiles[(size_t)x];
+    int per_xcd = tdd::g_num_cu / RJ_XCD * TUNE_INT("TEIDE_RJ_WG_PER_CU", 2);
+    if (per_xcd > maxt) per_xcd = maxt;
+    if (per_xcd < 1) per_xcd = 1;
+    LAUNCH("rjoin_scatter", k_rj_scatter, per_xcd * RJ_XCD, RJ_T, 0, B);
+    return TD_OK;
+}
+
+extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t* out) {
+    memset(out, 0, sizeof(*out));
+    const tdd_join_spec_t* J = &S->join;
+    if (J->join_type > 1 || J->n_keys < 1 || J->n_keys > TDD_MAX_KEYS) return TD_ERR_NYI;
+    if (J->force_path == 1 || J->force_path >= 3) return TD_ERR_NYI;
+    if (S->n_lcols < 0 || S->n_lcols > RJ_MAXC || S->n_rcols < 0 || S->n_rcols > RJ_MAXC)
+        return TD_ERR_NYI;
+    if (J->nl >= ((int64_t)1 << 31) || J->nr >= ((int64_t)1 << 31)) return TD_ERR_NYI;
+    for (int k = 0; k < J->n_keys; k++)
+        if ((J->lkeys[k].type == TD_F64) != (J->rkeys[k].type == TD_F64)) return TD_ERR_NYI;
+    const bool exact = J->n_keys == 1 && rj_is_int(J->lkeys[0].type) && rj_is_int(J->rkeys[0].type);
+    if (J->force_path == 0 && exact && J->nr > 0) {
+        /* dense build keys: the direct map of tdd_join is the better join */
+        int64_t mn, mx;
+        if (J->rkeys[0].ranged) { mn = J->rkeys[0].lo; mx = J->rkeys[0].hi; }
+        else {
+            int rc = tdd_minmax(&J->rkeys[0], J->nr, nullptr, &mn, &mx);
+            if (rc) return rc;
+        }
+        const uint64_t range = (uint64_t)mx - (uint64_t)mn + 1;
+        if (mn <= mx && range <= (uint64_t)(4 * J->nr + 4096) && range < ((uint64_t)1 << 31))
+            return TD_ERR_NYI;
+    }
+
+    /* radix bits: build partitions of about 0.6 of an LDS table, and enough
+     * partitions to spread the probe side over the chip */
+    int bits = J->radix_bits;
+    if (bits <= 0) {
+        int bb = 0, bp = 0;
+        while (bb < RJ_MAXB && ((int64_t)RJ_CAP * 6 / 10 << bb) < J->nr) bb++;
+        while (bp < 10 && ((int64_t)32768 << bp) < J->nl) bp++;
+        bits = bb > bp ? bb : bp;
+    }
+    if (bits > RJ_MAXB) bits = RJ_MAXB;
+    if (bits < 1) bits = 1;
+    const int P = 1 << bits;
+
+    RJBlocks own;                 /* output blocks until handed over */
+    /* probe side: every output left column, plus the key word when the key
+     * is hashed or is not among the carried columns, plus rows to verify */
+    RJHostSide L, R;
+    L.K.n = R.K.n = J->n_keys;
+    L.K.exact = R.K.exact = exact;
+    for (int k = 0; k < J->n_keys; k++) { L.K.cols[k] = J->lkeys[k]; R.K.cols[k] = J->rkeys[k]; }
+    L.sel = J->lsel ? J->lsel->bits : nullptr;
+    L.n = J->nl;
+    R.n = J->nr;
+    L.plain = rj_plain(L.K);
+    R.plain = rj_plain(R.K);
+    L.ncols = S->n_lcols;
+    for (int c = 0; c < S->n_lcols; c++) {
+        L.src[c] = S->lcols[c].data;
+        L.esz[c] = S->lcols[c].esz;
+        if (exact && L.key_col < 0 && S->lcols[c].data == J->lkeys[0].data &&
+            S->lcols[c].type == J->lkeys[0].type && S->lcols[c].attrs == J->lkeys[0].attrs)
+            L.key_col = c;
+    }
+    L.carry_kw = L.key_col < 0;
+    L.carry_row = R.carry_row = !exact;
+    R.carry_kw = true;
+    R.ncols = S->n_rcols;
+    for (int c = 0; c < S->n_rcols; c++) { R.src[c] = S->rcols[c].data; R.esz[c] = S->rcols[c].esz; }
+    rj_spans(L);
+    rj_spans(R);
+
+    /* counts of both sides, read back together */
+    SCRATCH(lc, (size_t)P * RJ_XCD * 4);
+    SCRATCH(rcn, (size_t)P * RJ_XCD * 4);
+    int rc = rj_hist(L, bits, lc);
+    if (!rc) rc = rj_hist(R, bits, rcn);
+    if (rc) return rc;
+    L.C.resize((size_t)P * RJ_XCD);
+    R.C.resize((size_t)P * RJ_XCD);
+    HC(hipMemcpyAsync(L.C.data(), lc.p, L.C.size() * 4, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipMemcpyAsync(R.C.data(), rcn.p, R.C.size() * 4, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    int64_t nlp = 0, nrp = 0;
+    for (uint32_t v : L.C) nlp += v;
+    for (uint32_t v : R.C) nrp += v;
+    POLL_CANCEL();
+    void* ldst[RJ_MAXC] = {};
+    void* rbuf[RJ_MAXC] = {};
+    struct Frees { void** p; int n; ~Frees() { for (int i = 0; i < n; i++) td__arena_free(p[i]); } }
+        rfree{rbuf, RJ_MAXC};
+    for (int c = 0; c < L.ncols; c++)
+        if (!(ldst[c] = own.take((size_t)nlp * L.esz[c]))) return TD_ERR_OOM;
+    for (int c = 0; c < R.ncols; c++)
+        if (!(rbuf[c] = td__arena_alloc((size_t)(nrp > 0 ? nrp : 1) * R.esz[c])))
+            return TD_ERR_OOM;
+    const size_t tiles_cap = (size_t)(ceil_div(nlp > nrp ? nlp : nrp, RJ_STEP) + P + 8) * 16;
+    SCRATCH(loff, ((size_t)P + 1) * 8);
+    SCRATCH(roff, ((size_t)P + 1) * 8);
+    SCRATCH(lcur1, (size_t)P * RJ_XCD * 8);
+    SCRATCH(rcur1, (size_t)P * RJ_XCD * 8);
+    SCRATCH(lcur2, (size_t)P * 8);
+    SCRATCH(rcur2, (size_t)P * 8);
+    SCRATCH(ltiles, tiles_cap);
+    SCRATCH(rtiles, tiles_cap);
+    SCRATCH(lxt, (RJ_XCD + 1) * 4);
+    SCRATCH(rxt, (RJ_XCD + 1) * 4);
+    Scratch lmid(8), rmid(8), lkw(8), lkwm(8), rkw(8), rkwm(8), lrow(8), lrowm(8), rrow(8), rrowm(8);
+    rc = rj_partition(L, bits, ldst, loff, lcur1, lcur2, ltiles, lxt, lmid, lkw, lkwm, lrow, lrowm);
+    if (!rc)
+        rc = rj_partition(R, bits, rbuf, roff, rcur1, rcur2, rtiles, rxt, rmid, rkw, rkwm, rrow,
+                          rrowm);
+    if (rc) return rc;
+    POLL_CANCEL();
+
+    RJBP A;
+    memset(&A, 0, sizeof(A));
+    A.loff = loff.as<int64_t>();
+    A.roff = roff.as<int64_t>();
+    int plain = 0;
+    if (L.key_col >= 0) {
+        A.lkey = S->lcols[L.key_col];
+        A.lkey.data = L.dst[L.key_col];
+        plain = L.plain;
+    } else {
+        A.lkw = L.kw;
+    }
+    A.rkw = R.kw;
+    A.lrow = L.row;
+    A.rrow = R.row;
+    A.LK = L.K;
+    A.RK = R.K;
+    A.exact = exact;
+    A.left_join = J->join_type == 1;
+    A.nrc = R.ncols;
+    for (int c = 0; c < R.ncols; c++) {
+        A.rsrc[c] = rbuf[c];
+        A.resz[c] = R.esz[c];
+        if (!(A.rdst[c] = own.take((size_t)nlp * R.esz[c]))) return TD_ERR_OOM;
+    }
+    SCRATCH(flag, (size_t)(J->join_type == 0 ? nlp : 1));
+    if (J->join_type == 0) A.lflag = flag.as<uint8_t>();
+    SCRATCH(stat, 32);
+    rc = tdd_fill(stat.p, 0, 32);
+    if (rc) return rc;
+    A.stat = stat.as<unsigned long long>();
+    const size_t lds = (size_t)RJ_SLOTS * 4 + (size_t)RJ_CAP * 8;
+    if (plain == 8) LAUNCH("rjoin_build_probe", k_rj_bp<8>, P, RJ_T, lds, A);
+    else if (plain == 4) LAUNCH("rjoin_build_probe", k_rj_bp<4>, P, RJ_T, lds, A);
+    else LAUNCH("rjoin_build_probe", k_rj_bp<0>, P, RJ_T, lds, A);
+    unsigned long long st[3];
+    HC(hipMemcpyAsync(st, stat.p, sizeof(st), hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    if (st[1] || st[2]) return TD_ERR_NYI;      /* repeated build keys / oversized partition */
+
+    int64_t nout = J->join_type == 1 ? nlp : (int64_t)st[0];
+    void* lo[RJ_MAXC] = {};
+    void* ro[RJ_MAXC] = {};
+    for (int c = 0; c < L.ncols; c++) lo[c] = L.dst[c];
+    for (int c = 0; c < R.ncols; c++) ro[c] = A.rdst[c];
+    if (J->join_type == 0 && nout < nlp) {
+        /* inner join with probe rows that found no partner: compact the
+         * slotted outputs through the match flags */
+        const int64_t nseg = ceil_div(nlp, 1024), nw = ceil_div(nlp, 64);
+        SCRATCH(selmem, 16 + (size_t)((nseg + 7) & ~7) + (size_t)((2 * nseg + 7) & ~7) +
+                            (size_t)nw * 8);
+        tdd_sel_t sv;
+        sv.base = selmem.p;
+        sv.flags = (uint8_t*)selmem.p + 16;
+        sv.popcnt = (uint16_t*)((uint8_t*)sv.flags + ((nseg + 7) & ~7));
+        sv.bits = (uint64_t*)((uint8_t*)sv.popcnt + ((2 * nseg + 7) & ~7));
+        sv.nrows = nlp;
+        sv.n_segs = (uint32_t)nseg;
+        sv.pad = 0;
+        rc = tdd_sel_from_bool(flag.as<uint8_t>(), &sv);
+        if (rc) return rc;
+        tdd_col_t src[2 * RJ_MAXC];
+        void* dst[2 * RJ_MAXC];
+        int nc = 0;
+        memset(src, 0, sizeof(src));
+        for (int c = 0; c < L.ncols; c++, nc++) {
+            src[nc].data = L.dst[c];
+            src[nc].esz = L.esz[c];
+            if (!(dst[nc] = own.take((size_t)nout * L.esz[c]))) return TD_ERR_OOM;
+            lo[c] = dst[nc];
+        }
+        for (int c = 0; c < R.ncols; c++, nc++) {
+            src[nc].data = A.rdst[c];
+            src[nc].esz = R.esz[c];
+            if (!(dst[nc] = own.take((size_t)nout * R.esz[c]))) return TD_ERR_OOM;
+            ro[c] = dst[nc];
+        }
+        for (int c0 = 0; c0 < nc; c0 += RJ_MAXC) {
+            rc = tdd_sel_compact(&sv, nc - c0 < RJ_MAXC ? nc - c0 : RJ_MAXC, src + c0, dst + c0);
+            if (rc) return rc;
+        }
+    }
+    for (int c = 0; c < L.ncols; c++) { own.keep(lo[c]); out->lcols[c] = lo[c]; }
+    for (int c = 0; c < R.ncols; c++) { own.keep(ro[c]); out->rcols[c] = ro[c]; }
+    out->n_rows = nout;
+    out->path = 3;
+    out->radix_bits = bits;
+    return TD_OK;
+}

@@ -1259,6 +1259,60 @@ static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int
     return out;
 }
 
+/* INNER / LEFT joins through the materialising radix join: the output
+ * columns (left columns, then right non-key columns, 16 each at most,
+ * exec.c:8607-8728) come back written.  NULL when that join does not take
+ * the query (repeated or dense build keys, oversized partitions): the
+ * caller joins through row-index pairs instead. */
+static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* right,
+                               const tdd_join_spec_t* J) {
+    td_graph_t* g = x->g;
+    tdd_join_mat_spec_t M;
+    memset(&M, 0, sizeof(M));
+    M.join = *J;
+    td_t* lc[16];
+    td_t* rc_[16];
+    int64_t rname[16], lname[16];
+    int64_t lnc = td_table_ncols(left), rnc = td_table_ncols(right);
+    for (int64_t c = 0; c < lnc && M.n_lcols < 16; c++) {
+        td_t* col = td_table_get_col_idx(left, c);
+        if (!col) continue;
+        if (dev_col(x, col, &M.lcols[M.n_lcols])) return NULL;
+        lname[M.n_lcols] = td_table_col_name(left, c);
+        lc[M.n_lcols++] = col;
+    }
+    for (int64_t c = 0; c < rnc && M.n_rcols < 16; c++) {
+        td_t* col = td_table_get_col_idx(right, c);
+        int64_t name = td_table_col_name(right, c);
+        if (!col || join_is_right_key(g, je, name)) continue;
+        if (dev_col(x, col, &M.rcols[M.n_rcols])) return NULL;
+        rname[M.n_rcols] = name;
+        rc_[M.n_rcols++] = col;
+    }
+    tdd_join_mat_result_t R;
+    int rc = tdd_join_mat(&M, &R);
+    if (rc == TD_ERR_NYI) return NULL;
+    if (rc) return TD_ERR_PTR(rc);
+    td__set_plan("join path=%d radix_bits=%d rows=%lld", R.path, R.radix_bits,
+                 (long long)R.n_rows);
+    td_t* result = td_table_new(M.n_lcols + M.n_rcols);
+    for (int c = 0; c < M.n_lcols + M.n_rcols; c++) {
+        int right_side = c >= M.n_lcols;
+        int i = right_side ? c - M.n_lcols : c;
+        td_t* src = right_side ? rc_[i] : lc[i];
+        void* d = right_side ? R.rcols[i] : R.lcols[i];
+        if (TD_IS_ERR(result)) { td__arena_free(d); continue; }
+        const tdd_col_t* dc = right_side ? &M.rcols[i] : &M.lcols[i];
+        td_t* o = td__hbm_wrap(src->type, src->attrs & TD_SYM_W_MASK, R.n_rows, d,
+                               (size_t)(R.n_rows > 0 ? R.n_rows : 1) * dc->esz);
+        if (TD_IS_ERR(o)) { td_release(result); result = o; continue; }
+        int64_t name = right_side ? rname[i] : lname[i];
+        result = td_table_add_col(result, name, o);
+        td_release(o);
+    }
+    return result;
+}
+
 static td_t* exec_join(xctx_t* x, td_op_t* op) {
     td_graph_t* g = x->g;
     td_op_ext_t* je = td__find_ext(g, op->id);
@@ -1291,6 +1345,13 @@ static td_t* exec_join(xctx_t* x, td_op_t* op) {
         if (sel->len != nl) { result = TD_ERR_PTR(TD_ERR_LENGTH); goto done; }
         if ((rc = dev_sel(x, sel, &sv))) { result = TD_ERR_PTR(rc); goto done; }
         J.lsel = &sv;
+    }
+    if (J.join_type <= 1) {
+        result = join_materialised(x, je, left, right, &J);
+        if (result) {
+            if (sel) { td_release(g->selection); g->selection = NULL; }
+            goto done;
+        }
     }
     tdd_join_result_t R;
     rc = tdd_join(&J, &R);

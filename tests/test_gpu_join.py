@@ -357,3 +357,67 @@ def test_jg_tiled_shapes(eng, forced, shape, resident, jpath):
     assert [c[1] for c in out[1:]] == ref["out_types"]
     compare_groups([out[0][3].astype(np.int64)], [c[3] for c in out[1:]], ref["keys"],
                    ref["vals"], f64, shape, exact=ref["comp"], sum_abs=ref["sum_abs"])
+
+
+# ------------------------------------------- materialising radix join --
+
+@pytest.mark.parametrize("jt", [0, 1])
+@pytest.mark.parametrize("case", ["i64", "i32_key", "two_keys", "f64_key", "filtered",
+                                  "oversized", "sym16_key"])
+def test_join_materialised_vs_oracle(eng, forced, jt, case):
+    """INNER / LEFT joins on unique build keys through the materialising radix
+    join (plan path 3): output columns of every type written straight from
+    the partition passes, probe rows without a partner compacted away (inner)
+    or zero-filled (left), against the oracle's pairs gathered column by
+    column.  `oversized` forces 2 partitions of a 30,000-row build side, more
+    than one LDS table holds: the pair-producing join answers instead."""
+    rng = np.random.default_rng(900 + 10 * jt + len(case))
+    nl, nr = 300_007, 30_011
+    pool = rng.choice(np.arange(-2 ** 50, 2 ** 50, 2 ** 31 + 7), 2 * nr, replace=False)
+    rk = pool[:nr]
+    lk = pool[rng.integers(0, 2 * nr, nl)]                # about half the probes miss
+    lcols = [("k", lk), ("a", rng.integers(-9, 9, nl).astype(np.int32)),
+             ("b", rng.normal(size=nl)), ("c", rng.integers(0, 255, nl).astype(np.uint8)),
+             ("d", rng.integers(-300, 300, nl).astype(np.int16))]
+    rcols = [("k", rk), ("w", rng.integers(0, 10 ** 9, nr)), ("x", rng.random(nr)),
+             ("y", rng.integers(0, 60000, nr).astype(np.uint16), 20, 1)]
+    lkeys = rkeys = ["k"]
+    if case == "i32_key":
+        u = rng.permutation(2 ** 20)[:2 * nr].astype(np.int32) * 997
+        lcols[0] = ("k", u[rng.integers(0, 2 * nr, nl)])
+        rcols[0] = ("k", u[:nr])
+    elif case == "sym16_key":
+        u = rng.permutation(65536)[:2 * 20000].astype(np.uint16)
+        nr2 = 20000
+        rcols = [("k", u[:nr2], 20, 1), ("w", rng.integers(0, 10 ** 9, nr2))]
+        lcols[0] = ("k", u[rng.integers(0, 2 * nr2, nl)], 20, 1)
+    elif case == "two_keys":
+        lcols.append(("k2", (lk % 7).astype(np.int32)))
+        rcols.append(("k2", (rk % 7).astype(np.int32)))
+        lkeys = rkeys = ["k", "k2"]
+    elif case == "f64_key":
+        lcols[0] = ("k", lk.astype(np.float64) / 4.0)
+        rcols[0] = ("k", rk.astype(np.float64) / 4.0)
+    forced(2, 1 if case == "oversized" else 0)
+    lt, rt = eng.table(lcols), eng.table(rcols)
+    g = eng.graph(lt)
+    left = g.const_table(lt)
+    if case == "filtered":
+        left = g.filter(left, g.gt(g.scan("a"), g.const_i64(0)))
+    root = g.join(left, [g.scan(k) for k in lkeys], g.const_table(rt),
+                  [g.scan(k) for k in rkeys], jt)
+    out = eng.read_table(g.execute(root))
+    plan = eng.last_plan()
+    g.free()
+    eng.release(lt)
+    eng.release(rt)
+    assert ("path=3" in plan) == (case != "oversized"), plan
+    L = {c[0]: c[1] for c in lcols}
+    Rr = {c[0]: c[1] for c in rcols}
+    m = L["a"] > 0 if case == "filtered" else np.ones(nl, bool)
+    li, ri = orc.join([L[k][m] for k in lkeys], [Rr[k] for k in rkeys], jt)
+    ref = [orc.gather(c[1][m], li) for c in lcols]
+    ref += [orc.gather(c[1], ri, jt == 1) for c in rcols if c[0] not in rkeys]
+    assert [c[0] for c in out] == [c[0] for c in lcols] + [c[0] for c in rcols
+                                                          if c[0] not in rkeys]
+    row_multiset_equal([c[3] for c in out], ref, f"mat {case} {jt}")
