@@ -135,33 +135,38 @@ __global__ __launch_bounds__(RJ_T) void k_rj_hist(RJHist H) {
         if (hist[q]) atomicAdd(&H.C[(int64_t)q * RJ_XCD + x], hist[q]);
 }
 
-/* Write a column's tile rows in sorted order: sorted position i holds tile
- * row perm[i] and goes to gpos(i). */
+/* One carried column of a tile through LDS: the thread's rows read in
+ * natural order (coalesced), placed at their sorted positions, then each
+ * thread's sorted positions stored to their runs (consecutive lanes,
+ * consecutive addresses).  The caller's barrier separates the uses. */
 template <class T>
-__device__ __forceinline__ void rj_move(const void* src, void* dst, int64_t tb, int cnt,
-                                        const uint16_t* perm, const uint32_t* pos) {
+__device__ __forceinline__ void rj_stage(const T* src, T* dst, int64_t tb, uint32_t live,
+                                         const uint32_t (&at)[RJ_RPT], int cnt,
+                                         const uint32_t (&pos)[RJ_RPT], uint64_t* stage) {
     T v[RJ_RPT];
 #pragma unroll
-    for (int k = 0; k < RJ_RPT; k++) {
-        const int i = threadIdx.x + k * RJ_T;
-        if (i < cnt) v[k] = ((const T*)src)[tb + perm[i]];
-    }
+    for (int j = 0; j < RJ_RPT; j++)
+        if ((live >> j) & 1) v[j] = src[tb + j * RJ_T + threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < RJ_RPT; j++)
+        if ((live >> j) & 1) stage[at[j]] = (uint64_t)v[j];
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < RJ_RPT; k++) {
         const int i = threadIdx.x + k * RJ_T;
-        if (i < cnt) ((T*)dst)[pos[k]] = v[k];
+        if (i < cnt) dst[pos[k]] = (T)stage[i];
     }
+    __syncthreads();
 }
 
 /* One scatter pass.  Per tile of up to RJ_STEP consecutive rows: each row's
  * digit and LDS rank, one global reservation per digit of the tile in its
- * run, then the rows written in digit order so that consecutive lanes store
- * consecutive addresses of a run. */
+ * run, then every carried column staged through LDS into digit order and
+ * written so that consecutive lanes store consecutive addresses of a run. */
 __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
     __shared__ uint32_t hist[RJ_FMAX], loff[RJ_FMAX], gbase[RJ_FMAX];
-    __shared__ uint16_t perm[RJ_STEP];
     __shared__ uint8_t spart[RJ_STEP];
-    __shared__ uint64_t skw[RJ_STEP];
+    __shared__ uint64_t stage[RJ_STEP];
     __shared__ uint32_t s_hi;                    /* the tile's digits above this pass */
     const int F = 1 << S.dbits;
     const int x = blockIdx.x % RJ_XCD;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
     if (threadIdx.x == 0) s_hi = 0;             /* pass 1: no digits above; published by
                                                    the first tile's barrier */
     int64_t tb = 0, te = 0;
-    int64_t next = blockIdx.x;                  /* pass 1: span; pass 2: tile of this XCD */
+    int64_t next = blockIdx.x;                  /* pass 2: tile of this XCD */
     int32_t t_end = 0, t_step = 1;
     int64_t r_end = 0;
     if (S.pass2) {
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
         for (int i = threadIdx.x; i < F; i += RJ_T) hist[i] = 0;
         __syncthreads();
         uint64_t kw[RJ_RPT];
-        uint32_t dig[RJ_RPT], rank[RJ_RPT];
+        uint32_t dig[RJ_RPT], at[RJ_RPT];
         uint32_t live = 0;
 #pragma unroll
         for (int j = 0; j < RJ_RPT; j++) {
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
             if (!((live >> j) & 1)) continue;
             const uint64_t h = jmix(kw[j]);
             dig[j] = (uint32_t)(h >> (64 - S.dshift - S.dbits)) & (uint32_t)(F - 1);
-            rank[j] = atomicAdd(&hist[dig[j]], 1u);
+            at[j] = atomicAdd(&hist[dig[j]], 1u);
             /* pass 2: every row of a tile lies in one pass-1 partition */
             if (S.dshift && j == 0 && threadIdx.x == 0) s_hi = (uint32_t)(h >> (64 - S.dshift));
         }
@@ -250,10 +255,8 @@ __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
 #pragma unroll
         for (int j = 0; j < RJ_RPT; j++) {
             if (!((live >> j) & 1)) continue;
-            const uint32_t at = loff[dig[j]] + rank[j];
-            perm[at] = (uint16_t)(j * RJ_T + threadIdx.x);
-            spart[at] = (uint8_t)dig[j];
-            if (S.kw_dst) skw[at] = kw[j];
+            at[j] += loff[dig[j]];               /* sorted position in the tile */
+            spart[at[j]] = (uint8_t)dig[j];
         }
         __syncthreads();
         const int cnt = (int)(loff[F - 1] + hist[F - 1]);
@@ -265,30 +268,45 @@ __global__ __launch_bounds__(RJ_T) void k_rj_scatter(RJPass S) {
         }
         for (int c = 0; c < S.ncols; c++) {                 /* uniform across the block */
             switch (S.esz[c]) {
-            case 8: rj_move<uint64_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
-            case 4: rj_move<uint32_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
-            case 2: rj_move<uint16_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
-            default: rj_move<uint8_t>(S.src[c], S.dst[c], tb, cnt, perm, pos); break;
+            case 8: rj_stage<uint64_t>((const uint64_t*)S.src[c], (uint64_t*)S.dst[c], tb, live,
+                                       at, cnt, pos, stage); break;
+            case 4: rj_stage<uint32_t>((const uint32_t*)S.src[c], (uint32_t*)S.dst[c], tb, live,
+                                       at, cnt, pos, stage); break;
+            case 2: rj_stage<uint16_t>((const uint16_t*)S.src[c], (uint16_t*)S.dst[c], tb, live,
+                                       at, cnt, pos, stage); break;
+            default: rj_stage<uint8_t>((const uint8_t*)S.src[c], (uint8_t*)S.dst[c], tb, live,
+                                       at, cnt, pos, stage); break;
             }
         }
         if (S.kw_dst) {
 #pragma unroll
-            for (int k = 0; k < RJ_RPT; k++) {
-                const int i = threadIdx.x + k * RJ_T;
-                if (i < cnt) S.kw_dst[pos[k]] = skw[i];
-            }
-        }
-        if (S.row_dst) {
+            for (int j = 0; j < RJ_RPT; j++)
+                if ((live >> j) & 1) stage[at[j]] = kw[j];
+            __syncthreads();
 #pragma unroll
             for (int k = 0; k < RJ_RPT; k++) {
                 const int i = threadIdx.x + k * RJ_T;
-                if (i < cnt)
-                    S.row_dst[pos[k]] = S.row_src ? S.row_src[tb + perm[i]]
-                                                  : (uint32_t)(tb + perm[i]);
+                if (i < cnt) S.kw_dst[pos[k]] = stage[i];
+            }
+            __syncthreads();
+        }
+        if (S.row_dst) {
+            if (S.row_src)
+                rj_stage<uint32_t>(S.row_src, S.row_dst, tb, live, at, cnt, pos, stage);
+            else {
+#pragma unroll
+                for (int j = 0; j < RJ_RPT; j++)
+                    if ((live >> j) & 1) stage[at[j]] = (uint64_t)(tb + j * RJ_T + threadIdx.x);
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < RJ_RPT; k++) {
+                    const int i = threadIdx.x + k * RJ_T;
+                    if (i < cnt) S.row_dst[pos[k]] = (uint32_t)stage[i];
+                }
+                __syncthreads();
             }
         }
         if (!S.pass2) tb = te;
-        __syncthreads();                         /* LDS reused by the next tile */
     }
 }
 
@@ -312,6 +330,7 @@ struct RJBP {
 };
 
 #define RJ_PB 4                  /* probe records per thread in flight */
+#define RJ_BT 1024               /* threads per build + probe workgroup */
 
 /* Payload of the build rows hit[] into a right output column at the probe
  * records i0 + j * RJ_T (zero for a LEFT row without a partner). */
@@ -323,17 +342,53 @@ __device__ __forceinline__ void rj_fetch(const void* src, void* dst, int64_t i0,
     for (int j = 0; j < RJ_PB; j++) v[j] = hit[j] >= 0 ? ((const T*)src)[hit[j]] : (T)0;
 #pragma unroll
     for (int j = 0; j < RJ_PB; j++) {
-        const int64_t i = i0 + j * RJ_T;
+        const int64_t i = i0 + j * RJ_BT;
         if (i < pe && (hit[j] >= 0 || left_join)) ((T*)dst)[i] = v[j];
     }
 }
 
+/* The first right column, from its LDS copy: build-local rows hl[] (-1: no
+ * partner) into the output at the probe records i0 + j * RJ_BT. */
+template <class T>
+__device__ __forceinline__ void rj_put_lds(void* dst, int64_t i0, int64_t pe,
+                                           const int32_t (&hl)[RJ_PB], const uint64_t* pay,
+                                           bool left_join) {
+#pragma unroll
+    for (int j = 0; j < RJ_PB; j++) {
+        const int64_t i = i0 + j * RJ_BT;
+        if (i < pe && (hl[j] >= 0 || left_join)) ((T*)dst)[i] = hl[j] >= 0 ? (T)pay[hl[j]] : (T)0;
+    }
+}
+
+#define RJ_SLOT_EMPTY 0xffffu
+
+/* 16-bit slot s of the packed table (two slots per word). */
+__device__ __forceinline__ uint32_t rj_slot_get(const uint32_t* tab, uint32_t s) {
+    return (tab[s >> 1] >> ((s & 1) * 16)) & 0xffffu;
+}
+
+/* Claim slot s for build row li when it is empty; returns the occupant
+ * otherwise (RJ_SLOT_EMPTY when the claim succeeded). */
+__device__ __forceinline__ uint32_t rj_slot_claim(uint32_t* tab, uint32_t s, uint32_t li) {
+    const uint32_t sh = (s & 1) * 16;
+    uint32_t old = tab[s >> 1];
+    for (;;) {
+        const uint32_t cur = (old >> sh) & 0xffffu;
+        if (cur != RJ_SLOT_EMPTY) return cur;
+        const uint32_t nw = (old & ~(0xffffu << sh)) | (li << sh);
+        const uint32_t prev = atomicCAS(&tab[s >> 1], old, nw);
+        if (prev == old) return RJ_SLOT_EMPTY;
+        old = prev;                              /* the other half changed: retry */
+    }
+}
+
 template <int PLAIN>
-__global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
+__global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
     extern __shared__ uint8_t sm[];
-    uint32_t* slot = (uint32_t*)sm;                            /* RJ_SLOTS */
-    uint64_t* tkey = (uint64_t*)(sm + RJ_SLOTS * 4);           /* RJ_CAP */
-    __shared__ uint32_t wcnt[RJ_T / WAVE];
+    uint32_t* slot = (uint32_t*)sm;                                 /* RJ_SLOTS / 2 words */
+    uint64_t* tkey = (uint64_t*)(sm + RJ_SLOTS * 2);                /* RJ_CAP */
+    uint64_t* pay = (uint64_t*)(sm + RJ_SLOTS * 2 + RJ_CAP * 8);    /* RJ_CAP: right column 0 */
+    __shared__ uint32_t wcnt[RJ_BT / WAVE];
     const int p = blockIdx.x;
     const int64_t bs = A.roff[p], be = A.roff[p + 1];
     const int64_t ps = A.loff[p], pe = A.loff[p + 1];
@@ -341,16 +396,21 @@ __global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
         if (threadIdx.x == 0) atomicAdd(&A.stat[2], 1ull);
         return;
     }
-    for (int i = threadIdx.x; i < RJ_SLOTS; i += RJ_T) slot[i] = RJ_EMPTY;
+    for (int i = threadIdx.x; i < RJ_SLOTS / 2; i += RJ_BT) slot[i] = 0xffffffffu;
     __syncthreads();
-    for (int64_t i = bs + threadIdx.x; i < be; i += RJ_T) {
+    const int e0 = A.nrc ? A.resz[0] : 0;
+    for (int64_t i = bs + threadIdx.x; i < be; i += RJ_BT) {
         const uint64_t k = A.rkw[i];
         const uint32_t li = (uint32_t)(i - bs);
         tkey[li] = k;                            /* before the slot is published */
+        if (e0 == 8) pay[li] = ((const uint64_t*)A.rsrc[0])[i];
+        else if (e0 == 4) pay[li] = ((const uint32_t*)A.rsrc[0])[i];
+        else if (e0 == 2) pay[li] = ((const uint16_t*)A.rsrc[0])[i];
+        else if (e0 == 1) pay[li] = ((const uint8_t*)A.rsrc[0])[i];
         uint32_t s = rj_slot(k);
         for (;;) {
-            const uint32_t prev = atomicCAS(&slot[s], RJ_EMPTY, li);
-            if (prev == RJ_EMPTY) break;
+            const uint32_t prev = rj_slot_claim(slot, s, li);
+            if (prev == RJ_SLOT_EMPTY) break;
             if (tkey[prev] == k &&
                 (A.exact || keys_equal(A.RK, A.rrow[bs + prev], A.RK, A.rrow[i])))
                 A.stat[1] = 1;                   /* a repeated build key */
@@ -359,12 +419,13 @@ __global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
     }
     __syncthreads();
     uint32_t hits = 0;
-    for (int64_t i0 = ps + threadIdx.x; i0 < pe; i0 += RJ_PB * RJ_T) {
+    for (int64_t i0 = ps + threadIdx.x; i0 < pe; i0 += RJ_PB * RJ_BT) {
         uint64_t k[RJ_PB];
         int64_t hit[RJ_PB];
+        int32_t hl[RJ_PB];
 #pragma unroll
         for (int j = 0; j < RJ_PB; j++) {           /* every key load first */
-            const int64_t i = i0 + j * RJ_T;
+            const int64_t i = i0 + j * RJ_BT;
             const int64_t ii = i < pe ? i : ps;
             if (PLAIN == 8) k[j] = (uint64_t)((const int64_t*)A.lkey.data)[ii];
             else if (PLAIN == 4)
@@ -372,25 +433,44 @@ __global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
                                              : (uint64_t)(int64_t)((const int32_t*)A.lkey.data)[ii];
             else k[j] = A.lkw ? A.lkw[ii] : (uint64_t)ld_i64(A.lkey, ii);
         }
+        /* the RJ_PB probes advance together, so their LDS reads overlap */
+        uint32_t sl[RJ_PB], live = 0;
 #pragma unroll
         for (int j = 0; j < RJ_PB; j++) {
-            const int64_t i = i0 + j * RJ_T;
             hit[j] = -1;
-            if (i >= pe) continue;
-            uint32_t s = rj_slot(k[j]);
-            for (;;) {
-                const uint32_t li = slot[s];
-                if (li == RJ_EMPTY) break;
-                if (tkey[li] == k[j] &&
-                    (A.exact || keys_equal(A.LK, A.lrow[i], A.RK, A.rrow[bs + li]))) {
-                    hit[j] = bs + li;
-                    break;
-                }
-                s = (s + 1) & (RJ_SLOTS - 1);
-            }
-            hits += hit[j] >= 0;
+            hl[j] = -1;
+            sl[j] = rj_slot(k[j]);
+            if (i0 + j * RJ_BT < pe) live |= 1u << j;
         }
-        for (int c = 0; c < A.nrc; c++) {                   /* uniform across the block */
+        while (live) {
+            uint32_t li[RJ_PB];
+#pragma unroll
+            for (int j = 0; j < RJ_PB; j++) li[j] = (live >> j) & 1 ? rj_slot_get(slot, sl[j]) : 0u;
+#pragma unroll
+            for (int j = 0; j < RJ_PB; j++) {
+                if (!((live >> j) & 1)) continue;
+                if (li[j] == RJ_SLOT_EMPTY) { live &= ~(1u << j); continue; }
+                if (tkey[li[j]] == k[j] &&
+                    (A.exact || keys_equal(A.LK, A.lrow[i0 + j * RJ_BT], A.RK,
+                                           A.rrow[bs + li[j]]))) {
+                    hit[j] = bs + li[j];
+                    hl[j] = (int32_t)li[j];
+                    live &= ~(1u << j);
+                    continue;
+                }
+                sl[j] = (sl[j] + 1) & (RJ_SLOTS - 1);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RJ_PB; j++) hits += hit[j] >= 0;
+        switch (e0) {                                       /* uniform across the block */
+        case 8: rj_put_lds<uint64_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
+        case 4: rj_put_lds<uint32_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
+        case 2: rj_put_lds<uint16_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
+        case 1: rj_put_lds<uint8_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
+        default: break;
+        }
+        for (int c = 1; c < A.nrc; c++) {                   /* uniform across the block */
             switch (A.resz[c]) {
             case 8: rj_fetch<uint64_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
             case 4: rj_fetch<uint32_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
@@ -401,7 +481,7 @@ __global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
         if (A.lflag) {
 #pragma unroll
             for (int j = 0; j < RJ_PB; j++)
-                if (i0 + j * RJ_T < pe) A.lflag[i0 + j * RJ_T] = hit[j] >= 0;
+                if (i0 + j * RJ_BT < pe) A.lflag[i0 + j * RJ_BT] = hit[j] >= 0;
         }
     }
     for (int d = WAVE / 2; d >= 1; d >>= 1) hits += __shfl_xor(hits, d);
@@ -409,7 +489,7 @@ __global__ __launch_bounds__(RJ_T) void k_rj_bp(RJBP A) {
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int w = 0; w < RJ_T / WAVE; w++) t += wcnt[w];
+        for (int w = 0; w < RJ_BT / WAVE; w++) t += wcnt[w];
         if (t) atomicAdd(&A.stat[0], t);
     }
 }
@@ -761,10 +841,10 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
     rc = tdd_fill(stat.p, 0, 32);
     if (rc) return rc;
     A.stat = stat.as<unsigned long long>();
-    const size_t lds = (size_t)RJ_SLOTS * 4 + (size_t)RJ_CAP * 8;
-    if (plain == 8) LAUNCH("rjoin_build_probe", k_rj_bp<8>, P, RJ_T, lds, A);
-    else if (plain == 4) LAUNCH("rjoin_build_probe", k_rj_bp<4>, P, RJ_T, lds, A);
-    else LAUNCH("rjoin_build_probe", k_rj_bp<0>, P, RJ_T, lds, A);
+    const size_t lds = (size_t)RJ_SLOTS * 2 + (size_t)RJ_CAP * 16;
+    if (plain == 8) LAUNCH("rjoin_build_probe", k_rj_bp<8>, P, RJ_BT, lds, A);
+    else if (plain == 4) LAUNCH("rjoin_build_probe", k_rj_bp<4>, P, RJ_BT, lds, A);
+    else LAUNCH("rjoin_build_probe", k_rj_bp<0>, P, RJ_BT, lds, A);
     unsigned long long st[3];
     HC(hipMemcpyAsync(st, stat.p, sizeof(st), hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
