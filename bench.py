@@ -183,6 +183,55 @@ class JoinGroup:
         self.e.release(self.td)
 
 
+class JoinGroupScattered(JoinGroup):
+    """The join + group-by workload with its join keys spread over 64-bit
+    values (fk, pk -> key * 7,000,000,019 - 2^62: still unique, no longer a
+    dense range), so the join runs radix partitioned: the materialising
+    join writes the filtered rows' grouped columns (price, qty, cat), which
+    are then grouped."""
+    name = "jgs"
+
+    def __init__(self, e, a):
+        from teide_amd import workloads as W
+        self.e, self.a = e, a
+        t0 = time.perf_counter()
+        fact = W.join_fact(a.fact, a.dim, SEED)
+        dim = W.join_dim(a.dim, a.cats, SEED)
+        spread = lambda v: v * 7_000_000_019 - (1 << 62)
+        hf = e.table([("fk", spread(fact["fk"])), ("qty", fact["qty"]), ("price", fact["price"])])
+        self.sel = int((fact["qty"] > 50).sum())
+        del fact
+        hd = e.table([("pk", spread(dim["pk"])), ("cat", dim["cat"])])
+        del dim
+        self.tf, self.td = e.put(hf), e.put(hd)
+        e.L.td_dev_sync()
+        e.release(hf)
+        e.release(hd)
+        print(f"# data: gen + h2d {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        self.rows = a.fact
+        self.query_bytes = a.fact * 24 + a.dim * 12 + a.cats * 28
+        self.config = {"workload": f"filter qty>50 -> join fact {a.fact:,} x dim {a.dim:,} on "
+                                   f"fk=pk with keys scattered over 64-bit values -> group by "
+                                   f"cat ({a.cats} cats): SUM price, SUM qty, COUNT"}
+
+    def kernel_bytes(self, name):
+        n, m, s = self.a.fact, self.a.dim, self.sel
+        if name == "rjoin_scatter":
+            # per launch on average over 2 passes x 2 sides: fact pass 1 reads
+            # fk of every row and qty, price of the passing rows and writes the
+            # key word, qty, price of those; pass 2 reads and writes the 24 B;
+            # dim passes read pk + cat, write key word + cat, then 12 B each way
+            return (n * 8 + s * 16 + s * 24 + s * 48 + m * 24 + m * 24) // 4
+        if name == "rjoin_build_probe":
+            return m * 12 + s * (8 + 4 + 1)    # build key words + cat; probe key, cat out, flag
+        if name == "rjoin_hist":
+            return (n * 8 + m * 8) // 2
+        return None
+
+    def cpu_baseline(self):
+        return None
+
+
 HC_AGGS = [("SUM", "v_i64"), ("MIN", "v_i64"), ("MAX", "v_i64"), ("SUM", "v_f64"),
            ("MIN", "v_f64"), ("MAX", "v_f64"), ("COUNT", "v_i64")]
 
@@ -818,8 +867,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "hc", "batch", "served", "load", "charge", "join",
-                                           "parted"],
+    ap.add_argument("--workload", choices=["jg", "jgs", "hc", "batch", "served", "load", "charge",
+                                           "join", "parted"],
                     default="jg",
                     help="jg: join + group-by (default); hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
@@ -898,7 +947,7 @@ def main():
         print(json.dumps(line), flush=True)
         return
 
-    wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge,
+    wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge, "jgs": JoinGroupScattered,
           "parted": Parted, "join": Join}.get(a.workload, JoinGroup)(e, a)
 
     for _ in range(a.warmup):

@@ -266,6 +266,8 @@ typedef struct {
     uint8_t           agg_side[TDD_MAX_AGGS];
 } tdd_join_group_spec_t;
 
+/* out->path is the group-by's path, plus 10 when the join ran as the
+ * materialising radix join of only the grouped columns before grouping. */
 int tdd_join_group(const tdd_join_group_spec_t* spec, tdd_group_result_t* out);
 
 /* Batched fused filter -> join -> group-by: many queries of one output

@@ -1188,6 +1188,65 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
 
 /* General form on the device: filter to a selection, join to index pairs,
  * gather the key and aggregate columns of the joined rows, group them. */
+/* Join -> group-by through the materialising radix join: the join writes
+ * only the columns the group-by reads (each distinct column once), which
+ * are then grouped; no pair arrays, no gathers.  TD_ERR_NYI when that join
+ * does not take the query (the caller joins through pairs). */
+static int jg_materialised(const tdd_join_group_spec_t* S, const tdd_join_spec_t* J,
+                           tdd_group_result_t* out) {
+    if (J->join_type != 0) return TD_ERR_NYI;
+    tdd_join_mat_spec_t M;
+    memset(&M, 0, sizeof(M));
+    M.join = *J;
+    int which[TDD_MAX_KEYS + TDD_MAX_AGGS];            /* side * 16 + column, -1: none */
+    for (int i = 0; i < S->n_keys + S->n_aggs; i++) {
+        const bool is_key = i < S->n_keys;
+        const tdd_col_t& c = is_key ? S->keys[i] : S->aggs[i - S->n_keys];
+        const int side = is_key ? S->key_side[i] : S->agg_side[i - S->n_keys];
+        which[i] = -1;
+        if (!c.data) continue;
+        tdd_col_t* cols = side ? M.rcols : M.lcols;
+        int32_t* n = side ? &M.n_rcols : &M.n_lcols;
+        int at = -1;
+        for (int j = 0; j < *n; j++)
+            if (cols[j].data == c.data && cols[j].type == c.type && cols[j].attrs == c.attrs) at = j;
+        if (at < 0) {
+            if (*n >= 16) return TD_ERR_NYI;
+            cols[*n] = c;
+            at = (*n)++;
+        }
+        which[i] = side * 16 + at;
+    }
+    tdd_join_mat_result_t R;
+    int rc = tdd_join_mat(&M, &R);
+    if (rc) return rc;
+    struct Free {
+        tdd_join_mat_result_t* r; int nl, nr;
+        ~Free() {
+            for (int c = 0; c < nl; c++) td__arena_free(r->lcols[c]);
+            for (int c = 0; c < nr; c++) td__arena_free(r->rcols[c]);
+        }
+    } fr{&R, M.n_lcols, M.n_rcols};
+    tdd_group_spec_t G;
+    memset(&G, 0, sizeof(G));
+    G.nrows = R.n_rows;
+    G.n_keys = S->n_keys;
+    G.n_aggs = S->n_aggs;
+    for (int i = 0; i < S->n_keys + S->n_aggs; i++) {
+        const bool is_key = i < S->n_keys;
+        tdd_col_t c = is_key ? S->keys[i] : S->aggs[i - S->n_keys];
+        if (which[i] >= 0) {
+            /* the source's bounds still hold for its joined copy */
+            c.data = which[i] >= 16 ? R.rcols[which[i] - 16] : R.lcols[which[i]];
+        }
+        if (is_key) G.keys[i] = c;
+        else { G.aggs[i - S->n_keys] = c; G.ops[i - S->n_keys] = S->ops[i - S->n_keys]; }
+    }
+    rc = tdd_group(&G, out);
+    if (!rc) out->path += 10;                    /* reported: materialised join, then group */
+    return rc;
+}
+
 static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
     tdd_join_spec_t J = S->join;
     int64_t n_segs = (J.nl + 1023) / 1024;
@@ -1208,8 +1267,11 @@ static int jg_compose(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
         if (rc) return rc;
         J.lsel = &sel;
     }
+    int rc = jg_materialised(S, &J, out);
+    if (rc != TD_ERR_NYI) return rc;
+    memset(out, 0, sizeof(*out));
     tdd_join_result_t jr;
-    int rc = tdd_join(&J, &jr);
+    rc = tdd_join(&J, &jr);
     if (rc) return rc;
     struct PairFree { tdd_join_result_t* r; ~PairFree() { td__arena_free(r->l_idx); td__arena_free(r->r_idx); } } pf{&jr};
     tdd_group_spec_t G;

@@ -239,12 +239,16 @@ def test_jg_golden(eng, forced, case, form, path, resident):
 
 
 @pytest.mark.parametrize("variant", ["dup_dim", "agg_on_dim", "key_on_fact", "left_join",
-                                     "zipf_fk"])
+                                     "zipf_fk", "scattered", "scattered_dim_aggs",
+                                     "scattered_dup"])
 @pytest.mark.parametrize("jpath", [0, 3, 4])
 def test_join_group_general(eng, forced, variant, jpath):
     """Fused JOIN -> GROUP shapes off the one-index fast path: duplicate
-    build keys, aggregates over build columns, keys on the probe side and a
-    left join (unmatched rows carry zero-filled build columns)."""
+    build keys, aggregates over build columns, keys on the probe side, a
+    left join (unmatched rows carry zero-filled build columns), and build
+    keys scattered over 64-bit values (the key range is no direct map: the
+    join runs radix partitioned into only the grouped columns, or through
+    row pairs when build keys repeat)."""
     rng = np.random.default_rng(5)
     nf, nd = 300000, (200000 if variant == "zipf_fk" else 20000)
     fk = rng.integers(0, nd + 500, nf)
@@ -257,6 +261,11 @@ def test_join_group_general(eng, forced, variant, jpath):
     pk = rng.permutation(nd)
     if variant == "dup_dim":
         pk = pk % (nd // 2)
+    if variant.startswith("scattered"):
+        spread = lambda v: v * 7_000_000_019 - 2 ** 62         # unique, 64-bit range
+        if variant == "scattered_dup":
+            pk = pk % (nd // 2)
+        fk, pk = spread(fk), spread(pk)
     cat = rng.integers(0, 77, nd).astype(np.int32)
     wgt = rng.normal(size=nd)
     tf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
@@ -264,7 +273,7 @@ def test_join_group_general(eng, forced, variant, jpath):
     jt = 1 if variant == "left_join" else 0
     if variant == "key_on_fact":
         keys, aggs = ["qty"], [("SUM", "price"), ("MAX", "cat"), ("COUNT", "price")]
-    elif variant == "agg_on_dim":
+    elif variant in ("agg_on_dim", "scattered_dim_aggs"):
         keys, aggs = ["cat"], [("SUM", "wgt"), ("MIN", "qty"), ("AVG", "price")]
     else:
         keys, aggs = ["cat"], [("SUM", "price"), ("SUM", "qty"), ("COUNT", "price")]
@@ -274,7 +283,10 @@ def test_join_group_general(eng, forced, variant, jpath):
     g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], jt)
     root = g.group([g.scan(k) for k in keys], [(o, g.scan(c)) for o, c in aggs])
     out = eng.read_table(g.execute(root))
+    plan = eng.last_plan()
     g.free()
+    if variant in ("scattered", "scattered_dim_aggs") and jpath == 0:
+        assert "join_group path=1" in plan and "path=1 " not in plan, plan   # 11..15
     # oracle composition
     m = qty > 50
     f = {"fk": fk[m], "qty": qty[m], "price": price[m]}
