@@ -169,6 +169,15 @@ typedef struct {
  * *mn > *mx when no row is selected. */
 int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
                int64_t* mn, int64_t* mx);
+/* tdd_minmax with no device allocation at all (no hipMalloc / hipFree, no
+ * arena block): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation.  No
+ * counterpart in the reference.  Its partial results live in a static
+ * device array of 2 + 2 * 2048 int64 words per GPU; TD_ERR_RANGE when a
+ * call would need more than 2048 partial pairs (more than 512 CUs).  Calls
+ * that overlap on one GPU race on that array. */
+int tdd_minmax_noalloc(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
+                       int64_t* mn, int64_t* mx);
 /* Group-by with COUNT / SUM / AVG / MIN / MAX / FIRST / LAST / VAR family.
  * Integer keys whose masked ranges multiply to a modest slot count use a
  * direct array (LDS-private per workgroup when it fits, else global atomics),

@@ -186,17 +186,12 @@ static RowMask row_mask(const tdd_sel_t* s, int64_t nrows) {
     return M;
 }
 
-extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
-                          int64_t* mn, int64_t* mx) {
-    if (col->ranged && !mask && nrows > 0) {          /* known column statistics */
-        *mn = col->lo;
-        *mx = col->hi;
-        return TD_OK;
-    }
-    int grid = stream_grid(ceil_div(nrows > 0 ? nrows : 1, MM_T * MM_R * 2), 4);
-    SCRATCH(buf, 16 + (size_t)grid * 16);
-    long long* out = buf.as<long long>();
-    long long* part = out + 2;
+/* The partial min / max pairs of one launch plus the folded pair: buf holds
+ * 2 + 2 * grid int64 words (folded pair first). */
+static int minmax_run(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask, int grid,
+                      long long* buf, int64_t* mn, int64_t* mx) {
+    long long* out = buf;
+    long long* part = buf + 2;
     LAUNCH("minmax", k_minmax, grid, MM_T, 0, *col, (int)tile_vec_ok(*col),
            row_mask(mask, nrows), part);
     LAUNCH("minmax", k_minmax_final, 1, MM_T, 0, (const long long*)part, grid, out);
@@ -206,6 +201,67 @@ extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* 
     *mn = h[0];
     *mx = h[1];
     return TD_OK;
+}
+
+static int minmax_grid(int64_t nrows) {
+    return stream_grid(ceil_div(nrows > 0 ? nrows : 1, MM_T * MM_R * 2), 4);
+}
+
+extern "C" int tdd_minmax(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
+                          int64_t* mn, int64_t* mx) {
+    if (col->ranged && !mask && nrows > 0) {          /* known column statistics */
+        *mn = col->lo;
+        *mx = col->hi;
+        return TD_OK;
+    }
+    const int grid = minmax_grid(nrows);
+    SCRATCH(buf, 16 + (size_t)grid * 16);
+    return minmax_run(col, nrows, mask, grid, buf.as<long long>(), mn, mx);
+}
+
+/* ------------------------------------------------ allocation-free min/max -- */
+/* tdd_minmax without any device allocation: its partial pairs live in a
+ * file-scope device array, one per GPU, looked up once per device.  Every
+ * call on a GPU reuses that GPU's array (calls that overlap on one GPU
+ * race); each launch writes every partial it folds, so nothing needs
+ * clearing between calls. */
+#define MM_NOALLOC_PARTS 2048                 /* partial pairs held: 32 KiB + 16 B */
+#define MM_MAX_DEVICES 64
+
+static __device__ __align__(256) long long scratch[2 + 2 * MM_NOALLOC_PARTS];
+static long long* g_scratch_dev[MM_MAX_DEVICES];   /* device address of scratch, per GPU */
+
+static int noalloc_scratch(long long** out) {
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MM_MAX_DEVICES) return TD_ERR_RANGE;
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return TD_ERR_IO;
+        g_scratch_dev[dev] = (long long*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return TD_OK;
+}
+
+extern "C" int tdd_minmax_noalloc(const tdd_col_t* col, int64_t nrows, const tdd_sel_t* mask,
+                                  int64_t* mn, int64_t* mx) {
+    if (col->ranged && !mask && nrows > 0) {
+        *mn = col->lo;
+        *mx = col->hi;
+        return TD_OK;
+    }
+    const int grid = minmax_grid(nrows);
+    if (grid > MM_NOALLOC_PARTS) {
+        fprintf(stderr, "tdd_minmax_noalloc: %d partial pairs needed, the static scratch "
+                        "holds %d\n", grid, MM_NOALLOC_PARTS);
+        return TD_ERR_RANGE;
+    }
+    long long* buf = nullptr;
+    int rc = noalloc_scratch(&buf);
+    if (rc) return rc;
+    return minmax_run(col, nrows, mask, grid, buf, mn, mx);
 }
 
 /* --------------------------------------------------------- direct array -- */
