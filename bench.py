@@ -37,7 +37,7 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E, GB/s
 N_FACT, N_DIM, N_CAT, SEED = 200_000_000, 16_000_000, 1000, 42
-CPU_SAMPLE = (50_000_000, 4_000_000)       # reference CPU path sample (rows)
+CPU_SAMPLE = (200_000_000, 16_000_000)     # reference CPU path sample: the full C2 size
 
 
 def kernel_bytes(name, nf, nd, ncat):
@@ -55,11 +55,33 @@ def kernel_bytes(name, nf, nd, ncat):
 
 
 def cpu_threads():
+    """Threads for the reference's CPU path: the reference's own default
+    (td_pool_init(0): nproc - 1 workers + the caller) applied to the host
+    cores this process may use, capped at the 16-core share a GPU box
+    allots per GPU."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
     return max(2, min(16, n))
+
+
+def host_info():
+    """nproc, the cores this process may use and the CPU model."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": aff, "cpu_model": model}
 
 
 def cpu_baseline_reference(nf, nd, ncat, reps=5):
@@ -71,11 +93,11 @@ def cpu_baseline_reference(nf, nd, ncat, reps=5):
     out = subprocess.run([exe, "bench", "jg", str(nf), str(nd), str(ncat), str(reps)],
                          check=True, capture_output=True, text=True, env=env, timeout=600)
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": nf / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
-            "kind": "reference",
-            "sample": f"fact {nf:,} x dim {nd:,} rows, {ncat} cats, same distributions and DAG "
-                      f"(join graph + group graph), median of {reps} after 1 warm-up, "
-                      f"td_pool_init({r['threads'] - 1}) workers + caller"}
+    return dict({"value": nf / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
+                 "kind": "reference", "median_ms": r["median_ms"],
+                 "sample": f"fact {nf:,} x dim {nd:,} rows, {ncat} cats, same distributions and "
+                           f"DAG (join graph + group graph), median of {reps} after 1 warm-up, "
+                           f"td_pool_init({r['threads'] - 1}) workers + caller"}, **host_info())
 
 
 def cpu_baseline_port(nf, nd, ncat):
@@ -276,6 +298,15 @@ class HighCard:
         self.e.release(self._group("sym"))
 
     def kernel_bytes(self, name):
+        n = self.a.hc_rows
+        if name == "group_pg_pass":
+            # per launch on average over the step's four scatter passes: the
+            # key group-by's two passes read and write 24-byte records (the
+            # first reads the rows: key, v_i64, v_f64), the symbol
+            # group-by's two read and write 20-byte ones (sym + the values)
+            return n * (48 + 48 + 40 + 40) // 4
+        if name == "group_pg_hagg":
+            return n * 24                      # one read of the key records
         return None
 
     def cpu_baseline(self):
@@ -289,10 +320,12 @@ class HighCard:
         out = subprocess.run([exe, "bench", "hc", str(n), str(nv), str(ns), "3"], check=True,
                              capture_output=True, text=True, env=env, timeout=900)
         r = json.loads(out.stdout.strip().splitlines()[-1])
-        return {"value": 2 * n / (r["median_ms"] / 1e3), "unit": "rows/s", "cores": r["threads"],
-                "kind": "reference",
-                "sample": f"{n:,} rows, {nv:,} zipf values, {ns:,} symbols, both group-bys, "
-                          f"median of 3 after 1 warm-up"}
+        return dict({"value": 2 * n / (r["median_ms"] / 1e3), "unit": "rows/s",
+                     "cores": r["threads"], "kind": "reference",
+                     "sample": f"{n:,} rows, {nv:,} zipf values, {ns:,} symbols, both "
+                               f"group-bys, median of 3 after 1 warm-up, "
+                               f"td_pool_init({r['threads'] - 1}) workers + caller"},
+                    **host_info())
 
     def release(self):
         self.e.release(self.t)
@@ -818,6 +851,89 @@ def load_summary(e, a):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def measure(e, wl, steps, warmup, prof=True):
+    """Warm up, then time `steps` steps with per-kernel HIP events; the
+    roofline prices the dominant kernel's algorithmic bytes (the
+    workload's kernel_bytes) against its average event duration."""
+    for _ in range(warmup):
+        wl.step()
+    plan = e.last_plan()
+    e.L.td_dev_sync()
+    e.prof_reset()
+    e.prof(prof)
+    e.L.td_dev_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wl.step()
+    e.L.td_dev_sync()
+    t1 = time.perf_counter()
+    pr = e.prof_read()
+    e.prof(False)
+    ms = (t1 - t0) * 1e3 / steps
+    roofline = None
+    kernels = {k: {"launches": n, "avg_ms": round(tot / max(n, 1), 5)} for k, (n, tot) in pr.items()}
+    if pr:
+        dom = max(pr, key=lambda k: pr[k][1])
+        n, tot = pr[dom]
+        avg_ms = tot / n
+        nbytes = wl.kernel_bytes(dom)
+        if nbytes:
+            ach = nbytes / (avg_ms / 1e3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": load_traffic(dom), "kernel": dom,
+                        "algorithmic_bytes": nbytes, "avg_ms": round(avg_ms, 5)}
+    return {"ms": ms, "plan": plan, "kernels": kernels, "roofline": roofline}
+
+
+def two_call_summary(e, wl, steps=5, warmup=2):
+    """The reference's own form of the join + group-by, timed on the GPU
+    beside the fused DAG: graph 1 runs JOIN(FILTER(fact, qty > 50), fk =
+    pk) to a joined table in HBM, graph 2 runs GROUP(cat; SUM price,
+    SUM qty, COUNT price) over it (bench_queries.c's shape; the CPU
+    baseline times exactly this)."""
+    def step():
+        g = e.graph(wl.tf)
+        flt = g.filter(g.const_table(wl.tf), g.gt(g.scan("qty"), g.const_i64(50)))
+        root = g.join(flt, [g.scan("fk")], g.const_table(wl.td), [g.scan("pk")], 0)
+        j = g.execute(root)
+        g.free()
+        g2 = e.graph(j)
+        r = g2.group([g2.scan("cat")], [("SUM", g2.scan("price")), ("SUM", g2.scan("qty")),
+                                         ("COUNT", g2.scan("price"))])
+        e.release(g2.execute(r))
+        g2.free()
+        e.release(j)
+
+    class TwoCall:
+        def step(self):
+            step()
+
+        def kernel_bytes(self, name):
+            return None
+
+    m = measure(e, TwoCall(), steps, warmup)
+    return {"workload": "the same query as the reference runs it: join graph, then group "
+                        "graph over the joined table (both on the GPU, resident)",
+            "ms_per_query": round(m["ms"], 4), "rows_per_s": wl.rows / (m["ms"] / 1e3),
+            "fused_vs_two_call": None, "kernels": m["kernels"]}
+
+
+def hc_summary(e, a, steps=5, warmup=2, cpu=True):
+    """The high-cardinality group-by (300 M rows, by key and by sym) with
+    its own roofline and CPU baseline."""
+    wl = HighCard(e, a)
+    m = measure(e, wl, steps, warmup)
+    out = {"workload": wl.config["workload"], "ms_per_step": round(m["ms"], 4),
+           "rows_per_s": wl.rows / (m["ms"] / 1e3),
+           "query_gbs": round(wl.query_bytes / (m["ms"] / 1e3) / 1e9, 1),
+           "query_frac_hbm": round(wl.query_bytes / (m["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+           "roofline": m["roofline"], "plan": m["plan"], "kernels": m["kernels"]}
+    wl.release()
+    out["cpu_baseline"] = run_form(wl.cpu_baseline) if cpu else None
+    return out
+
+
 def run_form(fn):
     try:
         return fn()
@@ -950,35 +1066,8 @@ def main():
     wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge, "jgs": JoinGroupScattered,
           "parted": Parted, "join": Join}.get(a.workload, JoinGroup)(e, a)
 
-    for _ in range(a.warmup):
-        wl.step()
-    plan = e.last_plan()
-    e.L.td_dev_sync()
-    e.prof_reset()
-    e.prof(not a.no_prof)
-    e.L.td_dev_sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        wl.step()
-    e.L.td_dev_sync()
-    t1 = time.perf_counter()
-    prof = e.prof_read()
-    e.prof(False)
-    ms = (t1 - t0) * 1e3 / a.steps
-
-    roofline = None
-    kernels = {k: {"launches": n, "avg_ms": round(tot / max(n, 1), 5)} for k, (n, tot) in prof.items()}
-    if prof:
-        dom = max(prof, key=lambda k: prof[k][1])
-        n, tot = prof[dom]
-        avg_ms = tot / n
-        nbytes = wl.kernel_bytes(dom)
-        if nbytes:
-            ach = nbytes / (avg_ms / 1e3) / 1e9
-            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": load_traffic(dom), "kernel": dom,
-                        "algorithmic_bytes": nbytes, "avg_ms": round(avg_ms, 5)}
+    m = measure(e, wl, a.steps, a.warmup, not a.no_prof)
+    ms, plan, kernels, roofline = m["ms"], m["plan"], m["kernels"], m["roofline"]
     cpu = None if a.no_cpu_baseline else wl.cpu_baseline()
     query_gbs = wl.query_bytes / (ms / 1e3) / 1e9
     extra = {}
@@ -1009,8 +1098,14 @@ def main():
         "kernels": kernels,
     }
     line.update(extra)
+    if a.workload == "jg" and not a.no_forms:
+        tc = run_form(lambda: two_call_summary(e, wl))
+        if "ms_per_query" in tc:
+            tc["fused_vs_two_call"] = round(tc["ms_per_query"] / ms, 2)
+        line["two_call"] = tc
     wl.release()
     if a.workload == "jg" and not a.no_forms:
+        line["hc"] = run_form(lambda: hc_summary(e, a, cpu=not a.no_cpu_baseline))
         line["batched"] = run_form(lambda: batched_summary(e, a))
         line["served"] = run_form(lambda: served_summary(e, a))
     print(json.dumps(line), flush=True)
