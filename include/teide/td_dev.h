@@ -145,8 +145,8 @@ int64_t  td_dev_cancel_polls(void);
  * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group,
  * 4 XCD-partitioned fused join-group) / group path (0 auto, 1 direct
  * array, 2 hash, 3 radix-partitioned direct array with small leaf tables,
- * 4 hash-partitioned hash with small LDS tables)
- * / radix bits (0 auto) for
+ * 4 hash-partitioned hash with small LDS tables, 5 hash-partitioned hash
+ * after hot-key pre-aggregation at any input size) / radix bits (0 auto) for
  * subsequent td_execute calls. */
 void     td_dev_force_paths(int join_path, int group_path, int radix_bits);
 /* The physical plan of the last td_execute as a short text. */

@@ -152,7 +152,7 @@ typedef struct {
     tdd_col_t         keys[TDD_MAX_KEYS];
     uint16_t          ops[TDD_MAX_AGGS];  /* OP_SUM ... OP_VAR_POP */
     tdd_col_t         aggs[TDD_MAX_AGGS]; /* data may be NULL for OP_COUNT */
-    int32_t           force_path;         /* 0 auto, 1 direct array, 2 hash, 3 radix-partitioned direct array, 4 hash-partitioned hash */
+    int32_t           force_path;         /* 0 auto, 1 direct array, 2 hash, 3 radix-partitioned direct array, 4 hash-partitioned hash, 5 hash-partitioned hash after hot-key pre-aggregation at any size */
     int32_t           pad;
 } tdd_group_spec_t;
 
@@ -161,7 +161,7 @@ typedef struct {
     void*    keys[TDD_MAX_KEYS];   /* arena blocks; element type = key column type */
     void*    vals[TDD_MAX_AGGS];   /* arena blocks; element type = out_types[a] */
     int8_t   out_types[TDD_MAX_AGGS];
-    int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash, 4 partitioned direct array, 5 partitioned hash */
+    int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash, 4 partitioned direct array, 5 partitioned hash, 6 partitioned hash with hot keys folded first */
     int32_t  pad;
 } tdd_group_result_t;
 

@@ -715,6 +715,14 @@ static double hll_estimate(const unsigned int* regs) {
 }
 
 struct tdd_pht;
+struct tdd_hot;
+int hot_split(const GroupPlan& P, const uint64_t* mask, int64_t nrows, bool forced,
+              tdd_hot** out);
+const uint64_t* hot_cold_bits(const tdd_hot* h);
+int hot_count(const tdd_hot* h);
+int hot_merge(const GroupPlan& P, const tdd_hot* h, unsigned long long* gkeys, uint64_t gmask,
+              uint64_t* gcells, int* overflow);
+void hot_free(tdd_hot* h);
 int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, double est,
                   int64_t lds_bytes, bool forced, tdd_pht** out);
 int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gcells,
@@ -747,17 +755,27 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
     while ((double)cap < 2.0 * est) cap <<= 1;
     tdd_pht* ph = nullptr;
     struct PhFree { tdd_pht** h; ~PhFree() { pht_free(*h); } } phf{&ph};
-    if (force == 4 || (force == 0 && n_rows_sel >= ((int64_t)1 << 22) && est > 16384)) {
-        int prc = pht_partition(P, M.bits, M.nrows, est, force == 4 ? 4096 : 96 * 1024,
-                                force == 4, &ph);
+    tdd_hot* hot = nullptr;
+    struct HotFree { tdd_hot** h; ~HotFree() { if (*h) hot_free(*h); } } hf{&hot};
+    if (force == 4 || force == 5 ||
+        (force == 0 && n_rows_sel >= ((int64_t)1 << 22) && est > 16384)) {
+        /* skewed keys: the hot keys' rows are folded first, the partitions
+         * carry the cold rows only */
+        if (force != 4) {
+            int hrc = hot_split(P, M.bits, M.nrows, force == 5, &hot);
+            if (hrc) return hrc;
+        }
+        int prc = pht_partition(P, hot ? hot_cold_bits(hot) : M.bits, M.nrows, est,
+                                force == 4 ? 4096 : 96 * 1024, force >= 4, &ph);
         if (prc && prc != TD_ERR_NYI) return prc;
+        if (!ph && hot) { hot_free(hot); hot = nullptr; }   /* every row goes the other way */
     }
     /* partitioned: most groups leave through the list (cap entries); the
      * global table takes the split parts' keys, so it starts at their
      * bound and grows to cap if that overflows */
     uint64_t gcap = cap;
     if (ph) {
-        const int64_t want = pht_global_keys(ph);
+        const int64_t want = pht_global_keys(ph) + (hot ? hot_count(hot) : 0);
         uint64_t g = 1024;
         while ((double)g < 2.0 * (double)want) g <<= 1;
         if (g < gcap) gcap = g;
@@ -781,6 +799,11 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         Scratch lkeys(ph ? (size_t)cap * 8 : 8), lcells(ph ? (size_t)cap * P.ncells * 8 : 8);
         if (!lkeys.ok() || !lcells.ok()) return TD_ERR_OOM;
         unsigned long long* lcount = (unsigned long long*)(ovf.as<uint8_t>() + 8);
+        if (hot) {
+            rc = hot_merge(P, hot, keys.as<unsigned long long>(), (uint64_t)(gcap - 1),
+                           cells.as<uint64_t>(), ovf.as<int>());
+            if (rc) return rc;
+        }
         if (ph) {
             rc = pht_fold(ph, keys.as<unsigned long long>(), (uint64_t)(gcap - 1),
                           cells.as<uint64_t>(), lkeys.as<unsigned long long>(),
@@ -819,7 +842,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
                    (const unsigned long long*)keys.as<unsigned long long>(),
                    (const uint64_t*)cells.as<uint64_t>(), n_slots,
                    (const int64_t*)pos.as<int64_t>(), n_list, *out);
-        out->path = ph ? 5 : 3;
+        out->path = ph ? (hot ? 6 : 5) : 3;
         return TD_OK;
     }
     return TD_ERR_OOM;
@@ -887,7 +910,7 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
          * (force 3 takes it at any size, with small leaf tables) */
         const bool pda_ok = composite && slots <= (double)((int64_t)1 << 26) &&
                             slots * P.ncells * 8 <= 8.0e9 && slots <= 4.0 * (double)(nsel + 1024);
-        if (pda_ok && spec->force_path != 4 && (spec->force_path == 3 ||
+        if (pda_ok && spec->force_path < 4 && (spec->force_path == 3 ||
                        (spec->force_path == 0 && nsel >= ((int64_t)1 << 22) &&
                         slots * P.ncells * 8 > LDS_TABLE_BYTES))) {
             rc = run_pda(P, M.bits, M.nrows, (int64_t)slots,
@@ -902,7 +925,7 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
         da = composite && (slots <= (double)g_da_max_slots || spec->force_path == 1) &&
              slots * P.ncells * 8 <= 8.0e9;
         if (da && spec->force_path != 1 && slots > 4.0 * (double)(nsel + 1024)) da = false;
-        if (spec->force_path == 4) da = false;
+        if (spec->force_path >= 4) da = false;
         if (da) {
             rc = run_da(P, M, (int64_t)slots, out);
             if (rc) return rc;

@@ -492,6 +492,75 @@ __global__ __launch_bounds__(PG_T) void k_pg_hagg(GroupPlan P, PGBuf in, const P
     }
 }
 
+/* Hot-key fold (the pre-aggregation step of the hash group-by): each wave
+ * takes PG_U consecutive 64-row words at a time, reads their rows the way
+ * the first scatter pass does (plain loads when the key and values allow),
+ * folds the rows of hot keys into this workgroup's LDS accumulator block
+ * of their key, and writes each word's cold rows as a selection word; the
+ * blocks merge into the global hot cells once per workgroup. */
+#define PH_SET 2048                       /* LDS hash set of hot keys (load <= 0.25) */
+
+template <int NV, int KE>
+__global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned long long* hot,
+                                                     int nhot, const int8_t* cw_dev,
+                                                     const uint8_t* vf_dev, uint64_t* cold,
+                                                     uint64_t* gcells) {
+    extern __shared__ uint64_t sm[];
+    unsigned long long* skey = (unsigned long long*)sm;      /* PH_SET */
+    uint16_t* sidx = (uint16_t*)(sm + PH_SET);               /* PH_SET */
+    uint64_t* cells = sm + PH_SET + PH_SET / 4;              /* nhot * ncells */
+    __shared__ int8_t cw[GP_MAX_CELLS];
+    __shared__ uint8_t vf[PG_MAXV];
+    const GroupPlan& P = S.P;
+    const int nc = P.ncells;
+    if (threadIdx.x < GP_MAX_CELLS) cw[threadIdx.x] = cw_dev[threadIdx.x];
+    if (threadIdx.x < PG_MAXV) vf[threadIdx.x] = vf_dev[threadIdx.x];
+    for (int i = threadIdx.x; i < PH_SET; i += PG_T) skey[i] = EMPTY_KEY;
+    for (int i = threadIdx.x; i < nhot * nc; i += PG_T) cells[i] = cell_identity(P.kind[i % nc]);
+    __syncthreads();
+    for (int h = threadIdx.x; h < nhot; h += PG_T) {        /* distinct keys */
+        uint32_t x = (uint32_t)(mix64(hot[h]) >> 32) & (PH_SET - 1);
+        while (atomicCAS(&skey[x], EMPTY_KEY, hot[h]) != EMPTY_KEY) x = (x + 1) & (PH_SET - 1);
+        sidx[x] = (uint16_t)h;
+    }
+    __syncthreads();
+    PGBuf none;
+    memset(&none, 0, sizeof(none));
+    const int lane = lane_id();
+    const int64_t nw = (S.nrows + 63) >> 6;
+    const int64_t wv = ((int64_t)blockIdx.x * PG_T + threadIdx.x) / WAVE;
+    const int64_t nwv = (int64_t)gridDim.x * PG_T / WAVE;
+    for (int64_t wb = wv * PG_U; wb < nw; wb += nwv * PG_U) {
+        uint64_t key[PG_U];
+        uint64_t v[PG_U][PG_MAXV];
+        bool live[PG_U];
+#pragma unroll
+        for (int u = 0; u < PG_U; u++)             /* every load of the step first */
+            live[u] = pg_item<true, true, NV, KE>(S, none, (wb + u) * 64 + lane, S.nrows, key[u],
+                                                   v[u]);
+#pragma unroll
+        for (int u = 0; u < PG_U; u++) {
+            int h = -1;
+            if (live[u] && key[u] != EMPTY_KEY) {
+                uint32_t x = (uint32_t)(mix64(key[u]) >> 32) & (PH_SET - 1);
+                for (;;) {
+                    const unsigned long long k = skey[x];
+                    if (k == key[u]) { h = sidx[x]; break; }
+                    if (k == EMPTY_KEY) break;
+                    x = (x + 1) & (PH_SET - 1);
+                }
+            }
+            if (h >= 0) pg_fold<NV, true>(&cells[(int64_t)h * nc], P, cw, vf, v[u]);
+            const uint64_t m = __ballot(live[u] && h < 0);
+            if (lane == 0 && wb + u < nw) cold[wb + u] = m;
+        }
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < nhot; h += PG_T)
+        if (cells[(int64_t)h * nc] != 0)
+            cells_merge(&gcells[(int64_t)h * nc], &cells[(int64_t)h * nc], P);
+}
+
 /* ------------------------------------------------------------------ host -- */
 
 /* Host: the element bytes ld_i64 reads a column with when that is a plain
@@ -883,4 +952,54 @@ void pht_free(tdd_pht* h) {
     if (!h) return;
     (void)hipStreamSynchronize(tdd::g_stream);
     delete h;
+}
+
+/* The hot-key fold over rows [0, nrows) selected by mask (k_pg_hotfold):
+ * the rows of the nhot keys of `hot` fold into hot_cells (nhot blocks of
+ * the plan's cells, initialised), every other selected row is set in the
+ * cold words.  TD_ERR_NYI when the plan's value words or the LDS blocks
+ * do not fit. */
+int pht_hot_fold(const GroupPlan& P, const uint64_t* mask, int64_t nrows,
+                 const unsigned long long* hot, int nhot, uint64_t* cold, uint64_t* hot_cells) {
+    PGRows S;
+    memset(&S, 0, sizeof(S));
+    S.P = P;
+    S.mask = mask;
+    S.nrows = nrows;
+    int8_t cw[GP_MAX_CELLS];
+    uint8_t vf[PG_MAXV];
+    const int nv = pg_words(P, S, cw, vf);
+    if (nv < 0) return TD_ERR_NYI;
+    const size_t lds = (size_t)PH_SET * 8 + (size_t)PH_SET * 2 + (size_t)nhot * P.ncells * 8;
+    if (lds > 64 * 1024) return TD_ERR_NYI;
+    int ke = P.n_keys == 1 && P.keys[0].data ? pg_plain(P.keys[0]) : 0;
+    for (int k = 0; k < nv; k++)
+        if (S.vsrc[k] >= 0 && pg_plain(P.aggs[S.vsrc[k]]) != 8) ke = 0;
+    SCRATCH(meta, GP_MAX_CELLS + PG_MAXV);
+    uint8_t hm[GP_MAX_CELLS + PG_MAXV];
+    memcpy(hm, cw, GP_MAX_CELLS);
+    memcpy(hm + GP_MAX_CELLS, vf, PG_MAXV);
+    HC(hipMemcpyAsync(meta.p, hm, sizeof(hm), hipMemcpyHostToDevice, tdd::g_stream));
+    const int8_t* dcw = (const int8_t*)meta.p;
+    const uint8_t* dvf = meta.as<uint8_t>() + GP_MAX_CELLS;
+    const int64_t nw = (nrows + 63) >> 6;
+    const int grid = resident_grid(k_pg_hotfold<2, 8>, PG_T, lds, ceil_div(nw, PG_W * PG_U));
+#define PGF1(NV_, KE_) LAUNCH("group_hot_fold", (k_pg_hotfold<NV_, KE_>), grid, PG_T, lds, S, hot, \
+                              nhot, dcw, dvf, cold, hot_cells)
+#define PGF(NV_)                                      \
+    do {                                              \
+        if (ke == 8) PGF1(NV_, 8);                    \
+        else if (ke == 4) PGF1(NV_, 4);               \
+        else PGF1(NV_, 0);                            \
+    } while (0)
+    switch (nv) {
+    case 0: PGF(0); break;
+    case 1: PGF(1); break;
+    case 2: PGF(2); break;
+    case 3: PGF(3); break;
+    default: PGF(4); break;
+    }
+#undef PGF
+#undef PGF1
+    return TD_OK;
 }
