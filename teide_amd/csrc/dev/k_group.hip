@@ -718,13 +718,15 @@ struct tdd_pht;
 struct tdd_hot;
 int hot_split(const GroupPlan& P, const uint64_t* mask, int64_t nrows, bool forced,
               tdd_hot** out);
-const uint64_t* hot_cold_bits(const tdd_hot* h);
 int hot_count(const tdd_hot* h);
+const unsigned long long* hot_keys(const tdd_hot* h);
+uint64_t* hot_cells(const tdd_hot* h);
 int hot_merge(const GroupPlan& P, const tdd_hot* h, unsigned long long* gkeys, uint64_t gmask,
               uint64_t* gcells, int* overflow);
 void hot_free(tdd_hot* h);
 int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, double est,
-                  int64_t lds_bytes, bool forced, tdd_pht** out);
+                  int64_t lds_bytes, bool forced, const unsigned long long* hot, int nhot,
+                  uint64_t* hot_cells, tdd_pht** out);
 int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gcells,
              unsigned long long* lkeys, uint64_t* lcells, unsigned long long* lcount,
              int64_t lcap_out, int* overflow);
@@ -765,8 +767,9 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
             int hrc = hot_split(P, M.bits, M.nrows, force == 5, &hot);
             if (hrc) return hrc;
         }
-        int prc = pht_partition(P, hot ? hot_cold_bits(hot) : M.bits, M.nrows, est,
-                                force == 4 ? 4096 : 96 * 1024, force >= 4, &ph);
+        int prc = pht_partition(P, M.bits, M.nrows, est, force == 4 ? 4096 : 96 * 1024,
+                                force >= 4, hot ? hot_keys(hot) : nullptr,
+                                hot ? hot_count(hot) : 0, hot ? hot_cells(hot) : nullptr, &ph);
         if (prc && prc != TD_ERR_NYI) return prc;
         if (!ph && hot) { hot_free(hot); hot = nullptr; }   /* every row goes the other way */
     }

@@ -27,8 +27,6 @@
  */
 #include "group_plan.h"
 
-int pht_hot_fold(const GroupPlan& P, const uint64_t* mask, int64_t nrows,
-                 const unsigned long long* hot, int nhot, uint64_t* cold, uint64_t* hot_cells);
 
 #define HOT_MAX       512                /* hot keys at most */
 #define HOT_SAMPLES   ((int64_t)1 << 21)
@@ -143,21 +141,19 @@ __global__ void k_hot_merge(GroupPlan P, const unsigned long long* hot, int nhot
 
 struct tdd_hot {
     void*  keys = nullptr;                /* hot list (device) */
-    void*  cells = nullptr;               /* nhot x ncells (device) */
-    void*  cold = nullptr;                /* cold selection words (device) */
+    void*  cells = nullptr;               /* nhot x ncells, identities (device) */
     int    nhot = 0;
     ~tdd_hot() {
         (void)hipStreamSynchronize(tdd::g_stream);
         if (keys) td__arena_free(keys);
         if (cells) td__arena_free(cells);
-        if (cold) td__arena_free(cold);
     }
 };
 
-/* Steps 1-4 over rows [0, nrows) selected by mask.  *out stays NULL when
+/* Steps 1-3 over rows [0, nrows) selected by mask.  *out stays NULL when
  * the keys are not skewed enough, there are fewer than 4 M rows (unless
  * forced) or the accumulator blocks would not fit LDS; otherwise it holds
- * the hot list, its cells and the cold words. */
+ * the hot list and its cells, ready for the fold (step 4, k_pg_hotfold). */
 int hot_split(const GroupPlan& P, const uint64_t* mask, int64_t nrows, bool forced,
               tdd_hot** out) {
     *out = nullptr;
@@ -198,11 +194,9 @@ int hot_split(const GroupPlan& P, const uint64_t* mask, int64_t nrows, bool forc
     if (tb < 0 || nk == 0 || total == 0 || (double)share < HOT_MIN_SHARE * (double)total)
         return TD_OK;
     tdd_hot* h = new tdd_hot();
-    const int64_t nw = (nrows + 63) >> 6;
     h->keys = td__arena_alloc((size_t)HOT_MAX * 8);
     h->cells = td__arena_alloc((size_t)cap * nc * 8);
-    h->cold = td__arena_alloc((size_t)(nw > 0 ? nw : 1) * 8);
-    if (!h->keys || !h->cells || !h->cold) { delete h; return TD_ERR_OOM; }
+    if (!h->keys || !h->cells) { delete h; return TD_ERR_OOM; }
     unsigned int* dn = (unsigned int*)(hist.as<unsigned long long>() + 64);
     LAUNCH("group_hot", k_hot_pick, stream_grid(ceil_div((int64_t)tcap, 256)), 256, 0,
            (const unsigned long long*)tkeys.as<unsigned long long>(),
@@ -214,15 +208,13 @@ int hot_split(const GroupPlan& P, const uint64_t* mask, int64_t nrows, bool forc
     h->nhot = (int)(nhot < (unsigned)cap ? nhot : (unsigned)cap);
     const int ib = stream_grid(ceil_div((int64_t)h->nhot * nc, 256 * 8));
     LAUNCH("group_hot", k_hot_init, ib, 256, 0, (uint64_t*)h->cells, (int64_t)h->nhot, P);
-    rc = pht_hot_fold(P, mask, nrows, (const unsigned long long*)h->keys, h->nhot,
-                      (uint64_t*)h->cold, (uint64_t*)h->cells);
-    if (rc) { delete h; return rc == TD_ERR_NYI ? TD_OK : rc; }
     *out = h;
     return TD_OK;
 }
 
-const uint64_t* hot_cold_bits(const tdd_hot* h) { return (const uint64_t*)h->cold; }
 int hot_count(const tdd_hot* h) { return h->nhot; }
+const unsigned long long* hot_keys(const tdd_hot* h) { return (const unsigned long long*)h->keys; }
+uint64_t* hot_cells(const tdd_hot* h) { return (uint64_t*)h->cells; }
 
 int hot_merge(const GroupPlan& P, const tdd_hot* h, unsigned long long* gkeys, uint64_t gmask,
               uint64_t* gcells, int* overflow) {

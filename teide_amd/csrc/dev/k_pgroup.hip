@@ -492,18 +492,22 @@ __global__ __launch_bounds__(PG_T) void k_pg_hagg(GroupPlan P, PGBuf in, const P
     }
 }
 
-/* Hot-key fold (the pre-aggregation step of the hash group-by): each wave
- * takes PG_U consecutive 64-row words at a time, reads their rows the way
- * the first scatter pass does (plain loads when the key and values allow),
- * folds the rows of hot keys into this workgroup's LDS accumulator block
- * of their key, and writes each word's cold rows as a selection word; the
- * blocks merge into the global hot cells once per workgroup. */
+/* Hot-key fold (the pre-aggregation step of the hash group-by): workgroup
+ * b takes the 64-row words [b * wpb, (b + 1) * wpb) and reads their rows
+ * the way the first scatter pass does (plain loads when the key and values
+ * allow), PG_U words per wave per step; rows of hot keys fold into this
+ * workgroup's LDS accumulator block of their key, every other selected
+ * row is written as a record (key + value words) to the workgroup's region
+ * [b * wpb * 64, ...) of `out`, in row order; counts[b] = its records.
+ * The blocks merge into the global hot cells once per workgroup. */
 #define PH_SET 2048                       /* LDS hash set of hot keys (load <= 0.25) */
+#define PH_STEP (PG_W * PG_U)             /* words per workgroup step */
 
 template <int NV, int KE>
 __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned long long* hot,
                                                      int nhot, const int8_t* cw_dev,
-                                                     const uint8_t* vf_dev, uint64_t* cold,
+                                                     const uint8_t* vf_dev, int64_t wpb,
+                                                     PGBuf out, int64_t* counts,
                                                      uint64_t* gcells) {
     extern __shared__ uint64_t sm[];
     unsigned long long* skey = (unsigned long long*)sm;      /* PH_SET */
@@ -511,10 +515,13 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
     uint64_t* cells = sm + PH_SET + PH_SET / 4;              /* nhot * ncells */
     __shared__ int8_t cw[GP_MAX_CELLS];
     __shared__ uint8_t vf[PG_MAXV];
+    __shared__ uint32_t wcnt[PH_STEP];
+    __shared__ uint32_t cursor;
     const GroupPlan& P = S.P;
     const int nc = P.ncells;
     if (threadIdx.x < GP_MAX_CELLS) cw[threadIdx.x] = cw_dev[threadIdx.x];
     if (threadIdx.x < PG_MAXV) vf[threadIdx.x] = vf_dev[threadIdx.x];
+    if (threadIdx.x == 0) cursor = 0;
     for (int i = threadIdx.x; i < PH_SET; i += PG_T) skey[i] = EMPTY_KEY;
     for (int i = threadIdx.x; i < nhot * nc; i += PG_T) cells[i] = cell_identity(P.kind[i % nc]);
     __syncthreads();
@@ -526,18 +533,22 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
     __syncthreads();
     PGBuf none;
     memset(&none, 0, sizeof(none));
-    const int lane = lane_id();
+    const int lane = lane_id(), wv = threadIdx.x / WAVE;
     const int64_t nw = (S.nrows + 63) >> 6;
-    const int64_t wv = ((int64_t)blockIdx.x * PG_T + threadIdx.x) / WAVE;
-    const int64_t nwv = (int64_t)gridDim.x * PG_T / WAVE;
-    for (int64_t wb = wv * PG_U; wb < nw; wb += nwv * PG_U) {
+    const int64_t w_begin = (int64_t)blockIdx.x * wpb;
+    const int64_t w_end = w_begin + wpb < nw ? w_begin + wpb : nw;
+    const int64_t region = w_begin * 64;
+    for (int64_t ws = w_begin; ws < w_end; ws += PH_STEP) {   /* uniform across the block */
         uint64_t key[PG_U];
         uint64_t v[PG_U][PG_MAXV];
         bool live[PG_U];
+        uint64_t m[PG_U];
 #pragma unroll
-        for (int u = 0; u < PG_U; u++)             /* every load of the step first */
-            live[u] = pg_item<true, true, NV, KE>(S, none, (wb + u) * 64 + lane, S.nrows, key[u],
-                                                   v[u]);
+        for (int u = 0; u < PG_U; u++) {           /* every load of the step first */
+            const int64_t w = ws + wv * PG_U + u;
+            live[u] = w < w_end && pg_item<true, true, NV, KE>(S, none, w * 64 + lane, S.nrows,
+                                                                key[u], v[u]);
+        }
 #pragma unroll
         for (int u = 0; u < PG_U; u++) {
             int h = -1;
@@ -551,11 +562,31 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
                 }
             }
             if (h >= 0) pg_fold<NV, true>(&cells[(int64_t)h * nc], P, cw, vf, v[u]);
-            const uint64_t m = __ballot(live[u] && h < 0);
-            if (lane == 0 && wb + u < nw) cold[wb + u] = m;
+            m[u] = __ballot(live[u] && h < 0);
+            if (lane == 0) wcnt[wv * PG_U + u] = (uint32_t)__popcll(m[u]);
         }
+        __syncthreads();
+        uint32_t pre = cursor;                    /* records of the step's earlier words */
+        for (int i = 0; i < wv * PG_U; i++) pre += wcnt[i];
+#pragma unroll
+        for (int u = 0; u < PG_U; u++) {
+            if ((m[u] >> lane) & 1) {
+                const int64_t at = region + pre + lanes_below(m[u]);
+                out.key[at] = key[u];
+#pragma unroll
+                for (int q = 0; q < NV; q++) out.val[q][at] = v[u][q];
+            }
+            pre += (uint32_t)__popcll(m[u]);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int i = 0; i < PH_STEP; i++) t += wcnt[i];
+            cursor += t;
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = cursor;
     for (int h = threadIdx.x; h < nhot; h += PG_T)
         if (cells[(int64_t)h * nc] != 0)
             cells_merge(&gcells[(int64_t)h * nc], &cells[(int64_t)h * nc], P);
@@ -709,19 +740,24 @@ static bool pg_alloc(PGBufs& m, int64_t n, int nv, bool hk, PGBuf& A, PGBuf& B) 
  * their records. */
 template <bool HK>
 static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, int npass, int top,
-                      const PGBuf& A, const PGBuf& B, std::vector<PGJob>& jobs, PGBuf& res) {
+                      const PGBuf& A, const PGBuf& B, std::vector<PGJob>& jobs, PGBuf& res,
+                      const std::vector<PGJob>* first = nullptr) {
     jobs.clear();
-    const int nj0 = (int)ceil_div(nrows, PG_JOB);
-    for (int k = 0; k < nj0; k++) {
-        PGJob J;
-        J.r0 = (int64_t)k * PG_JOB;
-        J.r1 = J.r0 + PG_JOB < nrows ? J.r0 + PG_JOB : nrows;
-        J.parent = 0;
-        J.js = 0;
-        J.nj = nj0;
-        jobs.push_back(J);
+    if (first) {
+        jobs = *first;                   /* the first pass reads records of A */
+    } else {
+        const int nj0 = (int)ceil_div(nrows, PG_JOB);
+        for (int k = 0; k < nj0; k++) {
+            PGJob J;
+            J.r0 = (int64_t)k * PG_JOB;
+            J.r1 = J.r0 + PG_JOB < nrows ? J.r0 + PG_JOB : nrows;
+            J.parent = 0;
+            J.js = 0;
+            J.nj = nj0;
+            jobs.push_back(J);
+        }
     }
-    PGBuf in = A, outb = B;          /* the first pass reads rows */
+    PGBuf in = A, outb = B;          /* the first pass reads rows, or A's records */
     int shift = top;
     int64_t parents = 1;
     for (int pass = 0; pass < npass; pass++) {
@@ -735,7 +771,7 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
         if (njobs > 0)
             HC(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(PGJob) * (size_t)njobs,
                               hipMemcpyHostToDevice, tdd::g_stream));
-        const bool rows = pass == 0;
+        const bool rows = pass == 0 && !first;
         int rc = pg_hist_and_offsets<HK>(rows, S, in, djobs.as<PGJob>(), njobs, shift, PL,
                                          Q.as<uint32_t>(), O.as<int64_t>());
         if (rc) return rc;
@@ -864,8 +900,67 @@ struct tdd_pht {
 /* Scatter the rows of a hash group-by into partitions of the key's hash
  * whose expected distinct keys (from the estimate) fill half of an LDS
  * table; TD_ERR_NYI when one LDS table already holds them all. */
+/* The hot-key fold of the rows into A (see k_pg_hotfold); *first receives
+ * the first pass's jobs over the workgroups' record regions.  TD_ERR_NYI
+ * when the LDS blocks do not fit. */
+static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t* vf,
+                        const unsigned long long* hot, int nhot, uint64_t* hot_cells,
+                        const PGBuf& A, std::vector<PGJob>* first) {
+    const GroupPlan& P = S.P;
+    const size_t lds = (size_t)PH_SET * 8 + (size_t)PH_SET * 2 + (size_t)nhot * P.ncells * 8;
+    if (lds > 64 * 1024) return TD_ERR_NYI;
+    int ke = P.n_keys == 1 && P.keys[0].data ? pg_plain(P.keys[0]) : 0;
+    for (int k = 0; k < nv; k++)
+        if (S.vsrc[k] >= 0 && pg_plain(P.aggs[S.vsrc[k]]) != 8) ke = 0;
+    SCRATCH(meta, GP_MAX_CELLS + PG_MAXV);
+    uint8_t hm[GP_MAX_CELLS + PG_MAXV];
+    memcpy(hm, cw, GP_MAX_CELLS);
+    memcpy(hm + GP_MAX_CELLS, vf, PG_MAXV);
+    HC(hipMemcpyAsync(meta.p, hm, sizeof(hm), hipMemcpyHostToDevice, tdd::g_stream));
+    const int8_t* dcw = (const int8_t*)meta.p;
+    const uint8_t* dvf = meta.as<uint8_t>() + GP_MAX_CELLS;
+    const int64_t nw = (S.nrows + 63) >> 6;
+    const int want = resident_grid(k_pg_hotfold<2, 8>, PG_T, lds, ceil_div(nw, PH_STEP));
+    const int64_t wpb = ceil_div(ceil_div(nw, want), PH_STEP) * PH_STEP;
+    const int grid = (int)ceil_div(nw, wpb);
+    SCRATCH(cnt, (size_t)grid * 8);
+#define PGF1(NV_, KE_) LAUNCH("group_hot_fold", (k_pg_hotfold<NV_, KE_>), grid, PG_T, lds, S, hot, \
+                              nhot, dcw, dvf, wpb, A, cnt.as<int64_t>(), hot_cells)
+#define PGF(NV_)                                      \
+    do {                                              \
+        if (ke == 8) PGF1(NV_, 8);                    \
+        else if (ke == 4) PGF1(NV_, 4);               \
+        else PGF1(NV_, 0);                            \
+    } while (0)
+    switch (nv) {
+    case 0: PGF(0); break;
+    case 1: PGF(1); break;
+    case 2: PGF(2); break;
+    case 3: PGF(3); break;
+    default: PGF(4); break;
+    }
+#undef PGF
+#undef PGF1
+    std::vector<int64_t> c((size_t)grid);
+    HC(hipMemcpyAsync(c.data(), cnt.p, c.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    first->clear();
+    for (int b = 0; b < grid; b++)
+        for (int64_t r = 0; r < c[(size_t)b]; r += PG_JOB) {
+            PGJob J;
+            J.r0 = (int64_t)b * wpb * 64 + r;
+            J.r1 = J.r0 + (c[(size_t)b] - r < PG_JOB ? c[(size_t)b] - r : PG_JOB);
+            J.parent = 0;
+            J.js = 0;
+            first->push_back(J);
+        }
+    for (PGJob& J : *first) J.nj = (int32_t)first->size();
+    return TD_OK;
+}
+
 int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, double est,
-                  int64_t lds_bytes, bool forced, tdd_pht** out) {
+                  int64_t lds_bytes, bool forced, const unsigned long long* hot, int nhot,
+                  uint64_t* hot_cells, tdd_pht** out) {
     *out = nullptr;
     if (nrows <= 0) return TD_ERR_NYI;
     const int64_t per = 8 + (int64_t)P.ncells * 8;
@@ -890,7 +985,16 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
     if (!pg_alloc(h->mem, nrows, h->nv, true, A, B)) { delete h; return TD_ERR_OOM; }
     int pbits[8], npass;
     pg_split_bits(bits, pbits, &npass);
-    int rc = pg_scatter<true>(S, h->nv, nrows, pbits, npass, 64, A, B, h->jobs, h->res);
+    std::vector<PGJob> first;
+    bool from_records = false;
+    if (nhot > 0) {
+        int frc = pht_hot_fold(S, h->nv, h->cw, h->vf, hot, nhot, hot_cells, A, &first);
+        if (frc && frc != TD_ERR_NYI) { delete h; return frc; }
+        from_records = frc == 0;
+    }
+    if (nhot > 0 && !from_records) { delete h; return TD_ERR_NYI; }   /* the hot rows need the fold */
+    int rc = pg_scatter<true>(S, h->nv, nrows, pbits, npass, 64, A, B, h->jobs, h->res,
+                              from_records ? &first : nullptr);
     if (rc) { delete h; return rc; }
     *out = h;
     return TD_OK;
@@ -954,52 +1058,3 @@ void pht_free(tdd_pht* h) {
     delete h;
 }
 
-/* The hot-key fold over rows [0, nrows) selected by mask (k_pg_hotfold):
- * the rows of the nhot keys of `hot` fold into hot_cells (nhot blocks of
- * the plan's cells, initialised), every other selected row is set in the
- * cold words.  TD_ERR_NYI when the plan's value words or the LDS blocks
- * do not fit. */
-int pht_hot_fold(const GroupPlan& P, const uint64_t* mask, int64_t nrows,
-                 const unsigned long long* hot, int nhot, uint64_t* cold, uint64_t* hot_cells) {
-    PGRows S;
-    memset(&S, 0, sizeof(S));
-    S.P = P;
-    S.mask = mask;
-    S.nrows = nrows;
-    int8_t cw[GP_MAX_CELLS];
-    uint8_t vf[PG_MAXV];
-    const int nv = pg_words(P, S, cw, vf);
-    if (nv < 0) return TD_ERR_NYI;
-    const size_t lds = (size_t)PH_SET * 8 + (size_t)PH_SET * 2 + (size_t)nhot * P.ncells * 8;
-    if (lds > 64 * 1024) return TD_ERR_NYI;
-    int ke = P.n_keys == 1 && P.keys[0].data ? pg_plain(P.keys[0]) : 0;
-    for (int k = 0; k < nv; k++)
-        if (S.vsrc[k] >= 0 && pg_plain(P.aggs[S.vsrc[k]]) != 8) ke = 0;
-    SCRATCH(meta, GP_MAX_CELLS + PG_MAXV);
-    uint8_t hm[GP_MAX_CELLS + PG_MAXV];
-    memcpy(hm, cw, GP_MAX_CELLS);
-    memcpy(hm + GP_MAX_CELLS, vf, PG_MAXV);
-    HC(hipMemcpyAsync(meta.p, hm, sizeof(hm), hipMemcpyHostToDevice, tdd::g_stream));
-    const int8_t* dcw = (const int8_t*)meta.p;
-    const uint8_t* dvf = meta.as<uint8_t>() + GP_MAX_CELLS;
-    const int64_t nw = (nrows + 63) >> 6;
-    const int grid = resident_grid(k_pg_hotfold<2, 8>, PG_T, lds, ceil_div(nw, PG_W * PG_U));
-#define PGF1(NV_, KE_) LAUNCH("group_hot_fold", (k_pg_hotfold<NV_, KE_>), grid, PG_T, lds, S, hot, \
-                              nhot, dcw, dvf, cold, hot_cells)
-#define PGF(NV_)                                      \
-    do {                                              \
-        if (ke == 8) PGF1(NV_, 8);                    \
-        else if (ke == 4) PGF1(NV_, 4);               \
-        else PGF1(NV_, 0);                            \
-    } while (0)
-    switch (nv) {
-    case 0: PGF(0); break;
-    case 1: PGF(1); break;
-    case 2: PGF(2); break;
-    case 3: PGF(3); break;
-    default: PGF(4); break;
-    }
-#undef PGF
-#undef PGF1
-    return TD_OK;
-}
