@@ -30,6 +30,8 @@
  * probe row (zeros for a missing partner); INNER joins compact the slotted
  * output when some probe rows found no partner.
  */
+#include "pred_eval.h"
+#include "sel_map.h"
 #include "join_keys.h"
 
 #include <vector>
@@ -702,28 +704,231 @@ static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch
     return TD_OK;
 }
 
+/* ---------------------------------------------- dense build keys: a map -- */
+/* When the build keys are unique integers in a dense range, a map replaces
+ * the partitions: each right column is scattered to its key's place
+ * (dense[c][key - kmin]) beside a present bit per key, and the output rows
+ * are the selected probe rows (INNER: those whose key is present) in row
+ * order, left columns compacted and right columns read at their keys in
+ * the same pass -- the direct-map counterpart of the reference's chained
+ * join for this case (exec.c:7974-8128). */
+struct RJDBuild {
+    tdd_col_t   key;
+    int64_t     nr, kmin;
+    uint32_t*   present;
+    int32_t     n, pad;
+    const void* src[RJ_MAXC];
+    void*       dst[RJ_MAXC];
+    uint8_t     esz[RJ_MAXC];
+};
+
+template <class T>
+__device__ __forceinline__ void rjd_put(const RJDBuild& B, int c, int64_t r, uint64_t i) {
+    ((T*)B.dst[c])[i] = ((const T*)B.src[c])[r];
+}
+
+__global__ void k_rjd_build(RJDBuild B) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < B.nr;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = (uint64_t)ld_i64(B.key, r) - (uint64_t)B.kmin;
+        atomicOr(&B.present[i >> 5], 1u << (i & 31));        /* result unused: no return */
+        for (int c = 0; c < B.n; c++) {
+            switch (B.esz[c]) {
+            case 8: rjd_put<uint64_t>(B, c, r, i); break;
+            case 4: rjd_put<uint32_t>(B, c, r, i); break;
+            case 2: rjd_put<uint16_t>(B, c, r, i); break;
+            default: rjd_put<uint8_t>(B, c, r, i); break;
+            }
+        }
+    }
+}
+
+/* Keys present: the build side repeats none exactly when this equals nr. */
+__global__ void k_rjd_count(const uint32_t* present, int64_t nw, unsigned long long* count) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += __popc(present[i]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane_id() == 0 && c) atomicAdd(count, c);
+}
+
+/* The output rows as a TD_SEL: selected probe rows (inner: with a present
+ * key; present == nullptr: every key of the range is), one 1024-row
+ * segment per workgroup step as k_sel_eval lays it out, with the segment
+ * popcounts, flags and total.  Only selected rows look their key up. */
+__global__ __launch_bounds__(256) void k_rjd_sel(tdd_col_t lk, const uint64_t* lsel, int64_t nl,
+                                                 int64_t kmin, int64_t range,
+                                                 const uint32_t* present, int inner,
+                                                 uint32_t n_segs, uint8_t* flags, uint16_t* pop,
+                                                 uint64_t* bits, unsigned long long* total) {
+    __shared__ uint32_t wcnt[4];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t n_words = (nl + 63) >> 6;
+    unsigned long long my_total = 0;
+    for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
+        const int64_t r0 = (seg * 16 + w * 4) * 64 + lane;
+        int64_t key[4] = {0, 0, 0, 0};
+        if (inner) ld4_i64(lk, r0, nl, key);
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t word = seg * 16 + w * 4 + k, row = r0 + 64 * k;
+            bool on = row < nl && (!lsel || ((lsel[word] >> lane) & 1));
+            if (inner && on) {
+                const uint64_t i = (uint64_t)key[k] - (uint64_t)kmin;
+                on = i < (uint64_t)range && (!present || ((present[i >> 5] >> (i & 31)) & 1));
+            }
+            const uint64_t m = __ballot(on);
+            if (word < n_words && lane == 0) bits[word] = m;
+            cnt += __popcll(m);
+        }
+        if (lane == 0) wcnt[w] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t c = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            int64_t seg_rows = nl - seg * 1024;
+            if (seg_rows > 1024) seg_rows = 1024;
+            pop[seg] = (uint16_t)c;
+            flags[seg] = c == 0 ? TD_SEL_NONE : (c == (uint32_t)seg_rows ? TD_SEL_ALL : TD_SEL_MIX);
+            my_total += c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && my_total) atomicAdd(total, my_total);
+}
+
+/* A device TD_SEL over n rows carved out of one block of `mem`. */
+static void rj_sel_view(void* mem, int64_t n, tdd_sel_t* sv) {
+    const int64_t nseg = ceil_div(n, 1024);
+    sv->base = mem;
+    sv->flags = (uint8_t*)mem + 16;
+    sv->popcnt = (uint16_t*)((uint8_t*)sv->flags + ((nseg + 7) & ~7));
+    sv->bits = (uint64_t*)((uint8_t*)sv->popcnt + ((2 * nseg + 7) & ~7));
+    sv->nrows = n;
+    sv->n_segs = (uint32_t)nseg;
+    sv->pad = 0;
+}
+
+static size_t rj_sel_bytes(int64_t n) {
+    const int64_t nseg = ceil_div(n, 1024), nw = ceil_div(n, 64);
+    return 16 + (size_t)((nseg + 7) & ~7) + (size_t)((2 * nseg + 7) & ~7) + (size_t)nw * 8 + 8;
+}
+
+static int rj_direct(const tdd_join_mat_spec_t* S, int64_t mn, int64_t range,
+                     tdd_join_mat_result_t* out) {
+    const tdd_join_spec_t* J = &S->join;
+    if (S->n_lcols > TDD_MAP_MAXC || S->n_rcols > TDD_MAP_MAXC) return TD_ERR_NYI;
+    for (int c = 0; c < S->n_rcols; c++) {
+        const int e = S->rcols[c].esz;
+        if (e != 1 && e != 2 && e != 4 && e != 8) return TD_ERR_NYI;
+    }
+    const int64_t pw = ceil_div(range, 32);
+    SCRATCH(present, (size_t)pw * 4);
+    SCRATCH(count, 8);
+    RJBlocks dense;
+    RJDBuild B;
+    memset(&B, 0, sizeof(B));
+    B.key = J->rkeys[0];
+    B.nr = J->nr;
+    B.kmin = mn;
+    B.present = present.as<uint32_t>();
+    B.n = S->n_rcols;
+    for (int c = 0; c < S->n_rcols; c++) {
+        B.src[c] = S->rcols[c].data;
+        B.esz[c] = S->rcols[c].esz;
+        if (!(B.dst[c] = dense.take((size_t)range * B.esz[c]))) return TD_ERR_OOM;
+    }
+    int rc = tdd_fill(present.p, 0, (size_t)pw * 4);
+    if (!rc) rc = tdd_fill(count.p, 0, 8);
+    if (rc) return rc;
+    if (J->nr > 0)
+        LAUNCH("rjoin_map", k_rjd_build, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0, B);
+    LAUNCH("rjoin_map_count", k_rjd_count, stream_grid(ceil_div(pw, 256 * 8), 4), 256, 0,
+           (const uint32_t*)present.as<uint32_t>(), pw, count.as<unsigned long long>());
+    int64_t distinct = 0;
+    HC(hipMemcpyAsync(&distinct, count.p, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    if (distinct != J->nr) return TD_ERR_NYI;         /* repeated build keys: pairs */
+    /* every key of the range present: a range check answers for the map */
+    const uint32_t* pres = J->nr == range ? nullptr : present.as<uint32_t>();
+    const int64_t nl = J->nl;
+    const bool inner = J->join_type == 0;
+    tdd_sel_t sv;
+    Scratch selmem(inner || !J->lsel ? rj_sel_bytes(nl) : 8);
+    if (!selmem.ok()) return TD_ERR_OOM;
+    if (inner || !J->lsel) {
+        rj_sel_view(selmem.p, nl, &sv);
+        rc = tdd_fill(selmem.p, 0, 16);
+        if (rc) return rc;
+        if (sv.n_segs > 0)
+            LAUNCH("rjoin_map_sel", k_rjd_sel, stream_grid(sv.n_segs, 8), 256, 0, J->lkeys[0],
+                   J->lsel ? (const uint64_t*)J->lsel->bits : nullptr, nl, mn, range,
+                   pres, (int)inner, sv.n_segs, sv.flags,
+                   sv.popcnt, sv.bits, (unsigned long long*)sv.base);
+    } else {
+        sv = *J->lsel;                                /* LEFT: every selected row */
+    }
+    int64_t nout = 0;
+    if (nl > 0) {
+        rc = tdd_sel_total(&sv, &nout);
+        if (rc) return rc;
+    }
+    POLL_CANCEL();
+    RJBlocks own;
+    void* ldst[TDD_MAP_MAXC];
+    for (int c = 0; c < S->n_lcols; c++)
+        if (!(ldst[c] = own.take((size_t)nout * S->lcols[c].esz))) return TD_ERR_OOM;
+    TddMapGather M;
+    memset(&M, 0, sizeof(M));
+    M.key = J->lkeys[0];
+    M.kmin = mn;
+    M.range = range;
+    M.present = pres;
+    M.n = S->n_rcols;
+    for (int c = 0; c < S->n_rcols; c++) {
+        M.src[c] = B.dst[c];
+        M.esz[c] = B.esz[c];
+        if (!(M.dst[c] = own.take((size_t)nout * M.esz[c]))) return TD_ERR_OOM;
+    }
+    if (nout > 0) {
+        rc = tdd__sel_compact_map(&sv, S->n_lcols, S->lcols, ldst, &M);
+        if (rc) return rc;
+    }
+    for (int c = 0; c < S->n_lcols; c++) { own.keep(ldst[c]); out->lcols[c] = ldst[c]; }
+    for (int c = 0; c < S->n_rcols; c++) { own.keep(M.dst[c]); out->rcols[c] = M.dst[c]; }
+    out->n_rows = nout;
+    out->path = 4;
+    out->radix_bits = 0;
+    return TD_OK;
+}
+
 extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t* out) {
     memset(out, 0, sizeof(*out));
     const tdd_join_spec_t* J = &S->join;
     if (J->join_type > 1 || J->n_keys < 1 || J->n_keys > TDD_MAX_KEYS) return TD_ERR_NYI;
-    if (J->force_path == 1 || J->force_path >= 3) return TD_ERR_NYI;
+    if (J->force_path >= 3) return TD_ERR_NYI;
     if (S->n_lcols < 0 || S->n_lcols > RJ_MAXC || S->n_rcols < 0 || S->n_rcols > RJ_MAXC)
         return TD_ERR_NYI;
     if (J->nl >= ((int64_t)1 << 31) || J->nr >= ((int64_t)1 << 31)) return TD_ERR_NYI;
     for (int k = 0; k < J->n_keys; k++)
         if ((J->lkeys[k].type == TD_F64) != (J->rkeys[k].type == TD_F64)) return TD_ERR_NYI;
     const bool exact = J->n_keys == 1 && rj_is_int(J->lkeys[0].type) && rj_is_int(J->rkeys[0].type);
-    if (J->force_path == 0 && exact && J->nr > 0) {
-        /* dense build keys: the direct map of tdd_join is the better join */
-        int64_t mn, mx;
-        if (J->rkeys[0].ranged) { mn = J->rkeys[0].lo; mx = J->rkeys[0].hi; }
-        else {
+    if (J->force_path == 1 && !exact) return TD_ERR_NYI;
+    if (J->force_path != 2 && exact) {
+        /* dense build keys: the map (forced: any range a map can span) */
+        int64_t mn = 0, mx = -1;
+        if (J->nr > 0 && J->rkeys[0].ranged) { mn = J->rkeys[0].lo; mx = J->rkeys[0].hi; }
+        else if (J->nr > 0) {
             int rc = tdd_minmax(&J->rkeys[0], J->nr, nullptr, &mn, &mx);
             if (rc) return rc;
         }
+        if (mn > mx) { mn = 0; mx = 0; }                  /* no build rows */
         const uint64_t range = (uint64_t)mx - (uint64_t)mn + 1;
-        if (mn <= mx && range <= (uint64_t)(4 * J->nr + 4096) && range < ((uint64_t)1 << 31))
-            return TD_ERR_NYI;
+        if (range < ((uint64_t)1 << 31) &&
+            (J->force_path == 1 || range <= (uint64_t)(4 * J->nr + 4096)))
+            return rj_direct(S, mn, (int64_t)range, out);
+        if (J->force_path == 1) return TD_ERR_NYI;
     }
 
     /* radix bits: build partitions of about 0.6 of an LDS table, and enough

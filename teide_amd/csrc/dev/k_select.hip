@@ -9,6 +9,7 @@
  * popcount and NONE / ALL / MIX flag from the ballots it already holds.
  */
 #include "pred_eval.h"
+#include "sel_map.h"
 
 #define SEG_ROWS 1024
 #define SEL_T 256
@@ -25,11 +26,13 @@ __global__ __launch_bounds__(SEL_T) void k_sel_eval(tdd_pred_t P, int64_t nrows,
     unsigned long long my_total = 0;
     for (int64_t seg = blockIdx.x; seg < n_segs; seg += gridDim.x) {
         uint32_t cnt = 0;
+        const int64_t r0 = (seg * 16 + w * 4) * 64 + lane;
+        const uint32_t pm = pred_eval4(P, r0, nrows);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             int64_t word = seg * 16 + w * 4 + k;
             int64_t row = word * 64 + lane;
-            bool p = row < nrows && pred_eval(P, row);
+            bool p = row < nrows && ((pm >> k) & 1);
             uint64_t m = __ballot(p);
             if (word < n_words) {
                 if (and_bits) m &= and_bits[word];
@@ -176,11 +179,39 @@ __device__ __forceinline__ void copy_elem(const GatherCols& G, int c, int64_t fr
     }
 }
 
-/* mode 0: gather columns; mode 1: write row ids into idx. */
+template <class T>
+__device__ __forceinline__ void compact_col(const void* src, void* dst, const uint64_t* m,
+                                            const int64_t* to, int64_t r0, int lane) {
+    T v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if ((m[k] >> lane) & 1) v[k] = ((const T*)src)[r0 + 64 * k];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if ((m[k] >> lane) & 1) ((T*)dst)[to[k]] = v[k];
+}
+
+/* Dense right column through the key map: the value at key - kmin where
+ * the key is present, else 0. */
+template <class T>
+__device__ __forceinline__ void map_col(const void* src, void* dst, const uint64_t* m,
+                                        const int64_t* to, const int64_t* at, const bool* hit,
+                                        int lane) {
+    T v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = hit[k] ? ((const T*)src)[at[k]] : (T)0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if ((m[k] >> lane) & 1) ((T*)dst)[to[k]] = v[k];
+}
+
+/* mode 0: gather columns (and the map's right columns); mode 1: write row
+ * ids into idx.  Columns are copied one at a time, four rows per lane with
+ * their loads in flight together. */
 __global__ __launch_bounds__(SEL_T) void k_compact(const uint64_t* bits, const uint8_t* flags,
                                                    const int64_t* seg_off, int64_t nrows,
-                                                   uint32_t n_segs, GatherCols G, int64_t* idx,
-                                                   int mode) {
+                                                   uint32_t n_segs, GatherCols G,
+                                                   TddMapGather M, int64_t* idx, int mode) {
     __shared__ uint32_t wpc[SEL_T / WAVE];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int64_t n_words = (nrows + 63) >> 6;
@@ -199,44 +230,76 @@ __global__ __launch_bounds__(SEL_T) void k_compact(const uint64_t* bits, const u
         __syncthreads();
         int64_t pos = seg_off[seg];
         for (int i = 0; i < w; i++) pos += wpc[i];
+        int64_t to[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            int64_t row = (seg * 16 + w * 4 + k) * 64 + lane;
-            if ((m[k] >> lane) & 1) {
-                int64_t to = pos + lanes_below(m[k]);
-                if (mode == 1) {
-                    idx[to] = row;
-                } else {
-                    for (int cc = 0; cc < G.n; cc++) copy_elem(G, cc, row, to);
+            to[k] = pos + lanes_below(m[k]);
+            pos += __popcll(m[k]);
+        }
+        const int64_t r0 = (seg * 16 + w * 4) * 64 + lane;
+        if (mode == 1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((m[k] >> lane) & 1) idx[to[k]] = r0 + 64 * k;
+        } else {
+            for (int cc = 0; cc < G.n; cc++) {
+                switch (G.esz[cc]) {
+                case 8: compact_col<uint64_t>(G.src[cc], G.dst[cc], m, to, r0, lane); break;
+                case 4: compact_col<uint32_t>(G.src[cc], G.dst[cc], m, to, r0, lane); break;
+                case 2: compact_col<uint16_t>(G.src[cc], G.dst[cc], m, to, r0, lane); break;
+                case 1: compact_col<uint8_t>(G.src[cc], G.dst[cc], m, to, r0, lane); break;
+                default:
+                    for (int k = 0; k < 4; k++)
+                        if ((m[k] >> lane) & 1) copy_elem(G, cc, r0 + 64 * k, to[k]);
                 }
             }
-            pos += __popcll(m[k]);
+            if (M.n > 0) {
+                int64_t key[4], at[4];
+                bool hit[4];
+                ld4_i64(M.key, r0, nrows, key);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t i = (uint64_t)key[k] - (uint64_t)M.kmin;
+                    hit[k] = ((m[k] >> lane) & 1) && i < (uint64_t)M.range &&
+                             (!M.present || ((M.present[i >> 5] >> (i & 31)) & 1));
+                    at[k] = hit[k] ? (int64_t)i : 0;
+                }
+                for (int cc = 0; cc < M.n; cc++) {
+                    switch (M.esz[cc]) {
+                    case 8: map_col<uint64_t>(M.src[cc], M.dst[cc], m, to, at, hit, lane); break;
+                    case 4: map_col<uint32_t>(M.src[cc], M.dst[cc], m, to, at, hit, lane); break;
+                    case 2: map_col<uint16_t>(M.src[cc], M.dst[cc], m, to, at, hit, lane); break;
+                    default: map_col<uint8_t>(M.src[cc], M.dst[cc], m, to, at, hit, lane); break;
+                    }
+                }
+            }
         }
         __syncthreads();
     }
 }
 
-static int compact_impl(const tdd_sel_t* s, const GatherCols& G, int64_t* idx, int mode) {
+static int compact_impl(const tdd_sel_t* s, const GatherCols& G, const TddMapGather* M,
+                        int64_t* idx, int mode) {
     if (s->n_segs == 0) return TD_OK;
     SCRATCH(off, (size_t)(s->n_segs + 1) * 8);
     int rc = tdd::scan_u16_i64(s->popcnt, s->n_segs, off.as<int64_t>());
     if (rc) return rc;
+    TddMapGather none;
+    memset(&none, 0, sizeof(none));
     int grid = stream_grid(s->n_segs, 8);
     LAUNCH("compact", k_compact, grid, SEL_T, 0, s->bits, s->flags,
-           (const int64_t*)off.as<int64_t>(), s->nrows, s->n_segs, G, idx, mode);
+           (const int64_t*)off.as<int64_t>(), s->nrows, s->n_segs, G, M ? *M : none, idx, mode);
     return TD_OK;
 }
 
 extern "C" int tdd_sel_index(const tdd_sel_t* s, int64_t* idx) {
     GatherCols G;
     memset(&G, 0, sizeof(G));
-    return compact_impl(s, G, idx, 1);
+    return compact_impl(s, G, nullptr, idx, 1);
 }
 
-extern "C" int tdd_sel_compact(const tdd_sel_t* s, int ncols, const tdd_col_t* src,
-                               void* const* dst) {
+static int fill_cols(GatherCols& G, int ncols, const tdd_col_t* src, void* const* dst) {
     if (ncols < 0 || ncols > MAXC) return TD_ERR_RANGE;
-    GatherCols G;
     memset(&G, 0, sizeof(G));
     for (int c = 0; c < ncols; c++) {
         G.src[c] = src[c].data;
@@ -244,7 +307,27 @@ extern "C" int tdd_sel_compact(const tdd_sel_t* s, int ncols, const tdd_col_t* s
         G.esz[c] = src[c].esz;
     }
     G.n = ncols;
-    return compact_impl(s, G, nullptr, 0);
+    return TD_OK;
+}
+
+int tdd__sel_compact_map(const tdd_sel_t* s, int ncols, const tdd_col_t* src, void* const* dst,
+                         const TddMapGather* M) {
+    GatherCols G;
+    int rc = fill_cols(G, ncols, src, dst);
+    if (rc) return rc;
+    if (M->n < 0 || M->n > TDD_MAP_MAXC) return TD_ERR_RANGE;
+    for (int c = 0; c < M->n; c++)
+        if (M->esz[c] != 1 && M->esz[c] != 2 && M->esz[c] != 4 && M->esz[c] != 8)
+            return TD_ERR_TYPE;
+    return compact_impl(s, G, M, nullptr, 0);
+}
+
+extern "C" int tdd_sel_compact(const tdd_sel_t* s, int ncols, const tdd_col_t* src,
+                               void* const* dst) {
+    GatherCols G;
+    int rc = fill_cols(G, ncols, src, dst);
+    if (rc) return rc;
+    return compact_impl(s, G, nullptr, nullptr, 0);
 }
 
 /* ---------------------------------------------------------------- gather -- */

@@ -1214,7 +1214,7 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
         for (int a = 0; a < ext->n_aggs; a++) td__arena_free(R.vals[a]);
         return TD_ERR_PTR(rc);
     }
-    td__set_plan("group path=%d groups=%lld", R.path, (long long)R.n_groups);
+    td__set_plan("group path=%d groups=%lld tries=%d", R.path, (long long)R.n_groups, R.tries);
     return group_table(g, ext, &R, key_src, agg_src);
 }
 
@@ -1262,8 +1262,8 @@ static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int
 /* INNER / LEFT joins through the materialising radix join: the output
  * columns (left columns, then right non-key columns, 16 each at most,
  * exec.c:8607-8728) come back written.  NULL when that join does not take
- * the query (repeated or dense build keys, oversized partitions): the
- * caller joins through row-index pairs instead. */
+ * the query (repeated build keys, oversized partitions, more columns than
+ * it writes): the caller joins through row-index pairs instead. */
 static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* right,
                                const tdd_join_spec_t* J) {
     td_graph_t* g = x->g;
@@ -1274,17 +1274,19 @@ static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* rig
     td_t* rc_[16];
     int64_t rname[16], lname[16];
     int64_t lnc = td_table_ncols(left), rnc = td_table_ncols(right);
-    for (int64_t c = 0; c < lnc && M.n_lcols < 16; c++) {
+    for (int64_t c = 0; c < lnc; c++) {
         td_t* col = td_table_get_col_idx(left, c);
         if (!col) continue;
+        if (M.n_lcols == 16) return NULL;
         if (dev_col(x, col, &M.lcols[M.n_lcols])) return NULL;
         lname[M.n_lcols] = td_table_col_name(left, c);
         lc[M.n_lcols++] = col;
     }
-    for (int64_t c = 0; c < rnc && M.n_rcols < 16; c++) {
+    for (int64_t c = 0; c < rnc; c++) {
         td_t* col = td_table_get_col_idx(right, c);
         int64_t name = td_table_col_name(right, c);
         if (!col || join_is_right_key(g, je, name)) continue;
+        if (M.n_rcols == 16) return NULL;
         if (dev_col(x, col, &M.rcols[M.n_rcols])) return NULL;
         rname[M.n_rcols] = name;
         rc_[M.n_rcols++] = col;

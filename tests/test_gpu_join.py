@@ -433,3 +433,67 @@ def test_join_materialised_vs_oracle(eng, forced, jt, case):
     assert [c[0] for c in out] == [c[0] for c in lcols] + [c[0] for c in rcols
                                                           if c[0] not in rkeys]
     row_multiset_equal([c[3] for c in out], ref, f"mat {case} {jt}")
+
+
+@pytest.mark.parametrize("jt", [0, 1])
+@pytest.mark.parametrize("case", ["perm", "gaps", "i32_key", "sym16_key", "filtered",
+                                  "far_probes", "empty_build", "dup"])
+def test_join_map_vs_oracle(eng, jt, case):
+    """INNER / LEFT joins whose single integer build key is dense in its range
+    take the materialising map (plan path 4) at the planner's choice: the
+    selected probe rows in row order, left columns compacted, right columns
+    read through the map (zeros without a partner).  `far_probes` puts probe
+    keys at the int64 extremes, outside the map; `dup` repeats build keys,
+    which the map declines for the pair-producing join."""
+    rng = np.random.default_rng(950 + 10 * jt + len(case))
+    nl, nr = 400_009, 50_021
+    rk = rng.permutation(nr) - 20_000
+    lk = rng.integers(-25_000, nr - 15_000, nl)          # some probes outside the range
+    if case == "gaps":
+        rk = np.sort(rng.choice(3 * nr, nr, replace=False)) - 7
+        rng.shuffle(rk)
+        lk = rng.integers(-10, 3 * nr, nl)
+    elif case == "far_probes":
+        lk[rng.integers(0, nl, 5000)] = np.iinfo(np.int64).max
+        lk[rng.integers(0, nl, 5000)] = np.iinfo(np.int64).min
+    elif case == "empty_build":
+        rk = rk[:0]
+    elif case == "dup":
+        rk = rng.integers(-20_000, 30_000, nr)
+    nr = len(rk)
+    lcols = [("k", lk), ("a", rng.integers(-9, 9, nl).astype(np.int32)),
+             ("b", rng.normal(size=nl)), ("c", rng.integers(0, 255, nl).astype(np.uint8))]
+    rcols = [("k", rk), ("w", rng.integers(0, 10 ** 9, nr)), ("x", rng.random(nr)),
+             ("y", rng.integers(0, 60000, nr).astype(np.uint16), 20, 1),
+             ("z", rng.integers(-100, 100, nr).astype(np.int16))]
+    if case == "i32_key":
+        lcols[0] = ("k", lk.astype(np.int32))
+        rcols[0] = ("k", rk.astype(np.int32))
+    elif case == "sym16_key":
+        u = rng.permutation(40000).astype(np.uint16) + 2000
+        rcols = [("k", u[:30000], 20, 1)] + [(c[0], c[1][:30000], *c[2:]) for c in rcols[1:]]
+        lcols[0] = ("k", rng.integers(0, 45000, nl).astype(np.uint16), 20, 1)
+    lt, rt = eng.table(lcols), eng.table(rcols)
+    g = eng.graph(lt)
+    left = g.const_table(lt)
+    if case == "filtered":
+        left = g.filter(left, g.gt(g.scan("a"), g.const_i64(0)))
+    root = g.join(left, [g.scan("k")], g.const_table(rt), [g.scan("k")], jt)
+    out = eng.read_table(g.execute(root))
+    plan = eng.last_plan()
+    g.free()
+    eng.release(lt)
+    eng.release(rt)
+    assert ("join path=4" in plan) == (case != "dup"), plan
+    L = {c[0]: c[1] for c in lcols}
+    Rr = {c[0]: c[1] for c in rcols}
+    m = L["a"] > 0 if case == "filtered" else np.ones(nl, bool)
+    li, ri = orc.join([L["k"][m]], [Rr["k"]], jt)
+    ref = [orc.gather(c[1][m], li) for c in lcols]
+    ref += [orc.gather(c[1], ri, jt == 1) for c in rcols if c[0] != "k"]
+    assert [c[0] for c in out] == [c[0] for c in lcols] + [c[0] for c in rcols[1:]]
+    row_multiset_equal([c[3] for c in out], ref, f"map {case} {jt}")
+    if case != "dup":
+        # the map keeps the probe side's row order
+        keep = np.sort(li) if jt == 0 else np.arange(int(m.sum()))
+        np.testing.assert_array_equal(out[0][3], L["k"][m][keep])
