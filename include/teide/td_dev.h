@@ -146,8 +146,9 @@ int64_t  td_dev_cancel_polls(void);
  * 4 XCD-partitioned fused join-group) / group path (0 auto, 1 direct
  * array, 2 hash, 3 radix-partitioned direct array with small leaf tables,
  * 4 hash-partitioned hash with small LDS tables, 5 hash-partitioned hash
- * after hot-key pre-aggregation at any input size) / radix bits (0 auto) for
- * subsequent td_execute calls. */
+ * after hot-key pre-aggregation at any input size, 6 as 4 with a first try
+ * of 1,024-entry global table and list, which then regrow) / radix bits
+ * (0 auto) for subsequent td_execute calls. */
 void     td_dev_force_paths(int join_path, int group_path, int radix_bits);
 /* The physical plan of the last td_execute as a short text. */
 const char* td_dev_last_plan(void);

@@ -152,7 +152,7 @@ typedef struct {
     tdd_col_t         keys[TDD_MAX_KEYS];
     uint16_t          ops[TDD_MAX_AGGS];  /* OP_SUM ... OP_VAR_POP */
     tdd_col_t         aggs[TDD_MAX_AGGS]; /* data may be NULL for OP_COUNT */
-    int32_t           force_path;         /* 0 auto, 1 direct array, 2 hash, 3 radix-partitioned direct array, 4 hash-partitioned hash, 5 hash-partitioned hash after hot-key pre-aggregation at any size */
+    int32_t           force_path;         /* 0 auto, 1 direct array, 2 hash, 3 radix-partitioned direct array, 4 hash-partitioned hash, 5 hash-partitioned hash after hot-key pre-aggregation at any size, 6 as 4 with a first try of 1,024-entry tables (the regrow path) */
     int32_t           pad;
 } tdd_group_spec_t;
 
@@ -161,8 +161,8 @@ typedef struct {
     void*    keys[TDD_MAX_KEYS];   /* arena blocks; element type = key column type */
     void*    vals[TDD_MAX_AGGS];   /* arena blocks; element type = out_types[a] */
     int8_t   out_types[TDD_MAX_AGGS];
-    int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash, 4 partitioned direct array, 5 partitioned hash, 6 partitioned hash with hot keys folded first */
-    int32_t  pad;
+    int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash, 4 partitioned direct array, 5 partitioned hash, 6 partitioned hash with hot keys folded first; 20 + p: key tuples wider than 64 bits (an F64 among several keys, or integer ranges multiplying past 2^63) numbered by a hash of their raw key words, then grouped by that number on path p */
+    int32_t  tries;                /* hash paths: table sizes tried (1 when the first fitted) */
 } tdd_group_result_t;
 
 /* Masked min/max of an integer key column (minmax_scan_fn, exec.c:4544-4604);
@@ -236,10 +236,13 @@ int tdd_join(const tdd_join_spec_t* spec, tdd_join_result_t* out);
  * so its partitioned buffers are the output); each partition's build keys
  * go into an LDS hash table and its probe rows stream through it.  Rows come
  * out in partition order, as the reference's radix join emits them
- * (exec.c:7431-7950, output columns exec.c:8607-8728).  Returns TD_ERR_NYI,
- * with nothing allocated, when the build side repeats a key, a build
- * partition outgrows an LDS table, the build keys are dense enough for the
- * direct map of tdd_join, or the join is FULL: run tdd_join then. */
+ * (exec.c:7431-7950, output columns exec.c:8607-8728).  A single integer
+ * build key dense in its range uses a map instead of the partitions: the
+ * output rows are then the selected probe rows in row order, left columns
+ * compacted, right columns read through the map (path 4).  Returns
+ * TD_ERR_NYI, with nothing allocated, when the build side repeats a key, a
+ * build partition outgrows an LDS table or the join is FULL: run tdd_join
+ * then. */
 typedef struct {
     tdd_join_spec_t   join;
     int32_t           n_lcols, n_rcols;
@@ -251,7 +254,7 @@ typedef struct {
     int64_t  n_rows;
     void*    lcols[16];           /* arena blocks, element bytes of lcols[c] */
     void*    rcols[16];
-    int32_t  path;                /* 3 materialising radix */
+    int32_t  path;                /* 3 materialising radix, 4 materialising map */
     int32_t  radix_bits;
 } tdd_join_mat_result_t;
 

@@ -739,6 +739,10 @@ int64_t pht_global_keys(const tdd_pht* h);
  * force 4: the split path with small LDS tables. */
 static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
                   tdd_group_result_t* out) {
+    /* force 6: force 4 with a first attempt of 1,024-entry tables (the
+     * overflow and regrow paths) */
+    const bool tiny_first = force == 6;
+    if (force == 6) force = 4;
     SCRATCH(regs, HLL_M * 4);
     int frc = tdd_fill(regs.p, 0, HLL_M * 4);
     if (frc) return frc;
@@ -783,11 +787,18 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         while ((double)g < 2.0 * (double)want) g <<= 1;
         if (g < gcap) gcap = g;
     }
+    if (tiny_first) cap = gcap = 1024;
+    int64_t ol[2] = {0, 0};
     for (int attempt = 0; attempt < 7; attempt++) {
         POLL_CANCEL();
         if (attempt > 0) {                                   /* the last try overflowed */
-            if (gcap < cap) gcap = cap;
-            else { cap <<= 1; gcap = cap; }
+            if ((int32_t)ol[0]) {                            /* the global table: grow it */
+                gcap = gcap < cap ? cap : gcap << 1;
+                if (!ph && cap < gcap) cap = gcap;
+            }
+            if ((int32_t)(ol[0] >> 32)) {                    /* the list: room for every entry */
+                while ((double)cap < 1.5 * (double)ol[1]) cap <<= 1;
+            }
         }
         int64_t n_slots = (int64_t)gcap + 1;                 /* + the EMPTY_KEY slot */
         SCRATCH(keys, (size_t)gcap * 8);
@@ -797,7 +808,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         if (rc) return rc;
         int ib = stream_grid(ceil_div(n_slots * P.ncells, 256 * 8));
         LAUNCH("group_init", k_cells_init, ib, 256, 0, cells.as<uint64_t>(), n_slots, P);
-        rc = tdd_fill(ovf.p, 0, 16);                         /* flag, list count */
+        rc = tdd_fill(ovf.p, 0, 16);          /* table flag, list flag, list count */
         if (rc) return rc;
         Scratch lkeys(ph ? (size_t)cap * 8 : 8), lcells(ph ? (size_t)cap * P.ncells * 8 : 8);
         if (!lkeys.ok() || !lcells.ok()) return TD_ERR_OOM;
@@ -818,10 +829,9 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
             LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
                    (uint64_t)(gcap - 1), cells.as<uint64_t>(), ovf.as<int>());
         }
-        int64_t ol[2] = {0, 0};
         HC(hipMemcpyAsync(ol, ovf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
-        if ((int32_t)ol[0]) continue;                        /* table too small: retry */
+        if (ol[0]) continue;                                 /* a table too small: retry */
         const int64_t n_list = ph ? ol[1] : 0;
         SCRATCH(flag, (size_t)n_slots * 4);
         SCRATCH(pos, (size_t)(n_slots + 1) * 8);
@@ -846,6 +856,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
                    (const uint64_t*)cells.as<uint64_t>(), n_slots,
                    (const int64_t*)pos.as<int64_t>(), n_list, *out);
         out->path = ph ? (hot ? 6 : 5) : 3;
+        out->tries = attempt + 1;
         return TD_OK;
     }
     return TD_ERR_OOM;
@@ -853,6 +864,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
 
 int run_pda(const GroupPlan& P, const uint64_t* mask, int64_t nrows, int64_t n_slots,
             int64_t leaf_bytes, tdd_group_result_t* out);
+int group_tuples(const tdd_group_spec_t* spec, int64_t nsel, tdd_group_result_t* out);
 
 /* --------------------------------------------------------------- driver -- */
 
@@ -936,7 +948,7 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
             return TD_OK;
         }
     }
-    if (P.n_keys > 1 && !composite) return TD_ERR_NYI;
+    if (P.n_keys > 1 && !composite) return group_tuples(spec, nsel, out);
     if (P.n_keys == 1 && !composite) P.kstride[0] = 0;      /* raw 64-bit key */
     if (P.n_keys == 1 && composite && P.keys[0].type != TD_F64) {
         P.kstride[0] = 0;                                    /* integer key: use its value */

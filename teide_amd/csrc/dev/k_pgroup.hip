@@ -476,7 +476,7 @@ __global__ __launch_bounds__(PG_T) void k_pg_hagg(GroupPlan P, PGBuf in, const P
             for (int i = threadIdx.x; i < lcap; i += blockDim.x) {
                 if (lk[i] == EMPTY_KEY || lc[(int64_t)i * nc] == 0) continue;
                 const int64_t at = (int64_t)(lbase + lidx[i]);
-                if (at >= L.cap) { *overflow = 1; continue; }
+                if (at >= L.cap) { overflow[1] = 1; continue; }   /* the list */
                 L.keys[at] = lk[i];
                 for (int c = 0; c < nc; c++) L.cells[at * nc + c] = lc[(int64_t)i * nc + c];
             }
@@ -1003,7 +1003,9 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
 /* Fold the partitions: groups of a part folded by a single job are appended
  * to the list (lkeys / lcells, lcap_out entries, *lcount zeroed); groups of
  * parts split over several jobs merge into the global table (keys / cells,
- * mask + 1 slots plus the extra all-ones slot, already initialised). */
+ * mask + 1 slots plus the extra all-ones slot, already initialised).
+ * overflow[0] is set when the global table has no slot for a key,
+ * overflow[1] when the list holds fewer than *lcount entries. */
 int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gcells,
              unsigned long long* lkeys, uint64_t* lcells, unsigned long long* lcount,
              int64_t lcap_out, int* overflow) {
@@ -1045,8 +1047,11 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
  * split over several jobs merge there (at most one leaf table of keys per
  * job of such a part), plus an allowance for keys that find no room in
  * their part's leaf table. */
+/* Keys the global table takes: every key of a part split over several
+ * jobs (at most a table each), plus an allowance for keys that find no
+ * slot within the probe limit of their part's table. */
 int64_t pht_global_keys(const tdd_pht* h) {
-    int64_t n = 4096;
+    int64_t n = 4096 + 16 * (int64_t)h->jobs.size();
     for (size_t j = 0; j < h->jobs.size(); j++)
         if (h->jobs[j].nj > 1) n += h->lcap;
     return n;
