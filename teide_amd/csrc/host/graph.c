@@ -386,30 +386,37 @@ td_op_t* td_filter(td_graph_t* g, td_op_t* input, td_op_t* pred) {
     return n;
 }
 
+/* GROUP node: after the ext header come the key node pointers, the
+ * aggregate opcodes and (pointer-aligned) the aggregate input pointers.
+ * Inputs are remembered by id, since adding a node may move g->nodes. */
 td_op_t* td_group(td_graph_t* g, td_op_t** keys, uint8_t n_keys,
                   uint16_t* agg_ops, td_op_t** agg_ins, uint8_t n_aggs) {
-    uint32_t kid[256], aid[256];
-    for (int i = 0; i < n_keys; i++) kid[i] = keys[i]->id;
-    for (int i = 0; i < n_aggs; i++) aid[i] = agg_ins[i]->id;
-    size_t ksz = (size_t)n_keys * sizeof(td_op_t*);
-    size_t osz = (size_t)n_aggs * sizeof(uint16_t);
-    size_t ioff = (ksz + osz + sizeof(void*) - 1) & ~(sizeof(void*) - 1);
-    td_op_ext_t* e = new_ext(g, ioff + (size_t)n_aggs * sizeof(td_op_t*));
+    uint32_t ids[512];
+    for (int i = 0; i < n_keys + n_aggs; i++)
+        ids[i] = i < n_keys ? keys[i]->id : agg_ins[i - n_keys]->id;
+    const size_t at_ops = (size_t)n_keys * sizeof(td_op_t*);
+    size_t at_ins = at_ops + (size_t)n_aggs * sizeof(uint16_t);
+    at_ins += (sizeof(td_op_t*) - at_ins % sizeof(td_op_t*)) % sizeof(td_op_t*);
+    td_op_ext_t* e = new_ext(g, at_ins + (size_t)n_aggs * sizeof(td_op_t*));
     if (!e) return NULL;
-    e->base.opcode = OP_GROUP;
-    e->base.out_type = TD_TABLE;
-    if (n_keys) {
-        e->base.est_rows = g->nodes[kid[0]].est_rows / 10;
-        e->base.inputs[0] = &g->nodes[kid[0]];
-    }
-    e->keys = (td_op_t**)TRAIL(e);
-    for (int i = 0; i < n_keys; i++) e->keys[i] = &g->nodes[kid[i]];
-    e->agg_ops = (uint16_t*)(TRAIL(e) + ksz);
-    if (n_aggs) memcpy(e->agg_ops, agg_ops, osz);
-    e->agg_ins = (td_op_t**)(TRAIL(e) + ioff);
-    for (int i = 0; i < n_aggs; i++) e->agg_ins[i] = &g->nodes[aid[i]];
+    char* trail = TRAIL(e);
+    e->keys = (td_op_t**)trail;
+    e->agg_ops = (uint16_t*)(trail + at_ops);
+    e->agg_ins = (td_op_t**)(trail + at_ins);
     e->n_keys = n_keys;
     e->n_aggs = n_aggs;
+    for (int i = 0; i < n_keys; i++) e->keys[i] = &g->nodes[ids[i]];
+    for (int a = 0; a < n_aggs; a++) {
+        e->agg_ops[a] = agg_ops[a];
+        e->agg_ins[a] = &g->nodes[ids[n_keys + a]];
+    }
+    e->base.opcode = OP_GROUP;
+    e->base.out_type = TD_TABLE;
+    if (n_keys > 0) {
+        /* a tenth of the first key's rows: the planner's group-count guess */
+        e->base.inputs[0] = e->keys[0];
+        e->base.est_rows = e->keys[0]->est_rows / 10;
+    }
     return publish(g, e);
 }
 

@@ -99,25 +99,25 @@ void* td_vec_get(td_t* vec, int64_t idx) {
     return (char*)td_data(vec) + idx * td__esz(vec);
 }
 
+/* A header-only view of elements [offset, offset + len): it records the
+ * vector that owns the bytes (for a view of a view, that view's owner) and
+ * holds a reference on it. */
 td_t* td_vec_slice(td_t* vec, int64_t offset, int64_t len) {
     if (!vec || TD_IS_ERR(vec)) return vec;
-    if (offset < 0 || len < 0 || offset > vec->len || len > vec->len - offset)
-        return TD_ERR_PTR(TD_ERR_RANGE);
-    td_t* parent = vec;
-    int64_t off = offset;
-    if (vec->attrs & TD_ATTR_SLICE) {
-        parent = vec->slice_parent;
-        off += vec->slice_offset;
-    }
-    td_t* s = td_alloc(0);
-    if (!s) return TD_ERR_PTR(TD_ERR_OOM);
-    s->type = parent->type;
-    s->attrs = TD_ATTR_SLICE | (parent->attrs & TD_SYM_W_MASK);
-    s->len = len;
-    s->slice_parent = parent;
-    s->slice_offset = off;
-    td_retain(parent);
-    return s;
+    const int in_range = offset >= 0 && len >= 0 && offset <= vec->len &&
+                         len <= vec->len - offset;
+    if (!in_range) return TD_ERR_PTR(TD_ERR_RANGE);
+    const int of_view = (vec->attrs & TD_ATTR_SLICE) != 0;
+    td_t* owner = of_view ? vec->slice_parent : vec;
+    td_t* view = td_alloc(0);
+    if (!view) return TD_ERR_PTR(TD_ERR_OOM);
+    view->type = owner->type;
+    view->attrs = (uint8_t)((owner->attrs & TD_SYM_W_MASK) | TD_ATTR_SLICE);
+    view->len = len;
+    view->slice_offset = (of_view ? vec->slice_offset : 0) + offset;
+    view->slice_parent = owner;
+    td_retain(owner);
+    return view;
 }
 
 td_t* td_vec_concat(td_t* a, td_t* b) {
