@@ -205,6 +205,44 @@ class JoinGroup:
         self.e.release(self.td)
 
 
+class JoinGroupTwoCall(JoinGroup):
+    """The join + group-by in the reference's own form: graph 1 runs
+    JOIN(FILTER(fact, qty > 50), fk = pk) to a joined table in HBM, graph 2
+    runs GROUP(cat; SUM price, SUM qty, COUNT price) over it."""
+    name = "jg2"
+
+    def __init__(self, e, a):
+        super().__init__(e, a)
+        self.config = dict(self.config, form="two graphs: join to a resident joined table, "
+                                             "then group-by over it")
+
+    def step(self):
+        e = self.e
+        g = e.graph(self.tf)
+        flt = g.filter(g.const_table(self.tf), g.gt(g.scan("qty"), g.const_i64(50)))
+        j = g.execute(g.join(flt, [g.scan("fk")], g.const_table(self.td), [g.scan("pk")], 0))
+        g.free()
+        g2 = e.graph(j)
+        r = g2.group([g2.scan("cat")], [("SUM", g2.scan("price")), ("SUM", g2.scan("qty")),
+                                         ("COUNT", g2.scan("price"))])
+        e.release(g2.execute(r))
+        g2.free()
+        e.release(j)
+
+    def kernel_bytes(self, name):
+        a = self.a
+        if name == "compact":
+            # the passing fact rows' fk, qty, price read (every line of the
+            # three columns at 50 % selectivity) and written, the selection
+            # words read, cat read at the row's key and written
+            return a.fact * 24 + a.fact // 2 * (24 + 4 + 4) + a.fact // 8
+        if name == "rjoin_map_sel":
+            return a.fact * 8 + a.fact // 8 * 2
+        if name == "rjoin_map":
+            return a.dim * (8 + 4 + 4)
+        return None
+
+
 class JoinGroupScattered(JoinGroup):
     """The join + group-by workload with its join keys spread over 64-bit
     values (fk, pk -> key * 7,000,000,019 - 2^62: still unique, no longer a
@@ -673,12 +711,21 @@ class Served:
             t = int(rng.integers(1, 101))
             key = "cat" if rng.random() < 0.5 else "k64"
             self.pool.append(self._dag(key, t) + (key, t))
+        # requests the cube cannot answer: price > t (f64), t from the
+        # lognormal(3, 1) price distribution's quantiles 5 %..95 %
+        self.wide_pool = []
+        for _ in range(min(a.serve_pool, 256)):
+            t = float(np.exp(3.0 + 1.6449 * rng.uniform(-1.0, 1.0)))
+            key = "cat" if rng.random() < 0.5 else "k64"
+            self.wide_pool.append(self._dag(key, t, wide=True) + (key, t))
         self.req_bytes = nf * 24 + nd * 12 + 2000 * 28      # one request scanned alone
 
-    def _dag(self, key, t):
+    def _dag(self, key, t, wide=False):
         e, tf, td = self.e, self.tf, self.td
         g = e.graph(tf)
-        flt = g.filter(g.const_table(tf), g.gt(g.scan("qty"), g.const_i64(t)))
+        pred = g.gt(g.scan("price"), g.const_f64(t)) if wide else \
+            g.gt(g.scan("qty"), g.const_i64(t))
+        flt = g.filter(g.const_table(tf), pred)
         g.join(flt, [g.scan("fk")], g.const_table(td), [g.scan("pk")], 0)
         root = g.group([g.scan(key)], [("SUM", g.scan("price")), ("SUM", g.scan("qty")),
                                         ("COUNT", g.scan("price"))])
@@ -689,14 +736,15 @@ class Served:
                                  budget=self.BUDGET, chunk_rows=self.a.serve_chunk,
                                  max_active=self.a.serve_slots)
 
-    def plain(self, n):
+    def plain(self, n, pool=None):
         """Sequential td_execute of n pool requests: seconds per request."""
         e = self.e
-        e.release(self.pool[0][0].execute(self.pool[0][1]))
+        pool = pool or self.pool
+        e.release(pool[0][0].execute(pool[0][1]))
         e.L.td_dev_sync()
         t0 = time.perf_counter()
         for i in range(n):
-            g, root = self.pool[i % len(self.pool)][:2]
+            g, root = pool[i % len(pool)][:2]
             e.release(g.execute(root))
         e.L.td_dev_sync()
         return (time.perf_counter() - t0) / n
@@ -740,15 +788,16 @@ class Served:
                 "max_ms": round(float(lat.max()) * 1e3, 2), "steps": steps,
                 "shared": info["shared"], "alone": info["alone"]}
 
-    def closed(self, clients, seconds):
+    def closed(self, clients, seconds, pool=None):
         """Closed loop: `clients` outstanding requests, each replaced on
         completion, for about `seconds`."""
         np, e = self.np, self.e
+        pool = pool or self.pool
         h = self._open()
         owner, lat, nxt = {}, [], 0
         t0 = time.perf_counter()
         for _ in range(clients):
-            g, root = self.pool[nxt % len(self.pool)][:2]
+            g, root = pool[nxt % len(pool)][:2]
             owner[e.serve_submit(h, g, root)] = 0.0
             nxt += 1
         done = 0
@@ -762,7 +811,7 @@ class Served:
                 e.release(res)
                 done += 1
                 if t < seconds:
-                    g, root = self.pool[nxt % len(self.pool)][:2]
+                    g, root = pool[nxt % len(pool)][:2]
                     owner[e.serve_submit(h, g, root)] = t
                     nxt += 1
             if not owner:
@@ -774,7 +823,8 @@ class Served:
         return {"clients": clients, "requests": done, "qps": round(done / end, 1),
                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                "chunks": info["chunks"], "held_gb_fixed": round(info["base"] / 1e9, 2)}
+                "chunks": info["chunks"], "held_gb_fixed": round(info["base"] / 1e9, 2),
+                "shared": info["shared"], "alone": info["alone"]}
 
     def run(self):
         e = self.e
@@ -786,6 +836,19 @@ class Served:
         cap = self.closed(self.a.serve_clients, self.a.serve_cap_seconds)
         plain_s = self.plain(self.a.serve_plain)
         plain_qps = 1.0 / plain_s
+        e.prof_reset()
+        e.prof(True)
+        wide = self.closed(self.a.serve_wide_clients, self.a.serve_cap_seconds, self.wide_pool)
+        wprof = e.prof_read()
+        e.prof(False)
+        wide_plain = 1.0 / self.plain(max(4, self.a.serve_plain // 4), self.wide_pool)
+        wide.update(workload="price > t (f64; t between the price's 5 % and 95 % quantiles), "
+                             "group by cat or k64, same aggregates: each request's predicate "
+                             "evaluated on the shared chunks",
+                    plain_qps=round(wide_plain, 1),
+                    served_vs_plain=round(wide["qps"] / wide_plain, 2),
+                    kernels={k: {"launches": n, "avg_ms": round(tot / max(n, 1), 4)}
+                             for k, (n, tot) in wprof.items()})
         kern = {k: {"launches": n, "avg_ms": round(tot / max(n, 1), 4)}
                 for k, (n, tot) in prof.items()}
         return {"workload": f"Poisson {self.a.serve_rate}/s for {self.a.serve_seconds}s over "
@@ -796,10 +859,10 @@ class Served:
                 "served_vs_plain": round(cap["qps"] / plain_qps, 2),
                 "stream_vs_plain": round(st["qps"] / plain_qps, 2),
                 "effective_gbs": round(cap["qps"] * self.req_bytes / 1e9, 1),
-                "kernels": kern}
+                "kernels": kern, "wide": wide}
 
     def release(self):
-        for g, _, _, _ in self.pool:
+        for g, _, _, _ in self.pool + self.wide_pool:
             g.free()
         self.e.release(self.tf)
         self.e.release(self.td)
@@ -983,10 +1046,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "jgs", "hc", "batch", "served", "load", "charge",
-                                           "join", "parted"],
+    ap.add_argument("--workload", choices=["jg", "jg2", "jgs", "hc", "batch", "served", "load",
+                                           "charge", "join", "parted"],
                     default="jg",
-                    help="jg: join + group-by (default); hc: high-cardinality group-by; "
+                    help="jg: join + group-by (default); jg2: the same query as two graphs "
+                         "(join, then group-by over the joined table); jgs: jg with scattered "
+                         "64-bit join keys; hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
                          "served: request stream through the request server; "
                          "load: splayed-table load into HBM")
@@ -1005,6 +1070,8 @@ def main():
     ap.add_argument("--serve-clients", type=int, default=2048,
                     help="outstanding requests of the closed-loop capacity run")
     ap.add_argument("--serve-cap-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-wide-clients", type=int, default=256,
+                    help="outstanding requests of the closed-loop run with price > t filters")
     ap.add_argument("--serve-chunk", type=int, default=0, help="fact rows per scan step")
     ap.add_argument("--serve-slots", type=int, default=4096)
     ap.add_argument("--serve-pool", type=int, default=512, help="distinct request DAGs")
@@ -1064,7 +1131,8 @@ def main():
         return
 
     wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge, "jgs": JoinGroupScattered,
-          "parted": Parted, "join": Join}.get(a.workload, JoinGroup)(e, a)
+          "jg2": JoinGroupTwoCall, "parted": Parted, "join": Join}.get(a.workload,
+                                                                       JoinGroup)(e, a)
 
     m = measure(e, wl, a.steps, a.warmup, not a.no_prof)
     ms, plan, kernels, roofline = m["ms"], m["plan"], m["kernels"], m["roofline"]
@@ -1089,7 +1157,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "i64/f64",
-        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "data": "synthetic (seeded generator, inputs resident in HBM; integer column "
+                "min/max statistics computed once at td_dev_put are reused by every step "
+                "where the reference scans key ranges per query)",
         "config": dict(wl.config, plan=plan),
         "query_gbs": round(query_gbs, 1),
         "query_frac_hbm": round(query_gbs / HBM_PEAK_GBS, 4),

@@ -375,6 +375,10 @@ int tdd_sort_perm(const tdd_sort_key_t* keys, int nk, int64_t n, int64_t* perm);
  * adds them into its own direct-array group table.  A request that starts
  * at any step sees every chunk once after n_chunks steps.  COUNT and
  * integer sums are exact; f64 sums add the same values in another order.
+ * A request with any other filter (req.pred: an f64 or wide-domain column,
+ * several comparisons) shares the same chunks: each step evaluates its
+ * predicate on the chunk's rows and folds the passing rows into its table
+ * (through LDS copies of the tables of up to 16 requests at a time).
  * The dim join keys must be unique integers (a direct map per group key,
  * built at open and kept resident).  No reference counterpart: the
  * reference runs each request as its own two td_execute calls. */
@@ -398,6 +402,10 @@ typedef struct {
     int32_t  n_aggs, pad;
     uint16_t ops[TDD_MAX_AGGS];           /* OP_SUM, OP_AVG or OP_COUNT */
     int8_t   src[TDD_MAX_AGGS];           /* value column index, -2: pcol, -1: none (COUNT) */
+    /* non-NULL: the request's own filter over fact rows (columns of nf
+     * rows), evaluated on every chunk of the shared scan instead of cmp / t;
+     * copied at tdd_srv_begin */
+    const tdd_pred_t* pred;
 } tdd_srv_req_t;
 
 typedef struct tdd_srv tdd_srv_t;

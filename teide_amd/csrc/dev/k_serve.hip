@@ -24,17 +24,41 @@
  * (3 x 8 bytes for the fact layout fk i64, qty i64, price f64) for every
  * request in the mix, so the HBM cost of the scan is shared; the atomics
  * (1 + value columns per key) bound it, not the bytes.
+ *
+ * Requests whose filter is anything else (a comparison on an f64 or
+ * wide-domain column, several comparisons) are direct requests: they share
+ * the same chunk with their own predicate program.
+ *
+ *   k_srv_bhist / k_srv_bscatter   once per step and group key: the
+ *                  chunk's rows with a dim row, counting-sorted by group
+ *                  slice (SG consecutive groups) into records holding the
+ *                  group and every column the step's direct requests read
+ *   k_srv_bfold    (slice, request group): each workgroup reads one slice's
+ *                  records once for up to 16 requests, evaluates each
+ *                  request's predicate on them and folds the passing records
+ *                  into LDS slices of the requests' tables, then adds the
+ *                  slices into the tables (one workgroup per slice and
+ *                  request: no atomics on the tables)
+ *   k_srv_direct   requests with more groups than 4,096 slices cover: every
+ *                  row of the chunk folded with global atomics
  */
 #include <vector>
 
 #include "dev_common.h"
 #include "group_plan.h"
+#include "pred_eval.h"
 #include "tile.h"
 
 #define SRV_MAX_Q     4096                    /* pcol domain */
 #define SRV_MAX_CELLS ((int64_t)1 << 26)      /* Q x G per key */
 #define SRV_R         4                       /* rows per lane of a scan tile */
 #define SRV_MAX_Y     65535                   /* grid.y limit */
+#define SRV_DIR_LDS   (64 * 1024)             /* LDS table slices of one request group */
+#define SRV_DIR_GRP   16                      /* direct requests per group */
+#define SRV_SLICE_B   4096                    /* LDS bytes of one request's table slice */
+#define SRV_MAX_S     4096                    /* group slices per key */
+#define SRV_REC_C     8                       /* record columns */
+#define SRV_TILE      4096                    /* rows per bucket-sort tile */
 
 enum { M_NONE = 0, M_RAW = 1, M_SFX = 2, M_PFX = 3 };
 
@@ -89,6 +113,64 @@ struct tdd_srv_acc {
     GroupPlan     P;
     tdd_srv_req_t req;
     SrvAccArgs    a;
+    int32_t       direct;          /* 1: own predicate, k_srv_direct */
+    tdd_pred_t    pred;
+};
+
+struct SrvDirReq {                 /* one direct request of a step */
+    tdd_pred_t P;
+    uint64_t*  tab;                /* [G][ncells] */
+    int64_t    G;
+    int32_t    key, ncells;
+    int32_t    lds;                /* word offset of its LDS table, -1: the global table */
+    int32_t    pad;
+    uint8_t    kind[GP_MAX_CELLS];
+    int8_t     src[GP_MAX_CELLS];  /* ADD cells: value column, -2 pcol */
+};
+
+struct SrvDirGroup { int32_t first, n, words, pad; };
+
+/* Request descriptors are read-only during a launch and uniform across a
+ * workgroup: read through the constant address space they are scalar loads
+ * (through a plain global pointer every byte of a predicate program would
+ * be a vector load per lane). */
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) SrvDirReq ConstReq;
+#else
+typedef const SrvDirReq ConstReq;        /* the host pass only parses the kernels */
+#endif
+
+struct SrvBucket {                 /* one key's bucket sort of a chunk */
+    tdd_col_t      fk;
+    const int32_t* map;
+    int64_t        r0, r1, kmin, range, ntiles;
+    int32_t        sg, S;          /* groups per slice, slices */
+    int32_t        nc, pad;
+    tdd_col_t      src[SRV_REC_C]; /* fact columns carried into the records */
+    uint32_t*      hist;           /* [S][ntiles] */
+    const int64_t* off;            /* exclusive scan of hist, + total */
+    uint32_t*      rg;             /* record group */
+    void*          rc[SRV_REC_C];  /* record columns (the source's element size) */
+};
+
+struct SrvFoldArgs {
+    const uint32_t*    rg;
+    const int64_t*     off;
+    int64_t            ntiles, G;
+    int32_t            sg, ngroups;
+    int32_t            parts, pad;     /* workgroups per (slice, request group) */
+    const SrvDirReq*   reqs;       /* P over record columns, src = record column */
+    const SrvDirGroup* groups;
+};
+
+struct SrvDirArgs {
+    tdd_col_t          fk, pcol;   /* pcol.data NULL: no request sums it */
+    tdd_col_t          vals[TDD_SRV_MAX_VALS];   /* data NULL: unused */
+    const int32_t*     map[TDD_SRV_MAX_KEYS];
+    int64_t            r0, r1, kmin, range, span;
+    int32_t            ngroups, pad;
+    const SrvDirReq*   reqs;
+    const SrvDirGroup* groups;
 };
 
 /* --------------------------------------------------------------- kernels -- */
@@ -153,6 +235,207 @@ __global__ __launch_bounds__(256) void k_srv_cube(SrvScanArgs S) {
                     else
                         atomicAdd((unsigned long long*)&K.val[j][cell], (unsigned long long)v[j][i]);
                 }
+            }
+        }
+    }
+}
+
+/* Row I of a lane's four: fold it into request R's table when its
+ * predicate bit is set and its key has a dim row.  I is a compile-time
+ * index so the row values stay in registers. */
+template <int I>
+__device__ __forceinline__ void dir_fold(ConstReq& R, const SrvDirArgs& A, uint32_t pm,
+                                         bool in, const int64_t (&fk)[4], const int64_t (&q)[4],
+                                         const int64_t (&v)[TDD_SRV_MAX_VALS][4], uint64_t* lt) {
+    const uint64_t k = (uint64_t)fk[I] - (uint64_t)A.kmin;
+    if (!in || !((pm >> I) & 1) || k >= (uint64_t)A.range) return;
+    const int32_t g = A.map[R.key][k];
+    if (g < 0) return;
+    uint64_t* c = R.lds >= 0 ? lt + R.lds + (int64_t)g * R.ncells : R.tab + (int64_t)g * R.ncells;
+    atomicAdd((unsigned long long*)&c[0], 1ull);
+    for (int cc = 1; cc < R.ncells; cc++) {
+        uint64_t x;
+        switch (R.src[cc]) {                       /* value column, or pcol (-2) */
+        case 0: x = (uint64_t)v[0][I]; break;
+        case 1: x = (uint64_t)v[1][I]; break;
+        case 2: x = (uint64_t)v[2][I]; break;
+        case 3: x = (uint64_t)v[3][I]; break;
+        default: x = (uint64_t)q[I]; break;
+        }
+        if (R.kind[cc] == C_ADD_F64)
+            atomicAdd((double*)&c[cc], __longlong_as_double((long long)x));
+        else
+            atomicAdd((unsigned long long*)&c[cc], (unsigned long long)x);
+    }
+}
+
+/* Group slice of row r, -1 without a dim row. */
+__device__ __forceinline__ int32_t bucket_slice(const SrvBucket& B, int64_t r, int32_t* g) {
+    const uint64_t k = (uint64_t)ld_i64(B.fk, r) - (uint64_t)B.kmin;
+    *g = k < (uint64_t)B.range ? B.map[k] : -1;
+    return *g >= 0 ? *g / B.sg : -1;
+}
+
+__global__ __launch_bounds__(256) void k_srv_bhist(SrvBucket B) {
+    extern __shared__ uint32_t h[];
+    for (int i = threadIdx.x; i < B.S; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int64_t t = blockIdx.x, lo = B.r0 + t * SRV_TILE;
+    const int64_t hi = lo + SRV_TILE < B.r1 ? lo + SRV_TILE : B.r1;
+    for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+        int32_t g;
+        const int32_t sl = bucket_slice(B, r, &g);
+        if (sl >= 0) atomicAdd(&h[sl], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < B.S; i += blockDim.x) B.hist[(int64_t)i * B.ntiles + t] = h[i];
+}
+
+template <class T>
+__device__ __forceinline__ void rec_put(const SrvBucket& B, int c, int64_t r, int64_t at) {
+    ((T*)B.rc[c])[at] = ((const T*)B.src[c].data)[r];
+}
+
+/* Records of a tile at their slice's cursor (a slice's records in no
+ * particular order within the tile: the folds are sums). */
+__global__ __launch_bounds__(256) void k_srv_bscatter(SrvBucket B) {
+    extern __shared__ unsigned long long cur[];
+    const int64_t t = blockIdx.x, lo = B.r0 + t * SRV_TILE;
+    const int64_t hi = lo + SRV_TILE < B.r1 ? lo + SRV_TILE : B.r1;
+    for (int i = threadIdx.x; i < B.S; i += blockDim.x)
+        cur[i] = (unsigned long long)B.off[(int64_t)i * B.ntiles + t];
+    __syncthreads();
+    for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+        int32_t g;
+        const int32_t sl = bucket_slice(B, r, &g);
+        if (sl < 0) continue;
+        const int64_t at = (int64_t)atomicAdd(&cur[sl], 1ull);
+        B.rg[at] = (uint32_t)g;
+        for (int c = 0; c < B.nc; c++) {
+            switch (B.src[c].esz) {
+            case 8: rec_put<uint64_t>(B, c, r, at); break;
+            case 4: rec_put<uint32_t>(B, c, r, at); break;
+            case 2: rec_put<uint16_t>(B, c, r, at); break;
+            default: rec_put<uint8_t>(B, c, r, at); break;
+            }
+        }
+    }
+}
+
+/* Record I of a lane's four into request R's LDS slice table. */
+template <int I>
+__device__ __forceinline__ void bfold_rec(ConstReq& R, uint32_t pm, bool in,
+                                          const uint32_t (&g)[4], int64_t gbase, int64_t rec,
+                                          uint64_t* lt) {
+    if (!in || !((pm >> I) & 1)) return;
+    uint64_t* c = lt + R.lds + ((int64_t)g[I] - gbase) * R.ncells;
+    atomicAdd((unsigned long long*)&c[0], 1ull);
+    for (int cc = 1; cc < R.ncells; cc++) {
+        const tdd_col_t col = R.P.cols[R.src[cc]];
+        if (R.kind[cc] == C_ADD_F64)
+            atomicAdd((double*)&c[cc], ld_f64(col, rec));
+        else
+            atomicAdd((unsigned long long*)&c[cc], (unsigned long long)ld_i64(col, rec));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_srv_bfold(SrvFoldArgs F) {
+    extern __shared__ uint64_t lt[];
+    /* block -> (slice, part, request group), the group varying fastest so
+     * that workgroups running together read the same records */
+    const int q = (int)(blockIdx.x % (unsigned)F.ngroups);
+    const int64_t sp = blockIdx.x / (unsigned)F.ngroups;
+    const int64_t sl = sp / F.parts, part = sp % F.parts;
+    const SrvDirGroup grp = F.groups[q];
+    for (int i = threadIdx.x; i < grp.words; i += blockDim.x) lt[i] = 0;
+    __syncthreads();
+    const int64_t b0 = F.off[sl * F.ntiles], b1 = F.off[(sl + 1) * F.ntiles];
+    const int64_t per = ((b1 - b0 + F.parts - 1) / F.parts + 255) / 256 * 256;
+    const int64_t beg = b0 + part * per < b1 ? b0 + part * per : b1;
+    const int64_t end = beg + per < b1 ? beg + per : b1;
+    const int64_t gbase = sl * F.sg;
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    for (int64_t t = beg + (int64_t)w * 256; t < end; t += 1024) {
+        const int64_t r0 = t + lane;
+        uint32_t g[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) g[i] = r0 + 64 * i < end ? F.rg[r0 + 64 * i] : 0u;
+        for (int j = 0; j < grp.n; j++) {
+            ConstReq& R = ((ConstReq*)F.reqs)[grp.first + j];
+            const uint32_t pm = pred_eval4(R.P, r0, end);
+            bfold_rec<0>(R, pm, r0 < end, g, gbase, r0, lt);
+            bfold_rec<1>(R, pm, r0 + 64 < end, g, gbase, r0 + 64, lt);
+            bfold_rec<2>(R, pm, r0 + 128 < end, g, gbase, r0 + 128, lt);
+            bfold_rec<3>(R, pm, r0 + 192 < end, g, gbase, r0 + 192, lt);
+        }
+    }
+    __syncthreads();
+    /* this slice of these requests' tables: added by this workgroup alone
+     * when the slice has one part, else with atomics */
+    for (int j = 0; j < grp.n; j++) {
+        ConstReq& R = ((ConstReq*)F.reqs)[grp.first + j];
+        const int64_t n = gbase + F.sg <= R.G ? F.sg : R.G - gbase;
+        for (int64_t i = threadIdx.x; i < n * R.ncells; i += blockDim.x) {
+            const uint64_t x = lt[R.lds + i];
+            if (!x) continue;
+            uint64_t* d = R.tab + gbase * R.ncells + i;
+            const bool f = R.kind[i % R.ncells] == C_ADD_F64;
+            if (F.parts > 1) {
+                if (f) atomicAdd((double*)d, __longlong_as_double((long long)x));
+                else atomicAdd((unsigned long long*)d, (unsigned long long)x);
+            } else if (f) {
+                *d = (uint64_t)__double_as_longlong(__longlong_as_double((long long)*d) +
+                                                    __longlong_as_double((long long)x));
+            } else {
+                *d += x;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_srv_direct(SrvDirArgs A) {
+    extern __shared__ uint64_t lt[];
+    const int gi = (int)(blockIdx.x % (unsigned)A.ngroups);
+    const int64_t rb = blockIdx.x / (unsigned)A.ngroups;
+    const SrvDirGroup grp = A.groups[gi];
+    for (int i = threadIdx.x; i < grp.words; i += blockDim.x) lt[i] = 0;  /* COUNT / ADD: 0 */
+    __syncthreads();
+    const int64_t lo = A.r0 + rb * A.span;
+    const int64_t hi = lo + A.span < A.r1 ? lo + A.span : A.r1;
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    for (int64_t t = lo + (int64_t)w * 256; t < hi; t += 1024) {
+        const int64_t r0 = t + lane;               /* rows r0 + 64 i, i = 0..3 */
+        int64_t fk[4], q[4] = {0, 0, 0, 0}, v[TDD_SRV_MAX_VALS][4];
+        ld4_i64(A.fk, r0, hi, fk);
+        if (A.pcol.data) ld4_i64(A.pcol, r0, hi, q);
+#pragma unroll
+        for (int j = 0; j < TDD_SRV_MAX_VALS; j++) {
+            if (A.vals[j].data) ld4_i64(A.vals[j], r0, hi, v[j]);
+            else for (int i = 0; i < 4; i++) v[j][i] = 0;
+        }
+        for (int j = 0; j < grp.n; j++) {
+            ConstReq& R = ((ConstReq*)A.reqs)[grp.first + j];
+            const uint32_t pm = pred_eval4(R.P, r0, hi);
+            dir_fold<0>(R, A, pm, r0 < hi, fk, q, v, lt);
+            dir_fold<1>(R, A, pm, r0 + 64 < hi, fk, q, v, lt);
+            dir_fold<2>(R, A, pm, r0 + 128 < hi, fk, q, v, lt);
+            dir_fold<3>(R, A, pm, r0 + 192 < hi, fk, q, v, lt);
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < grp.n; j++) {             /* LDS tables into the requests' tables */
+        ConstReq& R = ((ConstReq*)A.reqs)[grp.first + j];
+        if (R.lds < 0) continue;
+        for (int64_t g = threadIdx.x; g < R.G; g += blockDim.x) {
+            const uint64_t* c = lt + R.lds + g * R.ncells;
+            if (!c[0]) continue;
+            uint64_t* d = R.tab + g * R.ncells;
+            atomicAdd((unsigned long long*)&d[0], (unsigned long long)c[0]);
+            for (int cc = 1; cc < R.ncells; cc++) {
+                if (R.kind[cc] == C_ADD_F64)
+                    atomicAdd((double*)&d[cc], __longlong_as_double((long long)c[cc]));
+                else
+                    atomicAdd((unsigned long long*)&d[cc], (unsigned long long)c[cc]);
             }
         }
     }
@@ -395,8 +678,21 @@ static int req_plan(const tdd_srv_t* s, const tdd_srv_req_t* r, tdd_srv_acc_t* A
     if (r->key < 0 || r->key >= S.n_keys || !s->ok[r->key] || r->n_aggs < 0 ||
         r->n_aggs > TDD_MAX_AGGS)
         return TD_ERR_NYI;
-    if (r->cmp != OP_EQ && r->cmp != OP_LT && r->cmp != OP_LE && r->cmp != OP_GT && r->cmp != OP_GE)
+    if (r->pred) {                                  /* direct: its own predicate program */
+        if (r->pred->n_ins < 1 || r->pred->n_ins > TDD_PRED_MAX_INS) return TD_ERR_NYI;
+        for (int i = 0; i < r->pred->n_ins; i++) {
+            const tdd_pred_ins_t& in = r->pred->ins[i];
+            const bool cols = in.kind == TDD_P_CMP_CONST || in.kind == TDD_P_CMP_COL ||
+                              in.kind == TDD_P_BOOLCOL;
+            if (cols && (in.a >= r->pred->n_cols || !r->pred->cols[in.a].data)) return TD_ERR_NYI;
+            if (in.kind == TDD_P_CMP_COL &&
+                (in.b >= r->pred->n_cols || !r->pred->cols[in.b].data))
+                return TD_ERR_NYI;
+        }
+    } else if (r->cmp != OP_EQ && r->cmp != OP_LT && r->cmp != OP_LE && r->cmp != OP_GT &&
+               r->cmp != OP_GE) {
         return TD_ERR_NYI;
+    }
     GroupPlan& P = A->P;
     P.n_keys = 1;
     P.n_aggs = r->n_aggs;
@@ -427,7 +723,14 @@ static int req_plan(const tdd_srv_t* s, const tdd_srv_req_t* r, tdd_srv_acc_t* A
         a.src[i] = i ? (int8_t)r->src[P.col[i]] : -1;
         if (i && P.kind[i] != C_ADD_F64 && P.kind[i] != C_ADD_I64) return TD_ERR_NYI;
     }
-    req_cell(s, r->cmp, r->t, &a.mode, &a.idx);
+    if (r->pred) {
+        A->direct = 1;
+        A->pred = *r->pred;
+        A->req.pred = nullptr;                      /* the caller's program is not kept */
+        a.mode = M_NONE;                            /* not folded from the cube */
+    } else {
+        req_cell(s, r->cmp, r->t, &a.mode, &a.idx);
+    }
     return TD_OK;
 }
 
@@ -463,14 +766,256 @@ extern "C" void tdd_srv_acc_free(tdd_srv_acc_t* A) {
     free(A);
 }
 
+/* Distinct fact columns a direct request reads (its predicate's, then its
+ * sums' sources); -1 when its predicate program plus its sums would pass
+ * TDD_PRED_MAX_COLS columns. */
+static int req_cols(const tdd_srv_t* s, const tdd_srv_acc_t* A, tdd_col_t* out) {
+    int n = 0;
+    auto add = [&](const tdd_col_t& c) {
+        for (int i = 0; i < n; i++) if (out[i].data == c.data && out[i].type == c.type) return;
+        out[n++] = c;
+    };
+    const tdd_pred_t& P = A->pred;
+    for (int i = 0; i < P.n_ins; i++) {
+        const tdd_pred_ins_t& in = P.ins[i];
+        if (in.kind == TDD_P_CMP_CONST || in.kind == TDD_P_CMP_COL || in.kind == TDD_P_BOOLCOL)
+            add(P.cols[in.a]);
+        if (in.kind == TDD_P_CMP_COL) add(P.cols[in.b]);
+    }
+    for (int c = 1; c < A->a.ncells; c++)
+        add(A->a.src[c] == -2 ? s->spec.pcol : s->spec.vals[A->a.src[c]]);
+    return P.n_cols + A->a.ncells - 1 <= TDD_PRED_MAX_COLS ? n : -1;
+}
+
+static int64_t slice_groups(int ncells) {
+    const int64_t sg = SRV_SLICE_B / (8 * ncells);
+    return sg > 0 ? sg : 1;
+}
+
+/* One key's batch of direct requests over chunk rows [r0, r1): the rows
+ * bucket-sorted by group slice into records of `cols`, then folded per
+ * (slice, request group). */
+static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc_t*>& batch,
+                      const std::vector<tdd_col_t>& cols, int ncmax, int64_t r0, int64_t r1) {
+    const tdd_srv_spec_t& S = s->spec;
+    const int64_t G = s->G[key], sg = slice_groups(ncmax), nsl = ceil_div(G, sg);
+    const int64_t rows = r1 - r0, ntiles = ceil_div(rows, SRV_TILE);
+    SCRATCH(hist, (size_t)(nsl * ntiles) * 4);
+    SCRATCH(off, (size_t)(nsl * ntiles + 1) * 8);
+    SCRATCH(rg, (size_t)rows * 4);
+    SrvBucket B;
+    memset(&B, 0, sizeof(B));
+    B.fk = S.fk;
+    B.map = s->map[key];
+    B.r0 = r0;
+    B.r1 = r1;
+    B.kmin = s->kmin;
+    B.range = s->range;
+    B.ntiles = ntiles;
+    B.sg = (int32_t)sg;
+    B.S = (int32_t)nsl;
+    B.nc = (int32_t)cols.size();
+    B.hist = hist.as<uint32_t>();
+    B.off = off.as<int64_t>();
+    B.rg = rg.as<uint32_t>();
+    Scratch rc[SRV_REC_C] = {Scratch(8), Scratch(8), Scratch(8), Scratch(8),
+                             Scratch(8), Scratch(8), Scratch(8), Scratch(8)};
+    for (int c = 0; c < B.nc; c++) {
+        B.src[c] = cols[c];
+        if (!rc[c].reset((size_t)rows * cols[c].esz)) return TD_ERR_OOM;
+        B.rc[c] = rc[c].p;
+    }
+    LAUNCH("serve_bucket", k_srv_bhist, (int)ntiles, 256, (size_t)nsl * 4, B);
+    int rcode = tdd::scan_u32_i64(hist.as<uint32_t>(), nsl * ntiles, off.as<int64_t>());
+    if (rcode) return rcode;
+    LAUNCH("serve_bucket", k_srv_bscatter, (int)ntiles, 256, (size_t)nsl * 8, B);
+    /* the requests' programs over the record columns; sums read record columns */
+    std::vector<SrvDirReq> reqs;
+    std::vector<SrvDirGroup> groups;
+    auto rec_col = [&](const tdd_col_t& c) {
+        for (int i = 0; i < B.nc; i++)
+            if (cols[i].data == c.data && cols[i].type == c.type) {
+                tdd_col_t r = c;
+                r.data = B.rc[i];
+                return r;
+            }
+        return c;                                    /* not reached: req_cols lists them */
+    };
+    for (const tdd_srv_acc_t* A : batch) {
+        SrvDirReq R;
+        memset(&R, 0, sizeof(R));
+        R.P = A->pred;
+        for (int i = 0; i < R.P.n_cols; i++)
+            if (R.P.cols[i].data) R.P.cols[i] = rec_col(R.P.cols[i]);
+        int nextc = R.P.n_cols;
+        for (int c = 1; c < A->a.ncells; c++) {
+            R.P.cols[nextc] = rec_col(A->a.src[c] == -2 ? S.pcol : S.vals[A->a.src[c]]);
+            R.src[c] = (int8_t)nextc++;
+        }
+        R.tab = A->a.tab;
+        R.G = G;
+        R.key = key;
+        R.ncells = A->a.ncells;
+        memcpy(R.kind, A->a.kind, sizeof(R.kind));
+        const int32_t words = (int32_t)(sg * R.ncells);
+        SrvDirGroup* gp = groups.empty() ? nullptr : &groups.back();
+        if (!gp || gp->n == SRV_DIR_GRP || (int64_t)(gp->words + words) * 8 > SRV_DIR_LDS) {
+            groups.push_back({(int32_t)reqs.size(), 0, 0, 0});
+            gp = &groups.back();
+        }
+        R.lds = gp->words;
+        gp->words += words;
+        gp->n++;
+        reqs.push_back(R);
+    }
+    int32_t maxw = 0;
+    for (const auto& g : groups) if (g.words > maxw) maxw = g.words;
+    SCRATCH(dreq, sizeof(SrvDirReq) * reqs.size());
+    SCRATCH(dgrp, sizeof(SrvDirGroup) * groups.size());
+    HC(hipMemcpyAsync(dreq.p, reqs.data(), sizeof(SrvDirReq) * reqs.size(),
+                      hipMemcpyHostToDevice, tdd::g_stream));
+    HC(hipMemcpyAsync(dgrp.p, groups.data(), sizeof(SrvDirGroup) * groups.size(),
+                      hipMemcpyHostToDevice, tdd::g_stream));
+    SrvFoldArgs F;
+    memset(&F, 0, sizeof(F));
+    F.rg = rg.as<uint32_t>();
+    F.off = off.as<int64_t>();
+    F.ntiles = ntiles;
+    F.G = G;
+    F.sg = (int32_t)sg;
+    F.ngroups = (int32_t)groups.size();
+    F.reqs = dreq.as<SrvDirReq>();
+    F.groups = dgrp.as<SrvDirGroup>();
+    /* parts per slice: about 8 workgroups per CU in all, a part at least
+     * 4,096 records */
+    const int64_t units = nsl * F.ngroups, want = (int64_t)tdd::g_num_cu * 8;
+    int64_t parts = units < want ? ceil_div(want, units) : 1;
+    const int64_t most = ceil_div(ceil_div(rows, nsl), 4096);
+    if (parts > most) parts = most > 0 ? most : 1;
+    F.parts = (int32_t)parts;
+    LAUNCH("serve_fold", k_srv_bfold, (int)(units * parts), 256, (size_t)maxw * 8, F);
+    HC(hipStreamSynchronize(tdd::g_stream));         /* host argument arrays go out of scope */
+    return TD_OK;
+}
+
+/* Direct requests that no slice plan covers (more groups than SRV_MAX_S
+ * slices, or more columns than a record holds): every row of the chunk
+ * folded into their tables with global atomics. */
+static int srv_global(tdd_srv_t* s, const std::vector<const tdd_srv_acc_t*>& list, int64_t r0,
+                      int64_t r1) {
+    const tdd_srv_spec_t& S = s->spec;
+    std::vector<SrvDirReq> reqs;
+    std::vector<SrvDirGroup> groups;
+    int vused = 0, pused = 0;
+    for (const tdd_srv_acc_t* A : list) {
+        SrvDirReq R;
+        memset(&R, 0, sizeof(R));
+        R.P = A->pred;
+        R.tab = A->a.tab;
+        R.G = A->a.G;
+        R.key = A->a.key;
+        R.ncells = A->a.ncells;
+        R.lds = -1;
+        memcpy(R.kind, A->a.kind, sizeof(R.kind));
+        memcpy(R.src, A->a.src, sizeof(R.src));
+        for (int c = 1; c < R.ncells; c++) {
+            if (R.src[c] == -2) pused = 1;
+            else if (R.src[c] >= 0) vused |= 1 << R.src[c];
+        }
+        groups.push_back({(int32_t)reqs.size(), 1, 0, 0});
+        reqs.push_back(R);
+    }
+    SrvDirArgs D;
+    memset(&D, 0, sizeof(D));
+    D.fk = S.fk;
+    if (pused) D.pcol = S.pcol;
+    for (int j = 0; j < S.n_vals; j++) if ((vused >> j) & 1) D.vals[j] = S.vals[j];
+    for (int k = 0; k < S.n_keys; k++) D.map[k] = s->map[k];
+    D.r0 = r0;
+    D.r1 = r1;
+    D.kmin = s->kmin;
+    D.range = s->range;
+    D.ngroups = (int32_t)groups.size();
+    const int64_t rows = r1 - r0;
+    int64_t nb = (int64_t)tdd::g_num_cu * 4 / D.ngroups;
+    if (nb < 1) nb = 1;
+    if (nb > ceil_div(rows, 4096)) nb = ceil_div(rows, 4096);
+    D.span = ceil_div(ceil_div(rows, nb), 1024) * 1024;
+    nb = ceil_div(rows, D.span);
+    if (nb * D.ngroups > ((int64_t)1 << 31) - 1) return TD_ERR_NYI;
+    SCRATCH(dreq, sizeof(SrvDirReq) * reqs.size());
+    SCRATCH(dgrp, sizeof(SrvDirGroup) * groups.size());
+    HC(hipMemcpyAsync(dreq.p, reqs.data(), sizeof(SrvDirReq) * reqs.size(),
+                      hipMemcpyHostToDevice, tdd::g_stream));
+    HC(hipMemcpyAsync(dgrp.p, groups.data(), sizeof(SrvDirGroup) * groups.size(),
+                      hipMemcpyHostToDevice, tdd::g_stream));
+    D.reqs = dreq.as<SrvDirReq>();
+    D.groups = dgrp.as<SrvDirGroup>();
+    LAUNCH("serve_direct", k_srv_direct, (int)(nb * D.ngroups), 256, 0, D);
+    HC(hipStreamSynchronize(tdd::g_stream));         /* host argument arrays go out of scope */
+    return TD_OK;
+}
+
+/* The step's direct requests over chunk rows [r0, r1): per group key,
+ * batches whose columns fit one record go through the bucket sort and the
+ * slice folds; the rest through global atomics. */
+static int srv_direct(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n, int64_t r0,
+                      int64_t r1) {
+    if (r1 <= r0) return TD_OK;
+    std::vector<const tdd_srv_acc_t*> glob;
+    for (int k = 0; k < s->spec.n_keys; k++) {
+        std::vector<const tdd_srv_acc_t*> batch;
+        std::vector<tdd_col_t> cols;
+        int ncmax = 1;
+        auto flush = [&]() -> int {
+            int rc = batch.empty() ? TD_OK : srv_bucket(s, k, batch, cols, ncmax, r0, r1);
+            batch.clear();
+            cols.clear();
+            ncmax = 1;
+            return rc;
+        };
+        for (int i = 0; i < n; i++) {
+            const tdd_srv_acc_t* A = active[i];
+            if (!A->direct || A->a.key != k) continue;
+            tdd_col_t need[2 * TDD_PRED_MAX_COLS + GP_MAX_CELLS];
+            const int nn = req_cols(s, A, need);
+            if (nn < 0 || nn > SRV_REC_C ||
+                ceil_div(s->G[k], slice_groups(A->a.ncells)) > SRV_MAX_S) {
+                glob.push_back(A);
+                continue;
+            }
+            std::vector<tdd_col_t> merged = cols;
+            for (int j = 0; j < nn; j++) {
+                bool have = false;
+                for (const auto& c : merged) have = have || (c.data == need[j].data &&
+                                                            c.type == need[j].type);
+                if (!have) merged.push_back(need[j]);
+            }
+            const int nc = A->a.ncells > ncmax ? A->a.ncells : ncmax;
+            if (merged.size() > SRV_REC_C || ceil_div(s->G[k], slice_groups(nc)) > SRV_MAX_S) {
+                int rc = flush();                   /* this batch is full: start another */
+                if (rc) return rc;
+                merged.assign(need, need + nn);
+            }
+            cols = merged;
+            batch.push_back(A);
+            if (A->a.ncells > ncmax) ncmax = A->a.ncells;
+        }
+        int rc = flush();
+        if (rc) return rc;
+    }
+    return glob.empty() ? TD_OK : srv_global(s, glob, r0, r1);
+}
+
 extern "C" int tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n) {
     const tdd_srv_spec_t& S = s->spec;
     const int nv = S.n_vals;
     int on[TDD_SRV_MAX_KEYS] = {0}, vm[TDD_SRV_MAX_KEYS] = {0};
     int pfx[TDD_SRV_MAX_KEYS] = {0}, sfx[TDD_SRV_MAX_KEYS] = {0};
-    int live = 0;
+    int live = 0, direct = 0;
     for (int i = 0; i < n; i++) {
         const SrvAccArgs& a = active[i]->a;
+        direct |= active[i]->direct;
         if (a.mode == M_NONE) continue;
         live++;
         on[a.key] = 1;
@@ -480,6 +1025,11 @@ extern "C" int tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n) {
     }
     const int64_t c = s->step % s->nchunks;
     s->step++;
+    if (direct && S.nf > 0) {
+        const int64_t r0 = c * s->chunk, r1 = r0 + s->chunk < S.nf ? r0 + s->chunk : S.nf;
+        int rc = srv_direct(s, active, n, r0, r1);
+        if (rc) return rc;
+    }
     if (!live || S.nf == 0) return TD_OK;
     SrvScanArgs A;
     memset(&A, 0, sizeof(A));

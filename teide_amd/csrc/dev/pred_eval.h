@@ -94,25 +94,34 @@ __device__ __forceinline__ void ld4_f64(const tdd_col_t& c, int64_t r0, int64_t 
 /* pred_eval over the four rows r0 + 64k at once, one instruction at a time
  * for all four, so each instruction's loads are in flight together; bit k
  * of the result is row r0 + 64k's value (rows past nrows: unspecified). */
-__device__ __forceinline__ uint32_t pred_eval4(const tdd_pred_t& P, int64_t r0, int64_t nrows) {
+template <class Pred>
+__device__ __forceinline__ uint32_t pred_eval4(Pred& P, int64_t r0, int64_t nrows) {
     uint32_t st[4] = {0, 0, 0, 0};
     for (int i = 0; i < P.n_ins; i++) {
-        const tdd_pred_ins_t& in = P.ins[i];
+        const tdd_pred_ins_t in = P.ins[i];
         uint32_t v[4];
         switch (in.kind) {
         case TDD_P_CMP_CONST:
         case TDD_P_CMP_COL:
             if (in.as_f64) {
                 double a[4], b[4];
-                ld4_f64(P.cols[in.a], r0, nrows, a);
-                if (in.kind == TDD_P_CMP_COL) ld4_f64(P.cols[in.b], r0, nrows, b);
+                const tdd_col_t ca = P.cols[in.a];
+                ld4_f64(ca, r0, nrows, a);
+                if (in.kind == TDD_P_CMP_COL) {
+                    const tdd_col_t cb = P.cols[in.b];
+                    ld4_f64(cb, r0, nrows, b);
+                }
                 else for (int k = 0; k < 4; k++) b[k] = in.cf;
 #pragma unroll
                 for (int k = 0; k < 4; k++) v[k] = cmp_f(in.cmp, a[k], b[k]);
             } else {
                 int64_t a[4], b[4];
-                ld4_i64(P.cols[in.a], r0, nrows, a);
-                if (in.kind == TDD_P_CMP_COL) ld4_i64(P.cols[in.b], r0, nrows, b);
+                const tdd_col_t ca = P.cols[in.a];
+                ld4_i64(ca, r0, nrows, a);
+                if (in.kind == TDD_P_CMP_COL) {
+                    const tdd_col_t cb = P.cols[in.b];
+                    ld4_i64(cb, r0, nrows, b);
+                }
                 else for (int k = 0; k < 4; k++) b[k] = in.ci;
 #pragma unroll
                 for (int k = 0; k < 4; k++) v[k] = cmp_i(in.cmp, a[k], b[k]);
@@ -138,7 +147,8 @@ __device__ __forceinline__ uint32_t pred_eval4(const tdd_pred_t& P, int64_t r0, 
             break;
         default: { /* TDD_P_BOOLCOL */
             int64_t a[4];
-            ld4_i64(P.cols[in.a], r0, nrows, a);
+            const tdd_col_t ca = P.cols[in.a];
+            ld4_i64(ca, r0, nrows, a);
 #pragma unroll
             for (int k = 0; k < 4; k++) st[k] = (st[k] << 1) | (a[k] != 0);
         }

@@ -2377,11 +2377,13 @@ static int srv_match(const td_dev_server_t* s, srv_item_t* r) {
         return -1;
     if (J->lpred) {
         const tdd_pred_t* P = J->lpred;
-        if (P->n_ins != 1 || P->ins[0].kind != TDD_P_CMP_CONST || P->ins[0].as_f64 ||
-            P->cols[P->ins[0].a].data != S->pcol.data)
-            return -1;
-        q->cmp = P->ins[0].cmp;
-        q->t = P->ins[0].ci;
+        if (P->n_ins == 1 && P->ins[0].kind == TDD_P_CMP_CONST && !P->ins[0].as_f64 &&
+            P->cols[P->ins[0].a].data == S->pcol.data) {
+            q->cmp = P->ins[0].cmp;             /* a range of the cube's axis */
+            q->t = P->ins[0].ci;
+        } else {
+            q->pred = P;                        /* its own predicate on the shared chunks */
+        }
     } else {
         q->cmp = OP_GE;
         q->t = INT64_MIN;

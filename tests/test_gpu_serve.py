@@ -31,9 +31,11 @@ def tables(eng):
     pk = rng.permutation(nd) + 5
     cat = rng.integers(0, 50, nd).astype(np.int32)
     k64 = rng.integers(0, 3000, nd)
-    HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64)
-    tf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
-    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64)])
+    flag = rng.integers(0, 4, nf)
+    wide = rng.integers(0, 2_000_000, nd)              # 2 M-group key range
+    HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64, flag=flag, wide=wide)
+    tf = eng.table([("fk", fk), ("qty", qty), ("price", price), ("flag", flag)])
+    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide)])
     rf, rd = eng.put(tf), eng.put(td)
     eng.release(tf)
     eng.release(td)
@@ -227,3 +229,114 @@ def test_serve_slots_and_budget(eng, tables):
     sub[0][1].free()
     eng.serve_close(h)
     eng.release(ref)
+
+
+# filters the cube cannot answer: f64 and wide-domain columns, several
+# comparisons -- each request evaluates its own predicate on the shared chunks
+WIDE = [
+    ("price > t", lambda g, t: g.gt(g.scan("price"), g.const_f64(t)),
+     lambda h, t: h["price"] > t, [7.5, 20.0, 55.25, 1e9, -1.0]),
+    ("price <= t", lambda g, t: g.le(g.scan("price"), g.const_f64(t)),
+     lambda h, t: h["price"] <= t, [3.0, 30.0]),
+    ("fk < t", lambda g, t: g.lt(g.scan("fk"), g.const_i64(int(t))),
+     lambda h, t: h["fk"] < int(t), [5000, 30000]),
+    ("qty > 20 and price < t",
+     lambda g, t: g.and_(g.gt(g.scan("qty"), g.const_i64(20)), g.lt(g.scan("price"), g.const_f64(t))),
+     lambda h, t: (h["qty"] > 20) & (h["price"] < t), [15.0, 40.0]),
+    ("qty < 10 or price > t",
+     lambda g, t: g.or_(g.lt(g.scan("qty"), g.const_i64(10)), g.gt(g.scan("price"), g.const_f64(t))),
+     lambda h, t: (h["qty"] < 10) | (h["price"] > t), [60.0]),
+]
+
+
+def wide_dag(eng, rf, rd, r):
+    g = eng.graph(rf)
+    src = g.filter(g.const_table(rf), WIDE[r["w"]][1](g, r["t"]))
+    g.join(src, [g.scan("fk")], g.const_table(rd), [g.scan("pk")], 0)
+    root = g.group([g.scan(r["key"])], [(o, g.scan(c)) for o, c in r["aggs"]])
+    return g, root
+
+
+def check_wide_oracle(eng, got, r, what):
+    h = HOST
+    m = WIDE[r["w"]][2](h, r["t"])
+    li, ri = orc.join([h["fk"][m]], [h["pk"]], 0)
+    src = {"price": orc.gather(h["price"][m], li), "qty": orc.gather(h["qty"][m], li)}
+    ops = [o for o, _ in r["aggs"]]
+    ref = orc.group([orc.gather(h[r["key"]], ri)], ops, [src[c] for _, c in r["aggs"]])
+    a = eng.read_table(got)
+    assert [c[1] for c in a[1:]] == ref["out_types"], what
+    f64 = [t == 7 and o in ("SUM", "AVG") for t, o in zip(ref["out_types"], ops)]
+    compare_groups([a[0][3].astype(np.int64)], [c[3] for c in a[1:]], ref["keys"], ref["vals"],
+                   f64, what, exact=ref["comp"], sum_abs=ref["sum_abs"])
+
+
+def test_serve_wide_filters_share_the_scan(eng, tables):
+    """Requests filtering on price (f64), on fk (a wide integer domain) or
+    on several comparisons run in the shared scan beside cube requests:
+    none runs alone, each matches the oracle and its own td_execute."""
+    rf, rd = tables
+    rng = np.random.default_rng(33)
+    wide = [dict(w=w, t=t, key=k, aggs=AGG_SETS[(w + i) % len(AGG_SETS)])
+            for w, (_, _, _, ts) in enumerate(WIDE) for i, t in enumerate(ts)
+            for k in ("cat", "k64")]
+    cube = random_requests(rng, 12)
+    h = open_server(eng, rf, rd)
+    dags, ids, results = [], {}, {}
+    order = wide[:len(wide) // 2] + cube[:6] + wide[len(wide) // 2:] + cube[6:]
+    for i, r in enumerate(order):
+        g, root = wide_dag(eng, rf, rd, r) if "w" in r else \
+            request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"])
+        ids[eng.serve_submit(h, g, root)] = r
+        dags.append(g)
+        if i % 5 == 4:                                     # arrivals mid-scan
+            eng.serve_step(h)
+            for rid, res in eng.serve_poll(h):
+                results[rid] = res
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 0 and info["shared"] == len(order), info
+    assert info["held"] == info["base"]
+    assert sorted(results) == sorted(ids)
+    for rid, r in ids.items():
+        what = f"request {rid} {r}"
+        if "w" in r:
+            g, root = wide_dag(eng, rf, rd, r)
+            ref = g.execute(root)
+            g.free()
+            check_same(eng, results[rid], ref, what)
+            check_wide_oracle(eng, results[rid], r, what + " vs oracle")
+        else:
+            ref = alone(eng, rf, rd, r)
+            check_same(eng, results[rid], ref, what)
+            check_oracle(eng, results[rid], r, what + " vs oracle")
+        eng.release(results[rid])
+        eng.release(ref)
+    eng.serve_close(h)
+    for g in dags:
+        g.free()
+
+
+def test_serve_wide_filters_many_groups(eng, tables):
+    """price > t requests grouped by a key of 2 M groups: more table slices
+    than the bucket plan takes, so the shared scan folds them with global
+    atomics; still shared, still the oracle's groups."""
+    rf, rd = tables
+    h = eng.serve_open(rf, rd, "fk", "pk", "flag", ["price", "qty"], ["wide"],
+                       chunk_rows=65536)
+    reqs = [dict(w=0, t=t, key="wide", aggs=AGG_SETS[i % len(AGG_SETS)])
+            for i, t in enumerate([10.0, 25.0, 80.0])]
+    dags, ids, results = [], {}, {}
+    for r in reqs:
+        g, root = wide_dag(eng, rf, rd, r)
+        ids[eng.serve_submit(h, g, root)] = r
+        dags.append(g)
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 0 and info["shared"] == len(reqs), info
+    for rid, r in ids.items():
+        check_wide_oracle(eng, results[rid], r, f"request {rid} {r} vs oracle")
+        eng.release(results[rid])
+    eng.serve_close(h)
+    for g in dags:
+        g.free()
