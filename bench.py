@@ -28,6 +28,7 @@ one of those workloads (or the high-cardinality group-by) alone.
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import time
@@ -117,15 +118,15 @@ def cpu_baseline_port(nf, nd, ncat):
             "sample": f"fact {nf:,} x dim {nd:,} rows, one run of the scalar oracle"}
 
 
-def load_traffic(name):
-    """HBM bytes per launch of the kernel measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when recorded."""
+def load_traffic(workload, name):
+    """HBM bytes per launch of the workload's kernel measured with rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when recorded."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         rec = json.load(fh)
-    ent = rec.get(name)
+    ent = rec.get(f"{workload}/{name}")
     return ent.get("bytes_per_launch") if ent else None
 
 
@@ -317,6 +318,10 @@ class HighCard:
         self.rows = 2 * a.hc_rows                       # two group-bys per step
         out = self._group("key")
         self.g_key = self.e.L.td_table_nrows(out)
+        # rows the key group-by's partition passes carry: the rows of its
+        # hot keys are folded before them
+        m = re.search(r"part_rows=(\d+)", e.last_plan())
+        self.key_part_rows = int(m.group(1)) if m and int(m.group(1)) > 0 else a.hc_rows
         self.e.release(out)
         self.query_bytes = (a.hc_rows * 24 + self.g_key * 64) + (a.hc_rows * 20 + a.hc_syms * 60)
         self.config = {"workload": f"group {a.hc_rows:,} rows by key (zipf 1.1 over "
@@ -336,15 +341,18 @@ class HighCard:
         self.e.release(self._group("sym"))
 
     def kernel_bytes(self, name):
-        n = self.a.hc_rows
+        n, c = self.a.hc_rows, self.key_part_rows
         if name == "group_pg_pass":
             # per launch on average over the step's four scatter passes: the
-            # key group-by's two passes read and write 24-byte records (the
-            # first reads the rows: key, v_i64, v_f64), the symbol
-            # group-by's two read and write 20-byte ones (sym + the values)
-            return n * (48 + 48 + 40 + 40) // 4
+            # key group-by's two passes read and write 24-byte records of the
+            # rows its hot-key fold left (key, v_i64, v_f64), the symbol
+            # group-by's first reads its rows (sym u32, v_i64, v_f64) and
+            # writes 20-byte records, its second reads and writes records
+            return (c * (48 + 48) + n * (40 + 40)) // 4
         if name == "group_pg_hagg":
-            return n * 24                      # one read of the key records
+            return c * 24                      # one read of the key records
+        if name == "group_pg_hotfold":
+            return n * 24 + c * 24             # rows read, the cold rows' records written
         return None
 
     def cpu_baseline(self):
@@ -944,7 +952,7 @@ def measure(e, wl, steps, warmup, prof=True):
             ach = nbytes / (avg_ms / 1e3) / 1e9
             roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": load_traffic(dom), "kernel": dom,
+                        "traffic": load_traffic(getattr(wl, "name", ""), dom), "kernel": dom,
                         "algorithmic_bytes": nbytes, "avg_ms": round(avg_ms, 5)}
     return {"ms": ms, "plan": plan, "kernels": kernels, "roofline": roofline}
 
@@ -969,17 +977,19 @@ def two_call_summary(e, wl, steps=5, warmup=2):
         e.release(j)
 
     class TwoCall:
+        name = JoinGroupTwoCall.name
+
         def step(self):
             step()
 
         def kernel_bytes(self, name):
-            return None
+            return JoinGroupTwoCall.kernel_bytes(wl, name)
 
     m = measure(e, TwoCall(), steps, warmup)
     return {"workload": "the same query as the reference runs it: join graph, then group "
                         "graph over the joined table (both on the GPU, resident)",
             "ms_per_query": round(m["ms"], 4), "rows_per_s": wl.rows / (m["ms"] / 1e3),
-            "fused_vs_two_call": None, "kernels": m["kernels"]}
+            "fused_vs_two_call": None, "roofline": m["roofline"], "kernels": m["kernels"]}
 
 
 def hc_summary(e, a, steps=5, warmup=2, cpu=True):

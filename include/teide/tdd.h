@@ -163,6 +163,8 @@ typedef struct {
     int8_t   out_types[TDD_MAX_AGGS];
     int32_t  path;                 /* 1 LDS direct array, 2 global direct array, 3 hash, 4 partitioned direct array, 5 partitioned hash, 6 partitioned hash with hot keys folded first; 20 + p: key tuples wider than 64 bits (an F64 among several keys, or integer ranges multiplying past 2^63) numbered by a hash of their raw key words, then grouped by that number on path p */
     int32_t  tries;                /* hash paths: table sizes tried (1 when the first fitted) */
+    int64_t  part_rows;            /* partitioned hash: rows the partition passes carried (the
+                                      rows of hot keys are folded before them) */
 } tdd_group_result_t;
 
 /* Masked min/max of an integer key column (minmax_scan_fn, exec.c:4544-4604);

@@ -732,6 +732,7 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
              int64_t lcap_out, int* overflow);
 void pht_free(tdd_pht* h);
 int64_t pht_global_keys(const tdd_pht* h);
+int64_t pht_rows(const tdd_pht* h);
 
 /* Hash group-by.  Many rows with many distinct keys are first split by the
  * key's hash until each part fits an LDS table (pht_*); otherwise rows go
@@ -857,6 +858,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
                    (const int64_t*)pos.as<int64_t>(), n_list, *out);
         out->path = ph ? (hot ? 6 : 5) : 3;
         out->tries = attempt + 1;
+        if (ph) out->part_rows = pht_rows(ph) >= 0 ? pht_rows(ph) : n_rows_sel;
         return TD_OK;
     }
     return TD_ERR_OOM;

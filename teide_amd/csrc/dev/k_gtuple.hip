@@ -101,6 +101,7 @@ int group_tuples(const tdd_group_spec_t* spec, int64_t nsel, tdd_group_result_t*
     out->n_groups = ng;
     out->path = 20 + R.path;
     out->tries = R.tries;
+    out->part_rows = R.part_rows;
     HC(hipStreamSynchronize(tdd::g_stream));
     return TD_OK;
 }

@@ -893,6 +893,7 @@ struct tdd_pht {
     std::vector<PGJob> jobs;
     GroupPlan         P;
     int               nv, lcap;
+    int64_t           rows;            /* rows the partition passes carry */
     int8_t            cw[GP_MAX_CELLS];
     uint8_t           vf[PG_MAXV];
 };
@@ -905,7 +906,7 @@ struct tdd_pht {
  * when the LDS blocks do not fit. */
 static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t* vf,
                         const unsigned long long* hot, int nhot, uint64_t* hot_cells,
-                        const PGBuf& A, std::vector<PGJob>* first) {
+                        const PGBuf& A, std::vector<PGJob>* first, int64_t* cold) {
     const GroupPlan& P = S.P;
     const size_t lds = (size_t)PH_SET * 8 + (size_t)PH_SET * 2 + (size_t)nhot * P.ncells * 8;
     if (lds > 64 * 1024) return TD_ERR_NYI;
@@ -945,6 +946,8 @@ static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t
     HC(hipMemcpyAsync(c.data(), cnt.p, c.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
     first->clear();
+    *cold = 0;
+    for (int b = 0; b < grid; b++) *cold += c[(size_t)b];
     for (int b = 0; b < grid; b++)
         for (int64_t r = 0; r < c[(size_t)b]; r += PG_JOB) {
             PGJob J;
@@ -987,8 +990,10 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
     pg_split_bits(bits, pbits, &npass);
     std::vector<PGJob> first;
     bool from_records = false;
+    h->rows = -1;                                   /* every selected row */
     if (nhot > 0) {
-        int frc = pht_hot_fold(S, h->nv, h->cw, h->vf, hot, nhot, hot_cells, A, &first);
+        int frc = pht_hot_fold(S, h->nv, h->cw, h->vf, hot, nhot, hot_cells, A, &first,
+                               &h->rows);
         if (frc && frc != TD_ERR_NYI) { delete h; return frc; }
         from_records = frc == 0;
     }
@@ -1050,6 +1055,8 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
 /* Keys the global table takes: every key of a part split over several
  * jobs (at most a table each), plus an allowance for keys that find no
  * slot within the probe limit of their part's table. */
+int64_t pht_rows(const tdd_pht* h) { return h->rows; }
+
 int64_t pht_global_keys(const tdd_pht* h) {
     int64_t n = 4096 + 16 * (int64_t)h->jobs.size();
     for (size_t j = 0; j < h->jobs.size(); j++)

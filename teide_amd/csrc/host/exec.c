@@ -1214,7 +1214,8 @@ static td_t* group_over_ev(xctx_t* x, td_op_ext_t* ext, td_t* tbl, int use_sel, 
         for (int a = 0; a < ext->n_aggs; a++) td__arena_free(R.vals[a]);
         return TD_ERR_PTR(rc);
     }
-    td__set_plan("group path=%d groups=%lld tries=%d", R.path, (long long)R.n_groups, R.tries);
+    td__set_plan("group path=%d groups=%lld tries=%d part_rows=%lld", R.path,
+                 (long long)R.n_groups, R.tries, (long long)R.part_rows);
     return group_table(g, ext, &R, key_src, agg_src);
 }
 

@@ -315,6 +315,39 @@ def test_group_brute_force():
         assert r["vals"][7][gi] == len(rows)
 
 
+def test_group_wide_tuples_brute_force():
+    """Key tuples no 64-bit word holds (an f64 beside an i64; two scattered
+    i64): groups are the distinct raw 8-byte key words, as the reference's
+    memcmp over its key words makes them (exec.c:4131-4165): -0.0 and 0.0
+    apart, NaNs of one bit pattern together."""
+    rng = np.random.default_rng(13)
+    n = 30000
+    f = np.round(rng.normal(size=n), 1)
+    f[::101] = -0.0
+    f[::103] = 0.0
+    f[::211] = np.nan
+    k = rng.integers(-3, 3, n)
+    big = rng.integers(-2 ** 62, 2 ** 62, 40)[rng.integers(0, 40, n)]
+    v = rng.integers(-10 ** 9, 10 ** 9, n)
+    for keys in ([f, k], [big, k * 2 ** 61], [k.astype(np.int16), f, big]):
+        r = orc.group(keys, ["SUM", "COUNT"], [v, None])
+        words = np.stack([x.view(np.int64) if x.dtype == np.float64 else x.astype(np.int64)
+                          for x in keys], 1)
+        uk, inv = np.unique(words, axis=0, return_inverse=True)
+        inv = inv.ravel()
+        got = sorted(zip(map(tuple, np.stack(r["keys"], 1).tolist()), r["vals"][0].tolist(),
+                         r["vals"][1].tolist()))
+        sums = np.zeros(len(uk), np.int64)
+        np.add.at(sums, inv, v)
+        want = sorted(zip(map(tuple, uk.tolist()), sums.tolist(),
+                          np.bincount(inv, minlength=len(uk)).tolist()))
+        assert got == want
+    # -0.0 and 0.0 are two groups; the NaNs are one
+    r = orc.group([np.array([0.0, -0.0, np.nan, np.nan, 0.0]), np.zeros(5, np.int64)],
+                  ["COUNT"], [None])
+    assert sorted(r["vals"][0].tolist()) == [1, 2, 2]
+
+
 # ----------------------------------------------------------------- joins --
 
 def join_case(argv):
