@@ -141,6 +141,14 @@ void     td_dev_serve_close(td_dev_server_t* s);
 void     td_dev_cancel_after(int64_t polls);
 int64_t  td_dev_cancel_polls(void);
 
+/* Out of device memory: a join or group-by whose partitioned plan cannot
+ * get its buffers falls back to a plan that needs less (the pair-producing
+ * join; the global hash table or direct array), as the reference falls
+ * back to its chained join and sequential group table (exec.c:7677-7683,
+ * 8219-8231, 6486-6498); with a forced path it returns TD_ERR_OOM.  Test
+ * hook: the next `count` arena requests of at least `min_bytes` fail. */
+void     td_dev_fail_allocs(uint64_t min_bytes, int64_t count);
+
 /* Physical-plan hints for tests and measurement: force the join path
  * (0 auto, 1 direct map, 2 radix, 3 partitioned fused join-group,
  * 4 XCD-partitioned fused join-group) / group path (0 auto, 1 direct

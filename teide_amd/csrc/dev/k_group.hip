@@ -770,12 +770,14 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
          * carry the cold rows only */
         if (force != 4) {
             int hrc = hot_split(P, M.bits, M.nrows, force == 5, &hot);
-            if (hrc) return hrc;
+            if (hrc && !(hrc == TD_ERR_OOM && force == 0)) return hrc;
         }
         int prc = pht_partition(P, M.bits, M.nrows, est, force == 4 ? 4096 : 96 * 1024,
                                 force >= 4, hot ? hot_keys(hot) : nullptr,
                                 hot ? hot_count(hot) : 0, hot ? hot_cells(hot) : nullptr, &ph);
-        if (prc && prc != TD_ERR_NYI) return prc;
+        /* no memory for the partitions: the global table alone needs less
+         * (the reference's sequential fallback, exec.c:6486-6498) */
+        if (prc && prc != TD_ERR_NYI && !(prc == TD_ERR_OOM && force == 0)) return prc;
         if (!ph && hot) { hot_free(hot); hot = nullptr; }   /* every row goes the other way */
     }
     /* partitioned: most groups leave through the list (cap entries); the
@@ -932,7 +934,8 @@ extern "C" int tdd_group(const tdd_group_spec_t* spec, tdd_group_result_t* out) 
                         slots * P.ncells * 8 > LDS_TABLE_BYTES))) {
             rc = run_pda(P, M.bits, M.nrows, (int64_t)slots,
                          spec->force_path == 3 ? 64 : 96 * 1024, out);
-            if (rc != TD_ERR_NYI) {
+            const bool degrade = rc == TD_ERR_OOM && spec->force_path == 0;   /* global DA / hash */
+            if (rc != TD_ERR_NYI && !degrade) {
                 if (rc) return rc;
                 HC(hipStreamSynchronize(tdd::g_stream));
                 return TD_OK;

@@ -1219,6 +1219,7 @@ static int jg_materialised(const tdd_join_group_spec_t* S, const tdd_join_spec_t
     }
     tdd_join_mat_result_t R;
     int rc = tdd_join_mat(&M, &R);
+    if (rc == TD_ERR_OOM && J->force_path == 0) return TD_ERR_NYI;   /* pairs need less */
     if (rc) return rc;
     struct Free {
         tdd_join_mat_result_t* r; int nl, nr;

@@ -41,6 +41,15 @@ static int      g_nchunks;
 static direct_t g_direct[A_MAX_DIRECT];
 static size_t   g_in_use, g_peak, g_reserved, g_budget;
 static int      g_dev_state;                   /* 0 unknown, 1 ready, -1 unavailable */
+static size_t   g_fail_min;                    /* test hook: requests of at least this size */
+static int64_t  g_fail_left;                   /* ... fail, this many more times */
+
+void td_dev_fail_allocs(uint64_t min_bytes, int64_t count) {
+    pthread_mutex_lock(&g_amu);
+    g_fail_min = (size_t)min_bytes;
+    g_fail_left = count > 0 ? count : 0;
+    pthread_mutex_unlock(&g_amu);
+}
 
 static uint64_t* fb(chunk_t* c, int order) { return c->freebits[order - A_MIN_ORDER]; }
 
@@ -139,6 +148,11 @@ void* td__arena_alloc(size_t bytes) {
     if (bytes == 0) bytes = 1;
     pthread_mutex_lock(&g_amu);
     void* r = NULL;
+    if (g_fail_left > 0 && bytes >= g_fail_min) {   /* injected out-of-memory */
+        g_fail_left--;
+        pthread_mutex_unlock(&g_amu);
+        return NULL;
+    }
     if (bytes > A_CHUNK / 2) {
         size_t want = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
         int best = -1;

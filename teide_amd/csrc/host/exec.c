@@ -1264,7 +1264,8 @@ static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int
  * columns (left columns, then right non-key columns, 16 each at most,
  * exec.c:8607-8728) come back written.  NULL when that join does not take
  * the query (repeated build keys, oversized partitions, more columns than
- * it writes): the caller joins through row-index pairs instead. */
+ * it writes, no memory for its partitions): the caller joins through
+ * row-index pairs instead. */
 static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* right,
                                const tdd_join_spec_t* J) {
     td_graph_t* g = x->g;
@@ -1294,7 +1295,9 @@ static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* rig
     }
     tdd_join_mat_result_t R;
     int rc = tdd_join_mat(&M, &R);
-    if (rc == TD_ERR_NYI) return NULL;
+    /* declined, or out of memory for its partitions: the pair-producing
+     * join needs less (the reference's fallback to its chained join) */
+    if (rc == TD_ERR_NYI || (rc == TD_ERR_OOM && J->force_path == 0)) return NULL;
     if (rc) return TD_ERR_PTR(rc);
     td__set_plan("join path=%d radix_bits=%d rows=%lld", R.path, R.radix_bits,
                  (long long)R.n_rows);

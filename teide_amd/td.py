@@ -155,6 +155,7 @@ class Engine:
                 "td_dev_serve_step": (C.c_int, [P]), "td_dev_serve_poll": (C.c_int, [P, P, P]),
                 "td_dev_serve_info": (None, [P, P]), "td_dev_serve_close": (None, [P]),
                 "td_dev_cancel_after": (None, [I64]), "td_dev_cancel_polls": (I64, []),
+                "td_dev_fail_allocs": (None, [C.c_uint64, I64]),
                 "td_heap_gc": (None, []),
             })
         for name, (res, args) in sig.items():

@@ -154,3 +154,55 @@ def test_over_budget_returns_oom_and_frees(eng):
     eng.release(got)
     g.free()
     eng.release(t)
+
+
+@pytest.mark.parametrize("shape", ["hash_group", "radix_join", "join_group"])
+def test_out_of_memory_falls_back(eng, shape):
+    """The first large request of the partitioned plan fails (the injected
+    out-of-memory of td_dev_fail_allocs): the group-by falls back to the
+    global hash table, the materialising join to the pair-producing join,
+    as the reference falls back to its sequential group table and chained
+    join (exec.c:6486-6498, 8219-8231); same rows, nothing leaked.  With a
+    forced plan the query returns TD_ERR_OOM instead."""
+    rng = np.random.default_rng(43)
+    n, nd = 20_000_000, 1_000_003
+    keys = rng.integers(-2 ** 62, 2 ** 62, nd)
+    t = eng.put(eng.table([("k", keys[rng.integers(0, nd, n)]),
+                           ("v", rng.integers(-10 ** 6, 10 ** 6, n))]))
+    d = eng.put(eng.table([("k", keys), ("w", rng.integers(0, 700, nd))]))
+    g, root = {"hash_group": lambda: hash_group(eng, t), "radix_join": lambda: radix_join(eng, t, d),
+               "join_group": lambda: join_group(eng, t, d)}[shape]()
+    try:
+        ref = g.execute(root)
+        plan_ref = eng.last_plan()
+        eng.L.td_dev_sync()
+        base = eng.mem_stats()["in_use"]
+        # the smallest request size that only the partitioned plan makes
+        # (join_group partitions only the filtered rows' grouped columns)
+        big = (64 if shape == "join_group" else 128) << 20
+        eng.L.td_dev_fail_allocs(big, 1)
+        got = g.execute(root)
+        plan = eng.last_plan()
+        eng.L.td_dev_fail_allocs(0, 0)
+        assert plan != plan_ref, (plan_ref, plan)             # another plan answered
+        if shape == "hash_group":
+            assert "path=5" in plan_ref and "group path=3" in plan, (plan_ref, plan)
+        same(eng, got, ref, f"{shape} after out-of-memory")
+        eng.release(got)
+        eng.L.td_dev_sync()
+        assert eng.mem_stats()["in_use"] == base, shape
+        # forced plans report the failure
+        eng.force_paths(2, 4, 0)
+        eng.L.td_dev_fail_allocs(big, 1)
+        with pytest.raises(TdError) as ei:
+            eng.release(g.execute(root))
+        assert ei.value.code == TD_ERR_OOM
+        eng.L.td_dev_sync()
+        assert eng.mem_stats()["in_use"] == base, shape
+        eng.release(ref)
+    finally:
+        eng.L.td_dev_fail_allocs(0, 0)
+        eng.force_paths(0, 0, 0)
+    g.free()
+    eng.release(t)
+    eng.release(d)
