@@ -13,7 +13,13 @@
  *   probe    a persistent grid walks the probe tiles: star predicate, slot
  *            map gather, and a fold into the query's group table -- in LDS
  *            per tile when the table is small (flushed with one atomic per
- *            touched cell), else straight into the global table;
+ *            touched cell); a query whose table does not fit writes its
+ *            passing rows as records (slot + aggregate inputs) and counts
+ *            them per group slice instead;
+ *   bucket   those records counting-sorted by slice (a slice's table fits
+ *            LDS), then one workgroup per (query, slice) folds its records
+ *            in LDS and stores the slice of the table -- no global atomics
+ *            per row (they bound the global-table fold at ~26 G/s);
  *   emit     one workgroup per query counts non-empty slots, then writes its
  *            groups in ascending key order into shared output columns at the
  *            query's offset.
@@ -32,6 +38,9 @@
 
 #define BQ_R 4                      /* rows per lane per tile step */
 #define BQ_LDS_MAX (48 * 1024)      /* per-tile LDS group table limit */
+#define BQ_MAX_SL  4096             /* group slices of a bucketed query */
+#define BQ_SC      4096             /* records per bucket-scatter tile */
+#define BQ_SCT     256              /* threads of a bucket-scatter workgroup */
 
 /* Per-query device descriptor. */
 struct BQuery {
@@ -47,6 +56,17 @@ struct BQuery {
     int64_t   map_off;              /* bytes into the map arena */
     int64_t   tab_off;              /* cells into the table arena */
     int64_t   n_slots;
+    int32_t   nsl, sg;              /* bucketed fold: group slices, slots per slice (0: none) */
+    int64_t   rec_off;              /* its record region in the record buffers */
+    int64_t   sl_off;               /* its nsl + 1 slice entries */
+    int32_t   nrw, pad2;            /* record value words */
+    uint8_t   rw_slot[TL_NC];       /* word w holds tile slot rw_slot[w] */
+    uint8_t   cell_rw[GP_MAX_CELLS];/* cell c reads record word cell_rw[c] */
+};
+
+struct BRec {                       /* records of bucketed queries (SoA) */
+    uint32_t* slot;
+    uint64_t* val[TL_NC - 1];
 };
 
 struct BTile {
@@ -126,6 +146,103 @@ __device__ __forceinline__ void bq_flush(const GroupPlan& P, const uint64_t* src
     }
 }
 
+/* The star predicate and the slot of each of a lane's rows (-1: none). */
+__device__ __forceinline__ void bq_slots(const BQuery& q, const uint8_t* maps, int64_t base,
+                                         int64_t r1, int64_t (&cv)[TL_NC][BQ_R],
+                                         int32_t (&s)[BQ_R]) {
+    load_tile<BQ_R>(q.T, base, r1, cv);
+    const uint32_t m = tile_live<BQ_R>(nullptr, base, r1) & bq_pred(q, cv);
+    const uint16_t* m16 = (const uint16_t*)(maps + q.map_off);
+    const int32_t* m32 = (const int32_t*)(maps + q.map_off);
+#pragma unroll
+    for (int j = 0; j < BQ_R; j++) {
+        s[j] = -1;
+        const uint64_t i = (uint64_t)(cv[0][j] - q.kmin);
+        if (((m >> j) & 1) && i < (uint64_t)q.range) {
+            if (q.map_u16) {
+                uint16_t v = m16[i];
+                s[j] = v == 0xffff ? -1 : (int32_t)v;
+            } else {
+                s[j] = m32[i];
+            }
+        }
+    }
+}
+
+template <int J>
+__device__ __forceinline__ uint64_t bq_word(const int64_t (&cv)[TL_NC][BQ_R], int slot) {
+    switch (slot) {
+    case 1: return (uint64_t)cv[1][J];
+    case 2: return (uint64_t)cv[2][J];
+    case 3: return (uint64_t)cv[3][J];
+    default: return (uint64_t)cv[0][J];
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void bq_put(const BQuery& q, const BRec& R, uint32_t* h,
+                                       const int64_t (&cv)[TL_NC][BQ_R], const int32_t (&s)[BQ_R],
+                                       int64_t& at) {
+    if (s[J] < 0) return;
+    R.slot[at] = (uint32_t)s[J];
+    for (int w = 0; w < q.nrw; w++) R.val[w][at] = bq_word<J>(cv, q.rw_slot[w]);
+    atomicAdd(&h[s[J] / q.sg], 1u);
+    at++;
+}
+
+/* A wave's passing rows as records: one reservation per wave step, the
+ * slice counts of the tile in LDS (h).  Every lane of the wave calls it. */
+__device__ __forceinline__ void bq_records(const BQuery& q, const BRec& R,
+                                           unsigned long long* cur, uint32_t* h,
+                                           const int64_t (&cv)[TL_NC][BQ_R],
+                                           const int32_t (&s)[BQ_R], int lane) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < BQ_R; j++) c += s[j] >= 0;
+    uint32_t inc = c;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+    }
+    const uint32_t tot = __shfl(inc, WAVE - 1);
+    if (!tot) return;
+    unsigned long long b = 0;
+    if (lane == WAVE - 1) b = atomicAdd(cur, (unsigned long long)tot);
+    b = __shfl(b, WAVE - 1);
+    int64_t at = q.rec_off + (int64_t)b + (int64_t)(inc - c);
+    bq_put<0>(q, R, h, cv, s, at);
+    bq_put<1>(q, R, h, cv, s, at);
+    bq_put<2>(q, R, h, cv, s, at);
+    bq_put<3>(q, R, h, cv, s, at);
+}
+
+/* Probe tiles of bucketed queries: passing rows to records, slice counts. */
+__global__ __launch_bounds__(512) void k_bq_precord(const BQuery* Q, const BTile* tiles,
+                                                    int64_t n_tiles, const uint8_t* maps, BRec rec,
+                                                    unsigned long long* qcur, uint32_t* sl_cnt) {
+    extern __shared__ uint32_t h[];
+    const int lane = lane_id();
+    const int nwaves = blockDim.x / WAVE, wv = threadIdx.x / WAVE;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        for (int i = threadIdx.x; i < q.nsl; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        for (int64_t wb = W.r0 + (int64_t)wv * WAVE * BQ_R; wb < W.r1;
+             wb += (int64_t)nwaves * WAVE * BQ_R) {            /* the same trips for a wave */
+            int64_t cv[TL_NC][BQ_R];
+            int32_t s[BQ_R];
+            bq_slots(q, maps, wb + (int64_t)lane * BQ_R, W.r1, cv, s);
+            bq_records(q, rec, qcur + W.q, h, cv, s, lane);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < q.nsl; i += blockDim.x)
+            if (h[i]) atomicAdd(&sl_cnt[q.sl_off + i], h[i]);
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(512) void k_bq_probe(const BQuery* Q, const BTile* tiles,
                                                   int64_t n_tiles, const uint8_t* maps,
                                                   uint64_t* tabs) {
@@ -172,6 +289,146 @@ __global__ __launch_bounds__(512) void k_bq_probe(const BQuery* Q, const BTile* 
             bq_flush(P, lt, gtab, q.n_slots);
             __syncthreads();
         }
+    }
+}
+
+/* -------------------------------------------------------------- bucket -- */
+
+/* Slice starts of each bucketed query (record positions), and cursors. */
+__global__ void k_bq_bscan(const BQuery* Q, const uint32_t* cnt, int64_t* pos,
+                           unsigned long long* cur) {
+    const BQuery& q = Q[blockIdx.x];
+    if (!q.active || !q.nsl || threadIdx.x) return;
+    int64_t a = q.rec_off;
+    for (int i = 0; i < q.nsl; i++) {
+        pos[q.sl_off + i] = a;
+        cur[q.sl_off + i] = (unsigned long long)a;
+        a += cnt[q.sl_off + i];
+    }
+    pos[q.sl_off + q.nsl] = a;
+}
+
+/* Records of a tile into slice order: ranks from LDS counters, one
+ * reservation per (tile, slice). */
+__global__ __launch_bounds__(BQ_SCT) void k_bq_bscatter(const BQuery* Q, const BTile* tiles,
+                                                        int64_t n_tiles,
+                                                        const unsigned long long* qcur, BRec A,
+                                                        BRec B, unsigned long long* cur) {
+    extern __shared__ uint32_t sh[];
+    constexpr int K = BQ_SC / BQ_SCT;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        const int64_t nrec = (int64_t)qcur[W.q];
+        const int64_t r1 = W.r1 < nrec ? W.r1 : nrec;
+        if (W.r0 >= r1) continue;                 /* uniform */
+        uint32_t* cnt = sh;
+        unsigned long long* base = (unsigned long long*)(sh + ((q.nsl + 1) & ~1));
+        for (int i = threadIdx.x; i < q.nsl; i += BQ_SCT) cnt[i] = 0;
+        __syncthreads();
+        int32_t sl[K];
+        uint32_t rk[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int64_t r = W.r0 + threadIdx.x + (int64_t)k * BQ_SCT;
+            sl[k] = r < r1 ? (int32_t)(A.slot[q.rec_off + r] / (uint32_t)q.sg) : -1;
+            rk[k] = sl[k] >= 0 ? atomicAdd(&cnt[sl[k]], 1u) : 0u;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < q.nsl; i += BQ_SCT)
+            if (cnt[i]) base[i] = atomicAdd(&cur[q.sl_off + i], (unsigned long long)cnt[i]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (sl[k] < 0) continue;
+            const int64_t from = q.rec_off + W.r0 + threadIdx.x + (int64_t)k * BQ_SCT;
+            const int64_t to = (int64_t)base[sl[k]] + rk[k];
+            B.slot[to] = A.slot[from];
+            for (int w = 0; w < q.nrw; w++) B.val[w][to] = A.val[w][from];
+        }
+        __syncthreads();
+    }
+}
+
+/* Cell k of U records (bits of `live`) at LDS rows loc[u], the input from
+ * record word W: one dispatch on the cell's kind for all U records. */
+template <int U, int W>
+__device__ __forceinline__ void bq_fold_cells(uint64_t* lt, int nc, int k, int kind, bool f,
+                                              const uint32_t (&loc)[U], uint32_t live,
+                                              const uint64_t (&v)[TL_NC - 1][U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (!((live >> u) & 1)) continue;
+        uint64_t* c = &lt[(int64_t)loc[u] * nc + k];
+        const uint64_t x = v[W][u];
+        const double d = f ? __longlong_as_double((long long)x) : (double)(int64_t)x;
+        switch (kind) {                           /* uniform */
+        case C_ADD_F64: atomicAdd((double*)c, d); break;
+        case C_SQ_F64: atomicAdd((double*)c, d * d); break;
+        case C_ADD_I64: atomicAdd((unsigned long long*)c, (unsigned long long)x); break;
+        case C_MIN_I64: cell_min(c, (int64_t)x); break;
+        case C_MAX_I64: cell_max(c, (int64_t)x); break;
+        case C_MIN_F64: cell_min(c, f64_ord(d)); break;
+        case C_MAX_F64: cell_max(c, f64_ord(d)); break;
+        default: break;
+        }
+    }
+}
+
+#define BQ_FU 4                             /* records per thread per fold step */
+
+/* One workgroup per (query, slice): the slice's records folded in LDS, then
+ * the slice of the table stored (this workgroup alone writes it). */
+__global__ __launch_bounds__(512) void k_bq_bfold(const BQuery* Q, const BTile* tiles,
+                                                  int64_t n_tiles, const int64_t* pos, BRec B,
+                                                  uint64_t* tabs) {
+    extern __shared__ uint64_t lt[];
+    __shared__ int8_t ckind[GP_MAX_CELLS], cword[GP_MAX_CELLS], cf64[GP_MAX_CELLS];
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        const GroupPlan& P = q.P;
+        const int nc = P.ncells, nrw = q.nrw;
+        if (threadIdx.x < GP_MAX_CELLS && (int)threadIdx.x < nc) {
+            const int k = threadIdx.x;
+            ckind[k] = (int8_t)P.kind[k];
+            cword[k] = (int8_t)q.cell_rw[k];
+            cf64[k] = q.T.col[q.rw_slot[q.cell_rw[k]]].type == TD_F64;
+        }
+        const int64_t s0 = W.r0 * q.sg;
+        const int64_t ns = s0 + q.sg <= q.n_slots ? q.sg : q.n_slots - s0;
+        for (int64_t i = threadIdx.x; i < ns * nc; i += blockDim.x)
+            lt[i] = cell_identity(P.kind[i % nc]);
+        __syncthreads();
+        const int64_t beg = pos[q.sl_off + W.r0], end = pos[q.sl_off + W.r0 + 1];
+        for (int64_t r0 = beg + threadIdx.x; r0 < end; r0 += (int64_t)blockDim.x * BQ_FU) {
+            uint32_t loc[BQ_FU];
+            uint64_t v[TL_NC - 1][BQ_FU];
+            uint32_t live = 0;
+#pragma unroll
+            for (int u = 0; u < BQ_FU; u++) {      /* every load of the step first */
+                const int64_t r = r0 + (int64_t)u * blockDim.x;
+                const bool in = r < end;
+                loc[u] = in ? (uint32_t)((int64_t)B.slot[r] - s0) : 0u;
+#pragma unroll
+                for (int w = 0; w < TL_NC - 1; w++) v[w][u] = in && w < nrw ? B.val[w][r] : 0ull;
+                live |= (uint32_t)in << u;
+            }
+#pragma unroll
+            for (int u = 0; u < BQ_FU; u++)
+                if ((live >> u) & 1) atomicAdd((unsigned long long*)&lt[(int64_t)loc[u] * nc], 1ull);
+            for (int k = 1; k < nc; k++) {
+                switch (cword[k]) {                       /* compile-time record word */
+                case 0: bq_fold_cells<BQ_FU, 0>(lt, nc, k, ckind[k], cf64[k], loc, live, v); break;
+                case 1: bq_fold_cells<BQ_FU, 1>(lt, nc, k, ckind[k], cf64[k], loc, live, v); break;
+                default: bq_fold_cells<BQ_FU, 2>(lt, nc, k, ckind[k], cf64[k], loc, live, v); break;
+                }
+            }
+        }
+        __syncthreads();
+        uint64_t* dst = tabs + q.tab_off + s0 * nc;
+        for (int64_t i = threadIdx.x; i < ns * nc; i += blockDim.x) dst[i] = lt[i];
+        __syncthreads();
     }
 }
 
@@ -252,8 +509,20 @@ struct tdd_jg_batch {
     void* tl_map = nullptr;   int64_t n_map = 0;
     void* tl_tab = nullptr;   int64_t n_tab = 0;
     void* tl_probe = nullptr; int64_t n_probe = 0;
+    void* tl_prec = nullptr;  int64_t n_prec = 0;     /* probe tiles of bucketed queries */
+    void* tl_bsc = nullptr;   int64_t n_bsc = 0;      /* bucket scatter: (query, records) */
+    void* tl_bfold = nullptr; int64_t n_bfold = 0;    /* bucket fold: (query, slice) */
+    void* rec_mem = nullptr;                          /* the two record buffers */
+    BRec recA, recB;
+    void* qcur = nullptr;                             /* u64[n]: records per query */
+    void* sl_cnt = nullptr;                           /* u32: records per (query, slice) */
+    void* sl_pos = nullptr;                           /* i64: slice starts (+ end) */
+    void* sl_cur = nullptr;                           /* u64: slice cursors */
+    int64_t sl_total = 0;
+    int nsl_max = 0;
+    size_t lds_fold = 0;
     size_t map_bytes = 0, tab_cells = 0;
-    int grid_probe = 0, block_probe = 512;
+    int grid_probe = 0, grid_prec = 0, block_probe = 512;
     size_t lds_probe = 0;
     int64_t tile_rows = 16384;
     GroupPlan shape;                        /* output types of the shape */
@@ -263,7 +532,8 @@ struct tdd_jg_batch {
 
 static void bq_release(tdd_jg_batch* b) {
     void* ps[] = {b->dq, b->maps, b->tabs, b->occ, b->ng, b->offs, b->tl_build, b->tl_map,
-                  b->tl_tab, b->tl_probe};
+                  b->tl_tab, b->tl_probe, b->tl_prec, b->tl_bsc, b->tl_bfold, b->rec_mem, b->qcur,
+                  b->sl_cnt, b->sl_pos, b->sl_cur};
     for (void* p : ps) if (p) tdd::release(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -298,10 +568,111 @@ static int bq_launch(tdd_jg_batch* b) {
     if (b->n_tab)
         LAUNCH("batch_init", k_bq_init, stream_grid(b->n_tab, 8), 256, 0, Q, (const BTile*)b->tl_tab,
                b->n_tab, (uint64_t*)b->tabs);
+    if (b->n_bfold) {
+        rc = tdd_fill(b->qcur, 0, (size_t)b->n * 8);
+        if (!rc) rc = tdd_fill(b->sl_cnt, 0, (size_t)b->sl_total * 4);
+        if (rc) return rc;
+    }
     if (b->n_probe)
         LAUNCH("batch_probe", k_bq_probe, b->grid_probe, b->block_probe, b->lds_probe, Q,
                (const BTile*)b->tl_probe, b->n_probe, (const uint8_t*)b->maps, (uint64_t*)b->tabs);
+    if (b->n_prec)
+        LAUNCH("batch_records", k_bq_precord, b->grid_prec, b->block_probe,
+               (size_t)b->nsl_max * 4, Q, (const BTile*)b->tl_prec, b->n_prec,
+               (const uint8_t*)b->maps, b->recA, (unsigned long long*)b->qcur,
+               (uint32_t*)b->sl_cnt);
+    if (b->n_bfold) {
+        LAUNCH("batch_bucket", k_bq_bscan, b->n, 64, 0, Q, (const uint32_t*)b->sl_cnt,
+               (int64_t*)b->sl_pos, (unsigned long long*)b->sl_cur);
+        if (b->n_bsc)
+            LAUNCH("batch_bucket", k_bq_bscatter, stream_grid(b->n_bsc, 8), BQ_SCT,
+                   (size_t)((b->nsl_max + 1) & ~1) * 4 + (size_t)b->nsl_max * 8, Q,
+                   (const BTile*)b->tl_bsc, b->n_bsc, (const unsigned long long*)b->qcur, b->recA,
+                   b->recB, (unsigned long long*)b->sl_cur);
+        LAUNCH("batch_fold", k_bq_bfold, stream_grid(b->n_bfold, 4), 512, b->lds_fold, Q,
+               (const BTile*)b->tl_bfold, b->n_bfold, (const int64_t*)b->sl_pos, b->recB,
+               (uint64_t*)b->tabs);
+    }
     return TD_OK;
+}
+
+/* Bucketed fold of the queries whose group tables exceed the per-tile LDS
+ * budget: slices whose tables fit it, record buffers, work lists.  Queries
+ * with more slices than BQ_MAX_SL keep the global-table fold. */
+static int bq_bucket_plan(tdd_jg_batch* b) {
+    int64_t rec_total = 0, sl_total = 0;
+    int nrw_max = 0;
+    b->nsl_max = 0;
+    b->lds_fold = 0;
+    for (int i = 0; i < b->n; i++) {
+        BQuery& q = b->hq[i];
+        q.nsl = q.sg = 0;
+        const GroupPlan& P = q.P;
+        const int64_t tbytes = q.n_slots * P.ncells * 8;
+        if (b->status[i] || tbytes <= (int64_t)TUNE_INT("TEIDE_BQ_BUCKET_KB", 48) * 1024) continue;
+        const int64_t sg = BQ_LDS_MAX / (8 * (int64_t)P.ncells);
+        const int64_t nsl = ceil_div(q.n_slots, sg);
+        if (sg < 1 || nsl > BQ_MAX_SL) continue;
+        /* record words: the tile slots the cells read */
+        q.nrw = 0;
+        for (int c = 1; c < P.ncells; c++) {
+            const int sl = q.cell_slot[c];
+            int w = -1;
+            for (int k = 0; k < q.nrw; k++) if (q.rw_slot[k] == sl) w = k;
+            if (w < 0) { w = q.nrw++; q.rw_slot[w] = (uint8_t)sl; }
+            q.cell_rw[c] = (uint8_t)w;
+        }
+        if (q.nrw > TL_NC - 1) { q.nrw = 0; continue; }
+        q.nsl = (int32_t)nsl;
+        q.sg = (int32_t)sg;
+        q.rec_off = rec_total;
+        q.sl_off = sl_total;
+        rec_total += q.nl;
+        sl_total += nsl + 1;
+        if (q.nrw > nrw_max) nrw_max = q.nrw;
+        if ((int)nsl > b->nsl_max) b->nsl_max = (int)nsl;
+        const size_t lf = (size_t)sg * P.ncells * 8;
+        if (lf > b->lds_fold) b->lds_fold = lf;
+    }
+    b->sl_total = sl_total;
+    if (!sl_total) return TD_OK;
+    /* two record buffers: slot u32 + nrw_max words each */
+    const size_t per = 4 + 8 * (size_t)nrw_max;
+    b->rec_mem = tdd::scratch((size_t)rec_total * per * 2 + 1024);
+    b->qcur = tdd::scratch((size_t)b->n * 8);
+    b->sl_cnt = tdd::scratch((size_t)sl_total * 4);
+    b->sl_pos = tdd::scratch((size_t)sl_total * 8);
+    b->sl_cur = tdd::scratch((size_t)sl_total * 8);
+    if (!b->rec_mem || !b->qcur || !b->sl_cnt || !b->sl_pos || !b->sl_cur) return TD_ERR_OOM;
+    uint8_t* m = (uint8_t*)b->rec_mem;
+    auto carve = [&](size_t bytes) { uint8_t* r = m; m += (bytes + 255) & ~(size_t)255; return r; };
+    memset(&b->recA, 0, sizeof(BRec));
+    memset(&b->recB, 0, sizeof(BRec));
+    b->recA.slot = (uint32_t*)carve((size_t)rec_total * 4);
+    b->recB.slot = (uint32_t*)carve((size_t)rec_total * 4);
+    for (int w = 0; w < nrw_max; w++) {
+        b->recA.val[w] = (uint64_t*)carve((size_t)rec_total * 8);
+        b->recB.val[w] = (uint64_t*)carve((size_t)rec_total * 8);
+    }
+    std::vector<BTile> sc, fo;
+    for (int i = 0; i < b->n; i++) {
+        const BQuery& q = b->hq[i];
+        if (!q.nsl) continue;
+        for (int64_t r = 0; r < q.nl; r += BQ_SC)
+            sc.push_back(BTile{i, 0, r, r + BQ_SC < q.nl ? r + BQ_SC : q.nl});
+        for (int k = 0; k < q.nsl; k++) fo.push_back(BTile{i, 0, k, k + 1});
+    }
+    b->n_bsc = (int64_t)sc.size();
+    b->n_bfold = (int64_t)fo.size();
+    int rc = upload(&b->tl_bsc, sc.data(), sc.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_bfold, fo.data(), fo.size() * sizeof(BTile));
+    return rc;
+}
+
+/* Grids of the two probe kernels for the current block size and tiles. */
+static void bq_grids(tdd_jg_batch* b) {
+    b->grid_probe = resident_grid(k_bq_probe, b->block_probe, b->lds_probe, b->n_probe);
+    b->grid_prec = resident_grid(k_bq_precord, b->block_probe, (size_t)b->nsl_max * 4, b->n_prec);
 }
 
 /* Autotuned probe launch shape, per size class (log2 of the probe rows). */
@@ -309,12 +680,19 @@ static std::map<int, std::pair<int64_t, int>> g_bq_tuned;
 
 static int bq_tiles_for(tdd_jg_batch* b, int64_t chunk) {
     if (b->tl_probe) { tdd::release(b->tl_probe); b->tl_probe = nullptr; }
+    if (b->tl_prec) { tdd::release(b->tl_prec); b->tl_prec = nullptr; }
     std::vector<BTile> tp = make_tiles(b->n, chunk, [&](int q) {
-        return b->status[q] ? (int64_t)0 : b->hq[q].nl;
+        return b->status[q] || b->hq[q].nsl ? (int64_t)0 : b->hq[q].nl;
+    });
+    std::vector<BTile> tr = make_tiles(b->n, chunk, [&](int q) {
+        return b->status[q] || !b->hq[q].nsl ? (int64_t)0 : b->hq[q].nl;
     });
     b->n_probe = (int64_t)tp.size();
+    b->n_prec = (int64_t)tr.size();
     b->tile_rows = chunk;
-    return upload(&b->tl_probe, tp.data(), tp.size() * sizeof(BTile));
+    int rc = upload(&b->tl_probe, tp.data(), tp.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_prec, tr.data(), tr.size() * sizeof(BTile));
+    return rc;
 }
 
 static int bq_autotune(tdd_jg_batch* b, int64_t total_rows) {
@@ -338,7 +716,7 @@ static int bq_autotune(tdd_jg_batch* b, int64_t total_rows) {
         if (rc) return rc;
         for (int bs : blocks) {
             b->block_probe = bs;
-            b->grid_probe = resident_grid(k_bq_probe, bs, b->lds_probe, b->n_probe);
+            bq_grids(b);
             rc = bq_launch(b);                     /* warm + reset tables */
             if (rc) return rc;
             HC(hipEventRecord(e0, tdd::g_stream));
@@ -473,7 +851,7 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
     if (!rc) rc = upload(&b->tl_map, tm_.data(), tm_.size() * sizeof(BTile));
     if (!rc) rc = upload(&b->tl_tab, tt_.data(), tt_.size() * sizeof(BTile));
     if (!rc) rc = bq_tiles_for(b, 16384);
-    if (!rc) b->grid_probe = resident_grid(k_bq_probe, b->block_probe, b->lds_probe, b->n_probe);
+    if (!rc) bq_grids(b);
     /* unique build keys: one map pass and a count, checked on the host */
     if (!rc) rc = tdd_fill(b->maps, ~0ull, b->map_bytes);
     if (!rc) rc = tdd_fill(b->occ, 0, (size_t)n * 8);
@@ -506,8 +884,24 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
             if (!rc) rc = bq_tiles_for(b, 16384);
         }
     }
+    if (!rc) rc = bq_bucket_plan(b);
+    if (!rc && b->sl_total) {
+        HC(hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * n, hipMemcpyHostToDevice,
+                          tdd::g_stream));
+        HC(hipStreamSynchronize(tdd::g_stream));
+        rc = bq_tiles_for(b, b->tile_rows);          /* bucketed queries' own probe tiles */
+        /* the slice folds store whole tables: no identity fill for them */
+        std::vector<BTile> c3 = make_tiles(n, 65536, [&](int q) {
+            return b->status[q] || b->hq[q].nsl ? (int64_t)0
+                                                : b->hq[q].n_slots * b->hq[q].P.ncells;
+        });
+        tdd::release(b->tl_tab);
+        b->tl_tab = nullptr;
+        b->n_tab = (int64_t)c3.size();
+        if (!rc) rc = upload(&b->tl_tab, c3.data(), c3.size() * sizeof(BTile));
+    }
     if (!rc && tune) rc = bq_autotune(b, total_rows);
-    if (!rc) b->grid_probe = resident_grid(k_bq_probe, b->block_probe, b->lds_probe, b->n_probe);
+    if (!rc) bq_grids(b);
     if (rc) { bq_release(b); delete b; return rc; }
     *out = b;
     return TD_OK;
