@@ -607,11 +607,11 @@ class BatchQueries:
     beside it."""
     name = "batch"
 
-    def __init__(self, e, a):
+    def __init__(self, e, a, scattered=False, n=None):
         from teide_amd import workloads as W
         import numpy as np
         self.e, self.a = e, a
-        n = a.batch_queries
+        n = n or a.batch_queries
         nf, ncat, thr = W.batch_spec(n, SEED)
         t0 = time.perf_counter()
         self.tables, self.dags = [], []
@@ -620,6 +620,9 @@ class BatchQueries:
             nd = max(1, int(nf[q]) // 8)
             fact = W.join_fact(int(nf[q]), nd, SEED + q)
             dim = W.join_dim(nd, int(ncat[q]), SEED + q)
+            if scattered:            # the same matches on keys spread over 64-bit values
+                fact["fk"] = fact["fk"] * 7_000_000_019 - (1 << 62)
+                dim["pk"] = dim["pk"] * 7_000_000_019 - (1 << 62)
             hf = e.table([("fk", fact["fk"]), ("qty", fact["qty"]), ("price", fact["price"])])
             hd = e.table([("pk", dim["pk"]), ("cat", dim["cat"])])
             tf, td = e.put(hf), e.put(hd)
@@ -642,10 +645,14 @@ class BatchQueries:
         self.h = e.batch_prepare([g for g, _ in self.dags], [r for _, r in self.dags])
         self.prepare_s = time.perf_counter() - t0
         nb, tr, bl = e.batch_info(self.h)
+        nh, pr, mb = e.batch_radix(self.h)
+        keys = ("unique keys spread over 64-bit values (radix hash path)" if scattered
+                else "unique keys")
         self.config = {"workload": f"{n} filter+join+group-by queries, fact rows log-uniform "
-                                   f"[1K,2M] (sum {self.rows:,}), dim = fact/8 unique keys, "
+                                   f"[1K,2M] (sum {self.rows:,}), dim = fact/8 {keys}, "
                                    f"groups log-uniform [4,65536]; one batch, graph replay",
-                       "batched": nb, "tile_rows": tr, "block": bl,
+                       "batched": nb, "tile_rows": tr, "block": bl, "hash_queries": nh,
+                       "part_rows": pr, "max_radix_bits": mb,
                        "prepare_s": round(self.prepare_s, 3)}
 
     def step(self):
@@ -1014,11 +1021,12 @@ def run_form(fn):
         return {"error": f"{type(ex).__name__}: {ex}"}
 
 
-def batched_summary(e, a, steps=5, warmup=2):
+def batched_summary(e, a, steps=5, warmup=2, scattered=False, n=None):
     """The batched form on its own workload (4096 queries), beside the
-    plain form (one td_execute per query)."""
+    plain form (one td_execute per query); `scattered`: the join keys spread
+    over 64-bit values, so every query takes the radix hash path."""
     t0 = time.perf_counter()
-    wl = BatchQueries(e, a)
+    wl = BatchQueries(e, a, scattered, n)
     for _ in range(warmup):
         wl.step()
     e.L.td_dev_sync()
@@ -1033,7 +1041,8 @@ def batched_summary(e, a, steps=5, warmup=2):
            "batched_vs_plain": round(plain_s * 1e3 / ms, 2),
            "rows_per_s": wl.rows / (ms / 1e3),
            "tile_rows": wl.config["tile_rows"], "block": wl.config["block"],
-           "setup_s": round(t1 - t0, 1)}
+           "hash_queries": wl.config["hash_queries"], "part_rows": wl.config["part_rows"],
+           "max_radix_bits": wl.config["max_radix_bits"], "setup_s": round(t1 - t0, 1)}
     wl.release()
     return out
 
@@ -1056,13 +1065,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="join workload only: skip the batched and served forms")
-    ap.add_argument("--workload", choices=["jg", "jg2", "jgs", "hc", "batch", "served", "load",
+    ap.add_argument("--workload", choices=["jg", "jg2", "jgs", "hc", "batch", "batchs", "served", "load",
                                            "charge", "join", "parted"],
                     default="jg",
                     help="jg: join + group-by (default); jg2: the same query as two graphs "
                          "(join, then group-by over the joined table); jgs: jg with scattered "
                          "64-bit join keys; hc: high-cardinality group-by; "
                          "batch: many small join + group-by queries per batch; "
+                         "batchs: the same with scattered keys (radix hash path); "
                          "served: request stream through the request server; "
                          "load: splayed-table load into HBM")
     ap.add_argument("--load-rows", type=int, default=50_000_000)
@@ -1073,6 +1083,8 @@ def main():
     ap.add_argument("--parted-parts", type=int, default=30)
     ap.add_argument("--load-reps", type=int, default=3)
     ap.add_argument("--batch-queries", type=int, default=4096)
+    ap.add_argument("--batch-hash-queries", type=int, default=1024,
+                    help="queries of the batched form's scattered-key (radix hash) run")
     ap.add_argument("--serve-fact", type=int, default=100_000_000)
     ap.add_argument("--serve-dim", type=int, default=8_000_000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered requests/s")
@@ -1141,6 +1153,7 @@ def main():
         return
 
     wl = {"hc": HighCard, "batch": BatchQueries, "charge": Charge, "jgs": JoinGroupScattered,
+          "batchs": lambda e, a: BatchQueries(e, a, scattered=True),
           "jg2": JoinGroupTwoCall, "parted": Parted, "join": Join}.get(a.workload,
                                                                        JoinGroup)(e, a)
 
@@ -1149,7 +1162,7 @@ def main():
     cpu = None if a.no_cpu_baseline else wl.cpu_baseline()
     query_gbs = wl.query_bytes / (ms / 1e3) / 1e9
     extra = {}
-    if a.workload == "batch":
+    if a.workload in ("batch", "batchs"):
         plain_s = wl.plain(1)
         extra = {"batched_qps": round(wl.n / (ms / 1e3), 1),
                  "plain_qps": round(wl.n / plain_s, 1),
@@ -1187,6 +1200,8 @@ def main():
     if a.workload == "jg" and not a.no_forms:
         line["hc"] = run_form(lambda: hc_summary(e, a, cpu=not a.no_cpu_baseline))
         line["batched"] = run_form(lambda: batched_summary(e, a))
+        line["batched_hash"] = run_form(lambda: batched_summary(e, a, scattered=True,
+                                                              n=a.batch_hash_queries))
         line["served"] = run_form(lambda: served_summary(e, a))
     print(json.dumps(line), flush=True)
 

@@ -85,6 +85,9 @@ td_dev_batch_t* td_dev_batch_prepare(td_graph_t* const* graphs, td_op_t* const* 
 td_err_t td_dev_batch_run(td_dev_batch_t* b, td_t** results);
 /* batched queries, autotuned probe tile rows and workgroup size */
 int64_t  td_dev_batch_info(const td_dev_batch_t* b, int64_t* tile_rows, int* block);
+/* batched queries on the radix hash path (build keys not a dense unique
+ * range), their autotuned build rows per partition and largest radix bits */
+int64_t  td_dev_batch_radix(const td_dev_batch_t* b, int64_t* part_rows, int* max_bits);
 void     td_dev_batch_free(td_dev_batch_t* b);
 
 /* Serving: a request loop over a resident fact / dim pair.  Requests are

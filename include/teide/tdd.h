@@ -293,12 +293,18 @@ int tdd_join_group(const tdd_join_group_spec_t* spec, tdd_group_result_t* out);
  * the launch sequence as a hipGraph on the first run and replay it after),
  * writing every batched query's groups into one output block per column,
  * query q's rows at [offsets[q], offsets[q+1]).  status(q) != 0 marks a
- * query the batch could not take (run it with tdd_join_group). */
+ * query the batch could not take (run it with tdd_join_group).  Queries
+ * whose build keys are not a dense unique range join through radix
+ * partitions of both sides with LDS hash tables (the reference's radix
+ * join, exec.c:7431-7950, for every such query of the batch at once);
+ * radix() returns their number, the autotuned build rows per partition
+ * and the largest radix bits of any of them. */
 typedef struct tdd_jg_batch tdd_jg_batch_t;
 int  tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, int n, int tune,
                           tdd_jg_batch_t** out);
 int  tdd_jg_batch_status(const tdd_jg_batch_t* b, int q);
 int  tdd_jg_batch_shape(const tdd_jg_batch_t* b, int64_t* tile_rows, int* block);
+int  tdd_jg_batch_radix(const tdd_jg_batch_t* b, int64_t* part_rows, int* max_bits);
 int  tdd_jg_batch_run(tdd_jg_batch_t* b, int use_graph, tdd_group_result_t* out,
                       int64_t* offsets);
 void tdd_jg_batch_free(tdd_jg_batch_t* b);

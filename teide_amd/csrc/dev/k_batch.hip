@@ -32,6 +32,7 @@
 #include "group_plan.h"
 #include "tile.h"
 #include "star.h"
+#include "join_keys.h"
 
 #include <map>
 #include <vector>
@@ -62,6 +63,9 @@ struct BQuery {
     int32_t   nrw, pad2;            /* record value words */
     uint8_t   rw_slot[TL_NC];       /* word w holds tile slot rw_slot[w] */
     uint8_t   cell_rw[GP_MAX_CELLS];/* cell c reads record word cell_rw[c] */
+    int32_t   hash, hbits;          /* hash join (build keys not a dense unique range), its radix bits */
+    int64_t   hb_off, hp_off;       /* its (1 << hbits) + 1 partition entries per side */
+    int64_t   hbr_off, hpr_off;     /* its build / probe records */
 };
 
 struct BRec {                       /* records of bucketed queries (SoA) */
@@ -432,6 +436,285 @@ __global__ __launch_bounds__(512) void k_bq_bfold(const BQuery* Q, const BTile* 
     }
 }
 
+/* ---------------------------------------------------------------- hash -- */
+/* Queries whose build keys are not a dense unique range (scattered 64-bit
+ * keys, repeated keys) join through radix partitions of both sides, as the
+ * reference's radix join does (exec.c:7431-7950), every query of the batch
+ * in the same launches:
+ *
+ *   hist     rows per (query, partition) of each side: the top `hbits` bits
+ *            of the key's mix; probe rows only when they pass the predicate;
+ *   scan     per (query, side) exclusive sums -> record positions;
+ *   scatter  build rows -> (key, group slot) records, passing probe rows ->
+ *            (key, aggregate inputs) records, each in partition order;
+ *   join     one workgroup per (query, partition): the build records go into
+ *            an LDS hash table (linear probing, a repeated key takes a slot
+ *            of its own), each probe record walks its cluster and folds into
+ *            the group of every build row with its key -- LDS group table
+ *            when the query's table fits, flushed with one atomic per touched
+ *            cell, else global atomics.  A build partition larger than one
+ *            table is joined in table-sized chunks, the probe records
+ *            streamed once per chunk, so every (probe, build) pair is met
+ *            exactly once.
+ *
+ * The radix bits of a query follow from a target of build rows per
+ * partition that the autotuner picks per batch size class. */
+#define BH_MAXB  10                 /* radix bits of a hash query */
+#define BH_CAP   2048               /* build rows per LDS table */
+#define BH_TS    4096               /* LDS hash slots (load <= 0.5) */
+#define BH_ST    2048               /* rows per partition-kernel tile */
+#define BH_T     256                /* threads of the partition kernels */
+#define BH_JT    512                /* threads of the join kernel */
+#define BH_LDS_TAB (40 * 1024)      /* group table bytes kept in LDS by the join */
+#define BH_EMPTY 0xffffffffu
+
+struct BHRec {
+    uint64_t* bkey;                 /* build records: key, group slot */
+    int32_t*  bslot;
+    uint64_t* pkey;                 /* probe records: key, aggregate input words */
+    uint64_t* pval[TL_NC - 1];
+};
+
+__device__ __forceinline__ uint32_t bh_part(uint64_t k, int bits) {
+    return bits ? (uint32_t)(jmix(k) >> (64 - bits)) : 0u;
+}
+
+__device__ __forceinline__ uint32_t bh_slot(uint64_t k) {
+    return (uint32_t)jmix(k) & (BH_TS - 1);
+}
+
+/* Probe row r of q passes the star predicate. */
+__device__ __forceinline__ bool bh_pass(const BQuery& q, int64_t r) {
+    for (int i = 0; i < q.SP.npred; i++) {
+        const int64_t v[1] = {ld_raw(q.T.col[1 + i], r)};
+        if (!(range_mask<1>(q.SP, i, q.T.col[1 + i].type == TD_F64, v) & 1u)) return false;
+    }
+    return true;
+}
+
+/* Row r's key (side 0: build, 1: probe) and whether it takes part. */
+__device__ __forceinline__ bool bh_row(const BQuery& q, int side, int64_t r, uint64_t& k) {
+    if (side == 0) {
+        k = (uint64_t)ld_i64(q.rk, r);
+        return true;
+    }
+    k = (uint64_t)ld_i64(q.T.col[0], r);
+    return bh_pass(q, r);
+}
+
+__global__ __launch_bounds__(BH_T) void k_bh_hist(const BQuery* Q, const BTile* tiles,
+                                                  int64_t n_tiles, int side, uint32_t* cnt) {
+    __shared__ uint32_t h[1 << BH_MAXB];
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        const int nb = 1 << q.hbits;
+        for (int i = threadIdx.x; i < nb; i += BH_T) h[i] = 0;
+        __syncthreads();
+        for (int64_t r = W.r0 + threadIdx.x; r < W.r1; r += BH_T) {
+            uint64_t k;
+            if (bh_row(q, side, r, k)) atomicAdd(&h[bh_part(k, q.hbits)], 1u);
+        }
+        __syncthreads();
+        const int64_t off = side ? q.hp_off : q.hb_off;
+        for (int i = threadIdx.x; i < nb; i += BH_T)
+            if (h[i]) atomicAdd(&cnt[off + i], h[i]);
+        __syncthreads();
+    }
+}
+
+/* Block (query, side): positions and cursors of its partitions. */
+__global__ __launch_bounds__(256) void k_bh_scan(const BQuery* Q, const uint32_t* cnt,
+                                                 int64_t* pos, unsigned long long* cur) {
+    __shared__ int64_t ws[256 / WAVE];
+    const BQuery& q = Q[blockIdx.x >> 1];
+    const int side = blockIdx.x & 1;
+    if (!q.active || !q.hash) return;
+    const int nb = 1 << q.hbits;                 /* <= 1024: four entries a thread */
+    const int64_t off = side ? q.hp_off : q.hb_off;
+    const int64_t base = side ? q.hpr_off : q.hbr_off;
+    const int i0 = threadIdx.x * 4;
+    uint32_t c[4];
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[j] = i0 + j < nb ? cnt[off + i0 + j] : 0u;
+        s += c[j];
+    }
+    int64_t incl = s;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t y = __shfl_up(incl, d, WAVE);
+        if (lane_id() >= d) incl += y;
+    }
+    if (lane_id() == WAVE - 1) ws[threadIdx.x / WAVE] = incl;
+    __syncthreads();
+    int64_t a = base + incl - s;
+    for (int w = 0; w < (int)threadIdx.x / WAVE; w++) a += ws[w];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (i0 + j < nb) {
+            pos[off + i0 + j] = a;
+            cur[off + i0 + j] = (unsigned long long)a;
+        }
+        a += c[j];
+    }
+    if (threadIdx.x == 255) pos[off + nb] = a;   /* base + every count */
+}
+
+/* Rows of a tile to their partitions' records: LDS ranks, one reservation
+ * per (tile, partition). */
+__global__ __launch_bounds__(BH_T) void k_bh_scatter(const BQuery* Q, const BTile* tiles,
+                                                     int64_t n_tiles, int side,
+                                                     unsigned long long* cur, BHRec R) {
+    __shared__ uint32_t h[1 << BH_MAXB];
+    __shared__ unsigned long long hb[1 << BH_MAXB];
+    constexpr int K = BH_ST / BH_T;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const BTile W = tiles[t];
+        const BQuery& q = Q[W.q];
+        const int nb = 1 << q.hbits;
+        const int64_t off = side ? q.hp_off : q.hb_off;
+        for (int i = threadIdx.x; i < nb; i += BH_T) h[i] = 0;
+        __syncthreads();
+        uint64_t key[K];
+        uint32_t part[K], rank[K], live = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int64_t r = W.r0 + threadIdx.x + (int64_t)j * BH_T;
+            part[j] = rank[j] = 0;
+            key[j] = 0;
+            if (r < W.r1 && bh_row(q, side, r, key[j])) {
+                live |= 1u << j;
+                part[j] = bh_part(key[j], q.hbits);
+                rank[j] = atomicAdd(&h[part[j]], 1u);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += BH_T)
+            hb[i] = h[i] ? atomicAdd(&cur[off + i], (unsigned long long)h[i]) : 0ull;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (!((live >> j) & 1)) continue;
+            const int64_t r = W.r0 + threadIdx.x + (int64_t)j * BH_T;
+            const int64_t at = (int64_t)hb[part[j]] + rank[j];
+            if (side == 0) {
+                int64_t slot = 0;
+                for (int k = 0; k < q.P.n_keys; k++)
+                    slot += (ld_i64(q.P.keys[k], r) - q.P.kmin[k]) * q.P.kstride[k];
+                R.bkey[at] = key[j];
+                R.bslot[at] = (int32_t)slot;
+            } else {
+                R.pkey[at] = key[j];
+                for (int w = 0; w < q.nrw; w++) R.pval[w][at] = (uint64_t)ld_raw(q.T.col[q.rw_slot[w]], r);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* Every cell of group slot gs for one probe record (input words v0..v2). */
+__device__ __forceinline__ void bh_fold(uint64_t* tab, int nc, const int8_t* ckind,
+                                        const int8_t* cword, const int8_t* cf64, int32_t gs,
+                                        uint64_t v0, uint64_t v1, uint64_t v2) {
+    uint64_t* c = tab + (int64_t)gs * nc;
+    atomicAdd((unsigned long long*)&c[0], 1ull);
+    for (int k = 1; k < nc; k++) {
+        const int w = cword[k];
+        const uint64_t x = w == 0 ? v0 : (w == 1 ? v1 : v2);
+        const double d = cf64[k] ? __longlong_as_double((long long)x) : (double)(int64_t)x;
+        switch (ckind[k]) {
+        case C_ADD_F64: atomicAdd((double*)&c[k], d); break;
+        case C_SQ_F64: atomicAdd((double*)&c[k], d * d); break;
+        case C_ADD_I64: atomicAdd((unsigned long long*)&c[k], (unsigned long long)x); break;
+        case C_MIN_I64: cell_min(&c[k], (int64_t)x); break;
+        case C_MAX_I64: cell_max(&c[k], (int64_t)x); break;
+        case C_MIN_F64: cell_min(&c[k], f64_ord(d)); break;
+        case C_MAX_F64: cell_max(&c[k], f64_ord(d)); break;
+        default: break;
+        }
+    }
+}
+
+#define BH_PU 4                     /* probe records per thread in flight */
+
+__global__ __launch_bounds__(BH_JT) void k_bh_join(const BQuery* Q, const BTile* items,
+                                                   int64_t n_items, const int64_t* pos, BHRec R,
+                                                   uint64_t* tabs) {
+    extern __shared__ uint64_t lt[];             /* the query's group table (q.lds) */
+    __shared__ uint32_t hs[BH_TS];               /* slot -> build-local row */
+    __shared__ uint64_t hk[BH_CAP];
+    __shared__ int32_t hg[BH_CAP];
+    __shared__ int8_t ckind[GP_MAX_CELLS], cword[GP_MAX_CELLS], cf64[GP_MAX_CELLS];
+    for (int64_t t = blockIdx.x; t < n_items; t += gridDim.x) {
+        const BTile W = items[t];
+        const BQuery& q = Q[W.q];
+        const GroupPlan& P = q.P;
+        const int nc = P.ncells, nrw = q.nrw;
+        uint64_t* gtab = tabs + q.tab_off;
+        const bool in_lds = q.lds != 0;
+        if ((int)threadIdx.x < nc) {
+            const int k = threadIdx.x;
+            ckind[k] = (int8_t)P.kind[k];
+            cword[k] = (int8_t)q.cell_rw[k];
+            cf64[k] = k > 0 && q.T.col[q.rw_slot[q.cell_rw[k]]].type == TD_F64;
+        }
+        const int64_t ncell_total = q.n_slots * nc;
+        if (in_lds)
+            for (int64_t i = threadIdx.x; i < ncell_total; i += BH_JT)
+                lt[i] = cell_identity(P.kind[i % nc]);
+        const int64_t bs = pos[q.hb_off + W.r0], be = pos[q.hb_off + W.r0 + 1];
+        const int64_t ps = pos[q.hp_off + W.r0], pe = pos[q.hp_off + W.r0 + 1];
+        for (int64_t c0 = bs; c0 < be; c0 += BH_CAP) {       /* uniform */
+            const int cn = (int)(be - c0 < BH_CAP ? be - c0 : BH_CAP);
+            __syncthreads();
+            for (int i = threadIdx.x; i < BH_TS; i += BH_JT) hs[i] = BH_EMPTY;
+            __syncthreads();
+            for (int i = threadIdx.x; i < cn; i += BH_JT) {
+                const uint64_t k = R.bkey[c0 + i];
+                hk[i] = k;
+                hg[i] = R.bslot[c0 + i];
+                uint32_t s = bh_slot(k);
+                while (atomicCAS(&hs[s], BH_EMPTY, (uint32_t)i) != BH_EMPTY) s = (s + 1) & (BH_TS - 1);
+            }
+            __syncthreads();
+            for (int64_t i0 = ps + threadIdx.x; i0 < pe; i0 += (int64_t)BH_JT * BH_PU) {
+                uint64_t k[BH_PU], v[TL_NC - 1][BH_PU];
+                uint32_t s[BH_PU], live = 0;
+#pragma unroll
+                for (int u = 0; u < BH_PU; u++) {            /* every load of the step first */
+                    const int64_t i = i0 + (int64_t)u * BH_JT;
+                    const bool in = i < pe;
+                    k[u] = in ? R.pkey[i] : 0ull;
+#pragma unroll
+                    for (int w = 0; w < TL_NC - 1; w++) v[w][u] = in && w < nrw ? R.pval[w][i] : 0ull;
+                    live |= (uint32_t)in << u;
+                }
+#pragma unroll
+                for (int u = 0; u < BH_PU; u++) s[u] = bh_slot(k[u]);
+                while (live) {
+#pragma unroll
+                    for (int u = 0; u < BH_PU; u++) {
+                        if (!((live >> u) & 1)) continue;
+                        const uint32_t li = hs[s[u]];
+                        if (li == BH_EMPTY) { live &= ~(1u << u); continue; }
+                        if (hk[li] == k[u]) {
+                            if (in_lds) bh_fold(lt, nc, ckind, cword, cf64, hg[li], v[0][u], v[1][u], v[2][u]);
+                            else bh_fold(gtab, nc, ckind, cword, cf64, hg[li], v[0][u], v[1][u], v[2][u]);
+                        }
+                        s[u] = (s[u] + 1) & (BH_TS - 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (in_lds) {
+            bq_flush(P, lt, gtab, q.n_slots);
+            __syncthreads();
+        }
+    }
+}
+
 /* ---------------------------------------------------------------- emit -- */
 
 /* One workgroup per query: non-empty slots (count cell > 0). */
@@ -525,6 +808,20 @@ struct tdd_jg_batch {
     int grid_probe = 0, grid_prec = 0, block_probe = 512;
     size_t lds_probe = 0;
     int64_t tile_rows = 16384;
+    /* hash queries (build keys not a dense unique range) */
+    int n_hash = 0;
+    void* tl_hb = nullptr;    int64_t n_hb = 0;       /* build tiles */
+    void* tl_hp = nullptr;    int64_t n_hp = 0;       /* probe tiles */
+    void* tl_hj = nullptr;    int64_t n_hj = 0;       /* join items: (query, partition) */
+    void* h_mem = nullptr;                            /* build and probe records */
+    BHRec hrec;
+    void* h_cnt = nullptr;                            /* u32 rows per (query, side, partition) */
+    void* h_pos = nullptr;                            /* i64 record positions (+ end) */
+    void* h_cur = nullptr;                            /* u64 cursors */
+    int64_t h_ent = 0;
+    int64_t h_target = 0;                             /* build rows per partition */
+    size_t lds_join = 0;
+    int grid_join = 0;
     GroupPlan shape;                        /* output types of the shape */
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -533,7 +830,8 @@ struct tdd_jg_batch {
 static void bq_release(tdd_jg_batch* b) {
     void* ps[] = {b->dq, b->maps, b->tabs, b->occ, b->ng, b->offs, b->tl_build, b->tl_map,
                   b->tl_tab, b->tl_probe, b->tl_prec, b->tl_bsc, b->tl_bfold, b->rec_mem, b->qcur,
-                  b->sl_cnt, b->sl_pos, b->sl_cur};
+                  b->sl_cnt, b->sl_pos, b->sl_cur, b->tl_hb, b->tl_hp, b->tl_hj, b->h_mem,
+                  b->h_cnt, b->h_pos, b->h_cur};
     for (void* p : ps) if (p) tdd::release(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -593,6 +891,112 @@ static int bq_launch(tdd_jg_batch* b) {
                (const BTile*)b->tl_bfold, b->n_bfold, (const int64_t*)b->sl_pos, b->recB,
                (uint64_t*)b->tabs);
     }
+    if (b->n_hj) {
+        uint32_t* cnt = (uint32_t*)b->h_cnt;
+        rc = tdd_fill(b->h_cnt, 0, (size_t)b->h_ent * 4);
+        if (rc) return rc;
+        if (b->n_hb)
+            LAUNCH("batch_hash_part", k_bh_hist, stream_grid(b->n_hb, 8), BH_T, 0, Q,
+                   (const BTile*)b->tl_hb, b->n_hb, 0, cnt);
+        if (b->n_hp)
+            LAUNCH("batch_hash_part", k_bh_hist, stream_grid(b->n_hp, 8), BH_T, 0, Q,
+                   (const BTile*)b->tl_hp, b->n_hp, 1, cnt);
+        LAUNCH("batch_hash_part", k_bh_scan, 2 * b->n, 256, 0, Q, (const uint32_t*)cnt,
+               (int64_t*)b->h_pos, (unsigned long long*)b->h_cur);
+        if (b->n_hb)
+            LAUNCH("batch_hash_part", k_bh_scatter, stream_grid(b->n_hb, 8), BH_T, 0, Q,
+                   (const BTile*)b->tl_hb, b->n_hb, 0, (unsigned long long*)b->h_cur, b->hrec);
+        if (b->n_hp)
+            LAUNCH("batch_hash_part", k_bh_scatter, stream_grid(b->n_hp, 8), BH_T, 0, Q,
+                   (const BTile*)b->tl_hp, b->n_hp, 1, (unsigned long long*)b->h_cur, b->hrec);
+        LAUNCH("batch_hash_join", k_bh_join, b->grid_join, BH_JT, b->lds_join, Q,
+               (const BTile*)b->tl_hj, b->n_hj, (const int64_t*)b->h_pos, b->hrec,
+               (uint64_t*)b->tabs);
+    }
+    return TD_OK;
+}
+
+/* Record words of query q: the distinct tile slots its cells read; false
+ * when they exceed the record's TL_NC - 1 words. */
+static bool bq_record_words(BQuery& q) {
+    const GroupPlan& P = q.P;
+    q.nrw = 0;
+    for (int c = 1; c < P.ncells; c++) {
+        const int sl = q.cell_slot[c];
+        int w = -1;
+        for (int k = 0; k < q.nrw; k++) if (q.rw_slot[k] == sl) w = k;
+        if (w < 0) {
+            if (q.nrw >= TL_NC - 1) { q.nrw = 0; return false; }
+            w = q.nrw++;
+            q.rw_slot[w] = (uint8_t)sl;
+        }
+        q.cell_rw[c] = (uint8_t)w;
+    }
+    return true;
+}
+
+/* Radix plan of the hash queries for a target of build rows per partition:
+ * each query's bits, partition entries, record regions, the work lists;
+ * the query descriptors are uploaded again. */
+static int bh_plan(tdd_jg_batch* b, int64_t target) {
+    void* old[] = {b->tl_hb, b->tl_hp, b->tl_hj, b->h_mem, b->h_cnt, b->h_pos, b->h_cur};
+    for (void* p : old) if (p) tdd::release(p);
+    b->tl_hb = b->tl_hp = b->tl_hj = b->h_mem = b->h_cnt = b->h_pos = b->h_cur = nullptr;
+    b->n_hb = b->n_hp = b->n_hj = 0;
+    b->h_target = target;
+    b->lds_join = 0;
+    int64_t ent = 0, nbr = 0, npr = 0;
+    int nrw_max = 0;
+    std::vector<BTile> hb, hp, hj;
+    for (int i = 0; i < b->n; i++) {
+        BQuery& q = b->hq[i];
+        if (b->status[i] || !q.hash) continue;
+        int bits = 0;
+        while (bits < BH_MAXB && (target << bits) < q.nr) bits++;
+        q.hbits = bits;
+        q.hb_off = ent;
+        ent += ((int64_t)1 << bits) + 1;
+        q.hp_off = ent;
+        ent += ((int64_t)1 << bits) + 1;
+        q.hbr_off = nbr;
+        nbr += q.nr;
+        q.hpr_off = npr;
+        npr += q.nl;
+        if (q.nrw > nrw_max) nrw_max = q.nrw;
+        const size_t tb = (size_t)q.n_slots * q.P.ncells * 8;
+        q.lds = tb <= BH_LDS_TAB;
+        if (q.lds && tb > b->lds_join) b->lds_join = tb;
+        for (int64_t r = 0; r < q.nr; r += BH_ST)
+            hb.push_back(BTile{i, 0, r, r + BH_ST < q.nr ? r + BH_ST : q.nr});
+        for (int64_t r = 0; r < q.nl; r += BH_ST)
+            hp.push_back(BTile{i, 0, r, r + BH_ST < q.nl ? r + BH_ST : q.nl});
+        for (int p = 0; p < (1 << bits); p++) hj.push_back(BTile{i, 0, p, p + 1});
+    }
+    b->h_ent = (ent + 1) & ~(int64_t)1;
+    const size_t per_p = 8 + 8 * (size_t)nrw_max;
+    b->h_mem = tdd::scratch((size_t)nbr * 12 + (size_t)npr * per_p + 2048);
+    b->h_cnt = tdd::scratch((size_t)b->h_ent * 4);
+    b->h_pos = tdd::scratch((size_t)b->h_ent * 8);
+    b->h_cur = tdd::scratch((size_t)b->h_ent * 8);
+    if (!b->h_mem || !b->h_cnt || !b->h_pos || !b->h_cur) return TD_ERR_OOM;
+    uint8_t* m = (uint8_t*)b->h_mem;
+    auto carve = [&](size_t bytes) { uint8_t* r = m; m += (bytes + 255) & ~(size_t)255; return r; };
+    memset(&b->hrec, 0, sizeof(BHRec));
+    b->hrec.bkey = (uint64_t*)carve((size_t)nbr * 8);
+    b->hrec.bslot = (int32_t*)carve((size_t)nbr * 4);
+    b->hrec.pkey = (uint64_t*)carve((size_t)npr * 8);
+    for (int w = 0; w < nrw_max; w++) b->hrec.pval[w] = (uint64_t*)carve((size_t)npr * 8);
+    b->n_hb = (int64_t)hb.size();
+    b->n_hp = (int64_t)hp.size();
+    b->n_hj = (int64_t)hj.size();
+    int rc = upload(&b->tl_hb, hb.data(), hb.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_hp, hp.data(), hp.size() * sizeof(BTile));
+    if (!rc) rc = upload(&b->tl_hj, hj.data(), hj.size() * sizeof(BTile));
+    if (rc) return rc;
+    HC(hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * b->n, hipMemcpyHostToDevice,
+                      tdd::g_stream));
+    HC(hipStreamSynchronize(tdd::g_stream));
+    b->grid_join = resident_grid(k_bh_join, BH_JT, b->lds_join, b->n_hj);
     return TD_OK;
 }
 
@@ -609,20 +1013,13 @@ static int bq_bucket_plan(tdd_jg_batch* b) {
         q.nsl = q.sg = 0;
         const GroupPlan& P = q.P;
         const int64_t tbytes = q.n_slots * P.ncells * 8;
-        if (b->status[i] || tbytes <= (int64_t)TUNE_INT("TEIDE_BQ_BUCKET_KB", 48) * 1024) continue;
+        if (b->status[i] || q.hash ||
+            tbytes <= (int64_t)TUNE_INT("TEIDE_BQ_BUCKET_KB", 48) * 1024)
+            continue;
         const int64_t sg = BQ_LDS_MAX / (8 * (int64_t)P.ncells);
         const int64_t nsl = ceil_div(q.n_slots, sg);
         if (sg < 1 || nsl > BQ_MAX_SL) continue;
-        /* record words: the tile slots the cells read */
-        q.nrw = 0;
-        for (int c = 1; c < P.ncells; c++) {
-            const int sl = q.cell_slot[c];
-            int w = -1;
-            for (int k = 0; k < q.nrw; k++) if (q.rw_slot[k] == sl) w = k;
-            if (w < 0) { w = q.nrw++; q.rw_slot[w] = (uint8_t)sl; }
-            q.cell_rw[c] = (uint8_t)w;
-        }
-        if (q.nrw > TL_NC - 1) { q.nrw = 0; continue; }
+        if (!bq_record_words(q)) continue;            /* record words: the tile slots the cells read */
         q.nsl = (int32_t)nsl;
         q.sg = (int32_t)sg;
         q.rec_off = rec_total;
@@ -675,14 +1072,23 @@ static void bq_grids(tdd_jg_batch* b) {
     b->grid_prec = resident_grid(k_bq_precord, b->block_probe, (size_t)b->nsl_max * 4, b->n_prec);
 }
 
-/* Autotuned probe launch shape, per size class (log2 of the probe rows). */
-static std::map<int, std::pair<int64_t, int>> g_bq_tuned;
+/* Autotuned launch shape per size class (log2 of the probe rows, and
+ * whether the batch has hash queries): probe tile rows, block size, and
+ * the hash queries' build rows per partition (their radix bits). */
+struct BQTuned {
+    int64_t chunk;
+    int     block;
+    int64_t target;
+};
+static std::map<int, BQTuned> g_bq_tuned;
+
+#define BH_TARGET0 ((int64_t)BH_CAP * 5 / 8)   /* default: 0.3 of an LDS table's slots */
 
 static int bq_tiles_for(tdd_jg_batch* b, int64_t chunk) {
     if (b->tl_probe) { tdd::release(b->tl_probe); b->tl_probe = nullptr; }
     if (b->tl_prec) { tdd::release(b->tl_prec); b->tl_prec = nullptr; }
     std::vector<BTile> tp = make_tiles(b->n, chunk, [&](int q) {
-        return b->status[q] || b->hq[q].nsl ? (int64_t)0 : b->hq[q].nl;
+        return b->status[q] || b->hq[q].nsl || b->hq[q].hash ? (int64_t)0 : b->hq[q].nl;
     });
     std::vector<BTile> tr = make_tiles(b->n, chunk, [&](int q) {
         return b->status[q] || !b->hq[q].nsl ? (int64_t)0 : b->hq[q].nl;
@@ -698,42 +1104,58 @@ static int bq_tiles_for(tdd_jg_batch* b, int64_t chunk) {
 static int bq_autotune(tdd_jg_batch* b, int64_t total_rows) {
     int cls = 0;
     while ((int64_t)1 << (cls + 1) <= total_rows) cls++;
+    cls = cls * 2 + (b->n_hash > 0);
     auto it = g_bq_tuned.find(cls);
     if (it != g_bq_tuned.end()) {
-        b->block_probe = it->second.second;
-        return bq_tiles_for(b, it->second.first);
+        b->block_probe = it->second.block;
+        int rc = b->n_hash && it->second.target != b->h_target ? bh_plan(b, it->second.target)
+                                                                : TD_OK;
+        return rc ? rc : bq_tiles_for(b, it->second.chunk);
     }
     const int64_t chunks[] = {4096, 16384, 65536};
     const int blocks[] = {256, 512};
+    /* radix bits of the hash queries: partitions of 0.3, 0.15 or 0.075 of a
+     * table (fewer, larger builds against more, smaller ones) */
+    const int64_t targets[] = {BH_TARGET0, BH_TARGET0 / 2, BH_TARGET0 / 4};
+    const int n_targets = b->n_hash ? 3 : 1;
     float best = 1e30f;
-    int64_t best_c = 16384;
-    int best_b = 512;
+    BQTuned bt = {16384, 512, b->h_target};
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
-    for (int64_t c : chunks) {
-        int rc = bq_tiles_for(b, c);
-        if (rc) return rc;
-        for (int bs : blocks) {
-            b->block_probe = bs;
-            bq_grids(b);
-            rc = bq_launch(b);                     /* warm + reset tables */
+    for (int ti = 0; ti < n_targets; ti++) {
+        if (b->n_hash && b->h_target != targets[ti]) {
+            int rc = bh_plan(b, targets[ti]);
             if (rc) return rc;
-            HC(hipEventRecord(e0, tdd::g_stream));
-            rc = bq_launch(b);
+        }
+        for (int64_t c : chunks) {
+            int rc = bq_tiles_for(b, c);
             if (rc) return rc;
-            HC(hipEventRecord(e1, tdd::g_stream));
-            HC(hipEventSynchronize(e1));
-            float ms = 0;
-            HC(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; best_c = c; best_b = bs; }
+            for (int bs : blocks) {
+                b->block_probe = bs;
+                bq_grids(b);
+                rc = bq_launch(b);                     /* warm + reset tables */
+                if (rc) return rc;
+                HC(hipEventRecord(e0, tdd::g_stream));
+                rc = bq_launch(b);
+                if (rc) return rc;
+                HC(hipEventRecord(e1, tdd::g_stream));
+                HC(hipEventSynchronize(e1));
+                float ms = 0;
+                HC(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; bt = {c, bs, b->h_target}; }
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_bq_tuned[cls] = {best_c, best_b};
-    b->block_probe = best_b;
-    return bq_tiles_for(b, best_c);
+    g_bq_tuned[cls] = bt;
+    b->block_probe = bt.block;
+    if (b->n_hash && bt.target != b->h_target) {
+        int rc = bh_plan(b, bt.target);
+        if (rc) return rc;
+    }
+    return bq_tiles_for(b, bt.chunk);
 }
 
 extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, int n,
@@ -755,8 +1177,7 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
         memset(&q, 0, sizeof(q));
         int st = TD_OK;
         bool ok = J.join_type == 0 && J.n_keys == 1 && !J.lsel && J.lkeys[0].type != TD_F64 &&
-                  J.rkeys[0].type != TD_F64 && J.rkeys[0].ranged && S->n_keys >= 1 &&
-                  S->n_aggs <= TDD_MAX_AGGS;
+                  J.rkeys[0].type != TD_F64 && S->n_keys >= 1 && S->n_aggs <= TDD_MAX_AGGS;
         GroupPlan& P = q.P;
         if (ok) {
             P.n_keys = S->n_keys;
@@ -784,10 +1205,15 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
             int64_t st1 = 1;
             for (int k = P.n_keys - 1; k >= 0; k--) { P.kstride[k] = st1; st1 *= P.krange[k]; }
             q.n_slots = (int64_t)slots;
-            q.kmin = J.rkeys[0].lo;
-            q.range = J.rkeys[0].hi - J.rkeys[0].lo + 1;
-            ok = slots <= (double)(1 << 24) && q.range > 0 && q.range <= 4 * J.nr + 4096 &&
-                 q.range < ((int64_t)1 << 31);
+            ok = slots <= (double)(1 << 24);
+            /* a dense build key range takes the slot map, any other the hash join */
+            q.kmin = J.rkeys[0].ranged ? J.rkeys[0].lo : 0;
+            q.range = J.rkeys[0].ranged ? J.rkeys[0].hi - J.rkeys[0].lo + 1 : 0;
+            if (!J.rkeys[0].ranged || q.range <= 0 || q.range > 4 * J.nr + 4096 ||
+                q.range >= ((int64_t)1 << 31)) {
+                q.hash = 1;
+                q.kmin = q.range = 0;
+            }
         }
         {
             if (ok) ok = star_plan(J.lkeys[0], S->lpred, q.T, q.SP);
@@ -799,6 +1225,7 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
                 else agg_slot[a] = c;
             }
             for (int c = 1; ok && c < P.ncells; c++) q.cell_slot[c] = (uint8_t)agg_slot[P.col[c]];
+            if (ok && q.hash) ok = bq_record_words(q);
         }
         if (ok && shape_q >= 0) {                   /* one output shape per batch */
             const GroupPlan& P0 = b->hq[shape_q].P;
@@ -823,7 +1250,8 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
         tab_cells += (size_t)q.n_slots * P.ncells;
         const size_t tb = (size_t)q.n_slots * P.ncells * 8;
         q.lds = tb <= BQ_LDS_MAX;
-        if (q.lds && tb > lds_max) lds_max = tb;
+        if (q.lds && !q.hash && tb > lds_max) lds_max = tb;
+        b->n_hash += q.hash;
         total_rows += q.nl;
     }
     if (shape_q < 0) { delete b; return TD_ERR_NYI; }   /* nothing to batch */
@@ -838,7 +1266,9 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
     b->ng = tdd::scratch((size_t)n * 8);
     b->offs = tdd::scratch((size_t)(n + 1) * 8);
     if (!b->maps || !b->tabs || !b->occ || !b->ng || !b->offs) rc = TD_ERR_OOM;
-    std::vector<BTile> tb_ = make_tiles(n, 16384, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].nr; });
+    std::vector<BTile> tb_ = make_tiles(n, 16384, [&](int q) {
+        return b->status[q] || b->hq[q].hash ? (int64_t)0 : b->hq[q].nr;
+    });
     std::vector<BTile> tm_ = make_tiles(n, 65536, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].range; });
     std::vector<BTile> tt_ = make_tiles(n, 65536, [&](int q) {
         return b->status[q] ? (int64_t)0 : b->hq[q].n_slots * b->hq[q].P.ncells;
@@ -867,10 +1297,26 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
         HC(hipMemcpyAsync(occ.data(), b->occ, (size_t)n * 8, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
         bool any_dup = false;
-        for (int i = 0; i < n; i++)
-            if (!b->status[i] && (int64_t)occ[i] != b->hq[i].nr) { b->status[i] = TD_ERR_NYI; any_dup = true; }
-        if (any_dup) {                               /* drop them from the work lists */
-            std::vector<BTile> a2 = make_tiles(n, 16384, [&](int q) { return b->status[q] ? (int64_t)0 : b->hq[q].nr; });
+        for (int i = 0; i < n; i++) {
+            BQuery& q = b->hq[i];
+            if (b->status[i] || q.hash || (int64_t)occ[i] == q.nr) continue;
+            /* repeated build keys: the hash join */
+            any_dup = true;
+            q.hash = 1;
+            q.kmin = q.range = 0;
+            if (bq_record_words(q)) b->n_hash++;
+            else b->status[i] = TD_ERR_NYI;
+        }
+        if (any_dup) {                               /* drop them from the map work lists */
+            b->lds_probe = 0;
+            for (int i = 0; i < n; i++) {
+                const BQuery& q = b->hq[i];
+                const size_t tb = (size_t)q.n_slots * q.P.ncells * 8;
+                if (!b->status[i] && !q.hash && q.lds && tb > b->lds_probe) b->lds_probe = tb;
+            }
+            std::vector<BTile> a2 = make_tiles(n, 16384, [&](int q) {
+                return b->status[q] || b->hq[q].hash ? (int64_t)0 : b->hq[q].nr;
+            });
             std::vector<BTile> c2 = make_tiles(n, 65536, [&](int q) {
                 return b->status[q] ? (int64_t)0 : b->hq[q].n_slots * b->hq[q].P.ncells;
             });
@@ -900,6 +1346,7 @@ extern "C" int tdd_jg_batch_prepare(const tdd_join_group_spec_t* const* specs, i
         b->n_tab = (int64_t)c3.size();
         if (!rc) rc = upload(&b->tl_tab, c3.data(), c3.size() * sizeof(BTile));
     }
+    if (!rc && b->n_hash) rc = bh_plan(b, BH_TARGET0);
     if (!rc && tune) rc = bq_autotune(b, total_rows);
     if (!rc) bq_grids(b);
     if (rc) { bq_release(b); delete b; return rc; }
@@ -913,6 +1360,15 @@ extern "C" int tdd_jg_batch_shape(const tdd_jg_batch_t* b, int64_t* tile_rows, i
     *tile_rows = b->tile_rows;
     *block = b->block_probe;
     return TD_OK;
+}
+
+extern "C" int tdd_jg_batch_radix(const tdd_jg_batch_t* b, int64_t* part_rows, int* max_bits) {
+    int mb = 0;
+    for (int i = 0; i < b->n; i++)
+        if (!b->status[i] && b->hq[i].hash && b->hq[i].hbits > mb) mb = b->hq[i].hbits;
+    *part_rows = b->n_hash ? b->h_target : 0;
+    *max_bits = mb;
+    return b->n_hash;
 }
 
 /* One run: replays the captured graph after the first run.  Outputs: one

@@ -2191,6 +2191,13 @@ int64_t td_dev_batch_info(const td_dev_batch_t* b, int64_t* tile_rows, int* bloc
     return c;
 }
 
+int64_t td_dev_batch_radix(const td_dev_batch_t* b, int64_t* part_rows, int* max_bits) {
+    *part_rows = 0;
+    *max_bits = 0;
+    if (!b || !b->dev) return 0;
+    return tdd_jg_batch_radix(b->dev, part_rows, max_bits);
+}
+
 static td_err_t batch_run(td_dev_batch_t* b, int use_graph, td_t** results) {
     td_err_t err = TD_OK;
     for (int64_t i = 0; i < b->n; i++) results[i] = NULL;

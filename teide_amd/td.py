@@ -147,6 +147,7 @@ class Engine:
                 "td_dev_batch_prepare": (P, [P, P, I64]),
                 "td_dev_batch_run": (C.c_int, [P, P]),
                 "td_dev_batch_info": (I64, [P, P, P]),
+                "td_dev_batch_radix": (I64, [P, P, P]),
                 "td_dev_batch_free": (None, [P]),
                 "td_dev_col_load": (P, [C.c_char_p]),
                 "td_dev_splay_load": (P, [C.c_char_p, C.c_char_p]),
@@ -300,6 +301,12 @@ class Engine:
         tr, bl = C.c_int64(), C.c_int()
         nb = self.L.td_dev_batch_info(h, C.byref(tr), C.byref(bl))
         return nb, tr.value, bl.value
+
+    def batch_radix(self, h):
+        """(queries on the radix hash path, build rows per partition, max bits)"""
+        pr, mb = C.c_int64(), C.c_int()
+        nh = self.L.td_dev_batch_radix(h, C.byref(pr), C.byref(mb))
+        return nh, pr.value, mb.value
 
     # -------------------------------------------------------------- serving
     def serve_open(self, fact, dim, fk, pk, pcol, vals, keys, budget=0, chunk_rows=0,

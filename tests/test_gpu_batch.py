@@ -205,3 +205,67 @@ def test_batch_full_size_sample(eng):
         g.free()
     for t in tables:
         eng.release(t)
+
+
+def make_hash_query(eng, rng, nf, nd, ncat, thr, kind):
+    """A query whose build keys are not a dense unique range: `scattered`
+    unique 64-bit keys, `dup` keys repeated about twice, `hot` a handful of
+    keys shared by every dim row (build partitions beyond one LDS table)."""
+    if kind == "scattered":
+        pk = (rng.permutation(nd).astype(np.int64) * 7_000_000_019) - (1 << 62)
+    elif kind == "dup":
+        pk = rng.integers(0, max(1, nd // 2), nd) * 1_000_003 + 11
+    else:
+        pk = rng.integers(0, 3, nd) * (1 << 40) - 5
+    fk = np.concatenate([pk, rng.integers(-(1 << 62), 1 << 62, max(1, nd // 10))])
+    fk = fk[rng.integers(0, len(fk), nf)]                     # most probe keys hit, some miss
+    qty = rng.integers(1, 101, nf)
+    price = rng.lognormal(3, 1, nf)
+    cat = rng.integers(0, ncat, nd).astype(np.int32)
+    hf = eng.table([("fk", fk), ("qty", qty), ("price", price)])
+    hd = eng.table([("pk", pk), ("cat", cat)])
+    tf, td = eng.put(hf), eng.put(hd)
+    eng.release(hf)
+    eng.release(hd)
+    HOST[tf] = dict(fk=fk, qty=qty, price=price, pk=pk, cat=cat)
+    return tf, td, thr
+
+
+def test_batch_hash_queries(eng):
+    """Queries with scattered, repeated and hot build keys join in the batch
+    through its radix-partitioned hash path (no query runs alone), mixed
+    with slot-map queries, over a capture and two replays; every query is
+    checked against the oracle, MIN / MAX / AVG included."""
+    rng = np.random.default_rng(31)
+    qs = []
+    for i in range(24):
+        kind = ("scattered", "dup", "hot", "dense")[i % 4]
+        nf = int(np.exp(rng.uniform(np.log(500), np.log(300000))))
+        nd = max(1, nf // 8) if kind != "hot" else int(rng.integers(3000, 9000))
+        ncat = int(np.exp(rng.uniform(np.log(4), np.log(20000))))
+        thr = int(rng.integers(1, 100))
+        if kind == "dense":
+            qs.append(make_query(eng, rng, nf, nd, ncat, thr))
+        elif kind == "hot":
+            # ~nd/3 partners per probe row: a few thousand rows per group keep
+            # the f64 sums inside the 1e-12 relative tolerance
+            qs.append(make_hash_query(eng, rng, 3000, nd, max(ncat, 500), thr, kind))
+        else:
+            qs.append(make_hash_query(eng, rng, nf, nd, ncat, thr, kind))
+    aggs = [("SUM", "price"), ("MIN", "price"), ("MAX", "qty"), ("AVG", "qty"),
+            ("COUNT", "price")]
+    dags = [build_dag(eng, *q, aggs) for q in qs]
+    h = eng.batch_prepare([g for g, _ in dags], [r for _, r in dags])
+    nb, _, _ = eng.batch_info(h)
+    assert nb == len(qs)                             # every query in the batch
+    nh, part_rows, max_bits = eng.batch_radix(h)
+    assert nh == 18 and part_rows > 0 and max_bits >= 1   # the non-dense ones: radix path
+    for rep in range(3):
+        rc, got = eng.batch_run(h, len(qs))
+        assert rc == 0
+        for i, a in enumerate(got):
+            check_oracle(eng, a, HOST[qs[i][0]], qs[i][2], f"run {rep} query {i}", aggs)
+            eng.release(a)
+    eng.L.td_dev_batch_free(h)
+    for g, _ in dags:
+        g.free()
