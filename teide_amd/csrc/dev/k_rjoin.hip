@@ -41,7 +41,9 @@
 #define RJ_RPT   8               /* rows per thread per step */
 #define RJ_STEP  (RJ_T * RJ_RPT)
 #define RJ_SLOTS 8192            /* LDS hash slots per partition table */
-#define RJ_CAP   4096            /* build rows per partition table */
+#define RJ_CAP   4000            /* build rows per partition table: 16 KB of slots + 16 B a row,
+                                    so two build + probe workgroups fit one CU's 160 KB of LDS */
+#define RJ_PART_ROWS 2457       /* expected build rows per partition: 0.3 of the slots */
 #define RJ_EMPTY 0xffffffffu
 #define RJ_MAXB  14              /* radix bits */
 
@@ -334,18 +336,18 @@ struct RJBP {
 #define RJ_PB 4                  /* probe records per thread in flight */
 #define RJ_BT 1024               /* threads per build + probe workgroup */
 
-/* Payload of the build rows hit[] into a right output column at the probe
+/* Payload of the build rows bs + hl[] into a right output column at the probe
  * records i0 + j * RJ_T (zero for a LEFT row without a partner). */
 template <class T>
 __device__ __forceinline__ void rj_fetch(const void* src, void* dst, int64_t i0, int64_t pe,
-                                         const int64_t (&hit)[RJ_PB], bool left_join) {
+                                         int64_t bs, const int32_t (&hl)[RJ_PB], bool left_join) {
     T v[RJ_PB];
 #pragma unroll
-    for (int j = 0; j < RJ_PB; j++) v[j] = hit[j] >= 0 ? ((const T*)src)[hit[j]] : (T)0;
+    for (int j = 0; j < RJ_PB; j++) v[j] = hl[j] >= 0 ? ((const T*)src)[bs + hl[j]] : (T)0;
 #pragma unroll
     for (int j = 0; j < RJ_PB; j++) {
         const int64_t i = i0 + j * RJ_BT;
-        if (i < pe && (hit[j] >= 0 || left_join)) ((T*)dst)[i] = v[j];
+        if (i < pe && (hl[j] >= 0 || left_join)) ((T*)dst)[i] = v[j];
     }
 }
 
@@ -385,12 +387,11 @@ __device__ __forceinline__ uint32_t rj_slot_claim(uint32_t* tab, uint32_t s, uin
 }
 
 template <int PLAIN>
-__global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
+__global__ __launch_bounds__(RJ_BT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_rj_bp(RJBP A) {
     extern __shared__ uint8_t sm[];
     uint32_t* slot = (uint32_t*)sm;                                 /* RJ_SLOTS / 2 words */
     uint64_t* tkey = (uint64_t*)(sm + RJ_SLOTS * 2);                /* RJ_CAP */
     uint64_t* pay = (uint64_t*)(sm + RJ_SLOTS * 2 + RJ_CAP * 8);    /* RJ_CAP: right column 0 */
-    __shared__ uint32_t wcnt[RJ_BT / WAVE];
     const int p = blockIdx.x;
     const int64_t bs = A.roff[p], be = A.roff[p + 1];
     const int64_t ps = A.loff[p], pe = A.loff[p + 1];
@@ -423,7 +424,6 @@ __global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
     uint32_t hits = 0;
     for (int64_t i0 = ps + threadIdx.x; i0 < pe; i0 += RJ_PB * RJ_BT) {
         uint64_t k[RJ_PB];
-        int64_t hit[RJ_PB];
         int32_t hl[RJ_PB];
 #pragma unroll
         for (int j = 0; j < RJ_PB; j++) {           /* every key load first */
@@ -439,7 +439,6 @@ __global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
         uint32_t sl[RJ_PB], live = 0;
 #pragma unroll
         for (int j = 0; j < RJ_PB; j++) {
-            hit[j] = -1;
             hl[j] = -1;
             sl[j] = rj_slot(k[j]);
             if (i0 + j * RJ_BT < pe) live |= 1u << j;
@@ -455,7 +454,6 @@ __global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
                 if (tkey[li[j]] == k[j] &&
                     (A.exact || keys_equal(A.LK, A.lrow[i0 + j * RJ_BT], A.RK,
                                            A.rrow[bs + li[j]]))) {
-                    hit[j] = bs + li[j];
                     hl[j] = (int32_t)li[j];
                     live &= ~(1u << j);
                     continue;
@@ -464,7 +462,7 @@ __global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
             }
         }
 #pragma unroll
-        for (int j = 0; j < RJ_PB; j++) hits += hit[j] >= 0;
+        for (int j = 0; j < RJ_PB; j++) hits += hl[j] >= 0;
         switch (e0) {                                       /* uniform across the block */
         case 8: rj_put_lds<uint64_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
         case 4: rj_put_lds<uint32_t>(A.rdst[0], i0, pe, hl, pay, A.left_join); break;
@@ -474,25 +472,143 @@ __global__ __launch_bounds__(RJ_BT) void k_rj_bp(RJBP A) {
         }
         for (int c = 1; c < A.nrc; c++) {                   /* uniform across the block */
             switch (A.resz[c]) {
-            case 8: rj_fetch<uint64_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
-            case 4: rj_fetch<uint32_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
-            case 2: rj_fetch<uint16_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
-            default: rj_fetch<uint8_t>(A.rsrc[c], A.rdst[c], i0, pe, hit, A.left_join); break;
+            case 8: rj_fetch<uint64_t>(A.rsrc[c], A.rdst[c], i0, pe, bs, hl, A.left_join); break;
+            case 4: rj_fetch<uint32_t>(A.rsrc[c], A.rdst[c], i0, pe, bs, hl, A.left_join); break;
+            case 2: rj_fetch<uint16_t>(A.rsrc[c], A.rdst[c], i0, pe, bs, hl, A.left_join); break;
+            default: rj_fetch<uint8_t>(A.rsrc[c], A.rdst[c], i0, pe, bs, hl, A.left_join); break;
             }
         }
         if (A.lflag) {
 #pragma unroll
             for (int j = 0; j < RJ_PB; j++)
-                if (i0 + j * RJ_BT < pe) A.lflag[i0 + j * RJ_BT] = hit[j] >= 0;
+                if (i0 + j * RJ_BT < pe) A.lflag[i0 + j * RJ_BT] = hl[j] >= 0;
         }
     }
     for (int d = WAVE / 2; d >= 1; d >>= 1) hits += __shfl_xor(hits, d);
-    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = hits;
+    /* one atomic per workgroup (thousands of workgroups share the counter):
+     * the wave counts go through the slot table's words once every wave is
+     * past its probes */
+    __syncthreads();
+    if (lane_id() == 0) slot[threadIdx.x / WAVE] = hits;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int w = 0; w < RJ_BT / WAVE; w++) t += wcnt[w];
+        for (int w = 0; w < RJ_BT / WAVE; w++) t += slot[w];
         if (t) atomicAdd(&A.stat[0], t);
+    }
+}
+
+/* Plan of one side's scatter passes from its counts per (final partition,
+ * XCD), on the device so that no count travels to the host between the
+ * passes: the final partitions' offsets, the pass-1 run cursors (ordered
+ * pass-1 partition, then XCD), the pass-2 cursors and its tile list (the
+ * pass-1 partitions of XCD x are those p1 = x mod 8, each cut into tiles of
+ * RJ_STEP rows), and the side's row total.  One workgroup per side. */
+#define RJP_T 1024
+
+struct RJPlan {
+    const uint32_t*     C;       /* C[q * RJ_XCD + x] */
+    int32_t             bits, b1, b2, pad;
+    int64_t*            off;     /* Q + 1 */
+    unsigned long long* cur1;    /* P1 * RJ_XCD */
+    unsigned long long* cur2;    /* Q (b2 > 0) */
+    int64_t*            tiles;   /* (start, end) pairs (b2 > 0) */
+    int32_t*            xtiles;  /* RJ_XCD + 1 (b2 > 0) */
+    unsigned long long* total;   /* the side's rows */
+};
+
+/* Exclusive scan of one value per thread over the workgroup; *tot gets
+ * the sum.  Every thread calls it. */
+__device__ __forceinline__ int64_t rj_block_scan(int64_t v, int64_t* ws, int64_t* tot) {
+    int64_t incl = v;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t y = __shfl_up(incl, d, WAVE);
+        if (lane_id() >= d) incl += y;
+    }
+    if (lane_id() == WAVE - 1) ws[threadIdx.x / WAVE] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t a = 0;
+        for (int w = 0; w < RJP_T / WAVE; w++) {
+            const int64_t t = ws[w];
+            ws[w] = a;
+            a += t;
+        }
+        ws[RJP_T / WAVE] = a;
+    }
+    __syncthreads();
+    const int64_t ex = ws[threadIdx.x / WAVE] + incl - v;
+    *tot = ws[RJP_T / WAVE];
+    __syncthreads();
+    return ex;
+}
+
+struct RJPlans {
+    RJPlan side[2];
+};
+
+__global__ __launch_bounds__(RJP_T) void k_rj_plan(RJPlans PS) {
+    const RJPlan& A = PS.side[blockIdx.x];
+    __shared__ int64_t ws[RJP_T / WAVE + 1];
+    __shared__ int64_t p1s[RJ_FMAX + 1];         /* pass-1 partition starts */
+    __shared__ int64_t tb[RJ_FMAX];              /* first pass-2 tile of each pass-1 partition */
+    const int Q = 1 << A.bits, P1 = 1 << A.b1, sub = 1 << A.b2;
+    /* final partitions: per consecutive q of this thread */
+    const int per = (Q + RJP_T - 1) / RJP_T, q0 = (int)threadIdx.x * per;
+    const int q1 = q0 + per < Q ? q0 + per : Q;
+    int64_t s = 0;
+    for (int q = q0; q < q1; q++)
+        for (int x = 0; x < RJ_XCD; x++) s += A.C[(int64_t)q * RJ_XCD + x];
+    int64_t tot;
+    int64_t a = rj_block_scan(s, ws, &tot);
+    for (int q = q0; q < q1; q++) {
+        A.off[q] = a;
+        if (A.b2) A.cur2[q] = (unsigned long long)a;
+        for (int x = 0; x < RJ_XCD; x++) a += A.C[(int64_t)q * RJ_XCD + x];
+    }
+    if (threadIdx.x == 0) {
+        A.off[Q] = tot;
+        *A.total = (unsigned long long)tot;
+    }
+    /* pass-1 runs e = p1 * RJ_XCD + x, in that order */
+    const int E = P1 * RJ_XCD;
+    const int pe = (E + RJP_T - 1) / RJP_T, e0 = (int)threadIdx.x * pe;
+    const int e1 = e0 + pe < E ? e0 + pe : E;
+    s = 0;
+    for (int e = e0; e < e1; e++)
+        for (int q = (e / RJ_XCD) * sub; q < (e / RJ_XCD + 1) * sub; q++)
+            s += A.C[(int64_t)q * RJ_XCD + e % RJ_XCD];
+    a = rj_block_scan(s, ws, &tot);
+    for (int e = e0; e < e1; e++) {
+        A.cur1[e] = (unsigned long long)a;
+        if (e % RJ_XCD == 0) p1s[e / RJ_XCD] = a;
+        for (int q = (e / RJ_XCD) * sub; q < (e / RJ_XCD + 1) * sub; q++)
+            a += A.C[(int64_t)q * RJ_XCD + e % RJ_XCD];
+    }
+    if (threadIdx.x == 0) p1s[P1] = tot;
+    __syncthreads();
+    if (!A.b2) return;
+    /* pass-2 tiles: XCD by XCD, its pass-1 partitions in order, each in
+     * RJ_STEP-row tiles */
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int x = 0; x < RJ_XCD; x++) {
+            A.xtiles[x] = (int32_t)t;
+            for (int p1 = x; p1 < P1; p1 += RJ_XCD) {
+                tb[p1] = t;
+                t += (p1s[p1 + 1] - p1s[p1] + RJ_STEP - 1) / RJ_STEP;
+            }
+        }
+        A.xtiles[RJ_XCD] = (int32_t)t;
+    }
+    __syncthreads();
+    for (int p1 = threadIdx.x; p1 < P1; p1 += RJP_T) {
+        const int64_t b = p1s[p1], e = p1s[p1 + 1];
+        int64_t t = tb[p1];
+        for (int64_t r = b; r < e; r += RJ_STEP, t++) {
+            A.tiles[2 * t] = r;
+            A.tiles[2 * t + 1] = r + RJ_STEP < e ? r + RJ_STEP : e;
+        }
     }
 }
 
@@ -543,8 +659,6 @@ struct RJHostSide {
     bool            carry_kw = false, carry_row = false;
     int64_t         span = 0;
     int32_t         nspan = 0;
-    std::vector<uint32_t> C;             /* counts per (partition, XCD) */
-    int64_t         total = 0;
     /* partitioned results */
     void*           dst[RJ_MAXC] = {};
     uint64_t*       kw = nullptr;
@@ -578,58 +692,33 @@ static int rj_hist(RJHostSide& S, int bits, Scratch& C) {
     return TD_OK;
 }
 
-/* The scatter passes of one side after its counts came back: run offsets,
- * buffers, then one pass (bits <= 8) or two (b1 top bits, then b2). */
-static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch& offs,
-                        Scratch& cur1, Scratch& cur2, Scratch& tiles, Scratch& xt,
-                        Scratch& mid, Scratch& kw, Scratch& kwm, Scratch& row, Scratch& rowm) {
-    const int Q = 1 << bits;
+/* Device plan of one side's passes from its counts C; the side's row
+ * total lands in *total on the device. */
+static RJPlan rj_plan_of(int bits, const Scratch& C, Scratch& offs, Scratch& cur1, Scratch& cur2,
+                         Scratch& tiles, Scratch& xt, unsigned long long* total) {
+    RJPlan PL;
+    memset(&PL, 0, sizeof(PL));
+    PL.C = C.as<uint32_t>();
+    PL.bits = bits;
+    PL.b1 = bits <= 8 ? bits : (bits + 1) / 2;
+    PL.b2 = bits - PL.b1;
+    PL.off = offs.as<int64_t>();
+    PL.cur1 = cur1.as<unsigned long long>();
+    PL.cur2 = cur2.as<unsigned long long>();
+    PL.tiles = tiles.as<int64_t>();
+    PL.xtiles = xt.as<int32_t>();
+    PL.total = total;
+    return PL;
+}
+
+/* The scatter passes of one side after its plan: one pass (bits <= 8) or
+ * two (b1 top bits, then b2).  Buffers hold S.n rows (every selected row
+ * at most). */
+static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch& cur1,
+                        Scratch& cur2, Scratch& tiles, Scratch& xt, Scratch& mid, Scratch& kw,
+                        Scratch& kwm, Scratch& row, Scratch& rowm) {
     const int b1 = bits <= 8 ? bits : (bits + 1) / 2, b2 = bits - b1;
-    const int P1 = 1 << b1;
-    /* final partition offsets and pass-1 run cursors */
-    std::vector<int64_t> off((size_t)Q + 1), c1((size_t)P1 * RJ_XCD), c2;
-    int64_t acc = 0;
-    for (int q = 0; q < Q; q++) {
-        off[(size_t)q] = acc;
-        for (int x = 0; x < RJ_XCD; x++) acc += S.C[(size_t)q * RJ_XCD + x];
-    }
-    off[(size_t)Q] = acc;
-    S.total = acc;
-    acc = 0;
-    std::vector<int64_t> p1start((size_t)P1 + 1);
-    for (int p1 = 0; p1 < P1; p1++) {
-        p1start[(size_t)p1] = acc;
-        for (int x = 0; x < RJ_XCD; x++) {
-            c1[(size_t)p1 * RJ_XCD + x] = acc;
-            for (int q = p1 << b2; q < (p1 + 1) << b2; q++) acc += S.C[(size_t)q * RJ_XCD + x];
-        }
-    }
-    p1start[(size_t)P1] = acc;
-    HC(hipMemcpyAsync(offs.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, tdd::g_stream));
-    HC(hipMemcpyAsync(cur1.p, c1.data(), c1.size() * 8, hipMemcpyHostToDevice, tdd::g_stream));
-    const int64_t n = S.total > 0 ? S.total : 1;
-    /* pass-2 tiles: the pass-1 partitions of XCD x are those p1 = x mod 8 */
-    std::vector<int64_t> tl;
-    std::vector<int32_t> xtiles(RJ_XCD + 1, 0);
-    if (b2) {
-        for (int x = 0; x < RJ_XCD; x++) {
-            xtiles[(size_t)x] = (int32_t)(tl.size() / 2);
-            for (int p1 = x; p1 < P1; p1 += RJ_XCD)
-                for (int64_t t = p1start[(size_t)p1]; t < p1start[(size_t)p1 + 1]; t += RJ_STEP) {
-                    tl.push_back(t);
-                    tl.push_back(t + RJ_STEP < p1start[(size_t)p1 + 1] ? t + RJ_STEP
-                                                                       : p1start[(size_t)p1 + 1]);
-                }
-        }
-        xtiles[RJ_XCD] = (int32_t)(tl.size() / 2);
-        HC(hipMemcpyAsync(cur2.p, off.data(), (size_t)Q * 8, hipMemcpyHostToDevice,
-                          tdd::g_stream));
-        if (!tl.empty())
-            HC(hipMemcpyAsync(tiles.p, tl.data(), tl.size() * 8, hipMemcpyHostToDevice,
-                              tdd::g_stream));
-        HC(hipMemcpyAsync(xt.p, xtiles.data(), xtiles.size() * 4, hipMemcpyHostToDevice,
-                          tdd::g_stream));
-    }
+    const int64_t n = S.n > 0 ? S.n : 1;
     /* buffers: final, and the pass-1 copies when there is a second pass */
     size_t mid_off[RJ_MAXC] = {};
     if (b2) {
@@ -694,13 +783,13 @@ static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch
     B.row_src = S.carry_row ? rowm.as<uint32_t>() : nullptr;
     B.row_dst = S.carry_row ? S.row : nullptr;
     B.cur = cur2.as<unsigned long long>();
-    int maxt = 0;
-    for (int x = 0; x < RJ_XCD; x++)
-        if (xtiles[(size_t)x + 1] - xtiles[(size_t)x] > maxt) maxt = xtiles[(size_t)x + 1] - xtiles[(size_t)x];
-    int per_xcd = tdd::g_num_cu / RJ_XCD * TUNE_INT("TEIDE_RJ_WG_PER_CU", 2);
+    /* the tile list is on the device: workgroups to fill each XCD, capped
+     * by the most tiles an XCD can have (every row of the side on it) */
+    int64_t per_xcd = tdd::g_num_cu / RJ_XCD * TUNE_INT("TEIDE_RJ_WG_PER_CU", 2);
+    const int64_t maxt = ceil_div(n, RJ_STEP) + ((1 << b1) + RJ_XCD - 1) / RJ_XCD;
     if (per_xcd > maxt) per_xcd = maxt;
     if (per_xcd < 1) per_xcd = 1;
-    LAUNCH("rjoin_scatter", k_rj_scatter, per_xcd * RJ_XCD, RJ_T, 0, B);
+    LAUNCH("rjoin_scatter", k_rj_scatter, (int)per_xcd * RJ_XCD, RJ_T, 0, B);
     return TD_OK;
 }
 
@@ -936,7 +1025,7 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
     int bits = J->radix_bits;
     if (bits <= 0) {
         int bb = 0, bp = 0;
-        while (bb < RJ_MAXB && ((int64_t)RJ_CAP * 6 / 10 << bb) < J->nr) bb++;
+        while (bb < RJ_MAXB && ((int64_t)RJ_PART_ROWS << bb) < J->nr) bb++;
         while (bp < 10 && ((int64_t)32768 << bp) < J->nl) bp++;
         bits = bb > bp ? bb : bp;
     }
@@ -972,20 +1061,18 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
     rj_spans(L);
     rj_spans(R);
 
-    /* counts of both sides, read back together */
+    /* counts of both sides; the plans, passes and build + probe follow on
+     * the device, and the host reads the row totals with the match count at
+     * the end.  Buffers are sized for every selected row. */
     SCRATCH(lc, (size_t)P * RJ_XCD * 4);
     SCRATCH(rcn, (size_t)P * RJ_XCD * 4);
-    int rc = rj_hist(L, bits, lc);
+    SCRATCH(stat, 64);          /* matched, repeated key, oversized, left rows, right rows */
+    int rc = tdd_fill(stat.p, 0, 64);
+    if (!rc) rc = rj_hist(L, bits, lc);
     if (!rc) rc = rj_hist(R, bits, rcn);
     if (rc) return rc;
-    L.C.resize((size_t)P * RJ_XCD);
-    R.C.resize((size_t)P * RJ_XCD);
-    HC(hipMemcpyAsync(L.C.data(), lc.p, L.C.size() * 4, hipMemcpyDeviceToHost, tdd::g_stream));
-    HC(hipMemcpyAsync(R.C.data(), rcn.p, R.C.size() * 4, hipMemcpyDeviceToHost, tdd::g_stream));
-    HC(hipStreamSynchronize(tdd::g_stream));
-    int64_t nlp = 0, nrp = 0;
-    for (uint32_t v : L.C) nlp += v;
-    for (uint32_t v : R.C) nrp += v;
+    unsigned long long* st_dev = stat.as<unsigned long long>();
+    const int64_t nlp = L.n, nrp = R.n;         /* bounds; the totals come back with the stat */
     POLL_CANCEL();
     void* ldst[RJ_MAXC] = {};
     void* rbuf[RJ_MAXC] = {};
@@ -1008,10 +1095,12 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
     SCRATCH(lxt, (RJ_XCD + 1) * 4);
     SCRATCH(rxt, (RJ_XCD + 1) * 4);
     Scratch lmid(8), rmid(8), lkw(8), lkwm(8), rkw(8), rkwm(8), lrow(8), lrowm(8), rrow(8), rrowm(8);
-    rc = rj_partition(L, bits, ldst, loff, lcur1, lcur2, ltiles, lxt, lmid, lkw, lkwm, lrow, lrowm);
-    if (!rc)
-        rc = rj_partition(R, bits, rbuf, roff, rcur1, rcur2, rtiles, rxt, rmid, rkw, rkwm, rrow,
-                          rrowm);
+    RJPlans PS;
+    PS.side[0] = rj_plan_of(bits, lc, loff, lcur1, lcur2, ltiles, lxt, st_dev + 3);
+    PS.side[1] = rj_plan_of(bits, rcn, roff, rcur1, rcur2, rtiles, rxt, st_dev + 4);
+    LAUNCH("rjoin_plan", k_rj_plan, 2, RJP_T, 0, PS);
+    rc = rj_partition(L, bits, ldst, lcur1, lcur2, ltiles, lxt, lmid, lkw, lkwm, lrow, lrowm);
+    if (!rc) rc = rj_partition(R, bits, rbuf, rcur1, rcur2, rtiles, rxt, rmid, rkw, rkwm, rrow, rrowm);
     if (rc) return rc;
     POLL_CANCEL();
 
@@ -1042,28 +1131,26 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
     }
     SCRATCH(flag, (size_t)(J->join_type == 0 ? nlp : 1));
     if (J->join_type == 0) A.lflag = flag.as<uint8_t>();
-    SCRATCH(stat, 32);
-    rc = tdd_fill(stat.p, 0, 32);
-    if (rc) return rc;
-    A.stat = stat.as<unsigned long long>();
+    A.stat = st_dev;
     const size_t lds = (size_t)RJ_SLOTS * 2 + (size_t)RJ_CAP * 16;
     if (plain == 8) LAUNCH("rjoin_build_probe", k_rj_bp<8>, P, RJ_BT, lds, A);
     else if (plain == 4) LAUNCH("rjoin_build_probe", k_rj_bp<4>, P, RJ_BT, lds, A);
     else LAUNCH("rjoin_build_probe", k_rj_bp<0>, P, RJ_BT, lds, A);
-    unsigned long long st[3];
+    unsigned long long st[5];
     HC(hipMemcpyAsync(st, stat.p, sizeof(st), hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
     if (st[1] || st[2]) return TD_ERR_NYI;      /* repeated build keys / oversized partition */
+    const int64_t nl_rows = (int64_t)st[3];     /* probe rows taking part */
 
-    int64_t nout = J->join_type == 1 ? nlp : (int64_t)st[0];
+    int64_t nout = J->join_type == 1 ? nl_rows : (int64_t)st[0];
     void* lo[RJ_MAXC] = {};
     void* ro[RJ_MAXC] = {};
     for (int c = 0; c < L.ncols; c++) lo[c] = L.dst[c];
     for (int c = 0; c < R.ncols; c++) ro[c] = A.rdst[c];
-    if (J->join_type == 0 && nout < nlp) {
+    if (J->join_type == 0 && nout < nl_rows) {
         /* inner join with probe rows that found no partner: compact the
          * slotted outputs through the match flags */
-        const int64_t nseg = ceil_div(nlp, 1024), nw = ceil_div(nlp, 64);
+        const int64_t nseg = ceil_div(nl_rows, 1024), nw = ceil_div(nl_rows, 64);
         SCRATCH(selmem, 16 + (size_t)((nseg + 7) & ~7) + (size_t)((2 * nseg + 7) & ~7) +
                             (size_t)nw * 8);
         tdd_sel_t sv;
@@ -1071,7 +1158,7 @@ extern "C" int tdd_join_mat(const tdd_join_mat_spec_t* S, tdd_join_mat_result_t*
         sv.flags = (uint8_t*)selmem.p + 16;
         sv.popcnt = (uint16_t*)((uint8_t*)sv.flags + ((nseg + 7) & ~7));
         sv.bits = (uint64_t*)((uint8_t*)sv.popcnt + ((2 * nseg + 7) & ~7));
-        sv.nrows = nlp;
+        sv.nrows = nl_rows;
         sv.n_segs = (uint32_t)nseg;
         sv.pad = 0;
         rc = tdd_sel_from_bool(flag.as<uint8_t>(), &sv);
