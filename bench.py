@@ -999,6 +999,20 @@ def two_call_summary(e, wl, steps=5, warmup=2):
             "fused_vs_two_call": None, "roofline": m["roofline"], "kernels": m["kernels"]}
 
 
+def scattered_summary(e, a, steps=5, warmup=2):
+    """The join + group-by with its join keys spread over 64-bit values (no
+    dense slot map): the radix-partitioned path, with its own roofline."""
+    wl = JoinGroupScattered(e, a)
+    m = measure(e, wl, steps, warmup)
+    out = {"workload": wl.config["workload"], "ms_per_query": round(m["ms"], 4),
+           "rows_per_s": wl.rows / (m["ms"] / 1e3),
+           "query_gbs": round(wl.query_bytes / (m["ms"] / 1e3) / 1e9, 1),
+           "query_frac_hbm": round(wl.query_bytes / (m["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+           "roofline": m["roofline"], "plan": m["plan"], "kernels": m["kernels"]}
+    wl.release()
+    return out
+
+
 def hc_summary(e, a, steps=5, warmup=2, cpu=True):
     """The high-cardinality group-by (300 M rows, by key and by sym) with
     its own roofline and CPU baseline."""
@@ -1198,6 +1212,7 @@ def main():
         line["two_call"] = tc
     wl.release()
     if a.workload == "jg" and not a.no_forms:
+        line["scattered"] = run_form(lambda: scattered_summary(e, a))
         line["hc"] = run_form(lambda: hc_summary(e, a, cpu=not a.no_cpu_baseline))
         line["batched"] = run_form(lambda: batched_summary(e, a))
         line["batched_hash"] = run_form(lambda: batched_summary(e, a, scattered=True,

@@ -11,17 +11,10 @@
  * partition's slot range has a table that fits LDS; every partition is then
  * folded in LDS and merged into the global table once per workgroup.
  *
- * A pass is the counting scatter of a radix sort: a job (a range of items
- * of one parent partition, handled by one workgroup) first counts its items
- * per sub-partition (k_pg_hist), an exclusive scan orders those counts
- * parent by parent, sub-partition by sub-partition, job by job, and the
- * scatter (k_pg_pass) writes each job's items at its offsets -- ranks within
- * a wave from ballots, running cursors in LDS, no global atomics -- so
- * records keep their row order inside each partition.  Each round of 2,048
- * items is placed in LDS in sub-partition order first and then written out
- * by consecutive threads as contiguous runs: a wave's store touches a few
- * cache lines instead of up to 64 (measured on MI355X, 300 M rows in 256
- * sub-partitions: 6.2 -> 4.75 ms per pass).  A record is the
+ * A pass is the counting scatter of a radix sort over tiles of items
+ * (k_px_hist counts, an exclusive scan places, k_px_pass scatters; see
+ * the pass section below for the runs per XCD that keep its writes whole
+ * cache lines).  A record is the
  * group slot (32 bits) plus one 64-bit word per aggregate input (the column
  * value widened as the group-by reads it, or the row index for FIRST /
  * LAST).  The leaf fold (k_pg_agg) is the group-by's own cell fold, with
@@ -35,7 +28,6 @@
 
 #define PG_T     512                      /* threads per workgroup */
 #define PG_W     (PG_T / WAVE)
-#define PG_R     4                        /* items per lane per scatter round */
 #define PG_MAXB  8                        /* bits per pass: up to 256 sub-partitions */
 #define PG_MAXP  (1 << PG_MAXB)
 #define PG_MAXV  4                        /* value words per record */
@@ -104,160 +96,199 @@ __device__ __forceinline__ uint64_t pg_bits(uint64_t key) {
     return HK ? mix64(key) : key;
 }
 
-/* Counts of one job's items per sub-partition, written where the exclusive
- * scan turns them into write offsets: parent block (js * PL), then
- * sub-partition s (s * nj), then job (j - js). */
+/* A pass over tiles of PX_TILE consecutive items, each tile within one
+ * parent partition.  Pass 1 (one parent) appends each sub-partition's items
+ * to one run per XCD: tile t is handled by a workgroup on XCD t mod 8 (the
+ * grid is a multiple of 8 and workgroup b runs on XCD b mod 8), so the open
+ * end of every run -- a cache line per sub-partition and record array,
+ * ~100 KB per XCD for 256 sub-partitions -- stays in that XCD's L2 until it
+ * is whole, instead of one open run per workgroup and sub-partition.  Later
+ * passes take every tile of a parent on one XCD (the tile list is laid out
+ * so) and keep one run per (parent, sub-partition).  A histogram pass counts
+ * the items of every run; an exclusive scan of the counts, ordered parent,
+ * sub-partition, run, gives the runs' starts, so each partition is
+ * contiguous.  The scatter counting-sorts a tile in LDS, reserves each of
+ * its sub-partitions' runs with one atomic, and stages the records through
+ * LDS one array at a time so consecutive lanes write consecutive
+ * addresses of a run. */
+#define PX_R    8                         /* items per lane per tile */
+#define PX_TILE (PG_T * PX_R)
+#define PX_XCD  8
+
+struct PXTile {
+    int64_t r0, r1;                       /* items [r0, r1) */
+    int64_t parent;
+};
+
+template <bool HK>
+__device__ __forceinline__ uint32_t px_digit(uint64_t key, int shift, int PL) {
+    return (uint32_t)(pg_bits<HK>(key) >> shift) & (uint32_t)(PL - 1);
+}
+
+/* Value word q of item i: a record word, or the row's aggregate input (its
+ * index for FIRST / LAST); KE != 0: the value columns are plain 8-byte. */
+template <bool ROWS, int KE>
+__device__ __forceinline__ uint64_t px_val(const PGRows& S, const PGBuf& in, int q, int64_t i) {
+    if (!ROWS) return in.val[q][i];
+    if (S.vsrc[q] < 0) return (uint64_t)i;
+    const tdd_col_t& c = S.P.aggs[S.vsrc[q]];
+    return KE != 0 ? (uint64_t)((const int64_t*)c.data)[i] : (uint64_t)ld_i64(c, i);
+}
+
+/* Items per run: C[(parent * PL + s) * XR + run]. */
 template <bool ROWS, bool HK, int KE = 0>
-__global__ __launch_bounds__(PG_T) void k_pg_hist(PGRows S, PGBuf in, const PGJob* jobs, int njobs,
-                                                  int shift, int PL, uint32_t* Q) {
+__global__ __launch_bounds__(PG_T) void k_px_hist(PGRows S, PGBuf in, const PXTile* tiles,
+                                                  int ntiles, int shift, int PL, int XR,
+                                                  uint32_t* C) {
     __shared__ uint32_t h[PG_MAXP];
-    for (int j = blockIdx.x; j < njobs; j += gridDim.x) {
-        const PGJob J = jobs[j];
-        for (int s = threadIdx.x; s < PL; s += blockDim.x) h[s] = 0;
-        __syncthreads();
-        uint64_t v[PG_MAXV];
-        const int lane = lane_id();
-        /* every lane of a wave runs the same trip count (i advances by the
-         * block), so the wave can count a run of one sub-partition at once */
-        for (int64_t i0 = J.r0; i0 < J.r1; i0 += blockDim.x) {
-            const int64_t i = i0 + threadIdx.x;
-            uint64_t key = 0;
-            const bool live = pg_item<ROWS, HK, 0, KE>(S, in, i, J.r1, key, v);
-            const int s = live ? (int)((pg_bits<HK>(key) >> shift) & (PL - 1)) : -1;
-            const uint64_t lm = __ballot(live);
+    const int lane = lane_id();
+    for (int s = threadIdx.x; s < PL; s += PG_T) h[s] = 0;
+    __syncthreads();
+    int64_t par = -1;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PXTile T = tiles[t];                        /* uniform across the block */
+        if (T.parent != par) {
+            if (par >= 0) {                               /* later passes: flush a parent */
+                __syncthreads();
+                for (int s = threadIdx.x; s < PL; s += PG_T)
+                    if (h[s]) { atomicAdd(&C[((int64_t)par * PL + s) * XR], h[s]); h[s] = 0; }
+                __syncthreads();
+            }
+            par = T.parent;
+        }
+        uint64_t key[PX_R], vd[PG_MAXV];
+        bool live[PX_R];
+#pragma unroll
+        for (int k = 0; k < PX_R; k++)                    /* every load of the tile first */
+            live[k] = pg_item<ROWS, HK, 0, KE>(S, in, T.r0 + (int64_t)k * PG_T + threadIdx.x,
+                                               T.r1, key[k], vd);
+#pragma unroll
+        for (int k = 0; k < PX_R; k++) {
+            const int s = live[k] ? (int)px_digit<HK>(key[k], shift, PL) : -1;
+            const uint64_t lm = __ballot(live[k]);
             if (!lm) continue;
+            /* a wave whose items all go to one sub-partition counts them at once */
             const int first = __ffsll((unsigned long long)lm) - 1;
             const int s0 = __shfl(s, first);
-            if (__ballot(live && s != s0) == 0) {
+            if (__ballot(live[k] && s != s0) == 0) {
                 if (lane == first) atomicAdd(&h[s0], (uint32_t)__popcll(lm));
-            } else if (live) {
+            } else if (live[k]) {
                 atomicAdd(&h[s], 1u);
             }
         }
-        __syncthreads();
-        for (int s = threadIdx.x; s < PL; s += blockDim.x)
-            Q[(int64_t)J.js * PL + (int64_t)s * J.nj + (j - J.js)] = h[s];
-        __syncthreads();
     }
+    __syncthreads();
+    const int run = XR > 1 ? (int)(blockIdx.x % PX_XCD) : 0;
+    if (par >= 0)
+        for (int s = threadIdx.x; s < PL; s += PG_T)
+            if (h[s]) atomicAdd(&C[((int64_t)par * PL + s) * XR + run], h[s]);
 }
 
-/* The scatter of one pass. */
-template <bool ROWS, bool HK, int NV, int R, int KE = 0>
-__global__ __launch_bounds__(PG_T) void k_pg_pass(PGRows S, PGBuf in, PGBuf out, const PGJob* jobs,
-                                                  int njobs, int shift, int PL, const int64_t* O,
+/* The scatter of one pass: runs' cursors in `cur` (the scanned counts). */
+template <bool ROWS, bool HK, int NV, int KE = 0>
+__global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
+                                                  const PXTile* tiles, int ntiles, int shift,
+                                                  int PL, int XR, unsigned long long* cur,
                                                   int64_t nout) {
-    __shared__ int64_t cur[PG_MAXP];
-    __shared__ int64_t dbase[PG_MAXP];        /* round: global position of LDS record 0 of s */
-    __shared__ uint32_t tot[PG_MAXP];
-    __shared__ uint32_t rbase[PG_MAXP];       /* round: first LDS record of sub-partition s */
-    __shared__ uint32_t wcnt[PG_W][PG_MAXP];
-    __shared__ uint32_t ntot;
-    /* the round's records in sub-partition order, written out as runs */
-    __shared__ typename std::conditional<HK, uint64_t, uint32_t>::type skey[(PG_T * R)];
-    __shared__ uint64_t sval[NV > 0 ? NV : 1][(PG_T * R)];
-    __shared__ uint8_t ssub[(PG_T * R)];
-    const int lane = lane_id(), wv = threadIdx.x / WAVE;
-    for (int j = blockIdx.x; j < njobs; j += gridDim.x) {
-        const PGJob J = jobs[j];
-        for (int s = threadIdx.x; s < PL; s += blockDim.x)
-            cur[s] = O[(int64_t)J.js * PL + (int64_t)s * J.nj + (j - J.js)];
-        for (int64_t b = J.r0; b < J.r1; b += (PG_T * R)) {
-            uint64_t slot[R];
-            int32_t sub[R], rk[R];
-            uint64_t v[R][PG_MAXV];
+    __shared__ uint32_t hist[PG_MAXP], loff[PG_MAXP];
+    __shared__ unsigned long long gbase[PG_MAXP];
+    __shared__ uint8_t spart[PX_TILE];
+    __shared__ uint64_t stage[PX_TILE];
+    const int lane = lane_id();
+    const int run = XR > 1 ? (int)(blockIdx.x % PX_XCD) : 0;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PXTile T = tiles[t];                        /* uniform across the block */
+        for (int s = threadIdx.x; s < PL; s += PG_T) hist[s] = 0;
+        __syncthreads();
+        uint64_t key[PX_R], vd[PG_MAXV];
+        uint32_t dig[PX_R], at[PX_R], live = 0;
 #pragma unroll
-            for (int k = 0; k < R; k++) {
-                const int64_t i = b + (int64_t)(wv * R + k) * 64 + lane;
-                const bool live = pg_item<ROWS, HK, NV, KE>(S, in, i, J.r1, slot[k], v[k]);
-                sub[k] = live ? (int32_t)((pg_bits<HK>(slot[k]) >> shift) & (PL - 1)) : -1;
-            }
-            /* ranks within the wave, per sub-partition: the wave's own row of
-             * LDS counters (its LDS operations complete in program order) */
-            for (int s2 = lane; s2 < PL; s2 += WAVE) wcnt[wv][s2] = 0;   /* PL may exceed 64 */
+        for (int k = 0; k < PX_R; k++)
+            if (pg_item<ROWS, HK, 0, KE>(S, in, T.r0 + (int64_t)k * PG_T + threadIdx.x, T.r1,
+                                         key[k], vd))
+                live |= 1u << k;
 #pragma unroll
-            for (int k = 0; k < R; k++) {
-                /* a wave whose items all go to one sub-partition takes its
-                 * ranks with one counter update */
-                const uint64_t lm = __ballot(sub[k] >= 0);
-                const int first = lm ? __ffsll((unsigned long long)lm) - 1 : 0;
-                const int s0 = __shfl(sub[k], first);
-                if (lm && __ballot(sub[k] >= 0 && sub[k] != s0) == 0) {
-                    uint32_t base = 0;
-                    if (lane == first) base = atomicAdd(&wcnt[wv][s0], (uint32_t)__popcll(lm));
-                    base = __shfl(base, first);
-                    rk[k] = sub[k] >= 0 ? (int32_t)(base + lanes_below(lm)) : 0;
-                } else {
-                    rk[k] = sub[k] >= 0 ? (int32_t)atomicAdd(&wcnt[wv][sub[k]], 1u) : 0;
-                }
-            }
-            __syncthreads();
-            /* wave offsets within the round, then the round's totals */
-            if (threadIdx.x < PL) {
-                const int s = threadIdx.x;
-                uint32_t run = 0;
-#pragma unroll
-                for (int w = 0; w < PG_W; w++) {
-                    const uint32_t c = wcnt[w][s];
-                    wcnt[w][s] = run;
-                    run += c;
-                }
-                tot[s] = run;
-            }
-            __syncthreads();
-            /* wave 0: sub-partition bases within the round (exclusive scan of
-             * the totals, 4 per lane), their global positions, next cursors */
-            if (wv == 0) {
-                uint32_t t4[PG_MAXP / WAVE], sum = 0;
-#pragma unroll
-                for (int q = 0; q < PG_MAXP / WAVE; q++) {
-                    const int s = lane * (PG_MAXP / WAVE) + q;
-                    t4[q] = s < PL ? tot[s] : 0u;
-                    sum += t4[q];
-                }
-                uint32_t inc = sum;
-#pragma unroll
-                for (int d = 1; d < WAVE; d <<= 1) {
-                    const uint32_t y = __shfl_up(inc, d);
-                    if (lane >= d) inc += y;
-                }
-                uint32_t ex = inc - sum;
-#pragma unroll
-                for (int q = 0; q < PG_MAXP / WAVE; q++) {
-                    const int s = lane * (PG_MAXP / WAVE) + q;
-                    if (s < PL) {
-                        rbase[s] = ex;
-                        dbase[s] = cur[s] - (int64_t)ex;
-                        cur[s] += t4[q];
-                    }
-                    ex += t4[q];
-                }
-                if (lane == WAVE - 1) ntot = inc;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                if (sub[k] < 0) continue;
-                const uint32_t lp = rbase[sub[k]] + wcnt[wv][sub[k]] + (uint32_t)rk[k];
-                skey[lp] = (typename std::conditional<HK, uint64_t, uint32_t>::type)slot[k];
-#pragma unroll
-                for (int q = 0; q < NV; q++) sval[q][lp] = v[k][q];
-                ssub[lp] = (uint8_t)sub[k];
-            }
-            __syncthreads();
-            /* consecutive threads write consecutive records of a run; the
-             * next round's first barrier orders these reads before the
-             * staging is rewritten */
-            const uint32_t n_r = ntot;
-            for (uint32_t t = threadIdx.x; t < n_r; t += PG_T) {
-                const int64_t at = dbase[ssub[t]] + (int64_t)t;
-                if (at < 0 || at >= nout) continue;          /* never taken when counts agree */
-                if (HK) out.key[at] = skey[t];
-                else out.slot[at] = (uint32_t)skey[t];
-#pragma unroll
-                for (int q = 0; q < NV; q++) out.val[q][at] = sval[q][t];
+        for (int k = 0; k < PX_R; k++) {
+            dig[k] = at[k] = 0;
+            if ((live >> k) & 1) {
+                dig[k] = px_digit<HK>(key[k], shift, PL);
+                at[k] = atomicAdd(&hist[dig[k]], 1u);
             }
         }
-        __syncthreads();                      /* cur of the next job */
+        __syncthreads();
+        if (threadIdx.x < WAVE) {      /* wave 0: tile offsets and run reservations, 4 a lane */
+            uint32_t c4[PG_MAXP / WAVE], sum = 0;
+#pragma unroll
+            for (int q = 0; q < PG_MAXP / WAVE; q++) {
+                const int s = lane * (PG_MAXP / WAVE) + q;
+                c4[q] = s < PL ? hist[s] : 0u;
+                sum += c4[q];
+            }
+            uint32_t inc = sum;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d);
+                if (lane >= d) inc += y;
+            }
+            uint32_t ex = inc - sum;
+#pragma unroll
+            for (int q = 0; q < PG_MAXP / WAVE; q++) {
+                const int s = lane * (PG_MAXP / WAVE) + q;
+                if (s < PL) {
+                    loff[s] = ex;
+                    gbase[s] = c4[q] ? atomicAdd(&cur[((int64_t)T.parent * PL + s) * XR + run],
+                                                 (unsigned long long)c4[q])
+                                     : 0ull;
+                }
+                ex += c4[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PX_R; k++)
+            if ((live >> k) & 1) {
+                at[k] += loff[dig[k]];                    /* sorted place in the tile */
+                spart[at[k]] = (uint8_t)dig[k];
+                stage[at[k]] = key[k];
+            }
+        __syncthreads();
+        const int cnt = (int)(loff[PL - 1] + hist[PL - 1]);
+        int64_t pos[PX_R];
+#pragma unroll
+        for (int k = 0; k < PX_R; k++) {
+            const int i = threadIdx.x + k * PG_T;
+            pos[k] = -1;
+            if (i < cnt) {
+                const int d = spart[i];
+                pos[k] = (int64_t)gbase[d] + (i - (int)loff[d]);
+                if (pos[k] >= nout) pos[k] = -1;          /* never taken when counts agree */
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PX_R; k++) {
+            if (pos[k] < 0) continue;
+            const uint64_t x = stage[threadIdx.x + k * PG_T];
+            if (HK) out.key[pos[k]] = x;
+            else out.slot[pos[k]] = (uint32_t)x;
+        }
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            uint64_t v[PX_R];
+#pragma unroll
+            for (int k = 0; k < PX_R; k++)                /* the word's loads first */
+                v[k] = (live >> k) & 1
+                           ? px_val<ROWS, KE>(S, in, q, T.r0 + (int64_t)k * PG_T + threadIdx.x)
+                           : 0ull;
+            __syncthreads();                              /* the previous array is out */
+#pragma unroll
+            for (int k = 0; k < PX_R; k++)
+                if ((live >> k) & 1) stage[at[k]] = v[k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PX_R; k++)
+                if (pos[k] >= 0) out.val[q][pos[k]] = stage[threadIdx.x + k * PG_T];
+        }
+        __syncthreads();                                  /* the tile's LDS is free */
     }
 }
 
@@ -610,52 +641,51 @@ static int pg_plain(const tdd_col_t& c) {
     }
 }
 
-template <bool HK>
-static int pg_hist_and_offsets(bool rows, const PGRows& S, const PGBuf& in, const PGJob* d_jobs,
-                               int njobs, int shift, int PL, uint32_t* Q, int64_t* O) {
-    const int grid = stream_grid(njobs, 4);
-    const int ke = rows && S.P.n_keys == 1 && S.P.keys[0].data ? pg_plain(S.P.keys[0]) : 0;
-    if (rows && ke == 8)
-        LAUNCH("group_pg_hist", (k_pg_hist<true, HK, 8>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
-    else if (rows && ke == 4)
-        LAUNCH("group_pg_hist", (k_pg_hist<true, HK, 4>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
-    else if (rows)
-        LAUNCH("group_pg_hist", (k_pg_hist<true, HK>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
-    else
-        LAUNCH("group_pg_hist", (k_pg_hist<false, HK>), grid, PG_T, 0, S, in, d_jobs, njobs, shift, PL, Q);
-    return tdd::scan_u32_i64(Q, (int64_t)njobs * PL, O);
-}
-
-
+/* Histogram and scatter of one pass over the tiles at d_tiles (grids a
+ * multiple of 8, so a tile's run is the XCD its workgroup runs on). */
 template <bool ROWS, bool HK>
-static int pg_pass_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
-                          const PGJob* d_jobs, int njobs, int shift, int PL, const int64_t* O,
-                          int64_t nout) {
-    const int grid = stream_grid(njobs, 2);
-    /* rows of one 4- or 8-byte key and 8-byte value columns: the plain-load form */
+static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
+                     const PXTile* d_tiles, int ntiles, int shift, int PL, int XR, uint32_t* C,
+                     int64_t* O, unsigned long long* cur, int64_t nrun, int64_t nout) {
+    /* rows of one 4- or 8-byte key and 8-byte value columns: plain loads */
     int ke = 0;
     if (ROWS && S.P.n_keys == 1 && S.P.keys[0].data) {
         ke = pg_plain(S.P.keys[0]);
         for (int k = 0; k < nv; k++)
             if (S.vsrc[k] >= 0 && pg_plain(S.P.aggs[S.vsrc[k]]) != 8) ke = 0;
     }
-#define PGP1(NV_, KE_) LAUNCH("group_pg_pass", (k_pg_pass<ROWS, HK, NV_, PG_R, KE_>), grid, PG_T, 0, \
-                              S, in, out, d_jobs, njobs, shift, PL, O, nout)
-#define PGP(NV_)                                      \
-    do {                                              \
-        if (ROWS && ke == 8) PGP1(NV_, (ROWS ? 8 : 0)); \
-        else if (ROWS && ke == 4) PGP1(NV_, (ROWS ? 4 : 0)); \
-        else PGP1(NV_, 0);                            \
+    const int hke = ROWS && S.P.n_keys == 1 && S.P.keys[0].data ? pg_plain(S.P.keys[0]) : 0;
+    auto grid8 = [&](int g) { g = g / PX_XCD * PX_XCD; return g < PX_XCD ? PX_XCD : g; };
+    const int gh = grid8(stream_grid(ntiles, 4));
+    int rc = tdd_fill(C, 0, (size_t)nrun * 4);
+    if (rc) return rc;
+#define PXH(KE_) LAUNCH("group_pg_hist", (k_px_hist<ROWS, HK, KE_>), gh, PG_T, 0, S, in, d_tiles, \
+                        ntiles, shift, PL, XR, C)
+    if (ROWS && hke == 8) PXH((ROWS ? 8 : 0));
+    else if (ROWS && hke == 4) PXH((ROWS ? 4 : 0));
+    else PXH(0);
+#undef PXH
+    rc = tdd::scan_u32_i64(C, nrun, O);
+    if (rc) return rc;
+    HC(hipMemcpyAsync(cur, O, (size_t)nrun * 8, hipMemcpyDeviceToDevice, tdd::g_stream));
+    const int gp = grid8(resident_grid(k_px_pass<ROWS, HK, 2, 0>, PG_T, 0, ntiles));
+#define PXP1(NV_, KE_) LAUNCH("group_pg_pass", (k_px_pass<ROWS, HK, NV_, KE_>), gp, PG_T, 0, S, in, \
+                              out, d_tiles, ntiles, shift, PL, XR, cur, nout)
+#define PXP(NV_)                                          \
+    do {                                                  \
+        if (ROWS && ke == 8) PXP1(NV_, (ROWS ? 8 : 0));   \
+        else if (ROWS && ke == 4) PXP1(NV_, (ROWS ? 4 : 0)); \
+        else PXP1(NV_, 0);                                \
     } while (0)
     switch (nv) {
-    case 0: PGP(0); break;
-    case 1: PGP(1); break;
-    case 2: PGP(2); break;
-    case 3: PGP(3); break;
-    default: PGP(4); break;
+    case 0: PXP(0); break;
+    case 1: PXP(1); break;
+    case 2: PXP(2); break;
+    case 3: PXP(3); break;
+    default: PXP(4); break;
     }
-#undef PGP
-#undef PGP1
+#undef PXP
+#undef PXP1
     return TD_OK;
 }
 
@@ -742,70 +772,67 @@ template <bool HK>
 static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, int npass, int top,
                       const PGBuf& A, const PGBuf& B, std::vector<PGJob>& jobs, PGBuf& res,
                       const std::vector<PGJob>* first = nullptr) {
-    jobs.clear();
-    if (first) {
-        jobs = *first;                   /* the first pass reads records of A */
-    } else {
-        const int nj0 = (int)ceil_div(nrows, PG_JOB);
-        for (int k = 0; k < nj0; k++) {
-            PGJob J;
-            J.r0 = (int64_t)k * PG_JOB;
-            J.r1 = J.r0 + PG_JOB < nrows ? J.r0 + PG_JOB : nrows;
-            J.parent = 0;
-            J.js = 0;
-            J.nj = nj0;
-            jobs.push_back(J);
-        }
-    }
-    PGBuf in = A, outb = B;          /* the first pass reads rows, or A's records */
+    /* pass-1 tiles: the rows, or the first pass's record ranges in A */
+    std::vector<PXTile> tl;
+    auto add = [&](std::vector<PXTile>& v, int64_t r0, int64_t r1, int64_t parent) {
+        for (int64_t r = r0; r < r1; r += PX_TILE)
+            v.push_back(PXTile{r, r + PX_TILE < r1 ? r + PX_TILE : r1, parent});
+    };
+    if (first) for (const PGJob& J : *first) add(tl, J.r0, J.r1, 0);
+    else add(tl, 0, nrows, 0);
+    PGBuf in = A, outb = B;              /* the first pass reads rows, or A's records */
     int shift = top;
     int64_t parents = 1;
+    std::vector<int64_t> O;
+    int XR = PX_XCD;
     for (int pass = 0; pass < npass; pass++) {
         POLL_CANCEL();
         const int PL = 1 << pbits[pass];
         shift -= pbits[pass];
-        const int njobs = (int)jobs.size();
-        SCRATCH(djobs, sizeof(PGJob) * (size_t)(njobs > 0 ? njobs : 1));
-        SCRATCH(Q, (size_t)(njobs > 0 ? njobs : 1) * PL * 4);
-        SCRATCH(O, ((size_t)njobs * PL + 1) * 8);
-        if (njobs > 0)
-            HC(hipMemcpyAsync(djobs.p, jobs.data(), sizeof(PGJob) * (size_t)njobs,
+        XR = pass == 0 ? PX_XCD : 1;
+        const int64_t nrun = parents * PL * XR;
+        const int ntiles = (int)tl.size();
+        SCRATCH(dtl, sizeof(PXTile) * (size_t)(ntiles > 0 ? ntiles : 1));
+        SCRATCH(C, (size_t)nrun * 4);
+        SCRATCH(Od, ((size_t)nrun + 1) * 8);
+        SCRATCH(cur, (size_t)nrun * 8);
+        if (ntiles > 0)
+            HC(hipMemcpyAsync(dtl.p, tl.data(), sizeof(PXTile) * (size_t)ntiles,
                               hipMemcpyHostToDevice, tdd::g_stream));
         const bool rows = pass == 0 && !first;
-        int rc = pg_hist_and_offsets<HK>(rows, S, in, djobs.as<PGJob>(), njobs, shift, PL,
-                                         Q.as<uint32_t>(), O.as<int64_t>());
+        int rc = rows ? px_launch<true, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, shift, PL,
+                                            XR, C.as<uint32_t>(), Od.as<int64_t>(),
+                                            cur.as<unsigned long long>(), nrun, nrows)
+                      : px_launch<false, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, shift, PL,
+                                             XR, C.as<uint32_t>(), Od.as<int64_t>(),
+                                             cur.as<unsigned long long>(), nrun, nrows);
         if (rc) return rc;
-        if (njobs > 0) {
-            rc = rows ? pg_pass_launch<true, HK>(nv, S, in, outb, djobs.as<PGJob>(), njobs, shift,
-                                                 PL, O.as<int64_t>(), nrows)
-                      : pg_pass_launch<false, HK>(nv, S, in, outb, djobs.as<PGJob>(), njobs, shift,
-                                                  PL, O.as<int64_t>(), nrows);
-            if (rc) return rc;
-        }
-        /* partition bounds: (parent, s) starts at the offset of the parent's
-         * first job in sub-partition s */
-        std::vector<int64_t> off((size_t)njobs * PL + 1);
-        HC(hipMemcpyAsync(off.data(), O.p, off.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
+        O.resize((size_t)nrun + 1);
+        HC(hipMemcpyAsync(O.data(), Od.p, O.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
         const int64_t nparts = parents * PL;
-        std::vector<int64_t> start((size_t)nparts, 0), count((size_t)nparts, 0);
-        std::vector<int64_t> pstart((size_t)parents, -1);
-        for (const PGJob& J : jobs)
-            if (pstart[(size_t)J.parent] < 0) pstart[(size_t)J.parent] = J.js;
-        for (int64_t p = 0; p < parents; p++) {
-            if (pstart[(size_t)p] < 0) continue;               /* empty parent */
-            const int js = (int)pstart[(size_t)p];
-            const int nj = jobs[(size_t)js].nj;
-            for (int s2 = 0; s2 < PL; s2++) {
-                const int64_t q0 = (int64_t)js * PL + (int64_t)s2 * nj;
-                start[(size_t)(p * PL + s2)] = off[(size_t)q0];
-                count[(size_t)(p * PL + s2)] = off[(size_t)(q0 + nj)] - off[(size_t)q0];
-            }
+        if (pass + 1 < npass) {
+            /* the next pass: the tiles of partition p on XCD p mod 8, tile t
+             * of the list on XCD t mod 8 (empty tiles pad the shorter lists) */
+            std::vector<PXTile> per[PX_XCD];
+            for (int64_t p = 0; p < nparts; p++)
+                add(per[p % PX_XCD], O[(size_t)(p * XR)], O[(size_t)((p + 1) * XR)], p);
+            size_t longest = 0;
+            for (int x = 0; x < PX_XCD; x++) longest = per[x].size() > longest ? per[x].size() : longest;
+            tl.assign(longest * PX_XCD, PXTile{0, 0, 0});
+            for (int x = 0; x < PX_XCD; x++)
+                for (size_t j = 0; j < per[x].size(); j++) tl[j * PX_XCD + (size_t)x] = per[x][j];
         }
-        pg_jobs_from(start, count, jobs);
         parents = nparts;
         std::swap(in, outb);
     }
+    /* leaf jobs: partition q holds [O[q * XR], O[(q + 1) * XR]) */
+    std::vector<int64_t> start((size_t)parents), count((size_t)parents);
+    for (int64_t q = 0; q < parents; q++) {
+        start[(size_t)q] = O[(size_t)(q * XR)];
+        count[(size_t)q] = O[(size_t)((q + 1) * XR)] - start[(size_t)q];
+    }
+    pg_jobs_from(start, count, jobs);
     res = in;
     return TD_OK;
 }
@@ -970,8 +997,10 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
     int lcap = 1;
     while ((int64_t)(lcap * 2) * per <= lds_bytes) lcap *= 2;
     if (lcap < 16) return TD_ERR_NYI;
+    /* parts of at most fill% of a table's entries (by the estimate) */
+    const double fill = TUNE_INT("TEIDE_PH_FILL_PCT", 50) / 100.0;
     int bits = 0;
-    while (bits < 24 && est / (double)((int64_t)1 << bits) > lcap / 2) bits++;
+    while (bits < 24 && est / (double)((int64_t)1 << bits) > lcap * fill) bits++;
     if (bits == 0 && forced) bits = 1;
     if (bits == 0) return TD_ERR_NYI;
     tdd_pht* h = new tdd_pht();
