@@ -773,7 +773,7 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
             if (hrc && !(hrc == TD_ERR_OOM && force == 0)) return hrc;
         }
         int prc = pht_partition(P, M.bits, M.nrows, est,
-                                force == 4 ? 4096 : TUNE_INT("TEIDE_PH_LDS_KB", 96) * 1024,
+                                force == 4 ? 4096 : TUNE_INT("TEIDE_PH_LDS_KB", 48) * 1024,
                                 force >= 4, hot ? hot_keys(hot) : nullptr,
                                 hot ? hot_count(hot) : 0, hot ? hot_cells(hot) : nullptr, &ph);
         /* no memory for the partitions: the global table alone needs less

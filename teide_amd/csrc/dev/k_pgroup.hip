@@ -433,7 +433,7 @@ struct PHList {                           /* groups of parts folded by a single 
 };
 
 template <int NV>
-__global__ __launch_bounds__(PG_T) void k_pg_hagg(GroupPlan P, PGBuf in, const PGJob* jobs,
+__global__ __launch_bounds__(PG_T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pg_hagg(GroupPlan P, PGBuf in, const PGJob* jobs,
                                                   int njobs, int lcap, const int8_t* cw_dev,
                                                   const uint8_t* vf64_dev,
                                                   unsigned long long* gkeys, uint64_t gmask,
@@ -997,8 +997,11 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
     int lcap = 1;
     while ((int64_t)(lcap * 2) * per <= lds_bytes) lcap *= 2;
     if (lcap < 16) return TD_ERR_NYI;
-    /* parts of at most fill% of a table's entries (by the estimate) */
-    const double fill = TUNE_INT("TEIDE_PH_FILL_PCT", 50) / 100.0;
+    /* parts of at most fill% of a table's entries (by the estimate): 512
+     * entries at 60 % (~300 keys a part) leave room for four leaf
+     * workgroups per CU, which hide the per-part latency (table set-up,
+     * the part's loads, the list append) better than two with 1,024 */
+    const double fill = TUNE_INT("TEIDE_PH_FILL_PCT", 60) / 100.0;
     int bits = 0;
     while (bits < 24 && est / (double)((int64_t)1 << bits) > lcap * fill) bits++;
     if (bits == 0 && forced) bits = 1;
