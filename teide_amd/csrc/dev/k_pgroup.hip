@@ -532,7 +532,8 @@ __global__ __launch_bounds__(PG_T) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
  * [b * wpb * 64, ...) of `out`, in row order; counts[b] = its records.
  * The blocks merge into the global hot cells once per workgroup. */
 #define PH_SET 2048                       /* LDS hash set of hot keys (load <= 0.25) */
-#define PH_STEP (PG_W * PG_U)             /* words per workgroup step */
+#define PH_U    4                         /* 64-row words per wave per step */
+#define PH_STEP (PG_W * PH_U)             /* words per workgroup step */
 
 template <int NV, int KE>
 __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned long long* hot,
@@ -570,18 +571,18 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
     const int64_t w_end = w_begin + wpb < nw ? w_begin + wpb : nw;
     const int64_t region = w_begin * 64;
     for (int64_t ws = w_begin; ws < w_end; ws += PH_STEP) {   /* uniform across the block */
-        uint64_t key[PG_U];
-        uint64_t v[PG_U][PG_MAXV];
-        bool live[PG_U];
-        uint64_t m[PG_U];
+        uint64_t key[PH_U];
+        uint64_t v[PH_U][PG_MAXV];
+        bool live[PH_U];
+        uint64_t m[PH_U];
 #pragma unroll
-        for (int u = 0; u < PG_U; u++) {           /* every load of the step first */
-            const int64_t w = ws + wv * PG_U + u;
+        for (int u = 0; u < PH_U; u++) {           /* every load of the step first */
+            const int64_t w = ws + wv * PH_U + u;
             live[u] = w < w_end && pg_item<true, true, NV, KE>(S, none, w * 64 + lane, S.nrows,
                                                                 key[u], v[u]);
         }
 #pragma unroll
-        for (int u = 0; u < PG_U; u++) {
+        for (int u = 0; u < PH_U; u++) {
             int h = -1;
             if (live[u] && key[u] != EMPTY_KEY) {
                 uint32_t x = (uint32_t)(mix64(key[u]) >> 32) & (PH_SET - 1);
@@ -594,13 +595,13 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
             }
             if (h >= 0) pg_fold<NV, true>(&cells[(int64_t)h * nc], P, cw, vf, v[u]);
             m[u] = __ballot(live[u] && h < 0);
-            if (lane == 0) wcnt[wv * PG_U + u] = (uint32_t)__popcll(m[u]);
+            if (lane == 0) wcnt[wv * PH_U + u] = (uint32_t)__popcll(m[u]);
         }
         __syncthreads();
         uint32_t pre = cursor;                    /* records of the step's earlier words */
-        for (int i = 0; i < wv * PG_U; i++) pre += wcnt[i];
+        for (int i = 0; i < wv * PH_U; i++) pre += wcnt[i];
 #pragma unroll
-        for (int u = 0; u < PG_U; u++) {
+        for (int u = 0; u < PH_U; u++) {
             if ((m[u] >> lane) & 1) {
                 const int64_t at = region + pre + lanes_below(m[u]);
                 out.key[at] = key[u];
@@ -693,19 +694,21 @@ static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
  * holds items [start(q), start(q) + count(q)). */
 static void pg_jobs_from(const std::vector<int64_t>& start, const std::vector<int64_t>& count,
                          std::vector<PGJob>& jobs) {
-    jobs.clear();
+    size_t total = 0;
+    for (size_t q = 0; q < start.size(); q++) total += (size_t)ceil_div(count[q], PG_JOB);
+    jobs.resize(total);
+    PGJob* J = jobs.data();
     for (size_t q = 0; q < start.size(); q++) {
         if (count[q] == 0) continue;
-        const int js = (int)jobs.size();
+        const int js = (int)(J - jobs.data());
         const int nj = (int)ceil_div(count[q], PG_JOB);
-        for (int k = 0; k < nj; k++) {
-            PGJob J;
-            J.r0 = start[q] + (int64_t)k * PG_JOB;
-            J.r1 = J.r0 + PG_JOB < start[q] + count[q] ? J.r0 + PG_JOB : start[q] + count[q];
-            J.parent = (int64_t)q;
-            J.js = js;
-            J.nj = nj;
-            jobs.push_back(J);
+        const int64_t end = start[q] + count[q];
+        for (int k = 0; k < nj; k++, J++) {
+            J->r0 = start[q] + (int64_t)k * PG_JOB;
+            J->r1 = J->r0 + PG_JOB < end ? J->r0 + PG_JOB : end;
+            J->parent = (int64_t)q;
+            J->js = js;
+            J->nj = nj;
         }
     }
 }
@@ -814,14 +817,19 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
         if (pass + 1 < npass) {
             /* the next pass: the tiles of partition p on XCD p mod 8, tile t
              * of the list on XCD t mod 8 (empty tiles pad the shorter lists) */
-            std::vector<PXTile> per[PX_XCD];
+            size_t nx[PX_XCD] = {}, longest = 0;
             for (int64_t p = 0; p < nparts; p++)
-                add(per[p % PX_XCD], O[(size_t)(p * XR)], O[(size_t)((p + 1) * XR)], p);
-            size_t longest = 0;
-            for (int x = 0; x < PX_XCD; x++) longest = per[x].size() > longest ? per[x].size() : longest;
+                nx[p % PX_XCD] += (size_t)ceil_div(O[(size_t)((p + 1) * XR)] - O[(size_t)(p * XR)],
+                                                   PX_TILE);
+            for (int x = 0; x < PX_XCD; x++) longest = nx[x] > longest ? nx[x] : longest;
             tl.assign(longest * PX_XCD, PXTile{0, 0, 0});
-            for (int x = 0; x < PX_XCD; x++)
-                for (size_t j = 0; j < per[x].size(); j++) tl[j * PX_XCD + (size_t)x] = per[x][j];
+            size_t at[PX_XCD] = {};
+            for (int64_t p = 0; p < nparts; p++) {
+                const int x = (int)(p % PX_XCD);
+                const int64_t e = O[(size_t)((p + 1) * XR)];
+                for (int64_t r = O[(size_t)(p * XR)]; r < e; r += PX_TILE)
+                    tl[at[x]++ * PX_XCD + (size_t)x] = PXTile{r, r + PX_TILE < e ? r + PX_TILE : e, p};
+            }
         }
         parents = nparts;
         std::swap(in, outb);
