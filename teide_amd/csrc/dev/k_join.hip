@@ -145,6 +145,8 @@ struct PartIn {
     int32_t         bits;     /* radix bits */
     int32_t         n_tiles;
     int32_t         plain;    /* exact key read by a plain 8- or 4-byte load, else 0 */
+    int32_t         span;     /* tiles per workgroup: workgroup b takes [b * span, ...) */
+    int32_t         nwg;      /* workgroups */
 };
 
 /* Host: the element bytes of an exact single key that ld_i64 reads with a
@@ -203,36 +205,44 @@ __device__ __forceinline__ uint32_t part_of(uint64_t kw, int bits) {
     return (uint32_t)(jmix(kw) >> (64 - bits));
 }
 
+/* Counts per (partition, workgroup) over the workgroup's span of tiles:
+ * H[p * nwg + b], so the offsets take P x nwg entries, not P x tiles. */
 __global__ __launch_bounds__(JT) void k_part_hist(PartIn I, uint32_t* H) {
     extern __shared__ uint32_t hist[];
     const int P = 1 << I.bits;
     for (int i = threadIdx.x; i < P; i += JT) hist[i] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * JTILE;
-    uint64_t kw[JROWS];
-    const uint32_t live = part_keys(I, base, kw);
+    const int t1 = (blockIdx.x + 1) * I.span < I.n_tiles ? (blockIdx.x + 1) * I.span : I.n_tiles;
+    for (int t = blockIdx.x * I.span; t < t1; t++) {
+        uint64_t kw[JROWS];
+        const uint32_t live = part_keys(I, (int64_t)t * JTILE, kw);
 #pragma unroll
-    for (int j = 0; j < JROWS; j++)
-        if ((live >> j) & 1) atomicAdd(&hist[part_of(kw[j], I.bits)], 1u);
+        for (int j = 0; j < JROWS; j++)
+            if ((live >> j) & 1) atomicAdd(&hist[part_of(kw[j], I.bits)], 1u);
+    }
     __syncthreads();
-    for (int p = threadIdx.x; p < P; p += JT) H[(int64_t)p * I.n_tiles + blockIdx.x] = hist[p];
+    for (int p = threadIdx.x; p < P; p += JT) H[(int64_t)p * I.nwg + blockIdx.x] = hist[p];
 }
 
-/* LDS counting sort of one tile by partition, then each run is written to
- * its slot [Off[p][tile], ...) with consecutive lanes on consecutive
- * addresses. */
+/* Per tile of the workgroup's span: an LDS counting sort by partition, then
+ * each run is written at its partition's running cursor (starting at
+ * Off[p * nwg + b]) with consecutive lanes on consecutive addresses. */
 __global__ __launch_bounds__(JT) void k_part_scatter(PartIn I, const int64_t* Off,
                                                      uint64_t* out_key, uint32_t* out_row) {
     extern __shared__ uint8_t smem[];
     const int P = 1 << I.bits;
-    uint32_t* hist = (uint32_t*)smem;                       /* P */
+    int64_t* cur = (int64_t*)smem;                          /* P */
+    uint32_t* hist = (uint32_t*)(cur + P);                  /* P */
     uint32_t* loff = hist + P;                              /* P + 1 */
     uint64_t* skey = (uint64_t*)(loff + P + 2);             /* JTILE, 8-aligned (P even) */
     uint32_t* srow = (uint32_t*)(skey + JTILE);             /* JTILE */
     uint16_t* spart = (uint16_t*)(srow + JTILE);            /* JTILE */
+    for (int i = threadIdx.x; i < P; i += JT) cur[i] = Off[(int64_t)i * I.nwg + blockIdx.x];
+    const int t1 = (blockIdx.x + 1) * I.span < I.n_tiles ? (blockIdx.x + 1) * I.span : I.n_tiles;
+    for (int t = blockIdx.x * I.span; t < t1; t++) {
     for (int i = threadIdx.x; i < P; i += JT) hist[i] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * JTILE;
+    int64_t base = (int64_t)t * JTILE;
     uint64_t kw[JROWS];
     uint32_t pr[JROWS], rank[JROWS];
     const uint32_t live = part_keys(I, base, kw);
@@ -272,21 +282,25 @@ __global__ __launch_bounds__(JT) void k_part_scatter(PartIn I, const int64_t* Of
     uint32_t cnt = loff[P];
     for (uint32_t i = threadIdx.x; i < cnt; i += JT) {
         uint32_t p = spart[i];
-        int64_t dst = Off[(int64_t)p * I.n_tiles + blockIdx.x] + (i - loff[p]);
+        int64_t dst = cur[p] + (i - loff[p]);
         out_key[dst] = skey[i];
         out_row[dst] = srow[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += JT) cur[i] += hist[i];
+    __syncthreads();
     }
 }
 
 struct Parts {
     const uint64_t* key;
     const uint32_t* row;
-    const int64_t*  off;      /* tile-major offsets: partition p starts at off[p * n_tiles] */
-    int32_t         n_tiles;
+    const int64_t*  off;      /* partition-major offsets: partition p starts at off[p * stride] */
+    int32_t         stride;   /* workgroups of the partition passes */
 };
 
 __device__ __forceinline__ int64_t part_start(const Parts& S, int p) {
-    return S.off[(int64_t)p * S.n_tiles];
+    return S.off[(int64_t)p * S.stride];
 }
 
 struct BPArgs {
@@ -484,11 +498,11 @@ static int join_direct(const tdd_join_spec_t* S, tdd_join_result_t* out, bool fo
 static int partition_side(const PartIn& I, uint64_t** key_out, uint32_t** row_out,
                           int64_t** off_out, int64_t* total) {
     const int P = 1 << I.bits;
-    size_t nh = (size_t)P * I.n_tiles;
+    size_t nh = (size_t)P * I.nwg;
     SCRATCH(H, nh * 4);
     int64_t* off = (int64_t*)td__arena_alloc((nh + 1) * 8);
     if (!off) return TD_ERR_OOM;
-    LAUNCH("join_part_hist", k_part_hist, I.n_tiles, JT, (size_t)P * 4, I, H.as<uint32_t>());
+    LAUNCH("join_part_hist", k_part_hist, I.nwg, JT, (size_t)P * 4, I, H.as<uint32_t>());
     int rc = tdd::scan_u32_i64(H.as<uint32_t>(), (int64_t)nh, off);
     if (rc) { td__arena_free(off); return rc; }
     int64_t n = read_i64(off + nh);
@@ -498,8 +512,8 @@ static int partition_side(const PartIn& I, uint64_t** key_out, uint32_t** row_ou
         td__arena_free(off); td__arena_free(key); td__arena_free(row);
         return n < 0 ? TD_ERR_IO : TD_ERR_OOM;
     }
-    size_t lds = (size_t)(2 * P + 2) * 4 + (size_t)JTILE * (8 + 4 + 2);
-    LAUNCH("join_part_scatter", k_part_scatter, I.n_tiles, JT, lds, I,
+    size_t lds = (size_t)P * 8 + (size_t)(2 * P + 2) * 4 + (size_t)JTILE * (8 + 4 + 2);
+    LAUNCH("join_part_scatter", k_part_scatter, I.nwg, JT, lds, I,
            (const int64_t*)off, key, row);
     *key_out = key;
     *row_out = row;
@@ -529,6 +543,11 @@ static int join_radix(const tdd_join_spec_t* S, tdd_join_result_t* out) {
     PartIn IR{RK, nullptr, S->nr, bits, (int32_t)ceil_div(S->nr > 0 ? S->nr : 1, JTILE)};
     IL.plain = part_plain(LK);
     IR.plain = part_plain(RK);
+    for (PartIn* I : {&IL, &IR}) {             /* a span of tiles per workgroup */
+        const int64_t want = (int64_t)tdd::g_num_cu * 2;
+        I->span = (int32_t)ceil_div(I->n_tiles, want);
+        I->nwg = (int32_t)ceil_div(I->n_tiles, I->span);
+    }
     uint64_t *lkey = nullptr, *rkey = nullptr;
     uint32_t *lrow = nullptr, *rrow = nullptr;
     int64_t *loff = nullptr, *roff = nullptr, nlp = 0, nrp = 0;
@@ -540,8 +559,8 @@ static int join_radix(const tdd_join_spec_t* S, tdd_join_result_t* out) {
     POLL_CANCEL();
     BPArgs A;
     memset(&A, 0, sizeof(A));
-    A.L = Parts{lkey, lrow, loff, IL.n_tiles};
-    A.R = Parts{rkey, rrow, roff, IR.n_tiles};
+    A.L = Parts{lkey, lrow, loff, IL.nwg};
+    A.R = Parts{rkey, rrow, roff, IR.nwg};
     A.LK = LK;
     A.RK = RK;
     A.exact = exact;
