@@ -804,7 +804,7 @@ static int rj_partition(RJHostSide& S, int bits, void* const* final_dst, Scratch
 struct RJDBuild {
     tdd_col_t   key;
     int64_t     nr, kmin;
-    uint32_t*   present;
+    uint8_t*    present;     /* a byte per key of the range: plain stores, no atomics */
     int32_t     n, pad;
     const void* src[RJ_MAXC];
     void*       dst[RJ_MAXC];
@@ -816,28 +816,51 @@ __device__ __forceinline__ void rjd_put(const RJDBuild& B, int c, int64_t r, uin
     ((T*)B.dst[c])[i] = ((const T*)B.src[c])[r];
 }
 
+#define RJD_R 4                  /* build rows per thread per step */
+
 __global__ void k_rjd_build(RJDBuild B) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < B.nr;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = (uint64_t)ld_i64(B.key, r) - (uint64_t)B.kmin;
-        atomicOr(&B.present[i >> 5], 1u << (i & 31));        /* result unused: no return */
-        for (int c = 0; c < B.n; c++) {
-            switch (B.esz[c]) {
-            case 8: rjd_put<uint64_t>(B, c, r, i); break;
-            case 4: rjd_put<uint32_t>(B, c, r, i); break;
-            case 2: rjd_put<uint16_t>(B, c, r, i); break;
-            default: rjd_put<uint8_t>(B, c, r, i); break;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * RJD_R;
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RJD_R + threadIdx.x; r0 < B.nr;
+         r0 += stride) {
+        uint64_t i[RJD_R];
+#pragma unroll
+        for (int j = 0; j < RJD_R; j++) {           /* the step's key loads first */
+            const int64_t r = r0 + (int64_t)j * blockDim.x;
+            i[j] = r < B.nr ? (uint64_t)ld_i64(B.key, r) - (uint64_t)B.kmin : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < RJD_R; j++) {
+            const int64_t r = r0 + (int64_t)j * blockDim.x;
+            if (r >= B.nr) continue;
+            B.present[i[j]] = 1;
+            for (int c = 0; c < B.n; c++) {
+                switch (B.esz[c]) {
+                case 8: rjd_put<uint64_t>(B, c, r, i[j]); break;
+                case 4: rjd_put<uint32_t>(B, c, r, i[j]); break;
+                case 2: rjd_put<uint16_t>(B, c, r, i[j]); break;
+                default: rjd_put<uint8_t>(B, c, r, i[j]); break;
+                }
             }
         }
     }
 }
 
-/* Keys present: the build side repeats none exactly when this equals nr. */
-__global__ void k_rjd_count(const uint32_t* present, int64_t nw, unsigned long long* count) {
+/* Keys present (the build side repeats none exactly when this equals nr),
+ * and, when `bits` is given, the present bytes packed as bits: lane l of a
+ * wave reads key w * 64 + l and the wave's ballot is bits word w. */
+__global__ void k_rjd_count(const uint8_t* present, int64_t range, uint64_t* bits,
+                            unsigned long long* count) {
     unsigned long long c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
-         i += (int64_t)gridDim.x * blockDim.x)
-        c += __popc(present[i]);
+    const int64_t nw = (range + 63) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const int64_t nwave = (int64_t)gridDim.x * blockDim.x / WAVE;
+    for (int64_t w = wave; w < nw; w += nwave) {
+        const int64_t k = w * 64 + lane_id();
+        const bool on = k < range && present[k] != 0;
+        const uint64_t m = __ballot(on);
+        if (bits && lane_id() == 0) bits[w] = m;
+        c += on;
+    }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if (lane_id() == 0 && c) atomicAdd(count, c);
 }
@@ -912,8 +935,13 @@ static int rj_direct(const tdd_join_mat_spec_t* S, int64_t mn, int64_t range,
         const int e = S->rcols[c].esz;
         if (e != 1 && e != 2 && e != 4 && e != 8) return TD_ERR_NYI;
     }
-    const int64_t pw = ceil_div(range, 32);
-    SCRATCH(present, (size_t)pw * 4);
+    const int64_t pw = ceil_div(range, 64);
+    SCRATCH(pbytes, (size_t)range);
+    /* bits for the probe only when some key of the range is absent (a full
+     * permutation of the range is answered by the range check) */
+    const bool full = J->nr == range;
+    Scratch present(full ? 8 : (size_t)pw * 8);
+    if (!present.ok()) return TD_ERR_OOM;
     SCRATCH(count, 8);
     RJBlocks dense;
     RJDBuild B;
@@ -921,26 +949,27 @@ static int rj_direct(const tdd_join_mat_spec_t* S, int64_t mn, int64_t range,
     B.key = J->rkeys[0];
     B.nr = J->nr;
     B.kmin = mn;
-    B.present = present.as<uint32_t>();
+    B.present = pbytes.as<uint8_t>();
     B.n = S->n_rcols;
     for (int c = 0; c < S->n_rcols; c++) {
         B.src[c] = S->rcols[c].data;
         B.esz[c] = S->rcols[c].esz;
         if (!(B.dst[c] = dense.take((size_t)range * B.esz[c]))) return TD_ERR_OOM;
     }
-    int rc = tdd_fill(present.p, 0, (size_t)pw * 4);
+    int rc = tdd_fill(pbytes.p, 0, (size_t)range);
     if (!rc) rc = tdd_fill(count.p, 0, 8);
     if (rc) return rc;
     if (J->nr > 0)
-        LAUNCH("rjoin_map", k_rjd_build, stream_grid(ceil_div(J->nr, 256 * 4)), 256, 0, B);
-    LAUNCH("rjoin_map_count", k_rjd_count, stream_grid(ceil_div(pw, 256 * 8), 4), 256, 0,
-           (const uint32_t*)present.as<uint32_t>(), pw, count.as<unsigned long long>());
+        LAUNCH("rjoin_map", k_rjd_build, stream_grid(ceil_div(J->nr, 256 * RJD_R)), 256, 0, B);
+    LAUNCH("rjoin_map_count", k_rjd_count, stream_grid(ceil_div(pw, 256 / WAVE * 8), 4), 256, 0,
+           (const uint8_t*)pbytes.as<uint8_t>(), range,
+           full ? nullptr : present.as<uint64_t>(), count.as<unsigned long long>());
     int64_t distinct = 0;
     HC(hipMemcpyAsync(&distinct, count.p, 8, hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
     if (distinct != J->nr) return TD_ERR_NYI;         /* repeated build keys: pairs */
     /* every key of the range present: a range check answers for the map */
-    const uint32_t* pres = J->nr == range ? nullptr : present.as<uint32_t>();
+    const uint32_t* pres = full ? nullptr : present.as<uint32_t>();
     const int64_t nl = J->nl;
     const bool inner = J->join_type == 0;
     tdd_sel_t sv;
