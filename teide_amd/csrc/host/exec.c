@@ -1249,6 +1249,17 @@ static int join_keys(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* right, tdd_jo
     return TD_OK;
 }
 
+/* Column statistics of a join output column copied from `src`: the
+ * source's bounds hold for its rows (a subset), widened to 0 when rows
+ * without a partner are zero-filled. */
+static void join_out_stats(td_t* out, const tdd_col_t* src, int zero_filled) {
+    if (!src->ranged || TD_IS_ERR(out) || out->mmod != TD_MMOD_HBM) return;
+    td_hbm_ref_t* r = td_hbm_ref(out);
+    r->lo = zero_filled && src->lo > 0 ? 0 : src->lo;
+    r->hi = zero_filled && src->hi < 0 ? 0 : src->hi;
+    r->ranged = 1;
+}
+
 static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int nullable) {
     tdd_col_t src;
     int rc = dev_col(x, col, &src);
@@ -1257,6 +1268,7 @@ static td_t* gather_col(xctx_t* x, td_t* col, const int64_t* idx, int64_t n, int
     if (TD_IS_ERR(out)) return out;
     rc = tdd_gather(&src, idx, n, nullable, td_hbm_ref(out)->dptr);
     if (rc) { td_release(out); return TD_ERR_PTR(rc); }
+    join_out_stats(out, &src, nullable);
     return out;
 }
 
@@ -1312,6 +1324,7 @@ static td_t* join_materialised(xctx_t* x, td_op_ext_t* je, td_t* left, td_t* rig
         td_t* o = td__hbm_wrap(src->type, src->attrs & TD_SYM_W_MASK, R.n_rows, d,
                                (size_t)(R.n_rows > 0 ? R.n_rows : 1) * dc->esz);
         if (TD_IS_ERR(o)) { td_release(result); result = o; continue; }
+        join_out_stats(o, dc, right_side && J->join_type == 1);   /* LEFT: zero-filled right rows */
         int64_t name = right_side ? rname[i] : lname[i];
         result = td_table_add_col(result, name, o);
         td_release(o);
