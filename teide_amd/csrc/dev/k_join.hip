@@ -753,11 +753,14 @@ __global__ void k_sum_final(const int64_t* part, int n, int64_t* out) {
     if (threadIdx.x == 0) *out = s[0];
 }
 
-/* Build the slot map into `map` (range entries of MT); returns 1 in *unique
- * when the build keys are distinct. */
-template <typename MT>
+/* Slot map of unique build keys: map[key - kmin] = the row's group slot.
+ * *unique tells whether every build key was distinct (occupied entries ==
+ * build rows); with `deferred` the occupied count is left at *deferred on
+ * the device instead (no host sync) and *unique is set for the caller to
+ * check once its own work is queued. */
+template <class MT>
 static int sm_build(const tdd_col_t& rk, int64_t nr, int64_t kmin, int64_t range,
-                    const GroupPlan& P, MT* map, int* unique) {
+                    const GroupPlan& P, MT* map, int* unique, int64_t* deferred = nullptr) {
     TileCols T;
     memset(&T, 0, sizeof(T));
     tile_slot(T, rk);
@@ -777,7 +780,11 @@ static int sm_build(const tdd_col_t& rk, int64_t nr, int64_t kmin, int64_t range
     LAUNCH("jg_slotmap", k_sm_count<MT>, grid, 256, 0, (const MT*)map, range, (MT)~(MT)0,
            part.as<int64_t>() + 1);
     LAUNCH("jg_slotmap", k_sum_final, 1, 256, 0, (const int64_t*)part.as<int64_t>() + 1, grid,
-           part.as<int64_t>());
+           deferred ? deferred : part.as<int64_t>());
+    if (deferred) {                      /* checked by the caller after its own launches */
+        *unique = 1;
+        return TD_OK;
+    }
     int64_t cnt = 0;
     HC(hipMemcpyAsync(&cnt, part.p, 8, hipMemcpyDeviceToHost, tdd::g_stream));
     HC(hipStreamSynchronize(tdd::g_stream));
@@ -998,7 +1005,11 @@ int jg_xpart(const tdd_join_group_spec_t* S, const GroupPlan& GP, const TileCols
              const StarPred& SP, const uint8_t* cell_slot, const uint16_t* map, int64_t kmin,
              int64_t range, int64_t n_slots, tdd_group_result_t* out);
 
-extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
+/* The fused join -> group-by.  `speculate`: build the slot map and probe
+ * it without waiting for the build keys' uniqueness check, which is read
+ * after the result is out; repeated keys then discard that result and run
+ * the query again without speculation. */
+static int jg_fused(const tdd_join_group_spec_t* S, tdd_group_result_t* out, bool speculate) {
     memset(out, 0, sizeof(*out));
     POLL_CANCEL();
     const tdd_join_spec_t* J = &S->join;
@@ -1080,11 +1091,15 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
 
     /* Slot map for unique build keys; duplicates fall back to chains. */
     Scratch smap(slotmap ? (size_t)range * (small ? 2 : 4) : 8);
+    SCRATCH(smcnt, 8);
     if (!smap.ok()) return TD_ERR_OOM;
+    const bool deferred = speculate && slotmap && (star || tiled) && lds_table &&
+                          J->force_path != 4;
     if (slotmap) {
         int uniq = 0;
-        rc = small ? sm_build<uint16_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<uint16_t>(), &uniq)
-                   : sm_build<int32_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<int32_t>(), &uniq);
+        int64_t* dc = deferred ? smcnt.as<int64_t>() : nullptr;
+        rc = small ? sm_build<uint16_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<uint16_t>(), &uniq, dc)
+                   : sm_build<int32_t>(J->rkeys[0], J->nr, mn, range, P, smap.as<int32_t>(), &uniq, dc);
         if (rc == TD_ERR_NYI) rc = TD_OK;
         if (rc) return rc;
         slotmap = uniq != 0;
@@ -1188,6 +1203,16 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
         }
         rc = da_finish(P, parts.as<uint64_t>(), grid, n_slots, out);
         out->path = 1;
+        if (!rc && deferred) {
+            int64_t cnt = 0;
+            HC(hipMemcpyAsync(&cnt, smcnt.p, 8, hipMemcpyDeviceToHost, tdd::g_stream));
+            HC(hipStreamSynchronize(tdd::g_stream));
+            if (cnt != J->nr) {                  /* repeated build keys: the map was wrong */
+                for (int k = 0; k < P.n_keys; k++) td__arena_free(out->keys[k]);
+                for (int a = 0; a < P.n_aggs; a++) td__arena_free(out->vals[a]);
+                return jg_fused(S, out, false);
+            }
+        }
     } else {
         SCRATCH(tab, table_bytes);
         rc = da_table_init(P, tab.as<uint64_t>(), n_slots);
@@ -1203,6 +1228,10 @@ extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t
     if (rc) return rc;
     HC(hipStreamSynchronize(tdd::g_stream));
     return TD_OK;
+}
+
+extern "C" int tdd_join_group(const tdd_join_group_spec_t* S, tdd_group_result_t* out) {
+    return jg_fused(S, out, true);
 }
 
 /* General form on the device: filter to a selection, join to index pairs,
