@@ -19,6 +19,7 @@ namespace tdd {
 
 extern hipStream_t g_stream;
 extern int g_num_cu;
+extern int g_lds_block;                 /* LDS bytes one workgroup may allocate */
 
 int map_err(hipError_t e, const char* what, const char* file, int line);
 

@@ -16,6 +16,7 @@ namespace tdd {
 
 hipStream_t g_stream = nullptr;
 int g_num_cu = 256;
+int g_lds_block = 65536;
 static int g_ready = 0;                /* 1 ok, -1 unavailable */
 static std::mutex g_init_mu;
 static char g_arch[32];
@@ -108,6 +109,7 @@ extern "C" int tdd_init(void) {
         return TD_ERR_IO;
     }
     g_num_cu = g_props.multiProcessorCount > 0 ? g_props.multiProcessorCount : 256;
+    if (g_props.sharedMemPerBlock >= 32768) g_lds_block = (int)g_props.sharedMemPerBlock;
     snprintf(g_arch, sizeof(g_arch), "%s", g_props.gcnArchName);
     snprintf(g_name, sizeof(g_name), "%s", g_props.name);
     /* One stream at the default priority for everything the library runs. */

@@ -14,16 +14,19 @@
  * its range (one cell) into its own direct-array table, whose layout is the
  * group-by's (group_plan.h), so the emit is the group-by's emit.
  *
- *   k_srv_map      dim: direct map pk -> group of each key (once, at open)
- *   k_srv_cube     fact chunk: map gather + cube atomics per row and key
- *   k_srv_cumul    per (key, g): prefix and suffix sums over v
+ *   k_srv_map / k_srv_jmap   dim: direct map pk -> group of each key, and
+ *                  the joint map (every key's group per pk) (once, at open)
+ *   k_srv_cpass / k_srv_cfold   fact chunk -> cubes by slices of groups:
+ *                  cells counted, records placed in slice order, each slice
+ *                  folded in LDS and written with its prefix / suffix sums
+ *   k_srv_cube + k_srv_cumul   the same with global atomics per row and key,
+ *                  then the sums over v (cubes of more than 8,192 slices)
  *   k_srv_accum    per (request, g): table += the request's cube cell
  *   k_srv_flags / k_srv_emit   finished requests' groups, ascending keys
  *
  * Per row of a step the scan reads fk, pcol and the value columns once
  * (3 x 8 bytes for the fact layout fk i64, qty i64, price f64) for every
- * request in the mix, so the HBM cost of the scan is shared; the atomics
- * (1 + value columns per key) bound it, not the bytes.
+ * request in the mix, so the HBM cost of the scan is shared.
  *
  * Requests whose filter is anything else (a comparison on an f64 or
  * wide-domain column, several comparisons) are direct requests: they share
@@ -107,6 +110,16 @@ struct tdd_srv {
     int64_t  G[TDD_SRV_MAX_KEYS], klo[TDD_SRV_MAX_KEYS];
     int32_t  ok[TDD_SRV_MAX_KEYS];
     SrvCube* d_cubes;             /* device copy of cube[] */
+    /* the cube by slices: one block holding cells, records, counts, scan */
+    uint8_t*  cs_mem;             /* nullptr: k_srv_cube only */
+    uint32_t* cs_cells;           /* [key slot][chunk row] */
+    uint16_t* cs_roff;            /* [record] */
+    uint64_t* cs_rval;            /* [value][record] */
+    uint32_t* cs_C;
+    int64_t*  cs_O;
+    int32_t*  cs_jmap;            /* joint map: [pk - kmin][cs_js], slot order of the served keys */
+    int64_t   cs_rows;            /* chunk rows the arrays hold per key slot */
+    int32_t   cs_nk, cs_js;       /* key slots, ints per joint map row */
 };
 
 struct tdd_srv_acc {
@@ -234,6 +247,269 @@ __global__ __launch_bounds__(256) void k_srv_cube(SrvScanArgs S) {
                         atomicAdd((double*)&K.val[j][cell], __longlong_as_double(v[j][i]));
                     else
                         atomicAdd((unsigned long long*)&K.val[j][cell], (unsigned long long)v[j][i]);
+                }
+            }
+        }
+    }
+}
+
+/* The cube by slices: instead of two global atomics per row and key (they
+ * bound k_srv_cube at ~23 G/s), a step counts the chunk's rows per cube
+ * slice, places them as records in slice order and folds each slice in
+ * LDS, writing every cell of it -- no atomics outside LDS, no fills. A
+ * slice is gw consecutive groups x every pcol value, so the fold also
+ * writes the prefix / suffix sums over the values (what k_srv_cumul does
+ * after k_srv_cube) and the raw cube only when a request reads it.
+ *
+ *   k_srv_cpass<0>  rows of workgroup b's span: one gather of the joint map
+ *                   row (every served key's group of a dim row), each active
+ *                   key's cell (g * Q + v) kept per row, counts per (slice, b)
+ *   scan            counts -> record positions
+ *   k_srv_cpass<1>  the same rows again: records (cell within its slice,
+ *                   the key's value words) at the positions
+ *   k_srv_cfold     one workgroup per slice: its records (one segment per
+ *                   row-pass workgroup) folded into an LDS table of the
+ *                   slice's cells, then per group the sums over v out
+ *
+ * Record positions: R consecutive slices form a run group and the scan
+ * runs over (run group, XCD of b, b, slice in the group), so workgroup b
+ * writes one run per run group (~256 per key) and neighbouring runs come
+ * from workgroups of one XCD. */
+#define SRV_CS_MAXS  8192                     /* slices of a step, all keys */
+#define SRV_CS_WGMAX 512                      /* workgroups of the two row passes */
+#define SRV_CS_TS    256                      /* target run groups per key */
+#define SRV_CS_NONE  0xffffffffu
+
+struct SrvCubePlan {
+    int32_t   nk, ns, nwg, js;                /* js: ints per joint map row (1, 2, 4) */
+    int64_t   span;                           /* rows per workgroup (a multiple of 4,096) */
+    const int32_t* jmap;                      /* [pk - kmin][js] */
+    int64_t   qlo;
+    int32_t   Q;
+    uint32_t  ftype;                          /* bit j: value column j is f64 */
+    int32_t   on[TDD_SRV_MAX_KEYS], vmask[TDD_SRV_MAX_KEYS], kslot[TDD_SRV_MAX_KEYS];
+    int32_t   modes[TDD_SRV_MAX_KEYS];        /* bit M_RAW / M_SFX / M_PFX: cubes to write */
+    int32_t   R[TDD_SRV_MAX_KEYS];            /* slices per run group */
+    int32_t   gw[TDD_SRV_MAX_KEYS];           /* groups per slice */
+    int32_t   sbase[TDD_SRV_MAX_KEYS + 1];    /* first slice of key k (run groups whole) */
+    int64_t   G[TDD_SRV_MAX_KEYS];
+    uint32_t* cells[TDD_SRV_MAX_KEYS];        /* per row of the chunk: cell or NONE */
+    uint32_t* C;                              /* counts, srv_cs_at order */
+    const int64_t* O;                         /* their exclusive scan */
+    uint16_t* roff;                           /* records: cell within the slice */
+    uint64_t* rval[TDD_SRV_MAX_VALS];         /*          value words */
+    SrvCube   cube[TDD_SRV_MAX_KEYS];
+};
+
+__device__ __forceinline__ int srv_cs_key(const SrvCubePlan& C, int s) {
+    int kk = 0;
+    while (kk + 1 < C.nk && s >= C.sbase[kk + 1]) kk++;
+    return kk;
+}
+
+/* Scan position of the count (slice s of key kk, row-pass workgroup b):
+ * key, run group, b's XCD (b mod 8), b's order on it, slice in the group.
+ * The grid is a multiple of 8. */
+__device__ __forceinline__ int64_t srv_cs_at(const SrvCubePlan& C, int kk, int s, int64_t b) {
+    const int64_t l = s - C.sbase[kk], R = C.R[kk];
+    return (int64_t)C.sbase[kk] * C.nwg +
+           (((l / R) * 8 + (b & 7)) * (C.nwg >> 3) + (b >> 3)) * R + l % R;
+}
+
+/* Dim slot -> the joint map row (ints beyond the served keys hold -1). */
+__global__ void k_srv_jmap(const int32_t* m0, const int32_t* m1, const int32_t* m2,
+                           const int32_t* m3, int64_t range, int js, int32_t* jmap) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < range;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        jmap[r * js] = m0[r];
+        if (js > 1) jmap[r * js + 1] = m1 ? m1[r] : -1;
+        if (js > 2) {
+            jmap[r * js + 2] = m2 ? m2[r] : -1;
+            jmap[r * js + 3] = m3 ? m3[r] : -1;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t srv_pick(int4 v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+template <int PUT>
+__global__ __launch_bounds__(1024) void k_srv_cpass(SrvScanArgs S, SrvCubePlan C) {
+    extern __shared__ uint32_t lh[];          /* per slice: counts, or (PUT) cursors */
+    const int64_t b = blockIdx.x;
+    const int64_t s0 = S.r0 + b * C.span;
+    const int64_t s1 = s0 + C.span < S.r1 ? s0 + C.span : S.r1;
+    for (int i = threadIdx.x; i < C.ns; i += 1024)
+        lh[i] = PUT ? (uint32_t)C.O[srv_cs_at(C, srv_cs_key(C, i), i, b)] : 0u;
+    __syncthreads();
+    for (int64_t b0 = s0; b0 < s1; b0 += 1024 * SRV_R) {
+        const int64_t base = b0 + (int64_t)threadIdx.x * SRV_R;
+        if (!PUT) {
+            int64_t fk[SRV_R], q[SRV_R];
+            load_rows<SRV_R>(S.fk, base, s1, S.vec & 1u, fk);
+            load_rows<SRV_R>(S.pcol, base, s1, (S.vec >> 1) & 1u, q);
+            int4 jr[SRV_R];
+#pragma unroll
+            for (int i = 0; i < SRV_R; i++) {
+                const uint64_t k = (uint64_t)(fk[i] - S.kmin);
+                const uint64_t c = (uint64_t)(q[i] - S.qlo);
+                const bool live = base + i < s1 && k < (uint64_t)S.range && c < (uint64_t)S.Q;
+                q[i] = live ? (int64_t)c : -1;
+                jr[i] = make_int4(-1, -1, -1, -1);
+                if (live) {
+                    if (C.js == 4) {
+                        jr[i] = ((const int4*)C.jmap)[k];
+                    } else if (C.js == 2) {
+                        const int2 t = ((const int2*)C.jmap)[k];
+                        jr[i].x = t.x;
+                        jr[i].y = t.y;
+                    } else {
+                        jr[i].x = C.jmap[k];
+                    }
+                }
+            }
+            for (int kk = 0; kk < C.nk; kk++) {
+                if (!C.on[kk]) continue;
+                const int ks = C.kslot[kk];
+                const int64_t sl = (int64_t)C.gw[kk] * C.Q;
+#pragma unroll
+                for (int i = 0; i < SRV_R; i++) {
+                    if (base + i >= s1) continue;
+                    const int32_t g = srv_pick(jr[i], ks);
+                    uint32_t cell = SRV_CS_NONE;
+                    if (q[i] >= 0 && g >= 0) {
+                        cell = (uint32_t)((int64_t)g * C.Q + q[i]);
+                        atomicAdd(&lh[C.sbase[kk] + (int64_t)cell / sl], 1u);
+                    }
+                    C.cells[kk][base + i - S.r0] = cell;
+                }
+            }
+        } else {
+            int64_t v[TDD_SRV_MAX_VALS][SRV_R];
+#pragma unroll
+            for (int j = 0; j < TDD_SRV_MAX_VALS; j++) {
+                if (S.vals[j].data) {
+                    load_rows<SRV_R>(S.vals[j], base, s1, (S.vec >> (2 + j)) & 1u, v[j]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < SRV_R; i++) v[j][i] = 0;
+                }
+            }
+            for (int kk = 0; kk < C.nk; kk++) {
+                if (!C.on[kk]) continue;
+                const int64_t sl = (int64_t)C.gw[kk] * C.Q;
+#pragma unroll
+                for (int i = 0; i < SRV_R; i++) {
+                    if (base + i >= s1) continue;
+                    const uint32_t cell = C.cells[kk][base + i - S.r0];
+                    if (cell == SRV_CS_NONE) continue;
+                    const uint32_t at = atomicAdd(&lh[C.sbase[kk] + (int64_t)cell / sl], 1u);
+                    C.roff[at] = (uint16_t)((int64_t)cell % sl);
+#pragma unroll
+                    for (int j = 0; j < TDD_SRV_MAX_VALS; j++)
+                        if ((C.vmask[kk] >> j) & 1) C.rval[j][at] = (uint64_t)v[j][i];
+                }
+            }
+        }
+    }
+    if (!PUT) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < C.ns; i += 1024)
+            C.C[srv_cs_at(C, srv_cs_key(C, i), i, b)] = lh[i];
+    }
+}
+
+__global__ __launch_bounds__(512) void k_srv_cfold(SrvCubePlan C) {
+    extern __shared__ uint64_t lt[];          /* count, then each value word: [g][v] each */
+    __shared__ int64_t st[SRV_CS_WGMAX];      /* segment b: first record */
+    __shared__ uint32_t pre[SRV_CS_WGMAX + 1]; /* records before segment b */
+    const int s = blockIdx.x;
+    const int kk = srv_cs_key(C, s);
+    const int64_t G = C.G[kk], Q = C.Q, gw = C.gw[kk], sl = gw * Q;
+    const int64_t g0 = (int64_t)(s - C.sbase[kk]) * gw;
+    if (g0 >= G) return;                      /* past the cube: a run group's tail */
+    const int64_t ng = g0 + gw <= G ? gw : G - g0;
+    const int vm = C.vmask[kk];
+    int nw = 0;
+    for (int j = 0; j < TDD_SRV_MAX_VALS; j++) nw += (vm >> j) & 1;
+    const int nwg = C.nwg;
+    uint32_t len = 0;
+    if ((int)threadIdx.x < nwg) {
+        const int64_t at = srv_cs_at(C, kk, s, threadIdx.x);
+        st[threadIdx.x] = C.O[at];
+        len = (uint32_t)(C.O[at + 1] - C.O[at]);
+    }
+    for (int64_t i = threadIdx.x; i < sl * (1 + nw); i += 512) lt[i] = 0;
+    /* inclusive scan of len over the 512 threads (Hillis-Steele in pre) */
+    pre[threadIdx.x + 1] = len;
+    if (threadIdx.x == 0) pre[0] = 0;
+    __syncthreads();
+    for (int d = 1; d < 512; d <<= 1) {
+        const uint32_t add = (int)threadIdx.x >= d ? pre[threadIdx.x + 1 - d] : 0u;
+        __syncthreads();
+        pre[threadIdx.x + 1] += add;
+        __syncthreads();
+    }
+    const uint32_t total = pre[nwg];
+    for (uint32_t r = threadIdx.x; r < total; r += 512) {
+        int lo = 0, hi = nwg - 1;                 /* last segment with pre[b] <= r */
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= r) lo = mid; else hi = mid - 1;
+        }
+        const int64_t at = st[lo] + (r - pre[lo]);
+        const int64_t o = C.roff[at];
+        atomicAdd((unsigned long long*)&lt[o], 1ull);
+        int w = 0;
+        for (int j = 0; j < TDD_SRV_MAX_VALS; j++) {
+            if (!((vm >> j) & 1)) continue;
+            w++;
+            const uint64_t x = C.rval[j][at];
+            if ((C.ftype >> j) & 1)
+                atomicAdd((double*)&lt[w * sl + o], __longlong_as_double((long long)x));
+            else
+                atomicAdd((unsigned long long*)&lt[w * sl + o], (unsigned long long)x);
+        }
+    }
+    __syncthreads();
+    /* per (group, word): the raw cells, prefix and suffix sums over v; a
+     * count word also gives the pcol sums n * (qlo + v) */
+    const SrvCube& K = C.cube[kk];
+    const int modes = C.modes[kk];
+    for (int64_t t = threadIdx.x; t < ng * (1 + nw); t += 512) {
+        const int64_t gl = t % ng, word = t / ng, g = g0 + gl;
+        int j = -1;                                /* value column of the word */
+        for (int jj = 0, w = 0; jj < TDD_SRV_MAX_VALS; jj++) {
+            if (!((vm >> jj) & 1)) continue;
+            if (++w == word) j = jj;
+        }
+        const bool f64 = j >= 0 && ((C.ftype >> j) & 1);
+        const uint64_t* col = lt + word * sl + gl * Q;
+        for (int dir = 0; dir < 3; dir++) {
+            const int mode = dir == 0 ? M_RAW : dir == 1 ? M_PFX : M_SFX;
+            if (!((modes >> mode) & 1)) continue;
+            uint64_t acc = 0;
+            int64_t sq = 0;
+            for (int64_t i = 0; i < Q; i++) {
+                const int64_t v = mode == M_SFX ? Q - 1 - i : i;
+                const uint64_t x = col[v];
+                const int64_t at = v * G + g;
+                if (mode == M_RAW) {
+                    if (j < 0) K.cnt[M_RAW][at] = x; else K.val[M_RAW][j][at] = x;
+                    continue;
+                }
+                if (f64)
+                    acc = (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) +
+                                                         __longlong_as_double((long long)x));
+                else
+                    acc += x;
+                if (j < 0) {
+                    sq += (int64_t)x * (C.qlo + v);
+                    K.cnt[mode][at] = acc;
+                    K.sq[mode][at] = sq;
+                } else {
+                    K.val[mode][j][at] = acc;
                 }
             }
         }
@@ -617,6 +893,40 @@ extern "C" int tdd_srv_open(const tdd_srv_spec_t* spec, tdd_srv_t** out) {
         any = 1;
     }
     if (!any) { tdd_srv_close(s); return TD_ERR_NYI; }
+    {   /* arrays of the sliced cube; without them every step takes k_srv_cube */
+        int nok = 0;
+        for (int k = 0; k < S.n_keys; k++) nok += s->ok[k];
+        const int64_t rows = S.nf < chunk ? S.nf : chunk;
+        const int64_t nrec = rows * nok;
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };   /* sections 256-aligned */
+        const size_t b_cells = al((size_t)nrec * 4), b_roff = al((size_t)nrec * 2);
+        const size_t b_rval = (size_t)nrec * 8 * (size_t)nv;
+        const size_t b_C = (size_t)SRV_CS_MAXS * SRV_CS_WGMAX * 4;
+        const size_t b_O = ((size_t)SRV_CS_MAXS * SRV_CS_WGMAX + 1) * 8;
+        const int js = nok == 1 ? 1 : nok == 2 ? 2 : 4;
+        const size_t b_J = (size_t)s->range * 4 * (size_t)js;
+        const size_t tot = b_cells + b_roff + b_rval + b_C + b_O + b_J;
+        if (rows > 0 && nrec < ((int64_t)1 << 32)) {
+            s->cs_mem = (uint8_t*)td__arena_alloc(tot);
+            if (s->cs_mem) {
+                uint8_t* p = s->cs_mem;
+                s->cs_cells = (uint32_t*)p; p += b_cells;
+                s->cs_roff = (uint16_t*)p; p += b_roff;
+                s->cs_rval = (uint64_t*)p; p += b_rval;
+                s->cs_C = (uint32_t*)p; p += b_C;
+                s->cs_O = (int64_t*)p; p += b_O;
+                s->cs_jmap = (int32_t*)p;
+                s->cs_rows = rows;
+                s->cs_nk = nok;
+                s->cs_js = js;
+                s->bytes += (int64_t)tot;
+                const int32_t* m[4] = {nullptr, nullptr, nullptr, nullptr};
+                for (int k = 0, sl = 0; k < S.n_keys; k++) if (s->ok[k]) m[sl++] = s->map[k];
+                LAUNCH("serve_map", k_srv_jmap, stream_grid(ceil_div(s->range, 256 * 4)), 256, 0,
+                       m[0], m[1], m[2], m[3], s->range, js, s->cs_jmap);
+            }
+        }
+    }
     s->d_cubes = (SrvCube*)td__arena_alloc(sizeof(SrvCube) * TDD_SRV_MAX_KEYS);
     if (!s->d_cubes) { tdd_srv_close(s); return TD_ERR_OOM; }
     HC(hipMemcpyAsync(s->d_cubes, s->cube, sizeof(s->cube), hipMemcpyHostToDevice, tdd::g_stream));
@@ -636,6 +946,7 @@ extern "C" void tdd_srv_close(tdd_srv_t* s) {
         if (s->cube_mem[k]) td__arena_free(s->cube_mem[k]);
     }
     if (s->d_cubes) td__arena_free(s->d_cubes);
+    if (s->cs_mem) td__arena_free(s->cs_mem);
     free(s);
 }
 
@@ -1007,11 +1318,103 @@ static int srv_direct(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n, int64_t
     return glob.empty() ? TD_OK : srv_global(s, glob, r0, r1);
 }
 
+/* The step's raw cubes by global atomics (cubes the sliced form cannot take). */
+static int srv_cube_atomic(tdd_srv_t* s, const SrvScanArgs& A) {
+    for (int k = 0; k < A.nk; k++) {
+        const SrvKeyArgs& K = A.key[k];
+        if (!K.on) continue;
+        const size_t bytes = (size_t)(K.G * s->Q) * 8;
+        int rc = tdd_fill(K.cnt, 0, bytes);
+        for (int j = 0; j < s->spec.n_vals && !rc; j++)
+            if ((K.vmask >> j) & 1) rc = tdd_fill(K.val[j], 0, bytes);
+        if (rc) return rc;
+    }
+    const int64_t tiles = ceil_div(A.r1 - A.r0, 64 * SRV_R);
+    LAUNCH("serve_cube", k_srv_cube, resident_grid(k_srv_cube, 256, 0, ceil_div(tiles, 4)), 256,
+           0, A);
+    return TD_OK;
+}
+
+/* The step's cubes by slices, prefix / suffix sums included: TD_ERR_NYI
+ * when the server has no slice arrays, one group's pcol values do not fit
+ * an LDS table or the active cubes need more than SRV_CS_MAXS slices.
+ * modes[k]: bit M_RAW / M_PFX / M_SFX for the cubes key k's requests read. */
+static int srv_cube_sliced(tdd_srv_t* s, const SrvScanArgs& A, uint32_t ftype,
+                           const int* modes) {
+    const int64_t rows = A.r1 - A.r0;
+    if (!s->cs_mem || rows > s->cs_rows || TUNE_INT("TEIDE_SRV_CUBE_ATOMIC", 0)) return TD_ERR_NYI;
+    SrvCubePlan P;
+    memset(&P, 0, sizeof(P));
+    P.nk = A.nk;
+    P.ftype = ftype;
+    P.jmap = s->cs_jmap;
+    P.js = s->cs_js;
+    P.qlo = s->qlo;
+    P.Q = s->Q;
+    int ns = 0, slot = 0;
+    size_t fold_lds = 8;
+    /* LDS of a slice's table, beside the fold's 6 KB of segment tables */
+    int64_t lds = (int64_t)TUNE_INT("TEIDE_SRV_CS_LDS_KB", 40) * 1024;
+    if (lds > tdd::g_lds_block - 8192) lds = tdd::g_lds_block - 8192;
+    for (int k = 0; k < A.nk; k++) {
+        P.sbase[k] = ns;
+        const SrvKeyArgs& K = A.key[k];
+        if (!s->ok[k]) continue;
+        const int kslot = slot++;                  /* key slots follow the served keys */
+        if (!K.on) continue;
+        int nw = 0;
+        for (int j = 0; j < TDD_SRV_MAX_VALS; j++) nw += (K.vmask >> j) & 1;
+        int64_t gw = lds / ((1 + nw) * 8) / s->Q;             /* groups of an LDS table */
+        if (gw < 1) return TD_ERR_NYI;
+        if (gw >= 32) gw &= ~(int64_t)15;                      /* whole lines per value row */
+        const int64_t gpar = ceil_div(K.G, 256);              /* >= 256 slices where G allows */
+        if (gw > gpar) gw = gpar;
+        const int64_t nsl = ceil_div(K.G, gw);
+        const int64_t R = ceil_div(nsl, SRV_CS_TS);            /* slices per run group */
+        const int64_t nsk = ceil_div(nsl, R) * R;
+        if (ns + nsk > SRV_CS_MAXS) return TD_ERR_NYI;
+        ns += (int)nsk;
+        P.on[k] = 1;
+        P.vmask[k] = K.vmask;
+        P.kslot[k] = kslot;
+        P.modes[k] = modes[k];
+        P.R[k] = (int32_t)R;
+        P.gw[k] = (int32_t)gw;
+        P.G[k] = K.G;
+        P.cells[k] = s->cs_cells + kslot * s->cs_rows;
+        P.cube[k] = s->cube[k];
+        const size_t tab = (size_t)(gw * s->Q) * (1 + nw) * 8;
+        if (tab > fold_lds) fold_lds = tab;
+    }
+    P.sbase[A.nk] = ns;
+    if (ns == 0) return TD_OK;
+    const int64_t wgmax = TUNE_INT("TEIDE_SRV_CS_WG", 256);
+    int64_t nwg = ceil_div(ceil_div(rows, 4096), 8) * 8;
+    if (nwg > wgmax) nwg = wgmax;
+    if (nwg > SRV_CS_WGMAX) nwg = SRV_CS_WGMAX;
+    nwg = nwg < 8 ? 8 : nwg & ~(int64_t)7;
+    P.ns = ns;
+    P.nwg = (int32_t)nwg;
+    P.span = ceil_div(ceil_div(rows, nwg), 4096) * 4096;
+    P.C = s->cs_C;
+    P.O = s->cs_O;
+    P.roff = s->cs_roff;
+    const int64_t nrec = s->cs_rows * s->cs_nk;
+    for (int j = 0; j < TDD_SRV_MAX_VALS; j++)
+        P.rval[j] = j < s->spec.n_vals ? s->cs_rval + j * nrec : nullptr;
+    LAUNCH("serve_cube", k_srv_cpass<0>, (int)nwg, 1024, (size_t)ns * 4, A, P);
+    int rc = tdd::scan_u32_i64(s->cs_C, (int64_t)ns * nwg, s->cs_O);
+    if (rc) return rc;
+    LAUNCH("serve_cube", k_srv_cpass<1>, (int)nwg, 1024, (size_t)ns * 4, A, P);
+    LAUNCH("serve_cube", k_srv_cfold, ns, 512, fold_lds, P);
+    return TD_OK;
+}
+
 extern "C" int tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n) {
     const tdd_srv_spec_t& S = s->spec;
     const int nv = S.n_vals;
     int on[TDD_SRV_MAX_KEYS] = {0}, vm[TDD_SRV_MAX_KEYS] = {0};
-    int pfx[TDD_SRV_MAX_KEYS] = {0}, sfx[TDD_SRV_MAX_KEYS] = {0};
+    int pfx[TDD_SRV_MAX_KEYS] = {0}, sfx[TDD_SRV_MAX_KEYS] = {0}, modes[TDD_SRV_MAX_KEYS] = {0};
     int live = 0, direct = 0;
     for (int i = 0; i < n; i++) {
         const SrvAccArgs& a = active[i]->a;
@@ -1021,6 +1424,7 @@ extern "C" int tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n) {
         on[a.key] = 1;
         pfx[a.key] |= a.mode == M_PFX;
         sfx[a.key] |= a.mode == M_SFX;
+        modes[a.key] |= 1 << a.mode;
         for (int c = 1; c < a.ncells; c++) if (a.src[c] >= 0) vm[a.key] |= 1 << a.src[c];
     }
     const int64_t c = s->step % s->nchunks;
@@ -1054,27 +1458,25 @@ extern "C" int tdd_srv_step(tdd_srv_t* s, tdd_srv_acc_t* const* active, int n) {
     for (int j = 0; j < nv; j++) if (S.vals[j].type == TD_F64) ftype |= 1u << j;
     for (int k = 0; k < S.n_keys; k++) {
         if (!on[k]) continue;
-        const int64_t QG = s->G[k] * s->Q;
         SrvKeyArgs& K = A.key[k];
         K.map = s->map[k];
         K.cnt = s->cube[k].cnt[M_RAW];
         K.G = s->G[k];
         K.on = 1;
         K.vmask = vm[k];
-        int rc = tdd_fill(K.cnt, 0, (size_t)QG * 8);
-        for (int j = 0; j < nv && !rc; j++) {
-            K.val[j] = s->cube[k].val[M_RAW][j];
-            if ((vm[k] >> j) & 1) rc = tdd_fill(K.val[j], 0, (size_t)QG * 8);
-        }
-        if (rc) return rc;
+        for (int j = 0; j < nv; j++) K.val[j] = s->cube[k].val[M_RAW][j];
     }
-    const int64_t tiles = ceil_div(A.r1 - A.r0, 64 * SRV_R);
-    LAUNCH("serve_cube", k_srv_cube, resident_grid(k_srv_cube, 256, 0, ceil_div(tiles, 4)), 256,
-           0, A);
-    for (int k = 0; k < S.n_keys; k++) {
-        if (!on[k]) continue;
-        LAUNCH("serve_cumul", k_srv_cumul, (int)ceil_div(s->G[k], 256), 256, 0, s->cube[k],
-               s->G[k], s->Q, s->qlo, nv, ftype, vm[k], pfx[k], sfx[k]);
+    int rc = srv_cube_sliced(s, A, ftype, modes);
+    if (rc == TD_ERR_NYI) {
+        rc = srv_cube_atomic(s, A);
+        if (rc) return rc;
+        for (int k = 0; k < S.n_keys; k++) {
+            if (!on[k]) continue;
+            LAUNCH("serve_cumul", k_srv_cumul, (int)ceil_div(s->G[k], 256), 256, 0, s->cube[k],
+                   s->G[k], s->Q, s->qlo, nv, ftype, vm[k], pfx[k], sfx[k]);
+        }
+    } else if (rc) {
+        return rc;
     }
     std::vector<SrvAccArgs> h((size_t)n);
     int64_t gmax = 1;

@@ -33,9 +33,11 @@ def tables(eng):
     k64 = rng.integers(0, 3000, nd)
     flag = rng.integers(0, 4, nf)
     wide = rng.integers(0, 2_000_000, nd)              # 2 M-group key range
-    HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64, flag=flag, wide=wide)
+    mid = rng.integers(0, 300_000, nd)                 # 300 K groups x 100 qty values
+    HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64, flag=flag, wide=wide,
+                mid=mid)
     tf = eng.table([("fk", fk), ("qty", qty), ("price", price), ("flag", flag)])
-    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide)])
+    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide), ("mid", mid)])
     rf, rd = eng.put(tf), eng.put(td)
     eng.release(tf)
     eng.release(td)
@@ -168,6 +170,39 @@ def test_serve_mix_matches_alone(eng, tables):
         g.free()
     eng.serve_close(h)
     for g in dags.values():
+        g.free()
+
+
+def test_serve_large_cube_steps(eng, tables):
+    """A key whose cube (100 qty values x 300 K groups) has more slices
+    than one step folds in LDS: steps where it is active build the cubes
+    with global atomics, steps with only the small key by slices; requests
+    of both kinds, joining at different steps, give the oracle's groups."""
+    rf, rd = tables
+    h = eng.serve_open(rf, rd, "fk", "pk", "qty", ["price"], ["cat", "mid"], chunk_rows=40960)
+    rng = np.random.default_rng(33)
+    first = [dict(key="cat", cmp=c, t=int(rng.integers(1, 101)), aggs=AGG_SETS[i % 4])
+             for i, c in enumerate(["gt", "le", "eq"])]
+    later = [dict(key=k, cmp=c, t=int(rng.integers(1, 101)), aggs=AGG_SETS[i % 4])
+             for i, (k, c) in enumerate([("mid", "ge"), ("cat", "lt"), ("mid", "gt")])]
+    dags, ids, results = [], {}, {}
+    for wave in (first, later):
+        for r in wave:
+            g, root = request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"])
+            ids[eng.serve_submit(h, g, root)] = r
+            dags.append(g)
+        for _ in range(3):
+            eng.serve_step(h)
+            for rid, res in eng.serve_poll(h):
+                results[rid] = res
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 0 and info["shared"] == len(ids), info
+    for rid, r in ids.items():
+        check_oracle(eng, results[rid], r, f"request {rid} {r} vs oracle")
+        eng.release(results[rid])
+    eng.serve_close(h)
+    for g in dags:
         g.free()
 
 
