@@ -138,9 +138,10 @@ __device__ __forceinline__ uint64_t px_val(const PGRows& S, const PGBuf& in, int
 /* Items per run: C[(parent * PL + s) * XR + run]. */
 template <bool ROWS, bool HK, int KE = 0>
 __global__ __launch_bounds__(PG_T) void k_px_hist(PGRows S, PGBuf in, const PXTile* tiles,
-                                                  int ntiles, int shift, int PL, int XR,
-                                                  uint32_t* C) {
+                                                  int ntiles, const int* dn, int shift, int PL,
+                                                  int XR, uint32_t* C) {
     __shared__ uint32_t h[PG_MAXP];
+    if (dn) ntiles = *dn;                                 /* a list laid out on the device */
     const int lane = lane_id();
     for (int s = threadIdx.x; s < PL; s += PG_T) h[s] = 0;
     __syncthreads();
@@ -187,10 +188,11 @@ __global__ __launch_bounds__(PG_T) void k_px_hist(PGRows S, PGBuf in, const PXTi
 /* The scatter of one pass: runs' cursors in `cur` (the scanned counts). */
 template <bool ROWS, bool HK, int NV, int KE = 0>
 __global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
-                                                  const PXTile* tiles, int ntiles, int shift,
-                                                  int PL, int XR, unsigned long long* cur,
-                                                  int64_t nout) {
+                                                  const PXTile* tiles, int ntiles, const int* dn,
+                                                  int shift, int PL, int XR,
+                                                  unsigned long long* cur, int64_t nout) {
     __shared__ uint32_t hist[PG_MAXP], loff[PG_MAXP];
+    if (dn) ntiles = *dn;                                 /* a list laid out on the device */
     __shared__ unsigned long long gbase[PG_MAXP];
     __shared__ uint8_t spart[PX_TILE];
     __shared__ uint64_t stage[PX_TILE];
@@ -642,12 +644,69 @@ static int pg_plain(const tdd_col_t& c) {
     }
 }
 
+/* The next pass's tile list from a finished pass's run starts O (parent
+ * p = the finished pass's partition, items [O[p * XR], O[(p + 1) * XR])):
+ * the tiles of parent p go to XCD p mod 8, tile t of the list to XCD t mod 8
+ * (row i of XCD x at t = i * 8 + x), empty tiles pad the shorter XCD lists.
+ * k_px_tscan (one workgroup): each parent's first row on its XCD (base) and
+ * the list length (*ntiles = 8 x the longest XCD list); k_px_twrite (one
+ * workgroup per parent, then one per XCD for the padding) writes it. */
+__global__ __launch_bounds__(1024) void k_px_tscan(const int64_t* O, int64_t nparts, int XR,
+                                                   int64_t* base, int* ntiles) {
+    __shared__ int64_t v[1024];
+    __shared__ int64_t run[PX_XCD];
+    if (threadIdx.x < PX_XCD) run[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t c = 0; c < nparts; c += 1024) {       /* chunks of 1,024 parents */
+        const int64_t p = c + threadIdx.x;
+        const int64_t nt = p < nparts ? (O[(p + 1) * XR] - O[p * XR] + PX_TILE - 1) / PX_TILE : 0;
+        v[threadIdx.x] = nt;
+        __syncthreads();
+        for (int d = PX_XCD; d < 1024; d <<= 1) {      /* inclusive scan within p mod 8 */
+            const int64_t add = (int)threadIdx.x >= d ? v[threadIdx.x - d] : 0;
+            __syncthreads();
+            v[threadIdx.x] += add;
+            __syncthreads();
+        }
+        const int x = (int)(threadIdx.x % PX_XCD);
+        if (p < nparts) base[p] = run[x] + v[threadIdx.x] - nt;
+        __syncthreads();
+        if (threadIdx.x >= 1024 - PX_XCD) run[x] += v[threadIdx.x];   /* each class's total */
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int64_t longest = 0;
+        for (int x = 0; x < PX_XCD; x++) longest = run[x] > longest ? run[x] : longest;
+        base[nparts + PX_XCD] = longest;
+        for (int x = 0; x < PX_XCD; x++) base[nparts + x] = run[x];
+        *ntiles = (int)(longest * PX_XCD);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_px_twrite(const int64_t* O, int64_t nparts, int XR,
+                                                   const int64_t* base, PXTile* tl) {
+    const int64_t b = blockIdx.x;
+    if (b < nparts) {                                   /* parent b's tiles */
+        const int64_t r0 = O[b * XR], r1 = O[(b + 1) * XR], x = b % PX_XCD;
+        const int64_t nt = (r1 - r0 + PX_TILE - 1) / PX_TILE;
+        for (int64_t k = threadIdx.x; k < nt; k += 256) {
+            const int64_t a = r0 + k * PX_TILE;
+            tl[(base[b] + k) * PX_XCD + x] = PXTile{a, a + PX_TILE < r1 ? a + PX_TILE : r1, b};
+        }
+    } else {                                            /* XCD x's padding */
+        const int64_t x = b - nparts, longest = base[nparts + PX_XCD];
+        for (int64_t i = base[nparts + x] + threadIdx.x; i < longest; i += 256)
+            tl[i * PX_XCD + x] = PXTile{0, 0, 0};
+    }
+}
+
 /* Histogram and scatter of one pass over the tiles at d_tiles (grids a
- * multiple of 8, so a tile's run is the XCD its workgroup runs on). */
+ * multiple of 8, so a tile's run is the XCD its workgroup runs on); dn: the
+ * tile count on the device (a list laid out there), ntiles its bound. */
 template <bool ROWS, bool HK>
 static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
-                     const PXTile* d_tiles, int ntiles, int shift, int PL, int XR, uint32_t* C,
-                     int64_t* O, unsigned long long* cur, int64_t nrun, int64_t nout) {
+                     const PXTile* d_tiles, int ntiles, const int* dn, int shift, int PL, int XR,
+                     uint32_t* C, int64_t* O, unsigned long long* cur, int64_t nrun, int64_t nout) {
     /* rows of one 4- or 8-byte key and 8-byte value columns: plain loads */
     int ke = 0;
     if (ROWS && S.P.n_keys == 1 && S.P.keys[0].data) {
@@ -661,7 +720,7 @@ static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
     int rc = tdd_fill(C, 0, (size_t)nrun * 4);
     if (rc) return rc;
 #define PXH(KE_) LAUNCH("group_pg_hist", (k_px_hist<ROWS, HK, KE_>), gh, PG_T, 0, S, in, d_tiles, \
-                        ntiles, shift, PL, XR, C)
+                        ntiles, dn, shift, PL, XR, C)
     if (ROWS && hke == 8) PXH((ROWS ? 8 : 0));
     else if (ROWS && hke == 4) PXH((ROWS ? 4 : 0));
     else PXH(0);
@@ -671,7 +730,7 @@ static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
     HC(hipMemcpyAsync(cur, O, (size_t)nrun * 8, hipMemcpyDeviceToDevice, tdd::g_stream));
     const int gp = grid8(resident_grid(k_px_pass<ROWS, HK, 2, 0>, PG_T, 0, ntiles));
 #define PXP1(NV_, KE_) LAUNCH("group_pg_pass", (k_px_pass<ROWS, HK, NV_, KE_>), gp, PG_T, 0, S, in, \
-                              out, d_tiles, ntiles, shift, PL, XR, cur, nout)
+                              out, d_tiles, ntiles, dn, shift, PL, XR, cur, nout)
 #define PXP(NV_)                                          \
     do {                                                  \
         if (ROWS && ke == 8) PXP1(NV_, (ROWS ? 8 : 0));   \
@@ -788,48 +847,53 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
     int64_t parents = 1;
     std::vector<int64_t> O;
     int XR = PX_XCD;
+    /* pass 1's tiles from the host; later passes' lists are laid out on the
+     * device from the finished pass's run starts, with no host round trip */
+    int64_t items = 0;
+    for (const PXTile& T : tl) items += T.r1 - T.r0;
+    Scratch dtl(sizeof(PXTile) * (tl.empty() ? 1 : tl.size()));
+    Scratch dn(8);
+    if (!dtl.ok() || !dn.ok()) return TD_ERR_OOM;
+    if (!tl.empty())
+        HC(hipMemcpyAsync(dtl.p, tl.data(), sizeof(PXTile) * tl.size(), hipMemcpyHostToDevice,
+                          tdd::g_stream));
+    int ntiles = (int)tl.size();
+    const int* dcount = nullptr;                      /* pass 1: the host's count */
     for (int pass = 0; pass < npass; pass++) {
         POLL_CANCEL();
         const int PL = 1 << pbits[pass];
         shift -= pbits[pass];
         XR = pass == 0 ? PX_XCD : 1;
         const int64_t nrun = parents * PL * XR;
-        const int ntiles = (int)tl.size();
-        SCRATCH(dtl, sizeof(PXTile) * (size_t)(ntiles > 0 ? ntiles : 1));
         SCRATCH(C, (size_t)nrun * 4);
-        SCRATCH(Od, ((size_t)nrun + 1) * 8);
+        Scratch Od(((size_t)nrun + 1) * 8);
         SCRATCH(cur, (size_t)nrun * 8);
-        if (ntiles > 0)
-            HC(hipMemcpyAsync(dtl.p, tl.data(), sizeof(PXTile) * (size_t)ntiles,
-                              hipMemcpyHostToDevice, tdd::g_stream));
+        if (!Od.ok()) return TD_ERR_OOM;
         const bool rows = pass == 0 && !first;
-        int rc = rows ? px_launch<true, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, shift, PL,
-                                            XR, C.as<uint32_t>(), Od.as<int64_t>(),
+        int rc = rows ? px_launch<true, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, dcount,
+                                            shift, PL, XR, C.as<uint32_t>(), Od.as<int64_t>(),
                                             cur.as<unsigned long long>(), nrun, nrows)
-                      : px_launch<false, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, shift, PL,
-                                             XR, C.as<uint32_t>(), Od.as<int64_t>(),
+                      : px_launch<false, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, dcount,
+                                             shift, PL, XR, C.as<uint32_t>(), Od.as<int64_t>(),
                                              cur.as<unsigned long long>(), nrun, nrows);
         if (rc) return rc;
-        O.resize((size_t)nrun + 1);
-        HC(hipMemcpyAsync(O.data(), Od.p, O.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
-        HC(hipStreamSynchronize(tdd::g_stream));
         const int64_t nparts = parents * PL;
         if (pass + 1 < npass) {
-            /* the next pass: the tiles of partition p on XCD p mod 8, tile t
-             * of the list on XCD t mod 8 (empty tiles pad the shorter lists) */
-            size_t nx[PX_XCD] = {}, longest = 0;
-            for (int64_t p = 0; p < nparts; p++)
-                nx[p % PX_XCD] += (size_t)ceil_div(O[(size_t)((p + 1) * XR)] - O[(size_t)(p * XR)],
-                                                   PX_TILE);
-            for (int x = 0; x < PX_XCD; x++) longest = nx[x] > longest ? nx[x] : longest;
-            tl.assign(longest * PX_XCD, PXTile{0, 0, 0});
-            size_t at[PX_XCD] = {};
-            for (int64_t p = 0; p < nparts; p++) {
-                const int x = (int)(p % PX_XCD);
-                const int64_t e = O[(size_t)((p + 1) * XR)];
-                for (int64_t r = O[(size_t)(p * XR)]; r < e; r += PX_TILE)
-                    tl[at[x]++ * PX_XCD + (size_t)x] = PXTile{r, r + PX_TILE < e ? r + PX_TILE : e, p};
-            }
+            /* bound of the next list: every XCD list at most the items' tiles
+             * plus one partial tile per parent */
+            const int64_t bound = (ceil_div(items, (int64_t)PX_TILE) + nparts) * PX_XCD;
+            Scratch base(((size_t)nparts + PX_XCD + 1) * 8);
+            if (!base.ok() || !dtl.reset(sizeof(PXTile) * (size_t)bound)) return TD_ERR_OOM;
+            LAUNCH("group_pg_tiles", k_px_tscan, 1, 1024, 0, Od.as<int64_t>(), nparts, XR,
+                   base.as<int64_t>(), dn.as<int>());
+            LAUNCH("group_pg_tiles", k_px_twrite, (int)(nparts + PX_XCD), 256, 0,
+                   Od.as<int64_t>(), nparts, XR, base.as<int64_t>(), dtl.as<PXTile>());
+            ntiles = (int)bound;
+            dcount = dn.as<int>();
+        } else {
+            O.resize((size_t)nrun + 1);
+            HC(hipMemcpyAsync(O.data(), Od.p, O.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
+            HC(hipStreamSynchronize(tdd::g_stream));
         }
         parents = nparts;
         std::swap(in, outb);
