@@ -16,7 +16,7 @@
  *
  *   k_srv_map / k_srv_jmap   dim: direct map pk -> group of each key, and
  *                  the joint map (every key's group per pk) (once, at open)
- *   k_srv_cpass / k_srv_cfold   fact chunk -> cubes by slices of groups:
+ *   k_srv_ccount / k_srv_cput / k_srv_cfold   fact chunk -> cubes by slices:
  *                  cells counted, records placed in slice order, each slice
  *                  folded in LDS and written with its prefix / suffix sums
  *   k_srv_cube + k_srv_cumul   the same with global atomics per row and key,
@@ -261,12 +261,14 @@ __global__ __launch_bounds__(256) void k_srv_cube(SrvScanArgs S) {
  * writes the prefix / suffix sums over the values (what k_srv_cumul does
  * after k_srv_cube) and the raw cube only when a request reads it.
  *
- *   k_srv_cpass<0>  rows of workgroup b's span: one gather of the joint map
+ *   k_srv_ccount    rows of workgroup b's span: one gather of the joint map
  *                   row (every served key's group of a dim row), each active
  *                   key's cell (g * Q + v) kept per row, counts per (slice, b)
  *   scan            counts -> record positions
- *   k_srv_cpass<1>  the same rows again: records (cell within its slice,
- *                   the key's value words) at the positions
+ *   k_srv_cput      the same rows again: records (cell within its slice,
+ *                   the key's value words) at the positions, each tile's
+ *                   records counting-sorted by slice in LDS and written
+ *                   slice run by slice run
  *   k_srv_cfold     one workgroup per slice: its records (one segment per
  *                   row-pass workgroup) folded into an LDS table of the
  *                   slice's cells, then per group the sums over v out
@@ -334,18 +336,16 @@ __device__ __forceinline__ int32_t srv_pick(int4 v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
 }
 
-template <int PUT>
-__global__ __launch_bounds__(1024) void k_srv_cpass(SrvScanArgs S, SrvCubePlan C) {
-    extern __shared__ uint32_t lh[];          /* per slice: counts, or (PUT) cursors */
+__global__ __launch_bounds__(1024) void k_srv_ccount(SrvScanArgs S, SrvCubePlan C) {
+    extern __shared__ uint32_t lh[];          /* per slice: the workgroup's counts */
     const int64_t b = blockIdx.x;
     const int64_t s0 = S.r0 + b * C.span;
     const int64_t s1 = s0 + C.span < S.r1 ? s0 + C.span : S.r1;
-    for (int i = threadIdx.x; i < C.ns; i += 1024)
-        lh[i] = PUT ? (uint32_t)C.O[srv_cs_at(C, srv_cs_key(C, i), i, b)] : 0u;
+    for (int i = threadIdx.x; i < C.ns; i += 1024) lh[i] = 0u;
     __syncthreads();
     for (int64_t b0 = s0; b0 < s1; b0 += 1024 * SRV_R) {
         const int64_t base = b0 + (int64_t)threadIdx.x * SRV_R;
-        if (!PUT) {
+        {
             int64_t fk[SRV_R], q[SRV_R];
             load_rows<SRV_R>(S.fk, base, s1, S.vec & 1u, fk);
             load_rows<SRV_R>(S.pcol, base, s1, (S.vec >> 1) & 1u, q);
@@ -385,38 +385,120 @@ __global__ __launch_bounds__(1024) void k_srv_cpass(SrvScanArgs S, SrvCubePlan C
                     C.cells[kk][base + i - S.r0] = cell;
                 }
             }
-        } else {
-            int64_t v[TDD_SRV_MAX_VALS][SRV_R];
-#pragma unroll
-            for (int j = 0; j < TDD_SRV_MAX_VALS; j++) {
-                if (S.vals[j].data) {
-                    load_rows<SRV_R>(S.vals[j], base, s1, (S.vec >> (2 + j)) & 1u, v[j]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < SRV_R; i++) v[j][i] = 0;
-                }
-            }
-            for (int kk = 0; kk < C.nk; kk++) {
-                if (!C.on[kk]) continue;
-                const int64_t sl = (int64_t)C.gw[kk] * C.Q;
-#pragma unroll
-                for (int i = 0; i < SRV_R; i++) {
-                    if (base + i >= s1) continue;
-                    const uint32_t cell = C.cells[kk][base + i - S.r0];
-                    if (cell == SRV_CS_NONE) continue;
-                    const uint32_t at = atomicAdd(&lh[C.sbase[kk] + (int64_t)cell / sl], 1u);
-                    C.roff[at] = (uint16_t)((int64_t)cell % sl);
-#pragma unroll
-                    for (int j = 0; j < TDD_SRV_MAX_VALS; j++)
-                        if ((C.vmask[kk] >> j) & 1) C.rval[j][at] = (uint64_t)v[j][i];
-                }
-            }
         }
     }
-    if (!PUT) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < C.ns; i += 1024)
-            C.C[srv_cs_at(C, srv_cs_key(C, i), i, b)] = lh[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < C.ns; i += 1024) C.C[srv_cs_at(C, srv_cs_key(C, i), i, b)] = lh[i];
+}
+
+/* Exclusive scan of a[0, n) in place (n <= 8 * 1024), a[n] = the total;
+ * every thread of a 1,024-thread block calls it. */
+__device__ __forceinline__ void srv_block_scan(uint32_t* a, int n, uint32_t* wsum) {
+    const int per = (n + 1023) / 1024;
+    const int i0 = threadIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+    uint32_t t = 0;
+    for (int i = i0; i < i1; i++) t += a[i];
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    uint32_t inc = t;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    uint32_t run = before + inc - t;
+    for (int i = i0; i < i1; i++) {
+        const uint32_t c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) a[n] = run;                  /* the last thread ends at the total */
+    __syncthreads();
+}
+
+/* The records of the sliced cube, placed through LDS: per tile of 4,096
+ * rows and per active key, the tile's records are counting-sorted by slice
+ * in LDS and written slice by slice, so the records of one slice from one
+ * tile go out as one run by consecutive lanes (for a cube of ~256 slices,
+ * ~16 records a run) instead of one store per record. */
+__global__ __launch_bounds__(1024) void k_srv_cput(SrvScanArgs S, SrvCubePlan C, int nskmax) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* lh = lds;                                   /* ns: the workgroup's cursors */
+    uint32_t* tof = lh + C.ns;                            /* nsk + 1: the tile's offsets */
+    uint16_t* spos = (uint16_t*)(tof + nskmax + 1);       /* 4,096: slice of sorted record */
+    uint16_t* sroff = spos + 4096;                        /* 4,096: its cell in the slice */
+    uint64_t* sval = (uint64_t*)(((uintptr_t)(sroff + 4096) + 7) & ~(uintptr_t)7);
+    __shared__ uint32_t wsum[1024 / WAVE];
+    const int64_t b = blockIdx.x;
+    const int64_t s0 = S.r0 + b * C.span;
+    const int64_t s1 = s0 + C.span < S.r1 ? s0 + C.span : S.r1;
+    for (int i = threadIdx.x; i < C.ns; i += 1024)
+        lh[i] = (uint32_t)C.O[srv_cs_at(C, srv_cs_key(C, i), i, b)];
+    __syncthreads();
+    for (int64_t b0 = s0; b0 < s1; b0 += 1024 * SRV_R) {
+        const int64_t base = b0 + (int64_t)threadIdx.x * SRV_R;
+        int64_t v[TDD_SRV_MAX_VALS][SRV_R];
+#pragma unroll
+        for (int j = 0; j < TDD_SRV_MAX_VALS; j++) {
+            if (S.vals[j].data) {
+                load_rows<SRV_R>(S.vals[j], base, s1, (S.vec >> (2 + j)) & 1u, v[j]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < SRV_R; i++) v[j][i] = 0;
+            }
+        }
+        for (int kk = 0; kk < C.nk; kk++) {
+            if (!C.on[kk]) continue;
+            const int sb = C.sbase[kk], nsk = C.sbase[kk + 1] - sb;
+            const int64_t sl = (int64_t)C.gw[kk] * C.Q;
+            for (int i = threadIdx.x; i <= nsk; i += 1024) tof[i] = 0;
+            __syncthreads();
+            int s[SRV_R];
+            uint32_t rank[SRV_R], off[SRV_R];
+#pragma unroll
+            for (int i = 0; i < SRV_R; i++) {
+                s[i] = -1;
+                if (base + i >= s1) continue;
+                const uint32_t cell = C.cells[kk][base + i - S.r0];
+                if (cell == SRV_CS_NONE) continue;
+                s[i] = (int)((int64_t)cell / sl);
+                off[i] = (uint32_t)((int64_t)cell % sl);
+                rank[i] = atomicAdd(&tof[s[i]], 1u);
+            }
+            __syncthreads();
+            srv_block_scan(tof, nsk, wsum);
+#pragma unroll
+            for (int i = 0; i < SRV_R; i++) {
+                if (s[i] < 0) continue;
+                rank[i] += tof[s[i]];                     /* sorted place in the tile */
+                spos[rank[i]] = (uint16_t)s[i];
+                sroff[rank[i]] = (uint16_t)off[i];
+            }
+            __syncthreads();
+            const int n = (int)tof[nsk];
+            for (int j = threadIdx.x; j < n; j += 1024) {
+                const int sj = spos[j];
+                C.roff[lh[sb + sj] + (uint32_t)j - tof[sj]] = sroff[j];
+            }
+#pragma unroll
+            for (int q = 0; q < TDD_SRV_MAX_VALS; q++) {
+                if (!((C.vmask[kk] >> q) & 1)) continue;
+                __syncthreads();                          /* the previous word is out */
+#pragma unroll
+                for (int i = 0; i < SRV_R; i++)
+                    if (s[i] >= 0) sval[rank[i]] = (uint64_t)v[q][i];
+                __syncthreads();
+                for (int j = threadIdx.x; j < n; j += 1024) {
+                    const int sj = spos[j];
+                    C.rval[q][lh[sb + sj] + (uint32_t)j - tof[sj]] = sval[j];
+                }
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < nsk; i += 1024) lh[sb + i] += tof[i + 1] - tof[i];
+            __syncthreads();
+        }
     }
 }
 
@@ -1388,6 +1470,12 @@ static int srv_cube_sliced(tdd_srv_t* s, const SrvScanArgs& A, uint32_t ftype,
     }
     P.sbase[A.nk] = ns;
     if (ns == 0) return TD_OK;
+    /* k_srv_cput's LDS: cursors, one key's tile offsets, 4,096 sorted records */
+    int nskmax = 0;
+    for (int k = 0; k < A.nk; k++)
+        if (P.sbase[k + 1] - P.sbase[k] > nskmax) nskmax = P.sbase[k + 1] - P.sbase[k];
+    const size_t put_lds = (size_t)ns * 4 + ((size_t)nskmax + 1) * 4 + 4096 * 4 + 8 + 4096 * 8;
+    if (put_lds + 256 > (size_t)tdd::g_lds_block) return TD_ERR_NYI;
     const int64_t wgmax = TUNE_INT("TEIDE_SRV_CS_WG", 256);
     int64_t nwg = ceil_div(ceil_div(rows, 4096), 8) * 8;
     if (nwg > wgmax) nwg = wgmax;
@@ -1402,10 +1490,10 @@ static int srv_cube_sliced(tdd_srv_t* s, const SrvScanArgs& A, uint32_t ftype,
     const int64_t nrec = s->cs_rows * s->cs_nk;
     for (int j = 0; j < TDD_SRV_MAX_VALS; j++)
         P.rval[j] = j < s->spec.n_vals ? s->cs_rval + j * nrec : nullptr;
-    LAUNCH("serve_cube", k_srv_cpass<0>, (int)nwg, 1024, (size_t)ns * 4, A, P);
+    LAUNCH("serve_cube", k_srv_ccount, (int)nwg, 1024, (size_t)ns * 4, A, P);
     int rc = tdd::scan_u32_i64(s->cs_C, (int64_t)ns * nwg, s->cs_O);
     if (rc) return rc;
-    LAUNCH("serve_cube", k_srv_cpass<1>, (int)nwg, 1024, (size_t)ns * 4, A, P);
+    LAUNCH("serve_cube", k_srv_cput, (int)nwg, 1024, put_lds, A, P, nskmax);
     LAUNCH("serve_cube", k_srv_cfold, ns, 512, fold_lds, P);
     return TD_OK;
 }
