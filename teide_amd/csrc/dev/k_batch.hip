@@ -563,11 +563,50 @@ __global__ __launch_bounds__(256) void k_bh_scan(const BQuery* Q, const uint32_t
 
 /* Rows of a tile to their partitions' records: LDS ranks, one reservation
  * per (tile, partition). */
+/* Exclusive scan of a[0, n) in place by the NT threads of a block; returns
+ * the total to every thread. */
+template <int NT>
+__device__ __forceinline__ uint32_t bh_block_scan(uint32_t* a, int n, uint32_t* wsum) {
+    const int per = (n + NT - 1) / NT;
+    const int i0 = threadIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+    uint32_t t = 0;
+    for (int i = i0; i < i1; i++) t += a[i];
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    uint32_t inc = t;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < NT / WAVE; k++) {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    uint32_t run = before + inc - t;
+    for (int i = i0; i < i1; i++) {
+        const uint32_t c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+/* Records of one side in partition order.  Per tile of BH_ST rows the
+ * records are counting-sorted by partition in LDS, each partition's part
+ * of its run is reserved with one atomic, and the records go out one array
+ * at a time through an LDS stage, so consecutive lanes write consecutive
+ * addresses of a partition's run. */
 __global__ __launch_bounds__(BH_T) void k_bh_scatter(const BQuery* Q, const BTile* tiles,
                                                      int64_t n_tiles, int side,
                                                      unsigned long long* cur, BHRec R) {
-    __shared__ uint32_t h[1 << BH_MAXB];
+    __shared__ uint32_t h[1 << BH_MAXB];             /* counts, then tile offsets */
     __shared__ unsigned long long hb[1 << BH_MAXB];
+    __shared__ uint16_t spart[BH_ST];
+    __shared__ uint64_t stage[BH_ST];
+    __shared__ uint32_t wsum[BH_T / WAVE];
     constexpr int K = BH_ST / BH_T;
     for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const BTile W = tiles[t];
@@ -593,20 +632,47 @@ __global__ __launch_bounds__(BH_T) void k_bh_scatter(const BQuery* Q, const BTil
         for (int i = threadIdx.x; i < nb; i += BH_T)
             hb[i] = h[i] ? atomicAdd(&cur[off + i], (unsigned long long)h[i]) : 0ull;
         __syncthreads();
+        const int n = (int)bh_block_scan<BH_T>(h, nb, wsum);
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            if (!((live >> j) & 1)) continue;
-            const int64_t r = W.r0 + threadIdx.x + (int64_t)j * BH_T;
-            const int64_t at = (int64_t)hb[part[j]] + rank[j];
-            if (side == 0) {
-                int64_t slot = 0;
-                for (int k = 0; k < q.P.n_keys; k++)
-                    slot += (ld_i64(q.P.keys[k], r) - q.P.kmin[k]) * q.P.kstride[k];
-                R.bkey[at] = key[j];
-                R.bslot[at] = (int32_t)slot;
-            } else {
-                R.pkey[at] = key[j];
-                for (int w = 0; w < q.nrw; w++) R.pval[w][at] = (uint64_t)ld_raw(q.T.col[q.rw_slot[w]], r);
+        for (int j = 0; j < K; j++)
+            if ((live >> j) & 1) {
+                rank[j] += h[part[j]];                      /* sorted place in the tile */
+                spart[rank[j]] = (uint16_t)part[j];
+                stage[rank[j]] = key[j];
+            }
+        __syncthreads();
+        uint64_t* okey = side ? R.pkey : R.bkey;
+        for (int i = threadIdx.x; i < n; i += BH_T) {
+            const int p = spart[i];
+            okey[hb[p] + (uint32_t)i - h[p]] = stage[i];
+        }
+        const int nw = side ? q.nrw : 1;
+        for (int w = 0; w < nw; w++) {
+            uint64_t v[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {                   /* the word's loads first */
+                v[j] = 0;
+                if (!((live >> j) & 1)) continue;
+                const int64_t r = W.r0 + threadIdx.x + (int64_t)j * BH_T;
+                if (side == 0) {
+                    int64_t slot = 0;
+                    for (int k = 0; k < q.P.n_keys; k++)
+                        slot += (ld_i64(q.P.keys[k], r) - q.P.kmin[k]) * q.P.kstride[k];
+                    v[j] = (uint64_t)slot;
+                } else {
+                    v[j] = (uint64_t)ld_raw(q.T.col[q.rw_slot[w]], r);
+                }
+            }
+            __syncthreads();                                /* the previous array is out */
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                if ((live >> j) & 1) stage[rank[j]] = v[j];
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += BH_T) {
+                const int p = spart[i];
+                const int64_t at = (int64_t)hb[p] + (uint32_t)i - h[p];
+                if (side == 0) R.bslot[at] = (int32_t)stage[i];
+                else R.pval[w][at] = stage[i];
             }
         }
         __syncthreads();
