@@ -34,10 +34,12 @@ def tables(eng):
     flag = rng.integers(0, 4, nf)
     wide = rng.integers(0, 2_000_000, nd)              # 2 M-group key range
     mid = rng.integers(0, 300_000, nd)                 # 300 K groups x 100 qty values
+    c7 = (cat % 7).astype(np.int64)
     HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64, flag=flag, wide=wide,
-                mid=mid)
+                mid=mid, c7=c7)
     tf = eng.table([("fk", fk), ("qty", qty), ("price", price), ("flag", flag)])
-    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide), ("mid", mid)])
+    td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide), ("mid", mid),
+                    ("c7", c7)])
     rf, rd = eng.put(tf), eng.put(td)
     eng.release(tf)
     eng.release(td)
@@ -108,7 +110,7 @@ def check_oracle(eng, got, r, what):
     m = CMP_NP[r["cmp"]](h["qty"], r["t"]) if r.get("filtered", True) else \
         np.ones(len(h["qty"]), bool)
     li, ri = orc.join([h["fk"][m]], [h["pk"]], 0)
-    src = {"price": orc.gather(h["price"][m], li), "qty": orc.gather(h["qty"][m], li)}
+    src = {c: orc.gather(h[c][m], li) for c in ("price", "qty", "flag")}
     ops = [o for o, _ in r["aggs"]]
     ref = orc.group([orc.gather(h[r["key"]], ri)], ops, [src[c] for _, c in r["aggs"]])
     a = eng.read_table(got)
@@ -198,6 +200,43 @@ def test_serve_large_cube_steps(eng, tables):
     run_to_end(eng, h, results)
     info = eng.serve_info(h)
     assert info["alone"] == 0 and info["shared"] == len(ids), info
+    for rid, r in ids.items():
+        check_oracle(eng, results[rid], r, f"request {rid} {r} vs oracle")
+        eng.release(results[rid])
+    eng.serve_close(h)
+    for g in dags:
+        g.free()
+
+
+def test_serve_three_keys_two_values(eng, tables):
+    """Three served keys (the joint map's 4-wide rows) and two value columns
+    (two value words per cube cell): cube requests over every key, with
+    SUM / AVG of both value columns and every comparison, match the oracle
+    and all share the scan."""
+    rf, rd = tables
+    h = eng.serve_open(rf, rd, "fk", "pk", "qty", ["price", "flag"], ["cat", "k64", "c7"],
+                       chunk_rows=40960)
+    aggs = [[("SUM", "price"), ("SUM", "flag"), ("COUNT", "price")],
+            [("AVG", "flag"), ("SUM", "qty"), ("COUNT", "qty")],
+            [("SUM", "flag")],
+            [("AVG", "price"), ("AVG", "flag"), ("AVG", "qty")]]
+    rng = np.random.default_rng(34)
+    reqs = [dict(key=k, cmp=CMPS[i % len(CMPS)], t=int(rng.integers(1, 101)),
+                 aggs=aggs[i % len(aggs)])
+            for i, k in enumerate(["cat", "k64", "c7"] * 4)]
+    dags, ids, results = [], {}, {}
+    for i, r in enumerate(reqs):
+        g, root = request_dag(eng, rf, rd, r["key"], r["cmp"], r["t"], r["aggs"])
+        ids[eng.serve_submit(h, g, root)] = r
+        dags.append(g)
+        if i % 4 == 3:                                 # later requests join mid-scan
+            eng.serve_step(h)
+            for rid, res in eng.serve_poll(h):
+                results[rid] = res
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 0 and info["shared"] == len(reqs), info
+    assert sorted(results) == sorted(ids)
     for rid, r in ids.items():
         check_oracle(eng, results[rid], r, f"request {rid} {r} vs oracle")
         eng.release(results[rid])
@@ -296,7 +335,7 @@ def check_wide_oracle(eng, got, r, what):
     h = HOST
     m = WIDE[r["w"]][2](h, r["t"])
     li, ri = orc.join([h["fk"][m]], [h["pk"]], 0)
-    src = {"price": orc.gather(h["price"][m], li), "qty": orc.gather(h["qty"][m], li)}
+    src = {c: orc.gather(h[c][m], li) for c in ("price", "qty", "flag")}
     ops = [o for o, _ in r["aggs"]]
     ref = orc.group([orc.gather(h[r["key"]], ri)], ops, [src[c] for _, c in r["aggs"]])
     a = eng.read_table(got)
