@@ -472,11 +472,23 @@ __global__ void k_nonempty(const uint64_t* tab, int64_t n_slots, int ncells, uin
 }
 
 /* Direct-array emit: keys decoded from the slot, in ascending slot order. */
-__global__ void k_da_emit(GroupPlan P, const uint64_t* tab, int64_t n_slots, const int64_t* pos,
-                          tdd_group_result_t out) {
+/* A block's slots: their cells (row-major) through LDS with consecutive
+ * lanes on consecutive words, then each lane emits its slot from there. */
+__device__ __forceinline__ const uint64_t* emit_stage(uint64_t* lc, const uint64_t* tab,
+                                                      int64_t n_slots, int nc) {
+    const int64_t s0 = (int64_t)blockIdx.x * 256;
+    const int64_t ns = n_slots - s0 < 256 ? n_slots - s0 : 256;
+    for (int64_t w = threadIdx.x; w < ns * nc; w += 256) lc[w] = tab[s0 * nc + w];
+    __syncthreads();
+    return lc + (int64_t)threadIdx.x * nc;
+}
+
+__global__ __launch_bounds__(256) void k_da_emit(GroupPlan P, const uint64_t* tab, int64_t n_slots,
+                                                 const int64_t* pos, tdd_group_result_t out) {
+    extern __shared__ uint64_t lc[];              /* 256 x ncells */
+    const uint64_t* c = emit_stage(lc, tab, n_slots, P.ncells);
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_slots) return;
-    const uint64_t* c = tab + s * P.ncells;
     if (c[0] == 0) return;
     int64_t g = pos[s];
     for (int k = 0; k < P.n_keys; k++) {
@@ -501,7 +513,7 @@ int da_emit(const GroupPlan& P, const uint64_t* tab, int64_t n_slots,
     rc = alloc_group_outputs(P, ng, out);
     if (rc) return rc;
     if (ng > 0)
-        LAUNCH("group_emit", k_da_emit, blocks, 256, 0, P, tab, n_slots,
+        LAUNCH("group_emit", k_da_emit, blocks, 256, (size_t)256 * P.ncells * 8, P, tab, n_slots,
                (const int64_t*)pos.as<int64_t>(), *out);
     return TD_OK;
 }
@@ -680,20 +692,32 @@ __device__ __forceinline__ void ht_emit_key(const GroupPlan& P, uint64_t key, in
 }
 
 /* Groups of the list: entry i is output row i. */
-__global__ void k_ht_emit_list(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
-                               int64_t n, tdd_group_result_t out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+/* The list's groups in list order; a block's cells (row-major, ncells words
+ * a group) come in through LDS with consecutive lanes on consecutive words,
+ * then each lane emits its group from there. */
+__global__ __launch_bounds__(256) void k_ht_emit_list(GroupPlan P, const unsigned long long* keys,
+                                                      const uint64_t* cells, int64_t n,
+                                                      tdd_group_result_t out) {
+    extern __shared__ uint64_t lc[];              /* 256 x ncells */
+    const int64_t g0 = (int64_t)blockIdx.x * 256;
+    const int64_t ng = n - g0 < 256 ? n - g0 : 256;
+    const int nc = P.ncells;
+    for (int64_t w = threadIdx.x; w < ng * nc; w += 256) lc[w] = cells[g0 * nc + w];
+    __syncthreads();
+    const int64_t i = g0 + threadIdx.x;
     if (i >= n) return;
     ht_emit_key(P, keys[i], i, out);
-    for (int a = 0; a < P.n_aggs; a++) cells_emit(cells + i * P.ncells, P, a, out.vals[a], i);
+    for (int a = 0; a < P.n_aggs; a++) cells_emit(lc + (int64_t)threadIdx.x * nc, P, a, out.vals[a], i);
 }
 
-__global__ void k_ht_emit(GroupPlan P, const unsigned long long* keys, const uint64_t* cells,
-                          int64_t n_slots, const int64_t* pos, int64_t off,
-                          tdd_group_result_t out) {
+__global__ __launch_bounds__(256) void k_ht_emit(GroupPlan P, const unsigned long long* keys,
+                                                 const uint64_t* cells, int64_t n_slots,
+                                                 const int64_t* pos, int64_t off,
+                                                 tdd_group_result_t out) {
+    extern __shared__ uint64_t lc[];              /* 256 x ncells */
+    const uint64_t* c = emit_stage(lc, cells, n_slots, P.ncells);
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_slots) return;
-    const uint64_t* c = cells + s * P.ncells;
     if (c[0] == 0) return;
     int64_t g = off + pos[s];
     uint64_t key = s == n_slots - 1 ? EMPTY_KEY : keys[s];
@@ -851,11 +875,12 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         rc = alloc_group_outputs(P, ng, out);
         if (rc) return rc;
         if (n_list > 0)
-            LAUNCH("group_emit", k_ht_emit_list, (int)ceil_div(n_list, 256), 256, 0, P,
+            LAUNCH("group_emit", k_ht_emit_list, (int)ceil_div(n_list, 256), 256,
+                   (size_t)256 * P.ncells * 8, P,
                    (const unsigned long long*)lkeys.as<unsigned long long>(),
                    (const uint64_t*)lcells.as<uint64_t>(), n_list, *out);
         if (nh > 0)
-            LAUNCH("group_emit", k_ht_emit, blocks, 256, 0, P,
+            LAUNCH("group_emit", k_ht_emit, blocks, 256, (size_t)256 * P.ncells * 8, P,
                    (const unsigned long long*)keys.as<unsigned long long>(),
                    (const uint64_t*)cells.as<uint64_t>(), n_slots,
                    (const int64_t*)pos.as<int64_t>(), n_list, *out);
