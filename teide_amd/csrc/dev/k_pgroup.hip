@@ -443,7 +443,7 @@ __global__ __launch_bounds__(PG_T) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     extern __shared__ uint64_t sm[];
     unsigned long long* lk = (unsigned long long*)sm;
     uint64_t* lc = sm + lcap;
-    uint32_t* lidx = (uint32_t*)(lc + (int64_t)lcap * P.ncells);
+    uint32_t* linv = (uint32_t*)(lc + (int64_t)lcap * P.ncells);   /* list order -> entry */
     __shared__ uint32_t nloc;
     __shared__ unsigned long long lbase;
     __shared__ int8_t cw[GP_MAX_CELLS];
@@ -502,16 +502,19 @@ __global__ __launch_bounds__(PG_T) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
             if (threadIdx.x == 0) nloc = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < lcap; i += blockDim.x)
-                if (lk[i] != EMPTY_KEY && lc[(int64_t)i * nc] != 0) lidx[i] = atomicAdd(&nloc, 1u);
+                if (lk[i] != EMPTY_KEY && lc[(int64_t)i * nc] != 0) linv[atomicAdd(&nloc, 1u)] = i;
             __syncthreads();
             if (threadIdx.x == 0) lbase = atomicAdd(L.count, (unsigned long long)nloc);
             __syncthreads();
-            for (int i = threadIdx.x; i < lcap; i += blockDim.x) {
-                if (lk[i] == EMPTY_KEY || lc[(int64_t)i * nc] == 0) continue;
-                const int64_t at = (int64_t)(lbase + lidx[i]);
-                if (at >= L.cap) { overflow[1] = 1; continue; }   /* the list */
-                L.keys[at] = lk[i];
-                for (int c = 0; c < nc; c++) L.cells[at * nc + c] = lc[(int64_t)i * nc + c];
+            /* the job's groups are one run of the list: keys, then the cells
+             * word by word, consecutive lanes on consecutive words */
+            const int64_t n = lbase + nloc <= (unsigned long long)L.cap ? (int64_t)nloc
+                            : lbase < (unsigned long long)L.cap ? L.cap - (int64_t)lbase : 0;
+            if (threadIdx.x == 0 && n < (int64_t)nloc) overflow[1] = 1;   /* the list */
+            for (int64_t g = threadIdx.x; g < n; g += blockDim.x) L.keys[lbase + g] = lk[linv[g]];
+            for (int64_t w = threadIdx.x; w < n * nc; w += blockDim.x) {
+                const int64_t g = w / nc, c = w - g * nc;
+                L.cells[(int64_t)lbase * nc + w] = lc[(int64_t)linv[g] * nc + c];
             }
         } else {
             for (int i = threadIdx.x; i < lcap; i += blockDim.x) {
