@@ -857,10 +857,8 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
             LAUNCH("group_ht", k_ht_agg, grid, GT, lds, P, M, keys.as<unsigned long long>(),
                    (uint64_t)(gcap - 1), cells.as<uint64_t>(), ovf.as<int>());
         }
-        HC(hipMemcpyAsync(ol, ovf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
-        HC(hipStreamSynchronize(tdd::g_stream));
-        if (ol[0]) continue;                                 /* a table too small: retry */
-        const int64_t n_list = ph ? ol[1] : 0;
+        /* the table's occupied slots counted before the flags come back, so
+         * both come back with one wait (an overflowed try discards them) */
         SCRATCH(flag, (size_t)n_slots * 4);
         SCRATCH(pos, (size_t)(n_slots + 1) * 8);
         int blocks = (int)ceil_div(n_slots, 256);
@@ -869,8 +867,11 @@ static int run_ht(GroupPlan& P, const RowMask& M, int64_t n_rows_sel, int force,
         rc = tdd::scan_u32_i64(flag.as<uint32_t>(), n_slots, pos.as<int64_t>());
         if (rc) return rc;
         int64_t nh = 0;
+        HC(hipMemcpyAsync(ol, ovf.p, 16, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipMemcpyAsync(&nh, pos.as<int64_t>() + n_slots, 8, hipMemcpyDeviceToHost, tdd::g_stream));
         HC(hipStreamSynchronize(tdd::g_stream));
+        if (ol[0]) continue;                                 /* a table too small: retry */
+        const int64_t n_list = ph ? ol[1] : 0;
         const int64_t ng = n_list + nh;
         rc = alloc_group_outputs(P, ng, out);
         if (rc) return rc;

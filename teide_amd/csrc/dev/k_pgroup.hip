@@ -120,6 +120,13 @@ struct PXTile {
     int64_t parent;
 };
 
+struct PXFirst {                          /* pass-1 tiles laid out on the device */
+    const PXTile* tiles;
+    const int*    count;                  /* their number, on the device */
+    int64_t       bound;                  /* at most this many */
+    int64_t       items;                  /* at most this many records */
+};
+
 template <bool HK>
 __device__ __forceinline__ uint32_t px_digit(uint64_t key, int shift, int PL) {
     return (uint32_t)(pg_bits<HK>(key) >> shift) & (uint32_t)(PL - 1);
@@ -703,6 +710,37 @@ __global__ __launch_bounds__(256) void k_px_twrite(const int64_t* O, int64_t npa
     }
 }
 
+/* Pass-1 tiles over the hot fold's record regions (its workgroup b wrote
+ * counts[b] records from b * span on): one workgroup lays out every
+ * region's tiles back to back and their count. */
+__global__ __launch_bounds__(1024) void k_px_ftiles(const int64_t* counts, int nreg, int64_t span,
+                                                    PXTile* tl, int* ntiles) {
+    __shared__ int64_t v[1024];
+    __shared__ int64_t run;
+    if (threadIdx.x == 0) run = 0;
+    __syncthreads();
+    for (int c = 0; c < nreg; c += 1024) {
+        const int b = c + (int)threadIdx.x;
+        const int64_t cnt = b < nreg ? counts[b] : 0;
+        const int64_t nt = (cnt + PX_TILE - 1) / PX_TILE;
+        v[threadIdx.x] = nt;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {           /* inclusive scan of the tile counts */
+            const int64_t add = (int)threadIdx.x >= d ? v[threadIdx.x - d] : 0;
+            __syncthreads();
+            v[threadIdx.x] += add;
+            __syncthreads();
+        }
+        const int64_t base = run + v[threadIdx.x] - nt, r0 = (int64_t)b * span;
+        for (int64_t k = 0; k < nt; k++)
+            tl[base + k] = PXTile{r0 + k * PX_TILE, r0 + ((k + 1) * PX_TILE < cnt ? (k + 1) * PX_TILE : cnt), 0};
+        __syncthreads();
+        if (threadIdx.x == 1023) run += v[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *ntiles = (int)run;
+}
+
 /* Histogram and scatter of one pass over the tiles at d_tiles (grids a
  * multiple of 8, so a tile's run is the XCD its workgroup runs on); dn: the
  * tile count on the device (a list laid out there), ntiles its bound. */
@@ -836,15 +874,13 @@ static bool pg_alloc(PGBufs& m, int64_t n, int nv, bool hk, PGBuf& A, PGBuf& B) 
 template <bool HK>
 static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, int npass, int top,
                       const PGBuf& A, const PGBuf& B, std::vector<PGJob>& jobs, PGBuf& res,
-                      const std::vector<PGJob>* first = nullptr) {
-    /* pass-1 tiles: the rows, or the first pass's record ranges in A */
+                      const PXFirst* f0 = nullptr) {
+    /* pass-1 tiles: the rows, or (f0) tiles over records in A laid out on
+     * the device */
     std::vector<PXTile> tl;
-    auto add = [&](std::vector<PXTile>& v, int64_t r0, int64_t r1, int64_t parent) {
-        for (int64_t r = r0; r < r1; r += PX_TILE)
-            v.push_back(PXTile{r, r + PX_TILE < r1 ? r + PX_TILE : r1, parent});
-    };
-    if (first) for (const PGJob& J : *first) add(tl, J.r0, J.r1, 0);
-    else add(tl, 0, nrows, 0);
+    if (!f0)
+        for (int64_t r = 0; r < nrows; r += PX_TILE)
+            tl.push_back(PXTile{r, r + PX_TILE < nrows ? r + PX_TILE : nrows, 0});
     PGBuf in = A, outb = B;              /* the first pass reads rows, or A's records */
     int shift = top;
     int64_t parents = 1;
@@ -862,6 +898,13 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
                           tdd::g_stream));
     int ntiles = (int)tl.size();
     const int* dcount = nullptr;                      /* pass 1: the host's count */
+    const PXTile* t0 = dtl.as<PXTile>();
+    if (f0) {                                         /* pass 1: tiles laid out on the device */
+        t0 = f0->tiles;
+        ntiles = (int)f0->bound;
+        dcount = f0->count;
+        items = f0->items;
+    }
     for (int pass = 0; pass < npass; pass++) {
         POLL_CANCEL();
         const int PL = 1 << pbits[pass];
@@ -872,11 +915,12 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
         Scratch Od(((size_t)nrun + 1) * 8);
         SCRATCH(cur, (size_t)nrun * 8);
         if (!Od.ok()) return TD_ERR_OOM;
-        const bool rows = pass == 0 && !first;
-        int rc = rows ? px_launch<true, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, dcount,
+        const bool rows = pass == 0 && !f0;
+        const PXTile* tp = pass == 0 ? t0 : dtl.as<PXTile>();
+        int rc = rows ? px_launch<true, HK>(nv, S, in, outb, tp, ntiles, dcount,
                                             shift, PL, XR, C.as<uint32_t>(), Od.as<int64_t>(),
                                             cur.as<unsigned long long>(), nrun, nrows)
-                      : px_launch<false, HK>(nv, S, in, outb, dtl.as<PXTile>(), ntiles, dcount,
+                      : px_launch<false, HK>(nv, S, in, outb, tp, ntiles, dcount,
                                              shift, PL, XR, C.as<uint32_t>(), Od.as<int64_t>(),
                                              cur.as<unsigned long long>(), nrun, nrows);
         if (rc) return rc;
@@ -1003,12 +1047,13 @@ struct tdd_pht {
 /* Scatter the rows of a hash group-by into partitions of the key's hash
  * whose expected distinct keys (from the estimate) fill half of an LDS
  * table; TD_ERR_NYI when one LDS table already holds them all. */
-/* The hot-key fold of the rows into A (see k_pg_hotfold); *first receives
- * the first pass's jobs over the workgroups' record regions.  TD_ERR_NYI
- * when the LDS blocks do not fit. */
+/* The hot-key fold of the rows into A (see k_pg_hotfold); *f0 receives the
+ * first pass's tiles over the workgroups' record regions, laid out on the
+ * device (tiles and count in ftl / fcnt).  TD_ERR_NYI when the LDS blocks
+ * do not fit. */
 static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t* vf,
                         const unsigned long long* hot, int nhot, uint64_t* hot_cells,
-                        const PGBuf& A, std::vector<PGJob>* first, int64_t* cold) {
+                        const PGBuf& A, Scratch& ftl, Scratch& fcnt, PXFirst* f0) {
     const GroupPlan& P = S.P;
     const size_t lds = (size_t)PH_SET * 8 + (size_t)PH_SET * 2 + (size_t)nhot * P.ncells * 8;
     if (lds > 64 * 1024) return TD_ERR_NYI;
@@ -1044,22 +1089,15 @@ static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t
     }
 #undef PGF
 #undef PGF1
-    std::vector<int64_t> c((size_t)grid);
-    HC(hipMemcpyAsync(c.data(), cnt.p, c.size() * 8, hipMemcpyDeviceToHost, tdd::g_stream));
-    HC(hipStreamSynchronize(tdd::g_stream));
-    first->clear();
-    *cold = 0;
-    for (int b = 0; b < grid; b++) *cold += c[(size_t)b];
-    for (int b = 0; b < grid; b++)
-        for (int64_t r = 0; r < c[(size_t)b]; r += PG_JOB) {
-            PGJob J;
-            J.r0 = (int64_t)b * wpb * 64 + r;
-            J.r1 = J.r0 + (c[(size_t)b] - r < PG_JOB ? c[(size_t)b] - r : PG_JOB);
-            J.parent = 0;
-            J.js = 0;
-            first->push_back(J);
-        }
-    for (PGJob& J : *first) J.nj = (int32_t)first->size();
+    /* workgroup b's region holds at most wpb * 64 records */
+    const int64_t bound = (int64_t)grid * ceil_div(wpb * 64, (int64_t)PX_TILE);
+    if (!ftl.reset(sizeof(PXTile) * (size_t)bound) || !fcnt.reset(8)) return TD_ERR_OOM;
+    LAUNCH("group_pg_tiles", k_px_ftiles, 1, 1024, 0, (const int64_t*)cnt.as<int64_t>(), grid,
+           wpb * 64, ftl.as<PXTile>(), fcnt.as<int>());
+    f0->tiles = ftl.as<PXTile>();
+    f0->count = fcnt.as<int>();
+    f0->bound = bound;
+    f0->items = S.nrows;
     return TD_OK;
 }
 
@@ -1095,19 +1133,24 @@ int pht_partition(const GroupPlan& P, const uint64_t* mask, int64_t nrows, doubl
     if (!pg_alloc(h->mem, nrows, h->nv, true, A, B)) { delete h; return TD_ERR_OOM; }
     int pbits[8], npass;
     pg_split_bits(bits, pbits, &npass);
-    std::vector<PGJob> first;
     bool from_records = false;
     h->rows = -1;                                   /* every selected row */
+    Scratch ftl(8), fcnt(8);
+    PXFirst f0;
+    memset(&f0, 0, sizeof(f0));
     if (nhot > 0) {
-        int frc = pht_hot_fold(S, h->nv, h->cw, h->vf, hot, nhot, hot_cells, A, &first,
-                               &h->rows);
+        int frc = pht_hot_fold(S, h->nv, h->cw, h->vf, hot, nhot, hot_cells, A, ftl, fcnt, &f0);
         if (frc && frc != TD_ERR_NYI) { delete h; return frc; }
         from_records = frc == 0;
     }
     if (nhot > 0 && !from_records) { delete h; return TD_ERR_NYI; }   /* the hot rows need the fold */
     int rc = pg_scatter<true>(S, h->nv, nrows, pbits, npass, 64, A, B, h->jobs, h->res,
-                              from_records ? &first : nullptr);
+                              from_records ? &f0 : nullptr);
     if (rc) { delete h; return rc; }
+    if (from_records) {                             /* the rows the passes carried: the cold ones */
+        h->rows = 0;
+        for (const PGJob& J : h->jobs) h->rows += J.r1 - J.r0;
+    }
     *out = h;
     return TD_OK;
 }
@@ -1151,8 +1194,7 @@ int pht_fold(tdd_pht* h, unsigned long long* gkeys, uint64_t gmask, uint64_t* gc
     default: PGH(4); break;
     }
 #undef PGH
-    HC(hipStreamSynchronize(tdd::g_stream));
-    return TD_OK;
+    return TD_OK;                                   /* the caller reads the flags back */
 }
 
 /* Keys the global table can expect from a partitioned fold: the parts
