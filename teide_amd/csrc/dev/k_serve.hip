@@ -41,10 +41,17 @@
  *                  request's predicate on them and folds the passing records
  *                  into LDS slices of the requests' tables, then adds the
  *                  slices into the tables (one workgroup per slice and
- *                  request: no atomics on the tables)
+ *                  request: no atomics on the tables).  A group of up to
+ *                  15 requests `col CMP t` (< <= > >=) on one column with
+ *                  one cell layout folds each record once, into the bucket
+ *                  of the thresholds below its value; request j then sums
+ *                  its buckets (those above or below its threshold)
  *   k_srv_direct   requests with more groups than 4,096 slices cover: every
  *                  row of the chunk folded with global atomics
  */
+#include <algorithm>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "dev_common.h"
@@ -136,12 +143,22 @@ struct SrvDirReq {                 /* one direct request of a step */
     int64_t    G;
     int32_t    key, ncells;
     int32_t    lds;                /* word offset of its LDS table, -1: the global table */
-    int32_t    pad;
+    int16_t    blo, bhi;           /* threshold group: the buckets it sums */
     uint8_t    kind[GP_MAX_CELLS];
     int8_t     src[GP_MAX_CELLS];  /* ADD cells: value column, -2 pcol */
 };
 
-struct SrvDirGroup { int32_t first, n, words, pad; };
+/* A group of direct requests folded by one workgroup.  tmode 1: every
+ * request is `col CMP cut` on one record column with one cell layout; a
+ * record goes to bucket b = the cuts below it (strict: cut < x, else
+ * cut <= x) and folds once into that bucket's table; request j sums
+ * buckets [blo, bhi] at the end. */
+#define SRV_TCUTS 15
+struct SrvDirGroup {
+    int32_t first, n, words, tmode;
+    int32_t m, strict, as_f64, pad;
+    int64_t cut[SRV_TCUTS];        /* ascending, distinct (f64: the bits) */
+};
 
 /* Request descriptors are read-only during a launch and uniform across a
  * workgroup: read through the constant address space they are scalar loads
@@ -713,6 +730,63 @@ __global__ __launch_bounds__(256) void k_srv_bfold(SrvFoldArgs F) {
     const int64_t end = beg + per < b1 ? beg + per : b1;
     const int64_t gbase = sl * F.sg;
     const int w = threadIdx.x / WAVE, lane = lane_id();
+    if (grp.tmode) {                              /* threshold buckets */
+        ConstReq& R0 = ((ConstReq*)F.reqs)[grp.first];
+        const tdd_col_t xc = R0.P.cols[R0.P.ins[0].a];
+        const int nc = R0.ncells;
+        const int64_t bw = (int64_t)F.sg * nc;   /* words of one bucket's table */
+        for (int64_t r = beg + threadIdx.x; r < end; r += blockDim.x) {
+            int b = 0;
+            if (grp.as_f64) {
+                const double x = ld_f64(xc, r);
+                if (x != x) continue;             /* NaN: no comparison holds */
+                for (int k = 0; k < grp.m; k++) {
+                    const double c = __longlong_as_double((long long)F.groups[q].cut[k]);
+                    b += grp.strict ? c < x : c <= x;
+                }
+            } else {
+                const int64_t x = ld_i64(xc, r);
+                for (int k = 0; k < grp.m; k++) {
+                    const int64_t c = F.groups[q].cut[k];
+                    b += grp.strict ? c < x : c <= x;
+                }
+            }
+            uint64_t* c = lt + b * bw + ((int64_t)F.rg[r] - gbase) * nc;
+            atomicAdd((unsigned long long*)&c[0], 1ull);
+            for (int cc = 1; cc < nc; cc++) {
+                const tdd_col_t col = R0.P.cols[R0.src[cc]];
+                if (R0.kind[cc] == C_ADD_F64) atomicAdd((double*)&c[cc], ld_f64(col, r));
+                else atomicAdd((unsigned long long*)&c[cc], (unsigned long long)ld_i64(col, r));
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < grp.n; j++) {         /* request j: its buckets summed */
+            ConstReq& R = ((ConstReq*)F.reqs)[grp.first + j];
+            const int64_t n = gbase + F.sg <= R.G ? F.sg : R.G - gbase;
+            for (int64_t i = threadIdx.x; i < n * nc; i += blockDim.x) {
+                const bool f = R.kind[i % nc] == C_ADD_F64;
+                uint64_t x = 0;
+                double xf = 0.0;
+                for (int b = R.blo; b <= R.bhi; b++) {
+                    const uint64_t v = lt[b * bw + i];
+                    if (f) xf += __longlong_as_double((long long)v);
+                    else x += v;
+                }
+                if (f) x = (uint64_t)__double_as_longlong(xf);
+                if (!x) continue;
+                uint64_t* d = R.tab + gbase * nc + i;
+                if (F.parts > 1) {
+                    if (f) atomicAdd((double*)d, xf);
+                    else atomicAdd((unsigned long long*)d, (unsigned long long)x);
+                } else if (f) {
+                    *d = (uint64_t)__double_as_longlong(__longlong_as_double((long long)*d) + xf);
+                } else {
+                    *d += x;
+                }
+            }
+        }
+        return;
+    }
     for (int64_t t = beg + (int64_t)w * 256; t < end; t += 1024) {
         const int64_t r0 = t + lane;
         uint32_t g[4];
@@ -1234,7 +1308,7 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
             }
         return c;                                    /* not reached: req_cols lists them */
     };
-    for (const tdd_srv_acc_t* A : batch) {
+    auto make_req = [&](const tdd_srv_acc_t* A) {
         SrvDirReq R;
         memset(&R, 0, sizeof(R));
         R.P = A->pred;
@@ -1250,10 +1324,93 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
         R.key = key;
         R.ncells = A->a.ncells;
         memcpy(R.kind, A->a.kind, sizeof(R.kind));
+        return R;
+    };
+    /* requests `col CMP cut` (one comparison with a constant, < <= > >=)
+     * that share the column, the comparison's strictness class and the
+     * cell layout fold through threshold buckets, up to SRV_TCUTS a group */
+    std::map<std::string, std::vector<const tdd_srv_acc_t*>> fam;
+    std::vector<const tdd_srv_acc_t*> plain;
+    for (const tdd_srv_acc_t* A : batch) {
+        const tdd_pred_t& P = A->pred;
+        const tdd_pred_ins_t& in = P.ins[0];
+        const bool ok = !TUNE_INT("TEIDE_SRV_NO_TFOLD", 0) && P.n_ins == 1 &&
+                        in.kind == TDD_P_CMP_CONST &&
+                        (in.cmp == OP_GT || in.cmp == OP_GE || in.cmp == OP_LT || in.cmp == OP_LE) &&
+                        !(in.as_f64 && in.cf != in.cf);
+        if (!ok) { plain.push_back(A); continue; }
+        const int strict = in.cmp == OP_GT || in.cmp == OP_LE;
+        std::string k;
+        auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
+        put(&P.cols[in.a].data, sizeof(void*));
+        put(&P.cols[in.a].type, sizeof(P.cols[in.a].type));
+        put(&P.cols[in.a].attrs, sizeof(P.cols[in.a].attrs));
+        put(&in.as_f64, 1);
+        put(&strict, sizeof(strict));
+        put(&A->a.ncells, sizeof(A->a.ncells));
+        put(A->a.kind, (size_t)A->a.ncells);
+        put(A->a.src, (size_t)A->a.ncells);
+        fam[k].push_back(A);
+    }
+    for (const auto& kv : fam) {
+        const std::vector<const tdd_srv_acc_t*>& v = kv.second;
+        for (size_t i0 = 0; i0 < v.size(); i0 += SRV_TCUTS) {
+            const size_t n = v.size() - i0 < SRV_TCUTS ? v.size() - i0 : SRV_TCUTS;
+            const tdd_pred_ins_t& in0 = v[i0]->pred.ins[0];
+            SrvDirGroup gr;
+            memset(&gr, 0, sizeof(gr));
+            gr.first = (int32_t)reqs.size();
+            gr.n = (int32_t)n;
+            gr.tmode = 1;
+            gr.as_f64 = in0.as_f64;
+            gr.strict = in0.cmp == OP_GT || in0.cmp == OP_LE;
+            std::vector<int64_t> cuts;                /* ascending, distinct */
+            for (size_t j = 0; j < n; j++) {
+                const tdd_pred_ins_t& in = v[i0 + j]->pred.ins[0];
+                int64_t c;
+                if (gr.as_f64) memcpy(&c, &in.cf, 8);
+                else c = in.ci;
+                cuts.push_back(c);
+            }
+            auto less = [&](int64_t a, int64_t b) {
+                if (!gr.as_f64) return a < b;
+                double x, y;
+                memcpy(&x, &a, 8);
+                memcpy(&y, &b, 8);
+                return x < y;
+            };
+            std::sort(cuts.begin(), cuts.end(), less);
+            cuts.erase(std::unique(cuts.begin(), cuts.end(),
+                                   [&](int64_t a, int64_t b) { return !less(a, b) && !less(b, a); }),
+                       cuts.end());
+            gr.m = (int32_t)cuts.size();
+            for (int k = 0; k < gr.m; k++) gr.cut[k] = cuts[(size_t)k];
+            gr.words = (int32_t)((gr.m + 1) * sg * v[i0]->a.ncells);
+            for (size_t j = 0; j < n; j++) {
+                const tdd_pred_ins_t& in = v[i0 + j]->pred.ins[0];
+                int64_t c;
+                if (gr.as_f64) memcpy(&c, &in.cf, 8);
+                else c = in.ci;
+                const int ci = (int)(std::lower_bound(cuts.begin(), cuts.end(), c, less) - cuts.begin());
+                SrvDirReq R = make_req(v[i0 + j]);
+                const bool up = in.cmp == OP_GT || in.cmp == OP_GE;   /* passes above the cut */
+                R.blo = (int16_t)(up ? ci + 1 : 0);
+                R.bhi = (int16_t)(up ? gr.m : ci);
+                R.lds = 0;
+                reqs.push_back(R);
+            }
+            groups.push_back(gr);
+        }
+    }
+    for (const tdd_srv_acc_t* A : plain) {
+        SrvDirReq R = make_req(A);
         const int32_t words = (int32_t)(sg * R.ncells);
-        SrvDirGroup* gp = groups.empty() ? nullptr : &groups.back();
+        SrvDirGroup* gp = groups.empty() || groups.back().tmode ? nullptr : &groups.back();
         if (!gp || gp->n == SRV_DIR_GRP || (int64_t)(gp->words + words) * 8 > SRV_DIR_LDS) {
-            groups.push_back({(int32_t)reqs.size(), 0, 0, 0});
+            SrvDirGroup ng;
+            memset(&ng, 0, sizeof(ng));
+            ng.first = (int32_t)reqs.size();
+            groups.push_back(ng);
             gp = &groups.back();
         }
         R.lds = gp->words;

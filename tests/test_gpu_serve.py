@@ -35,9 +35,12 @@ def tables(eng):
     wide = rng.integers(0, 2_000_000, nd)              # 2 M-group key range
     mid = rng.integers(0, 300_000, nd)                 # 300 K groups x 100 qty values
     c7 = (cat % 7).astype(np.int64)
+    pz = np.round(price, 0) - 20.0                     # ties, zeros of both signs, NaN
+    pz[rng.random(nf) < 0.01] = np.nan
+    pz[pz == 0.0] = np.where(rng.random(int((pz == 0.0).sum())) < 0.5, -0.0, 0.0)
     HOST.update(fk=fk, qty=qty, price=price, pk=pk, cat=cat, k64=k64, flag=flag, wide=wide,
-                mid=mid, c7=c7)
-    tf = eng.table([("fk", fk), ("qty", qty), ("price", price), ("flag", flag)])
+                mid=mid, c7=c7, pz=pz)
+    tf = eng.table([("fk", fk), ("qty", qty), ("price", price), ("flag", flag), ("pz", pz)])
     td = eng.table([("pk", pk), ("cat", cat), ("k64", k64), ("wide", wide), ("mid", mid),
                     ("c7", c7)])
     rf, rd = eng.put(tf), eng.put(td)
@@ -386,6 +389,49 @@ def test_serve_wide_filters_share_the_scan(eng, tables):
             check_oracle(eng, results[rid], r, what + " vs oracle")
         eng.release(results[rid])
         eng.release(ref)
+    eng.serve_close(h)
+    for g in dags:
+        g.free()
+
+
+def test_serve_threshold_buckets(eng, tables):
+    """Requests `pz CMP t` (< <= > >=) fold through threshold buckets: more
+    requests than one group holds, repeated thresholds, thresholds equal to
+    column values, +0.0 / -0.0 and NaN in the column; each matches the
+    oracle (NumPy's comparisons, false for NaN)."""
+    rf, rd = tables
+    cmps = ["gt", "ge", "lt", "le"]
+    ts = [-20.0, -5.0, -0.0, 0.0, 0.0, 3.0, 7.0, 7.0, 15.5, 40.0, 1e9, -1e9]
+    reqs = [dict(cmp=cmps[i % 4], t=t, key="cat" if i % 3 else "k64",
+                 aggs=AGG_SETS[0] if i % 5 else AGG_SETS[2])
+            for i, t in enumerate(ts * 3)]
+    h = open_server(eng, rf, rd)
+    dags, ids, results = [], {}, {}
+    for r in reqs:
+        g = eng.graph(rf)
+        pred = getattr(g, r["cmp"])(g.scan("pz"), g.const_f64(r["t"]))
+        src = g.filter(g.const_table(rf), pred)
+        g.join(src, [g.scan("fk")], g.const_table(rd), [g.scan("pk")], 0)
+        root = g.group([g.scan(r["key"])], [(o, g.scan(c)) for o, c in r["aggs"]])
+        ids[eng.serve_submit(h, g, root)] = r
+        dags.append(g)
+    run_to_end(eng, h, results)
+    info = eng.serve_info(h)
+    assert info["alone"] == 0 and info["shared"] == len(reqs), info
+    hh = HOST
+    for rid, r in ids.items():
+        m = CMP_NP[r["cmp"]](hh["pz"], r["t"])
+        li, ri = orc.join([hh["fk"][m]], [hh["pk"]], 0)
+        src = {c: orc.gather(hh[c][m], li) for c in ("price", "qty", "flag")}
+        ops = [o for o, _ in r["aggs"]]
+        ref = orc.group([orc.gather(hh[r["key"]], ri)], ops, [src[c] for _, c in r["aggs"]])
+        a = eng.read_table(results[rid])
+        what = f"request {rid} {r}"
+        assert [c[1] for c in a[1:]] == ref["out_types"], what
+        f64 = [t == 7 and o in ("SUM", "AVG") for t, o in zip(ref["out_types"], ops)]
+        compare_groups([a[0][3].astype(np.int64)], [c[3] for c in a[1:]], ref["keys"],
+                       ref["vals"], f64, what, exact=ref["comp"], sum_abs=ref["sum_abs"])
+        eng.release(results[rid])
     eng.serve_close(h)
     for g in dags:
         g.free()
