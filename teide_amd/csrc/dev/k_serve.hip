@@ -42,7 +42,7 @@
  *                  into LDS slices of the requests' tables, then adds the
  *                  slices into the tables (one workgroup per slice and
  *                  request: no atomics on the tables).  A group of up to
- *                  15 requests `col CMP t` (< <= > >=) on one column with
+ *                  63 requests `col CMP t` (< <= > >=) on one column with
  *                  one cell layout folds each record once, into the bucket
  *                  of the thresholds below its value; request j then sums
  *                  its buckets (those above or below its threshold)
@@ -153,7 +153,7 @@ struct SrvDirReq {                 /* one direct request of a step */
  * record goes to bucket b = the cuts below it (strict: cut < x, else
  * cut <= x) and folds once into that bucket's table; request j sums
  * buckets [blo, bhi] at the end. */
-#define SRV_TCUTS 15
+#define SRV_TCUTS 63
 struct SrvDirGroup {
     int32_t first, n, words, tmode;
     int32_t m, strict, as_f64, pad;
@@ -731,26 +731,32 @@ __global__ __launch_bounds__(256) void k_srv_bfold(SrvFoldArgs F) {
     const int64_t gbase = sl * F.sg;
     const int w = threadIdx.x / WAVE, lane = lane_id();
     if (grp.tmode) {                              /* threshold buckets */
+        __shared__ int64_t scut[SRV_TCUTS];
         ConstReq& R0 = ((ConstReq*)F.reqs)[grp.first];
         const tdd_col_t xc = R0.P.cols[R0.P.ins[0].a];
         const int nc = R0.ncells;
         const int64_t bw = (int64_t)F.sg * nc;   /* words of one bucket's table */
+        if ((int)threadIdx.x < grp.m) scut[threadIdx.x] = F.groups[q].cut[threadIdx.x];
+        __syncthreads();
         for (int64_t r = beg + threadIdx.x; r < end; r += blockDim.x) {
-            int b = 0;
+            int lo = 0, hi = grp.m;              /* b = the cuts below x */
             if (grp.as_f64) {
                 const double x = ld_f64(xc, r);
                 if (x != x) continue;             /* NaN: no comparison holds */
-                for (int k = 0; k < grp.m; k++) {
-                    const double c = __longlong_as_double((long long)F.groups[q].cut[k]);
-                    b += grp.strict ? c < x : c <= x;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    const double c = __longlong_as_double((long long)scut[mid]);
+                    if (grp.strict ? c < x : c <= x) lo = mid + 1; else hi = mid;
                 }
             } else {
                 const int64_t x = ld_i64(xc, r);
-                for (int k = 0; k < grp.m; k++) {
-                    const int64_t c = F.groups[q].cut[k];
-                    b += grp.strict ? c < x : c <= x;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    const int64_t c = scut[mid];
+                    if (grp.strict ? c < x : c <= x) lo = mid + 1; else hi = mid;
                 }
             }
+            const int b = lo;
             uint64_t* c = lt + b * bw + ((int64_t)F.rg[r] - gbase) * nc;
             atomicAdd((unsigned long long*)&c[0], 1ull);
             for (int cc = 1; cc < nc; cc++) {
@@ -1265,7 +1271,43 @@ static int64_t slice_groups(int ncells) {
 static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc_t*>& batch,
                       const std::vector<tdd_col_t>& cols, int ncmax, int64_t r0, int64_t r1) {
     const tdd_srv_spec_t& S = s->spec;
-    const int64_t G = s->G[key], sg = slice_groups(ncmax), nsl = ceil_div(G, sg);
+    const int64_t G = s->G[key];
+    /* requests `col CMP cut` (one comparison with a constant, < <= > >=)
+     * that share the column, the comparison's strictness class and the
+     * cell layout fold through threshold buckets, up to SRV_TCUTS a group */
+    std::map<std::string, std::vector<const tdd_srv_acc_t*>> fam;
+    std::vector<const tdd_srv_acc_t*> plain;
+    for (const tdd_srv_acc_t* A : batch) {
+        const tdd_pred_t& P = A->pred;
+        const tdd_pred_ins_t& in = P.ins[0];
+        const bool ok = !TUNE_INT("TEIDE_SRV_NO_TFOLD", 0) && P.n_ins == 1 &&
+                        in.kind == TDD_P_CMP_CONST &&
+                        (in.cmp == OP_GT || in.cmp == OP_GE || in.cmp == OP_LT || in.cmp == OP_LE) &&
+                        !(in.as_f64 && in.cf != in.cf);
+        if (!ok) { plain.push_back(A); continue; }
+        const int strict = in.cmp == OP_GT || in.cmp == OP_LE;
+        std::string k;
+        auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
+        put(&P.cols[in.a].data, sizeof(void*));
+        put(&P.cols[in.a].type, sizeof(P.cols[in.a].type));
+        put(&P.cols[in.a].attrs, sizeof(P.cols[in.a].attrs));
+        put(&in.as_f64, 1);
+        put(&strict, sizeof(strict));
+        put(&A->a.ncells, sizeof(A->a.ncells));
+        put(A->a.kind, (size_t)A->a.ncells);
+        put(A->a.src, (size_t)A->a.ncells);
+        fam[k].push_back(A);
+    }
+    /* threshold groups hold one bucket table per threshold + 1 in LDS: 1 KB
+     * slices of a request's table let a group take 63 thresholds (4 KB: 15) */
+    int64_t sg = slice_groups(ncmax);
+    if (!fam.empty()) {
+        const int64_t sg1 = sg / 4 > 0 ? sg / 4 : 1;
+        if (ceil_div(G, sg1) <= SRV_MAX_S) sg = sg1;
+    }
+    const int64_t tcuts = (SRV_DIR_LDS / (sg * ncmax * 8)) - 1 < SRV_TCUTS
+                              ? (SRV_DIR_LDS / (sg * ncmax * 8)) - 1 : SRV_TCUTS;
+    const int64_t nsl = ceil_div(G, sg);
     const int64_t rows = r1 - r0, ntiles = ceil_div(rows, SRV_TILE);
     SCRATCH(hist, (size_t)(nsl * ntiles) * 4);
     SCRATCH(off, (size_t)(nsl * ntiles + 1) * 8);
@@ -1326,36 +1368,10 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
         memcpy(R.kind, A->a.kind, sizeof(R.kind));
         return R;
     };
-    /* requests `col CMP cut` (one comparison with a constant, < <= > >=)
-     * that share the column, the comparison's strictness class and the
-     * cell layout fold through threshold buckets, up to SRV_TCUTS a group */
-    std::map<std::string, std::vector<const tdd_srv_acc_t*>> fam;
-    std::vector<const tdd_srv_acc_t*> plain;
-    for (const tdd_srv_acc_t* A : batch) {
-        const tdd_pred_t& P = A->pred;
-        const tdd_pred_ins_t& in = P.ins[0];
-        const bool ok = !TUNE_INT("TEIDE_SRV_NO_TFOLD", 0) && P.n_ins == 1 &&
-                        in.kind == TDD_P_CMP_CONST &&
-                        (in.cmp == OP_GT || in.cmp == OP_GE || in.cmp == OP_LT || in.cmp == OP_LE) &&
-                        !(in.as_f64 && in.cf != in.cf);
-        if (!ok) { plain.push_back(A); continue; }
-        const int strict = in.cmp == OP_GT || in.cmp == OP_LE;
-        std::string k;
-        auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
-        put(&P.cols[in.a].data, sizeof(void*));
-        put(&P.cols[in.a].type, sizeof(P.cols[in.a].type));
-        put(&P.cols[in.a].attrs, sizeof(P.cols[in.a].attrs));
-        put(&in.as_f64, 1);
-        put(&strict, sizeof(strict));
-        put(&A->a.ncells, sizeof(A->a.ncells));
-        put(A->a.kind, (size_t)A->a.ncells);
-        put(A->a.src, (size_t)A->a.ncells);
-        fam[k].push_back(A);
-    }
     for (const auto& kv : fam) {
         const std::vector<const tdd_srv_acc_t*>& v = kv.second;
-        for (size_t i0 = 0; i0 < v.size(); i0 += SRV_TCUTS) {
-            const size_t n = v.size() - i0 < SRV_TCUTS ? v.size() - i0 : SRV_TCUTS;
+        for (size_t i0 = 0; i0 < v.size(); i0 += (size_t)tcuts) {
+            const size_t n = v.size() - i0 < (size_t)tcuts ? v.size() - i0 : (size_t)tcuts;
             const tdd_pred_ins_t& in0 = v[i0]->pred.ins[0];
             SrvDirGroup gr;
             memset(&gr, 0, sizeof(gr));

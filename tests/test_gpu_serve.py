@@ -405,6 +405,9 @@ def test_serve_threshold_buckets(eng, tables):
     reqs = [dict(cmp=cmps[i % 4], t=t, key="cat" if i % 3 else "k64",
                  aggs=AGG_SETS[0] if i % 5 else AGG_SETS[2])
             for i, t in enumerate(ts * 3)]
+    # one family of more requests than a group's thresholds (63): two groups
+    reqs += [dict(cmp="gt", t=float(t), key="cat", aggs=AGG_SETS[0])
+             for t in np.arange(-30.0, 40.0, 1.0)]
     h = open_server(eng, rf, rd)
     dags, ids, results = [], {}, {}
     for r in reqs:
