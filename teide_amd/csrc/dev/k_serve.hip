@@ -666,33 +666,106 @@ __global__ __launch_bounds__(256) void k_srv_bhist(SrvBucket B) {
     for (int i = threadIdx.x; i < B.S; i += blockDim.x) B.hist[(int64_t)i * B.ntiles + t] = h[i];
 }
 
-template <class T>
-__device__ __forceinline__ void rec_put(const SrvBucket& B, int c, int64_t r, int64_t at) {
-    ((T*)B.rc[c])[at] = ((const T*)B.src[c].data)[r];
+/* Records of a tile at their slice's run (a slice's records in no
+ * particular order within the tile: the folds are sums).  The tile's rows
+ * are counting-sorted by slice in LDS and each record array goes out
+ * through an LDS stage, so a slice's records from the tile are written as
+ * one run by consecutive lanes. */
+#define SRV_BR (SRV_TILE / 256)                   /* rows per thread of a tile */
+__device__ __forceinline__ uint64_t rec_get(const tdd_col_t& c, int64_t r) {
+    switch (c.esz) {
+    case 8: return ((const uint64_t*)c.data)[r];
+    case 4: return ((const uint32_t*)c.data)[r];
+    case 2: return ((const uint16_t*)c.data)[r];
+    default: return ((const uint8_t*)c.data)[r];
+    }
 }
 
-/* Records of a tile at their slice's cursor (a slice's records in no
- * particular order within the tile: the folds are sums). */
+__device__ __forceinline__ void rec_set(void* d, int esz, int64_t at, uint64_t v) {
+    switch (esz) {
+    case 8: ((uint64_t*)d)[at] = v; break;
+    case 4: ((uint32_t*)d)[at] = (uint32_t)v; break;
+    case 2: ((uint16_t*)d)[at] = (uint16_t)v; break;
+    default: ((uint8_t*)d)[at] = (uint8_t)v; break;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_srv_bscatter(SrvBucket B) {
-    extern __shared__ unsigned long long cur[];
+    extern __shared__ uint64_t bsm[];
+    uint64_t* stage = bsm;                                /* SRV_TILE */
+    uint32_t* toff = (uint32_t*)(stage + SRV_TILE);       /* S + 1: tile offsets */
+    uint16_t* spos = (uint16_t*)(toff + B.S + 1);         /* SRV_TILE: slice of record i */
+    __shared__ uint32_t wsum[256 / WAVE];
     const int64_t t = blockIdx.x, lo = B.r0 + t * SRV_TILE;
     const int64_t hi = lo + SRV_TILE < B.r1 ? lo + SRV_TILE : B.r1;
-    for (int i = threadIdx.x; i < B.S; i += blockDim.x)
-        cur[i] = (unsigned long long)B.off[(int64_t)i * B.ntiles + t];
+    for (int i = threadIdx.x; i <= B.S; i += blockDim.x) toff[i] = 0;
     __syncthreads();
-    for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-        int32_t g;
-        const int32_t sl = bucket_slice(B, r, &g);
-        if (sl < 0) continue;
-        const int64_t at = (int64_t)atomicAdd(&cur[sl], 1ull);
-        B.rg[at] = (uint32_t)g;
-        for (int c = 0; c < B.nc; c++) {
-            switch (B.src[c].esz) {
-            case 8: rec_put<uint64_t>(B, c, r, at); break;
-            case 4: rec_put<uint32_t>(B, c, r, at); break;
-            case 2: rec_put<uint16_t>(B, c, r, at); break;
-            default: rec_put<uint8_t>(B, c, r, at); break;
-            }
+    int32_t sl[SRV_BR], g[SRV_BR];
+    uint32_t rk[SRV_BR];
+#pragma unroll
+    for (int k = 0; k < SRV_BR; k++) {
+        const int64_t r = lo + threadIdx.x + (int64_t)k * 256;
+        sl[k] = -1;
+        if (r < hi) sl[k] = bucket_slice(B, r, &g[k]);
+        rk[k] = sl[k] >= 0 ? atomicAdd(&toff[sl[k]], 1u) : 0u;
+    }
+    __syncthreads();
+    /* exclusive scan of the tile's counts (S <= 4,096: 16 a thread) */
+    {
+        const int per = (B.S + 255) / 256;
+        const int i0 = threadIdx.x * per, i1 = i0 + per < B.S ? i0 + per : B.S;
+        uint32_t tsum = 0;
+        for (int i = i0; i < i1; i++) tsum += toff[i];
+        const int lane = lane_id(), w = threadIdx.x / WAVE;
+        uint32_t inc = tsum;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == WAVE - 1) wsum[w] = inc;
+        __syncthreads();
+        uint32_t run = inc - tsum;
+        for (int k = 0; k < w; k++) run += wsum[k];
+        for (int i = i0; i < i1; i++) {
+            const uint32_t c = toff[i];
+            toff[i] = run;
+            run += c;
+        }
+        __syncthreads();
+    }
+    uint32_t n = 0;
+    for (int k = 0; k < 256 / WAVE; k++) n += wsum[k];
+#pragma unroll
+    for (int k = 0; k < SRV_BR; k++)
+        if (sl[k] >= 0) {
+            rk[k] += toff[sl[k]];                         /* sorted place in the tile */
+            spos[rk[k]] = (uint16_t)sl[k];
+        }
+    __syncthreads();
+    int64_t at[SRV_BR];                                   /* sorted record i = tid + 256 j */
+#pragma unroll
+    for (int j = 0; j < SRV_BR; j++) {
+        const uint32_t i = threadIdx.x + 256u * j;
+        at[j] = -1;
+        if (i < n) {
+            const int s = spos[i];
+            at[j] = B.off[(int64_t)s * B.ntiles + t] + (i - toff[s]);
+        }
+    }
+    for (int c = -1; c < B.nc; c++) {                     /* the group, then each column */
+        __syncthreads();                                  /* the previous array is out */
+#pragma unroll
+        for (int k = 0; k < SRV_BR; k++)
+            if (sl[k] >= 0)
+                stage[rk[k]] = c < 0 ? (uint64_t)(uint32_t)g[k]
+                                     : rec_get(B.src[c], lo + threadIdx.x + (int64_t)k * 256);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SRV_BR; j++) {
+            if (at[j] < 0) continue;
+            const uint64_t v = stage[threadIdx.x + 256u * j];
+            if (c < 0) B.rg[at[j]] = (uint32_t)v;
+            else rec_set(B.rc[c], B.src[c].esz, at[j], v);
         }
     }
 }
@@ -1337,7 +1410,8 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
     LAUNCH("serve_bucket", k_srv_bhist, (int)ntiles, 256, (size_t)nsl * 4, B);
     int rcode = tdd::scan_u32_i64(hist.as<uint32_t>(), nsl * ntiles, off.as<int64_t>());
     if (rcode) return rcode;
-    LAUNCH("serve_bucket", k_srv_bscatter, (int)ntiles, 256, (size_t)nsl * 8, B);
+    LAUNCH("serve_bucket", k_srv_bscatter, (int)ntiles, 256,
+           (size_t)SRV_TILE * 8 + ((size_t)nsl + 1) * 4 + (size_t)SRV_TILE * 2, B);
     /* the requests' programs over the record columns; sums read record columns */
     std::vector<SrvDirReq> reqs;
     std::vector<SrvDirGroup> groups;
