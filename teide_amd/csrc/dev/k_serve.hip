@@ -42,10 +42,10 @@
  *                  into LDS slices of the requests' tables, then adds the
  *                  slices into the tables (one workgroup per slice and
  *                  request: no atomics on the tables).  A group of up to
- *                  63 requests `col CMP t` (< <= > >=) on one column with
+ *                  77 requests `col CMP t` (< <= > >=) on one column with
  *                  one cell layout folds each record once, into the bucket
- *                  of the thresholds below its value; request j then sums
- *                  its buckets (those above or below its threshold)
+ *                  of the thresholds below its value; running sums over the
+ *                  buckets give each request the sum of its buckets
  *   k_srv_direct   requests with more groups than 4,096 slices cover: every
  *                  row of the chunk folded with global atomics
  */
@@ -151,12 +151,15 @@ struct SrvDirReq {                 /* one direct request of a step */
 /* A group of direct requests folded by one workgroup.  tmode 1: every
  * request is `col CMP cut` on one record column with one cell layout; a
  * record goes to bucket b = the cuts below it (strict: cut < x, else
- * cut <= x) and folds once into that bucket's table; request j sums
- * buckets [blo, bhi] at the end. */
-#define SRV_TCUTS 63
+ * cut <= x) and folds once into that bucket's table; at the end the
+ * tables are summed over the buckets from the top (up: > and >=) or from
+ * the bottom (< and <=) in place, and request j takes bucket blo (up) or
+ * bhi: the sum of its buckets [blo, bhi]. */
+#define SRV_TCUTS 127
+#define SRV_DIR_TLDS (78 * 1024)       /* LDS of a threshold group: two workgroups a CU */
 struct SrvDirGroup {
     int32_t first, n, words, tmode;
-    int32_t m, strict, as_f64, pad;
+    int32_t m, strict, as_f64, up;     /* up: every request passes above its cut */
     int64_t cut[SRV_TCUTS];        /* ascending, distinct (f64: the bits) */
 };
 
@@ -839,19 +842,37 @@ __global__ __launch_bounds__(256) void k_srv_bfold(SrvFoldArgs F) {
             }
         }
         __syncthreads();
-        for (int j = 0; j < grp.n; j++) {         /* request j: its buckets summed */
+        /* running sums over the buckets, in place: from the top when the
+         * requests pass above their cuts, else from the bottom */
+        const int64_t ns = gbase + F.sg <= R0.G ? F.sg : R0.G - gbase;
+        for (int64_t i = threadIdx.x; i < ns * nc; i += blockDim.x) {
+            const bool f = R0.kind[i % nc] == C_ADD_F64;
+            uint64_t a = 0;
+            double af = 0.0;
+            for (int k = 0; k <= grp.m; k++) {
+                uint64_t* p = lt + (grp.up ? grp.m - k : k) * bw + i;
+                if (f) {
+                    af += __longlong_as_double((long long)*p);
+                    *p = (uint64_t)__double_as_longlong(af);
+                } else {
+                    a += *p;
+                    *p = a;
+                }
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < grp.n; j++) {         /* request j: its buckets' sum */
             ConstReq& R = ((ConstReq*)F.reqs)[grp.first + j];
             const int64_t n = gbase + F.sg <= R.G ? F.sg : R.G - gbase;
+            const int64_t b0 = (int64_t)(grp.up ? R.blo : R.bhi) * bw;
             for (int64_t i = threadIdx.x; i < n * nc; i += blockDim.x) {
                 const bool f = R.kind[i % nc] == C_ADD_F64;
                 uint64_t x = 0;
                 double xf = 0.0;
-                for (int b = R.blo; b <= R.bhi; b++) {
-                    const uint64_t v = lt[b * bw + i];
-                    if (f) xf += __longlong_as_double((long long)v);
-                    else x += v;
+                if (grp.up ? R.blo <= grp.m : true) {
+                    x = lt[b0 + i];
+                    xf = __longlong_as_double((long long)x);
                 }
-                if (f) x = (uint64_t)__double_as_longlong(xf);
                 if (!x) continue;
                 uint64_t* d = R.tab + gbase * nc + i;
                 if (F.parts > 1) {
@@ -1359,6 +1380,7 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
                         !(in.as_f64 && in.cf != in.cf);
         if (!ok) { plain.push_back(A); continue; }
         const int strict = in.cmp == OP_GT || in.cmp == OP_LE;
+        const int up = in.cmp == OP_GT || in.cmp == OP_GE;
         std::string k;
         auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
         put(&P.cols[in.a].data, sizeof(void*));
@@ -1366,20 +1388,22 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
         put(&P.cols[in.a].attrs, sizeof(P.cols[in.a].attrs));
         put(&in.as_f64, 1);
         put(&strict, sizeof(strict));
+        put(&up, sizeof(up));
         put(&A->a.ncells, sizeof(A->a.ncells));
         put(A->a.kind, (size_t)A->a.ncells);
         put(A->a.src, (size_t)A->a.ncells);
         fam[k].push_back(A);
     }
     /* threshold groups hold one bucket table per threshold + 1 in LDS: 1 KB
-     * slices of a request's table let a group take 63 thresholds (4 KB: 15) */
+     * slices of a request's table let a group take 77 thresholds (4 KB: 18;
+     * 512 B slices, 153, measured no faster: twice the slices) */
     int64_t sg = slice_groups(ncmax);
     if (!fam.empty()) {
         const int64_t sg1 = sg / 4 > 0 ? sg / 4 : 1;
         if (ceil_div(G, sg1) <= SRV_MAX_S) sg = sg1;
     }
-    const int64_t tcuts = (SRV_DIR_LDS / (sg * ncmax * 8)) - 1 < SRV_TCUTS
-                              ? (SRV_DIR_LDS / (sg * ncmax * 8)) - 1 : SRV_TCUTS;
+    const int64_t tcuts = (SRV_DIR_TLDS / (sg * ncmax * 8)) - 1 < SRV_TCUTS
+                              ? (SRV_DIR_TLDS / (sg * ncmax * 8)) - 1 : SRV_TCUTS;
     const int64_t nsl = ceil_div(G, sg);
     const int64_t rows = r1 - r0, ntiles = ceil_div(rows, SRV_TILE);
     SCRATCH(hist, (size_t)(nsl * ntiles) * 4);
@@ -1454,6 +1478,7 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
             gr.tmode = 1;
             gr.as_f64 = in0.as_f64;
             gr.strict = in0.cmp == OP_GT || in0.cmp == OP_LE;
+            gr.up = in0.cmp == OP_GT || in0.cmp == OP_GE;
             std::vector<int64_t> cuts;                /* ascending, distinct */
             for (size_t j = 0; j < n; j++) {
                 const tdd_pred_ins_t& in = v[i0 + j]->pred.ins[0];
