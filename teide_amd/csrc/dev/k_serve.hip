@@ -183,6 +183,7 @@ struct SrvBucket {                 /* one key's bucket sort of a chunk */
     uint32_t*      hist;           /* [S][ntiles] */
     const int64_t* off;            /* exclusive scan of hist, + total */
     uint32_t*      rg;             /* record group */
+    int32_t*       rowg;           /* per chunk row: its group (-1: none), k_srv_bhist -> bscatter */
     void*          rc[SRV_REC_C];  /* record columns (the source's element size) */
 };
 
@@ -663,6 +664,7 @@ __global__ __launch_bounds__(256) void k_srv_bhist(SrvBucket B) {
     for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x) {
         int32_t g;
         const int32_t sl = bucket_slice(B, r, &g);
+        B.rowg[r - B.r0] = g;                             /* the scatter reads it back */
         if (sl >= 0) atomicAdd(&h[sl], 1u);
     }
     __syncthreads();
@@ -709,7 +711,10 @@ __global__ __launch_bounds__(256) void k_srv_bscatter(SrvBucket B) {
     for (int k = 0; k < SRV_BR; k++) {
         const int64_t r = lo + threadIdx.x + (int64_t)k * 256;
         sl[k] = -1;
-        if (r < hi) sl[k] = bucket_slice(B, r, &g[k]);
+        if (r < hi) {
+            g[k] = B.rowg[r - B.r0];
+            sl[k] = g[k] >= 0 ? g[k] / B.sg : -1;
+        }
         rk[k] = sl[k] >= 0 ? atomicAdd(&toff[sl[k]], 1u) : 0u;
     }
     __syncthreads();
@@ -1409,6 +1414,7 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
     SCRATCH(hist, (size_t)(nsl * ntiles) * 4);
     SCRATCH(off, (size_t)(nsl * ntiles + 1) * 8);
     SCRATCH(rg, (size_t)rows * 4);
+    SCRATCH(rowg, (size_t)rows * 4);
     SrvBucket B;
     memset(&B, 0, sizeof(B));
     B.fk = S.fk;
@@ -1424,6 +1430,7 @@ static int srv_bucket(tdd_srv_t* s, int key, const std::vector<const tdd_srv_acc
     B.hist = hist.as<uint32_t>();
     B.off = off.as<int64_t>();
     B.rg = rg.as<uint32_t>();
+    B.rowg = rowg.as<int32_t>();
     Scratch rc[SRV_REC_C] = {Scratch(8), Scratch(8), Scratch(8), Scratch(8),
                              Scratch(8), Scratch(8), Scratch(8), Scratch(8)};
     for (int c = 0; c < B.nc; c++) {
