@@ -31,7 +31,10 @@
 #define PG_MAXB  8                        /* bits per pass: up to 256 sub-partitions */
 #define PG_MAXP  (1 << PG_MAXB)
 #define PG_MAXV  4                        /* value words per record */
-#define PG_JOB   ((int64_t)1 << 17)       /* items per job */
+#define PG_JOB   ((int64_t)1 << 17)       /* items per job (hash leaf) */
+#define PG_JOB_DA ((int64_t)1 << 20)      /* items per job (direct-array leaf): a part of
+                                             fewer items is one job, which stores its
+                                             slots instead of merging them with atomics */
 #define PG_U     4                        /* records per thread per leaf step */
 
 struct PGBuf {                            /* records of one pass (SoA) */
@@ -418,10 +421,18 @@ __global__ __launch_bounds__(PG_T) void k_pg_agg(GroupPlan P, PGBuf in, const PG
             pg_fold_rows<PG_U, NV>(lt, P, cw, vf, loc, live, v);
         }
         __syncthreads();
-        for (int64_t s = threadIdx.x; s < nl; s += blockDim.x) {
-            const uint64_t* c = &lt[s * nc];
-            if (c[0] == 0 || (int64_t)base + s >= n_slots) continue;
-            cells_merge(&tab[((int64_t)base + s) * nc], c, P);
+        if (J.nj == 1) {                       /* the part's only job: its slots are its own */
+            for (int64_t w = threadIdx.x; w < nl * nc; w += blockDim.x) {
+                const int64_t s = w / nc;
+                if (lt[s * nc] == 0 || (int64_t)base + s >= n_slots) continue;
+                tab[(int64_t)base * nc + w] = lt[w];
+            }
+        } else {
+            for (int64_t s = threadIdx.x; s < nl; s += blockDim.x) {
+                const uint64_t* c = &lt[s * nc];
+                if (c[0] == 0 || (int64_t)base + s >= n_slots) continue;
+                cells_merge(&tab[((int64_t)base + s) * nc], c, P);
+            }
         }
         __syncthreads();
     }
@@ -793,19 +804,19 @@ static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
 /* Jobs over the partitions of a finished pass: partition q = parent * PL + s
  * holds items [start(q), start(q) + count(q)). */
 static void pg_jobs_from(const std::vector<int64_t>& start, const std::vector<int64_t>& count,
-                         std::vector<PGJob>& jobs) {
+                         std::vector<PGJob>& jobs, int64_t per) {
     size_t total = 0;
-    for (size_t q = 0; q < start.size(); q++) total += (size_t)ceil_div(count[q], PG_JOB);
+    for (size_t q = 0; q < start.size(); q++) total += (size_t)ceil_div(count[q], per);
     jobs.resize(total);
     PGJob* J = jobs.data();
     for (size_t q = 0; q < start.size(); q++) {
         if (count[q] == 0) continue;
         const int js = (int)(J - jobs.data());
-        const int nj = (int)ceil_div(count[q], PG_JOB);
+        const int nj = (int)ceil_div(count[q], per);
         const int64_t end = start[q] + count[q];
         for (int k = 0; k < nj; k++, J++) {
-            J->r0 = start[q] + (int64_t)k * PG_JOB;
-            J->r1 = J->r0 + PG_JOB < end ? J->r0 + PG_JOB : end;
+            J->r0 = start[q] + (int64_t)k * per;
+            J->r1 = J->r0 + per < end ? J->r0 + per : end;
             J->parent = (int64_t)q;
             J->js = js;
             J->nj = nj;
@@ -951,7 +962,7 @@ static int pg_scatter(const PGRows& S, int nv, int64_t nrows, const int* pbits, 
         start[(size_t)q] = O[(size_t)(q * XR)];
         count[(size_t)q] = O[(size_t)((q + 1) * XR)] - start[(size_t)q];
     }
-    pg_jobs_from(start, count, jobs);
+    pg_jobs_from(start, count, jobs, HK ? PG_JOB : PG_JOB_DA);
     res = in;
     return TD_OK;
 }
