@@ -642,9 +642,39 @@ __global__ __launch_bounds__(PG_T) void k_pg_hotfold(PGRows S, const unsigned lo
         __syncthreads();
     }
     if (threadIdx.x == 0) counts[blockIdx.x] = cursor;
-    for (int h = threadIdx.x; h < nhot; h += PG_T)
-        if (cells[(int64_t)h * nc] != 0)
-            cells_merge(&gcells[(int64_t)h * nc], &cells[(int64_t)h * nc], P);
+    /* this workgroup's hot cells, row b of the partials (k_pg_hotreduce
+     * merges them: 1,024 workgroups merging into the same few thousand
+     * cells with atomics queue on those addresses) */
+    uint64_t* part = gcells + (int64_t)blockIdx.x * nhot * nc;
+    for (int64_t w = threadIdx.x; w < (int64_t)nhot * nc; w += PG_T) part[w] = cells[w];
+}
+
+/* Hot cells: partials [nwg][nhot * ncells] folded per cell over a chunk of
+ * workgroups (blockIdx.y), then merged into the hot cells with one atomic
+ * per (chunk, cell). */
+__global__ __launch_bounds__(256) void k_pg_hotreduce(GroupPlan P, const uint64_t* part, int nwg,
+                                                      int64_t ncell, int per, uint64_t* gcells) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ncell) return;
+    const int kind = P.kind[i % P.ncells];
+    const int w0 = blockIdx.y * per, w1 = w0 + per < nwg ? w0 + per : nwg;
+    uint64_t a = cell_identity(kind);
+    double af = 0.0;
+    for (int w = w0; w < w1; w++) {
+        const uint64_t s = part[(int64_t)w * ncell + i];
+        switch (kind) {
+        case C_ADD_F64: case C_SQ_F64: af += __longlong_as_double((long long)s); break;
+        case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW: a = (int64_t)s < (int64_t)a ? s : a; break;
+        case C_MAX_I64: case C_MAX_F64: case C_MAX_ROW: a = (int64_t)s > (int64_t)a ? s : a; break;
+        default: a += s; break;                              /* count, integer sums */
+        }
+    }
+    switch (kind) {
+    case C_ADD_F64: case C_SQ_F64: atomicAdd((double*)&gcells[i], af); break;
+    case C_MIN_I64: case C_MIN_F64: case C_MIN_ROW: cell_min(&gcells[i], (int64_t)a); break;
+    case C_MAX_I64: case C_MAX_F64: case C_MAX_ROW: cell_max(&gcells[i], (int64_t)a); break;
+    default: atomicAdd((unsigned long long*)&gcells[i], (unsigned long long)a); break;
+    }
 }
 
 /* ------------------------------------------------------------------ host -- */
@@ -1083,8 +1113,9 @@ static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t
     const int64_t wpb = ceil_div(ceil_div(nw, want), PH_STEP) * PH_STEP;
     const int grid = (int)ceil_div(nw, wpb);
     SCRATCH(cnt, (size_t)grid * 8);
+    SCRATCH(hpart, (size_t)grid * nhot * P.ncells * 8);   /* per workgroup hot cells */
 #define PGF1(NV_, KE_) LAUNCH("group_hot_fold", (k_pg_hotfold<NV_, KE_>), grid, PG_T, lds, S, hot, \
-                              nhot, dcw, dvf, wpb, A, cnt.as<int64_t>(), hot_cells)
+                              nhot, dcw, dvf, wpb, A, cnt.as<int64_t>(), hpart.as<uint64_t>())
 #define PGF(NV_)                                      \
     do {                                              \
         if (ke == 8) PGF1(NV_, 8);                    \
@@ -1100,6 +1131,13 @@ static int pht_hot_fold(const PGRows& S, int nv, const int8_t* cw, const uint8_t
     }
 #undef PGF
 #undef PGF1
+    {
+        const int64_t ncell = (int64_t)nhot * P.ncells;
+        const int per = 64;
+        dim3 rg((unsigned)ceil_div(ncell, 256), (unsigned)ceil_div(grid, per));
+        LAUNCH("group_hot_fold", k_pg_hotreduce, rg, 256, 0, P, (const uint64_t*)hpart.as<uint64_t>(),
+               grid, ncell, per, hot_cells);
+    }
     /* workgroup b's region holds at most wpb * 64 records */
     const int64_t bound = (int64_t)grid * ceil_div(wpb * 64, (int64_t)PX_TILE);
     if (!ftl.reset(sizeof(PXTile) * (size_t)bound) || !fcnt.reset(8)) return TD_ERR_OOM;
