@@ -169,10 +169,28 @@ __global__ __launch_bounds__(PG_T) void k_px_hist(PGRows S, PGBuf in, const PXTi
         }
         uint64_t key[PX_R], vd[PG_MAXV];
         bool live[PX_R];
+        const tdd_col_t& kc = S.P.keys[0];
+        if (ROWS && KE == 4 && !S.mask && T.r1 - T.r0 == PX_TILE && ((uintptr_t)kc.data & 15) == 0) {
+            /* a full tile of 4-byte keys, no selection: 8 consecutive keys a
+             * lane in two 16-byte loads (the count does not care which lane
+             * holds which row) */
+            const uint4* p = (const uint4*)((const uint32_t*)kc.data + T.r0 +
+                                            (int64_t)threadIdx.x * PX_R);
+            const uint4 a = p[0], b = p[1];
+            const uint32_t w[PX_R] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-        for (int k = 0; k < PX_R; k++)                    /* every load of the tile first */
-            live[k] = pg_item<ROWS, HK, 0, KE>(S, in, T.r0 + (int64_t)k * PG_T + threadIdx.x,
-                                               T.r1, key[k], vd);
+            for (int k = 0; k < PX_R; k++) {
+                const int64_t raw = kc.type == TD_SYM ? (int64_t)w[k] : (int64_t)(int32_t)w[k];
+                key[k] = HK && S.P.kstride[0] == 0 ? (uint64_t)raw
+                                                   : (uint64_t)((raw - S.P.kmin[0]) * S.P.kstride[0]);
+                live[k] = true;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PX_R; k++)                /* every load of the tile first */
+                live[k] = pg_item<ROWS, HK, 0, KE>(S, in, T.r0 + (int64_t)k * PG_T + threadIdx.x,
+                                                   T.r1, key[k], vd);
+        }
 #pragma unroll
         for (int k = 0; k < PX_R; k++) {
             const int s = live[k] ? (int)px_digit<HK>(key[k], shift, PL) : -1;
