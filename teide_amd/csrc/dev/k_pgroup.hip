@@ -226,17 +226,38 @@ __global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
     __shared__ uint64_t stage[PX_TILE];
     const int lane = lane_id();
     const int run = XR > 1 ? (int)(blockIdx.x % PX_XCD) : 0;
+    /* rows of 4-byte keys and 8-byte values, no selection, 16-byte aligned:
+     * a full tile's lane takes 8 consecutive rows with 16-byte loads */
+    bool vcols = ROWS && KE == 4 && !S.mask && ((uintptr_t)S.P.keys[0].data & 15) == 0;
+    for (int q = 0; q < NV; q++)
+        if (S.vsrc[q] >= 0 && ((uintptr_t)S.P.aggs[S.vsrc[q]].data & 15) != 0) vcols = false;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const PXTile T = tiles[t];                        /* uniform across the block */
         for (int s = threadIdx.x; s < PL; s += PG_T) hist[s] = 0;
         __syncthreads();
         uint64_t key[PX_R], vd[PG_MAXV];
         uint32_t dig[PX_R], at[PX_R], live = 0;
+        const bool vrow = vcols && T.r1 - T.r0 == PX_TILE;
+        const int64_t i0 = vrow ? T.r0 + (int64_t)threadIdx.x * PX_R : T.r0 + threadIdx.x;
+        const int64_t istep = vrow ? 1 : PG_T;            /* item k of the lane: i0 + k * istep */
+        if (vrow) {
+            const tdd_col_t& kc = S.P.keys[0];
+            const uint4* p = (const uint4*)((const uint32_t*)kc.data + i0);
+            const uint4 a = p[0], b = p[1];
+            const uint32_t w[PX_R] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-        for (int k = 0; k < PX_R; k++)
-            if (pg_item<ROWS, HK, 0, KE>(S, in, T.r0 + (int64_t)k * PG_T + threadIdx.x, T.r1,
-                                         key[k], vd))
-                live |= 1u << k;
+            for (int k = 0; k < PX_R; k++) {
+                const int64_t raw = kc.type == TD_SYM ? (int64_t)w[k] : (int64_t)(int32_t)w[k];
+                key[k] = HK && S.P.kstride[0] == 0 ? (uint64_t)raw
+                                                   : (uint64_t)((raw - S.P.kmin[0]) * S.P.kstride[0]);
+            }
+            live = (1u << PX_R) - 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < PX_R; k++)
+                if (pg_item<ROWS, HK, 0, KE>(S, in, i0 + (int64_t)k * istep, T.r1, key[k], vd))
+                    live |= 1u << k;
+        }
 #pragma unroll
         for (int k = 0; k < PX_R; k++) {
             dig[k] = at[k] = 0;
@@ -304,11 +325,21 @@ __global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
 #pragma unroll
         for (int q = 0; q < NV; q++) {
             uint64_t v[PX_R];
+            if (vrow && S.vsrc[q] >= 0) {                 /* 8 consecutive values, 16-byte loads */
+                const ulonglong2* p =
+                    (const ulonglong2*)((const uint64_t*)S.P.aggs[S.vsrc[q]].data + i0);
 #pragma unroll
-            for (int k = 0; k < PX_R; k++)                /* the word's loads first */
-                v[k] = (live >> k) & 1
-                           ? px_val<ROWS, KE>(S, in, q, T.r0 + (int64_t)k * PG_T + threadIdx.x)
-                           : 0ull;
+                for (int h = 0; h < PX_R / 2; h++) {
+                    const ulonglong2 x = p[h];
+                    v[2 * h] = x.x;
+                    v[2 * h + 1] = x.y;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < PX_R; k++)            /* the word's loads first */
+                    v[k] = (live >> k) & 1 ? px_val<ROWS, KE>(S, in, q, i0 + (int64_t)k * istep)
+                                           : 0ull;
+            }
             __syncthreads();                              /* the previous array is out */
 #pragma unroll
             for (int k = 0; k < PX_R; k++)
