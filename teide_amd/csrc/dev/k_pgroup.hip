@@ -237,10 +237,31 @@ __global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
         __syncthreads();
         uint64_t key[PX_R], vd[PG_MAXV];
         uint32_t dig[PX_R], at[PX_R], live = 0;
-        const bool vrow = vcols && T.r1 - T.r0 == PX_TILE;
+        bool vrow = T.r1 - T.r0 == PX_TILE && (ROWS ? vcols : true);
+        if (!ROWS && vrow) {                              /* records: 16-byte aligned at r0 */
+            vrow = (HK ? ((uintptr_t)(in.key + T.r0) & 15) : ((uintptr_t)(in.slot + T.r0) & 15)) == 0;
+            for (int q = 0; q < NV; q++)
+                if ((uintptr_t)(in.val[q] + T.r0) & 15) vrow = false;
+        }
         const int64_t i0 = vrow ? T.r0 + (int64_t)threadIdx.x * PX_R : T.r0 + threadIdx.x;
         const int64_t istep = vrow ? 1 : PG_T;            /* item k of the lane: i0 + k * istep */
-        if (vrow) {
+        if (vrow && !ROWS) {
+            if (HK) {
+                const ulonglong2* p = (const ulonglong2*)(in.key + i0);
+#pragma unroll
+                for (int h = 0; h < PX_R / 2; h++) {
+                    const ulonglong2 x = p[h];
+                    key[2 * h] = x.x;
+                    key[2 * h + 1] = x.y;
+                }
+            } else {
+                const uint4* p = (const uint4*)(in.slot + i0);
+                const uint4 a = p[0], b = p[1];
+                key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
+                key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
+            }
+            live = (1u << PX_R) - 1;
+        } else if (vrow) {
             const tdd_col_t& kc = S.P.keys[0];
             const uint4* p = (const uint4*)((const uint32_t*)kc.data + i0);
             const uint4 a = p[0], b = p[1];
@@ -325,9 +346,10 @@ __global__ __launch_bounds__(PG_T) void k_px_pass(PGRows S, PGBuf in, PGBuf out,
 #pragma unroll
         for (int q = 0; q < NV; q++) {
             uint64_t v[PX_R];
-            if (vrow && S.vsrc[q] >= 0) {                 /* 8 consecutive values, 16-byte loads */
+            if (vrow && (!ROWS || S.vsrc[q] >= 0)) {      /* 8 consecutive values, 16-byte loads */
                 const ulonglong2* p =
-                    (const ulonglong2*)((const uint64_t*)S.P.aggs[S.vsrc[q]].data + i0);
+                    (const ulonglong2*)(ROWS ? (const uint64_t*)S.P.aggs[S.vsrc[q]].data + i0
+                                             : in.val[q] + i0);
 #pragma unroll
                 for (int h = 0; h < PX_R / 2; h++) {
                     const ulonglong2 x = p[h];
