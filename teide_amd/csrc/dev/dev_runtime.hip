@@ -328,11 +328,13 @@ extern "C" int tdd_prof_read(tdd_prof_row_t* rows, int cap, int* n_rows) {
 /* ---------------------------------------------------------------- scans -- */
 /* Three-phase exclusive scan: per-block sums, one block scans those sums,
  * then each block rescans its chunk from its base.  Chunk = 256 threads x
- * 16 elements. */
+ * 16 elements.  Inputs of at most SCAN_ONE chunks take one launch: a single
+ * block walks the chunks in order, carrying the running total. */
 
 #define SCAN_T 256
 #define SCAN_V 16
 #define SCAN_CHUNK (SCAN_T * SCAN_V)
+#define SCAN_ONE 4
 
 template <class T>
 __device__ __forceinline__ int64_t scan_ld(const T* in, int64_t i, int64_t n) {
@@ -401,6 +403,24 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const T* in, int64_t n,
 }
 
 template <class T>
+__global__ __launch_bounds__(SCAN_T) void k_scan_one(const T* in, int64_t n, int64_t* out) {
+    int64_t carry = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += SCAN_CHUNK) {     /* uniform across the block */
+        const int64_t base = c0 + threadIdx.x * SCAN_V;
+        int64_t v[SCAN_V], s = 0;
+        for (int j = 0; j < SCAN_V; j++) { v[j] = scan_ld(in, base + j, n); s += v[j]; }
+        int64_t tot;
+        int64_t p = block_excl_scan(s, &tot) + carry;
+        for (int j = 0; j < SCAN_V; j++) {
+            if (base + j < n) out[base + j] = p;
+            p += v[j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+template <class T>
 static int scan_impl(const T* in, int64_t n, int64_t* out, const char* name) {
     if (n <= 0) {
         int64_t z = 0;
@@ -409,6 +429,10 @@ static int scan_impl(const T* in, int64_t n, int64_t* out, const char* name) {
         return TD_OK;
     }
     int64_t nb = ceil_div(n, SCAN_CHUNK);
+    if (nb <= SCAN_ONE) {
+        LAUNCH(name, k_scan_one<T>, 1, SCAN_T, 0, in, n, out);
+        return TD_OK;
+    }
     SCRATCH(sums, (size_t)(nb + 1) * 8);
     LAUNCH(name, k_scan_sums<T>, (int)nb, SCAN_T, 0, in, n, sums.as<int64_t>());
     LAUNCH(name, k_scan_top, 1, SCAN_T, 0, sums.as<int64_t>(), nb);
