@@ -184,4 +184,6 @@ namespace tdd {
 int scan_u16_i64(const uint16_t* in, int64_t n, int64_t* out);
 int scan_i64(const int64_t* in, int64_t n, int64_t* out);
 int scan_u32_i64(const uint32_t* in, int64_t n, int64_t* out);
+/* the same, each prefix also written to out2[0 .. n) */
+int scan_u32_i64_dup(const uint32_t* in, int64_t n, int64_t* out, int64_t* out2);
 }  // namespace tdd

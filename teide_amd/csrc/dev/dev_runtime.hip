@@ -389,21 +389,26 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_top(int64_t* sums, int64_t nb) 
 
 template <class T>
 __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const T* in, int64_t n,
-                                                       const int64_t* sums, int64_t* out) {
+                                                       const int64_t* sums, int64_t* out,
+                                                       int64_t* out2) {
     int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_V;
     int64_t v[SCAN_V], s = 0;
     for (int j = 0; j < SCAN_V; j++) { v[j] = scan_ld(in, base + j, n); s += v[j]; }
     int64_t tot;
     int64_t p = block_excl_scan(s, &tot) + sums[blockIdx.x];
     for (int j = 0; j < SCAN_V; j++) {
-        if (base + j < n) out[base + j] = p;
+        if (base + j < n) {
+            out[base + j] = p;
+            if (out2) out2[base + j] = p;
+        }
         p += v[j];
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_T - 1) out[n] = sums[gridDim.x];
 }
 
 template <class T>
-__global__ __launch_bounds__(SCAN_T) void k_scan_one(const T* in, int64_t n, int64_t* out) {
+__global__ __launch_bounds__(SCAN_T) void k_scan_one(const T* in, int64_t n, int64_t* out,
+                                                     int64_t* out2) {
     int64_t carry = 0;
     for (int64_t c0 = 0; c0 < n; c0 += SCAN_CHUNK) {     /* uniform across the block */
         const int64_t base = c0 + threadIdx.x * SCAN_V;
@@ -412,7 +417,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_one(const T* in, int64_t n, int
         int64_t tot;
         int64_t p = block_excl_scan(s, &tot) + carry;
         for (int j = 0; j < SCAN_V; j++) {
-            if (base + j < n) out[base + j] = p;
+            if (base + j < n) {
+                out[base + j] = p;
+                if (out2) out2[base + j] = p;
+            }
             p += v[j];
         }
         carry += tot;
@@ -421,7 +429,8 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_one(const T* in, int64_t n, int
 }
 
 template <class T>
-static int scan_impl(const T* in, int64_t n, int64_t* out, const char* name) {
+static int scan_impl(const T* in, int64_t n, int64_t* out, const char* name,
+                     int64_t* out2 = nullptr) {
     if (n <= 0) {
         int64_t z = 0;
         HC(hipMemcpyAsync(out, &z, 8, hipMemcpyHostToDevice, g_stream));
@@ -430,13 +439,14 @@ static int scan_impl(const T* in, int64_t n, int64_t* out, const char* name) {
     }
     int64_t nb = ceil_div(n, SCAN_CHUNK);
     if (nb <= SCAN_ONE) {
-        LAUNCH(name, k_scan_one<T>, 1, SCAN_T, 0, in, n, out);
+        LAUNCH(name, k_scan_one<T>, 1, SCAN_T, 0, in, n, out, out2);
         return TD_OK;
     }
     SCRATCH(sums, (size_t)(nb + 1) * 8);
     LAUNCH(name, k_scan_sums<T>, (int)nb, SCAN_T, 0, in, n, sums.as<int64_t>());
     LAUNCH(name, k_scan_top, 1, SCAN_T, 0, sums.as<int64_t>(), nb);
-    LAUNCH(name, k_scan_apply<T>, (int)nb, SCAN_T, 0, in, n, (const int64_t*)sums.as<int64_t>(), out);
+    LAUNCH(name, k_scan_apply<T>, (int)nb, SCAN_T, 0, in, n, (const int64_t*)sums.as<int64_t>(), out,
+           out2);
     return TD_OK;
 }
 
@@ -444,4 +454,7 @@ namespace tdd {
 int scan_u16_i64(const uint16_t* in, int64_t n, int64_t* out) { return scan_impl(in, n, out, "scan"); }
 int scan_i64(const int64_t* in, int64_t n, int64_t* out) { return scan_impl(in, n, out, "scan"); }
 int scan_u32_i64(const uint32_t* in, int64_t n, int64_t* out) { return scan_impl(in, n, out, "scan"); }
+int scan_u32_i64_dup(const uint32_t* in, int64_t n, int64_t* out, int64_t* out2) {
+    return scan_impl(in, n, out, "scan", out2);
+}
 }  // namespace tdd

@@ -878,9 +878,8 @@ static int px_launch(int nv, const PGRows& S, const PGBuf& in, const PGBuf& out,
     else if (ROWS && hke == 4) PXH((ROWS ? 4 : 0));
     else PXH(0);
 #undef PXH
-    rc = tdd::scan_u32_i64(C, nrun, O);
+    rc = tdd::scan_u32_i64_dup(C, nrun, O, (int64_t*)cur);   /* run starts, and cursors */
     if (rc) return rc;
-    HC(hipMemcpyAsync(cur, O, (size_t)nrun * 8, hipMemcpyDeviceToDevice, tdd::g_stream));
     const int gp = grid8(resident_grid(k_px_pass<ROWS, HK, 2, 0>, PG_T, 0, ntiles));
 #define PXP1(NV_, KE_) LAUNCH("group_pg_pass", (k_px_pass<ROWS, HK, NV_, KE_>), gp, PG_T, 0, S, in, \
                               out, d_tiles, ntiles, dn, shift, PL, XR, cur, nout)
